@@ -1,0 +1,207 @@
+/*
+ * tacuda_amd -- MI355X-native indicator core behind the TACUDA C ABI.
+ *
+ * Status codes, error state, version, host (ct_*) calls and series-major
+ * batch calls (ct_*_batch, ct_*_batch_d) for the hot-path families.  Every
+ * declaration keeps the reference's name, argument order and meaning; the
+ * only change is hipStream_t in place of cudaStream_t (an opaque pointer of
+ * the same size).  Citations are to the reference's include/tacuda.h.
+ */
+#ifndef TACUDA_AMD_TACUDA_H
+#define TACUDA_AMD_TACUDA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifndef HIP_INCLUDE_HIP_HIP_RUNTIME_API_H
+typedef struct ihipStream_t* hipStream_t;
+#endif
+
+#if defined(_WIN32)
+#define CTAPI_EXPORT __declspec(dllexport)
+#else
+#define CTAPI_EXPORT __attribute__((visibility("default")))
+#endif
+
+#ifdef __cplusplus
+#define CTAPI_STREAM_DEFAULT = 0
+extern "C" {
+#else
+#define CTAPI_STREAM_DEFAULT
+#endif
+
+/* tacuda.h:32-40 */
+typedef enum ctStatus {
+  CT_STATUS_SUCCESS = 0,
+  CT_STATUS_ALLOC_FAILED = 1,
+  CT_STATUS_COPY_FAILED = 2,
+  CT_STATUS_KERNEL_FAILED = 3,
+  CT_STATUS_INVALID_ARGUMENT = 4,
+  CT_STATUS_DEVICE_ERROR = 5,
+  CT_STATUS_UNSUPPORTED = 6,
+} ctStatus_t;
+
+/* tacuda.h:45 */
+CTAPI_EXPORT const char* ct_status_string(ctStatus_t status);
+/* tacuda.h:54 -- thread-local, cleared on entry of every call */
+CTAPI_EXPORT const char* ct_get_last_error(void);
+
+#define CT_VERSION_MAJOR 0
+#define CT_VERSION_MINOR 9
+#define CT_VERSION_PATCH 3
+#define CT_VERSION_STRING "0.9.3"
+/* tacuda.h:70 */
+CTAPI_EXPORT const char* ct_version(void);
+
+/* tacuda.h:72-89 */
+typedef enum ctMaType {
+  CT_MA_SMA = 0,
+  CT_MA_EMA = 1,
+  CT_MA_WMA = 2,
+  CT_MA_DEMA = 3,
+  CT_MA_TEMA = 4,
+  CT_MA_TRIMA = 5,
+  CT_MA_KAMA = 6,
+  CT_MA_MAMA = 7,
+  CT_MA_T3 = 8,
+  CT_MA_HMA = 9,
+  CT_MA_DISABLED = 10,
+  CT_MA_DEFAULT = 11,
+  CT_MA_ZLEMA = 12,
+  CT_MA_RMA = 13,
+  CT_MA_VIDYA = 14,
+  CT_MA_ALMA = 15,
+} ctMaType_t;
+
+/* tacuda.h:91-95 */
+typedef enum ctDataType {
+  CT_DTYPE_F16 = 0,
+  CT_DTYPE_F32 = 1,
+  CT_DTYPE_F64 = 2,
+} ctDataType_t;
+
+/* ---------------------------------------------------------------------------
+ * Host calls: upload, run the device kernel, download, synchronize the stream
+ * (reference api.cpp:230-287).  Lines cite tacuda.h.
+ * ------------------------------------------------------------------------- */
+/* :98 */ CTAPI_EXPORT ctStatus_t ct_sma(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :109 */ CTAPI_EXPORT ctStatus_t ct_wma(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :123 */ CTAPI_EXPORT ctStatus_t ct_ema(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :128 */ CTAPI_EXPORT ctStatus_t ct_dema(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :130 */ CTAPI_EXPORT ctStatus_t ct_tema(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :132 */ CTAPI_EXPORT ctStatus_t ct_t3(const float* host_input, float* host_output, int size, int period, float vFactor, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :138 */ CTAPI_EXPORT ctStatus_t ct_max(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :140 */ CTAPI_EXPORT ctStatus_t ct_min(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :151 */ CTAPI_EXPORT ctStatus_t ct_stddev(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :155 */ CTAPI_EXPORT ctStatus_t ct_var(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :157 */ CTAPI_EXPORT ctStatus_t ct_sum(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :159 */ CTAPI_EXPORT ctStatus_t ct_rsi(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :164 */ CTAPI_EXPORT ctStatus_t ct_kama(const float* host_input, float* host_output, int size, int period, int fastPeriod, int slowPeriod, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :171 */ CTAPI_EXPORT ctStatus_t ct_macd(const float* host_input, float* host_macd, float* host_signal, float* host_hist, int size, int fastPeriod, int slowPeriod, int signalPeriod, ctMaType_t type, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :299 */ CTAPI_EXPORT ctStatus_t ct_bbands(const float* host_input, float* host_upper, float* host_middle, float* host_lower, int size, int period, float upperMul, float lowerMul, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :308 */ CTAPI_EXPORT ctStatus_t ct_atr(const float* host_high, const float* host_low, const float* host_close, float* host_output, int size, int period, float initial, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :317 */ CTAPI_EXPORT ctStatus_t ct_natr(const float* host_high, const float* host_low, const float* host_close, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :321 */ CTAPI_EXPORT ctStatus_t ct_trange(const float* host_high, const float* host_low, const float* host_close, float* host_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :324 */ CTAPI_EXPORT ctStatus_t ct_stochastic(const float* host_high, const float* host_low, const float* host_close, float* host_k, float* host_d, int size, int kPeriod, int dPeriod, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :337 */ CTAPI_EXPORT ctStatus_t ct_cci(const float* host_high, const float* host_low, const float* host_close, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :340 */ CTAPI_EXPORT ctStatus_t ct_adx(const float* host_high, const float* host_low, const float* host_close, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :346 */ CTAPI_EXPORT ctStatus_t ct_plus_dm(const float* host_high, const float* host_low, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :349 */ CTAPI_EXPORT ctStatus_t ct_minus_dm(const float* host_high, const float* host_low, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :352 */ CTAPI_EXPORT ctStatus_t ct_plus_di(const float* host_high, const float* host_low, const float* host_close, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :356 */ CTAPI_EXPORT ctStatus_t ct_minus_di(const float* host_high, const float* host_low, const float* host_close, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :385 */ CTAPI_EXPORT ctStatus_t ct_aroon(const float* host_high, const float* host_low, float* host_up, float* host_down, float* host_osc, int size, int upPeriod, int downPeriod, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :389 */ CTAPI_EXPORT ctStatus_t ct_aroonosc(const float* host_high, const float* host_low, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :419 */ CTAPI_EXPORT ctStatus_t ct_willr(const float* host_high, const float* host_low, const float* host_close, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :739 */ CTAPI_EXPORT ctStatus_t ct_dx(const float* host_high, const float* host_low, const float* host_close, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+
+/* tacuda.h:824-827 -- the workspace cache of this build stands in for the
+ * reference's DeviceBufferPool. */
+CTAPI_EXPORT void ct_device_buffer_pool_cleanup(void);
+CTAPI_EXPORT size_t ct_device_buffer_pool_allocation_count(void);
+CTAPI_EXPORT size_t ct_device_buffer_pool_set_max_cache_bytes(size_t bytes);
+
+/* ---------------------------------------------------------------------------
+ * Batch calls over dense series-major matrices: numSeries rows of seriesLen
+ * contiguous floats (tacuda.h:830-847).  Every family runs as one fused
+ * launch; maxConcurrency is accepted and ignored.  Stream 0 returns when the
+ * batch is done; a non-null stream only enqueues (capturable).
+ * ------------------------------------------------------------------------- */
+#define CT_BATCH1(name, line)                                                                 \
+  CTAPI_EXPORT ctStatus_t ct_##name##_batch(const float* host_inputs, float* host_outputs,    \
+                                            int seriesLen, int numSeries, int period,        \
+                                            int maxConcurrency,                              \
+                                            hipStream_t stream CTAPI_STREAM_DEFAULT);        \
+  CTAPI_EXPORT ctStatus_t ct_##name##_batch_d(const float* d_inputs, float* d_outputs,        \
+                                              int seriesLen, int numSeries, int period,      \
+                                              int maxConcurrency,                            \
+                                              hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :848/:904 */ CT_BATCH1(sma, 848)
+/* :853/:909 */ CT_BATCH1(ema, 853)
+/* :858/:914 */ CT_BATCH1(rsi, 858)
+/* :953/:958 */ CT_BATCH1(wma, 953)
+/* :962/:967 */ CT_BATCH1(dema, 962)
+/* :972/:977 */ CT_BATCH1(tema, 972)
+/* :1050/:1055 */ CT_BATCH1(stddev, 1050)
+/* :1060/:1065 */ CT_BATCH1(var, 1060)
+/* :1069/:1074 */ CT_BATCH1(sum, 1069)
+/* :1078/:1083 */ CT_BATCH1(max, 1078)
+/* :1087/:1092 */ CT_BATCH1(min, 1087)
+#undef CT_BATCH1
+
+#define CT_BATCH_HLC(name)                                                                    \
+  CTAPI_EXPORT ctStatus_t ct_##name##_batch(const float* host_highs, const float* host_lows,  \
+                                            const float* host_closes, float* host_outputs,   \
+                                            int seriesLen, int numSeries, int period,        \
+                                            int maxConcurrency,                              \
+                                            hipStream_t stream CTAPI_STREAM_DEFAULT);        \
+  CTAPI_EXPORT ctStatus_t ct_##name##_batch_d(const float* d_highs, const float* d_lows,      \
+                                              const float* d_closes, float* d_outputs,       \
+                                              int seriesLen, int numSeries, int period,      \
+                                              int maxConcurrency,                            \
+                                              hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1173/:1180 */ CT_BATCH_HLC(natr)
+/* :1187/:1194 */ CT_BATCH_HLC(adx)
+/* :1214/:1221 */ CT_BATCH_HLC(dx)
+/* :1227/:1234 */ CT_BATCH_HLC(plus_di)
+/* :1241/:1248 */ CT_BATCH_HLC(minus_di)
+/* :1255/:1262 */ CT_BATCH_HLC(willr)
+/* :1269/:1276 */ CT_BATCH_HLC(cci)
+#undef CT_BATCH_HLC
+
+/* :863 / :919 */
+CTAPI_EXPORT ctStatus_t ct_atr_batch(const float* host_highs, const float* host_lows, const float* host_closes, float* host_outputs, int seriesLen, int numSeries, int period, float initial, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_atr_batch_d(const float* d_highs, const float* d_lows, const float* d_closes, float* d_outputs, int seriesLen, int numSeries, int period, float initial, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :870 / :926 */
+CTAPI_EXPORT ctStatus_t ct_macd_batch(const float* host_inputs, float* host_macds, float* host_signals, float* host_hists, int seriesLen, int numSeries, int fastPeriod, int slowPeriod, int signalPeriod, int maType, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_macd_batch_d(const float* d_inputs, float* d_macds, float* d_signals, float* d_hists, int seriesLen, int numSeries, int fastPeriod, int slowPeriod, int signalPeriod, int maType, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :877 / :933 */
+CTAPI_EXPORT ctStatus_t ct_bbands_batch(const float* host_inputs, float* host_uppers, float* host_middles, float* host_lowers, int seriesLen, int numSeries, int period, float upperMul, float lowerMul, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_bbands_batch_d(const float* d_inputs, float* d_uppers, float* d_middles, float* d_lowers, int seriesLen, int numSeries, int period, float upperMul, float lowerMul, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :885 / :940 */
+CTAPI_EXPORT ctStatus_t ct_stochastic_batch(const float* host_highs, const float* host_lows, const float* host_closes, float* host_ks, float* host_ds, int seriesLen, int numSeries, int kPeriod, int dPeriod, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_stochastic_batch_d(const float* d_highs, const float* d_lows, const float* d_closes, float* d_ks, float* d_ds, int seriesLen, int numSeries, int kPeriod, int dPeriod, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1152 / :1158 */
+CTAPI_EXPORT ctStatus_t ct_kama_batch(const float* host_inputs, float* host_outputs, int seriesLen, int numSeries, int period, int fastPeriod, int slowPeriod, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_kama_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int period, int fastPeriod, int slowPeriod, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1164 / :1169 */
+CTAPI_EXPORT ctStatus_t ct_t3_batch(const float* host_inputs, float* host_outputs, int seriesLen, int numSeries, int period, float vFactor, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_t3_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int period, float vFactor, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1282 / :1288 */
+CTAPI_EXPORT ctStatus_t ct_trange_batch(const float* host_highs, const float* host_lows, const float* host_closes, float* host_outputs, int seriesLen, int numSeries, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_trange_batch_d(const float* d_highs, const float* d_lows, const float* d_closes, float* d_outputs, int seriesLen, int numSeries, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1344 / :1350 */
+CTAPI_EXPORT ctStatus_t ct_aroonosc_batch(const float* host_highs, const float* host_lows, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_aroonosc_batch_d(const float* d_highs, const float* d_lows, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1356 / :1363 (down before up, as in the reference) */
+CTAPI_EXPORT ctStatus_t ct_aroon_batch(const float* host_highs, const float* host_lows, float* host_aroon_downs, float* host_aroon_ups, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_aroon_batch_d(const float* d_highs, const float* d_lows, float* d_aroon_downs, float* d_aroon_ups, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1733 / :1739, :1745 / :1751 */
+CTAPI_EXPORT ctStatus_t ct_plus_dm_batch(const float* host_highs, const float* host_lows, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_plus_dm_batch_d(const float* d_highs, const float* d_lows, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_minus_dm_batch(const float* host_highs, const float* host_lows, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_minus_dm_batch_d(const float* d_highs, const float* d_lows, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TACUDA_AMD_TACUDA_H */

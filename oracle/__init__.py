@@ -1,0 +1,201 @@
+"""CPU oracle for the indicator core (test infrastructure only).
+
+``oracle/oracle.c`` restates the TA-Lib 0.8.3 algorithms the reference vendors
+(``third_party/ta_lib_0_8_3``) plus the reference's own semantics for the
+families it does not vendor; ``oracle/candles.c`` restates the 61 TA-Lib
+candlestick predicates.  ``oracle/_ref/libtaref.so`` is the reference's own
+host build of TA-Lib 0.8.3 (``src/compat/Talib08Formulas.h``), used only to
+check the restatement.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline
+leg import this package.  Product code never does.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB: Optional[ctypes.CDLL] = None
+_REF: Optional[ctypes.CDLL] = None
+
+_F = ctypes.POINTER(ctypes.c_float)
+_I = ctypes.c_int
+_D = ctypes.c_double
+
+
+def lib() -> ctypes.CDLL:
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle library not built: {path} (run __graft_entry__.build())")
+        _LIB = ctypes.CDLL(path)
+    return _LIB
+
+
+def ref() -> Optional[ctypes.CDLL]:
+    """The reference's TA-Lib host build, or None when it was not built."""
+    global _REF
+    if _REF is None:
+        path = os.path.join(_HERE, "_ref", "libtaref.so")
+        if not os.path.exists(path):
+            return None
+        _REF = ctypes.CDLL(path)
+    return _REF
+
+
+def _fp(a: np.ndarray):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_F)
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _call(fn, ins, nout, n, *params):
+    ins = [_f32(a) for a in ins]
+    outs = [np.empty(n, dtype=np.float32) for _ in range(nout)]
+    args = [_fp(a) for a in ins] + [_fp(o) for o in outs] + [n] + list(params)
+    rc = fn(*args)
+    if rc != 0:
+        raise ValueError(f"{fn.__name__} returned status {rc}")
+    return outs[0] if nout == 1 else tuple(outs)
+
+
+def _sig(fn, nin, nout, params):
+    fn.argtypes = [_F] * (nin + nout) + [_I] + params
+    fn.restype = _I
+    return fn
+
+
+class _Family:
+    """Binds a C entry point family (``orc_*`` or ``ref_*``) to numpy."""
+
+    def __init__(self, getlib, prefix):
+        self._get = getlib
+        self._prefix = prefix
+
+    def _fn(self, name, nin, nout, params):
+        L = self._get()
+        if L is None:
+            raise RuntimeError("library not available")
+        return _sig(getattr(L, self._prefix + name), nin, nout, params)
+
+    # single input, one period
+    def sma(self, x, p):
+        return _call(self._fn("sma", 1, 1, [_I]), [x], 1, len(x), p)
+
+    def sum(self, x, p):
+        return _call(self._fn("sum", 1, 1, [_I]), [x], 1, len(x), p)
+
+    def wma(self, x, p):
+        return _call(self._fn("wma", 1, 1, [_I]), [x], 1, len(x), p)
+
+    def ema(self, x, p):
+        return _call(self._fn("ema", 1, 1, [_I]), [x], 1, len(x), p)
+
+    def dema(self, x, p):
+        return _call(self._fn("dema", 1, 1, [_I]), [x], 1, len(x), p)
+
+    def tema(self, x, p):
+        return _call(self._fn("tema", 1, 1, [_I]), [x], 1, len(x), p)
+
+    def t3(self, x, p, v):
+        return _call(self._fn("t3", 1, 1, [_I, _D]), [x], 1, len(x), p, v)
+
+    def kama(self, x, p, fast=2, slow=30):
+        if self._prefix == "ref_":
+            return _call(self._fn("kama", 1, 1, [_I]), [x], 1, len(x), p)
+        return _call(self._fn("kama", 1, 1, [_I, _I, _I]), [x], 1, len(x), p, fast, slow)
+
+    def rsi(self, x, p):
+        return _call(self._fn("rsi", 1, 1, [_I]), [x], 1, len(x), p)
+
+    def rma(self, x, p):
+        return _call(self._fn("rma", 1, 1, [_I]), [x], 1, len(x), p)
+
+    def var(self, x, p):
+        return _call(self._fn("var", 1, 1, [_I]), [x], 1, len(x), p)
+
+    def stddev(self, x, p):
+        return _call(self._fn("stddev", 1, 1, [_I]), [x], 1, len(x), p)
+
+    def min(self, x, p):
+        return _call(self._fn("min", 1, 1, [_I]), [x], 1, len(x), p)
+
+    def max(self, x, p):
+        return _call(self._fn("max", 1, 1, [_I]), [x], 1, len(x), p)
+
+    def macd(self, x, fast, slow, sig):
+        return _call(self._fn("macd", 1, 3, [_I, _I, _I]), [x], 3, len(x), fast, slow, sig)
+
+    def bbands(self, x, p, up, dn):
+        return _call(self._fn("bbands", 1, 3, [_I, _D, _D]), [x], 3, len(x), p, up, dn)
+
+    def atr(self, h, l, c, p):
+        return _call(self._fn("atr", 3, 1, [_I]), [h, l, c], 1, len(c), p)
+
+    def natr(self, h, l, c, p):
+        return _call(self._fn("natr", 3, 1, [_I]), [h, l, c], 1, len(c), p)
+
+    def trange(self, h, l, c):
+        return _call(self._fn("trange", 3, 1, []), [h, l, c], 1, len(c))
+
+    def stoch(self, h, l, c, kp, dp):
+        if self._prefix == "ref_":
+            return _call(self._fn("stoch", 3, 2, [_I, _I, _I]), [h, l, c], 2, len(c), kp, dp, dp)
+        return _call(self._fn("stoch", 3, 2, [_I, _I]), [h, l, c], 2, len(c), kp, dp)
+
+    def willr(self, h, l, c, p):
+        return _call(self._fn("willr", 3, 1, [_I]), [h, l, c], 1, len(c), p)
+
+    def cci(self, h, l, c, p):
+        return _call(self._fn("cci", 3, 1, [_I]), [h, l, c], 1, len(c), p)
+
+    def aroon(self, h, l, p):
+        """Returns (down, up) like the reference ct_aroon_d."""
+        return _call(self._fn("aroon", 2, 2, [_I]), [h, l], 2, len(h), p)
+
+    def aroonosc(self, h, l, p):
+        return _call(self._fn("aroonosc", 2, 1, [_I]), [h, l], 1, len(h), p)
+
+    def plus_dm(self, h, l, p):
+        return _call(self._fn("plus_dm", 2, 1, [_I]), [h, l], 1, len(h), p)
+
+    def minus_dm(self, h, l, p):
+        return _call(self._fn("minus_dm", 2, 1, [_I]), [h, l], 1, len(h), p)
+
+    def plus_di(self, h, l, c, p):
+        return _call(self._fn("plus_di", 3, 1, [_I]), [h, l, c], 1, len(c), p)
+
+    def minus_di(self, h, l, c, p):
+        return _call(self._fn("minus_di", 3, 1, [_I]), [h, l, c], 1, len(c), p)
+
+    def dx(self, h, l, c, p):
+        return _call(self._fn("dx", 3, 1, [_I]), [h, l, c], 1, len(c), p)
+
+    def adx(self, h, l, c, p):
+        return _call(self._fn("adx", 3, 1, [_I]), [h, l, c], 1, len(c), p)
+
+    def candle(self, name, o, h, l, c):
+        """int32 signal of one of the 61 TA-Lib patterns (see candles.c)."""
+        L = self._get()
+        fn = L.orc_cdl
+        fn.argtypes = [ctypes.c_char_p, _F, _F, _F, _F, ctypes.POINTER(ctypes.c_int32), _I]
+        fn.restype = _I
+        o, h, l, c = (_f32(a) for a in (o, h, l, c))
+        out = np.empty(len(c), dtype=np.int32)
+        rc = fn(name.encode(), _fp(o), _fp(h), _fp(l), _fp(c),
+                out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), len(c))
+        if rc != 0:
+            raise ValueError(f"orc_cdl({name}) returned {rc}")
+        return out
+
+
+ORC = _Family(lib, "orc_")
+REF = _Family(ref, "ref_")

@@ -1,0 +1,944 @@
+/*
+ * CPU oracle for the indicator core -- TEST INFRASTRUCTURE ONLY.
+ *
+ * This file is a plain-C restatement of the TA-Lib 0.8.3 algorithms that the
+ * reference vendors (third_party/ta_lib_0_8_3/ta_*.c) and of the reference's
+ * own semantics for the families TA-Lib C is not vendored for.  It is the
+ * checker for the HIP kernels: only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  No product code links it.
+ *
+ * Conventions shared by every entry point (the drop-in contract of the
+ * reference C ABI, include/tacuda.h:32-40 and docs/IMPLEMENTATION_NOTES.md):
+ *   - inputs are float32, arithmetic is double, outputs are float32 arrays of
+ *     the full input length with NaN warm-up;
+ *   - return 0 on success, 4 (CT_STATUS_INVALID_ARGUMENT) on a bad argument;
+ *   - a period longer than the data is success with an all-NaN line;
+ *   - EMA, DEMA, TEMA, T3, MACD, RSI and KAMA skip a leading NaN prefix
+ *     (reference BatchFused.cu:34-40, EMA.cu:65, MACDEXT.cu:199, KAMA.cu:186)
+ *     and then run the TA-Lib recurrence on the remaining samples;
+ *   - window families (SMA, SUM, WMA, VAR, STDDEV, BBANDS) evaluate every
+ *     window directly, so a NaN/Inf affects exactly the windows containing it
+ *     (docs/IMPLEMENTATION_NOTES.md:60-84).  For float32 inputs of ordinary
+ *     magnitude the double window sums are exact, so this equals TA-Lib's
+ *     running totals bit for bit on finite data.
+ *
+ * Parity status per function is listed in the header comment of each one.
+ * "pinned" names the independent check in tests/ that fixes it.
+ */
+#include <math.h>
+#include <stddef.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORC_OK 0
+#define ORC_BAD 4
+#define API __attribute__((visibility("default")))
+
+static void fill_nan(float* o, int n) {
+  for (int i = 0; i < n; ++i) o[i] = NAN;
+}
+static int lead_nan(const float* x, int n) {
+  int f = 0;
+  while (f < n && isnan(x[f])) ++f;
+  return f;
+}
+/* TA_IS_ZERO / TA_IS_ZERO_SCALED with TA_EPSILON = 1e-14
+ * (reference src/compat/Talib08Formulas.h:82-84). */
+static int is_zero(double v) { return (-1e-14 < v) && (v < 1e-14); }
+static int is_zero_scaled(double v, double s) { return fabs(v) <= 1e-14 * s; }
+
+/* EMA coefficients, beta first (ta_EMA.c:88-93). */
+static void ema_coef(int p, double* beta, double* k) {
+  double b = (double)(p - 1) / (double)(p + 1);
+  double kk = 1.0 - b;
+  if (b < 0.5) b = 1.0 - kk;
+  *beta = b;
+  *k = kk;
+}
+
+/* ------------------------------------------------------------------ SMA ---
+ * ta_SMA.c:36-117 evaluated per window (window semantics).  pinned: _ref
+ * TA_SMA bit-equality on finite data; brute-force rational sums. */
+API int orc_sma(const float* x, float* out, int n, int p) {
+  if (!x || !out || n <= 0 || p <= 0) return ORC_BAD;
+  fill_nan(out, n);
+  for (int t = p - 1; t < n; ++t) {
+    double s = 0.0;
+    for (int j = t - p + 1; j <= t; ++j) s += (double)x[j];
+    out[t] = (float)(s / (double)p);
+  }
+  return ORC_OK;
+}
+
+/* SUM: window total (reference SUM.cu; TA-Lib ta_SUM).  pinned: brute force. */
+API int orc_sum(const float* x, float* out, int n, int p) {
+  if (!x || !out || n <= 0 || p <= 0) return ORC_BAD;
+  fill_nan(out, n);
+  for (int t = p - 1; t < n; ++t) {
+    double s = 0.0;
+    for (int j = t - p + 1; j <= t; ++j) s += (double)x[j];
+    out[t] = (float)s;
+  }
+  return ORC_OK;
+}
+
+/* ------------------------------------------------------------------ WMA ---
+ * ta_WMA.c:36-244: weights 1..p oldest to newest, divider p(p+1)/2.
+ * pinned: _ref TA_WMA. */
+API int orc_wma(const float* x, float* out, int n, int p) {
+  if (!x || !out || n <= 0 || p <= 0) return ORC_BAD;
+  fill_nan(out, n);
+  if (p == 1) {
+    for (int t = 0; t < n; ++t) out[t] = x[t];
+    return ORC_OK;
+  }
+  const double divider = (double)p * (p + 1) / 2.0;
+  for (int t = p - 1; t < n; ++t) {
+    double s = 0.0;
+    int w = 1;
+    for (int j = t - p + 1; j <= t; ++j, ++w) s += (double)x[j] * w;
+    out[t] = (float)(s / divider);
+  }
+  return ORC_OK;
+}
+
+/* ------------------------------------------------------------------ EMA ---
+ * ta_EMA.c:37-147 on x[f..n) (f = leading NaNs): SMA seed of the first p
+ * samples, then prev = fma(beta, prev, k*x).  pinned: _ref TA_EMA bit
+ * equality; closed form on a constant and on a step input. */
+API int orc_ema(const float* x, float* out, int n, int p) {
+  if (!x || !out || n <= 0 || p <= 0) return ORC_BAD;
+  fill_nan(out, n);
+  if (p == 1) {
+    for (int t = 0; t < n; ++t) out[t] = x[t];
+    return ORC_OK;
+  }
+  const int f = lead_nan(x, n);
+  const int m = n - f;
+  if (p > m) return ORC_OK;
+  double beta, k;
+  ema_coef(p, &beta, &k);
+  const float* in = x + f;
+  double t = 0.0;
+  for (int i = 0; i < p; ++i) t += (double)in[i];
+  double prev = t / p;
+  out[f + p - 1] = (float)prev;
+  for (int i = p; i < m; ++i) {
+    prev = fma(beta, prev, k * (double)in[i]);
+    out[f + i] = (float)prev;
+  }
+  return ORC_OK;
+}
+
+/* ----------------------------------------------------------------- DEMA ---
+ * ta_DEMA.c:37-206: EMA2 is seeded with the mean of the first p EMA1
+ * values (kept in double); out = 2*e1 - e2 from index 2(p-1).
+ * pinned: _ref TA_DEMA. */
+API int orc_dema(const float* x, float* out, int n, int p) {
+  if (!x || !out || n <= 0 || p <= 0) return ORC_BAD;
+  fill_nan(out, n);
+  if (p == 1) {
+    for (int t = 0; t < n; ++t) out[t] = x[t];
+    return ORC_OK;
+  }
+  const int f = lead_nan(x, n);
+  const int m = n - f;
+  const int lb = 2 * (p - 1);
+  if (lb >= m) return ORC_OK;
+  double beta, k;
+  ema_coef(p, &beta, &k);
+  const float* in = x + f;
+  int today = 0;
+  double tmp = 0.0;
+  for (int i = 0; i < p; ++i) tmp += (double)in[today++];
+  double e1 = tmp / p;
+  tmp = 0.0;
+  tmp += e1;
+  for (int i = 0; i < p - 1; ++i) {
+    e1 = fma(beta, e1, k * (double)in[today++]);
+    tmp += e1;
+  }
+  double e2 = tmp / p;
+  out[f + lb] = (float)(2.0 * e1 - e2);
+  for (; today < m; ++today) {
+    e1 = fma(beta, e1, k * (double)in[today]);
+    e2 = fma(beta, e2, k * e1);
+    out[f + today] = (float)(2.0 * e1 - e2);
+  }
+  return ORC_OK;
+}
+
+/* ----------------------------------------------------------------- TEMA ---
+ * ta_TEMA.c:37-227: three cascaded SMA-seeded EMAs, out = e3 + (3e1 - 3e2)
+ * from index 3(p-1).  pinned: _ref TA_TEMA. */
+API int orc_tema(const float* x, float* out, int n, int p) {
+  if (!x || !out || n <= 0 || p <= 0) return ORC_BAD;
+  fill_nan(out, n);
+  if (p == 1) {
+    for (int t = 0; t < n; ++t) out[t] = x[t];
+    return ORC_OK;
+  }
+  const int f = lead_nan(x, n);
+  const int m = n - f;
+  const int lb = 3 * (p - 1);
+  if (lb >= m) return ORC_OK;
+  double beta, k;
+  ema_coef(p, &beta, &k);
+  const float* in = x + f;
+  int today = 0;
+  double tmp = 0.0;
+  for (int i = 0; i < p; ++i) tmp += (double)in[today++];
+  double e1 = tmp / p;
+  tmp = 0.0;
+  tmp += e1;
+  for (int i = 0; i < p - 1; ++i) {
+    e1 = fma(beta, e1, k * (double)in[today++]);
+    tmp += e1;
+  }
+  double e2 = tmp / p;
+  tmp = 0.0;
+  tmp += e2;
+  for (int i = 0; i < p - 1; ++i) {
+    e1 = fma(beta, e1, k * (double)in[today++]);
+    e2 = fma(beta, e2, k * e1);
+    tmp += e2;
+  }
+  double e3 = tmp / p;
+  out[f + lb] = (float)(e3 + (3.0 * e1 - 3.0 * e2));
+  for (; today < m; ++today) {
+    e1 = fma(beta, e1, k * (double)in[today]);
+    e2 = fma(beta, e2, k * e1);
+    e3 = fma(beta, e3, k * e2);
+    out[f + today] = (float)(e3 + (3.0 * e1 - 3.0 * e2));
+  }
+  return ORC_OK;
+}
+
+/* ------------------------------------------------------------------- T3 ---
+ * ta_T3.c:37-226: six cascaded EMAs with k = 2/(p+1), each seeded by the
+ * mean of the first p values of the level below; combined with c1..c4 from
+ * the volume factor.  Lookback 6(p-1).  pinned: _ref TA_T3. */
+API int orc_t3(const float* x, float* out, int n, int p, double v) {
+  if (!x || !out || n <= 0 || p < 2) return ORC_BAD;
+  fill_nan(out, n);
+  const int f = lead_nan(x, n);
+  const int m = n - f;
+  const int lb = 6 * (p - 1);
+  if (lb >= m) return ORC_OK;
+  const float* in = x + f;
+  const double k = 2.0 / (p + 1.0);
+  const double omk = 1.0 - k;
+  int today = 0;
+  double tmp = (double)in[today++];
+  for (int i = p - 1; i > 0; --i) tmp += (double)in[today++];
+  double e1 = tmp / p;
+  tmp = e1;
+  for (int i = p - 1; i > 0; --i) {
+    e1 = fma(omk, e1, k * (double)in[today++]);
+    tmp += e1;
+  }
+  double e2 = tmp / p;
+  tmp = e2;
+  for (int i = p - 1; i > 0; --i) {
+    e1 = fma(omk, e1, k * (double)in[today++]);
+    e2 = fma(omk, e2, k * e1);
+    tmp += e2;
+  }
+  double e3 = tmp / p;
+  tmp = e3;
+  for (int i = p - 1; i > 0; --i) {
+    e1 = fma(omk, e1, k * (double)in[today++]);
+    e2 = fma(omk, e2, k * e1);
+    e3 = fma(omk, e3, k * e2);
+    tmp += e3;
+  }
+  double e4 = tmp / p;
+  tmp = e4;
+  for (int i = p - 1; i > 0; --i) {
+    e1 = fma(omk, e1, k * (double)in[today++]);
+    e2 = fma(omk, e2, k * e1);
+    e3 = fma(omk, e3, k * e2);
+    e4 = fma(omk, e4, k * e3);
+    tmp += e4;
+  }
+  double e5 = tmp / p;
+  tmp = e5;
+  for (int i = p - 1; i > 0; --i) {
+    e1 = fma(omk, e1, k * (double)in[today++]);
+    e2 = fma(omk, e2, k * e1);
+    e3 = fma(omk, e3, k * e2);
+    e4 = fma(omk, e4, k * e3);
+    e5 = fma(omk, e5, k * e4);
+    tmp += e5;
+  }
+  double e6 = tmp / p;
+  /* ta_T3.c:200-205 coefficient algebra */
+  tmp = v * v;
+  const double c1 = -(tmp * v);
+  const double c2 = 3.0 * (tmp - c1);
+  const double c3 = -6.0 * tmp - 3.0 * (v - c1);
+  const double c4 = fma(3.0, tmp, fma(3.0, v, 1.0) - c1);
+  /* today == lb + 1 here; the first output uses the states at lb. */
+  out[f + lb] = (float)fma(c4, e3, fma(c3, e4, fma(c1, e6, c2 * e5)));
+  for (; today < m; ++today) {
+    e1 = fma(omk, e1, k * (double)in[today]);
+    e2 = fma(omk, e2, k * e1);
+    e3 = fma(omk, e3, k * e2);
+    e4 = fma(omk, e4, k * e3);
+    e5 = fma(omk, e5, k * e4);
+    e6 = fma(omk, e6, k * e5);
+    out[f + today] = (float)fma(c4, e3, fma(c3, e4, fma(c1, e6, c2 * e5)));
+  }
+  return ORC_OK;
+}
+
+/* ----------------------------------------------------------------- KAMA ---
+ * ta_KAMA.c:37-297 with fast/slow smoothing periods (TA-Lib fixes 2/30):
+ * constMax = 2/(slow+1), constDiff = 2/(fast+1) - constMax.  The |delta| sum
+ * is a window total, evaluated directly (TA-Lib runs it add-then-subtract
+ * with a flat-window reset; both give 0 on a flat window).
+ * pinned: _ref TA_KAMA (fast 2, slow 30). */
+API int orc_kama(const float* x, float* out, int n, int p, int fast, int slow) {
+  if (!x || !out || n <= 0 || p <= 0 || fast <= 0 || slow <= 0) return ORC_BAD;
+  fill_nan(out, n);
+  const int f = lead_nan(x, n);
+  const int m = n - f;
+  if (p == 1) {
+    for (int t = 0; t < n; ++t) out[t] = x[t];
+    return ORC_OK;
+  }
+  if (p >= m) return ORC_OK;
+  const float* in = x + f;
+  const double cmax = 2.0 / (slow + 1.0);
+  const double cdiff = 2.0 / (fast + 1.0) - cmax;
+  /* first KAMA at index p, previous KAMA = in[p-1] (ta_KAMA.c:148-178) */
+  double prev = (double)in[p - 1];
+  for (int t = p; t < m; ++t) {
+    double sum = 0.0;
+    for (int j = t - p + 1; j <= t; ++j) sum += fabs((double)in[j] - (double)in[j - 1]);
+    const double roc = (double)in[t] - (double)in[t - p];
+    double er;
+    if (sum <= 0.0 || sum <= roc) {
+      er = 1.0;
+    } else {
+      er = fabs(roc / sum);
+      if (er > 1.0) er = 1.0;
+    }
+    double sc = fma(er, cdiff, cmax);
+    sc *= sc;
+    prev = fma((double)in[t] - prev, sc, prev);
+    out[f + t] = (float)prev;
+  }
+  return ORC_OK;
+}
+
+/* ------------------------------------------------------------------ RMA ---
+ * ta_RMA.c:37-135: Wilder smoothing, wBeta = (p-1)/p, wAlpha = 1 - wBeta,
+ * SMA seed at p-1.  pinned: reference golden fixture
+ * tests/fixtures/talib08_typed.json "rma" (TA-Lib 0.8.3 output) and _ref. */
+API int orc_rma(const float* x, float* out, int n, int p) {
+  if (!x || !out || n <= 0 || p <= 0) return ORC_BAD;
+  fill_nan(out, n);
+  if (p > n) return ORC_OK;
+  const double wb = (double)(p - 1) / (double)p;
+  const double wa = 1.0 - wb;
+  double t = 0.0;
+  for (int i = 0; i < p; ++i) t += (double)x[i];
+  double prev = t / p;
+  out[p - 1] = (float)prev;
+  for (int i = p; i < n; ++i) {
+    prev = fma(wb, prev, wa * (double)x[i]);
+    out[i] = (float)prev;
+  }
+  return ORC_OK;
+}
+
+/* ----------------------------------------------------------------- MACD ---
+ * ta_MACD.c:37-288 on x[f..]: reversed periods swapped; slow EMA seeded
+ * over the first `slow` samples, fast over the fast window ENDING at slow-1;
+ * signal EMA seeded by the mean of the first `signal` MACD values; all
+ * three outputs start at slow+signal-2.  pinned: _ref TA_MACD. */
+API int orc_macd(const float* x, float* macd, float* sig, float* hist, int n,
+                 int fast, int slow, int signal) {
+  if (!x || !macd || !sig || !hist || n <= 0 || fast <= 0 || slow <= 0 || signal <= 0)
+    return ORC_BAD;
+  fill_nan(macd, n);
+  fill_nan(sig, n);
+  fill_nan(hist, n);
+  if (slow < fast) {
+    int t = slow;
+    slow = fast;
+    fast = t;
+  }
+  const int f = lead_nan(x, n);
+  const int m = n - f;
+  const int lbSlow = slow - 1, lbSig = signal - 1, lbFast = fast - 1;
+  const int lb = lbSlow + lbSig;
+  if (lb >= m) return ORC_OK;
+  const float* in = x + f;
+  double sb, sk, fb, fk, gb, gk;
+  ema_coef(slow, &sb, &sk);
+  ema_coef(fast, &fb, &fk);
+  ema_coef(signal, &gb, &gk);
+  int today = 0;
+  double tmp = 0.0;
+  for (int i = 0; i < slow; ++i) tmp += (double)in[today++];
+  double ps = tmp / slow;
+  int ft = lbSlow - lbFast;
+  double pf = 0.0;
+  for (int i = 0; i < fast; ++i) pf += (double)in[ft++];
+  pf = pf / fast;
+  while (today < ft) ps = fma(sb, ps, sk * (double)in[today++]);
+  while (today <= lb - lbSig) {
+    const double v = (double)in[today++];
+    pf = fma(fb, pf, fk * v);
+    ps = fma(sb, ps, sk * v);
+  }
+  double mv = pf - ps;
+  double psig = 0.0;
+  psig += mv;
+  for (int i = 0; i < signal - 1; ++i) {
+    const double v = (double)in[today++];
+    pf = fma(fb, pf, fk * v);
+    ps = fma(sb, ps, sk * v);
+    mv = pf - ps;
+    psig += mv;
+  }
+  psig = psig / signal;
+  /* today == lb + 1 */
+  macd[f + lb] = (float)mv;
+  sig[f + lb] = (float)psig;
+  hist[f + lb] = (float)(mv - psig);
+  for (; today < m; ++today) {
+    const double v = (double)in[today];
+    pf = fma(fb, pf, fk * v);
+    ps = fma(sb, ps, sk * v);
+    mv = pf - ps;
+    if (signal == 1)
+      psig = mv;
+    else
+      psig = fma(gb, psig, gk * mv);
+    macd[f + today] = (float)mv;
+    sig[f + today] = (float)psig;
+    hist[f + today] = (float)(mv - psig);
+  }
+  return ORC_OK;
+}
+
+/* ------------------------------------------------------------------ RSI ---
+ * ta_RSI.c:36-200 on x[f..]: Wilder averages seeded by the mean of the
+ * first p gains/losses (scaled by invPeriod), updated as
+ * (avg*(p-1)+delta)*invPeriod; RSI = 100*G/(G+L), 50 when G+L == 0.
+ * pinned: _ref TA_RSI; flat pivot 200/3 and 100/3 (reference
+ * tests/cpp/test_rsi.cpp:26-53); flat series = 50. */
+API int orc_rsi(const float* x, float* out, int n, int p) {
+  if (!x || !out || n <= 0 || p <= 0) return ORC_BAD;
+  fill_nan(out, n);
+  const int f = lead_nan(x, n);
+  const int m = n - f;
+  if (p >= m) return ORC_OK;
+  const float* in = x + f;
+  const double inv = 1.0 / (double)p;
+  double prevValue = (double)in[0];
+  double g = 0.0, l = 0.0;
+  int today = 1;
+  for (int i = p; i > 0; --i) {
+    const double v = (double)in[today++];
+    const double d = v - prevValue;
+    prevValue = v;
+    const double gd = (d > 0.0) ? d : 0.0;
+    g += gd;
+    l += gd - d;
+  }
+  l *= inv;
+  g *= inv;
+  {
+    const double s = g + l;
+    out[f + p] = (float)(s > 0.0 ? 100.0 * (g / s) : 50.0);
+  }
+  for (; today < m; ++today) {
+    const double v = (double)in[today];
+    const double d = v - prevValue;
+    prevValue = v;
+    l *= (double)(p - 1);
+    g *= (double)(p - 1);
+    const double gd = (d > 0.0) ? d : 0.0;
+    g += gd;
+    l += gd - d;
+    l *= inv;
+    g *= inv;
+    const double s = g + l;
+    out[f + today] = (float)(s > 0.0 ? 100.0 * (g / s) : 50.0);
+  }
+  return ORC_OK;
+}
+
+/* True range, ta_ATR.c:135-150 order: start from high-low, replace by the
+ * previous-close distances only when strictly greater. */
+static double true_range(double h, double l, double pc) {
+  double g = h - l;
+  double v2 = fabs(pc - h);
+  if (v2 > g) g = v2;
+  double v3 = fabs(pc - l);
+  if (v3 > g) g = v3;
+  return g;
+}
+
+/* ---------------------------------------------------------------- TRANGE ---
+ * ta_TRANGE.c:36-123: NaN at bar 0, TR afterwards.  pinned: _ref. */
+API int orc_trange(const float* h, const float* l, const float* c, float* out, int n) {
+  if (!h || !l || !c || !out || n <= 0) return ORC_BAD;
+  fill_nan(out, n);
+  for (int t = 1; t < n; ++t) out[t] = (float)true_range(h[t], l[t], c[t - 1]);
+  return ORC_OK;
+}
+
+/* ------------------------------------------------------------------ ATR ---
+ * ta_ATR.c:37-222: seed = mean TR[1..p] at index p; then
+ * fma(wBeta, prev, wAlpha*TR).  No leading-NaN skip (reference
+ * StreamStates.h:675-706).  pinned: _ref TA_ATR; ATR == RMA(TRANGE). */
+static int atr_core(const float* h, const float* l, const float* c, int n, int p,
+                    double* dst /* n doubles, NaN-filled */) {
+  for (int i = 0; i < n; ++i) dst[i] = NAN;
+  if (p >= n) return 0;
+  const double wb = (double)(p - 1) / (double)p;
+  const double wa = 1.0 - wb;
+  double tot = 0.0;
+  int today = 1;
+  for (int i = 0; i < p; ++i, ++today) tot += true_range(h[today], l[today], c[today - 1]);
+  double prev = tot / p;
+  dst[p] = prev;
+  for (; today < n; ++today) {
+    prev = fma(wb, prev, wa * true_range(h[today], l[today], c[today - 1]));
+    dst[today] = prev;
+  }
+  return 1;
+}
+
+API int orc_atr(const float* h, const float* l, const float* c, float* out, int n, int p) {
+  if (!h || !l || !c || !out || n <= 0 || p <= 0) return ORC_BAD;
+  double* tmp = (double*)malloc(sizeof(double) * (size_t)n);
+  atr_core(h, l, c, n, p, tmp);
+  for (int i = 0; i < n; ++i) out[i] = (float)tmp[i];
+  free(tmp);
+  return ORC_OK;
+}
+
+/* NATR = ATR / close * 100, 0 where close is zero (TA_IS_ZERO); reference
+ * NATR.cu:42-231 and docs/IMPLEMENTATION_NOTES.md:46-58.  pinned: derived
+ * from the pinned ATR by an independent division in the test. */
+API int orc_natr(const float* h, const float* l, const float* c, float* out, int n, int p) {
+  if (!h || !l || !c || !out || n <= 0 || p <= 0) return ORC_BAD;
+  double* tmp = (double*)malloc(sizeof(double) * (size_t)n);
+  atr_core(h, l, c, n, p, tmp);
+  for (int i = 0; i < n; ++i) {
+    if (i < p) {
+      out[i] = NAN;
+      continue;
+    }
+    const double cl = (double)c[i];
+    out[i] = is_zero(cl) ? 0.0f : (float)((tmp[i] / cl) * 100.0);
+  }
+  free(tmp);
+  return ORC_OK;
+}
+
+/* ------------------------------------------------------- VAR / STDDEV ---
+ * Population variance of the window with the data shifted by the window's
+ * oldest sample, E[d^2] - E[d]^2, floored to 0 below 1e-12*E[d^2] (the floor
+ * of ta_BBANDS.c:236-239 / var.c).  Exactly 0 on a constant window.
+ * pinned: _ref TA_VAR/TA_STDDEV within tolerance; two-pass brute force. */
+static double window_var(const float* x, int t, int p) {
+  const double shift = (double)x[t - p + 1];
+  const double inv = 1.0 / (double)p;
+  double s1 = 0.0, s2 = 0.0;
+  for (int j = t - p + 1; j <= t; ++j) {
+    double d = (double)x[j] - shift;
+    s1 += d;
+    s2 += d * d;
+  }
+  const double mean = s1 * inv;
+  const double e2 = s2 * inv;
+  double var = e2 - mean * mean;
+  if (var < 1e-12 * e2) var = 0.0;
+  if (!(var >= 0.0) && !isnan(var)) var = 0.0;
+  return var;
+}
+
+API int orc_var(const float* x, float* out, int n, int p) {
+  if (!x || !out || n <= 0 || p <= 0) return ORC_BAD;
+  fill_nan(out, n);
+  for (int t = p - 1; t < n; ++t) out[t] = (float)window_var(x, t, p);
+  return ORC_OK;
+}
+
+API int orc_stddev(const float* x, float* out, int n, int p) {
+  if (!x || !out || n <= 0 || p <= 0) return ORC_BAD;
+  fill_nan(out, n);
+  for (int t = p - 1; t < n; ++t) out[t] = (float)sqrt(window_var(x, t, p));
+  return ORC_OK;
+}
+
+/* --------------------------------------------------------------- BBANDS ---
+ * ta_BBANDS.c:37-397, SMA middle band: mid = SMA, sd = population std;
+ * equal multipliers: mid +/- sd*k, else fma(sd, up, mid) and mid - sd*dn.
+ * pinned: _ref TA_BBANDS within tolerance. */
+API int orc_bbands(const float* x, float* up, float* mid, float* lo, int n, int p,
+                   double kup, double kdn) {
+  if (!x || !up || !mid || !lo || n <= 0 || p <= 0) return ORC_BAD;
+  fill_nan(up, n);
+  fill_nan(mid, n);
+  fill_nan(lo, n);
+  for (int t = p - 1; t < n; ++t) {
+    double s = 0.0;
+    for (int j = t - p + 1; j <= t; ++j) s += (double)x[j];
+    const double m = s / p;
+    const double var = window_var(x, t, p);
+    const double sd = (var != 0.0) ? sqrt(var) : 0.0;
+    mid[t] = (float)m;
+    if (kup == kdn) {
+      const double d = sd * kup;
+      up[t] = (float)(m + d);
+      lo[t] = (float)(m - d);
+    } else {
+      up[t] = (float)fma(sd, kup, m);
+      lo[t] = (float)(m - sd * kdn);
+    }
+  }
+  return ORC_OK;
+}
+
+/* Rolling extrema with the comparison order of the reference WILLR.cu:21-27:
+ * start from the newest bar, replace only on strict improvement. */
+static void window_hl(const float* h, const float* l, int t, int p, float* hh, float* ll) {
+  float hi = h[t], lw = l[t];
+  for (int j = 1; j < p; ++j) {
+    const float a = h[t - j], b = l[t - j];
+    if (a > hi) hi = a;
+    if (b < lw) lw = b;
+  }
+  *hh = hi;
+  *ll = lw;
+}
+
+/* ---------------------------------------------------------------- STOCH ---
+ * ta_STOCH.c:36-322 with SMA smoothing of both stages over dPeriod (the
+ * reference ct_stochastic_d ABI: kPeriod, dPeriod): fastK = (C-LL)/(HH-LL)
+ * *100, 0 when HH-LL is zero against |HH|+|LL| (ta_STOCH.c:262); both
+ * outputs start at K-1 + 2(D-1).  pinned: _ref TA_STOCH within tolerance;
+ * ramp closed form 100K/(K+1) (reference
+ * tests/cpp/test_indicator_audit_regressions.cpp:64-100). */
+API int orc_stoch(const float* h, const float* l, const float* c, float* k, float* d,
+                  int n, int kp, int dp) {
+  if (!h || !l || !c || !k || !d || n <= 0 || kp <= 0 || dp <= 0) return ORC_BAD;
+  fill_nan(k, n);
+  fill_nan(d, n);
+  const int lb = kp - 1 + 2 * (dp - 1);
+  if (lb >= n) return ORC_OK;
+  double* fk = (double*)malloc(sizeof(double) * (size_t)n);
+  double* sk = (double*)malloc(sizeof(double) * (size_t)n);
+  for (int t = kp - 1; t < n; ++t) {
+    float hh, ll;
+    window_hl(h, l, t, kp, &hh, &ll);
+    const double range = (double)hh - (double)ll;
+    if (!is_zero_scaled(range, fabs((double)hh) + fabs((double)ll)))
+      fk[t] = ((double)c[t] - (double)ll) / range * 100.0;
+    else
+      fk[t] = 0.0;
+  }
+  for (int t = kp - 1 + dp - 1; t < n; ++t) {
+    double s = 0.0;
+    for (int j = t - dp + 1; j <= t; ++j) s += fk[j];
+    sk[t] = s / dp;
+  }
+  for (int t = lb; t < n; ++t) {
+    double s = 0.0;
+    for (int j = t - dp + 1; j <= t; ++j) s += sk[j];
+    k[t] = (float)sk[t];
+    d[t] = (float)(s / dp);
+  }
+  free(fk);
+  free(sk);
+  return ORC_OK;
+}
+
+/* ---------------------------------------------------------------- WILLR ---
+ * Reference WILLR.cu:12-30 (TA-Lib WILLR not vendored): extrema in float,
+ * ratio in double, 0 on a zero range, clamped to [-100, 0].
+ * parity vs TA-Lib unpinned; pinned to the reference formula by range,
+ * flat-window and brute-force checks. */
+API int orc_willr(const float* h, const float* l, const float* c, float* out, int n, int p) {
+  if (!h || !l || !c || !out || n <= 0 || p <= 0) return ORC_BAD;
+  fill_nan(out, n);
+  for (int t = p - 1; t < n; ++t) {
+    float hh, ll;
+    window_hl(h, l, t, p, &hh, &ll);
+    const double den = (double)hh - (double)ll;
+    const double v = den == 0.0 ? 0.0 : (((double)hh - (double)c[t]) / den) * -100.0;
+    out[t] = isnan(v) ? NAN : (float)fmin(0.0, fmax(-100.0, v));
+  }
+  return ORC_OK;
+}
+
+/* ------------------------------------------------------------------ CCI ---
+ * Reference CCI.cu:11-37: typical price ((H+L)+C)/3 in double, window mean,
+ * mean absolute deviation; 0 unless numerator and deviation are nonzero.
+ * parity vs TA-Lib unpinned; pinned by constant-window zero and a
+ * brute-force restatement in numpy. */
+API int orc_cci(const float* h, const float* l, const float* c, float* out, int n, int p) {
+  if (!h || !l || !c || !out || n <= 0 || p <= 0) return ORC_BAD;
+  fill_nan(out, n);
+  for (int t = p - 1; t < n; ++t) {
+    double s = 0.0;
+    for (int j = 0; j < p; ++j) {
+      const int i = t - j;
+      s += ((double)h[i] + (double)l[i] + (double)c[i]) / 3.0;
+    }
+    const double sma = s / p;
+    double dev = 0.0;
+    for (int j = 0; j < p; ++j) {
+      const int i = t - j;
+      const double tp = ((double)h[i] + (double)l[i] + (double)c[i]) / 3.0;
+      dev += fabs(tp - sma);
+    }
+    const double md = dev / p;
+    const double tpc = ((double)h[t] + (double)l[t] + (double)c[t]) / 3.0;
+    const double num = tpc - sma;
+    out[t] = (num != 0.0 && md != 0.0) ? (float)(num / (0.015 * md)) : 0.0f;
+  }
+  return ORC_OK;
+}
+
+/* ---------------------------------------------------------------- AROON ---
+ * TA-Lib AROON semantics as restated by reference Aroon.cu:12-50: window of
+ * p+1 bars, newest extremum wins ties, value = (100/p)*(p - bars since).
+ * parity vs TA-Lib unpinned; pinned by the {100k/p} value set and a
+ * brute-force argmax restatement. */
+API int orc_aroon(const float* h, const float* l, float* down, float* up, int n, int p) {
+  if (!h || !l || !down || !up || n <= 0 || p <= 0) return ORC_BAD;
+  fill_nan(down, n);
+  fill_nan(up, n);
+  const double factor = 100.0 / (double)p;
+  for (int t = p; t < n; ++t) {
+    int sh = 0, sl = 0;
+    float mx = h[t], mn = l[t];
+    for (int j = 1; j <= p; ++j) {
+      if (h[t - j] > mx) {
+        mx = h[t - j];
+        sh = j;
+      }
+      if (l[t - j] < mn) {
+        mn = l[t - j];
+        sl = j;
+      }
+    }
+    up[t] = (float)(factor * (double)(p - sh));
+    down[t] = (float)(factor * (double)(p - sl));
+  }
+  return ORC_OK;
+}
+
+API int orc_aroonosc(const float* h, const float* l, float* out, int n, int p) {
+  if (!h || !l || !out || n <= 0 || p <= 0) return ORC_BAD;
+  fill_nan(out, n);
+  const double factor = 100.0 / (double)p;
+  for (int t = p; t < n; ++t) {
+    int sh = 0, sl = 0;
+    float mx = h[t], mn = l[t];
+    for (int j = 1; j <= p; ++j) {
+      if (h[t - j] > mx) {
+        mx = h[t - j];
+        sh = j;
+      }
+      if (l[t - j] < mn) {
+        mn = l[t - j];
+        sl = j;
+      }
+    }
+    /* TA-Lib AROONOSC: factor * (index of high - index of low) */
+    out[t] = (float)(factor * (double)(sl - sh));
+  }
+  return ORC_OK;
+}
+
+/* ------------------------------------------------------------ MIN / MAX ---
+ * Reference MAX.cu:12-30 / MIN.cu: NaN-ignoring window extremum (fmaxf),
+ * except a window that starts inside the leading NaN prefix is NaN
+ * (LeadingNaN.cuh:22-28).  pinned: numpy nanmax brute force. */
+static int in_lead_prefix(const float* x, int idx) {
+  if (!isnan(x[0])) return 0;
+  for (int i = idx - 1; i > 0; --i)
+    if (!isnan(x[i])) return 0;
+  return 1;
+}
+
+API int orc_max(const float* x, float* out, int n, int p) {
+  if (!x || !out || n <= 0 || p <= 0) return ORC_BAD;
+  fill_nan(out, n);
+  for (int s = 0; s + p - 1 < n; ++s) {
+    float m = x[s];
+    for (int k = 1; k < p; ++k) m = fmaxf(m, x[s + k]);
+    if (!isnan(m) && isnan(x[s]) && in_lead_prefix(x, s)) m = NAN;
+    out[s + p - 1] = m;
+  }
+  return ORC_OK;
+}
+
+API int orc_min(const float* x, float* out, int n, int p) {
+  if (!x || !out || n <= 0 || p <= 0) return ORC_BAD;
+  fill_nan(out, n);
+  for (int s = 0; s + p - 1 < n; ++s) {
+    float m = x[s];
+    for (int k = 1; k < p; ++k) m = fminf(m, x[s + k]);
+    if (!isnan(m) && isnan(x[s]) && in_lead_prefix(x, s)) m = NAN;
+    out[s + p - 1] = m;
+  }
+  return ORC_OK;
+}
+
+/* ------------------------------------------------------ directional move ---
+ * TA-Lib DM/DI/DX/ADX (ta_PLUS_DM, ta_PLUS_DI, ta_DX, ta_ADX; not vendored).
+ * Per bar t >= 1: up = H[t]-H[t-1], dn = L[t-1]-L[t]; +DM = up if up > 0 and
+ * up > dn, -DM = dn if dn > 0 and dn > up (reference BatchShort.cu:181-185).
+ * Wilder sums: seed = sum over bars 1..p-1, then prev - prev/p + x.
+ * DI = 100*DM/TR (0 when TR is zero), DX = 100*|+DI - -DI|/(+DI + -DI)
+ * (0 when undefined, reference BatchShort.cu:206-210), ADX = mean of the
+ * first p DX values at 2p-1, then (ADX*(p-1)+DX)/p.
+ * parity vs TA-Lib unpinned; pinned by DI/DX/ADX range invariants, DX
+ * scale invariance, flat data, and a brute-force restatement. */
+static void dm_bar(const float* h, const float* l, int t, double* pdm, double* mdm) {
+  const double up = (double)h[t] - (double)h[t - 1];
+  const double dn = (double)l[t - 1] - (double)l[t];
+  *pdm = (up > 0.0 && up > dn) ? up : 0.0;
+  *mdm = (dn > 0.0 && dn > up) ? dn : 0.0;
+}
+
+enum { DIR_PDM, DIR_MDM, DIR_PDI, DIR_MDI, DIR_DX, DIR_ADX };
+
+static int directional(const float* h, const float* l, const float* c, float* out, int n,
+                       int p, int what) {
+  fill_nan(out, n);
+  if (p == 1) {
+    /* TA-Lib period-1 contracts: raw DM / DI / DX from bar 1. */
+    for (int t = 1; t < n; ++t) {
+      double pd, md;
+      dm_bar(h, l, t, &pd, &md);
+      if (what == DIR_PDM) {
+        out[t] = (float)pd;
+        continue;
+      }
+      if (what == DIR_MDM) {
+        out[t] = (float)md;
+        continue;
+      }
+      const double tr = true_range(h[t], l[t], c[t - 1]);
+      const double pdi = is_zero(tr) ? 0.0 : 100.0 * (pd / tr);
+      const double mdi = is_zero(tr) ? 0.0 : 100.0 * (md / tr);
+      if (what == DIR_PDI) {
+        out[t] = (float)pdi;
+        continue;
+      }
+      if (what == DIR_MDI) {
+        out[t] = (float)mdi;
+        continue;
+      }
+      const double s = pdi + mdi;
+      const double dx = is_zero(s) ? 0.0 : 100.0 * (fabs(mdi - pdi) / s);
+      if (what == DIR_DX || what == DIR_ADX) out[t] = (float)dx;
+    }
+    return ORC_OK;
+  }
+  const int dmOnly = (what == DIR_PDM || what == DIR_MDM);
+  const int lbAll = dmOnly ? p - 1 : (what == DIR_ADX ? 2 * p - 1 : p);
+  if (lbAll >= n) return ORC_OK;
+  const double pp = (double)p;
+  double sp = 0.0, sm = 0.0, st = 0.0;
+  for (int t = 1; t <= p - 1; ++t) {
+    double pd, md;
+    dm_bar(h, l, t, &pd, &md);
+    sp += pd;
+    sm += md;
+    st += true_range(h[t], l[t], c[t - 1]);
+  }
+  if (dmOnly) out[p - 1] = (float)(what == DIR_PDM ? sp : sm);
+  double adx = 0.0, sumdx = 0.0;
+  for (int t = p; t < n; ++t) {
+    double pd, md;
+    dm_bar(h, l, t, &pd, &md);
+    sp = sp - sp / pp + pd;
+    sm = sm - sm / pp + md;
+    st = st - st / pp + true_range(h[t], l[t], c[t - 1]);
+    if (what == DIR_PDM) {
+      out[t] = (float)sp;
+      continue;
+    }
+    if (what == DIR_MDM) {
+      out[t] = (float)sm;
+      continue;
+    }
+    const double pdi = is_zero(st) ? 0.0 : 100.0 * (sp / st);
+    const double mdi = is_zero(st) ? 0.0 : 100.0 * (sm / st);
+    if (what == DIR_PDI) {
+      out[t] = (float)pdi;
+      continue;
+    }
+    if (what == DIR_MDI) {
+      out[t] = (float)mdi;
+      continue;
+    }
+    const double s = pdi + mdi;
+    const double dx = is_zero(s) ? 0.0 : 100.0 * (fabs(mdi - pdi) / s);
+    if (what == DIR_DX) {
+      out[t] = (float)dx;
+      continue;
+    }
+    if (t < 2 * p - 1) {
+      sumdx += dx;
+    } else if (t == 2 * p - 1) {
+      sumdx += dx;
+      adx = sumdx / pp;
+      out[t] = (float)adx;
+    } else {
+      adx = ((adx * (pp - 1.0)) + dx) / pp;
+      out[t] = (float)adx;
+    }
+  }
+  return ORC_OK;
+}
+
+API int orc_plus_dm(const float* h, const float* l, float* out, int n, int p) {
+  if (!h || !l || !out || n <= 0 || p <= 0) return ORC_BAD;
+  return directional(h, l, h, out, n, p, DIR_PDM);
+}
+API int orc_minus_dm(const float* h, const float* l, float* out, int n, int p) {
+  if (!h || !l || !out || n <= 0 || p <= 0) return ORC_BAD;
+  return directional(h, l, h, out, n, p, DIR_MDM);
+}
+API int orc_plus_di(const float* h, const float* l, const float* c, float* out, int n, int p) {
+  if (!h || !l || !c || !out || n <= 0 || p <= 0) return ORC_BAD;
+  return directional(h, l, c, out, n, p, DIR_PDI);
+}
+API int orc_minus_di(const float* h, const float* l, const float* c, float* out, int n, int p) {
+  if (!h || !l || !c || !out || n <= 0 || p <= 0) return ORC_BAD;
+  return directional(h, l, c, out, n, p, DIR_MDI);
+}
+API int orc_dx(const float* h, const float* l, const float* c, float* out, int n, int p) {
+  if (!h || !l || !c || !out || n <= 0 || p <= 0) return ORC_BAD;
+  return directional(h, l, c, out, n, p, DIR_DX);
+}
+API int orc_adx(const float* h, const float* l, const float* c, float* out, int n, int p) {
+  if (!h || !l || !c || !out || n <= 0 || p <= 0) return ORC_BAD;
+  return directional(h, l, c, out, n, p, DIR_ADX);
+}
+
+/* ------------------------------------------------- batch convenience ---
+ * Row loop over a series-major [rows x len] matrix, used by the tests and
+ * the CPU baseline to time the oracle over many series. */
+API int orc_ema_rows(const float* x, float* out, int len, int rows, int p) {
+  for (int r = 0; r < rows; ++r) {
+    int rc = orc_ema(x + (size_t)r * len, out + (size_t)r * len, len, p);
+    if (rc) return rc;
+  }
+  return ORC_OK;
+}

@@ -1,0 +1,131 @@
+// Driver that exposes the reference's own host build of TA-Lib 0.8.3
+// (src/compat/Talib08Formulas.h + talib08_formulas.inc, compiled for the host
+// when __CUDACC__ is not defined) through plain C entry points.
+//
+// TEST INFRASTRUCTURE ONLY: built by oracle/Makefile into oracle/_ref/ from
+// the sources where they lie under the reference checkout; it is the
+// independent check of oracle/oracle.c and an optional CPU baseline.  Nothing
+// here is product code.
+//
+// Every wrapper widens float32 inputs to double, runs the TA_* function over
+// the whole array and returns a full-length float32 line with NaN before
+// outBegIdx, which is the reference ABI's layout.
+#include "Talib08Formulas.h"
+
+#include <cmath>
+#include <cstdlib>
+#include <vector>
+
+using namespace tacuda::talib08::ref;
+
+namespace {
+thread_local std::vector<unsigned char> g_arena_mem;
+
+Arena make_arena(size_t n) {
+  size_t need = n * 64 + (1u << 20);
+  if (g_arena_mem.size() < need) g_arena_mem.resize(need);
+  return Arena{g_arena_mem.data(), g_arena_mem.size()};
+}
+
+std::vector<double> widen(const float* x, int n) { return std::vector<double>(x, x + n); }
+
+void place(float* out, int n, int beg, int nb, const double* v) {
+  for (int i = 0; i < n; ++i) out[i] = NAN;
+  for (int i = 0; i < nb; ++i) out[beg + i] = static_cast<float>(v[i]);
+}
+}  // namespace
+
+#define API extern "C" __attribute__((visibility("default")))
+
+#define REF_1IN_1OUT(NAME, CALL)                                  \
+  API int ref_##NAME(const float* x, float* out, int n, int p) {  \
+    std::vector<double> in = widen(x, n), o(n);                   \
+    Arena a = make_arena(n);                                      \
+    int beg = 0, nb = 0;                                          \
+    int rc = CALL;                                                \
+    if (rc) return rc;                                            \
+    place(out, n, beg, nb, o.data());                             \
+    return 0;                                                     \
+  }
+
+REF_1IN_1OUT(sma, TA_SMA(a, 0, n - 1, in.data(), p, &beg, &nb, o.data()))
+REF_1IN_1OUT(ema, TA_EMA(a, 0, n - 1, in.data(), p, &beg, &nb, o.data()))
+REF_1IN_1OUT(wma, TA_WMA(a, 0, n - 1, in.data(), p, &beg, &nb, o.data()))
+REF_1IN_1OUT(dema, TA_DEMA(a, 0, n - 1, in.data(), p, &beg, &nb, o.data()))
+REF_1IN_1OUT(tema, TA_TEMA(a, 0, n - 1, in.data(), p, &beg, &nb, o.data()))
+REF_1IN_1OUT(kama, TA_KAMA(a, 0, n - 1, in.data(), p, &beg, &nb, o.data()))
+REF_1IN_1OUT(rsi, TA_RSI(a, 0, n - 1, in.data(), p, &beg, &nb, o.data()))
+REF_1IN_1OUT(rma, TA_RMA(a, 0, n - 1, in.data(), p, &beg, &nb, o.data()))
+REF_1IN_1OUT(var, TA_VAR(a, 0, n - 1, in.data(), p, 1.0, &beg, &nb, o.data()))
+REF_1IN_1OUT(stddev, TA_STDDEV(a, 0, n - 1, in.data(), p, 1.0, &beg, &nb, o.data()))
+
+API int ref_t3(const float* x, float* out, int n, int p, double v) {
+  std::vector<double> in = widen(x, n), o(n);
+  Arena a = make_arena(n);
+  int beg = 0, nb = 0;
+  int rc = TA_T3(a, 0, n - 1, in.data(), p, v, &beg, &nb, o.data());
+  if (rc) return rc;
+  place(out, n, beg, nb, o.data());
+  return 0;
+}
+
+API int ref_macd(const float* x, float* m, float* s, float* h, int n, int fast, int slow,
+                 int sig) {
+  std::vector<double> in = widen(x, n), om(n), os(n), oh(n);
+  Arena a = make_arena(n);
+  int beg = 0, nb = 0;
+  int rc = TA_MACD(a, 0, n - 1, in.data(), fast, slow, sig, &beg, &nb, om.data(), os.data(),
+                   oh.data());
+  if (rc) return rc;
+  place(m, n, beg, nb, om.data());
+  place(s, n, beg, nb, os.data());
+  place(h, n, beg, nb, oh.data());
+  return 0;
+}
+
+API int ref_atr(const float* hi, const float* lo, const float* cl, float* out, int n, int p) {
+  std::vector<double> h = widen(hi, n), l = widen(lo, n), c = widen(cl, n), o(n);
+  Arena a = make_arena(n);
+  int beg = 0, nb = 0;
+  int rc = TA_ATR(a, 0, n - 1, h.data(), l.data(), c.data(), p, &beg, &nb, o.data());
+  if (rc) return rc;
+  place(out, n, beg, nb, o.data());
+  return 0;
+}
+
+API int ref_trange(const float* hi, const float* lo, const float* cl, float* out, int n) {
+  std::vector<double> h = widen(hi, n), l = widen(lo, n), c = widen(cl, n), o(n);
+  Arena a = make_arena(n);
+  int beg = 0, nb = 0;
+  int rc = TA_TRANGE(a, 0, n - 1, h.data(), l.data(), c.data(), &beg, &nb, o.data());
+  if (rc) return rc;
+  place(out, n, beg, nb, o.data());
+  return 0;
+}
+
+API int ref_bbands(const float* x, float* up, float* mid, float* lo, int n, int p, double ku,
+                   double kd) {
+  std::vector<double> in = widen(x, n), u(n), m(n), l(n);
+  Arena a = make_arena(n);
+  int beg = 0, nb = 0;
+  int rc = TA_BBANDS(a, 0, n - 1, in.data(), p, ku, kd, 0, &beg, &nb, u.data(), m.data(),
+                     l.data());
+  if (rc) return rc;
+  place(up, n, beg, nb, u.data());
+  place(mid, n, beg, nb, m.data());
+  place(lo, n, beg, nb, l.data());
+  return 0;
+}
+
+API int ref_stoch(const float* hi, const float* lo, const float* cl, float* k, float* d, int n,
+                  int kp, int sp, int dp) {
+  std::vector<double> h = widen(hi, n), l = widen(lo, n), c = widen(cl, n), ok(n), od(n);
+  Arena a = make_arena(n);
+  int beg = 0, nb = 0;
+  int rc = TA_STOCH(a, 0, n - 1, h.data(), l.data(), c.data(), kp, sp, 0, dp, 0, &beg, &nb,
+                    ok.data(), od.data());
+  if (rc) return rc;
+  place(k, n, beg, nb, ok.data());
+  place(d, n, beg, nb, od.data());
+  return 0;
+}

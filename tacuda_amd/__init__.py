@@ -1,0 +1,130 @@
+"""tacuda_amd: MI355X-native TACUDA indicator core.
+
+The compute path is ``libtacuda_amd.so`` (hand-written HIP kernels for gfx950
+behind the reference's C ABI).  This module binds it with ctypes, the way the
+reference's Python binding does (``bindings/python/tacuda/__init__.py:46-123``),
+and adds torch-tensor helpers that pass ``data_ptr()`` and the current HIP
+stream.  There is no CPU fallback: every helper raises if the library is
+missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtacuda_amd.so")
+
+_lib: Optional[ctypes.CDLL] = None
+
+c_int = ctypes.c_int
+c_float = ctypes.c_float
+c_void_p = ctypes.c_void_p
+P = ctypes.c_void_p  # device or host float pointer
+
+# name -> argument types (after which `stream` follows unless noted)
+_SIGS = {}
+
+
+def _sig(name, *args, stream=True, restype=c_int):
+    _SIGS[name] = (list(args) + ([c_void_p] if stream else []), restype)
+
+
+# device single series (include/tacuda_device.h)
+for _n in ("sma", "ema", "wma", "dema", "tema", "rsi", "stddev", "var", "sum", "max", "min"):
+    _sig(f"ct_{_n}_d", P, P, c_int, c_int)
+_sig("ct_t3_d", P, P, c_int, c_int, c_float)
+_sig("ct_kama_d", P, P, c_int, c_int, c_int, c_int)
+_sig("ct_macd_d", P, P, P, P, c_int, c_int, c_int, c_int, c_int)
+_sig("ct_bbands_d", P, P, P, P, c_int, c_int, c_float, c_float)
+_sig("ct_atr_d", P, P, P, P, c_int, c_int, c_float)
+for _n in ("natr", "adx", "dx", "plus_di", "minus_di", "willr", "cci"):
+    _sig(f"ct_{_n}_d", P, P, P, P, c_int, c_int)
+_sig("ct_trange_d", P, P, P, P, c_int)
+_sig("ct_stochastic_d", P, P, P, P, P, c_int, c_int, c_int)
+_sig("ct_aroon_d", P, P, P, P, c_int, c_int)
+_sig("ct_aroonosc_d", P, P, P, c_int, c_int)
+_sig("ct_plus_dm_d", P, P, P, c_int, c_int)
+_sig("ct_minus_dm_d", P, P, P, c_int, c_int)
+# dense batch on device (include/tacuda.h)
+for _n in ("sma", "ema", "rsi", "wma", "dema", "tema", "stddev", "var", "sum", "max", "min"):
+    _sig(f"ct_{_n}_batch_d", P, P, c_int, c_int, c_int, c_int)
+for _n in ("natr", "adx", "dx", "plus_di", "minus_di", "willr", "cci"):
+    _sig(f"ct_{_n}_batch_d", P, P, P, P, c_int, c_int, c_int, c_int)
+_sig("ct_atr_batch_d", P, P, P, P, c_int, c_int, c_int, c_float, c_int)
+_sig("ct_macd_batch_d", P, P, P, P, c_int, c_int, c_int, c_int, c_int, c_int, c_int)
+_sig("ct_bbands_batch_d", P, P, P, P, c_int, c_int, c_int, c_float, c_float, c_int)
+_sig("ct_stochastic_batch_d", P, P, P, P, P, c_int, c_int, c_int, c_int, c_int)
+_sig("ct_kama_batch_d", P, P, c_int, c_int, c_int, c_int, c_int, c_int)
+_sig("ct_t3_batch_d", P, P, c_int, c_int, c_int, c_float, c_int)
+_sig("ct_trange_batch_d", P, P, P, P, c_int, c_int, c_int)
+_sig("ct_aroonosc_batch_d", P, P, P, c_int, c_int, c_int, c_int)
+_sig("ct_aroon_batch_d", P, P, P, P, c_int, c_int, c_int, c_int)
+_sig("ct_plus_dm_batch_d", P, P, P, c_int, c_int, c_int, c_int)
+_sig("ct_minus_dm_batch_d", P, P, P, c_int, c_int, c_int, c_int)
+# host calls
+_sig("ct_sma", P, P, c_int, c_int)
+_sig("ct_ema", P, P, c_int, c_int)
+_sig("ct_rsi", P, P, c_int, c_int)
+# memory / errors
+_sig("ct_device_alloc", c_int, stream=False, restype=c_void_p)
+_sig("ct_device_free", c_void_p, stream=False, restype=None)
+_sig("ct_device_upload", P, P, c_int)
+_sig("ct_device_download", P, P, c_int)
+_sig("ct_device_sync")
+_sig("ct_get_last_error", stream=False, restype=ctypes.c_char_p)
+_sig("ct_status_string", c_int, stream=False, restype=ctypes.c_char_p)
+_sig("ct_version", stream=False, restype=ctypes.c_char_p)
+# extensions (include/tacuda_amd.h)
+_sig("ctx_synth_ohlcv", ctypes.c_ulonglong, c_int, c_int, c_int, P, P, P, P, P, c_int, stream=False)
+_sig("ctx_workspace_reserve", c_void_p, ctypes.c_size_t, stream=False)
+
+
+class TacudaError(RuntimeError):
+    def __init__(self, fn: str, status: int, msg: str):
+        super().__init__(f"{fn} failed with status {status}: {msg}")
+        self.status = status
+
+
+def lib() -> ctypes.CDLL:
+    """Load the HIP library; raise if it was not built (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                "(tacuda_amd has no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    return lib().ct_get_last_error().decode()
+
+
+def call(name: str, *args) -> int:
+    """Call an entry point and raise TacudaError on a non-success status."""
+    rc = getattr(lib(), name)(*args)
+    if rc != 0:
+        raise TacudaError(name, rc, last_error())
+    return rc
+
+
+def synth_ohlcv(n: int, rows: int = 1, seed: int = 0, first: int = 0, threads: int = 0):
+    """Seeded float32 OHLCV as numpy arrays of shape [rows, n] (or [n])."""
+    import numpy as np
+
+    arrs = [np.empty((rows, n), dtype=np.float32) for _ in range(5)]
+    call("ctx_synth_ohlcv", seed, first, rows, n, *[a.ctypes.data for a in arrs], threads)
+    if rows == 1:
+        arrs = [a[0] for a in arrs]
+    return dict(zip(("open", "high", "low", "close", "volume"), arrs))
+
+
+from . import ops  # noqa: E402,F401

@@ -1,0 +1,198 @@
+// Status strings, thread-local errors, version, device memory helpers and the
+// per-stream workspace cache.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "../../include/tacuda_device.h"
+#include "host.hpp"
+
+namespace tca {
+
+namespace {
+thread_local std::string g_error;
+
+struct WsEntry {
+  void* base = nullptr;
+  size_t slots = 0;
+  int device = -1;
+};
+std::mutex g_ws_mu;
+std::map<std::pair<int, hipStream_t>, WsEntry> g_ws;
+size_t g_ws_allocs = 0;
+}  // namespace
+
+void clear_error() { g_error.clear(); }
+
+ctStatus_t fail(ctStatus_t st, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_error = buf;
+  return st;
+}
+
+ctStatus_t hip_fail(hipError_t e, const char* what) {
+  (void)hipGetLastError();
+  ctStatus_t st = CT_STATUS_DEVICE_ERROR;
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) st = CT_STATUS_ALLOC_FAILED;
+  if (e == hipErrorInvalidValue || e == hipErrorInvalidDevicePointer) st = CT_STATUS_INVALID_ARGUMENT;
+  if (e == hipErrorLaunchFailure || e == hipErrorInvalidDeviceFunction) st = CT_STATUS_KERNEL_FAILED;
+  if (e == hipErrorStreamCaptureUnsupported || e == hipErrorStreamCaptureInvalidated)
+    st = CT_STATUS_UNSUPPORTED;
+  return fail(st, "%s: %s", what, hipGetErrorString(e));
+}
+
+ctStatus_t workspace_for(hipStream_t stream, size_t slots, Ws* out) {
+  int dev = 0;
+  TCA_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  WsEntry& w = g_ws[{dev, stream}];
+  if (w.base == nullptr || w.slots < slots) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (stream != nullptr) TCA_HIP(hipStreamIsCapturing(stream, &cs));
+    if (cs != hipStreamCaptureStatusNone)
+      return fail(CT_STATUS_UNSUPPORTED,
+                  "workspace of %zu look-back slots is not reserved for this stream; run the "
+                  "call once eagerly (or call ctx_workspace_reserve) before capturing",
+                  slots);
+    size_t want = slots < 4096 ? 4096 : slots;
+    if (w.base) {
+      TCA_HIP(hipStreamSynchronize(stream));
+      TCA_HIP(hipFree(w.base));
+      w.base = nullptr;
+    }
+    const size_t bytes = sizeof(WsHeader) + want * sizeof(Slot);
+    TCA_HIP(hipMalloc(&w.base, bytes));
+    ++g_ws_allocs;
+    TCA_HIP(hipMemset(w.base, 0, bytes));
+    WsHeader h{};
+    h.epoch = 1;
+    TCA_HIP(hipMemcpy(w.base, &h, sizeof h, hipMemcpyHostToDevice));
+    w.slots = want;
+    w.device = dev;
+  }
+  out->hdr = static_cast<WsHeader*>(w.base);
+  out->slots = reinterpret_cast<Slot*>(static_cast<char*>(w.base) + sizeof(WsHeader));
+  return CT_STATUS_SUCCESS;
+}
+
+void workspace_release_all() {
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  for (auto& kv : g_ws) {
+    if (kv.second.base) {
+      (void)hipSetDevice(kv.second.device);
+      (void)hipDeviceSynchronize();
+      (void)hipFree(kv.second.base);
+    }
+  }
+  g_ws.clear();
+}
+
+size_t workspace_allocations() {
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  return g_ws_allocs;
+}
+
+}  // namespace tca
+
+using namespace tca;
+
+extern "C" {
+
+CTAPI_EXPORT const char* ct_status_string(ctStatus_t s) {
+  switch (s) {
+    case CT_STATUS_SUCCESS: return "success";
+    case CT_STATUS_ALLOC_FAILED: return "allocation failed";
+    case CT_STATUS_COPY_FAILED: return "copy failed";
+    case CT_STATUS_KERNEL_FAILED: return "kernel failed";
+    case CT_STATUS_INVALID_ARGUMENT: return "invalid argument";
+    case CT_STATUS_DEVICE_ERROR: return "device error";
+    case CT_STATUS_UNSUPPORTED: return "unsupported";
+  }
+  return "unknown status";
+}
+
+CTAPI_EXPORT const char* ct_get_last_error(void) { return g_error.c_str(); }
+
+CTAPI_EXPORT const char* ct_version(void) { return CT_VERSION_STRING; }
+
+CTAPI_EXPORT float* ct_device_alloc(int count) {
+  clear_error();
+  if (count <= 0) {
+    fail(CT_STATUS_INVALID_ARGUMENT, "ct_device_alloc: count must be positive");
+    return nullptr;
+  }
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, (size_t)count * sizeof(float));
+  if (e != hipSuccess) {
+    hip_fail(e, "hipMalloc");
+    return nullptr;
+  }
+  return static_cast<float*>(p);
+}
+
+CTAPI_EXPORT void* ct_device_alloc_ex(int count, ctDataType_t t) {
+  clear_error();
+  size_t es = t == CT_DTYPE_F16 ? 2 : t == CT_DTYPE_F32 ? 4 : t == CT_DTYPE_F64 ? 8 : 0;
+  if (count <= 0 || es == 0) {
+    fail(CT_STATUS_INVALID_ARGUMENT, "ct_device_alloc_ex: bad count or data type");
+    return nullptr;
+  }
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, (size_t)count * es);
+  if (e != hipSuccess) {
+    hip_fail(e, "hipMalloc");
+    return nullptr;
+  }
+  return p;
+}
+
+CTAPI_EXPORT void ct_device_free(float* p) {
+  if (p) (void)hipFree(p);
+}
+CTAPI_EXPORT void ct_device_free_ex(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+CTAPI_EXPORT ctStatus_t ct_device_upload(float* d, const float* h, int count, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    if (!d || !h || count <= 0) return fail(CT_STATUS_INVALID_ARGUMENT, "ct_device_upload: bad arguments");
+    hipError_t e = hipMemcpyAsync(d, h, (size_t)count * 4, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) {
+      hip_fail(e, "hipMemcpyAsync");
+      return CT_STATUS_COPY_FAILED;
+    }
+    return CT_STATUS_SUCCESS;
+  });
+}
+
+CTAPI_EXPORT ctStatus_t ct_device_download(float* h, const float* d, int count, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    if (!d || !h || count <= 0) return fail(CT_STATUS_INVALID_ARGUMENT, "ct_device_download: bad arguments");
+    hipError_t e = hipMemcpyAsync(h, d, (size_t)count * 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+      hip_fail(e, "hipMemcpyAsync");
+      return CT_STATUS_COPY_FAILED;
+    }
+    return CT_STATUS_SUCCESS;
+  });
+}
+
+CTAPI_EXPORT ctStatus_t ct_device_sync(hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    TCA_HIP(hipStreamSynchronize(s));
+    return CT_STATUS_SUCCESS;
+  });
+}
+
+CTAPI_EXPORT void ct_device_buffer_pool_cleanup(void) { workspace_release_all(); }
+CTAPI_EXPORT size_t ct_device_buffer_pool_allocation_count(void) { return workspace_allocations(); }
+CTAPI_EXPORT size_t ct_device_buffer_pool_set_max_cache_bytes(size_t bytes) { return bytes; }
+
+}  // extern "C"

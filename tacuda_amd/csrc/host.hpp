@@ -1,0 +1,76 @@
+// Host-side plumbing shared by the C ABI translation units: thread-local
+// error state (reference src/utils/ErrorState.cpp:6-60), argument checks,
+// the per-stream look-back workspace cache and launch helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <exception>
+#include <mutex>
+#include <new>
+#include <string>
+
+#include "../../include/tacuda.h"
+#include "engine.hpp"
+
+namespace tca {
+
+// ------------------------------------------------------------- errors --
+void clear_error();
+ctStatus_t fail(ctStatus_t st, const char* fmt, ...);
+ctStatus_t hip_fail(hipError_t e, const char* what);
+
+#define TCA_HIP(call)                                  \
+  do {                                                 \
+    hipError_t e_ = (call);                            \
+    if (e_ != hipSuccess) return ::tca::hip_fail(e_, #call); \
+  } while (0)
+
+// Runs `body` (returning ctStatus_t) with the error cleared on entry and no
+// C++ exception escaping the ABI (reference api.cpp:217-228).
+template <class F>
+ctStatus_t guarded(F&& body) {
+  clear_error();
+  try {
+    return body();
+  } catch (const std::bad_alloc&) {
+    return fail(CT_STATUS_ALLOC_FAILED, "host allocation failed");
+  } catch (const std::exception& ex) {
+    return fail(CT_STATUS_DEVICE_ERROR, "%s", ex.what());
+  } catch (...) {
+    return fail(CT_STATUS_DEVICE_ERROR, "unknown error");
+  }
+}
+
+// ---------------------------------------------------------- workspace --
+// Look-back slots plus the ticket/epoch header, one per HIP stream so that
+// launches ordered on one stream share it safely.  Growing it is refused
+// while the stream is being captured (the graph would keep a stale pointer).
+ctStatus_t workspace_for(hipStream_t stream, size_t slots, Ws* out);
+void workspace_release_all();
+size_t workspace_allocations();
+
+// ------------------------------------------------------------ geometry --
+inline int div_up(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+inline Geometry dense_geometry(int len, int rows, int seg) {
+  Geometry g{};
+  g.offsets = nullptr;
+  g.segStart = nullptr;
+  g.ld = len;
+  g.len = len;
+  g.rows = rows;
+  g.segsPerRow = div_up(len, seg);
+  g.totalSegs = g.segsPerRow * rows;
+  return g;
+}
+
+// Synchronize when the caller used the legacy default stream: batch calls
+// on stream 0 return once the batch is done (reference api_batch.cu:377-386).
+inline ctStatus_t finish_batch(hipStream_t stream) {
+  if (stream == nullptr) TCA_HIP(hipStreamSynchronize(nullptr));
+  return CT_STATUS_SUCCESS;
+}
+
+}  // namespace tca

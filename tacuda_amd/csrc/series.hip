@@ -1,0 +1,765 @@
+// Kernel template, launch policy and the C ABI for single-series (ct_*_d),
+// host (ct_*) and dense batch (ct_*_batch[_d]) calls of the indicator core.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "../../include/tacuda_amd.h"
+#include "../../include/tacuda_device.h"
+#include "families.hpp"
+#include "host.hpp"
+#include "series.hpp"
+
+namespace tca {
+
+// One wave per segment; W waves per block share nothing but the block.
+template <class F, int K, int H, int NIN, int W>
+__global__ __launch_bounds__(W* kWave) void k_seg(Geometry g, Ws ws, int useLB,
+                                                   typename F::P prm, IO io) {
+  using T = Tile<K, H>;
+  __shared__ __align__(16) float s_in[W][NIN * T::PLEN];
+  __shared__ __align__(16) float s_out[W][T::POUT];
+  const int w = threadIdx.x / kWave;
+  Seg c;
+  c.epoch = 0;
+  int ticket;
+  if (useLB)
+    ticket = take_ticket(ws, c.epoch);
+  else
+    ticket = blockIdx.x * W + w;
+  if (resolve(g, T::SEG, ticket, c)) {
+    c.ws = ws;
+    F::template run<T>(c, s_in[w], s_out[w], prm, io);
+  }
+  if (useLB) finish_wave(ws, gridDim.x * W);
+}
+
+template <class F, int K, int H, int NIN, int W>
+static ctStatus_t launch_geom(Geometry g, int maxSegsPerRow, hipStream_t s, typename F::P prm,
+                              IO io) {
+  if (g.totalSegs <= 0) return CT_STATUS_SUCCESS;
+  const bool useLB = F::kScan && maxSegsPerRow > 1;
+  Ws ws{nullptr, nullptr};
+  if (useLB) {
+    ctStatus_t st = workspace_for(s, (size_t)(F::kLevels > 0 ? F::kLevels : 1) * g.totalSegs, &ws);
+    if (st != CT_STATUS_SUCCESS) return st;
+  }
+  const int blocks = div_up(g.totalSegs, W);
+  hipLaunchKernelGGL((k_seg<F, K, H, NIN, W>), dim3(blocks), dim3(W * kWave), 0, s, g, ws,
+                     useLB ? 1 : 0, prm, io);
+  TCA_HIP(hipGetLastError());
+  return CT_STATUS_SUCCESS;
+}
+
+// Short rows (<= 256 bars) use 4 bars per lane so a row fills one wave;
+// longer rows use the family's long tile and chain segments by look-back.
+template <class F, int KL, int HL, int KS, int HS, int NIN, int W>
+static ctStatus_t launch_dense(int len, int rows, hipStream_t s, typename F::P prm, IO io) {
+  if (len <= KS * kWave) {
+    Geometry g = dense_geometry(len, rows, KS * kWave);
+    return launch_geom<F, KS, HS, NIN, W>(g, g.segsPerRow, s, prm, io);
+  }
+  Geometry g = dense_geometry(len, rows, KL * kWave);
+  return launch_geom<F, KL, HL, NIN, W>(g, g.segsPerRow, s, prm, io);
+}
+
+// ----------------------------------------------------- per-family policy --
+#define TCA_POLICY(FAM, KL, HL, KS, HS, NIN, W)                                            \
+  ctStatus_t run_##FAM(int len, int rows, hipStream_t s, const F##FAM::P& p, IO io) {     \
+    return launch_dense<F##FAM, KL, HL, KS, HS, NIN, W>(len, rows, s, p, io);              \
+  }
+
+TCA_POLICY(Ema, 16, 16, 4, 4, 1, 4)
+TCA_POLICY(Dema, 16, 16, 4, 4, 1, 4)
+TCA_POLICY(Tema, 16, 16, 4, 4, 1, 4)
+TCA_POLICY(T3, 8, 8, 4, 4, 1, 4)
+TCA_POLICY(Macd, 16, 16, 4, 4, 1, 4)
+TCA_POLICY(Rsi, 16, 16, 4, 4, 1, 4)
+TCA_POLICY(Kama, 16, 64, 4, 64, 1, 4)
+TCA_POLICY(Atr, 16, 16, 4, 4, 3, 2)
+TCA_POLICY(Natr, 16, 16, 4, 4, 3, 2)
+TCA_POLICY(Sma, 16, 64, 4, 64, 1, 4)
+TCA_POLICY(Sum, 16, 64, 4, 64, 1, 4)
+TCA_POLICY(Wma, 16, 64, 4, 64, 1, 4)
+TCA_POLICY(Var, 16, 64, 4, 64, 1, 4)
+TCA_POLICY(Stddev, 16, 64, 4, 64, 1, 4)
+TCA_POLICY(Bbands, 16, 64, 4, 64, 1, 4)
+TCA_POLICY(Max, 16, 64, 4, 64, 1, 4)
+TCA_POLICY(Min, 16, 64, 4, 64, 1, 4)
+TCA_POLICY(Willr, 16, 64, 4, 64, 2, 2)
+TCA_POLICY(Cci, 8, 64, 4, 64, 3, 2)
+TCA_POLICY(Aroon, 16, 64, 4, 64, 2, 2)
+TCA_POLICY(AroonOsc, 16, 64, 4, 64, 2, 2)
+TCA_POLICY(Trange, 16, 16, 4, 4, 3, 2)
+TCA_POLICY(Stoch, 8, 64, 4, 64, 3, 2)
+#undef TCA_POLICY
+
+#define TCA_DIR_POLICY(NAME, KIND)                                                          \
+  ctStatus_t run_##NAME(int len, int rows, hipStream_t s, int p, IO io) {                   \
+    typename FDir<KIND>::P pr{p};                                                           \
+    return launch_dense<FDir<KIND>, 16, 16, 4, 4, 3, 2>(len, rows, s, pr, io);              \
+  }
+TCA_DIR_POLICY(PlusDm, kPDM)
+TCA_DIR_POLICY(MinusDm, kMDM)
+TCA_DIR_POLICY(PlusDi, kPDI)
+TCA_DIR_POLICY(MinusDi, kMDI)
+TCA_DIR_POLICY(Dx, kDX)
+TCA_DIR_POLICY(Adx, kADX)
+#undef TCA_DIR_POLICY
+
+}  // namespace tca
+
+// =================================================================== C ABI ==
+using namespace tca;
+
+namespace {
+
+inline IO io1(const float* a, float* o0, float* o1 = nullptr, float* o2 = nullptr) {
+  IO io{};
+  io.in[0] = a;
+  io.out[0] = o0;
+  io.out[1] = o1;
+  io.out[2] = o2;
+  return io;
+}
+inline IO io3(const float* h, const float* l, const float* c, float* o0, float* o1 = nullptr) {
+  IO io{};
+  io.in[0] = h;
+  io.in[1] = l;
+  io.in[2] = c;
+  io.out[0] = o0;
+  io.out[1] = o1;
+  return io;
+}
+
+ctStatus_t bad(const char* fn, const char* why) {
+  return fail(CT_STATUS_INVALID_ARGUMENT, "%s: %s", fn, why);
+}
+
+// Shape checks shared by single-series and batch entry points.
+ctStatus_t check_shape(const char* fn, int len, int rows) {
+  if (len <= 0) return bad(fn, "size/seriesLen must be positive");
+  if (rows <= 0) return bad(fn, "numSeries must be positive");
+  if ((long long)len * rows > (long long)INT32_MAX * 16)
+    return bad(fn, "matrix too large");
+  return CT_STATUS_SUCCESS;
+}
+
+}  // namespace
+
+// Core device-pointer dispatch used by _d, _batch_d and the host wrappers.
+namespace tca {
+
+#define CHECK_PTR(p, fn) \
+  if (!(p)) return bad(fn, "null pointer")
+#define CHECK_RET(x)                            \
+  do {                                          \
+    ctStatus_t st_ = (x);                       \
+    if (st_ != CT_STATUS_SUCCESS) return st_;   \
+  } while (0)
+
+ctStatus_t dev_sma(const float* x, float* y, int len, int rows, int p, hipStream_t s) {
+  CHECK_PTR(x, "sma"); CHECK_PTR(y, "sma");
+  CHECK_RET(check_shape("sma", len, rows));
+  if (p <= 0) return bad("sma", "invalid period");
+  return run_Sma(len, rows, s, {p}, io1(x, y));
+}
+ctStatus_t dev_sum(const float* x, float* y, int len, int rows, int p, hipStream_t s) {
+  CHECK_PTR(x, "sum"); CHECK_PTR(y, "sum");
+  CHECK_RET(check_shape("sum", len, rows));
+  if (p <= 0) return bad("sum", "invalid period");
+  return run_Sum(len, rows, s, {p}, io1(x, y));
+}
+ctStatus_t dev_wma(const float* x, float* y, int len, int rows, int p, hipStream_t s) {
+  CHECK_PTR(x, "wma"); CHECK_PTR(y, "wma");
+  CHECK_RET(check_shape("wma", len, rows));
+  if (p <= 0) return bad("wma", "invalid period");
+  return run_Wma(len, rows, s, {p}, io1(x, y));
+}
+ctStatus_t dev_ema(const float* x, float* y, int len, int rows, int p, hipStream_t s) {
+  CHECK_PTR(x, "ema"); CHECK_PTR(y, "ema");
+  CHECK_RET(check_shape("ema", len, rows));
+  if (p <= 0) return bad("ema", "invalid period");
+  return run_Ema(len, rows, s, {p}, io1(x, y));
+}
+ctStatus_t dev_dema(const float* x, float* y, int len, int rows, int p, hipStream_t s) {
+  CHECK_PTR(x, "dema"); CHECK_PTR(y, "dema");
+  CHECK_RET(check_shape("dema", len, rows));
+  if (p <= 0) return bad("dema", "invalid period");
+  return run_Dema(len, rows, s, {p}, io1(x, y));
+}
+ctStatus_t dev_tema(const float* x, float* y, int len, int rows, int p, hipStream_t s) {
+  CHECK_PTR(x, "tema"); CHECK_PTR(y, "tema");
+  CHECK_RET(check_shape("tema", len, rows));
+  if (p <= 0) return bad("tema", "invalid period");
+  return run_Tema(len, rows, s, {p}, io1(x, y));
+}
+ctStatus_t dev_t3(const float* x, float* y, int len, int rows, int p, float v, hipStream_t s) {
+  CHECK_PTR(x, "t3"); CHECK_PTR(y, "t3");
+  CHECK_RET(check_shape("t3", len, rows));
+  if (p < 2) return bad("t3", "invalid period (T3 needs period >= 2)");
+  return run_T3(len, rows, s, {p, (double)v}, io1(x, y));
+}
+ctStatus_t dev_kama(const float* x, float* y, int len, int rows, int p, int fast, int slow,
+                    hipStream_t s) {
+  CHECK_PTR(x, "kama"); CHECK_PTR(y, "kama");
+  CHECK_RET(check_shape("kama", len, rows));
+  if (p <= 0 || fast <= 0 || slow <= 0) return bad("kama", "invalid period");
+  return run_Kama(len, rows, s, {p, fast, slow}, io1(x, y));
+}
+ctStatus_t dev_rsi(const float* x, float* y, int len, int rows, int p, hipStream_t s) {
+  CHECK_PTR(x, "rsi"); CHECK_PTR(y, "rsi");
+  CHECK_RET(check_shape("rsi", len, rows));
+  if (p <= 0) return bad("rsi", "invalid period");
+  return run_Rsi(len, rows, s, {p}, io1(x, y));
+}
+ctStatus_t dev_macd(const float* x, float* m, float* sg, float* h, int len, int rows, int fast,
+                    int slow, int sig, int maType, hipStream_t s) {
+  CHECK_PTR(x, "macd"); CHECK_PTR(m, "macd"); CHECK_PTR(sg, "macd"); CHECK_PTR(h, "macd");
+  CHECK_RET(check_shape("macd", len, rows));
+  if (fast <= 0 || slow <= 0 || sig <= 0) return bad("macd", "invalid periods");
+  if (maType != CT_MA_EMA && maType != CT_MA_DEFAULT)
+    return fail(CT_STATUS_UNSUPPORTED, "macd: only the EMA moving-average type is provided");
+  if (slow < fast) std::swap(slow, fast);
+  return run_Macd(len, rows, s, {fast, slow, sig}, io1(x, m, sg, h));
+}
+ctStatus_t dev_bbands(const float* x, float* u, float* m, float* l, int len, int rows, int p,
+                      float up, float dn, hipStream_t s) {
+  CHECK_PTR(x, "bbands"); CHECK_PTR(u, "bbands"); CHECK_PTR(m, "bbands"); CHECK_PTR(l, "bbands");
+  CHECK_RET(check_shape("bbands", len, rows));
+  if (p <= 0) return bad("bbands", "invalid period");
+  return run_Bbands(len, rows, s, {p, (double)up, (double)dn}, io1(x, u, m, l));
+}
+ctStatus_t dev_var(const float* x, float* y, int len, int rows, int p, hipStream_t s) {
+  CHECK_PTR(x, "var"); CHECK_PTR(y, "var");
+  CHECK_RET(check_shape("var", len, rows));
+  if (p <= 0) return bad("var", "invalid period");
+  return run_Var(len, rows, s, {p}, io1(x, y));
+}
+ctStatus_t dev_stddev(const float* x, float* y, int len, int rows, int p, hipStream_t s) {
+  CHECK_PTR(x, "stddev"); CHECK_PTR(y, "stddev");
+  CHECK_RET(check_shape("stddev", len, rows));
+  if (p <= 0) return bad("stddev", "invalid period");
+  return run_Stddev(len, rows, s, {p}, io1(x, y));
+}
+ctStatus_t dev_max(const float* x, float* y, int len, int rows, int p, hipStream_t s) {
+  CHECK_PTR(x, "max"); CHECK_PTR(y, "max");
+  CHECK_RET(check_shape("max", len, rows));
+  if (p <= 0) return bad("max", "invalid period");
+  return run_Max(len, rows, s, {p}, io1(x, y));
+}
+ctStatus_t dev_min(const float* x, float* y, int len, int rows, int p, hipStream_t s) {
+  CHECK_PTR(x, "min"); CHECK_PTR(y, "min");
+  CHECK_RET(check_shape("min", len, rows));
+  if (p <= 0) return bad("min", "invalid period");
+  return run_Min(len, rows, s, {p}, io1(x, y));
+}
+ctStatus_t dev_atr(const float* h, const float* l, const float* c, float* y, int len, int rows,
+                   int p, hipStream_t s) {
+  CHECK_PTR(h, "atr"); CHECK_PTR(l, "atr"); CHECK_PTR(c, "atr"); CHECK_PTR(y, "atr");
+  CHECK_RET(check_shape("atr", len, rows));
+  if (p <= 0) return bad("atr", "invalid period");
+  return run_Atr(len, rows, s, {p}, io3(h, l, c, y));
+}
+ctStatus_t dev_natr(const float* h, const float* l, const float* c, float* y, int len, int rows,
+                    int p, hipStream_t s) {
+  CHECK_PTR(h, "natr"); CHECK_PTR(l, "natr"); CHECK_PTR(c, "natr"); CHECK_PTR(y, "natr");
+  CHECK_RET(check_shape("natr", len, rows));
+  if (p <= 0) return bad("natr", "invalid period");
+  return run_Natr(len, rows, s, {p}, io3(h, l, c, y));
+}
+ctStatus_t dev_trange(const float* h, const float* l, const float* c, float* y, int len, int rows,
+                      hipStream_t s) {
+  CHECK_PTR(h, "trange"); CHECK_PTR(l, "trange"); CHECK_PTR(c, "trange"); CHECK_PTR(y, "trange");
+  CHECK_RET(check_shape("trange", len, rows));
+  return run_Trange(len, rows, s, {0}, io3(h, l, c, y));
+}
+ctStatus_t dev_stoch(const float* h, const float* l, const float* c, float* k, float* d, int len,
+                     int rows, int kp, int dp, hipStream_t s) {
+  CHECK_PTR(h, "stochastic"); CHECK_PTR(l, "stochastic"); CHECK_PTR(c, "stochastic");
+  CHECK_PTR(k, "stochastic"); CHECK_PTR(d, "stochastic");
+  CHECK_RET(check_shape("stochastic", len, rows));
+  if (kp <= 0 || dp <= 0) return bad("stochastic", "invalid period");
+  if (dp > kStochMaxD)
+    return fail(CT_STATUS_UNSUPPORTED, "stochastic: dPeriod above %d is not provided", kStochMaxD);
+  return run_Stoch(len, rows, s, {kp, dp}, io3(h, l, c, k, d));
+}
+ctStatus_t dev_willr(const float* h, const float* l, const float* c, float* y, int len, int rows,
+                     int p, hipStream_t s) {
+  CHECK_PTR(h, "willr"); CHECK_PTR(l, "willr"); CHECK_PTR(c, "willr"); CHECK_PTR(y, "willr");
+  CHECK_RET(check_shape("willr", len, rows));
+  if (p <= 0) return bad("willr", "invalid period");
+  return run_Willr(len, rows, s, {p}, io3(h, l, c, y));
+}
+ctStatus_t dev_cci(const float* h, const float* l, const float* c, float* y, int len, int rows,
+                   int p, hipStream_t s) {
+  CHECK_PTR(h, "cci"); CHECK_PTR(l, "cci"); CHECK_PTR(c, "cci"); CHECK_PTR(y, "cci");
+  CHECK_RET(check_shape("cci", len, rows));
+  if (p <= 0) return bad("cci", "invalid period");
+  return run_Cci(len, rows, s, {p}, io3(h, l, c, y));
+}
+ctStatus_t dev_aroon(const float* h, const float* l, float* dn, float* up, int len, int rows,
+                     int p, hipStream_t s) {
+  CHECK_PTR(h, "aroon"); CHECK_PTR(l, "aroon"); CHECK_PTR(dn, "aroon"); CHECK_PTR(up, "aroon");
+  CHECK_RET(check_shape("aroon", len, rows));
+  if (p <= 0) return bad("aroon", "invalid period");
+  return run_Aroon(len, rows, s, {p}, io3(h, l, nullptr, dn, up));
+}
+ctStatus_t dev_aroonosc(const float* h, const float* l, float* y, int len, int rows, int p,
+                        hipStream_t s) {
+  CHECK_PTR(h, "aroonosc"); CHECK_PTR(l, "aroonosc"); CHECK_PTR(y, "aroonosc");
+  CHECK_RET(check_shape("aroonosc", len, rows));
+  if (p <= 0) return bad("aroonosc", "invalid period");
+  return run_AroonOsc(len, rows, s, {p}, io3(h, l, nullptr, y));
+}
+#define DIR_DEV(NAME, RUN, NEEDC)                                                            \
+  ctStatus_t dev_##NAME(const float* h, const float* l, const float* c, float* y, int len,  \
+                        int rows, int p, hipStream_t s) {                                   \
+    CHECK_PTR(h, #NAME); CHECK_PTR(l, #NAME); CHECK_PTR(y, #NAME);                          \
+    if (NEEDC) CHECK_PTR(c, #NAME);                                                         \
+    CHECK_RET(check_shape(#NAME, len, rows));                                               \
+    if (p <= 0) return bad(#NAME, "invalid period");                                        \
+    return RUN(len, rows, s, p, io3(h, l, c, y));                                           \
+  }
+DIR_DEV(plus_dm, run_PlusDm, false)
+DIR_DEV(minus_dm, run_MinusDm, false)
+DIR_DEV(plus_di, run_PlusDi, true)
+DIR_DEV(minus_di, run_MinusDi, true)
+DIR_DEV(dx, run_Dx, true)
+DIR_DEV(adx, run_Adx, true)
+#undef DIR_DEV
+
+}  // namespace tca
+
+// ------------------------------------------------------ host staging --
+namespace {
+
+// Device staging for the host entry points: upload the inputs, run, download
+// the outputs, synchronize (reference api.cpp:230-287).
+struct HostRun {
+  hipStream_t s;
+  std::vector<void*> bufs;
+  explicit HostRun(hipStream_t st) : s(st) {}
+  ~HostRun() {
+    for (void* b : bufs) (void)hipFree(b);
+  }
+  ctStatus_t in(const float* h, size_t n, const float** d) {
+    if (!h) return bad("host call", "null pointer");
+    void* p = nullptr;
+    TCA_HIP(hipMalloc(&p, n * 4));
+    bufs.push_back(p);
+    TCA_HIP(hipMemcpyAsync(p, h, n * 4, hipMemcpyHostToDevice, s));
+    *d = static_cast<const float*>(p);
+    return CT_STATUS_SUCCESS;
+  }
+  ctStatus_t out(float* h, size_t n, float** d) {
+    if (!h) return bad("host call", "null pointer");
+    void* p = nullptr;
+    TCA_HIP(hipMalloc(&p, n * 4));
+    bufs.push_back(p);
+    *d = static_cast<float*>(p);
+    return CT_STATUS_SUCCESS;
+  }
+  ctStatus_t back(float* h, const float* d, size_t n) {
+    TCA_HIP(hipMemcpyAsync(h, d, n * 4, hipMemcpyDeviceToHost, s));
+    return CT_STATUS_SUCCESS;
+  }
+  ctStatus_t sync() {
+    TCA_HIP(hipStreamSynchronize(s));
+    return CT_STATUS_SUCCESS;
+  }
+};
+
+size_t elems(int len, int rows) { return (size_t)(len > 0 ? len : 0) * (rows > 0 ? rows : 0); }
+
+}  // namespace
+
+#define R_(x) CHECK_RET(x)
+
+extern "C" {
+
+// ---------------------------------------------------------- device (_d) --
+CTAPI_EXPORT ctStatus_t ct_sma_d(const float* x, float* y, int n, int p, hipStream_t s) { return guarded([&] { return dev_sma(x, y, n, 1, p, s); }); }
+CTAPI_EXPORT ctStatus_t ct_sum_d(const float* x, float* y, int n, int p, hipStream_t s) { return guarded([&] { return dev_sum(x, y, n, 1, p, s); }); }
+CTAPI_EXPORT ctStatus_t ct_wma_d(const float* x, float* y, int n, int p, hipStream_t s) { return guarded([&] { return dev_wma(x, y, n, 1, p, s); }); }
+CTAPI_EXPORT ctStatus_t ct_ema_d(const float* x, float* y, int n, int p, hipStream_t s) { return guarded([&] { return dev_ema(x, y, n, 1, p, s); }); }
+CTAPI_EXPORT ctStatus_t ct_dema_d(const float* x, float* y, int n, int p, hipStream_t s) { return guarded([&] { return dev_dema(x, y, n, 1, p, s); }); }
+CTAPI_EXPORT ctStatus_t ct_tema_d(const float* x, float* y, int n, int p, hipStream_t s) { return guarded([&] { return dev_tema(x, y, n, 1, p, s); }); }
+CTAPI_EXPORT ctStatus_t ct_t3_d(const float* x, float* y, int n, int p, float v, hipStream_t s) { return guarded([&] { return dev_t3(x, y, n, 1, p, v, s); }); }
+CTAPI_EXPORT ctStatus_t ct_kama_d(const float* x, float* y, int n, int p, int fast, int slow, hipStream_t s) { return guarded([&] { return dev_kama(x, y, n, 1, p, fast, slow, s); }); }
+CTAPI_EXPORT ctStatus_t ct_rsi_d(const float* x, float* y, int n, int p, hipStream_t s) { return guarded([&] { return dev_rsi(x, y, n, 1, p, s); }); }
+CTAPI_EXPORT ctStatus_t ct_macd_d(const float* x, float* m, float* sg, float* h, int n, int fast, int slow, int sig, int maType, hipStream_t s) { return guarded([&] { return dev_macd(x, m, sg, h, n, 1, fast, slow, sig, maType, s); }); }
+CTAPI_EXPORT ctStatus_t ct_bbands_d(const float* x, float* u, float* m, float* l, int n, int p, float up, float dn, hipStream_t s) { return guarded([&] { return dev_bbands(x, u, m, l, n, 1, p, up, dn, s); }); }
+CTAPI_EXPORT ctStatus_t ct_var_d(const float* x, float* y, int n, int p, hipStream_t s) { return guarded([&] { return dev_var(x, y, n, 1, p, s); }); }
+CTAPI_EXPORT ctStatus_t ct_stddev_d(const float* x, float* y, int n, int p, hipStream_t s) { return guarded([&] { return dev_stddev(x, y, n, 1, p, s); }); }
+CTAPI_EXPORT ctStatus_t ct_max_d(const float* x, float* y, int n, int p, hipStream_t s) { return guarded([&] { return dev_max(x, y, n, 1, p, s); }); }
+CTAPI_EXPORT ctStatus_t ct_min_d(const float* x, float* y, int n, int p, hipStream_t s) { return guarded([&] { return dev_min(x, y, n, 1, p, s); }); }
+CTAPI_EXPORT ctStatus_t ct_atr_d(const float* h, const float* l, const float* c, float* y, int n, int p, float, hipStream_t s) { return guarded([&] { return dev_atr(h, l, c, y, n, 1, p, s); }); }
+CTAPI_EXPORT ctStatus_t ct_natr_d(const float* h, const float* l, const float* c, float* y, int n, int p, hipStream_t s) { return guarded([&] { return dev_natr(h, l, c, y, n, 1, p, s); }); }
+CTAPI_EXPORT ctStatus_t ct_trange_d(const float* h, const float* l, const float* c, float* y, int n, hipStream_t s) { return guarded([&] { return dev_trange(h, l, c, y, n, 1, s); }); }
+CTAPI_EXPORT ctStatus_t ct_stochastic_d(const float* h, const float* l, const float* c, float* k, float* d, int n, int kp, int dp, hipStream_t s) { return guarded([&] { return dev_stoch(h, l, c, k, d, n, 1, kp, dp, s); }); }
+CTAPI_EXPORT ctStatus_t ct_willr_d(const float* h, const float* l, const float* c, float* y, int n, int p, hipStream_t s) { return guarded([&] { return dev_willr(h, l, c, y, n, 1, p, s); }); }
+CTAPI_EXPORT ctStatus_t ct_cci_d(const float* h, const float* l, const float* c, float* y, int n, int p, hipStream_t s) { return guarded([&] { return dev_cci(h, l, c, y, n, 1, p, s); }); }
+CTAPI_EXPORT ctStatus_t ct_aroon_d(const float* h, const float* l, float* dn, float* up, int n, int p, hipStream_t s) { return guarded([&] { return dev_aroon(h, l, dn, up, n, 1, p, s); }); }
+CTAPI_EXPORT ctStatus_t ct_aroonosc_d(const float* h, const float* l, float* y, int n, int p, hipStream_t s) { return guarded([&] { return dev_aroonosc(h, l, y, n, 1, p, s); }); }
+CTAPI_EXPORT ctStatus_t ct_plus_dm_d(const float* h, const float* l, float* y, int n, int p, hipStream_t s) { return guarded([&] { return dev_plus_dm(h, l, nullptr, y, n, 1, p, s); }); }
+CTAPI_EXPORT ctStatus_t ct_minus_dm_d(const float* h, const float* l, float* y, int n, int p, hipStream_t s) { return guarded([&] { return dev_minus_dm(h, l, nullptr, y, n, 1, p, s); }); }
+CTAPI_EXPORT ctStatus_t ct_plus_di_d(const float* h, const float* l, const float* c, float* y, int n, int p, hipStream_t s) { return guarded([&] { return dev_plus_di(h, l, c, y, n, 1, p, s); }); }
+CTAPI_EXPORT ctStatus_t ct_minus_di_d(const float* h, const float* l, const float* c, float* y, int n, int p, hipStream_t s) { return guarded([&] { return dev_minus_di(h, l, c, y, n, 1, p, s); }); }
+CTAPI_EXPORT ctStatus_t ct_dx_d(const float* h, const float* l, const float* c, float* y, int n, int p, hipStream_t s) { return guarded([&] { return dev_dx(h, l, c, y, n, 1, p, s); }); }
+CTAPI_EXPORT ctStatus_t ct_adx_d(const float* h, const float* l, const float* c, float* y, int n, int p, hipStream_t s) { return guarded([&] { return dev_adx(h, l, c, y, n, 1, p, s); }); }
+
+// ------------------------------------------------------------ batch (_d) --
+#define B1D(NAME)                                                                                     \
+  CTAPI_EXPORT ctStatus_t ct_##NAME##_batch_d(const float* x, float* y, int len, int rows, int p,     \
+                                              int, hipStream_t s) {                                  \
+    return guarded([&]() -> ctStatus_t {                                                              \
+      R_(dev_##NAME(x, y, len, rows, p, s));                                                          \
+      return finish_batch(s);                                                                         \
+    });                                                                                               \
+  }
+B1D(sma) B1D(ema) B1D(rsi) B1D(wma) B1D(dema) B1D(tema) B1D(stddev) B1D(var) B1D(sum) B1D(max) B1D(min)
+#undef B1D
+
+#define BHLCD(NAME)                                                                                   \
+  CTAPI_EXPORT ctStatus_t ct_##NAME##_batch_d(const float* h, const float* l, const float* c,         \
+                                              float* y, int len, int rows, int p, int,               \
+                                              hipStream_t s) {                                       \
+    return guarded([&]() -> ctStatus_t {                                                              \
+      R_(dev_##NAME(h, l, c, y, len, rows, p, s));                                                    \
+      return finish_batch(s);                                                                         \
+    });                                                                                               \
+  }
+BHLCD(natr) BHLCD(adx) BHLCD(dx) BHLCD(plus_di) BHLCD(minus_di) BHLCD(willr) BHLCD(cci)
+#undef BHLCD
+
+CTAPI_EXPORT ctStatus_t ct_atr_batch_d(const float* h, const float* l, const float* c, float* y, int len, int rows, int p, float, int, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t { R_(dev_atr(h, l, c, y, len, rows, p, s)); return finish_batch(s); });
+}
+CTAPI_EXPORT ctStatus_t ct_macd_batch_d(const float* x, float* m, float* sg, float* h, int len, int rows, int fast, int slow, int sig, int maType, int, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t { R_(dev_macd(x, m, sg, h, len, rows, fast, slow, sig, maType, s)); return finish_batch(s); });
+}
+CTAPI_EXPORT ctStatus_t ct_bbands_batch_d(const float* x, float* u, float* m, float* l, int len, int rows, int p, float up, float dn, int, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t { R_(dev_bbands(x, u, m, l, len, rows, p, up, dn, s)); return finish_batch(s); });
+}
+CTAPI_EXPORT ctStatus_t ct_stochastic_batch_d(const float* h, const float* l, const float* c, float* k, float* d, int len, int rows, int kp, int dp, int, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t { R_(dev_stoch(h, l, c, k, d, len, rows, kp, dp, s)); return finish_batch(s); });
+}
+CTAPI_EXPORT ctStatus_t ct_kama_batch_d(const float* x, float* y, int len, int rows, int p, int fast, int slow, int, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t { R_(dev_kama(x, y, len, rows, p, fast, slow, s)); return finish_batch(s); });
+}
+CTAPI_EXPORT ctStatus_t ct_t3_batch_d(const float* x, float* y, int len, int rows, int p, float v, int, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t { R_(dev_t3(x, y, len, rows, p, v, s)); return finish_batch(s); });
+}
+CTAPI_EXPORT ctStatus_t ct_trange_batch_d(const float* h, const float* l, const float* c, float* y, int len, int rows, int, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t { R_(dev_trange(h, l, c, y, len, rows, s)); return finish_batch(s); });
+}
+CTAPI_EXPORT ctStatus_t ct_aroonosc_batch_d(const float* h, const float* l, float* y, int len, int rows, int p, int, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t { R_(dev_aroonosc(h, l, y, len, rows, p, s)); return finish_batch(s); });
+}
+CTAPI_EXPORT ctStatus_t ct_aroon_batch_d(const float* h, const float* l, float* dn, float* up, int len, int rows, int p, int, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t { R_(dev_aroon(h, l, dn, up, len, rows, p, s)); return finish_batch(s); });
+}
+CTAPI_EXPORT ctStatus_t ct_plus_dm_batch_d(const float* h, const float* l, float* y, int len, int rows, int p, int, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t { R_(dev_plus_dm(h, l, nullptr, y, len, rows, p, s)); return finish_batch(s); });
+}
+CTAPI_EXPORT ctStatus_t ct_minus_dm_batch_d(const float* h, const float* l, float* y, int len, int rows, int p, int, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t { R_(dev_minus_dm(h, l, nullptr, y, len, rows, p, s)); return finish_batch(s); });
+}
+
+// ----------------------------------------------------- host single + batch --
+// Host variants upload, run the device path and download synchronously.
+#define H1(NAME, DEVCALL)                                                                          \
+  CTAPI_EXPORT ctStatus_t ct_##NAME(const float* hx, float* hy, int n, int p, hipStream_t s) {     \
+    return guarded([&]() -> ctStatus_t {                                                           \
+      if (n <= 0) return bad(#NAME, "size must be positive");                                      \
+      HostRun r(s);                                                                                \
+      const float* x;                                                                              \
+      float* y;                                                                                    \
+      R_(r.in(hx, n, &x));                                                                         \
+      R_(r.out(hy, n, &y));                                                                        \
+      R_(DEVCALL);                                                                                 \
+      R_(r.back(hy, y, n));                                                                        \
+      return r.sync();                                                                             \
+    });                                                                                            \
+  }
+H1(sma, dev_sma(x, y, n, 1, p, s))
+H1(wma, dev_wma(x, y, n, 1, p, s))
+H1(ema, dev_ema(x, y, n, 1, p, s))
+H1(dema, dev_dema(x, y, n, 1, p, s))
+H1(tema, dev_tema(x, y, n, 1, p, s))
+H1(max, dev_max(x, y, n, 1, p, s))
+H1(min, dev_min(x, y, n, 1, p, s))
+H1(stddev, dev_stddev(x, y, n, 1, p, s))
+H1(var, dev_var(x, y, n, 1, p, s))
+H1(sum, dev_sum(x, y, n, 1, p, s))
+H1(rsi, dev_rsi(x, y, n, 1, p, s))
+#undef H1
+
+CTAPI_EXPORT ctStatus_t ct_t3(const float* hx, float* hy, int n, int p, float v, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    if (n <= 0) return bad("t3", "size must be positive");
+    HostRun r(s); const float* x; float* y;
+    R_(r.in(hx, n, &x)); R_(r.out(hy, n, &y)); R_(dev_t3(x, y, n, 1, p, v, s)); R_(r.back(hy, y, n));
+    return r.sync();
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_kama(const float* hx, float* hy, int n, int p, int fast, int slow, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    if (n <= 0) return bad("kama", "size must be positive");
+    HostRun r(s); const float* x; float* y;
+    R_(r.in(hx, n, &x)); R_(r.out(hy, n, &y)); R_(dev_kama(x, y, n, 1, p, fast, slow, s)); R_(r.back(hy, y, n));
+    return r.sync();
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_macd(const float* hx, float* hm, float* hs, float* hh, int n, int fast, int slow, int sig, ctMaType_t t, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    if (n <= 0) return bad("macd", "size must be positive");
+    HostRun r(s); const float* x; float *m, *g, *h;
+    R_(r.in(hx, n, &x)); R_(r.out(hm, n, &m)); R_(r.out(hs, n, &g)); R_(r.out(hh, n, &h));
+    R_(dev_macd(x, m, g, h, n, 1, fast, slow, sig, (int)t, s));
+    R_(r.back(hm, m, n)); R_(r.back(hs, g, n)); R_(r.back(hh, h, n));
+    return r.sync();
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_bbands(const float* hx, float* hu, float* hm, float* hl, int n, int p, float up, float dn, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    if (n <= 0) return bad("bbands", "size must be positive");
+    HostRun r(s); const float* x; float *u, *m, *l;
+    R_(r.in(hx, n, &x)); R_(r.out(hu, n, &u)); R_(r.out(hm, n, &m)); R_(r.out(hl, n, &l));
+    R_(dev_bbands(x, u, m, l, n, 1, p, up, dn, s));
+    R_(r.back(hu, u, n)); R_(r.back(hm, m, n)); R_(r.back(hl, l, n));
+    return r.sync();
+  });
+}
+
+#define HHLC(NAME, DEVCALL)                                                                        \
+  CTAPI_EXPORT ctStatus_t ct_##NAME(const float* hh, const float* hl, const float* hc, float* hy,  \
+                                    int n, int p, hipStream_t s) {                                 \
+    return guarded([&]() -> ctStatus_t {                                                           \
+      if (n <= 0) return bad(#NAME, "size must be positive");                                      \
+      HostRun r(s);                                                                                \
+      const float *h, *l, *c;                                                                      \
+      float* y;                                                                                    \
+      R_(r.in(hh, n, &h)); R_(r.in(hl, n, &l)); R_(r.in(hc, n, &c)); R_(r.out(hy, n, &y));         \
+      R_(DEVCALL);                                                                                 \
+      R_(r.back(hy, y, n));                                                                        \
+      return r.sync();                                                                             \
+    });                                                                                            \
+  }
+HHLC(natr, dev_natr(h, l, c, y, n, 1, p, s))
+HHLC(cci, dev_cci(h, l, c, y, n, 1, p, s))
+HHLC(adx, dev_adx(h, l, c, y, n, 1, p, s))
+HHLC(dx, dev_dx(h, l, c, y, n, 1, p, s))
+HHLC(plus_di, dev_plus_di(h, l, c, y, n, 1, p, s))
+HHLC(minus_di, dev_minus_di(h, l, c, y, n, 1, p, s))
+HHLC(willr, dev_willr(h, l, c, y, n, 1, p, s))
+#undef HHLC
+
+CTAPI_EXPORT ctStatus_t ct_atr(const float* hh, const float* hl, const float* hc, float* hy, int n, int p, float, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    if (n <= 0) return bad("atr", "size must be positive");
+    HostRun r(s); const float *h, *l, *c; float* y;
+    R_(r.in(hh, n, &h)); R_(r.in(hl, n, &l)); R_(r.in(hc, n, &c)); R_(r.out(hy, n, &y));
+    R_(dev_atr(h, l, c, y, n, 1, p, s)); R_(r.back(hy, y, n));
+    return r.sync();
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_trange(const float* hh, const float* hl, const float* hc, float* hy, int n, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    if (n <= 0) return bad("trange", "size must be positive");
+    HostRun r(s); const float *h, *l, *c; float* y;
+    R_(r.in(hh, n, &h)); R_(r.in(hl, n, &l)); R_(r.in(hc, n, &c)); R_(r.out(hy, n, &y));
+    R_(dev_trange(h, l, c, y, n, 1, s)); R_(r.back(hy, y, n));
+    return r.sync();
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_stochastic(const float* hh, const float* hl, const float* hc, float* hk, float* hd, int n, int kp, int dp, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    if (n <= 0) return bad("stochastic", "size must be positive");
+    HostRun r(s); const float *h, *l, *c; float *k, *d;
+    R_(r.in(hh, n, &h)); R_(r.in(hl, n, &l)); R_(r.in(hc, n, &c)); R_(r.out(hk, n, &k)); R_(r.out(hd, n, &d));
+    R_(dev_stoch(h, l, c, k, d, n, 1, kp, dp, s)); R_(r.back(hk, k, n)); R_(r.back(hd, d, n));
+    return r.sync();
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_plus_dm(const float* hh, const float* hl, float* hy, int n, int p, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    if (n <= 0) return bad("plus_dm", "size must be positive");
+    HostRun r(s); const float *h, *l; float* y;
+    R_(r.in(hh, n, &h)); R_(r.in(hl, n, &l)); R_(r.out(hy, n, &y));
+    R_(dev_plus_dm(h, l, nullptr, y, n, 1, p, s)); R_(r.back(hy, y, n));
+    return r.sync();
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_minus_dm(const float* hh, const float* hl, float* hy, int n, int p, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    if (n <= 0) return bad("minus_dm", "size must be positive");
+    HostRun r(s); const float *h, *l; float* y;
+    R_(r.in(hh, n, &h)); R_(r.in(hl, n, &l)); R_(r.out(hy, n, &y));
+    R_(dev_minus_dm(h, l, nullptr, y, n, 1, p, s)); R_(r.back(hy, y, n));
+    return r.sync();
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_aroonosc(const float* hh, const float* hl, float* hy, int n, int p, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    if (n <= 0) return bad("aroonosc", "size must be positive");
+    HostRun r(s); const float *h, *l; float* y;
+    R_(r.in(hh, n, &h)); R_(r.in(hl, n, &l)); R_(r.out(hy, n, &y));
+    R_(dev_aroonosc(h, l, y, n, 1, p, s)); R_(r.back(hy, y, n));
+    return r.sync();
+  });
+}
+// Host AROON keeps the reference's host order (up, down, osc) and two periods
+// (tacuda.h:385-388); the oscillator uses the up period's window.
+CTAPI_EXPORT ctStatus_t ct_aroon(const float* hh, const float* hl, float* hup, float* hdn, float* hosc, int n, int upP, int dnP, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    if (n <= 0) return bad("aroon", "size must be positive");
+    if (upP != dnP) return fail(CT_STATUS_UNSUPPORTED, "aroon: distinct up/down periods are not provided");
+    HostRun r(s); const float *h, *l; float *u, *d, *o;
+    R_(r.in(hh, n, &h)); R_(r.in(hl, n, &l)); R_(r.out(hup, n, &u)); R_(r.out(hdn, n, &d)); R_(r.out(hosc, n, &o));
+    R_(dev_aroon(h, l, d, u, n, 1, upP, s)); R_(dev_aroonosc(h, l, o, n, 1, upP, s));
+    R_(r.back(hup, u, n)); R_(r.back(hdn, d, n)); R_(r.back(hosc, o, n));
+    return r.sync();
+  });
+}
+
+#define HB1(NAME)                                                                                   \
+  CTAPI_EXPORT ctStatus_t ct_##NAME##_batch(const float* hx, float* hy, int len, int rows, int p,   \
+                                            int, hipStream_t s) {                                  \
+    return guarded([&]() -> ctStatus_t {                                                            \
+      R_(check_shape(#NAME, len, rows));                                                            \
+      const size_t n = elems(len, rows);                                                            \
+      HostRun r(s);                                                                                 \
+      const float* x;                                                                               \
+      float* y;                                                                                     \
+      R_(r.in(hx, n, &x)); R_(r.out(hy, n, &y));                                                    \
+      R_(dev_##NAME(x, y, len, rows, p, s));                                                        \
+      R_(r.back(hy, y, n));                                                                         \
+      return r.sync();                                                                              \
+    });                                                                                             \
+  }
+HB1(sma) HB1(ema) HB1(rsi) HB1(wma) HB1(dema) HB1(tema) HB1(stddev) HB1(var) HB1(sum) HB1(max) HB1(min)
+#undef HB1
+
+#define HBHLC(NAME)                                                                                 \
+  CTAPI_EXPORT ctStatus_t ct_##NAME##_batch(const float* hh, const float* hl, const float* hc,      \
+                                            float* hy, int len, int rows, int p, int,              \
+                                            hipStream_t s) {                                       \
+    return guarded([&]() -> ctStatus_t {                                                            \
+      R_(check_shape(#NAME, len, rows));                                                            \
+      const size_t n = elems(len, rows);                                                            \
+      HostRun r(s);                                                                                 \
+      const float *h, *l, *c;                                                                       \
+      float* y;                                                                                     \
+      R_(r.in(hh, n, &h)); R_(r.in(hl, n, &l)); R_(r.in(hc, n, &c)); R_(r.out(hy, n, &y));          \
+      R_(dev_##NAME(h, l, c, y, len, rows, p, s));                                                  \
+      R_(r.back(hy, y, n));                                                                         \
+      return r.sync();                                                                              \
+    });                                                                                             \
+  }
+HBHLC(natr) HBHLC(adx) HBHLC(dx) HBHLC(plus_di) HBHLC(minus_di) HBHLC(willr) HBHLC(cci)
+#undef HBHLC
+
+CTAPI_EXPORT ctStatus_t ct_atr_batch(const float* hh, const float* hl, const float* hc, float* hy, int len, int rows, int p, float, int, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    R_(check_shape("atr", len, rows)); const size_t n = elems(len, rows);
+    HostRun r(s); const float *h, *l, *c; float* y;
+    R_(r.in(hh, n, &h)); R_(r.in(hl, n, &l)); R_(r.in(hc, n, &c)); R_(r.out(hy, n, &y));
+    R_(dev_atr(h, l, c, y, len, rows, p, s)); R_(r.back(hy, y, n));
+    return r.sync();
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_macd_batch(const float* hx, float* hm, float* hs, float* hh, int len, int rows, int fast, int slow, int sig, int maType, int, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    R_(check_shape("macd", len, rows)); const size_t n = elems(len, rows);
+    HostRun r(s); const float* x; float *m, *g, *h;
+    R_(r.in(hx, n, &x)); R_(r.out(hm, n, &m)); R_(r.out(hs, n, &g)); R_(r.out(hh, n, &h));
+    R_(dev_macd(x, m, g, h, len, rows, fast, slow, sig, maType, s));
+    R_(r.back(hm, m, n)); R_(r.back(hs, g, n)); R_(r.back(hh, h, n));
+    return r.sync();
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_bbands_batch(const float* hx, float* hu, float* hm, float* hl, int len, int rows, int p, float up, float dn, int, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    R_(check_shape("bbands", len, rows)); const size_t n = elems(len, rows);
+    HostRun r(s); const float* x; float *u, *m, *l;
+    R_(r.in(hx, n, &x)); R_(r.out(hu, n, &u)); R_(r.out(hm, n, &m)); R_(r.out(hl, n, &l));
+    R_(dev_bbands(x, u, m, l, len, rows, p, up, dn, s));
+    R_(r.back(hu, u, n)); R_(r.back(hm, m, n)); R_(r.back(hl, l, n));
+    return r.sync();
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_stochastic_batch(const float* hh, const float* hl, const float* hc, float* hk, float* hd, int len, int rows, int kp, int dp, int, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    R_(check_shape("stochastic", len, rows)); const size_t n = elems(len, rows);
+    HostRun r(s); const float *h, *l, *c; float *k, *d;
+    R_(r.in(hh, n, &h)); R_(r.in(hl, n, &l)); R_(r.in(hc, n, &c)); R_(r.out(hk, n, &k)); R_(r.out(hd, n, &d));
+    R_(dev_stoch(h, l, c, k, d, len, rows, kp, dp, s)); R_(r.back(hk, k, n)); R_(r.back(hd, d, n));
+    return r.sync();
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_kama_batch(const float* hx, float* hy, int len, int rows, int p, int fast, int slow, int, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    R_(check_shape("kama", len, rows)); const size_t n = elems(len, rows);
+    HostRun r(s); const float* x; float* y;
+    R_(r.in(hx, n, &x)); R_(r.out(hy, n, &y)); R_(dev_kama(x, y, len, rows, p, fast, slow, s)); R_(r.back(hy, y, n));
+    return r.sync();
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_t3_batch(const float* hx, float* hy, int len, int rows, int p, float v, int, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    R_(check_shape("t3", len, rows)); const size_t n = elems(len, rows);
+    HostRun r(s); const float* x; float* y;
+    R_(r.in(hx, n, &x)); R_(r.out(hy, n, &y)); R_(dev_t3(x, y, len, rows, p, v, s)); R_(r.back(hy, y, n));
+    return r.sync();
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_trange_batch(const float* hh, const float* hl, const float* hc, float* hy, int len, int rows, int, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    R_(check_shape("trange", len, rows)); const size_t n = elems(len, rows);
+    HostRun r(s); const float *h, *l, *c; float* y;
+    R_(r.in(hh, n, &h)); R_(r.in(hl, n, &l)); R_(r.in(hc, n, &c)); R_(r.out(hy, n, &y));
+    R_(dev_trange(h, l, c, y, len, rows, s)); R_(r.back(hy, y, n));
+    return r.sync();
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_aroonosc_batch(const float* hh, const float* hl, float* hy, int len, int rows, int p, int, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    R_(check_shape("aroonosc", len, rows)); const size_t n = elems(len, rows);
+    HostRun r(s); const float *h, *l; float* y;
+    R_(r.in(hh, n, &h)); R_(r.in(hl, n, &l)); R_(r.out(hy, n, &y));
+    R_(dev_aroonosc(h, l, y, len, rows, p, s)); R_(r.back(hy, y, n));
+    return r.sync();
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_aroon_batch(const float* hh, const float* hl, float* hd, float* hu, int len, int rows, int p, int, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    R_(check_shape("aroon", len, rows)); const size_t n = elems(len, rows);
+    HostRun r(s); const float *h, *l; float *d, *u;
+    R_(r.in(hh, n, &h)); R_(r.in(hl, n, &l)); R_(r.out(hd, n, &d)); R_(r.out(hu, n, &u));
+    R_(dev_aroon(h, l, d, u, len, rows, p, s)); R_(r.back(hd, d, n)); R_(r.back(hu, u, n));
+    return r.sync();
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_plus_dm_batch(const float* hh, const float* hl, float* hy, int len, int rows, int p, int, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    R_(check_shape("plus_dm", len, rows)); const size_t n = elems(len, rows);
+    HostRun r(s); const float *h, *l; float* y;
+    R_(r.in(hh, n, &h)); R_(r.in(hl, n, &l)); R_(r.out(hy, n, &y));
+    R_(dev_plus_dm(h, l, nullptr, y, len, rows, p, s)); R_(r.back(hy, y, n));
+    return r.sync();
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_minus_dm_batch(const float* hh, const float* hl, float* hy, int len, int rows, int p, int, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    R_(check_shape("minus_dm", len, rows)); const size_t n = elems(len, rows);
+    HostRun r(s); const float *h, *l; float* y;
+    R_(r.in(hh, n, &h)); R_(r.in(hl, n, &l)); R_(r.out(hy, n, &y));
+    R_(dev_minus_dm(h, l, nullptr, y, len, rows, p, s)); R_(r.back(hy, y, n));
+    return r.sync();
+  });
+}
+
+}  // extern "C"

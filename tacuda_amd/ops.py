@@ -1,0 +1,149 @@
+"""Torch-tensor front end of the C ABI.
+
+Each function takes CUDA (HIP) float32 tensors, allocates outputs with
+``torch.empty_like`` and calls the matching ``ct_*_d`` (1-D input) or
+``ct_*_batch_d`` (2-D series-major input) entry point on the current stream.
+Names and argument meaning follow the reference's ctypes binding.
+"""
+from __future__ import annotations
+
+import torch
+
+from . import call
+
+_ONE_IN = ("sma", "ema", "wma", "dema", "tema", "rsi", "stddev", "var", "sum", "max", "min")
+_HLC = ("natr", "adx", "dx", "plus_di", "minus_di", "willr", "cci")
+
+
+def _stream() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _check(*ts):
+    for t in ts:
+        if not (t.is_cuda and t.dtype == torch.float32 and t.is_contiguous()):
+            raise ValueError("expected contiguous float32 tensors on the GPU")
+        if t.dim() not in (1, 2):
+            raise ValueError("expected a 1-D series or a 2-D [series, bars] matrix")
+
+
+def _shape(x):
+    return (x.shape[0], 1) if x.dim() == 1 else (x.shape[1], x.shape[0])
+
+
+def _dispatch(name, x, ins, outs, *params, batch_params=None):
+    n, rows = _shape(x)
+    if x.dim() == 1:
+        call(f"ct_{name}_d", *[t.data_ptr() for t in ins + outs], n, *params, _stream())
+    else:
+        bp = params if batch_params is None else batch_params
+        call(f"ct_{name}_batch_d", *[t.data_ptr() for t in ins + outs], n, rows, *bp, 0,
+             _stream())
+
+
+def _make_one(name):
+    def f(x: torch.Tensor, period: int) -> torch.Tensor:
+        _check(x)
+        y = torch.empty_like(x)
+        _dispatch(name, x, [x], [y], int(period))
+        return y
+
+    f.__name__ = name
+    f.__doc__ = f"{name.upper()} of a series (ct_{name}_d / ct_{name}_batch_d)."
+    return f
+
+
+def _make_hlc(name):
+    def f(high, low, close, period: int) -> torch.Tensor:
+        _check(high, low, close)
+        y = torch.empty_like(close)
+        _dispatch(name, close, [high, low, close], [y], int(period))
+        return y
+
+    f.__name__ = name
+    f.__doc__ = f"{name.upper()} of high/low/close (ct_{name}_d / ct_{name}_batch_d)."
+    return f
+
+
+for _n in _ONE_IN:
+    globals()[_n] = _make_one(_n)
+for _n in _HLC:
+    globals()[_n] = _make_hlc(_n)
+
+
+def t3(x, period: int, vfactor: float = 0.7):
+    _check(x)
+    y = torch.empty_like(x)
+    _dispatch("t3", x, [x], [y], int(period), float(vfactor))
+    return y
+
+
+def kama(x, period: int, fast: int = 2, slow: int = 30):
+    _check(x)
+    y = torch.empty_like(x)
+    _dispatch("kama", x, [x], [y], int(period), int(fast), int(slow))
+    return y
+
+
+def macd(x, fast: int = 12, slow: int = 26, signal: int = 9, ma_type: int = 1):
+    _check(x)
+    m, s, h = (torch.empty_like(x) for _ in range(3))
+    _dispatch("macd", x, [x], [m, s, h], fast, slow, signal, ma_type)
+    return m, s, h
+
+
+def bbands(x, period: int = 20, up: float = 2.0, dn: float = 2.0):
+    _check(x)
+    u, m, l = (torch.empty_like(x) for _ in range(3))
+    _dispatch("bbands", x, [x], [u, m, l], int(period), float(up), float(dn))
+    return u, m, l
+
+
+def atr(high, low, close, period: int = 14):
+    _check(high, low, close)
+    y = torch.empty_like(close)
+    _dispatch("atr", close, [high, low, close], [y], int(period), 0.0)
+    return y
+
+
+def trange(high, low, close):
+    _check(high, low, close)
+    y = torch.empty_like(close)
+    _dispatch("trange", close, [high, low, close], [y])
+    return y
+
+
+def stochastic(high, low, close, k_period: int = 14, d_period: int = 3):
+    _check(high, low, close)
+    k, d = torch.empty_like(close), torch.empty_like(close)
+    _dispatch("stochastic", close, [high, low, close], [k, d], int(k_period), int(d_period))
+    return k, d
+
+
+def aroon(high, low, period: int = 14):
+    """Returns (down, up) like the reference ct_aroon_d."""
+    _check(high, low)
+    d, u = torch.empty_like(high), torch.empty_like(high)
+    _dispatch("aroon", high, [high, low], [d, u], int(period))
+    return d, u
+
+
+def aroonosc(high, low, period: int = 14):
+    _check(high, low)
+    y = torch.empty_like(high)
+    _dispatch("aroonosc", high, [high, low], [y], int(period))
+    return y
+
+
+def plus_dm(high, low, period: int = 14):
+    _check(high, low)
+    y = torch.empty_like(high)
+    _dispatch("plus_dm", high, [high, low], [y], int(period))
+    return y
+
+
+def minus_dm(high, low, period: int = 14):
+    _check(high, low)
+    y = torch.empty_like(high)
+    _dispatch("minus_dm", high, [high, low], [y], int(period))
+    return y
