@@ -14,9 +14,10 @@
 extern "C" {
 #endif
 
-/* Reserve the per-stream look-back workspace for `slots` segment slots so a
- * later call can be captured into a hipGraph without allocating. */
-CTAPI_EXPORT ctStatus_t ctx_workspace_reserve(hipStream_t stream, size_t slots);
+/* Reserve `bytes` of the per-stream look-back workspace so a later call can
+ * be captured into a hipGraph without allocating (an eager call of the same
+ * shape on the stream reserves it too). */
+CTAPI_EXPORT ctStatus_t ctx_workspace_reserve(hipStream_t stream, size_t bytes);
 
 /* Seeded synthetic OHLCV, float32, series-major [numSeries x n]:
  * close_0 = 100, close_t = close_{t-1} * exp(0.01*z_r); open_t = close_{t-1};

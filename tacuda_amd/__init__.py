@@ -14,7 +14,7 @@ import os
 from typing import Optional
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libtacuda_amd.so")
+LIB_PATH = os.environ.get("TACUDA_AMD_LIB", os.path.join(_HERE, "libtacuda_amd.so"))
 
 _lib: Optional[ctypes.CDLL] = None
 

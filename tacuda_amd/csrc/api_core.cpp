@@ -16,7 +16,7 @@ thread_local std::string g_error;
 
 struct WsEntry {
   void* base = nullptr;
-  size_t slots = 0;
+  size_t bytes = 0;
   int device = -1;
 };
 std::mutex g_ws_mu;
@@ -47,37 +47,37 @@ ctStatus_t hip_fail(hipError_t e, const char* what) {
   return fail(st, "%s: %s", what, hipGetErrorString(e));
 }
 
-ctStatus_t workspace_for(hipStream_t stream, size_t slots, Ws* out) {
+ctStatus_t workspace_for(hipStream_t stream, size_t treeBytes, Ws* out) {
   int dev = 0;
   TCA_HIP(hipGetDevice(&dev));
   std::lock_guard<std::mutex> lk(g_ws_mu);
   WsEntry& w = g_ws[{dev, stream}];
-  if (w.base == nullptr || w.slots < slots) {
+  if (w.base == nullptr || w.bytes < treeBytes) {
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (stream != nullptr) TCA_HIP(hipStreamIsCapturing(stream, &cs));
     if (cs != hipStreamCaptureStatusNone)
       return fail(CT_STATUS_UNSUPPORTED,
-                  "workspace of %zu look-back slots is not reserved for this stream; run the "
+                  "look-back workspace of %zu bytes is not reserved for this stream; run the "
                   "call once eagerly (or call ctx_workspace_reserve) before capturing",
-                  slots);
-    size_t want = slots < 4096 ? 4096 : slots;
+                  treeBytes);
+    size_t want = treeBytes < (1u << 20) ? (1u << 20) : treeBytes;
     if (w.base) {
       TCA_HIP(hipStreamSynchronize(stream));
       TCA_HIP(hipFree(w.base));
       w.base = nullptr;
     }
-    const size_t bytes = sizeof(WsHeader) + want * sizeof(Slot);
+    const size_t bytes = sizeof(WsHeader) + want;
     TCA_HIP(hipMalloc(&w.base, bytes));
     ++g_ws_allocs;
     TCA_HIP(hipMemset(w.base, 0, bytes));
     WsHeader h{};
     h.epoch = 1;
     TCA_HIP(hipMemcpy(w.base, &h, sizeof h, hipMemcpyHostToDevice));
-    w.slots = want;
+    w.bytes = want;
     w.device = dev;
   }
   out->hdr = static_cast<WsHeader*>(w.base);
-  out->slots = reinterpret_cast<Slot*>(static_cast<char*>(w.base) + sizeof(WsHeader));
+  out->area = reinterpret_cast<double*>(static_cast<char*>(w.base) + sizeof(WsHeader));
   return CT_STATUS_SUCCESS;
 }
 

@@ -9,9 +9,14 @@
 //    memory (L2).
 //  * Recurrences are affine maps y -> A*y + b.  Each lane composes its bars,
 //    a DPP wave scan (row_shr 1/2/4/8, row_bcast 15/31, wave_shr 1) gives the
-//    exclusive carry, and rows longer than one segment chain segments with a
-//    decoupled look-back whose flags carry a device-side epoch, so the
-//    workspace needs no clearing between launches or graph replays.
+//    exclusive carry.  Rows longer than one segment combine segments through
+//    a two-level tree in global memory: every segment publishes its
+//    transform, the last wave of each group of 64 consecutive segments
+//    publishes the group's composite, and a segment composes the group
+//    composites and in-group transforms before it -- a fixed number of
+//    memory round trips however long the row.  Slot flags carry a
+//    device-side epoch, so the workspace needs no clearing between launches
+//    or graph replays.
 //  * Arithmetic of every recurrence is FP64; lanes replay the TA-Lib update
 //    order from their carry so the outputs match the sequential reference to
 //    the last float32 bit except where a carry differs in the last FP64 ulp.
@@ -24,20 +29,18 @@ namespace tca {
 constexpr int kWave = 64;
 
 // --------------------------------------------------------------- workspace --
+// Launch bookkeeping at the head of a workspace.  Completion is counted on
+// 64 striped counters (one cache line each) so blocks do not serialize on a
+// single address; the block that completes the last stripe bumps the epoch.
+constexpr int kStripes = 64;
 struct WsHeader {
-  unsigned int ticket;  // wave tickets of the running launch
-  unsigned int done;    // finished waves of the running launch
-  unsigned int epoch;   // launch generation, bumped by the last wave
-  unsigned int pad[61];
+  unsigned int epoch;         // launch generation, bumped when a launch ends
+  unsigned int stripesDone;   // stripes completed by the running launch
+  unsigned int pad0[14];
+  unsigned int stripe[kStripes][16];  // blocks completed per stripe
 };
 
-// One look-back slot: flag = epoch*4 + state, payload = (A, b) per recurrence.
-constexpr unsigned kAgg = 1, kIncl = 2;
-constexpr int kSlotDoubles = 7;  // up to 3 recurrences (A,b) + 1 spare
-struct alignas(64) Slot {
-  unsigned long long flag;
-  double v[kSlotDoubles];
-};
+constexpr int kGroup = 64;  // segments per group of the look-back tree
 
 struct Geometry {
   // Row r of a dense matrix starts at r*ld; ragged rows use offsets[].
@@ -50,9 +53,12 @@ struct Geometry {
   int totalSegs;
 };
 
+// Look-back area: per level, one slot per segment then one per group.  A
+// slot is `stride` doubles: [0] = flag (epoch), [1..2R] = (A, b) pairs.
 struct Ws {
   WsHeader* hdr;
-  Slot* slots;  // levels * totalSegs
+  double* area;
+  int stride;
 };
 
 // ------------------------------------------------------------ lane helpers --
@@ -96,13 +102,29 @@ __device__ __forceinline__ void aff_id(Aff<R>& v) {
   }
 }
 
+// Multipliers are products of decay factors in [0, 1], combined with plain
+// fma so a zero factor keeps TA-Lib's literal arithmetic (0 * NaN is NaN).
+// A product of nonzero factors that underflows is clamped to the smallest
+// normal double, so a carried Inf or NaN survives it as it does in the
+// sequential recurrence.  A structural reset -- a segment that (re)starts the
+// state at its seed bar, or holds no valid bar yet -- is marked by A = -0.0
+// and drops whatever came before it, which keeps rows independent.
+__device__ __forceinline__ bool is_reset(double a) { return a == 0.0 && signbit(a); }
+__device__ __forceinline__ double decay_mul(double a, double m) {
+  const double p = a * m;
+  return (p == 0.0 && a != 0.0 && m != 0.0) ? 2.2250738585072014e-308 : p;
+}
+__device__ __forceinline__ double fma_carry(double A, double y, double b) {
+  return is_reset(A) ? b : fma(A, y, b);
+}
+
 template <int R>
 __device__ __forceinline__ void aff_after(const Aff<R>& e, Aff<R>& l) {
   // l := l o e
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    l.b[r] = fma(l.A[r], e.b[r], l.b[r]);
-    l.A[r] = l.A[r] * e.A[r];
+    l.b[r] = fma_carry(l.A[r], e.b[r], l.b[r]);
+    l.A[r] = (is_reset(l.A[r]) || is_reset(e.A[r])) ? -0.0 : decay_mul(l.A[r], e.A[r]);
   }
 }
 
@@ -143,12 +165,24 @@ __device__ __forceinline__ void wave_scan_excl(Aff<R>& v, Aff<R>& tot) {
 
 template <int R>
 __device__ __forceinline__ double aff_apply(const Aff<R>& t, int r, double y) {
-  return fma(t.A[r], y, t.b[r]);
+  return fma_carry(t.A[r], y, t.b[r]);
 }
 
+__device__ __forceinline__ double qnan_d() { return __longlong_as_double(0x7ff8000000000000ll); }
+
 // ----------------------------------------------------------- look-back --
-__device__ __forceinline__ unsigned long long ld_acq(const unsigned long long* p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+// Slots are read and written only with relaxed agent-scope atomics, which
+// are coherent across the XCDs' L2s by themselves.  Ordering payload before
+// flag (and flag before payload on the reader) needs only the wave's own
+// memory counters drained -- no L2 writeback or invalidate, which an
+// agent-scope release/acquire would add to every publish and every poll.
+__device__ __forceinline__ void vmem_order() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  __builtin_amdgcn_s_waitcnt(0);
+}
+__device__ __forceinline__ unsigned long long ld_flag(const double* p) {
+  return __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ double ld_rel_d(const double* p) {
   unsigned long long u = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p),
@@ -171,82 +205,109 @@ struct Seg {
   unsigned epoch;
   Ws ws;
   int totalSegs;
+  bool tree;         // this launch combines segments through the tree
 };
 
+__device__ __forceinline__ int total_groups(int totalSegs) { return totalSegs / kGroup; }
+
+__device__ __forceinline__ double* slot_ptr(const Seg& c, int level, int idx, bool group) {
+  const size_t per = (size_t)c.totalSegs + total_groups(c.totalSegs);
+  return c.ws.area + ((size_t)level * per + (group ? c.totalSegs + idx : idx)) * c.ws.stride;
+}
+
 template <int R>
-__device__ void publish(const Seg& c, int level, unsigned state, const Aff<R>& v) {
-  if (c.nseg <= 1) return;
-  Slot* s = c.ws.slots + (size_t)level * c.totalSegs + c.gseg;
+__device__ __forceinline__ void write_slot(const Seg& c, double* s, const Aff<R>& v) {
   if (lane_id() == 0) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      st_rel_d(&s->v[2 * r], v.A[r]);
-      st_rel_d(&s->v[2 * r + 1], v.b[r]);
+      st_rel_d(s + 1 + 2 * r, v.A[r]);
+      st_rel_d(s + 2 + 2 * r, v.b[r]);
     }
-    __hip_atomic_store(&s->flag, (unsigned long long)c.epoch * 4ull + state, __ATOMIC_RELEASE,
-                       __HIP_MEMORY_SCOPE_AGENT);
+    vmem_order();
+    __hip_atomic_store(reinterpret_cast<unsigned long long*>(s), (unsigned long long)c.epoch,
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
-// Composite of all predecessors of this segment within its row: returns the
-// state at bar s0-1 per recurrence.  Stops at the nearest INCLUSIVE slot.
+// Reduce across the wave, lane 0 earliest in time; result broadcast.
 template <int R>
-__device__ void lookback(const Seg& c, int level, double (&y)[R]) {
+__device__ __forceinline__ void wave_compose(Aff<R>& v) {
   const int lane = lane_id();
-  const int first = c.gseg - c.seg;
-  Aff<R> acc;
-  aff_id(acc);
-  int base = c.gseg - 1;
-  const Slot* slots = c.ws.slots + (size_t)level * c.totalSegs;
-  for (;;) {
-    const int pred = base - lane;
-    const bool valid = pred >= first;
-    unsigned st = kIncl;
-    if (valid) {
-      const unsigned long long want = (unsigned long long)c.epoch;
-      unsigned long long f;
-      do {
-        f = ld_acq(&slots[pred].flag);
-      } while ((f >> 2) != want);
-      st = (unsigned)(f & 3ull);
-    }
-    const unsigned long long inclMask = __ballot(st == kIncl);
-    const int j = inclMask ? (__ffsll((long long)inclMask) - 1) : 64;
-    Aff<R> v;
-    aff_id(v);
-    if (valid && lane <= j) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        v.A[r] = ld_rel_d(&slots[pred].v[2 * r]);
-        v.b[r] = ld_rel_d(&slots[pred].v[2 * r + 1]);
-      }
-    } else if (!valid && lane <= j) {
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        v.A[r] = 0.0;
-        v.b[r] = __longlong_as_double(0x7ff8000000000000ll);
-      }
-    }
-    // reduce: higher lanes are earlier in time
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-      Aff<R> o;
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        o.A[r] = __shfl_down(v.A[r], off);
-        o.b[r] = __shfl_down(v.b[r], off);
-      }
-      if (lane + off < kWave) aff_after(o, v);
-    }
-    Aff<R> chunk;
+  for (int off = 1; off < kWave; off <<= 1) {
+    Aff<R> o;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      chunk.A[r] = bcast_d(v.A[r], 0);
-      chunk.b[r] = bcast_d(v.b[r], 0);
+      o.A[r] = __shfl_down(v.A[r], off);
+      o.b[r] = __shfl_down(v.b[r], off);
     }
-    aff_after(chunk, acc);
-    if (j < 64) break;
-    base -= 64;
+    // o holds later lanes: v := o after v
+    if (lane + off < kWave) {
+      aff_after(v, o);
+      v = o;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    v.A[r] = bcast_d(v.A[r], 0);
+    v.b[r] = bcast_d(v.b[r], 0);
+  }
+}
+
+// Lane i loads slot idx0 + i when i < cnt (identity otherwise), spinning
+// until it carries this launch's epoch, and the wave composes them.
+template <int R>
+__device__ Aff<R> gather(const Seg& c, int level, int idx0, int cnt, bool group) {
+  const int lane = lane_id();
+  Aff<R> v;
+  aff_id(v);
+  const bool mine = lane < cnt;
+  const double* s = mine ? slot_ptr(c, level, idx0 + lane, group) : nullptr;
+  if (mine) {
+    while ((unsigned)ld_flag(s) != c.epoch) __builtin_amdgcn_s_sleep(1);
+  }
+  vmem_order();
+  if (mine) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      v.A[r] = ld_rel_d(s + 1 + 2 * r);
+      v.b[r] = ld_rel_d(s + 2 + 2 * r);
+    }
+  }
+  wave_compose(v);
+  return v;
+}
+
+// Publish this segment's transform; the last wave of a full group then
+// publishes the group's composite.
+template <int R>
+__device__ void publish_tree(const Seg& c, int level, const Aff<R>& t) {
+  write_slot(c, slot_ptr(c, level, c.gseg, false), t);
+  const int g = c.gseg / kGroup;
+  if ((c.gseg % kGroup) == kGroup - 1) {
+    Aff<R> gv = gather<R>(c, level, g * kGroup, kGroup, false);
+    write_slot(c, slot_ptr(c, level, g, true), gv);
+  }
+}
+
+// State at bar s0-1: composite of everything from the group holding the
+// row's first segment (which resets the state) up to this segment.
+template <int R>
+__device__ void lookback(const Seg& c, int level, double (&y)[R]) {
+  const int first = c.gseg - c.seg;
+  const int G = c.gseg / kGroup;
+  Aff<R> acc;
+  aff_id(acc);
+  for (int g0 = first / kGroup; g0 < G; g0 += kWave) {
+    Aff<R> ch = gather<R>(c, level, g0, min(kWave, G - g0), true);
+    aff_after(acc, ch);
+    acc = ch;
+  }
+  const int i0 = max(first, G * kGroup);
+  if (i0 < c.gseg) {
+    Aff<R> ch = gather<R>(c, level, i0, c.gseg - i0, false);
+    aff_after(acc, ch);
+    acc = ch;
   }
 #pragma unroll
   for (int r = 0; r < R; ++r) y[r] = acc.b[r];
@@ -261,34 +322,36 @@ __device__ void level_carry(const Seg& c, int level, int Ls, const double (&yL)[
                             double (&yin)[R]) {
   Aff<R> tot;
   wave_scan_excl(loc, tot);
-  const double qnan = __longlong_as_double(0x7ff8000000000000ll);
+  const double qnan = qnan_d();
   if (Ls >= c.s1) {
-    Aff<R> inc;
+    // nothing valid yet: the row restarts from "undefined"
+    Aff<R> t;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      inc.A[r] = 0.0;
-      inc.b[r] = qnan;
+      t.A[r] = -0.0;
+      t.b[r] = qnan;
       yin[r] = qnan;
     }
-    publish(c, level, kIncl, inc);
+    if (c.tree) publish_tree(c, level, t);
     return;
   }
-  double ys[R];
   if (Ls >= c.s0) {
+    // the seed lies here: this segment resets the state
+    Aff<R> t;
 #pragma unroll
-    for (int r = 0; r < R; ++r) ys[r] = yL[r];
-  } else {
-    publish(c, level, kAgg, tot);
-    lookback<R>(c, level, ys);
+    for (int r = 0; r < R; ++r) {
+      yin[r] = aff_apply(loc, r, yL[r]);
+      t.A[r] = -0.0;
+      t.b[r] = aff_apply(tot, r, yL[r]);
+    }
+    if (c.tree) publish_tree(c, level, t);
+    return;
   }
-  Aff<R> inc;
+  publish_tree(c, level, tot);
+  double ys[R];
+  lookback<R>(c, level, ys);
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
-    yin[r] = aff_apply(loc, r, ys[r]);
-    inc.A[r] = 0.0;
-    inc.b[r] = aff_apply(tot, r, ys[r]);
-  }
-  publish(c, level, kIncl, inc);
+  for (int r = 0; r < R; ++r) yin[r] = aff_apply(loc, r, ys[r]);
 }
 
 // ------------------------------------------------------------ scheduling --
@@ -327,36 +390,36 @@ __device__ __forceinline__ bool resolve(const Geometry& g, int seg_size, int tic
   return true;
 }
 
-// Wave prologue: take a ticket (in launch order, which guarantees every
-// predecessor is resident) and read the launch epoch.
-__device__ __forceinline__ int take_ticket(const Ws& ws, unsigned& epoch) {
-  int t = 0;
+// Launch epoch, read by every wave at its start.  Segments are ordered by
+// workgroup id: each XCD's dispatcher issues its workgroups in id order, so
+// every predecessor a wave waits for is resident or already finished.
+__device__ __forceinline__ unsigned read_epoch(const Ws& ws) {
   unsigned e = 0;
-  if (lane_id() == 0) {
-    t = (int)__hip_atomic_fetch_add(&ws.hdr->ticket, 1u, __ATOMIC_RELAXED,
-                                    __HIP_MEMORY_SCOPE_AGENT);
+  if (lane_id() == 0)
     e = __hip_atomic_load(&ws.hdr->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  t = __builtin_amdgcn_readfirstlane(t);
-  epoch = (unsigned)__builtin_amdgcn_readfirstlane((int)e);
-  return t;
+  return (unsigned)__builtin_amdgcn_readfirstlane((int)e);
 }
 
-// Wave epilogue: the last wave of the launch resets the counters and bumps
-// the epoch so the next launch on this workspace starts clean.
-__device__ __forceinline__ void finish_wave(const Ws& ws, unsigned totalWaves) {
-  if (lane_id() == 0) {
-    const unsigned d = __hip_atomic_fetch_add(&ws.hdr->done, 1u, __ATOMIC_ACQ_REL,
-                                              __HIP_MEMORY_SCOPE_AGENT);
-    if (d + 1 == totalWaves) {
-      __hip_atomic_store(&ws.hdr->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&ws.hdr->done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const unsigned e = __hip_atomic_load(&ws.hdr->epoch, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&ws.hdr->epoch, (e + 1u) & 0x3fffffffu ? (e + 1u) & 0x3fffffffu : 1u,
-                         __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    }
-  }
+// Block epilogue (thread 0, after the block's waves are done): the block
+// completing the last stripe resets the counters and bumps the epoch, so the
+// next launch on this workspace (ordered after this one by the stream)
+// starts clean.  Relaxed atomics suffice: kernel end publishes every write.
+__device__ __forceinline__ void finish_block(const Ws& ws, unsigned nblocks) {
+  const unsigned b = blockIdx.x;
+  const unsigned st = b % kStripes;
+  const unsigned inStripe = (nblocks - st + kStripes - 1) / kStripes;
+  const unsigned d = __hip_atomic_fetch_add(&ws.hdr->stripe[st][0], 1u, __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT);
+  if (d + 1 != inStripe) return;
+  __hip_atomic_store(&ws.hdr->stripe[st][0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const unsigned nst = nblocks < (unsigned)kStripes ? nblocks : (unsigned)kStripes;
+  const unsigned s2 = __hip_atomic_fetch_add(&ws.hdr->stripesDone, 1u, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+  if (s2 + 1 != nst) return;
+  __hip_atomic_store(&ws.hdr->stripesDone, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const unsigned e = __hip_atomic_load(&ws.hdr->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const unsigned ne = (e + 1u) & 0x3fffffffu;
+  __hip_atomic_store(&ws.hdr->epoch, ne ? ne : 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ float qnanf_() { return __int_as_float(0x7fc00000); }

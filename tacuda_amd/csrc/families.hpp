@@ -29,6 +29,8 @@ struct Tile {
   static_assert(K % 4 == 0 && H % K == 0, "tile shape");
 };
 
+typedef float v4f __attribute__((ext_vector_type(4)));
+
 struct IO {
   const float* in[4];
   float* out[4];
@@ -39,15 +41,47 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
+// Stage bars [s0-H, s0+SEG) of one input into LDS.  Every load of the wave
+// is issued before the first LDS store; rows whose range is 16-byte aligned
+// and fully inside the row move as float4.
 template <class T>
 __device__ __forceinline__ void stage(float* lds, const float* src, const Seg& c) {
   const int lane = lane_id();
   const float* row = src + c.rowOff;
-  for (int e = lane; e < T::LEN; e += kWave) {
+  const long long g0 = c.rowOff + c.s0 - T::H;
+  const bool vec = ((g0 & 3) == 0) && (c.s0 - T::H >= 0) && (c.s0 + T::SEG <= c.n);
+  if (vec) {
+    constexpr int NV = (T::LEN / 4 + kWave - 1) / kWave;
+    v4f v[NV];
+    const v4f* p = reinterpret_cast<const v4f*>(src + g0);
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      const int q = lane + i * kWave;
+      if (q < T::LEN / 4) v[i] = p[q];
+    }
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      const int q = lane + i * kWave;
+      if (q < T::LEN / 4) {
+        const int e = 4 * q;  // 4 | K: the quad stays inside one K-block
+        *reinterpret_cast<v4f*>(lds + T::pos(e)) = v[i];
+      }
+    }
+    return;
+  }
+  constexpr int NS = (T::LEN + kWave - 1) / kWave;
+  float v[NS];
+#pragma unroll
+  for (int i = 0; i < NS; ++i) {
+    const int e = lane + i * kWave;
     const int t = c.s0 - T::H + e;
-    float v = qnanf_();
-    if (t >= 0 && t < c.n) v = row[t];
-    lds[T::pos(e)] = v;
+    v[i] = qnanf_();
+    if (e < T::LEN && t >= 0 && t < c.n) v[i] = row[t];
+  }
+#pragma unroll
+  for (int i = 0; i < NS; ++i) {
+    const int e = lane + i * kWave;
+    if (e < T::LEN) lds[T::pos(e)] = v[i];
   }
 }
 
@@ -72,10 +106,22 @@ __device__ __forceinline__ void lane_vals(const float* lds, float (&v)[T::K]) {
   }
 }
 
-// Writes the lane's K outputs through LDS so the global stores coalesce.
+// Writes the lane's K outputs.  Full, 16-byte aligned segments store
+// straight from registers as float4 (each lane owns K contiguous bars);
+// other segments go through LDS so the global stores coalesce.
 template <class T>
 __device__ __forceinline__ void store_lane(float* ldo, const float (&v)[T::K], float* dst,
                                            const Seg& c) {
+  const long long g0 = c.rowOff + c.s0;
+  if (((g0 & 3) == 0) && (c.s0 + T::SEG <= c.n)) {
+    v4f* p = reinterpret_cast<v4f*>(dst + g0 + lane_id() * T::K);
+#pragma unroll
+    for (int q = 0; q < T::K / 4; ++q) {
+      v4f w = {v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]};
+      __builtin_nontemporal_store(w, p + q);
+    }
+    return;
+  }
   wave_sync();
   float* p = ldo + T::pos(lane_id() * T::K);
 #pragma unroll
@@ -187,7 +233,7 @@ __device__ void ema_cascade(const Seg& c, const float* li, const float* x, const
       if (t > L[lv] && t < c.n) {
         const double u = lv == 0 ? (double)xv[i] : e[lv > 0 ? lv - 1 : 0][i];
         loc.b[0] = fma(beta, loc.b[0], k * u);
-        loc.A[0] *= beta;
+        loc.A[0] = decay_mul(loc.A[0], beta);
       }
     }
     double yl[1] = {seed[lv]}, yin[1];
@@ -210,6 +256,7 @@ __device__ void ema_cascade(const Seg& c, const float* li, const float* x, const
 struct FEma {
   static constexpr bool kScan = true;
   static constexpr int kLevels = 1;
+  static constexpr int kMaxR = 1;
   struct P {
     int p;
   };
@@ -239,6 +286,7 @@ struct FEma {
 struct FDema {
   static constexpr bool kScan = true;
   static constexpr int kLevels = 2;
+  static constexpr int kMaxR = 1;
   struct P {
     int p;
   };
@@ -270,6 +318,7 @@ struct FDema {
 struct FTema {
   static constexpr bool kScan = true;
   static constexpr int kLevels = 3;
+  static constexpr int kMaxR = 1;
   struct P {
     int p;
   };
@@ -301,6 +350,7 @@ struct FTema {
 struct FT3 {
   static constexpr bool kScan = true;
   static constexpr int kLevels = 6;
+  static constexpr int kMaxR = 1;
   struct P {
     int p;
     double v;
@@ -340,6 +390,7 @@ struct FT3 {
 struct FMacd {
   static constexpr bool kScan = true;
   static constexpr int kLevels = 2;
+  static constexpr int kMaxR = 2;
   struct P {
     int fast, slow, sig;
   };
@@ -396,9 +447,9 @@ struct FMacd {
       if (t > L1 && t < c.n) {
         const double v = (double)xv[i];
         loc.b[0] = fma(fb, loc.b[0], fk * v);
-        loc.A[0] *= fb;
+        loc.A[0] = decay_mul(loc.A[0], fb);
         loc.b[1] = fma(sb, loc.b[1], sk * v);
-        loc.A[1] *= sb;
+        loc.A[1] = decay_mul(loc.A[1], sb);
       }
     }
     double yin[2];
@@ -430,7 +481,7 @@ struct FMacd {
       const int t = a + i;
       if (t > L2 && t < c.n) {
         l2.b[0] = fma(gb, l2.b[0], gk * mv[i]);
-        l2.A[0] *= gb;
+        l2.A[0] = decay_mul(l2.A[0], gb);
       }
     }
     double yin2[1];
@@ -463,6 +514,7 @@ struct FMacd {
 struct FRsi {
   static constexpr bool kScan = true;
   static constexpr int kLevels = 1;
+  static constexpr int kMaxR = 2;
   struct P {
     int p;
   };
@@ -518,8 +570,8 @@ struct FRsi {
         const double gd = d > 0.0 ? d : 0.0;
         loc.b[0] = (loc.b[0] * pm1 + gd) * inv;
         loc.b[1] = (loc.b[1] * pm1 + (gd - d)) * inv;
-        loc.A[0] *= am;
-        loc.A[1] *= am;
+        loc.A[0] = decay_mul(loc.A[0], am);
+        loc.A[1] = decay_mul(loc.A[1], am);
       }
     }
     double yin[2];
@@ -558,6 +610,7 @@ template <bool NORM>
 struct FAtrT {
   static constexpr bool kScan = true;
   static constexpr int kLevels = 1;
+  static constexpr int kMaxR = 1;
   struct P {
     int p;
   };
@@ -605,7 +658,7 @@ struct FAtrT {
       const int t = a + i;
       if (t > L && t < c.n) {
         loc.b[0] = fma(wb, loc.b[0], wa * tr[i]);
-        loc.A[0] *= wb;
+        loc.A[0] = decay_mul(loc.A[0], wb);
       }
     }
     double yin[1];
@@ -658,6 +711,7 @@ template <int KIND>
 struct FDir {
   static constexpr bool kScan = true;
   static constexpr int kLevels = (KIND == kADX) ? 2 : 1;
+  static constexpr int kMaxR = ((KIND == kPDM || KIND == kMDM) ? 1 : 3);
   struct P {
     int p;
   };
@@ -766,13 +820,13 @@ struct FDir {
         if (R == 1) {
           const double u = KIND == kMDM ? mdm[i] : pdm[i];
           loc.b[0] = loc.b[0] - loc.b[0] / pp + u;
-          loc.A[0] *= am;
+          loc.A[0] = decay_mul(loc.A[0], am);
         } else {
           const double u[3] = {pdm[i], mdm[i], tr[i]};
 #pragma unroll
           for (int r = 0; r < R; ++r) {
             loc.b[r] = loc.b[r] - loc.b[r] / pp + u[r];
-            loc.A[r] *= am;
+            loc.A[r] = decay_mul(loc.A[r], am);
           }
         }
       }
@@ -827,7 +881,7 @@ struct FDir {
       const int t = a + i;
       if (t > L2 && t < c.n) {
         l2.b[0] = ((l2.b[0] * (pp - 1.0)) + dxv[i]) / pp;
-        l2.A[0] *= (pp - 1.0) / pp;
+        l2.A[0] = decay_mul(l2.A[0], (pp - 1.0) / pp);
       }
     }
     double yin2[1];
@@ -853,6 +907,7 @@ struct FDir {
 struct FKama {
   static constexpr bool kScan = true;
   static constexpr int kLevels = 1;
+  static constexpr int kMaxR = 1;
   struct P {
     int p, fast, slow;
   };
@@ -934,7 +989,7 @@ struct FKama {
       const int t = a + i;
       if (t > L && t < c.n) {
         loc.b[0] = fma((double)xv[i] - loc.b[0], sc[i], loc.b[0]);
-        loc.A[0] *= (1.0 - sc[i]);
+        loc.A[0] = decay_mul(loc.A[0], (1.0 - sc[i]));
       }
     }
     double yin[1];
@@ -963,6 +1018,7 @@ template <bool MEAN>
 struct FSumT {
   static constexpr bool kScan = false;
   static constexpr int kLevels = 0;
+  static constexpr int kMaxR = 0;
   struct P {
     int p;
   };
@@ -1007,6 +1063,7 @@ using FSum = FSumT<false>;
 struct FWma {
   static constexpr bool kScan = false;
   static constexpr int kLevels = 0;
+  static constexpr int kMaxR = 0;
   struct P {
     int p;
   };
@@ -1087,6 +1144,7 @@ template <int MODE>  // 0 = VAR, 1 = STDDEV
 struct FVarT {
   static constexpr bool kScan = false;
   static constexpr int kLevels = 0;
+  static constexpr int kMaxR = 0;
   struct P {
     int p;
   };
@@ -1117,6 +1175,7 @@ using FStddev = FVarT<1>;
 struct FBbands {
   static constexpr bool kScan = false;
   static constexpr int kLevels = 0;
+  static constexpr int kMaxR = 0;
   struct P {
     int p;
     double up, dn;
@@ -1160,6 +1219,7 @@ template <bool ISMAX>
 struct FExtT {
   static constexpr bool kScan = false;
   static constexpr int kLevels = 0;
+  static constexpr int kMaxR = 0;
   struct P {
     int p;
   };
@@ -1212,6 +1272,7 @@ __device__ __forceinline__ void window_hl(const float* lh, const float* ll, cons
 struct FWillr {
   static constexpr bool kScan = false;
   static constexpr int kLevels = 0;
+  static constexpr int kMaxR = 0;
   struct P {
     int p;
   };
@@ -1246,6 +1307,7 @@ struct FWillr {
 struct FCci {
   static constexpr bool kScan = false;
   static constexpr int kLevels = 0;
+  static constexpr int kMaxR = 0;
   struct P {
     int p;
   };
@@ -1294,6 +1356,7 @@ template <int MODE>
 struct FAroonT {
   static constexpr bool kScan = false;
   static constexpr int kLevels = 0;
+  static constexpr int kMaxR = 0;
   struct P {
     int p;
   };
@@ -1346,6 +1409,7 @@ using FAroonOsc = FAroonT<1>;
 struct FTrange {
   static constexpr bool kScan = false;
   static constexpr int kLevels = 0;
+  static constexpr int kMaxR = 0;
   struct P {
     int unused;
   };
@@ -1380,6 +1444,7 @@ constexpr int kStochMaxD = 9;
 struct FStoch {
   static constexpr bool kScan = false;
   static constexpr int kLevels = 0;
+  static constexpr int kMaxR = 0;
   struct P {
     int kp, dp;
   };

@@ -44,10 +44,10 @@ ctStatus_t guarded(F&& body) {
 }
 
 // ---------------------------------------------------------- workspace --
-// Look-back slots plus the ticket/epoch header, one per HIP stream so that
+// Look-back tree area plus the ticket/epoch header, one per HIP stream so that
 // launches ordered on one stream share it safely.  Growing it is refused
 // while the stream is being captured (the graph would keep a stale pointer).
-ctStatus_t workspace_for(hipStream_t stream, size_t slots, Ws* out);
+ctStatus_t workspace_for(hipStream_t stream, size_t treeBytes, Ws* out);
 void workspace_release_all();
 size_t workspace_allocations();
 

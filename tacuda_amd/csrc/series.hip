@@ -22,17 +22,23 @@ __global__ __launch_bounds__(W* kWave) void k_seg(Geometry g, Ws ws, int useLB,
   __shared__ __align__(16) float s_out[W][T::POUT];
   const int w = threadIdx.x / kWave;
   Seg c;
-  c.epoch = 0;
-  int ticket;
-  if (useLB)
-    ticket = take_ticket(ws, c.epoch);
-  else
-    ticket = blockIdx.x * W + w;
-  if (resolve(g, T::SEG, ticket, c)) {
+  c.tree = useLB != 0;
+  c.epoch = useLB ? read_epoch(ws) : 0u;
+  const int segIdx = blockIdx.x * W + w;
+  if (resolve(g, T::SEG, segIdx, c)) {
     c.ws = ws;
     F::template run<T>(c, s_in[w], s_out[w], prm, io);
   }
-  if (useLB) finish_wave(ws, gridDim.x * W);
+  if (useLB) {
+    __syncthreads();
+    if (threadIdx.x == 0) finish_block(ws, gridDim.x);
+  }
+}
+
+// Look-back area of a launch: levels x (segments + groups) slots of
+// 2R+1 doubles.
+inline size_t tree_bytes(int levels, int maxR, int totalSegs) {
+  return (size_t)levels * ((size_t)totalSegs + totalSegs / kGroup) * (2 * maxR + 1) * 8;
 }
 
 template <class F, int K, int H, int NIN, int W>
@@ -40,9 +46,9 @@ static ctStatus_t launch_geom(Geometry g, int maxSegsPerRow, hipStream_t s, type
                               IO io) {
   if (g.totalSegs <= 0) return CT_STATUS_SUCCESS;
   const bool useLB = F::kScan && maxSegsPerRow > 1;
-  Ws ws{nullptr, nullptr};
+  Ws ws{nullptr, nullptr, 2 * (F::kMaxR > 0 ? F::kMaxR : 1) + 1};
   if (useLB) {
-    ctStatus_t st = workspace_for(s, (size_t)(F::kLevels > 0 ? F::kLevels : 1) * g.totalSegs, &ws);
+    ctStatus_t st = workspace_for(s, tree_bytes(F::kLevels, F::kMaxR, g.totalSegs), &ws);
     if (st != CT_STATUS_SUCCESS) return st;
   }
   const int blocks = div_up(g.totalSegs, W);
@@ -65,40 +71,43 @@ static ctStatus_t launch_dense(int len, int rows, hipStream_t s, typename F::P p
 }
 
 // ----------------------------------------------------- per-family policy --
+#ifndef TCA_KL
+#define TCA_KL 8
+#endif
 #define TCA_POLICY(FAM, KL, HL, KS, HS, NIN, W)                                            \
   ctStatus_t run_##FAM(int len, int rows, hipStream_t s, const F##FAM::P& p, IO io) {     \
     return launch_dense<F##FAM, KL, HL, KS, HS, NIN, W>(len, rows, s, p, io);              \
   }
 
-TCA_POLICY(Ema, 16, 16, 4, 4, 1, 4)
-TCA_POLICY(Dema, 16, 16, 4, 4, 1, 4)
-TCA_POLICY(Tema, 16, 16, 4, 4, 1, 4)
+TCA_POLICY(Ema, TCA_KL, 16, 4, 4, 1, 4)
+TCA_POLICY(Dema, TCA_KL, 16, 4, 4, 1, 4)
+TCA_POLICY(Tema, TCA_KL, 16, 4, 4, 1, 4)
 TCA_POLICY(T3, 8, 8, 4, 4, 1, 4)
-TCA_POLICY(Macd, 16, 16, 4, 4, 1, 4)
-TCA_POLICY(Rsi, 16, 16, 4, 4, 1, 4)
-TCA_POLICY(Kama, 16, 64, 4, 64, 1, 4)
-TCA_POLICY(Atr, 16, 16, 4, 4, 3, 2)
-TCA_POLICY(Natr, 16, 16, 4, 4, 3, 2)
-TCA_POLICY(Sma, 16, 64, 4, 64, 1, 4)
-TCA_POLICY(Sum, 16, 64, 4, 64, 1, 4)
-TCA_POLICY(Wma, 16, 64, 4, 64, 1, 4)
-TCA_POLICY(Var, 16, 64, 4, 64, 1, 4)
-TCA_POLICY(Stddev, 16, 64, 4, 64, 1, 4)
-TCA_POLICY(Bbands, 16, 64, 4, 64, 1, 4)
-TCA_POLICY(Max, 16, 64, 4, 64, 1, 4)
-TCA_POLICY(Min, 16, 64, 4, 64, 1, 4)
-TCA_POLICY(Willr, 16, 64, 4, 64, 2, 2)
+TCA_POLICY(Macd, TCA_KL, 16, 4, 4, 1, 4)
+TCA_POLICY(Rsi, TCA_KL, 16, 4, 4, 1, 4)
+TCA_POLICY(Kama, TCA_KL, 64, 4, 64, 1, 4)
+TCA_POLICY(Atr, TCA_KL, 16, 4, 4, 3, 2)
+TCA_POLICY(Natr, TCA_KL, 16, 4, 4, 3, 2)
+TCA_POLICY(Sma, TCA_KL, 64, 4, 64, 1, 4)
+TCA_POLICY(Sum, TCA_KL, 64, 4, 64, 1, 4)
+TCA_POLICY(Wma, TCA_KL, 64, 4, 64, 1, 4)
+TCA_POLICY(Var, TCA_KL, 64, 4, 64, 1, 4)
+TCA_POLICY(Stddev, TCA_KL, 64, 4, 64, 1, 4)
+TCA_POLICY(Bbands, TCA_KL, 64, 4, 64, 1, 4)
+TCA_POLICY(Max, TCA_KL, 64, 4, 64, 1, 4)
+TCA_POLICY(Min, TCA_KL, 64, 4, 64, 1, 4)
+TCA_POLICY(Willr, TCA_KL, 64, 4, 64, 2, 2)
 TCA_POLICY(Cci, 8, 64, 4, 64, 3, 2)
-TCA_POLICY(Aroon, 16, 64, 4, 64, 2, 2)
-TCA_POLICY(AroonOsc, 16, 64, 4, 64, 2, 2)
-TCA_POLICY(Trange, 16, 16, 4, 4, 3, 2)
+TCA_POLICY(Aroon, TCA_KL, 64, 4, 64, 2, 2)
+TCA_POLICY(AroonOsc, TCA_KL, 64, 4, 64, 2, 2)
+TCA_POLICY(Trange, TCA_KL, 16, 4, 4, 3, 2)
 TCA_POLICY(Stoch, 8, 64, 4, 64, 3, 2)
 #undef TCA_POLICY
 
 #define TCA_DIR_POLICY(NAME, KIND)                                                          \
   ctStatus_t run_##NAME(int len, int rows, hipStream_t s, int p, IO io) {                   \
     typename FDir<KIND>::P pr{p};                                                           \
-    return launch_dense<FDir<KIND>, 16, 16, 4, 4, 3, 2>(len, rows, s, pr, io);              \
+    return launch_dense<FDir<KIND>, TCA_KL, TCA_KL, 4, 4, 3, 2>(len, rows, s, pr, io);              \
   }
 TCA_DIR_POLICY(PlusDm, kPDM)
 TCA_DIR_POLICY(MinusDm, kMDM)

@@ -91,9 +91,9 @@ extern "C" CTAPI_EXPORT ctStatus_t ctx_synth_ohlcv(unsigned long long seed, int 
   });
 }
 
-extern "C" CTAPI_EXPORT ctStatus_t ctx_workspace_reserve(hipStream_t stream, size_t slots) {
+extern "C" CTAPI_EXPORT ctStatus_t ctx_workspace_reserve(hipStream_t stream, size_t bytes) {
   return tca::guarded([&]() -> ctStatus_t {
     tca::Ws ws;
-    return tca::workspace_for(stream, slots, &ws);
+    return tca::workspace_for(stream, bytes, &ws);
   });
 }

@@ -36,6 +36,8 @@ def _data(n, rows=1, seed=0):
     return ta.synth_ohlcv(n, rows=rows, seed=seed)
 
 
+V07 = float(np.float32(0.7))
+
 # name -> (inputs, gpu fn, oracle fn, exact?)
 SPECS = {
     "sma20": ("c", lambda o, c: o.sma(c, 20), lambda R, c: R.sma(c, 20), True),
@@ -53,7 +55,8 @@ SPECS = {
     "ema2": ("c", lambda o, c: o.ema(c, 2), lambda R, c: R.ema(c, 2), False),
     "dema20": ("c", lambda o, c: o.dema(c, 20), lambda R, c: R.dema(c, 20), False),
     "tema20": ("c", lambda o, c: o.tema(c, 20), lambda R, c: R.tema(c, 20), False),
-    "t3_5": ("c", lambda o, c: o.t3(c, 5, 0.7), lambda R, c: R.t3(c, 5, 0.7), False),
+    # the C ABI takes vFactor as float: the oracle gets the same float32 value
+    "t3_5": ("c", lambda o, c: o.t3(c, 5, 0.7), lambda R, c: R.t3(c, 5, V07), False),
     "kama30": ("c", lambda o, c: o.kama(c, 30), lambda R, c: R.kama(c, 30), False),
     "kama10_3_20": ("c", lambda o, c: o.kama(c, 10, 3, 20), lambda R, c: R.kama(c, 10, 3, 20), False),
     "rsi14": ("c", lambda o, c: o.rsi(c, 14), lambda R, c: R.rsi(c, 14), False),
