@@ -1,17 +1,25 @@
 """Benchmark of the indicator core on one MI355X.
 
-Default workload: the device-API 7-indicator pipeline on one GPU-resident
-float32 series of 1,000,000 bars (seeded OHLCV, seed 0): SMA(20), EMA(20),
-RSI(14), MACD(12,26,9), BBANDS(20,2,2), ATR(14), STOCH(14,3,3).  A step is
-one pass of the seven ct_*_d calls, replayed from a hipGraph.  Inputs and
-outputs stay in HBM; five buffer sets (300 MB) rotate between steps so the
-256 MB Infinity Cache cannot hold the working set.
+Headline workload: the 7-indicator pipeline on one GPU-resident float32
+series of 1,000,000 bars (seeded OHLCV, seed 0): SMA(20), EMA(20), RSI(14),
+MACD(12,26,9), BBANDS(20,2,2), ATR(14), STOCH(14,3,3).  A step is one pass
+over the series computing all twelve outputs with the fused launch
+(ctx_pack_batch_d), replayed from a hipGraph; the same pass through the
+seven drop-in ct_*_d calls is timed beside it.  Inputs and outputs stay in
+HBM; five buffer sets rotate between steps so the 256 MB Infinity Cache
+cannot hold the working set.
 
-Prints one JSON line: bars/s for the whole pass, the roofline of the
-dominant kernel (algorithmic bytes / its own event-timed duration against
-MI355X's 8 TB/s), and the reference's TA-Lib 0.8.3 C timed on the host.
+Also measured in the same run (reported under "workloads"):
+  batch13  4000 series x 234 bars, the 13 indicators SMA, EMA, WMA, DEMA,
+           TEMA, RSI, MACD, BBANDS, ATR, ADX, STOCH, WILLR, CCI (18 outputs)
+           in one fused call, hipGraph replay, five rotated buffer sets;
+  scan256  256 series x 2^20 bars, EMA(200), KAMA(30), T3(5, 0.7), SMA(200)
+           through the dense batch calls (look-back scan across segments).
 
-Other workloads (--workload): batch13 (4000 x 234 13-call fused batch).
+Prints one JSON line: bars/s of the headline pass, the roofline of its
+dominant kernel (algorithmic bytes per launch / event-timed launch duration
+against MI355X's 8 TB/s), and the reference's TA-Lib 0.8.3 C timed on the
+host.
 """
 from __future__ import annotations
 
@@ -27,37 +35,51 @@ sys.path.insert(0, HERE)
 PEAK_HBM_GBS = 8000.0
 METRIC = "bars/sec and %HBM peak, fused batch + 1M-bar scans; fwd+bwd/fwd time ratio"
 
+PIPE7 = dict(sma_period=20, ema_period=20, rsi_period=14, macd_fast=12, macd_slow=26,
+             macd_signal=9, bb_period=20, bb_up=2.0, bb_dn=2.0, atr_period=14, stoch_k=14,
+             stoch_d=3)
+PIPE7_OUT = ("sma", "ema", "rsi", "macd", "macd_signal", "macd_hist", "bb_upper", "bb_middle",
+             "bb_lower", "atr", "stoch_k", "stoch_d")
+# the fused launch runs as up to three kernels; each is timed alone through
+# the output subset that launches only it: (inputs read, outputs)
+PACK_KERNELS = {
+    "k_pack<windows>": (3, ("sma", "bb_upper", "bb_middle", "bb_lower", "stoch_k", "stoch_d")),
+    "k_pack<close recurrences>": (1, ("ema", "rsi", "macd", "macd_signal", "macd_hist")),
+    "k_pack<hlc recurrences>": (3, ("atr",)),
+}
+
 
 def _args():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="pipeline7", choices=["pipeline7"])
     ap.add_argument("--bars", type=int, default=1_000_000)
     ap.add_argument("--sets", type=int, default=5, help="rotated buffer sets")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="time budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", default="", help="comma list of workloads: pipeline7,batch13,scan256")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="also run a plain device copy of this many MB (counter calibration)")
     return ap.parse_args()
 
 
-# (name, inputs, outputs, call)
-def _pipeline(ops):
+def _drop_in(ops):
     return [
-        ("sma20", ("c",), 1, lambda h, l, c: ops.sma(c, 20)),
-        ("ema20", ("c",), 1, lambda h, l, c: ops.ema(c, 20)),
-        ("rsi14", ("c",), 1, lambda h, l, c: ops.rsi(c, 14)),
-        ("macd12_26_9", ("c",), 3, lambda h, l, c: ops.macd(c, 12, 26, 9)),
-        ("bbands20", ("c",), 3, lambda h, l, c: ops.bbands(c, 20, 2.0, 2.0)),
-        ("atr14", ("h", "l", "c"), 1, lambda h, l, c: ops.atr(h, l, c, 14)),
-        ("stoch14_3_3", ("h", "l", "c"), 2, lambda h, l, c: ops.stochastic(h, l, c, 14, 3)),
+        ("sma20", 1, 1, lambda h, l, c: ops.sma(c, 20)),
+        ("ema20", 1, 1, lambda h, l, c: ops.ema(c, 20)),
+        ("rsi14", 1, 1, lambda h, l, c: ops.rsi(c, 14)),
+        ("macd12_26_9", 1, 3, lambda h, l, c: ops.macd(c, 12, 26, 9)),
+        ("bbands20", 1, 3, lambda h, l, c: ops.bbands(c, 20, 2.0, 2.0)),
+        ("atr14", 3, 1, lambda h, l, c: ops.atr(h, l, c, 14)),
+        ("stoch14_3_3", 3, 2, lambda h, l, c: ops.stochastic(h, l, c, 14, 3)),
     ]
 
 
 def _cpu_baseline(d, budget_s):
     """TA-Lib 0.8.3 C (the reference's own host build) on one core."""
-    from oracle import REF, ORC, ref
+    from oracle import ORC, REF, ref
 
     fam = REF if ref() is not None else ORC
     kind = "reference" if ref() is not None else "port"
@@ -76,8 +98,129 @@ def _cpu_baseline(d, budget_s):
             break
     bars = reps * len(c)
     return {"value": bars / el, "unit": "bars/s", "cores": 1, "kind": kind,
-            "sample": f"{reps} passes of the 7 indicators over the same {len(c)}-bar series, "
-                      f"single thread, {el:.1f} s"}
+            "sample": f"{reps} passes of the 7 indicators over the same {len(c)}-bar series "
+                      f"(seed 0), single thread, {el:.1f} s"}
+
+
+class Timer:
+    """hipGraph capture and event timing on one dedicated stream."""
+
+    def __init__(self, torch):
+        self.torch = torch
+        self.stream = torch.cuda.Stream()
+        self.keep = []
+
+    def graphs(self, sets, fn):
+        """Eager call per set (reserves workspace), then one graph per set."""
+        torch = self.torch
+        with torch.cuda.stream(self.stream):
+            for s in sets:
+                fn(s)
+        self.stream.synchronize()
+        out = []
+        for s in sets:
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g, stream=self.stream):
+                self.keep.append(fn(s))
+            out.append(g)
+        torch.cuda.synchronize()
+        return out
+
+    def time(self, glist, steps, warm):
+        """ms per replay, events recorded on the stream the kernels run on."""
+        torch = self.torch
+        with torch.cuda.stream(self.stream):
+            for i in range(warm):
+                glist[i % len(glist)].replay()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            e0.record(self.stream)
+            for i in range(steps):
+                glist[i % len(glist)].replay()
+            e1.record(self.stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+
+def _gbs(nbytes, ms):
+    return nbytes / (ms * 1e-3) / 1e9
+
+
+def _upload(torch, d, dev, nsets):
+    return [{k: torch.from_numpy(d[v]).to(dev) for k, v in
+             (("h", "high"), ("l", "low"), ("c", "close"))} for _ in range(nsets)]
+
+
+def _pipeline7(a, torch, ops, T, dev):
+    import tacuda_amd as ta
+
+    n = a.bars
+    d = ta.synth_ohlcv(n, seed=0)
+    sets = _upload(torch, d, dev, a.sets)
+    fused = T.graphs(sets, lambda s: ops.pack(s["h"], s["l"], s["c"], outputs=PIPE7_OUT, **PIPE7))
+    ms = T.time(fused, a.steps, a.warmup)
+    print(f"pipeline7 fused: {ms * 1e3:.1f} us/pass", file=sys.stderr, flush=True)
+    kernels = {}
+    for name, (nin, outs) in PACK_KERNELS.items():
+        g = T.graphs(sets, lambda s, outs=outs: ops.pack(s["h"], s["l"], s["c"], outputs=outs,
+                                                        **PIPE7))
+        kms = T.time(g, max(a.steps, 20), a.warmup)
+        alg = (nin + len(outs)) * 4 * n
+        kernels[name] = {"us": kms * 1e3, "alg_bytes": alg, "GBs": _gbs(alg, kms)}
+        print(f"  {name}: {kms * 1e3:.1f} us", file=sys.stderr, flush=True)
+    drop = {}
+    pipe = _drop_in(ops)
+    g = T.graphs(sets, lambda s: [f(s["h"], s["l"], s["c"]) for *_, f in pipe])
+    drop_ms = T.time(g, a.steps, a.warmup)
+    for name, nin, nout, f in pipe:
+        gk = T.graphs(sets, lambda s, f=f: f(s["h"], s["l"], s["c"]))
+        kms = T.time(gk, max(a.steps, 20), a.warmup)
+        drop[name] = round(kms * 1e3, 2)
+    print(f"pipeline7 drop-in: {drop_ms * 1e3:.1f} us/pass {drop}", file=sys.stderr, flush=True)
+    alg_total = (3 + 12) * 4 * n
+    return d, {"ms": ms, "bars": n, "alg_bytes": alg_total, "kernels": kernels,
+               "drop_in_ms": drop_ms, "drop_in_kernels_us": drop}
+
+
+def _batch13(a, torch, ops, T, dev):
+    import tacuda_amd as ta
+
+    rows, n = 4000, 234
+    d = ta.synth_ohlcv(n, rows=rows, seed=0)
+    sets = _upload(torch, d, dev, a.sets)
+    g = T.graphs(sets, lambda s: ops.pack(s["h"], s["l"], s["c"]))
+    ms = T.time(g, a.steps, a.warmup)
+    bars = rows * n
+    alg = (3 + 18) * 4 * bars
+    print(f"batch13 4000x234: {ms * 1e3:.1f} us/pass", file=sys.stderr, flush=True)
+    return {"ms": ms, "bars_per_s": bars / (ms * 1e-3), "alg_bytes": alg,
+            "GBs": _gbs(alg, ms), "frac_hbm": _gbs(alg, ms) / PEAK_HBM_GBS,
+            "shape": [rows, n], "graph": True, "rotated_sets": a.sets}
+
+
+def _scan256(a, torch, ops, T, dev):
+    import tacuda_amd as ta
+
+    rows, n = 256, 1 << 20
+    c = torch.from_numpy(ta.synth_ohlcv(n, rows=rows, seed=0)["close"]).to(dev)
+    calls = {"ema200": lambda x: ops.ema(x, 200), "kama30": lambda x: ops.kama(x, 30),
+             "t3_5": lambda x: ops.t3(x, 5, 0.7), "sma200": lambda x: ops.sma(x, 200)}
+    per = {}
+    total = 0.0
+    for name, f in calls.items():
+        g = T.graphs([c], f)
+        kms = T.time(g, max(5, a.steps // 10), 2)
+        per[name] = {"ms": kms, "GBs": _gbs(2 * 4 * rows * n, kms)}
+        total += kms
+        T.keep.clear()
+        torch.cuda.empty_cache()
+    bars = rows * n
+    alg = (1 + 4) * 4 * bars
+    print(f"scan256 256x2^20: {total:.3f} ms/pass", file=sys.stderr, flush=True)
+    return {"ms": total, "bars_per_s": bars / (total * 1e-3), "alg_bytes": alg,
+            "GBs": _gbs(alg, total), "frac_hbm": _gbs(alg, total) / PEAK_HBM_GBS,
+            "shape": [rows, n], "kernels": per}
 
 
 def main():
@@ -86,73 +229,39 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not "
                                    "supported (series are independent: shard rows by GPU)"}))
         sys.exit(2)
-    import numpy as np
     import torch
 
-    import tacuda_amd as ta
     from tacuda_amd import ops
 
     dev = torch.device("cuda:0")
-    n = a.bars
-    d = ta.synth_ohlcv(n, seed=0)
-    stream = torch.cuda.Stream()
-    pipe = _pipeline(ops)
-    sets = []
-    with torch.cuda.stream(stream):
-        for _ in range(a.sets):
-            sets.append({k: torch.from_numpy(d[v]).to(dev) for k, v in
-                         (("h", "high"), ("l", "low"), ("c", "close"))})
-        # eager warm-up on the capture stream reserves its look-back workspace
-        for s in sets:
-            for _, _, _, f in pipe:
-                f(s["h"], s["l"], s["c"])
-    stream.synchronize()
-
-    # one graph per buffer set for the whole pipeline, and one per kernel
-    graphs, kgraphs, keep = [], {name: [] for name, *_ in pipe}, []
-    for s in sets:
-        g = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g, stream=stream):
-            outs = [f(s["h"], s["l"], s["c"]) for _, _, _, f in pipe]
-        graphs.append(g)
-        keep.append(outs)
-        for name, _, _, f in pipe:
-            kg = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(kg, stream=stream):
-                keep.append(f(s["h"], s["l"], s["c"]))
-            kgraphs[name].append(kg)
-    torch.cuda.synchronize()
-
-    def timed(glist, steps, warm):
-        with torch.cuda.stream(stream):
-            for i in range(warm):
-                glist[i % len(glist)].replay()
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            torch.cuda.synchronize()
-            e0.record(stream)
-            for i in range(steps):
-                glist[i % len(glist)].replay()
-            e1.record(stream)
-        e1.synchronize()
+    only = set(filter(None, a.only.split(","))) or {"pipeline7", "batch13", "scan256"}
+    T = Timer(torch)
+    work = {}
+    d = p7 = None
+    if "pipeline7" in only:
+        d, p7 = _pipeline7(a, torch, ops, T, dev)
+    if "batch13" in only:
+        work["batch13_4000x234"] = _batch13(a, torch, ops, T, dev)
+    T.keep.clear()
+    torch.cuda.empty_cache()
+    if "scan256" in only:
+        work["scan256_ema200_kama30_t3_sma200"] = _scan256(a, torch, ops, T, dev)
+    if a.copy_mb:
+        x = torch.empty(a.copy_mb << 18, dtype=torch.float32, device=dev)
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
         torch.cuda.synchronize()
-        return e0.elapsed_time(e1) / steps  # ms per replay
-
-    ms = timed(graphs, a.steps, a.warmup)
-    per_kernel = {}
-    for name, ins, nout, _ in pipe:
-        kms = timed(kgraphs[name], max(a.steps, 20), a.warmup)
-        alg = (len(ins) + nout) * 4 * n
-        per_kernel[name] = {"ms": kms, "alg_bytes": alg, "GBs": alg / (kms * 1e-3) / 1e9}
-    dom = max(per_kernel, key=lambda k: per_kernel[k]["ms"])
-    dk = per_kernel[dom]
-    alg_total = (3 + 12) * 4 * n
+    if p7 is None:
+        print(json.dumps({"workloads": work}))
+        return
+    n, ms = p7["bars"], p7["ms"]
+    dom = max(p7["kernels"], key=lambda k: p7["kernels"][k]["us"])
+    dk = p7["kernels"][dom]
     traffic = None
     tpath = os.path.join(HERE, "profiles", "traffic.json")
     if os.path.exists(tpath):
-        try:
-            traffic = json.load(open(tpath)).get(dom)
-        except Exception:
-            traffic = None
+        traffic = json.load(open(tpath)).get(dom)
     line = {
         "metric": METRIC,
         "value": n / (ms * 1e-3),
@@ -166,15 +275,19 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 io, f64 recurrences",
         "data": "synthetic seeded OHLCV (geometric random walk, seed 0)",
-        "config": {"workload": "device-API 7-indicator pipeline, 1 x 1,000,000 bars "
-                               "(SMA20, EMA20, RSI14, MACD12/26/9, BBANDS20, ATR14, STOCH14/3/3)",
+        "config": {"workload": "7-indicator pipeline, 1 x 1,000,000 bars (SMA20, EMA20, RSI14, "
+                               "MACD12/26/9, BBANDS20, ATR14, STOCH14/3/3), fused launch",
                    "bars": n, "graph": True, "rotated_sets": a.sets,
-                   "pass_alg_bytes": alg_total,
-                   "pass_GBs": alg_total / (ms * 1e-3) / 1e9,
-                   "pass_frac_hbm": alg_total / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
-                   "kernels": {k: round(v["ms"] * 1e3, 2) for k, v in per_kernel.items()}},
+                   "pass_alg_bytes": p7["alg_bytes"],
+                   "pass_GBs": _gbs(p7["alg_bytes"], ms),
+                   "pass_frac_hbm": _gbs(p7["alg_bytes"], ms) / PEAK_HBM_GBS,
+                   "kernels_us": {k: round(v["us"], 2) for k, v in p7["kernels"].items()},
+                   "drop_in_7call_ms": p7["drop_in_ms"],
+                   "drop_in_bars_per_s": n / (p7["drop_in_ms"] * 1e-3),
+                   "drop_in_kernels_us": p7["drop_in_kernels_us"]},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": dk["GBs"], "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": dk["GBs"] / PEAK_HBM_GBS, "traffic": traffic},
+        "workloads": work,
     }
     if not a.no_cpu:
         line["cpu_baseline"] = _cpu_baseline(d, a.cpu_seconds)

@@ -31,6 +31,39 @@ CTAPI_EXPORT ctStatus_t ctx_synth_ohlcv(unsigned long long seed, int firstSeries
                                         int n, float* open, float* high, float* low,
                                         float* close, float* volume, int threads);
 
+/* ---------------------------------------------------------------------------
+ * Fused multi-indicator pack: one launch computes any subset of SMA, EMA,
+ * WMA, DEMA, TEMA, RSI, MACD, BBANDS(SMA), ATR, ADX, STOCH (SMA/SMA), WILLR
+ * and CCI over every row, reading high/low/close once.  A null output skips
+ * that indicator; EMA, DEMA and TEMA of one pack share their period.  Each
+ * output has the layout and NaN warm-up of its ct_*_batch_d counterpart and
+ * agrees with it within float32 rounding of the FP64 recurrences.
+ * ------------------------------------------------------------------------- */
+typedef struct ctxPackParams {
+  int sma_period, ema_period, wma_period, dema_period, tema_period, rsi_period;
+  int macd_fast, macd_slow, macd_signal;
+  int bb_period;
+  float bb_up, bb_dn;
+  int atr_period, adx_period;
+  int stoch_k, stoch_d;
+  int willr_period, cci_period;
+} ctxPackParams;
+
+typedef struct ctxPackOutputs {
+  float *sma, *ema, *wma, *dema, *tema, *rsi;
+  float *macd, *macd_signal, *macd_hist;
+  float *bb_upper, *bb_middle, *bb_lower;
+  float *atr, *adx, *stoch_k, *stoch_d, *willr, *cci;
+} ctxPackOutputs;
+
+/* Dense series-major rows (numSeries x seriesLen), device pointers.  A single
+ * series is numSeries = 1.  Stream 0 blocks until done; other streams only
+ * enqueue and can be captured. */
+CTAPI_EXPORT ctStatus_t ctx_pack_batch_d(const float* d_high, const float* d_low,
+                                         const float* d_close, int seriesLen, int numSeries,
+                                         const ctxPackParams* params,
+                                         const ctxPackOutputs* outputs, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

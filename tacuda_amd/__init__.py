@@ -81,6 +81,30 @@ _sig("ctx_synth_ohlcv", ctypes.c_ulonglong, c_int, c_int, c_int, P, P, P, P, P, 
 _sig("ctx_workspace_reserve", c_void_p, ctypes.c_size_t, stream=False)
 
 
+class PackParams(ctypes.Structure):
+    """ctxPackParams of include/tacuda_amd.h."""
+    _fields_ = [(n, c_int) for n in ("sma_period", "ema_period", "wma_period", "dema_period",
+                                      "tema_period", "rsi_period", "macd_fast", "macd_slow",
+                                      "macd_signal", "bb_period")] + \
+        [("bb_up", c_float), ("bb_dn", c_float)] + \
+        [(n, c_int) for n in ("atr_period", "adx_period", "stoch_k", "stoch_d", "willr_period",
+                              "cci_period")]
+
+
+PACK_OUTPUTS = ("sma", "ema", "wma", "dema", "tema", "rsi", "macd", "macd_signal", "macd_hist",
+                "bb_upper", "bb_middle", "bb_lower", "atr", "adx", "stoch_k", "stoch_d",
+                "willr", "cci")
+
+
+class PackOutputs(ctypes.Structure):
+    """ctxPackOutputs of include/tacuda_amd.h (device pointers, NULL = skip)."""
+    _fields_ = [(n, c_void_p) for n in PACK_OUTPUTS]
+
+
+_sig("ctx_pack_batch_d", P, P, P, c_int, c_int, ctypes.POINTER(PackParams),
+     ctypes.POINTER(PackOutputs))
+
+
 class TacudaError(RuntimeError):
     def __init__(self, fn: str, status: int, msg: str):
         super().__init__(f"{fn} failed with status {status}: {msg}")

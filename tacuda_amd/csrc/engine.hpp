@@ -257,7 +257,7 @@ __device__ __forceinline__ void wave_compose(Aff<R>& v) {
 // Lane i loads slot idx0 + i when i < cnt (identity otherwise), spinning
 // until it carries this launch's epoch, and the wave composes them.
 template <int R>
-__device__ Aff<R> gather(const Seg& c, int level, int idx0, int cnt, bool group) {
+__device__ __forceinline__ Aff<R> gather(const Seg& c, int level, int idx0, int cnt, bool group) {
   const int lane = lane_id();
   Aff<R> v;
   aff_id(v);
@@ -281,7 +281,7 @@ __device__ Aff<R> gather(const Seg& c, int level, int idx0, int cnt, bool group)
 // Publish this segment's transform; the last wave of a full group then
 // publishes the group's composite.
 template <int R>
-__device__ void publish_tree(const Seg& c, int level, const Aff<R>& t) {
+__device__ __forceinline__ void publish_tree(const Seg& c, int level, const Aff<R>& t) {
   write_slot(c, slot_ptr(c, level, c.gseg, false), t);
   const int g = c.gseg / kGroup;
   if ((c.gseg % kGroup) == kGroup - 1) {
@@ -293,7 +293,7 @@ __device__ void publish_tree(const Seg& c, int level, const Aff<R>& t) {
 // State at bar s0-1: composite of everything from the group holding the
 // row's first segment (which resets the state) up to this segment.
 template <int R>
-__device__ void lookback(const Seg& c, int level, double (&y)[R]) {
+__device__ __forceinline__ void lookback(const Seg& c, int level, double (&y)[R]) {
   const int first = c.gseg - c.seg;
   const int G = c.gseg / kGroup;
   Aff<R> acc;
@@ -318,7 +318,7 @@ __device__ void lookback(const Seg& c, int level, double (&y)[R]) {
 // bar and `yL` its state there (valid when Ls lies in this segment).
 // Returns the state entering each lane's first bar in `yin`.
 template <int R>
-__device__ void level_carry(const Seg& c, int level, int Ls, const double (&yL)[R], Aff<R> loc,
+__device__ __forceinline__ void level_carry(const Seg& c, int level, int Ls, const double (&yL)[R], Aff<R> loc,
                             double (&yin)[R]) {
   Aff<R> tot;
   wave_scan_excl(loc, tot);

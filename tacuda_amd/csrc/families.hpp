@@ -1482,9 +1482,11 @@ struct FStoch {
     for (int j = 0; j < T::K + EX; ++j) {
       sk[j] = qnan_();
       if (j >= EX - (dp - 1)) {
-        // ascending window order, as the oracle's direct window sum
+        // ascending window order; indices stay compile-time after unrolling
         double s = 0.0;
-        for (int q = dp - 1; q >= 0; --q) s += fk[j - q >= 0 ? j - q : 0];
+#pragma unroll
+        for (int q = kStochMaxD - 1; q >= 0; --q)
+          if (q < dp && j - q >= 0) s += fk[j - q >= 0 ? j - q : 0];
         sk[j] = s / dp;
       }
     }
@@ -1498,7 +1500,9 @@ struct FStoch {
       }
       const int j = EX + i;
       double s = 0.0;
-      for (int q = dp - 1; q >= 0; --q) s += sk[j - q];
+#pragma unroll
+      for (int q = kStochMaxD - 1; q >= 0; --q)
+        if (q < dp) s += sk[j - q];
       ok[i] = (float)sk[j];
       od[i] = (float)(s / dp);
     }

@@ -9,7 +9,7 @@ from __future__ import annotations
 
 import torch
 
-from . import call
+from . import PACK_OUTPUTS, PackOutputs, PackParams, call
 
 _ONE_IN = ("sma", "ema", "wma", "dema", "tema", "rsi", "stddev", "var", "sum", "max", "min")
 _HLC = ("natr", "adx", "dx", "plus_di", "minus_di", "willr", "cci")
@@ -147,3 +147,32 @@ def minus_dm(high, low, period: int = 14):
     y = torch.empty_like(high)
     _dispatch("minus_dm", high, [high, low], [y], int(period))
     return y
+
+
+PACK_DEFAULTS = dict(sma_period=20, ema_period=20, wma_period=20, dema_period=20, tema_period=20,
+                     rsi_period=14, macd_fast=12, macd_slow=26, macd_signal=9, bb_period=20,
+                     bb_up=2.0, bb_dn=2.0, atr_period=14, adx_period=14, stoch_k=14, stoch_d=3,
+                     willr_period=14, cci_period=20)
+
+
+def pack(high, low, close, outputs=PACK_OUTPUTS, **periods):
+    """Several indicators of the same bars in one fused call (ctx_pack_batch_d).
+
+    ``outputs`` names the ctxPackOutputs fields to compute; ``periods``
+    overrides fields of ctxPackParams (defaults: PACK_DEFAULTS).  EMA, DEMA
+    and TEMA share one period.  Returns a dict of tensors shaped like close.
+    """
+    _check(close)
+    if high is None:
+        high = low = close
+    _check(high, low)
+    unknown = set(outputs) - set(PACK_OUTPUTS)
+    if unknown:
+        raise ValueError(f"unknown pack outputs {sorted(unknown)}")
+    prm = PackParams(**{**PACK_DEFAULTS, **periods})
+    res = {k: torch.empty_like(close) for k in outputs}
+    outs = PackOutputs(**{k: v.data_ptr() for k, v in res.items()})
+    n, rows = _shape(close)
+    call("ctx_pack_batch_d", high.data_ptr(), low.data_ptr(), close.data_ptr(), n, rows, prm, outs,
+         _stream())
+    return res

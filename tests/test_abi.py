@@ -95,3 +95,26 @@ def test_synthetic_generator_is_seeded_and_well_formed():
     assert abs(r.std() - 0.01) < 0.001 and abs(r.mean()) < 0.001
     v = np.log(a["volume"].astype(np.float64))
     assert abs(v.mean() - 10) < 0.1 and abs(v.std() - 1) < 0.1
+
+
+def test_pack_arguments_are_validated_without_the_device():
+    import tacuda_amd as ta
+    from tacuda_amd.ops import PACK_DEFAULTS
+
+    L = ta.lib()
+    buf = np.zeros(8, np.float32)
+    p = buf.ctypes.data
+    prm = ta.PackParams(**PACK_DEFAULTS)
+    none = ta.PackOutputs()
+    assert L.ctx_pack_batch_d(p, p, p, 8, 1, prm, none, None) == 4
+    assert b"no output" in L.ct_get_last_error()
+    assert L.ctx_pack_batch_d(p, p, p, 0, 1, prm, ta.PackOutputs(ema=p), None) == 4
+    # EMA, DEMA and TEMA share one cascade in a pack
+    mixed = ta.PackParams(**{**PACK_DEFAULTS, "dema_period": 30})
+    assert L.ctx_pack_batch_d(p, p, p, 8, 1, mixed, ta.PackOutputs(ema=p, dema=p), None) == 6
+    bad = ta.PackParams(**{**PACK_DEFAULTS, "rsi_period": 0})
+    assert L.ctx_pack_batch_d(p, p, p, 8, 1, bad, ta.PackOutputs(rsi=p), None) == 4
+    deep = ta.PackParams(**{**PACK_DEFAULTS, "stoch_d": 10})
+    assert L.ctx_pack_batch_d(p, p, p, 8, 1, deep, ta.PackOutputs(stoch_k=p), None) == 6
+    # high/low are required only by the outputs that read them
+    assert L.ctx_pack_batch_d(None, None, p, 8, 1, prm, ta.PackOutputs(atr=p), None) == 4

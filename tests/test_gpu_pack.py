@@ -1,0 +1,168 @@
+"""GPU parity of the fused multi-indicator launch (ctx_pack_batch_d).
+
+Every output of the pack is checked against the oracle's single-family
+function on the same bars: window outputs (SMA, WMA, BBANDS, STOCH, WILLR,
+CCI) bit-identical, recurrence outputs (EMA, DEMA, TEMA, RSI, MACD, ATR,
+ADX) within 1e-5 and at least 99.9% bit-identical, NaN masks identical.
+"""
+import numpy as np
+import pytest
+
+from tests.parity import compare
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+EXACT = {"sma", "wma", "bb_upper", "bb_middle", "bb_lower", "stoch_k", "stoch_d", "willr", "cci"}
+
+DEFAULT = dict(sma_period=20, ema_period=20, wma_period=20, dema_period=20, tema_period=20,
+               rsi_period=14, macd_fast=12, macd_slow=26, macd_signal=9, bb_period=20, bb_up=2.0,
+               bb_dn=2.0, atr_period=14, adx_period=14, stoch_k=14, stoch_d=3, willr_period=14,
+               cci_period=20)
+# short and degenerate periods: period-1 contracts, swapped MACD periods,
+# signal 1, asymmetric bands, ADX 2, STOCH 1/1
+EDGE = dict(sma_period=1, ema_period=2, wma_period=1, dema_period=2, tema_period=2, rsi_period=1,
+            macd_fast=26, macd_slow=12, macd_signal=1, bb_period=1, bb_up=2.0, bb_dn=1.5,
+            atr_period=1, adx_period=2, stoch_k=1, stoch_d=1, willr_period=1, cci_period=1)
+LONG = dict(sma_period=200, ema_period=50, wma_period=100, dema_period=50, tema_period=50,
+            rsi_period=30, macd_fast=19, macd_slow=39, macd_signal=9, bb_period=50, bb_up=2.5,
+            bb_dn=2.5, atr_period=50, adx_period=30, stoch_k=50, stoch_d=9, willr_period=60,
+            cci_period=40)
+PERIOD1 = dict(DEFAULT, ema_period=1, dema_period=1, tema_period=1)
+PARAMS = {"default": DEFAULT, "edge": EDGE, "long": LONG, "ema1": PERIOD1}
+
+
+def _want(R, p, h, l, c, name):
+    if name == "sma":
+        return R.sma(c, p["sma_period"])
+    if name == "ema":
+        return R.ema(c, p["ema_period"])
+    if name == "wma":
+        return R.wma(c, p["wma_period"])
+    if name == "dema":
+        return R.dema(c, p["dema_period"])
+    if name == "tema":
+        return R.tema(c, p["tema_period"])
+    if name == "rsi":
+        return R.rsi(c, p["rsi_period"])
+    if name.startswith("macd"):
+        m = R.macd(c, p["macd_fast"], p["macd_slow"], p["macd_signal"])
+        return m[{"macd": 0, "macd_signal": 1, "macd_hist": 2}[name]]
+    if name.startswith("bb_"):
+        b = R.bbands(c, p["bb_period"], float(np.float32(p["bb_up"])), float(np.float32(p["bb_dn"])))
+        return b[{"bb_upper": 0, "bb_middle": 1, "bb_lower": 2}[name]]
+    if name == "atr":
+        return R.atr(h, l, c, p["atr_period"])
+    if name == "adx":
+        return R.adx(h, l, c, p["adx_period"])
+    if name.startswith("stoch"):
+        return R.stoch(h, l, c, p["stoch_k"], p["stoch_d"])[0 if name == "stoch_k" else 1]
+    if name == "willr":
+        return R.willr(h, l, c, p["willr_period"])
+    if name == "cci":
+        return R.cci(h, l, c, p["cci_period"])
+    raise KeyError(name)
+
+
+def _run(dev, d, params, outputs=None, close_only=False):
+    import tacuda_amd as ta
+    from oracle import ORC
+
+    outputs = outputs or ta.PACK_OUTPUTS
+    T = {k: torch.from_numpy(np.ascontiguousarray(d[k])).to(dev) for k in ("high", "low", "close")}
+    if close_only:
+        got = ta.ops.pack(None, None, T["close"], outputs=outputs, **params)
+    else:
+        got = ta.ops.pack(T["high"], T["low"], T["close"], outputs=outputs, **params)
+    torch.cuda.synchronize()
+    h, l, c = d["high"], d["low"], d["close"]
+    for name in outputs:
+        g = got[name].cpu().numpy()
+        if c.ndim == 1:
+            w = _want(ORC, params, h, l, c, name)
+        else:
+            w = np.stack([_want(ORC, params, h[r], l[r], c[r], name) for r in range(c.shape[0])])
+        if name in EXACT:
+            compare(g, w, tol=0.0, min_exact=1.0, name=name)
+        else:
+            compare(g, w, tol=1e-5, min_exact=0.999, name=name)
+
+
+def _data(n, rows=1, seed=0):
+    import tacuda_amd as ta
+
+    return ta.synth_ohlcv(n, rows=rows, seed=seed)
+
+
+@pytest.mark.parametrize("pname", sorted(PARAMS))
+@pytest.mark.parametrize("n", [1, 37, 256, 1000, 1025, 70001])
+def test_pack_single_series(dev, pname, n):
+    print(f"pack single {pname} n={n}", flush=True)
+    _run(dev, _data(n, seed=11), PARAMS[pname])
+
+
+@pytest.mark.parametrize("pname", ["default", "edge", "long"])
+@pytest.mark.parametrize("shape", [(4000, 234), (300, 257), (5, 3000), (2, 300000)])
+def test_pack_dense_batch(dev, pname, shape):
+    rows, n = shape
+    print(f"pack batch {pname} {rows}x{n}", flush=True)
+    _run(dev, _data(n, rows=rows, seed=13), PARAMS[pname])
+
+
+@pytest.mark.parametrize("subset", [
+    ("ema",), ("dema",), ("tema", "rsi"), ("macd_hist",), ("atr",), ("adx",),
+    ("sma", "adx"), ("stoch_d", "willr"), ("cci", "tema", "atr"),
+])
+@pytest.mark.parametrize("n", [300, 20000])
+def test_pack_subsets(dev, subset, n):
+    print(f"pack subset {subset} n={n}", flush=True)
+    _run(dev, _data(n, seed=17), DEFAULT, outputs=subset)
+
+
+def test_pack_close_only_inputs(dev):
+    print("pack close-only", flush=True)
+    outs = ("sma", "ema", "wma", "dema", "tema", "rsi", "macd", "macd_signal", "macd_hist",
+            "bb_upper", "bb_middle", "bb_lower")
+    _run(dev, _data(5000, seed=19), DEFAULT, outputs=outs, close_only=True)
+
+
+def _poison(d, prefix=0, holes=(), infs=()):
+    out = {k: v.copy() for k, v in d.items()}
+    for k in ("open", "high", "low", "close"):
+        out[k][..., :prefix] = np.nan
+        for i in holes:
+            out[k][..., i] = np.nan
+        for i in infs:
+            out[k][..., i] = np.inf
+    return out
+
+
+@pytest.mark.parametrize("case", ["prefix7", "prefix1500", "hole", "inf"])
+def test_pack_non_finite(dev, case):
+    print(f"pack non-finite {case}", flush=True)
+    d = _data(6000, seed=23)
+    kw = {"prefix7": dict(prefix=7), "prefix1500": dict(prefix=1500),
+          "hole": dict(holes=(3000,)), "inf": dict(infs=(2500,))}[case]
+    _run(dev, _poison(d, **kw), DEFAULT)
+
+
+def test_pack_period_longer_than_series(dev):
+    print("pack period > n", flush=True)
+    _run(dev, _data(40, seed=29), LONG)
+
+
+def test_pack_repeat_launches_agree(dev):
+    """Back-to-back launches on one stream reuse the look-back workspace."""
+    import tacuda_amd as ta
+
+    print("pack repeat", flush=True)
+    d = _data(200000, seed=31)
+    T = [torch.from_numpy(d[k]).to(dev) for k in ("high", "low", "close")]
+    first = ta.ops.pack(*T)
+    for _ in range(3):
+        again = ta.ops.pack(*T)
+    torch.cuda.synchronize()
+    for k in first:
+        a, b = first[k].cpu().numpy(), again[k].cpu().numpy()
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), k
