@@ -85,11 +85,18 @@ __device__ __forceinline__ void stage(float* lds, const float* src, const Seg& c
   }
 }
 
+// Load through an LDS-qualified pointer: a generic pointer that may alias
+// global memory would otherwise be read with flat loads.
+typedef __attribute__((address_space(3))) const float lds_cfloat;
+__device__ __forceinline__ float lds_ld(const float* p, int i) {
+  return ((lds_cfloat*)p)[i];
+}
+
 // Row-relative bar t (t < s0 + SEG): LDS when staged, global before that.
 template <class T>
 __device__ __forceinline__ float at(const float* lds, const float* src, const Seg& c, int t) {
   const int e = t - (c.s0 - T::H);
-  if (e >= 0) return lds[T::pos(e)];
+  if (e >= 0) return lds_ld(lds, T::pos(e));
   return t >= 0 ? src[c.rowOff + t] : qnanf_();
 }
 
@@ -181,7 +188,7 @@ __device__ __forceinline__ bool is_zero14(double v) { return (-1e-14 < v) && (v 
 // f + (j+1)*(p-1).  ta_EMA.c:37-147, ta_DEMA.c:37-206, ta_TEMA.c:37-227,
 // ta_T3.c:37-226.
 template <class T, int NLV>
-__device__ void ema_cascade(const Seg& c, const float* li, const float* x, const float (&xv)[T::K],
+__device__ __forceinline__ void ema_cascade(const Seg& c, const float* li, const float* x, const float (&xv)[T::K],
                             int f, int p, double beta, double k, double (&e)[NLV][T::K],
                             int (&L)[NLV], int levelBase) {
   const int lane = lane_id();
@@ -261,7 +268,7 @@ struct FEma {
     int p;
   };
   template <class T>
-  __device__ static void run(const Seg& c, float* li, float* lo, const P& pr, const IO& io) {
+  __device__ __forceinline__ static void run(const Seg& c, float* li, float* lo, const P& pr, const IO& io) {
     const float* x = io.in[0];
     stage<T>(li, x, c);
     wave_sync();
@@ -291,7 +298,7 @@ struct FDema {
     int p;
   };
   template <class T>
-  __device__ static void run(const Seg& c, float* li, float* lo, const P& pr, const IO& io) {
+  __device__ __forceinline__ static void run(const Seg& c, float* li, float* lo, const P& pr, const IO& io) {
     const float* x = io.in[0];
     stage<T>(li, x, c);
     wave_sync();
@@ -323,7 +330,7 @@ struct FTema {
     int p;
   };
   template <class T>
-  __device__ static void run(const Seg& c, float* li, float* lo, const P& pr, const IO& io) {
+  __device__ __forceinline__ static void run(const Seg& c, float* li, float* lo, const P& pr, const IO& io) {
     const float* x = io.in[0];
     stage<T>(li, x, c);
     wave_sync();
@@ -356,7 +363,7 @@ struct FT3 {
     double v;
   };
   template <class T>
-  __device__ static void run(const Seg& c, float* li, float* lo, const P& pr, const IO& io) {
+  __device__ __forceinline__ static void run(const Seg& c, float* li, float* lo, const P& pr, const IO& io) {
     const float* x = io.in[0];
     stage<T>(li, x, c);
     wave_sync();
@@ -395,7 +402,7 @@ struct FMacd {
     int fast, slow, sig;
   };
   template <class T>
-  __device__ static void run(const Seg& c, float* li, float* lo, const P& pr, const IO& io) {
+  __device__ __forceinline__ static void run(const Seg& c, float* li, float* lo, const P& pr, const IO& io) {
     const float* x = io.in[0];
     stage<T>(li, x, c);
     wave_sync();
@@ -519,7 +526,7 @@ struct FRsi {
     int p;
   };
   template <class T>
-  __device__ static void run(const Seg& c, float* li, float* lo, const P& pr, const IO& io) {
+  __device__ __forceinline__ static void run(const Seg& c, float* li, float* lo, const P& pr, const IO& io) {
     const float* x = io.in[0];
     stage<T>(li, x, c);
     wave_sync();
@@ -615,7 +622,7 @@ struct FAtrT {
     int p;
   };
   template <class T>
-  __device__ static void run(const Seg& c, float* li, float* lo, const P& pr, const IO& io) {
+  __device__ __forceinline__ static void run(const Seg& c, float* li, float* lo, const P& pr, const IO& io) {
     float* lh = li;
     float* ll = li + T::PLEN;
     float* lc = li + 2 * T::PLEN;
@@ -716,7 +723,7 @@ struct FDir {
     int p;
   };
   template <class T>
-  __device__ static void run(const Seg& c, float* li, float* lo, const P& pr, const IO& io) {
+  __device__ __forceinline__ static void run(const Seg& c, float* li, float* lo, const P& pr, const IO& io) {
     constexpr bool needTR = KIND >= kPDI;
     float* lh = li;
     float* ll = li + T::PLEN;
@@ -933,7 +940,7 @@ struct FKama {
     return s;
   }
   template <class T>
-  __device__ static void run(const Seg& c, float* li, float* lo, const P& pr, const IO& io) {
+  __device__ __forceinline__ static void run(const Seg& c, float* li, float* lo, const P& pr, const IO& io) {
     const float* x = io.in[0];
     stage<T>(li, x, c);
     wave_sync();
@@ -1023,7 +1030,7 @@ struct FSumT {
     int p;
   };
   template <class T>
-  __device__ static void run(const Seg& c, float* li, float* lo, const P& pr, const IO& io) {
+  __device__ __forceinline__ static void run(const Seg& c, float* li, float* lo, const P& pr, const IO& io) {
     const float* x = io.in[0];
     stage<T>(li, x, c);
     wave_sync();
@@ -1068,7 +1075,7 @@ struct FWma {
     int p;
   };
   template <class T>
-  __device__ static void run(const Seg& c, float* li, float* lo, const P& pr, const IO& io) {
+  __device__ __forceinline__ static void run(const Seg& c, float* li, float* lo, const P& pr, const IO& io) {
     const float* x = io.in[0];
     stage<T>(li, x, c);
     wave_sync();
@@ -1149,7 +1156,7 @@ struct FVarT {
     int p;
   };
   template <class T>
-  __device__ static void run(const Seg& c, float* li, float* lo, const P& pr, const IO& io) {
+  __device__ __forceinline__ static void run(const Seg& c, float* li, float* lo, const P& pr, const IO& io) {
     const float* x = io.in[0];
     stage<T>(li, x, c);
     wave_sync();
@@ -1181,7 +1188,7 @@ struct FBbands {
     double up, dn;
   };
   template <class T>
-  __device__ static void run(const Seg& c, float* li, float* lo, const P& pr, const IO& io) {
+  __device__ __forceinline__ static void run(const Seg& c, float* li, float* lo, const P& pr, const IO& io) {
     const float* x = io.in[0];
     stage<T>(li, x, c);
     wave_sync();
@@ -1224,7 +1231,7 @@ struct FExtT {
     int p;
   };
   template <class T>
-  __device__ static void run(const Seg& c, float* li, float* lo, const P& pr, const IO& io) {
+  __device__ __forceinline__ static void run(const Seg& c, float* li, float* lo, const P& pr, const IO& io) {
     const float* x = io.in[0];
     stage<T>(li, x, c);
     wave_sync();
@@ -1277,7 +1284,7 @@ struct FWillr {
     int p;
   };
   template <class T>
-  __device__ static void run(const Seg& c, float* li, float* lo, const P& pr, const IO& io) {
+  __device__ __forceinline__ static void run(const Seg& c, float* li, float* lo, const P& pr, const IO& io) {
     float* lh = li;
     float* ll = li + T::PLEN;
     stage<T>(lh, io.in[0], c);
@@ -1319,7 +1326,7 @@ struct FCci {
            3.0;
   }
   template <class T>
-  __device__ static void run(const Seg& c, float* li, float* lo, const P& pr, const IO& io) {
+  __device__ __forceinline__ static void run(const Seg& c, float* li, float* lo, const P& pr, const IO& io) {
     float* lh = li;
     float* ll = li + T::PLEN;
     float* lc = li + 2 * T::PLEN;
@@ -1361,7 +1368,7 @@ struct FAroonT {
     int p;
   };
   template <class T>
-  __device__ static void run(const Seg& c, float* li, float* lo, const P& pr, const IO& io) {
+  __device__ __forceinline__ static void run(const Seg& c, float* li, float* lo, const P& pr, const IO& io) {
     float* lh = li;
     float* ll = li + T::PLEN;
     stage<T>(lh, io.in[0], c);
@@ -1414,7 +1421,7 @@ struct FTrange {
     int unused;
   };
   template <class T>
-  __device__ static void run(const Seg& c, float* li, float* lo, const P&, const IO& io) {
+  __device__ __forceinline__ static void run(const Seg& c, float* li, float* lo, const P&, const IO& io) {
     float* lh = li;
     float* ll = li + T::PLEN;
     float* lc = li + 2 * T::PLEN;
@@ -1449,7 +1456,7 @@ struct FStoch {
     int kp, dp;
   };
   template <class T>
-  __device__ static void run(const Seg& c, float* li, float* lo, const P& pr, const IO& io) {
+  __device__ __forceinline__ static void run(const Seg& c, float* li, float* lo, const P& pr, const IO& io) {
     float* lh = li;
     float* ll = li + T::PLEN;
     float* lc = li + 2 * T::PLEN;

@@ -19,6 +19,7 @@
 #include "families.hpp"
 #include "host.hpp"
 #include "series.hpp"
+#include "windows.hpp"
 
 namespace tca {
 
@@ -96,211 +97,29 @@ __device__ __forceinline__ void pack_carry(const Seg& c, int level, Aff<R> loc, 
   for (int r = 0; r < R; ++r) yin[r] = aff_apply(loc, r, ys[r]);
 }
 
-// PART 0: window families; PART 1: recurrences.  They run as two launches so
-// each kernel keeps its registers under 128 (four waves per SIMD).
+// Recurrences.  PART 1 reads close only (EMA cascade, RSI, MACD); PART 2
+// reads high/low/close (ATR, ADX).  Separate launches keep each kernel's
+// registers low enough for several waves per SIMD.
 template <class T, int PART>
 __device__ __forceinline__ void pack_run(const Seg& c, float* li, float* lo, const PackP& P, const PackIO& io) {
   constexpr int K = T::K;
   float* lh = li;
   float* ll = li + T::PLEN;
   float* lc = li + 2 * T::PLEN;
-  stage<T>(lh, io.h, c);
-  stage<T>(ll, io.l, c);
+  if (PART == 2) {
+    stage<T>(lh, io.h, c);
+    stage<T>(ll, io.l, c);
+  }
   stage<T>(lc, io.c, c);
   wave_sync();
-  float hv[K], lv[K], cv[K], ov[K], ov2[K], ov3[K];
-  lane_vals<T>(lh, hv);
-  lane_vals<T>(ll, lv);
+  float hv[K], lv[K], cv[K], ov[K];
+  if (PART == 2) {
+    lane_vals<T>(lh, hv);
+    lane_vals<T>(ll, lv);
+  }
   lane_vals<T>(lc, cv);
   const int a = c.s0 + lane_id() * K;
   const unsigned m = P.mask;
-  IO wio{};
-  wio.in[0] = io.h;
-  wio.in[1] = io.l;
-  wio.in[2] = io.c;
-
-  // ------------------------------------------------------------ windows --
-  if (PART == 0) {
-  if (m & (1u << kOSma)) {
-    const int p = P.smaP;
-    double s = 0.0;
-    bool fresh = true;
-#pragma unroll
-    for (int i = 0; i < K; ++i) {
-      const int t = a + i;
-      if (t < p - 1 || t >= c.n) {
-        ov[i] = qnanf_();
-        fresh = true;
-        continue;
-      }
-      const float xn = cv[i];
-      const float xo = (t - p >= 0) ? at<T>(lc, io.c, c, t - p) : 0.0f;
-      if (fresh || !isfinite(xn) || !isfinite(xo) || !isfinite(s)) {
-        s = 0.0;
-        for (int j = t - p + 1; j <= t; ++j) s += (double)at<T>(lc, io.c, c, j);
-        fresh = false;
-      } else {
-        s += (double)xn;
-        s -= (double)xo;
-      }
-      ov[i] = (float)(s / (double)p);
-    }
-    pack_store<T>(c, lo, ov, io.out[kOSma]);
-  }
-  if (m & (1u << kOWma)) {
-    const int p = P.wmaP;
-    const double divider = (double)p * (p + 1) / 2.0;
-    double sw = 0.0, sp = 0.0;
-    bool fresh = true;
-#pragma unroll
-    for (int i = 0; i < K; ++i) {
-      const int t = a + i;
-      if (t < p - 1 || t >= c.n) {
-        ov[i] = qnanf_();
-        fresh = true;
-        continue;
-      }
-      if (p == 1) {
-        ov[i] = cv[i];
-        continue;
-      }
-      const float xn = cv[i];
-      const float xo = (t - p >= 0) ? at<T>(lc, io.c, c, t - p) : 0.0f;
-      if (fresh || !isfinite(xn) || !isfinite(xo) || !isfinite(sw) || !isfinite(sp)) {
-        sw = 0.0;
-        sp = 0.0;
-        int w = 1;
-        for (int j = t - p + 1; j <= t; ++j, ++w) {
-          const double v = (double)at<T>(lc, io.c, c, j);
-          sp += v;
-          sw += v * w;
-        }
-        fresh = false;
-      } else {
-        sw -= sp;
-        sw += (double)xn * p;
-        sp += (double)xn;
-        sp -= (double)xo;
-      }
-      ov[i] = (float)(sw / divider);
-    }
-    pack_store<T>(c, lo, ov, io.out[kOWma]);
-  }
-  if (m & ((1u << kOBbU) | (1u << kOBbM) | (1u << kOBbL))) {
-    const int p = P.bbP;
-#pragma unroll
-    for (int i = 0; i < K; ++i) {
-      const int t = a + i;
-      if (t < p - 1 || t >= c.n) {
-        ov[i] = ov2[i] = ov3[i] = qnanf_();
-        continue;
-      }
-      double mean;
-      const double var = window_var<T>(lc, io.c, c, t, p, &mean);
-      const double sd = (var != 0.0) ? sqrt(var) : 0.0;
-      ov2[i] = (float)mean;
-      if (P.bbUp == P.bbDn) {
-        const double d = sd * P.bbUp;
-        ov[i] = (float)(mean + d);
-        ov3[i] = (float)(mean - d);
-      } else {
-        ov[i] = (float)fma(sd, P.bbUp, mean);
-        ov3[i] = (float)(mean - sd * P.bbDn);
-      }
-    }
-    pack_store<T>(c, lo, ov, io.out[kOBbU]);
-    pack_store<T>(c, lo, ov2, io.out[kOBbM]);
-    pack_store<T>(c, lo, ov3, io.out[kOBbL]);
-  }
-  if (m & ((1u << kOStK) | (1u << kOStD))) {
-    const int kp = P.stK, dp = P.stD;
-    const int lb = kp - 1 + 2 * (dp - 1);
-    constexpr int EX = 2 * (kStochMaxD - 1);
-    double fk[K + EX], sk[K + EX];
-    const int ex = 2 * (dp - 1);
-#pragma unroll
-    for (int j = 0; j < K + EX; ++j) {
-      const int t = a - EX + j;
-      fk[j] = qnan_();
-      if (j >= EX - ex && t >= kp - 1 && t < c.n) {
-        float hh, lw;
-        window_hl<T>(lh, ll, wio, c, t, kp, hh, lw);
-        const double range = (double)hh - (double)lw;
-        const double cl = (double)at<T>(lc, io.c, c, t);
-        fk[j] = (fabs(range) <= 1e-14 * (fabs((double)hh) + fabs((double)lw)))
-                    ? 0.0
-                    : (cl - (double)lw) / range * 100.0;
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < K + EX; ++j) {
-      sk[j] = qnan_();
-      if (j >= EX - (dp - 1)) {
-        // ascending window order; indices stay compile-time after unrolling
-        double s = 0.0;
-#pragma unroll
-        for (int q = kStochMaxD - 1; q >= 0; --q)
-          if (q < dp && j - q >= 0) s += fk[j - q >= 0 ? j - q : 0];
-        sk[j] = s / dp;
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < K; ++i) {
-      const int t = a + i;
-      if (t < lb || t >= c.n) {
-        ov[i] = ov2[i] = qnanf_();
-        continue;
-      }
-      const int j = EX + i;
-      double s = 0.0;
-#pragma unroll
-      for (int q = kStochMaxD - 1; q >= 0; --q)
-        if (q < dp) s += sk[j - q];
-      ov[i] = (float)sk[j];
-      ov2[i] = (float)(s / dp);
-    }
-    pack_store<T>(c, lo, ov, io.out[kOStK]);
-    pack_store<T>(c, lo, ov2, io.out[kOStD]);
-  }
-  if (m & (1u << kOWillr)) {
-    const int p = P.willrP;
-#pragma unroll
-    for (int i = 0; i < K; ++i) {
-      const int t = a + i;
-      if (t < p - 1 || t >= c.n) {
-        ov[i] = qnanf_();
-        continue;
-      }
-      float hh, lw;
-      window_hl<T>(lh, ll, wio, c, t, p, hh, lw);
-      const double den = (double)hh - (double)lw;
-      const double v = den == 0.0 ? 0.0 : (((double)hh - (double)cv[i]) / den) * -100.0;
-      ov[i] = isnan(v) ? qnanf_() : (float)fmin(0.0, fmax(-100.0, v));
-    }
-    pack_store<T>(c, lo, ov, io.out[kOWillr]);
-  }
-  if (m & (1u << kOCci)) {
-    const int p = P.cciP;
-#pragma unroll
-    for (int i = 0; i < K; ++i) {
-      const int t = a + i;
-      if (t < p - 1 || t >= c.n) {
-        ov[i] = qnanf_();
-        continue;
-      }
-      double s = 0.0;
-      for (int j = 0; j < p; ++j) s += FCci::tp<T>(lh, ll, lc, wio, c, t - j);
-      const double sma = s / p;
-      double dev = 0.0;
-      for (int j = 0; j < p; ++j) dev += fabs(FCci::tp<T>(lh, ll, lc, wio, c, t - j) - sma);
-      const double md = dev / p;
-      const double num = FCci::tp<T>(lh, ll, lc, wio, c, t) - sma;
-      ov[i] = (num != 0.0 && md != 0.0) ? (float)(num / (0.015 * md)) : 0.0f;
-    }
-    pack_store<T>(c, lo, ov, io.out[kOCci]);
-  }
-  return;
-  }
 
   // ------------------------------------------------------------ level 1 --
   // PART 1: close recurrences (EMA cascade, RSI, MACD); PART 2: high/low/close
@@ -640,6 +459,183 @@ __device__ __forceinline__ void pack_run(const Seg& c, float* li, float* lo, con
   }
 }
 
+// ================================================================ windows ==
+// SMA, WMA, BBANDS, STOCH, WILLR and CCI over a column tile (one bar per lane
+// per 64-bar chunk), each window evaluated directly in the oracle's order.
+template <class C>
+__device__ __forceinline__ void pack_windows(const Seg& c, ColLds<C::E>& s, const PackP& P,
+                                             const PackIO& io) {
+  const unsigned m = P.mask;
+  const int lane = lane_id();
+  const int base = c.s0 - C::H;  // bar of e = 0
+  const bool needHL = m & ((1u << kOStK) | (1u << kOStD) | (1u << kOWillr) | (1u << kOCci));
+  if (needHL) {
+    col_stage<C>(s.h, io.h, c);
+    col_stage<C>(s.l, io.l, c);
+  }
+  col_stage<C>(s.c, io.c, c);
+  col_sync();
+
+  if (m & (1u << kOSma)) {
+    const int p = P.smaP;
+    for (int j = C::NH; j < C::NC; ++j) {
+      const int e = j * kWave + lane;
+      double sum = 0.0;
+      for (int q = e - p + 1; q <= e; ++q) sum += (double)lds_f(s.c, q);
+      const float v = (base + e >= p - 1) ? (float)(sum / (double)p) : qnanf_();
+      col_store<C>(io.out[kOSma], c, j, v);
+    }
+  }
+  if (m & (1u << kOWma)) {
+    const int p = P.wmaP;
+    const double divider = (double)p * (p + 1) / 2.0;
+    for (int j = C::NH; j < C::NC; ++j) {
+      const int e = j * kWave + lane;
+      float v;
+      if (p == 1) {
+        v = lds_f(s.c, e);
+      } else {
+        double sum = 0.0;
+        double w = 1.0;
+        for (int q = e - p + 1; q <= e; ++q, w += 1.0) sum += (double)lds_f(s.c, q) * w;
+        v = (float)(sum / divider);
+      }
+      col_store<C>(io.out[kOWma], c, j, (base + e >= p - 1) ? v : qnanf_());
+    }
+  }
+  if (m & ((1u << kOBbU) | (1u << kOBbM) | (1u << kOBbL))) {
+    const int p = P.bbP;
+    const double inv = 1.0 / (double)p;
+    for (int j = C::NH; j < C::NC; ++j) {
+      const int e = j * kWave + lane;
+      const double shift = (double)lds_f(s.c, e - p + 1);
+      double sum = 0.0, s1 = 0.0, s2 = 0.0;
+      for (int q = e - p + 1; q <= e; ++q) {
+        const double x = (double)lds_f(s.c, q);
+        sum += x;
+        const double d = x - shift;
+        s1 += d;
+        s2 += d * d;
+      }
+      const double mean = sum / p;
+      const double var = col_var_finish(s1, s2, inv);
+      const double sd = (var != 0.0) ? sqrt(var) : 0.0;
+      float up, lo;
+      if (P.bbUp == P.bbDn) {
+        const double d = sd * P.bbUp;
+        up = (float)(mean + d);
+        lo = (float)(mean - d);
+      } else {
+        up = (float)fma(sd, P.bbUp, mean);
+        lo = (float)(mean - sd * P.bbDn);
+      }
+      const bool ok = base + e >= p - 1;
+      col_store<C>(io.out[kOBbU], c, j, ok ? up : qnanf_());
+      col_store<C>(io.out[kOBbM], c, j, ok ? (float)mean : qnanf_());
+      col_store<C>(io.out[kOBbL], c, j, ok ? lo : qnanf_());
+    }
+  }
+  const bool wantSt = m & ((1u << kOStK) | (1u << kOStD));
+  const bool wantW = m & (1u << kOWillr);
+  if (wantSt || wantW) {
+    // WILLR first when it shares the extrema table with STOCH, whose
+    // smoothing then reuses the table's LDS
+    const int kp = wantSt ? P.stK : P.willrP;
+    int k = col_extrema_table<C>(s, kp);
+    auto willr = [&]() {
+      const int p = P.willrP;
+      for (int j = C::NH; j < C::NC; ++j) {
+        const int e = j * kWave + lane;
+        float hh, ll;
+        col_hl<C>(s, e, p, k, hh, ll);
+        const double den = (double)hh - (double)ll;
+        const double v =
+            den == 0.0 ? 0.0 : (((double)hh - (double)lds_f(s.c, e)) / den) * -100.0;
+        const float o = isnan(v) ? qnanf_() : (float)fmin(0.0, fmax(-100.0, v));
+        col_store<C>(io.out[kOWillr], c, j, (base + e >= p - 1) ? o : qnanf_());
+      }
+    };
+    if (wantW && P.willrP == kp) willr();
+    if (wantSt) {
+      const int dp = P.stD;
+      const int lb = kp - 1 + 2 * (dp - 1);
+      // fast %K from bar s0 - 2(d-1), slow %K from s0 - (d-1)
+      const int eK = C::H - 2 * (dp - 1), eS = C::H - (dp - 1);
+      for (int j = eK / kWave; j < C::NC; ++j) {
+        const int e = j * kWave + lane;
+        float hh, ll;
+        col_hl<C>(s, e < eK ? eK : e, kp, k, hh, ll);
+        const double range = (double)hh - (double)ll;
+        const double cl = (double)lds_f(s.c, e);
+        const double fk = (fabs(range) <= 1e-14 * (fabs((double)hh) + fabs((double)ll)))
+                              ? 0.0
+                              : (cl - (double)ll) / range * 100.0;
+        if (e >= eK) lds_wd(s.d0, e, fk);
+      }
+      col_sync();
+      double* d1 = s.d1();  // the extrema table is no longer read
+      for (int j = eS / kWave; j < C::NC; ++j) {
+        const int e = j * kWave + lane;
+        double sum = 0.0;
+        if (e >= eS)
+          for (int q = e - dp + 1; q <= e; ++q) sum += lds_d(s.d0, q);
+        if (e >= eS) lds_wd(d1, e, sum / dp);
+      }
+      col_sync();
+      for (int j = C::NH; j < C::NC; ++j) {
+        const int e = j * kWave + lane;
+        double sum = 0.0;
+        for (int q = e - dp + 1; q <= e; ++q) sum += lds_d(d1, q);
+        const bool ok = base + e >= lb;
+        col_store<C>(io.out[kOStK], c, j, ok ? (float)lds_d(d1, e) : qnanf_());
+        col_store<C>(io.out[kOStD], c, j, ok ? (float)(sum / dp) : qnanf_());
+      }
+      col_sync();
+      if (wantW && P.willrP != kp) {
+        k = col_extrema_table<C>(s, P.willrP);
+        willr();
+      }
+    } else {
+      willr();
+    }
+  }
+  if (m & (1u << kOCci)) {
+    const int p = P.cciP;
+    col_sync();
+    const int e0 = C::H - (p - 1);
+    for (int j = e0 / kWave; j < C::NC; ++j) {
+      const int e = j * kWave + lane;
+      const double tp =
+          ((double)lds_f(s.h, e) + (double)lds_f(s.l, e) + (double)lds_f(s.c, e)) / 3.0;
+      lds_wd(s.d0, e, tp);
+    }
+    col_sync();
+    for (int j = C::NH; j < C::NC; ++j) {
+      const int e = j * kWave + lane;
+      double sum = 0.0;
+      for (int q = e; q > e - p; --q) sum += lds_d(s.d0, q);
+      const double sma = sum / p;
+      double dev = 0.0;
+      for (int q = e; q > e - p; --q) dev += fabs(lds_d(s.d0, q) - sma);
+      const double md = dev / p;
+      const double num = lds_d(s.d0, e) - sma;
+      const float v = (num != 0.0 && md != 0.0) ? (float)(num / (0.015 * md)) : 0.0f;
+      col_store<C>(io.out[kOCci], c, j, (base + e >= p - 1) ? v : qnanf_());
+    }
+  }
+}
+
+template <int H, int W>
+__global__ __launch_bounds__(W* kWave) void k_pack_win(Geometry g, PackP prm, PackIO io) {
+  using C = ColTile<H>;
+  __shared__ ColLds<C::E> s_lds[W];
+  const int w = threadIdx.x / kWave;
+  Seg c;
+  c.tree = false;
+  c.epoch = 0;
+  if (resolve(g, C::SEG, blockIdx.x * W + w, c)) pack_windows<C>(c, s_lds[w], prm, io);
+}
+
 template <int K, int H, int W, int PART>
 __global__ __launch_bounds__(W* kWave) void k_pack(Geometry g, Ws ws, int useLB, PackP prm,
                                                     PackIO io) {
@@ -667,20 +663,58 @@ constexpr unsigned kWinMask = (1u << kOSma) | (1u << kOWma) | (1u << kOBbU) | (1
 constexpr unsigned kHlcRecMask = (1u << kOAtr) | (1u << kOAdx);
 constexpr unsigned kCloseRecMask = ~(kWinMask | kHlcRecMask) & ((1u << kONum) - 1);
 
+// Longest window look-back the fused launch stages (bars before the segment).
+constexpr int kPackMaxWindow = 1024;
+
 #ifndef TCA_PACK_W
 #define TCA_PACK_W 4
 #endif
 
+// Halo of the window kernel: the longest look-back of the requested window
+// outputs, rounded up to whole 64-bar chunks.
+static int window_halo(const PackP& P) {
+  const unsigned m = P.mask;
+  int need = 0;
+  auto upd = [&](unsigned bits, int v) {
+    if ((m & bits) && v > need) need = v;
+  };
+  upd(1u << kOSma, P.smaP - 1);
+  upd(1u << kOWma, P.wmaP - 1);
+  upd((1u << kOBbU) | (1u << kOBbM) | (1u << kOBbL), P.bbP - 1);
+  upd((1u << kOStK) | (1u << kOStD), P.stK - 1 + 2 * (P.stD - 1));
+  upd(1u << kOWillr, P.willrP - 1);
+  upd(1u << kOCci, P.cciP - 1);
+  return need;
+}
+
+template <int H, int W>
+static ctStatus_t launch_win_h(int len, int rows, hipStream_t s, const PackP& prm,
+                               const PackIO& io) {
+  Geometry g = dense_geometry(len, rows, ColTile<H>::SEG);
+  const int blocks = div_up(g.totalSegs, W);
+  hipLaunchKernelGGL((k_pack_win<H, W>), dim3(blocks), dim3(W * kWave), 0, s, g, prm, io);
+  TCA_HIP(hipGetLastError());
+  return CT_STATUS_SUCCESS;
+}
+
+static ctStatus_t launch_windows(int len, int rows, hipStream_t s, const PackP& prm,
+                                 const PackIO& io) {
+  const int need = window_halo(prm);
+  if (need <= 64) return launch_win_h<64, 4>(len, rows, s, prm, io);
+  if (need <= 192) return launch_win_h<192, 2>(len, rows, s, prm, io);
+  if (need <= 512) return launch_win_h<512, 1>(len, rows, s, prm, io);
+  return launch_win_h<kPackMaxWindow, 1>(len, rows, s, prm, io);
+}
+
 template <int K>
-static ctStatus_t launch_pack(Geometry g, int maxSegsPerRow, hipStream_t s, const PackP& prm,
-                              const PackIO& io) {
+static ctStatus_t launch_pack(Geometry g, int maxSegsPerRow, int len, int rows, hipStream_t s,
+                              const PackP& prm, const PackIO& io) {
   if (g.totalSegs <= 0) return CT_STATUS_SUCCESS;
   constexpr int W = TCA_PACK_W;
   const int blocks = div_up(g.totalSegs, W);
   if (prm.mask & kWinMask) {
-    hipLaunchKernelGGL((k_pack<K, 64, W, 0>), dim3(blocks), dim3(W * kWave), 0, s, g,
-                       Ws{nullptr, nullptr, 1}, 0, prm, io);
-    TCA_HIP(hipGetLastError());
+    ctStatus_t st = launch_windows(len, rows, s, prm, io);
+    if (st != CT_STATUS_SUCCESS) return st;
   }
   const bool useLB = maxSegsPerRow > 1;
   // 3 levels x (segments + groups) slots of up to 5 (A, b) pairs
@@ -707,7 +741,7 @@ static ctStatus_t launch_pack(Geometry g, int maxSegsPerRow, hipStream_t s, cons
 // row), segments chained by the look-back tree for longer rows.
 ctStatus_t pack_dense(int len, int rows, hipStream_t s, const PackP& p, const PackIO& io) {
   Geometry g = dense_geometry(len, rows, 4 * kWave);
-  return launch_pack<4>(g, g.segsPerRow, s, p, io);
+  return launch_pack<4>(g, g.segsPerRow, len, rows, s, p, io);
 }
 
 }  // namespace tca
@@ -793,6 +827,9 @@ ctStatus_t pack_validate(const ctxPackParams* p, const ctxPackOutputs* o, PackP&
   }
   if ((P.mask & ((1u << kOStK) | (1u << kOStD))) && P.stD > kStochMaxD)
     return fail(CT_STATUS_UNSUPPORTED, "pack: STOCH dPeriod above %d is not provided", kStochMaxD);
+  if (window_halo(P) > kPackMaxWindow)
+    return fail(CT_STATUS_UNSUPPORTED, "pack: window look-back above %d bars is not provided",
+                kPackMaxWindow);
   if (!(P.mask & (1u << kORsi))) P.rsiP = 2;
   if (!(P.mask & (1u << kOAtr))) P.atrP = 2;
   if (!(P.mask & (1u << kOAdx))) P.adxP = 2;

@@ -152,7 +152,7 @@ def minus_dm(high, low, period: int = 14):
 PACK_DEFAULTS = dict(sma_period=20, ema_period=20, wma_period=20, dema_period=20, tema_period=20,
                      rsi_period=14, macd_fast=12, macd_slow=26, macd_signal=9, bb_period=20,
                      bb_up=2.0, bb_dn=2.0, atr_period=14, adx_period=14, stoch_k=14, stoch_d=3,
-                     willr_period=14, cci_period=20)
+                     willr_period=14, cci_period=14)
 
 
 def pack(high, low, close, outputs=PACK_OUTPUTS, **periods):

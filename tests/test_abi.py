@@ -116,5 +116,7 @@ def test_pack_arguments_are_validated_without_the_device():
     assert L.ctx_pack_batch_d(p, p, p, 8, 1, bad, ta.PackOutputs(rsi=p), None) == 4
     deep = ta.PackParams(**{**PACK_DEFAULTS, "stoch_d": 10})
     assert L.ctx_pack_batch_d(p, p, p, 8, 1, deep, ta.PackOutputs(stoch_k=p), None) == 6
+    wide = ta.PackParams(**{**PACK_DEFAULTS, "sma_period": 1026})
+    assert L.ctx_pack_batch_d(p, p, p, 8, 1, wide, ta.PackOutputs(sma=p), None) == 6
     # high/low are required only by the outputs that read them
     assert L.ctx_pack_batch_d(None, None, p, 8, 1, prm, ta.PackOutputs(atr=p), None) == 4

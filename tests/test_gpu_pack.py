@@ -30,7 +30,13 @@ LONG = dict(sma_period=200, ema_period=50, wma_period=100, dema_period=50, tema_
             bb_dn=2.5, atr_period=50, adx_period=30, stoch_k=50, stoch_d=9, willr_period=60,
             cci_period=40)
 PERIOD1 = dict(DEFAULT, ema_period=1, dema_period=1, tema_period=1)
-PARAMS = {"default": DEFAULT, "edge": EDGE, "long": LONG, "ema1": PERIOD1}
+# window look-backs that need 192- and 1024-bar halos in the window kernel
+MID = dict(DEFAULT, sma_period=150, wma_period=33, bb_period=120, stoch_k=100, stoch_d=5,
+           willr_period=90, cci_period=64)
+HUGE = dict(DEFAULT, sma_period=1000, wma_period=700, bb_period=600, stoch_k=900, stoch_d=9,
+            willr_period=1025, cci_period=513)
+PARAMS = {"default": DEFAULT, "edge": EDGE, "long": LONG, "ema1": PERIOD1, "mid": MID,
+          "huge": HUGE}
 
 
 def _want(R, p, h, l, c, name):
