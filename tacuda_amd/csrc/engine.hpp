@@ -354,6 +354,193 @@ __device__ __forceinline__ void level_carry(const Seg& c, int level, int Ls, con
   for (int r = 0; r < R; ++r) yin[r] = aff_apply(loc, r, ys[r]);
 }
 
+// ------------------------------------------------- seeded offset scans --
+// A recurrence y_t = beta*y_{t-1} + u_t that starts at a seed bar: bars
+// before the seed contribute nothing (offset 0), the seed bar sets y to the
+// seed, later bars apply the step.  The composite of any K-bar run of a
+// row is then y -> beta^K * y + b -- for the runs holding or preceding the
+// seed too, because everything before the seed is exactly 0 -- so scans and
+// look-back carry only the offsets b; the multipliers are known powers.
+// Rows stay independent: a segment's look-back reads only its own row's
+// slots (whole groups only once they lie inside the row).
+//
+// Slots hold (flag, b[0..R-1]).  Powers of nonzero factors that underflow
+// are clamped to the smallest normal double so a carried Inf or NaN
+// survives them as in the sequential recurrence; a zero factor stays zero
+// (0 * NaN is NaN, TA-Lib's literal fma).
+__device__ __forceinline__ double pow_clamp(double p, double beta) {
+  return (p == 0.0 && beta != 0.0) ? 2.2250738585072014e-308 : p;
+}
+
+// beta^n for a small n (n >= 0), by squaring; exact for n <= 2.
+__device__ __forceinline__ double pow_n(double beta, int n) {
+  double r = 1.0, b = beta;
+  while (n) {
+    if (n & 1) r = pow_clamp(r * b, beta);
+    b = pow_clamp(b * b, beta);
+    n >>= 1;
+  }
+  return r;
+}
+
+// Inclusive lane scan of (A, b) pairs (lane 0 earliest).  A starts as each
+// lane's multiplier; CLAMP guards products that may underflow.
+template <int R, bool CLAMP, int CTRL, int RM, int BM>
+__device__ __forceinline__ void bscan_step(double (&b)[R], double (&A)[R], const double (&beta)[R]) {
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const double As = dpp_d<CTRL, RM, BM>(1.0, A[r]);
+    const double bs = dpp_d<CTRL, RM, BM>(0.0, b[r]);
+    b[r] = fma(A[r], bs, b[r]);
+    A[r] = CLAMP ? pow_clamp(A[r] * As, beta[r]) : A[r] * As;
+  }
+}
+
+template <int R, bool CLAMP>
+__device__ __forceinline__ void bscan_incl(double (&b)[R], double (&A)[R], const double (&beta)[R]) {
+  bscan_step<R, CLAMP, kRowShr1, 0xf, 0xf>(b, A, beta);
+  bscan_step<R, CLAMP, kRowShr2, 0xf, 0xf>(b, A, beta);
+  bscan_step<R, CLAMP, kRowShr4, 0xf, 0xf>(b, A, beta);
+  bscan_step<R, CLAMP, kRowShr8, 0xf, 0xf>(b, A, beta);
+  bscan_step<R, CLAMP, kRowBcast15, 0xa, 0xf>(b, A, beta);
+  bscan_step<R, CLAMP, kRowBcast31, 0xc, 0xf>(b, A, beta);
+}
+
+template <int R>
+__device__ __forceinline__ void write_slot_b(const Seg& c, double* s, const double (&b)[R]) {
+  if (lane_id() == 0) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) st_rel_d(s + 1 + r, b[r]);
+    vmem_order();
+    __hip_atomic_store(reinterpret_cast<unsigned long long*>(s), (unsigned long long)c.epoch,
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// Lanes 0..cnt-1 load slots idx0.. (spinning on this launch's epoch); the
+// wave folds them oldest first with the per-slot multiplier P.  Returns the
+// folded offsets and P^cnt (both uniform).
+template <int R>
+__device__ __forceinline__ void gather_b(const Seg& c, int level, int idx0, int cnt, bool group,
+                                         const double (&P)[R], const double (&beta)[R],
+                                         double (&tb)[R], double (&tp)[R]) {
+  const int lane = lane_id();
+  const bool mine = lane < cnt;
+  const double* s = mine ? slot_ptr(c, level, idx0 + lane, group) : nullptr;
+  if (mine) {
+    while ((unsigned)ld_flag(s) != c.epoch) __builtin_amdgcn_s_sleep(1);
+  }
+  vmem_order();
+  double b[R], A[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    b[r] = mine ? ld_rel_d(s + 1 + r) : 0.0;
+    A[r] = mine ? P[r] : 1.0;
+  }
+  bscan_incl<R, true>(b, A, beta);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    tb[r] = bcast_d(b[r], 63);
+    tp[r] = bcast_d(A[r], 63);
+  }
+}
+
+// Offsets of this segment's row up to bar s0-1 (0 before the seeds).
+template <int R>
+__device__ __forceinline__ void lookback_b(const Seg& c, int level, const double (&Pseg)[R],
+                                           const double (&Pgrp)[R], const double (&beta)[R],
+                                           double (&y)[R]) {
+  const int first = c.gseg - c.seg;
+  const int G = c.gseg / kGroup;
+#pragma unroll
+  for (int r = 0; r < R; ++r) y[r] = 0.0;
+  double tb[R], tp[R];
+  auto fold = [&]() {
+#pragma unroll
+    for (int r = 0; r < R; ++r) y[r] = fma(tp[r], y[r], tb[r]);
+  };
+  // leading segments of a group the row shares with the previous row
+  const int hEnd = min(c.gseg, ((first + kGroup - 1) / kGroup) * kGroup);
+  if (first < hEnd) {
+    gather_b<R>(c, level, first, hEnd - first, false, Pseg, beta, tb, tp);
+    fold();
+  }
+  for (int g0 = hEnd / kGroup; g0 < G; g0 += kWave) {
+    gather_b<R>(c, level, g0, min(kWave, G - g0), true, Pgrp, beta, tb, tp);
+    fold();
+  }
+  const int i0 = max(hEnd, G * kGroup);
+  if (i0 < c.gseg) {
+    gather_b<R>(c, level, i0, c.gseg - i0, false, Pseg, beta, tb, tp);
+    fold();
+  }
+}
+
+// Sum over the wave of N per-lane values (lane order), result in every lane.
+template <int N>
+__device__ __forceinline__ void wave_total(double (&v)[N]) {
+#define TCA_ADD_STEP(CTRL, RM)                                         \
+  _Pragma("unroll") for (int r = 0; r < N; ++r) v[r] += dpp_d<CTRL, RM, 0xf>(0.0, v[r]);
+  TCA_ADD_STEP(kRowShr1, 0xf)
+  TCA_ADD_STEP(kRowShr2, 0xf)
+  TCA_ADD_STEP(kRowShr4, 0xf)
+  TCA_ADD_STEP(kRowShr8, 0xf)
+  TCA_ADD_STEP(kRowBcast15, 0xa)
+  TCA_ADD_STEP(kRowBcast31, 0xc)
+#undef TCA_ADD_STEP
+#pragma unroll
+  for (int r = 0; r < N; ++r) v[r] = bcast_d(v[r], 63);
+}
+
+// One level of seeded recurrences.  `b` holds each lane's K-bar offsets
+// (folded from 0), `P` = beta^K per recurrence.  Returns in `yin` the state
+// entering each lane's first bar.  Publishes this segment's offsets (and,
+// for the last segment of a full group, the group's) when the launch chains
+// segments.
+template <int R>
+__device__ __forceinline__ void bcarry(const Seg& c, int level, double (&b)[R],
+                                       const double (&P)[R], const double (&beta)[R],
+                                       double (&yin)[R]) {
+  double A[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) A[r] = P[r];
+  bscan_incl<R, false>(b, A, beta);
+  double bex[R], Aex[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    bex[r] = dpp_d<kWaveShr1, 0xf, 0xf>(0.0, b[r]);
+    Aex[r] = dpp_d<kWaveShr1, 0xf, 0xf>(1.0, A[r]);
+  }
+  if (!c.tree) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) yin[r] = bex[r];
+    return;
+  }
+  double tot[R], Pseg[R], Pgrp[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    tot[r] = bcast_d(b[r], 63);
+    Pseg[r] = pow_clamp(bcast_d(A[r], 63), beta[r]);
+    Pgrp[r] = pow_n(Pseg[r], kGroup);
+    Pgrp[r] = (Pgrp[r] == 0.0 && beta[r] != 0.0) ? 2.2250738585072014e-308 : Pgrp[r];
+  }
+  write_slot_b(c, slot_ptr(c, level, c.gseg, false), tot);
+  if ((c.gseg % kGroup) == kGroup - 1) {
+    double gb[R], gp[R];
+    gather_b<R>(c, level, c.gseg - (kGroup - 1), kGroup, false, Pseg, beta, gb, gp);
+    write_slot_b(c, slot_ptr(c, level, c.gseg / kGroup, true), gb);
+  }
+  if (c.seg == 0) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) yin[r] = bex[r];
+    return;
+  }
+  double y[R];
+  lookback_b<R>(c, level, Pseg, Pgrp, beta, y);
+#pragma unroll
+  for (int r = 0; r < R; ++r) yin[r] = fma(Aex[r], y[r], bex[r]);
+}
+
 // ------------------------------------------------------------ scheduling --
 // Ticket -> segment.  Dense rows: consecutive tickets walk a row's segments
 // in order; ragged rows use the precomputed first-segment table.

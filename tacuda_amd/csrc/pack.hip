@@ -42,420 +42,560 @@ struct PackIO {
 };
 
 
-// Per-bar affine step of a recurrence with warm-up [start, L) summed and the
-// seed divided at L.  Returns (A, b) for the composite; `steady` gives the
-// update after L.
-struct Phase {
-  int start, L;
-};
-
-__device__ __forceinline__ void warm_coef(const Phase& ph, int t, double u, double invP, double& A,
-                                          double& b) {
-  if (t == ph.start) {
-    A = -0.0;  // structural reset: a new recurrence starts here
-    b = (t == ph.L) ? u * invP : u;
-  } else if (t < ph.L) {
-    A = 1.0;
-    b = u;
-  } else {  // t == ph.L
-    A = invP;
-    b = u * invP;
-  }
-}
-
-// Replay of the warm-up part, TA-Lib's arithmetic: sum, then (sum+u)/p.
-__device__ __forceinline__ double warm_step(const Phase& ph, int t, double y, double u, int p) {
-  if (t == ph.start) return (t == ph.L) ? u / p : u;
-  if (t < ph.L) return y + u;
-  return (y + u) / p;
-}
-
-__device__ __forceinline__ double warm_step_inv(const Phase& ph, int t, double y, double u,
-                                                double inv) {
-  if (t == ph.start) return (t == ph.L) ? u * inv : u;
-  if (t < ph.L) return y + u;
-  return (y + u) * inv;
-}
-
 template <class T>
-__device__ void pack_store(const Seg& c, float* lo, const float (&v)[T::K], float* dst) {
+__device__ __forceinline__ void pack_store(const Seg& c, float* lo, const float (&v)[T::K], float* dst) {
   if (dst) store_lane<T>(lo, v, dst, c);
 }
 
-template <int R>
-__device__ __forceinline__ void pack_carry(const Seg& c, int level, Aff<R> loc, double (&yin)[R]) {
-  Aff<R> tot;
-  wave_scan_excl(loc, tot);
-  double ys[R];
-#pragma unroll
-  for (int r = 0; r < R; ++r) ys[r] = qnan_d();
-  if (c.tree) {
-    publish_tree(c, level, tot);
-    if (c.seg > 0) lookback<R>(c, level, ys);
-  }
-#pragma unroll
-  for (int r = 0; r < R; ++r) yin[r] = aff_apply(loc, r, ys[r]);
+// Seeds are TA-Lib's warm-up sequences, evaluated by every lane of the wave
+// that holds the seed bar in lockstep (same bar, broadcast reads), so all
+// lanes know them without a broadcast.  Bars before a seed contribute 0 to
+// the offset scans (bcarry), which keeps the scans free of reset logic.
+
+template <class T>
+__device__ __forceinline__ double seed_sum(const float* lds, const float* src, const Seg& c,
+                                           int t0, int t1) {
+  double s = 0.0;
+  for (int t = t0; t <= t1; ++t) s += (double)at<T>(lds, src, c, t);
+  return s;
 }
 
-// Recurrences.  PART 1 reads close only (EMA cascade, RSI, MACD); PART 2
-// reads high/low/close (ATR, ADX).  Separate launches keep each kernel's
-// registers low enough for several waves per SIMD.
-template <class T, int PART>
-__device__ __forceinline__ void pack_run(const Seg& c, float* li, float* lo, const PackP& P, const PackIO& io) {
+__device__ __forceinline__ bool here(const Seg& c, int L) { return L >= c.s0 && L < c.s1; }
+
+// Close recurrences: EMA and its DEMA/TEMA cascade, RSI, MACD.
+template <class T>
+__device__ __forceinline__ void pack_close(const Seg& c, float* li, float* lo, const PackP& P,
+                                           const PackIO& io) {
   constexpr int K = T::K;
-  float* lh = li;
-  float* ll = li + T::PLEN;
-  float* lc = li + 2 * T::PLEN;
-  if (PART == 2) {
-    stage<T>(lh, io.h, c);
-    stage<T>(ll, io.l, c);
-  }
+  const unsigned m = P.mask;
+  const bool wantE = m & ((1u << kOEma) | (1u << kODema) | (1u << kOTema));
+  const bool wantR = m & (1u << kORsi);
+  const bool wantM = m & ((1u << kOMacd) | (1u << kOMacdSig) | (1u << kOMacdHist));
+  const int pe = P.emaP;
+  const bool casc = (m & ((1u << kODema) | (1u << kOTema))) && pe > 1;
+  const bool wantT = (m & (1u << kOTema)) && pe > 1;
+  float* lc = li;
   stage<T>(lc, io.c, c);
   wave_sync();
-  float hv[K], lv[K], cv[K], ov[K];
-  if (PART == 2) {
-    lane_vals<T>(lh, hv);
-    lane_vals<T>(ll, lv);
-  }
+  float cv[K], ov[K];
   lane_vals<T>(lc, cv);
   const int a = c.s0 + lane_id() * K;
-  const unsigned m = P.mask;
-
-  // ------------------------------------------------------------ level 1 --
-  // PART 1: close recurrences (EMA cascade, RSI, MACD); PART 2: high/low/close
-  // recurrences (ATR, ADX).  Each shares one scan per level.
-  constexpr bool P1 = PART == 1;
-  constexpr int R1 = P1 ? 5 : 4;
-  constexpr int R2 = P1 ? 2 : 1;
-  // level-1 slot indices
-  constexpr int iEma = 0, iG = 1, iL = 2, iFast = 3, iSlow = 4;
-  constexpr int iAtr = 0, iPdm = 1, iMdm = 2, iTr = 3;
-  const bool wantE = P1 && (m & ((1u << kOEma) | (1u << kODema) | (1u << kOTema)));
-  const bool wantR = P1 && (m & (1u << kORsi));
-  const bool wantM = P1 && (m & ((1u << kOMacd) | (1u << kOMacdSig) | (1u << kOMacdHist)));
-  const bool wantA = !P1 && (m & (1u << kOAtr));
-  const bool wantX = !P1 && (m & (1u << kOAdx));
-  if (!(wantE || wantR || wantM || wantA || wantX)) return;
-  const int f = P1 ? first_valid(c, io.c) : 0;
-
-  const int pe = P.emaP;
-  const Phase phE{f, sat_add(f, pe - 1)};
-  const Phase phG{sat_add(f, 1), sat_add(f, P.rsiP)};
-  const Phase phS{f, sat_add(f, P.mSlow - 1)};
-  const Phase phF{sat_add(f, P.mSlow - P.mFast), sat_add(f, P.mSlow - 1)};
-  const Phase phA{1, P.atrP};
-  const Phase phX{1, P.adxP - 1};  // sums over bars 1..p-1, state is the sum
-  double eb, ek, fb, fk2, sb, sk2, gb, gk;
+  const float* x = io.c;
+  if (pe == 1) {
+    if (m & (1u << kOEma)) pack_store<T>(c, lo, cv, io.out[kOEma]);
+    if (m & (1u << kODema)) pack_store<T>(c, lo, cv, io.out[kODema]);
+    if (m & (1u << kOTema)) pack_store<T>(c, lo, cv, io.out[kOTema]);
+  }
+  const int f = first_valid(c, x);
+  const int pr = P.rsiP, slow = P.mSlow, fast = P.mFast, sig = P.mSig;
+  const int Le = sat_add(f, pe - 1), L2 = sat_add(Le, pe - 1), L3 = sat_add(L2, pe - 1);
+  const int Lg = sat_add(f, pr);
+  const int Lm = sat_add(f, slow - 1), Ls = sat_add(Lm, sig - 1);
+  double eb, ek, fb, fk, sb, sk, gb, gk;
   ema_coef(pe, eb, ek);
-  ema_coef(P.mFast, fb, fk2);
-  ema_coef(P.mSlow, sb, sk2);
-  ema_coef(P.mSig, gb, gk);
-  const double invE = 1.0 / pe, invR = 1.0 / (double)P.rsiP, pm1R = (double)(P.rsiP - 1);
-  const double invF = 1.0 / P.mFast, invS = 1.0 / P.mSlow;
-  const double wbA = (double)(P.atrP - 1) / (double)P.atrP, waA = 1.0 - wbA, invA = 1.0 / P.atrP;
-  const double ppX = (double)P.adxP, amX = 1.0 - 1.0 / ppX;
+  ema_coef(fast, fb, fk);
+  ema_coef(slow, sb, sk);
+  ema_coef(sig, gb, gk);
+  const double inv = 1.0 / (double)pr, pm1 = (double)(pr - 1);
 
-  // per-bar inputs: close deltas (PART 1) or TR and +DM/-DM (PART 2)
-  double u0[K], u1[K], u2[K];
+  // ---- level-1 seeds (ta_EMA.c, ta_RSI.c, ta_MACD.c warm-ups).  A window
+  // inside this segment is summed across the wave (exact for float inputs
+  // of ordinary range, where any order gives TA-Lib's sum); one that starts
+  // in an earlier segment is summed in TA-Lib's order from the inputs.
+  double sE = 0, sG = 0, sL = 0, sF = 0, sS = 0, s2 = 0, sSig = 0, s3 = 0;
+  double dlt[K];
   {
-    double pc = (double)at<T>(lc, io.c, c, a - 1);
-    double ph = P1 ? 0.0 : (double)at<T>(lh, io.h, c, a - 1);
-    double pl = P1 ? 0.0 : (double)at<T>(ll, io.l, c, a - 1);
+    float pc = at<T>(lc, x, c, a - 1);
 #pragma unroll
     for (int i = 0; i < K; ++i) {
-      if (P1) {
-        u0[i] = (double)cv[i] - pc;
-        u1[i] = u2[i] = 0.0;
-      } else {
-        u0[i] = true_range(hv[i], lv[i], pc);
-        dm_pair(ph, hv[i], pl, lv[i], u1[i], u2[i]);
-        ph = hv[i];
-        pl = lv[i];
-      }
+      dlt[i] = (double)cv[i] - (double)pc;
       pc = cv[i];
     }
   }
+  const bool hE = wantE && pe > 1 && here(c, Le), hR = wantR && here(c, Lg);
+  const bool hM = wantM && here(c, Lm);
+  const bool inE = hE && f >= c.s0, inR = hR && f >= c.s0, inM = hM && f >= c.s0;
+  if (inE || inR || inM) {
+    double part[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      const int t = a + i;
+      const double v = (double)cv[i];
+      if (inE && t >= f && t <= Le) part[0] += v;
+      if (inR && t > f && t <= Lg) {
+        const double gd = dlt[i] > 0.0 ? dlt[i] : 0.0;
+        part[1] += gd;
+        part[2] += gd - dlt[i];
+      }
+      if (inM && t >= f && t <= Lm) part[3] += v;
+      if (inM && t > Lm - fast && t <= Lm) part[4] += v;
+    }
+    wave_total<5>(part);
+    sE = part[0] / pe;
+    sG = part[1] * inv;
+    sL = part[2] * inv;
+    sS = part[3] / slow;
+    sF = part[4] / fast;
+  }
+  if (hE && !inE) sE = seed_sum<T>(lc, x, c, f, Le) / pe;
+  if (hR && !inR) {
+    double prev = (double)at<T>(lc, x, c, f), g = 0.0, l = 0.0;
+    for (int t = f + 1; t <= Lg; ++t) {
+      const double v = (double)at<T>(lc, x, c, t);
+      const double d = v - prev;
+      prev = v;
+      const double gd = d > 0.0 ? d : 0.0;
+      g += gd;
+      l += gd - d;
+    }
+    l *= inv;
+    g *= inv;
+    sG = g;
+    sL = l;
+  }
+  if (hM && !inM) {
+    sS = seed_sum<T>(lc, x, c, f, Lm) / slow;
+    sF = seed_sum<T>(lc, x, c, Lm - fast + 1, Lm) / fast;
+  }
 
-  Aff<R1> l1;
-  aff_id(l1);
-  auto push = [&](int r, double A, double b) {
-    l1.b[r] = fma_carry(A, l1.b[r], b);
-    l1.A[r] = is_reset(A) ? -0.0 : decay_mul(l1.A[r], A);
-  };
+  // ---- level 1: EMA, RSI gain / loss, MACD fast / slow
+  constexpr int R1 = 5;
+  const double beta1[R1] = {eb, pm1 * inv, pm1 * inv, fb, sb};
+  double P1[R1], b1[R1], y1[R1];
+#pragma unroll
+  for (int r = 0; r < R1; ++r) {
+    P1[r] = pow_n(beta1[r], K);
+    b1[r] = 0.0;
+  }
 #pragma unroll
   for (int i = 0; i < K; ++i) {
     const int t = a + i;
     if (t >= c.n) continue;
-    double A, b;
-    if (P1) {
-      const double x = (double)cv[i];
-      if (wantE && t >= phE.start) {
-        if (t <= phE.L) warm_coef(phE, t, x, invE, A, b);
-        else { A = eb; b = ek * x; }
-        push(iEma, A, b);
+    const double v = (double)cv[i];
+    if (wantE) {
+      if (t == Le) b1[0] = sE;
+      else if (t > Le) b1[0] = fma(eb, b1[0], ek * v);
+    }
+    if (wantR) {
+      if (t == Lg) {
+        b1[1] = sG;
+        b1[2] = sL;
+      } else if (t > Lg) {
+        const double gd = dlt[i] > 0.0 ? dlt[i] : 0.0;
+        b1[1] = (b1[1] * pm1 + gd) * inv;
+        b1[2] = (b1[2] * pm1 + (gd - dlt[i])) * inv;
       }
-      if (wantR && t >= phG.start) {
-        const double gd = u0[i] > 0.0 ? u0[i] : 0.0, ld = gd - u0[i];
-        if (t <= phG.L) {
-          warm_coef(phG, t, gd, invR, A, b);
-          push(iG, A, b);
-          warm_coef(phG, t, ld, invR, A, b);
-          push(iL, A, b);
-        } else {
-          l1.b[iG] = (l1.b[iG] * pm1R + gd) * invR;
-          l1.b[iL] = (l1.b[iL] * pm1R + ld) * invR;
-          l1.A[iG] = decay_mul(l1.A[iG], pm1R * invR);
-          l1.A[iL] = decay_mul(l1.A[iL], pm1R * invR);
-        }
-      }
-      if (wantM) {
-        if (t >= phS.start) {
-          if (t <= phS.L) warm_coef(phS, t, x, invS, A, b);
-          else { A = sb; b = sk2 * x; }
-          push(iSlow, A, b);
-        }
-        if (t >= phF.start) {
-          if (t <= phF.L) warm_coef(phF, t, x, invF, A, b);
-          else { A = fb; b = fk2 * x; }
-          push(iFast, A, b);
-        }
-      }
-    } else {
-      if (wantA && t >= phA.start) {
-        if (t <= phA.L) warm_coef(phA, t, u0[i], invA, A, b);
-        else { A = wbA; b = waA * u0[i]; }
-        push(iAtr, A, b);
-      }
-      if (wantX && t >= phX.start) {
-        const double u[3] = {u1[i], u2[i], u0[i]};
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-          const int r = iPdm + q;
-          if (t == phX.start) {
-            l1.b[r] = u[q];
-            l1.A[r] = -0.0;
-          } else if (t <= phX.L) {
-            l1.b[r] = l1.b[r] + u[q];
-          } else {
-            l1.b[r] = l1.b[r] - l1.b[r] / ppX + u[q];
-            l1.A[r] = is_reset(l1.A[r]) ? -0.0 : decay_mul(l1.A[r], amX);
-          }
-        }
+    }
+    if (wantM) {
+      if (t == Lm) {
+        b1[3] = sF;
+        b1[4] = sS;
+      } else if (t > Lm) {
+        b1[3] = fma(fb, b1[3], fk * v);
+        b1[4] = fma(sb, b1[4], sk * v);
       }
     }
   }
-  double y1[R1];
-  pack_carry<R1>(c, 0, l1, y1);
+  bcarry<R1>(c, 0, b1, P1, beta1, y1);
 
-  // replay level 1: e1 / macd line (PART 1) or DX (PART 2) per bar
-  double v1[K], v2[K];
+  double e1v[K], mv[K];
   {
-    float oa[K], ob[K];
+    float ob[K];
+    double ye = y1[0], yg = y1[1], yl = y1[2], yf = y1[3], ys = y1[4];
 #pragma unroll
     for (int i = 0; i < K; ++i) {
-      v1[i] = v2[i] = qnan_();
-      oa[i] = ob[i] = qnanf_();
-    }
-    if (P1) {
-      double ye = y1[iEma], yg = y1[iG], yl = y1[iL], yf = y1[iFast], ys = y1[iSlow];
-#pragma unroll
-      for (int i = 0; i < K; ++i) {
-        const int t = a + i;
-        const double x = (double)cv[i];
-        if (wantE && t >= phE.start) {
-          ye = (t <= phE.L) ? warm_step(phE, t, ye, x, pe) : fma(eb, ye, ek * x);
-          if (t >= phE.L) {
-            v1[i] = ye;
-            oa[i] = (float)ye;
-          }
-        }
-        if (pe == 1) oa[i] = cv[i];
-        if (wantR && t >= phG.start) {
-          const double gd = u0[i] > 0.0 ? u0[i] : 0.0;
-          if (t <= phG.L) {
-            yg = warm_step_inv(phG, t, yg, gd, invR);
-            yl = warm_step_inv(phG, t, yl, gd - u0[i], invR);
-          } else {
-            yl *= pm1R;
-            yg *= pm1R;
-            yg += gd;
-            yl += gd - u0[i];
-            yl *= invR;
-            yg *= invR;
-          }
-          if (t >= phG.L) {
-            const double s = yg + yl;
-            ob[i] = (float)(s > 0.0 ? 100.0 * (yg / s) : 50.0);
-          }
-        }
-        if (wantM) {
-          if (t >= phS.start) ys = (t <= phS.L) ? warm_step(phS, t, ys, x, P.mSlow) : fma(sb, ys, sk2 * x);
-          if (t >= phF.start) yf = (t <= phF.L) ? warm_step(phF, t, yf, x, P.mFast) : fma(fb, yf, fk2 * x);
-          if (t >= phS.L) v2[i] = yf - ys;
-        }
+      const int t = a + i;
+      const double v = (double)cv[i];
+      e1v[i] = mv[i] = qnan_();
+      ov[i] = ob[i] = qnanf_();
+      if (t == Le) ye = sE;
+      else if (t > Le) ye = fma(eb, ye, ek * v);
+      if (t >= Le) {
+        e1v[i] = ye;
+        ov[i] = (float)ye;
       }
-      if (m & (1u << kOEma)) pack_store<T>(c, lo, oa, io.out[kOEma]);
-      if (wantR) pack_store<T>(c, lo, ob, io.out[kORsi]);
-    } else {
-      double ya = y1[iAtr], yp = y1[iPdm], yq = y1[iMdm], yt = y1[iTr];
-#pragma unroll
-      for (int i = 0; i < K; ++i) {
-        const int t = a + i;
-        if (wantA && t >= phA.start) {
-          ya = (t <= phA.L) ? warm_step(phA, t, ya, u0[i], P.atrP) : fma(wbA, ya, waA * u0[i]);
-          if (t >= phA.L) oa[i] = (float)ya;
-        }
-        if (wantX && t >= phX.start) {
-          if (t == phX.start) {
-            yp = u1[i];
-            yq = u2[i];
-            yt = u0[i];
-          } else if (t <= phX.L) {
-            yp += u1[i];
-            yq += u2[i];
-            yt += u0[i];
-          } else {
-            yp = yp - yp / ppX + u1[i];
-            yq = yq - yq / ppX + u2[i];
-            yt = yt - yt / ppX + u0[i];
-          }
-          if (t > phX.L) {
-            double pdi, mdi, d;
-            dir_outputs(yp, yq, yt, pdi, mdi, d);
-            v1[i] = d;
-          }
-        }
+      if (t == Lg) {
+        yg = sG;
+        yl = sL;
+      } else if (t > Lg) {
+        const double gd = dlt[i] > 0.0 ? dlt[i] : 0.0;
+        yl *= pm1;
+        yg *= pm1;
+        yg += gd;
+        yl += gd - dlt[i];
+        yl *= inv;
+        yg *= inv;
       }
-      if (wantA) pack_store<T>(c, lo, oa, io.out[kOAtr]);
+      if (t >= Lg) {
+        const double s = yg + yl;
+        ob[i] = (float)(s > 0.0 ? 100.0 * (yg / s) : 50.0);
+      }
+      if (t == Lm) {
+        yf = sF;
+        ys = sS;
+      } else if (t > Lm) {
+        yf = fma(fb, yf, fk * v);
+        ys = fma(sb, ys, sk * v);
+      }
+      if (t >= Lm) mv[i] = yf - ys;
     }
+    if ((m & (1u << kOEma)) && pe > 1) pack_store<T>(c, lo, ov, io.out[kOEma]);
+    if (wantR) pack_store<T>(c, lo, ob, io.out[kORsi]);
   }
 
-  // ------------------------------------------------------------ level 2 --
-  // PART 1: DEMA/TEMA e2 (input e1) and the MACD signal (input macd line);
-  // PART 2: ADX (input DX).
-  const bool wantE2 = P1 && (m & ((1u << kODema) | (1u << kOTema))) && pe > 1;
-  if (P1 && (m & (1u << kODema)) && pe == 1) pack_store<T>(c, lo, cv, io.out[kODema]);
-  if (P1 && (m & (1u << kOTema)) && pe == 1) pack_store<T>(c, lo, cv, io.out[kOTema]);
-  if (!(wantE2 || wantM || wantX)) return;
-  const Phase phE2{phE.L, sat_add(phE.L, pe - 1)};
-  const Phase phSg{phS.L, sat_add(phS.L, P.mSig - 1)};
-  const Phase phAx{P.adxP, 2 * P.adxP - 1};
-  const double invSg = 1.0 / P.mSig;
-  constexpr int jE2 = 0, jSig = 1, jAdx = 0;
-  Aff<R2> l2;
-  aff_id(l2);
+  // ---- level 2: DEMA/TEMA e2 (input e1), MACD signal (input MACD line)
+  if (!(casc || wantM)) return;
+  // ---- level-2 seeds: mean of the first p e1 values (ta_DEMA.c) and of
+  // the first `signal` MACD values (ta_MACD.c), from this wave's level-1
+  // values when the window starts here, else TA-Lib's warm-up from inputs
+  {
+    const bool h2 = casc && here(c, L2), hS = wantM && here(c, Ls);
+    const bool in2 = h2 && Le >= c.s0, inS = hS && Lm >= c.s0;
+    if (in2 || inS) {
+      double part[2] = {0.0, 0.0};
+#pragma unroll
+      for (int i = 0; i < K; ++i) {
+        const int t = a + i;
+        if (in2 && t >= Le && t <= L2) part[0] += e1v[i];
+        if (inS && t >= Lm && t <= Ls) part[1] += mv[i];
+      }
+      wave_total<2>(part);
+      s2 = part[0] / pe;
+      sSig = part[1] / sig;
+    }
+    if (h2 && !in2) {
+      double e1 = seed_sum<T>(lc, x, c, f, Le) / pe;
+      double tmp = 0.0;
+      tmp += e1;
+      for (int t = Le + 1; t <= L2; ++t) {
+        e1 = fma(eb, e1, ek * (double)at<T>(lc, x, c, t));
+        tmp += e1;
+      }
+      s2 = tmp / pe;
+    }
+    if (hS && !inS) {
+      double ps = seed_sum<T>(lc, x, c, f, Lm) / slow;
+      double pf = seed_sum<T>(lc, x, c, Lm - fast + 1, Lm) / fast;
+      double psig = 0.0;
+      psig += pf - ps;
+      for (int t = Lm + 1; t <= Ls; ++t) {
+        const double v = (double)at<T>(lc, x, c, t);
+        pf = fma(fb, pf, fk * v);
+        ps = fma(sb, ps, sk * v);
+        psig += pf - ps;
+      }
+      sSig = psig / sig;
+    }
+  }
+  constexpr int R2 = 2;
+  const double beta2[R2] = {eb, gb};
+  double P2[R2], b2[R2] = {0.0, 0.0}, y2[R2];
+#pragma unroll
+  for (int r = 0; r < R2; ++r) P2[r] = pow_n(beta2[r], K);
 #pragma unroll
   for (int i = 0; i < K; ++i) {
     const int t = a + i;
     if (t >= c.n) continue;
-    double A, b;
-    if (P1) {
-      if (wantE2 && t >= phE2.start) {
-        if (t <= phE2.L) warm_coef(phE2, t, v1[i], invE, A, b);
-        else { A = eb; b = ek * v1[i]; }
-        l2.b[jE2] = fma_carry(A, l2.b[jE2], b);
-        l2.A[jE2] = is_reset(A) ? -0.0 : decay_mul(l2.A[jE2], A);
-      }
-      if (wantM && t >= phSg.start) {
-        if (t <= phSg.L) warm_coef(phSg, t, v2[i], invSg, A, b);
-        else if (P.mSig == 1) { A = -0.0; b = v2[i]; }
-        else { A = gb; b = gk * v2[i]; }
-        l2.b[jSig % R2] = fma_carry(A, l2.b[jSig % R2], b);
-        l2.A[jSig % R2] = is_reset(A) ? -0.0 : decay_mul(l2.A[jSig % R2], A);
-      }
-    } else if (t >= phAx.start) {
-      if (t <= phAx.L) {
-        warm_coef(phAx, t, v1[i], 1.0 / ppX, A, b);
-        l2.b[jAdx] = fma_carry(A, l2.b[jAdx], b);
-      } else {
-        A = (ppX - 1.0) / ppX;
-        l2.b[jAdx] = ((l2.b[jAdx] * (ppX - 1.0)) + v1[i]) / ppX;
-      }
-      l2.A[jAdx] = is_reset(A) ? -0.0 : decay_mul(l2.A[jAdx], A);
+    if (casc) {
+      if (t == L2) b2[0] = s2;
+      else if (t > L2) b2[0] = fma(eb, b2[0], ek * e1v[i]);
+    }
+    if (wantM) {
+      if (t == Ls) b2[1] = sSig;
+      else if (t > Ls) b2[1] = (sig == 1) ? mv[i] : fma(gb, b2[1], gk * mv[i]);
     }
   }
-  double y2[R2];
-  pack_carry<R2>(c, 1, l2, y2);
-  double e2[K];
+  bcarry<R2>(c, 1, b2, P2, beta2, y2);
+  double e2v[K];
   {
-    float od[K], om[K], os[K], oh[K];
+    float om[K], os[K], oh[K];
+    double ye = y2[0], yg = y2[1];
 #pragma unroll
     for (int i = 0; i < K; ++i) {
-      e2[i] = qnan_();
-      od[i] = om[i] = os[i] = oh[i] = qnanf_();
+      const int t = a + i;
+      e2v[i] = qnan_();
+      ov[i] = om[i] = os[i] = oh[i] = qnanf_();
+      if (t == L2) ye = s2;
+      else if (t > L2) ye = fma(eb, ye, ek * e1v[i]);
+      if (t >= L2) {
+        e2v[i] = ye;
+        ov[i] = (float)(2.0 * e1v[i] - ye);
+      }
+      if (t == Ls) yg = sSig;
+      else if (t > Ls) yg = (sig == 1) ? mv[i] : fma(gb, yg, gk * mv[i]);
+      if (t >= Ls) {
+        om[i] = (float)mv[i];
+        os[i] = (float)yg;
+        oh[i] = (float)(mv[i] - yg);
+      }
     }
-    if (P1) {
-      double ye = y2[jE2], yg = y2[jSig % R2];
-#pragma unroll
-      for (int i = 0; i < K; ++i) {
-        const int t = a + i;
-        if (wantE2 && t >= phE2.start) {
-          ye = (t <= phE2.L) ? warm_step(phE2, t, ye, v1[i], pe) : fma(eb, ye, ek * v1[i]);
-          if (t >= phE2.L) {
-            e2[i] = ye;
-            od[i] = (float)(2.0 * v1[i] - ye);
-          }
-        }
-        if (wantM && t >= phSg.start) {
-          if (t <= phSg.L) yg = warm_step(phSg, t, yg, v2[i], P.mSig);
-          else yg = (P.mSig == 1) ? v2[i] : fma(gb, yg, gk * v2[i]);
-          if (t >= phSg.L) {
-            om[i] = (float)v2[i];
-            os[i] = (float)yg;
-            oh[i] = (float)(v2[i] - yg);
-          }
-        }
-      }
-      if ((m & (1u << kODema)) && pe > 1) pack_store<T>(c, lo, od, io.out[kODema]);
-      if (wantM) {
-        pack_store<T>(c, lo, om, io.out[kOMacd]);
-        pack_store<T>(c, lo, os, io.out[kOMacdSig]);
-        pack_store<T>(c, lo, oh, io.out[kOMacdHist]);
-      }
-    } else {
-      double yx = y2[jAdx];
-#pragma unroll
-      for (int i = 0; i < K; ++i) {
-        const int t = a + i;
-        if (t >= phAx.start) {
-          yx = (t <= phAx.L) ? warm_step(phAx, t, yx, v1[i], P.adxP)
-                             : ((yx * (ppX - 1.0)) + v1[i]) / ppX;
-          if (t >= phAx.L) od[i] = (float)yx;
-        }
-      }
-      pack_store<T>(c, lo, od, io.out[kOAdx]);
+    if ((m & (1u << kODema)) && pe > 1) pack_store<T>(c, lo, ov, io.out[kODema]);
+    if (wantM) {
+      pack_store<T>(c, lo, om, io.out[kOMacd]);
+      pack_store<T>(c, lo, os, io.out[kOMacdSig]);
+      pack_store<T>(c, lo, oh, io.out[kOMacdHist]);
     }
   }
 
-  // ------------------------------------------------------------ level 3 --
-  if (!P1 || !(m & (1u << kOTema)) || pe == 1) return;
-  const Phase phE3{phE2.L, sat_add(phE2.L, pe - 1)};
-  Aff<1> l3;
-  aff_id(l3);
+  // ---- level 3: TEMA e3 (input e2)
+  if (!wantT) return;
+  // ---- level-3 seed: mean of the first p e2 values (ta_TEMA.c)
+  {
+    const bool h3 = here(c, L3), in3 = h3 && L2 >= c.s0;
+    if (in3) {
+      double part[1] = {0.0};
+#pragma unroll
+      for (int i = 0; i < K; ++i) {
+        const int t = a + i;
+        if (t >= L2 && t <= L3) part[0] += e2v[i];
+      }
+      wave_total<1>(part);
+      s3 = part[0] / pe;
+    }
+    if (h3 && !in3) {
+      double e1 = seed_sum<T>(lc, x, c, f, Le) / pe;
+      double tmp = 0.0;
+      tmp += e1;
+      for (int t = Le + 1; t <= L2; ++t) {
+        e1 = fma(eb, e1, ek * (double)at<T>(lc, x, c, t));
+        tmp += e1;
+      }
+      double e2 = tmp / pe;
+      tmp = 0.0;
+      tmp += e2;
+      for (int t = L2 + 1; t <= L3; ++t) {
+        e1 = fma(eb, e1, ek * (double)at<T>(lc, x, c, t));
+        e2 = fma(eb, e2, ek * e1);
+        tmp += e2;
+      }
+      s3 = tmp / pe;
+    }
+  }
+  const double beta3[1] = {eb};
+  double P3[1] = {pow_n(eb, K)}, b3[1] = {0.0}, y3[1];
 #pragma unroll
   for (int i = 0; i < K; ++i) {
     const int t = a + i;
-    if (t >= c.n || t < phE3.start) continue;
-    double A, b;
-    if (t <= phE3.L) warm_coef(phE3, t, e2[i], invE, A, b);
-    else { A = eb; b = ek * e2[i]; }
-    l3.b[0] = fma_carry(A, l3.b[0], b);
-    l3.A[0] = is_reset(A) ? -0.0 : decay_mul(l3.A[0], A);
+    if (t >= c.n) continue;
+    if (t == L3) b3[0] = s3;
+    else if (t > L3) b3[0] = fma(eb, b3[0], ek * e2v[i]);
   }
-  double y3[1];
-  pack_carry<1>(c, 2, l3, y3);
+  bcarry<1>(c, 2, b3, P3, beta3, y3);
   {
     double ye = y3[0];
 #pragma unroll
     for (int i = 0; i < K; ++i) {
       const int t = a + i;
       ov[i] = qnanf_();
-      if (t >= phE3.start) {
-        ye = (t <= phE3.L) ? warm_step(phE3, t, ye, e2[i], pe) : fma(eb, ye, ek * e2[i]);
-        if (t >= phE3.L) ov[i] = (float)(ye + (3.0 * v1[i] - 3.0 * e2[i]));
-      }
+      if (t == L3) ye = s3;
+      else if (t > L3) ye = fma(eb, ye, ek * e2v[i]);
+      if (t >= L3) ov[i] = (float)(ye + (3.0 * e1v[i] - 3.0 * e2v[i]));
     }
     pack_store<T>(c, lo, ov, io.out[kOTema]);
+  }
+}
+
+// High/low/close recurrences: ATR (ta_ATR.c) and ADX (Wilder sums of +DM,
+// -DM and TR, DX, Wilder mean of DX; the reference ADX.cu/DX.cu semantics).
+template <class T>
+__device__ __forceinline__ void pack_hlc(const Seg& c, float* li, float* lo, const PackP& P,
+                                         const PackIO& io) {
+  constexpr int K = T::K;
+  const unsigned m = P.mask;
+  const bool wantA = m & (1u << kOAtr);
+  const bool wantX = m & (1u << kOAdx);
+  float* lh = li;
+  float* ll = li + T::PLEN;
+  float* lc = li + 2 * T::PLEN;
+  stage<T>(lh, io.h, c);
+  stage<T>(ll, io.l, c);
+  stage<T>(lc, io.c, c);
+  wave_sync();
+  float hv[K], lv[K], cv[K], ov[K];
+  lane_vals<T>(lh, hv);
+  lane_vals<T>(ll, lv);
+  lane_vals<T>(lc, cv);
+  const int a = c.s0 + lane_id() * K;
+  const int pa = P.atrP, px = P.adxP;
+  const int La = pa, Lx = px - 1, Lax = 2 * px - 1;
+  const double wb = (double)(pa - 1) / (double)pa, wa = 1.0 - wb;
+  const double pp = (double)px, am = 1.0 - 1.0 / pp, bx = (pp - 1.0) / pp;
+  auto tr_at = [&](int t) {
+    return true_range(at<T>(lh, io.h, c, t), at<T>(ll, io.l, c, t), at<T>(lc, io.c, c, t - 1));
+  };
+  auto dm_at = [&](int t, double& pd, double& md) {
+    dm_pair(at<T>(lh, io.h, c, t - 1), at<T>(lh, io.h, c, t), at<T>(ll, io.l, c, t - 1),
+            at<T>(ll, io.l, c, t), pd, md);
+  };
+
+  // ---- level 1: ATR, +DM, -DM, TR sums
+  double tr[K], pdm[K], mdm[K];
+  {
+    double ph = at<T>(lh, io.h, c, a - 1), pl = at<T>(ll, io.l, c, a - 1),
+           pc = at<T>(lc, io.c, c, a - 1);
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      tr[i] = true_range(hv[i], lv[i], pc);
+      dm_pair(ph, hv[i], pl, lv[i], pdm[i], mdm[i]);
+      ph = hv[i];
+      pl = lv[i];
+      pc = cv[i];
+    }
+  }
+  // ---- level-1 seeds: mean TR over bars 1..p (ta_ATR.c) and the Wilder
+  // sums of +DM, -DM, TR over bars 1..p-1
+  double sA = 0, sP = 0, sM = 0, sT = 0, sX = 0;
+  const bool hA = wantA && here(c, La), hX = wantX && here(c, Lx);
+  const bool inA = hA && c.s0 <= 1, inX = hX && c.s0 <= 1;
+  if (inA || inX) {
+    double part[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      const int t = a + i;
+      if (inA && t >= 1 && t <= La) part[0] += tr[i];
+      if (inX && t >= 1 && t <= Lx) {
+        part[1] += pdm[i];
+        part[2] += mdm[i];
+        part[3] += tr[i];
+      }
+    }
+    wave_total<4>(part);
+    sA = part[0] / pa;
+    sP = part[1];
+    sM = part[2];
+    sT = part[3];
+  }
+  if (hA && !inA) {
+    double tot = 0.0;
+    for (int t = 1; t <= pa; ++t) tot += tr_at(t);
+    sA = tot / pa;
+  }
+  if (hX && !inX) {
+    double sp = 0.0, sm = 0.0, st = 0.0;
+    for (int t = 1; t <= px - 1; ++t) {
+      double pd, md;
+      dm_at(t, pd, md);
+      sp += pd;
+      sm += md;
+      st += tr_at(t);
+    }
+    sP = sp;
+    sM = sm;
+    sT = st;
+  }
+  constexpr int R1 = 4;
+  const double beta1[R1] = {wb, am, am, am};
+  double P1[R1], b1[R1], y1[R1];
+#pragma unroll
+  for (int r = 0; r < R1; ++r) {
+    P1[r] = pow_n(beta1[r], K);
+    b1[r] = 0.0;
+  }
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    const int t = a + i;
+    if (t >= c.n) continue;
+    if (wantA) {
+      if (t == La) b1[0] = sA;
+      else if (t > La) b1[0] = fma(wb, b1[0], wa * tr[i]);
+    }
+    if (wantX) {
+      if (t == Lx) {
+        b1[1] = sP;
+        b1[2] = sM;
+        b1[3] = sT;
+      } else if (t > Lx) {
+        b1[1] = b1[1] - b1[1] / pp + pdm[i];
+        b1[2] = b1[2] - b1[2] / pp + mdm[i];
+        b1[3] = b1[3] - b1[3] / pp + tr[i];
+      }
+    }
+  }
+  bcarry<R1>(c, 0, b1, P1, beta1, y1);
+  double dxv[K];
+  {
+    double ya = y1[0], yp = y1[1], yq = y1[2], yt = y1[3];
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      const int t = a + i;
+      ov[i] = qnanf_();
+      dxv[i] = qnan_();
+      if (t == La) ya = sA;
+      else if (t > La) ya = fma(wb, ya, wa * tr[i]);
+      if (t >= La) ov[i] = (float)ya;
+      if (t == Lx) {
+        yp = sP;
+        yq = sM;
+        yt = sT;
+      } else if (t > Lx) {
+        yp = yp - yp / pp + pdm[i];
+        yq = yq - yq / pp + mdm[i];
+        yt = yt - yt / pp + tr[i];
+        double pdi, mdi, dx;
+        dir_outputs(yp, yq, yt, pdi, mdi, dx);
+        dxv[i] = dx;
+      }
+    }
+    if (wantA) pack_store<T>(c, lo, ov, io.out[kOAtr]);
+  }
+
+  // ---- level 2: ADX (input DX), seeded by the mean of DX over bars p..2p-1
+  if (!wantX) return;
+  {
+    const bool hAx = here(c, Lax), inAx = hAx && px >= c.s0;
+    if (inAx) {
+      double part[1] = {0.0};
+#pragma unroll
+      for (int i = 0; i < K; ++i) {
+        const int t = a + i;
+        if (t >= px && t <= Lax) part[0] += dxv[i];
+      }
+      wave_total<1>(part);
+      sX = part[0] / pp;
+    }
+    if (hAx && !inAx) {
+      double sp = 0.0, sm = 0.0, st = 0.0, sum = 0.0;
+      for (int t = 1; t <= px - 1; ++t) {
+        double pd, md;
+        dm_at(t, pd, md);
+        sp += pd;
+        sm += md;
+        st += tr_at(t);
+      }
+      for (int t = px; t <= Lax; ++t) {
+        double pd, md;
+        dm_at(t, pd, md);
+        sp = sp - sp / pp + pd;
+        sm = sm - sm / pp + md;
+        st = st - st / pp + tr_at(t);
+        double pdi, mdi, dx;
+        dir_outputs(sp, sm, st, pdi, mdi, dx);
+        sum += dx;
+      }
+      sX = sum / pp;
+    }
+  }
+  const double beta2[1] = {bx};
+  double P2[1] = {pow_n(bx, K)}, b2[1] = {0.0}, y2[1];
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    const int t = a + i;
+    if (t >= c.n) continue;
+    if (t == Lax) b2[0] = sX;
+    else if (t > Lax) b2[0] = ((b2[0] * (pp - 1.0)) + dxv[i]) / pp;
+  }
+  bcarry<1>(c, 1, b2, P2, beta2, y2);
+  {
+    double yx = y2[0];
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      const int t = a + i;
+      ov[i] = qnanf_();
+      if (t == Lax) yx = sX;
+      else if (t > Lax) yx = ((yx * (pp - 1.0)) + dxv[i]) / pp;
+      if (t >= Lax) ov[i] = (float)yx;
+    }
+    pack_store<T>(c, lo, ov, io.out[kOAdx]);
   }
 }
 
@@ -648,7 +788,10 @@ __global__ __launch_bounds__(W* kWave) void k_pack(Geometry g, Ws ws, int useLB,
   c.epoch = useLB ? read_epoch(ws) : 0u;
   if (resolve(g, T::SEG, blockIdx.x * W + w, c)) {
     c.ws = ws;
-    pack_run<T, PART>(c, s_in[w], s_out[w], prm, io);
+    if (PART == 1)
+      pack_close<T>(c, s_in[w], s_out[w], prm, io);
+    else
+      pack_hlc<T>(c, s_in[w], s_out[w], prm, io);
   }
   if (useLB) {
     __syncthreads();
@@ -717,10 +860,10 @@ static ctStatus_t launch_pack(Geometry g, int maxSegsPerRow, int len, int rows, 
     if (st != CT_STATUS_SUCCESS) return st;
   }
   const bool useLB = maxSegsPerRow > 1;
-  // 3 levels x (segments + groups) slots of up to 5 (A, b) pairs
-  Ws ws{nullptr, nullptr, 2 * 5 + 1};
+  // 3 levels x (segments + groups) slots of a flag and up to 5 offsets
+  Ws ws{nullptr, nullptr, 1 + 5};
   if (useLB && (prm.mask & ~kWinMask)) {
-    const size_t bytes = (size_t)3 * ((size_t)g.totalSegs + g.totalSegs / kGroup) * (2 * 5 + 1) * 8;
+    const size_t bytes = (size_t)3 * ((size_t)g.totalSegs + g.totalSegs / kGroup) * (1 + 5) * 8;
     ctStatus_t st = workspace_for(s, bytes, &ws);
     if (st != CT_STATUS_SUCCESS) return st;
   }
