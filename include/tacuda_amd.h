@@ -64,6 +64,93 @@ CTAPI_EXPORT ctStatus_t ctx_pack_batch_d(const float* d_high, const float* d_low
                                          const ctxPackParams* params,
                                          const ctxPackOutputs* outputs, hipStream_t stream);
 
+/* ---------------------------------------------------------------------------
+ * Ragged batches: series r occupies elements [offsets[r], offsets[r+1]) of
+ * concatenated input and output arrays (offsets are int64, host memory,
+ * numSeries+1 entries, non-decreasing; empty series are allowed).  A layout
+ * is built once (it uploads its tables synchronously) and reused by any
+ * number of calls, including calls captured into a hipGraph.  Each series
+ * gets exactly the result of its single-series ct_*_d call.
+ * ------------------------------------------------------------------------- */
+typedef struct ctxRagged_st* ctxRagged;
+CTAPI_EXPORT ctStatus_t ctx_ragged_create(const long long* offsets, int numSeries,
+                                          ctxRagged* layout);
+CTAPI_EXPORT ctStatus_t ctx_ragged_destroy(ctxRagged layout);
+
+CTAPI_EXPORT ctStatus_t ctx_pack_ragged_d(const float* d_high, const float* d_low,
+                                          const float* d_close, ctxRagged layout,
+                                          const ctxPackParams* params,
+                                          const ctxPackOutputs* outputs, hipStream_t stream);
+
+/* Single-family ragged calls; arguments as the family's ct_*_d call. */
+#define CTX_RAGGED1(name) \
+  CTAPI_EXPORT ctStatus_t ctx_##name##_ragged_d(const float* in, float* out, ctxRagged layout, \
+                                                int period, hipStream_t stream);
+CTX_RAGGED1(sma)
+CTX_RAGGED1(ema)
+CTX_RAGGED1(wma)
+CTX_RAGGED1(dema)
+CTX_RAGGED1(tema)
+CTX_RAGGED1(rsi)
+CTX_RAGGED1(var)
+CTX_RAGGED1(stddev)
+CTX_RAGGED1(sum)
+CTX_RAGGED1(max)
+CTX_RAGGED1(min)
+#undef CTX_RAGGED1
+#define CTX_RAGGED_HLC(name)                                                              \
+  CTAPI_EXPORT ctStatus_t ctx_##name##_ragged_d(const float* high, const float* low,     \
+                                                const float* close, float* out,           \
+                                                ctxRagged layout, int period, hipStream_t stream);
+CTX_RAGGED_HLC(atr)
+CTX_RAGGED_HLC(natr)
+CTX_RAGGED_HLC(adx)
+CTX_RAGGED_HLC(dx)
+CTX_RAGGED_HLC(plus_di)
+CTX_RAGGED_HLC(minus_di)
+CTX_RAGGED_HLC(willr)
+CTX_RAGGED_HLC(cci)
+#undef CTX_RAGGED_HLC
+CTAPI_EXPORT ctStatus_t ctx_t3_ragged_d(const float* in, float* out, ctxRagged layout,
+                                        int period, float vFactor, hipStream_t stream);
+CTAPI_EXPORT ctStatus_t ctx_kama_ragged_d(const float* in, float* out, ctxRagged layout,
+                                          int period, int fastPeriod, int slowPeriod,
+                                          hipStream_t stream);
+CTAPI_EXPORT ctStatus_t ctx_macd_ragged_d(const float* in, float* macd, float* signal,
+                                          float* hist, ctxRagged layout, int fastPeriod,
+                                          int slowPeriod, int signalPeriod, hipStream_t stream);
+CTAPI_EXPORT ctStatus_t ctx_bbands_ragged_d(const float* in, float* upper, float* middle,
+                                            float* lower, ctxRagged layout, int period,
+                                            float nbDevUp, float nbDevDn, hipStream_t stream);
+CTAPI_EXPORT ctStatus_t ctx_stochastic_ragged_d(const float* high, const float* low,
+                                                const float* close, float* k, float* d,
+                                                ctxRagged layout, int kPeriod, int dPeriod,
+                                                hipStream_t stream);
+CTAPI_EXPORT ctStatus_t ctx_trange_ragged_d(const float* high, const float* low,
+                                            const float* close, float* out, ctxRagged layout,
+                                            hipStream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * Persistent tuned plans: one fused pack pass over fixed device buffers
+ * (dense seriesLen x numSeries when `layout` is NULL, else the ragged
+ * layout), captured into a hipGraph.  With `tune` set, creation times every
+ * tile configuration (bars per lane of the scan, segments per workgroup) on
+ * these buffers and keeps the fastest; the choice is cached per shape for
+ * the process.  Creation runs the pass several times (outputs are
+ * overwritten).  A plan's launches must be serialized (they share the
+ * plan's look-back workspace); stream 0 blocks until the pass is done.
+ * ------------------------------------------------------------------------- */
+typedef struct ctxPlan_st* ctxPlan;
+CTAPI_EXPORT ctStatus_t ctx_pack_plan_create(const float* d_high, const float* d_low,
+                                             const float* d_close, int seriesLen, int numSeries,
+                                             ctxRagged layout, const ctxPackParams* params,
+                                             const ctxPackOutputs* outputs, int tune,
+                                             ctxPlan* plan);
+CTAPI_EXPORT ctStatus_t ctx_plan_launch(ctxPlan plan, hipStream_t stream);
+CTAPI_EXPORT ctStatus_t ctx_plan_info(ctxPlan plan, int* barsPerLane, int* wavesPerGroup,
+                                      float* tunedUs, float* defaultUs);
+CTAPI_EXPORT ctStatus_t ctx_plan_destroy(ctxPlan plan);
+
 #ifdef __cplusplus
 }
 #endif

@@ -103,6 +103,28 @@ class PackOutputs(ctypes.Structure):
 
 _sig("ctx_pack_batch_d", P, P, P, c_int, c_int, ctypes.POINTER(PackParams),
      ctypes.POINTER(PackOutputs))
+# ragged layouts (include/tacuda_amd.h)
+_sig("ctx_ragged_create", P, c_int, ctypes.POINTER(c_void_p), stream=False)
+_sig("ctx_ragged_destroy", c_void_p, stream=False)
+_sig("ctx_pack_ragged_d", P, P, P, c_void_p, ctypes.POINTER(PackParams),
+     ctypes.POINTER(PackOutputs))
+for _n in ("sma", "ema", "wma", "dema", "tema", "rsi", "var", "stddev", "sum", "max", "min"):
+    _sig(f"ctx_{_n}_ragged_d", P, P, c_void_p, c_int)
+for _n in ("atr", "natr", "adx", "dx", "plus_di", "minus_di", "willr", "cci"):
+    _sig(f"ctx_{_n}_ragged_d", P, P, P, P, c_void_p, c_int)
+_sig("ctx_t3_ragged_d", P, P, c_void_p, c_int, c_float)
+_sig("ctx_kama_ragged_d", P, P, c_void_p, c_int, c_int, c_int)
+_sig("ctx_macd_ragged_d", P, P, P, P, c_void_p, c_int, c_int, c_int)
+_sig("ctx_bbands_ragged_d", P, P, P, P, c_void_p, c_int, c_float, c_float)
+_sig("ctx_stochastic_ragged_d", P, P, P, P, P, c_void_p, c_int, c_int)
+_sig("ctx_trange_ragged_d", P, P, P, P, c_void_p)
+# persistent tuned plans (include/tacuda_amd.h)
+_sig("ctx_pack_plan_create", P, P, P, c_int, c_int, c_void_p, ctypes.POINTER(PackParams),
+     ctypes.POINTER(PackOutputs), c_int, ctypes.POINTER(c_void_p), stream=False)
+_sig("ctx_plan_launch", c_void_p, c_void_p, stream=False)
+_sig("ctx_plan_info", c_void_p, ctypes.POINTER(c_int), ctypes.POINTER(c_int),
+     ctypes.POINTER(c_float), ctypes.POINTER(c_float), stream=False)
+_sig("ctx_plan_destroy", c_void_p, stream=False)
 
 
 class TacudaError(RuntimeError):

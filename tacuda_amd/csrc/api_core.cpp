@@ -24,6 +24,24 @@ std::map<std::pair<int, hipStream_t>, WsEntry> g_ws;
 size_t g_ws_allocs = 0;
 }  // namespace
 
+thread_local const Ragged* t_ragged = nullptr;
+
+bool ragged_geometry(const Ragged* r, int seg, Geometry* g, int* maxSegsPerRow) {
+  for (int i = 0; i < 3; ++i) {
+    if (kRaggedSegSizes[i] != seg) continue;
+    g->offsets = r->dOff;
+    g->segStart = r->dSegStart[i];
+    g->ld = 0;
+    g->len = r->maxLen;
+    g->rows = r->rows;
+    g->segsPerRow = 0;
+    g->totalSegs = r->totalSegs[i];
+    *maxSegsPerRow = r->maxSegs[i];
+    return true;
+  }
+  return false;
+}
+
 void clear_error() { g_error.clear(); }
 
 ctStatus_t fail(ctStatus_t st, const char* fmt, ...) {
@@ -79,6 +97,19 @@ ctStatus_t workspace_for(hipStream_t stream, size_t treeBytes, Ws* out) {
   out->hdr = static_cast<WsHeader*>(w.base);
   out->area = reinterpret_cast<double*>(static_cast<char*>(w.base) + sizeof(WsHeader));
   return CT_STATUS_SUCCESS;
+}
+
+void workspace_release(hipStream_t stream) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return;
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  auto it = g_ws.find({dev, stream});
+  if (it == g_ws.end()) return;
+  if (it->second.base) {
+    (void)hipStreamSynchronize(stream);
+    (void)hipFree(it->second.base);
+  }
+  g_ws.erase(it);
 }
 
 void workspace_release_all() {

@@ -10,8 +10,9 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <vector>
 
-#include "../../include/tacuda.h"
+#include "../../include/tacuda_amd.h"
 #include "engine.hpp"
 
 namespace tca {
@@ -49,6 +50,7 @@ ctStatus_t guarded(F&& body) {
 // while the stream is being captured (the graph would keep a stale pointer).
 ctStatus_t workspace_for(hipStream_t stream, size_t treeBytes, Ws* out);
 void workspace_release_all();
+void workspace_release(hipStream_t stream);
 size_t workspace_allocations();
 
 // ------------------------------------------------------------ geometry --
@@ -65,6 +67,34 @@ inline Geometry dense_geometry(int len, int rows, int seg) {
   g.totalSegs = g.segsPerRow * rows;
   return g;
 }
+
+// ------------------------------------------------------------- ragged --
+// Layout of a ragged batch: series r occupies [off[r], off[r+1]) of the
+// concatenated arrays.  Built once on the host; keeps the device copy of the
+// offsets and, per segment size, the first global segment of every row.
+constexpr int kRaggedSegSizes[3] = {256, 512, 1024};
+struct Ragged {
+  int rows = 0;
+  int maxLen = 0;
+  long long total = 0;
+  long long* dOff = nullptr;             // device, rows + 1
+  int* dSegStart[3] = {nullptr, nullptr, nullptr};  // device, rows + 1
+  int totalSegs[3] = {0, 0, 0};
+  int maxSegs[3] = {0, 0, 0};
+};
+
+// Set for the duration of a ragged entry point: the launchers build ragged
+// geometry from it instead of the dense (len, rows) shape.
+extern thread_local const Ragged* t_ragged;
+struct RaggedScope {
+  explicit RaggedScope(const Ragged* r) { t_ragged = r; }
+  ~RaggedScope() { t_ragged = nullptr; }
+};
+
+// Geometry of the active ragged layout for a segment size; false when the
+// size has no table (never for the tile sizes the kernels use).
+bool ragged_geometry(const Ragged* r, int seg, Geometry* g, int* maxSegsPerRow);
+const Ragged* ragged_of(struct ctxRagged_st* layout);
 
 // Synchronize when the caller used the legacy default stream: batch calls
 // on stream 0 return once the batch is done (reference api_batch.cu:377-386).

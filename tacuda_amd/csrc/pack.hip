@@ -15,6 +15,10 @@
 // level 2 (DEMA/TEMA e2, MACD signal, ADX) and level 3 (TEMA e3) follow.
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
+#include <tuple>
+
 #include "../../include/tacuda_amd.h"
 #include "families.hpp"
 #include "host.hpp"
@@ -32,6 +36,11 @@ struct PackP {
   int smaP, emaP, wmaP, rsiP, mFast, mSlow, mSig, bbP, atrP, adxP, stK, stD, willrP, cciP;
   double bbUp, bbDn;
   unsigned mask;  // bit i = output i requested
+};
+
+struct PackCfg {
+  int K;  // bars per lane of the recurrence kernels
+  int W;  // segments (waves) per workgroup
 };
 
 struct PackIO {
@@ -809,9 +818,6 @@ constexpr unsigned kCloseRecMask = ~(kWinMask | kHlcRecMask) & ((1u << kONum) - 
 // Longest window look-back the fused launch stages (bars before the segment).
 constexpr int kPackMaxWindow = 1024;
 
-#ifndef TCA_PACK_W
-#define TCA_PACK_W 4
-#endif
 
 // Halo of the window kernel: the longest look-back of the requested window
 // outputs, rounded up to whole 64-bar chunks.
@@ -834,6 +840,10 @@ template <int H, int W>
 static ctStatus_t launch_win_h(int len, int rows, hipStream_t s, const PackP& prm,
                                const PackIO& io) {
   Geometry g = dense_geometry(len, rows, ColTile<H>::SEG);
+  int maxSegs = 0;
+  if (t_ragged && !ragged_geometry(t_ragged, ColTile<H>::SEG, &g, &maxSegs))
+    return fail(CT_STATUS_UNSUPPORTED, "pack: no ragged table for the window tile");
+  if (g.totalSegs <= 0) return CT_STATUS_SUCCESS;
   const int blocks = div_up(g.totalSegs, W);
   hipLaunchKernelGGL((k_pack_win<H, W>), dim3(blocks), dim3(W * kWave), 0, s, g, prm, io);
   TCA_HIP(hipGetLastError());
@@ -849,20 +859,22 @@ static ctStatus_t launch_windows(int len, int rows, hipStream_t s, const PackP& 
   return launch_win_h<kPackMaxWindow, 1>(len, rows, s, prm, io);
 }
 
-template <int K>
-static ctStatus_t launch_pack(Geometry g, int maxSegsPerRow, int len, int rows, hipStream_t s,
-                              const PackP& prm, const PackIO& io) {
+// Recurrence kernels of one tile configuration: K bars per lane (the scan
+// chunk), W waves (segments) per workgroup.
+template <int K, int W>
+static ctStatus_t launch_rec(int len, int rows, hipStream_t s, const PackP& prm,
+                             const PackIO& io) {
+  constexpr int SEG = K * kWave;
+  Geometry g = dense_geometry(len, rows, SEG);
+  int maxSegs = g.segsPerRow;
+  if (t_ragged && !ragged_geometry(t_ragged, SEG, &g, &maxSegs))
+    return fail(CT_STATUS_UNSUPPORTED, "pack: no ragged table for the recurrence tile");
   if (g.totalSegs <= 0) return CT_STATUS_SUCCESS;
-  constexpr int W = TCA_PACK_W;
   const int blocks = div_up(g.totalSegs, W);
-  if (prm.mask & kWinMask) {
-    ctStatus_t st = launch_windows(len, rows, s, prm, io);
-    if (st != CT_STATUS_SUCCESS) return st;
-  }
-  const bool useLB = maxSegsPerRow > 1;
+  const bool useLB = maxSegs > 1;
   // 3 levels x (segments + groups) slots of a flag and up to 5 offsets
   Ws ws{nullptr, nullptr, 1 + 5};
-  if (useLB && (prm.mask & ~kWinMask)) {
+  if (useLB) {
     const size_t bytes = (size_t)3 * ((size_t)g.totalSegs + g.totalSegs / kGroup) * (1 + 5) * 8;
     ctStatus_t st = workspace_for(s, bytes, &ws);
     if (st != CT_STATUS_SUCCESS) return st;
@@ -880,11 +892,33 @@ static ctStatus_t launch_pack(Geometry g, int maxSegsPerRow, int len, int rows, 
   return CT_STATUS_SUCCESS;
 }
 
-// Dense or single-series pack: 4 bars per lane (one wave covers a 256-bar
-// row), segments chained by the look-back tree for longer rows.
+// Tile configurations the autotuner chooses from.
+const PackCfg kPackCfgs[] = {{4, 4}, {4, 2}, {4, 1}, {4, 8}, {8, 2}, {8, 1}, {8, 4}};
+
+ctStatus_t pack_launch(int len, int rows, hipStream_t s, const PackP& p, const PackIO& io,
+                       PackCfg cfg) {
+  if (p.mask & kWinMask) {
+    ctStatus_t st = launch_windows(len, rows, s, p, io);
+    if (st != CT_STATUS_SUCCESS) return st;
+  }
+  if (!(p.mask & ~kWinMask)) return CT_STATUS_SUCCESS;
+#define TCA_CFG(KK, WW) \
+  if (cfg.K == KK && cfg.W == WW) return launch_rec<KK, WW>(len, rows, s, p, io);
+  TCA_CFG(4, 4)
+  TCA_CFG(4, 2)
+  TCA_CFG(4, 1)
+  TCA_CFG(4, 8)
+  TCA_CFG(8, 2)
+  TCA_CFG(8, 1)
+  TCA_CFG(8, 4)
+#undef TCA_CFG
+  return fail(CT_STATUS_INVALID_ARGUMENT, "pack: unknown tile configuration %dx%d", cfg.K, cfg.W);
+}
+
+// Dense or single-series pack with the default tile (4 bars per lane, one
+// wave covers a 256-bar row; longer rows chain segments by look-back).
 ctStatus_t pack_dense(int len, int rows, hipStream_t s, const PackP& p, const PackIO& io) {
-  Geometry g = dense_geometry(len, rows, 4 * kWave);
-  return launch_pack<4>(g, g.segsPerRow, len, rows, s, p, io);
+  return pack_launch(len, rows, s, p, io, PackCfg{4, 4});
 }
 
 }  // namespace tca
@@ -996,6 +1030,193 @@ CTAPI_EXPORT ctStatus_t ctx_pack_batch_d(const float* h, const float* l, const f
     st = pack_dense(seriesLen, numSeries, s, P, io);
     if (st != CT_STATUS_SUCCESS) return st;
     return finish_batch(s);
+  });
+}
+
+CTAPI_EXPORT ctStatus_t ctx_pack_ragged_d(const float* h, const float* l, const float* c,
+                                          ctxRagged layout, const ctxPackParams* p,
+                                          const ctxPackOutputs* o, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    const Ragged* r = ragged_of(layout);
+    if (!r) return fail(CT_STATUS_INVALID_ARGUMENT, "pack: null layout");
+    PackP P;
+    PackIO io;
+    ctStatus_t st = pack_validate(p, o, P, io, h, l, c);
+    if (st != CT_STATUS_SUCCESS) return st;
+    RaggedScope scope(r);
+    st = pack_dense(r->maxLen, r->rows, s, P, io);
+    if (st != CT_STATUS_SUCCESS) return st;
+    return finish_batch(s);
+  });
+}
+
+}  // extern "C"
+
+// ================================================================== plans ==
+// A plan is one fused pass over fixed buffers, tuned once and captured into
+// a hipGraph: later passes are a single graph launch.
+struct ctxPlan_st {
+  hipStream_t stream = nullptr;  // private stream the graph was captured on
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  PackCfg cfg{4, 4};
+  float tunedUs = 0.0f, defaultUs = 0.0f;
+};
+
+namespace {
+
+// Tuned tile per shape class, so a second plan of the same shape skips the
+// measurement.
+struct TuneKey {
+  int device, rows, len, ragged;
+  unsigned mask;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(device, rows, len, ragged, mask) <
+           std::tie(o.device, o.rows, o.len, o.ragged, o.mask);
+  }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, std::pair<PackCfg, std::pair<float, float>>> g_tuned;
+
+// Average microseconds of `reps` passes with tile `cfg` on `st`.
+ctStatus_t time_cfg(int len, int rows, hipStream_t st, const PackP& P, const PackIO& io,
+                    PackCfg cfg, int reps, float* us) {
+  for (int i = 0; i < 2; ++i) {
+    ctStatus_t r = pack_launch(len, rows, st, P, io, cfg);
+    if (r != CT_STATUS_SUCCESS) return r;
+  }
+  hipEvent_t e0, e1;
+  TCA_HIP(hipEventCreate(&e0));
+  TCA_HIP(hipEventCreate(&e1));
+  ctStatus_t r = CT_STATUS_SUCCESS;
+  (void)hipEventRecord(e0, st);
+  for (int i = 0; i < reps && r == CT_STATUS_SUCCESS; ++i) r = pack_launch(len, rows, st, P, io, cfg);
+  (void)hipEventRecord(e1, st);
+  hipError_t e = hipEventSynchronize(e1);
+  float ms = 0.0f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (r != CT_STATUS_SUCCESS) return r;
+  if (e != hipSuccess) return hip_fail(e, "plan: timing");
+  *us = ms * 1000.0f / reps;
+  return CT_STATUS_SUCCESS;
+}
+
+void plan_free(ctxPlan p) {
+  if (!p) return;
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  if (p->stream) {
+    workspace_release(p->stream);
+    (void)hipStreamDestroy(p->stream);
+  }
+  delete p;
+}
+
+}  // namespace
+
+extern "C" {
+
+CTAPI_EXPORT ctStatus_t ctx_pack_plan_create(const float* h, const float* l, const float* c,
+                                             int seriesLen, int numSeries, ctxRagged layout,
+                                             const ctxPackParams* params,
+                                             const ctxPackOutputs* outputs, int tune,
+                                             ctxPlan* out) {
+  return guarded([&]() -> ctStatus_t {
+    if (!out) return fail(CT_STATUS_INVALID_ARGUMENT, "plan: null output handle");
+    *out = nullptr;
+    const Ragged* rg = ragged_of(layout);
+    if (!rg && (seriesLen <= 0 || numSeries <= 0))
+      return fail(CT_STATUS_INVALID_ARGUMENT, "plan: seriesLen and numSeries must be positive");
+    PackP P;
+    PackIO io;
+    ctStatus_t st = pack_validate(params, outputs, P, io, h, l, c);
+    if (st != CT_STATUS_SUCCESS) return st;
+    const int len = rg ? rg->maxLen : seriesLen, rows = rg ? rg->rows : numSeries;
+    RaggedScope scope(rg);
+    ctxPlan p = new ctxPlan_st;
+    auto bail = [&](ctStatus_t r) {
+      plan_free(p);
+      return r;
+    };
+    hipError_t e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return bail(hip_fail(e, "plan: stream"));
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    const TuneKey key{dev, rows, rg ? (int)(rg->total & 0x7fffffff) : len, rg ? 1 : 0, P.mask};
+    bool cached = false;
+    {
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      auto it = g_tuned.find(key);
+      if (it != g_tuned.end()) {
+        p->cfg = it->second.first;
+        p->tunedUs = it->second.second.first;
+        p->defaultUs = it->second.second.second;
+        cached = true;
+      }
+    }
+    // every candidate runs eagerly before capture: each reserves the
+    // look-back workspace its tile needs on the plan's stream
+    if (tune && !cached) {
+      float best = 0.0f;
+      for (const PackCfg& cf : kPackCfgs) {
+        float us = 0.0f;
+        ctStatus_t r = time_cfg(len, rows, p->stream, P, io, cf, 10, &us);
+        if (r != CT_STATUS_SUCCESS) return bail(r);
+        if (cf.K == 4 && cf.W == 4) p->defaultUs = us;
+        if (best == 0.0f || us < best) {
+          best = us;
+          p->cfg = cf;
+        }
+      }
+      p->tunedUs = best;
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      g_tuned[key] = {p->cfg, {p->tunedUs, p->defaultUs}};
+    } else {
+      float us = 0.0f;
+      ctStatus_t r = time_cfg(len, rows, p->stream, P, io, p->cfg, 1, &us);
+      if (r != CT_STATUS_SUCCESS) return bail(r);
+      if (!cached) p->tunedUs = p->defaultUs = us;
+    }
+    e = hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return bail(hip_fail(e, "plan: begin capture"));
+    ctStatus_t r = pack_launch(len, rows, p->stream, P, io, p->cfg);
+    e = hipStreamEndCapture(p->stream, &p->graph);
+    if (r != CT_STATUS_SUCCESS) return bail(r);
+    if (e != hipSuccess) return bail(hip_fail(e, "plan: end capture"));
+    e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) return bail(hip_fail(e, "plan: instantiate"));
+    *out = p;
+    return CT_STATUS_SUCCESS;
+  });
+}
+
+CTAPI_EXPORT ctStatus_t ctx_plan_launch(ctxPlan plan, hipStream_t stream) {
+  return guarded([&]() -> ctStatus_t {
+    if (!plan || !plan->exec) return fail(CT_STATUS_INVALID_ARGUMENT, "plan: null plan");
+    TCA_HIP(hipGraphLaunch(plan->exec, stream));
+    return finish_batch(stream);
+  });
+}
+
+CTAPI_EXPORT ctStatus_t ctx_plan_info(ctxPlan plan, int* barsPerLane, int* wavesPerGroup,
+                                      float* tunedUs, float* defaultUs) {
+  return guarded([&]() -> ctStatus_t {
+    if (!plan) return fail(CT_STATUS_INVALID_ARGUMENT, "plan: null plan");
+    if (barsPerLane) *barsPerLane = plan->cfg.K;
+    if (wavesPerGroup) *wavesPerGroup = plan->cfg.W;
+    if (tunedUs) *tunedUs = plan->tunedUs;
+    if (defaultUs) *defaultUs = plan->defaultUs;
+    return CT_STATUS_SUCCESS;
+  });
+}
+
+CTAPI_EXPORT ctStatus_t ctx_plan_destroy(ctxPlan plan) {
+  return guarded([&]() -> ctStatus_t {
+    plan_free(plan);
+    return CT_STATUS_SUCCESS;
   });
 }
 

@@ -62,6 +62,19 @@ static ctStatus_t launch_geom(Geometry g, int maxSegsPerRow, hipStream_t s, type
 // longer rows use the family's long tile and chain segments by look-back.
 template <class F, int KL, int HL, int KS, int HS, int NIN, int W>
 static ctStatus_t launch_dense(int len, int rows, hipStream_t s, typename F::P prm, IO io) {
+  if (t_ragged) {
+    // ragged batch: the short tile when every series fits one wave
+    Geometry g;
+    int maxSegs = 0;
+    if (t_ragged->maxLen <= KS * kWave) {
+      if (!ragged_geometry(t_ragged, KS * kWave, &g, &maxSegs))
+        return fail(CT_STATUS_UNSUPPORTED, "ragged: no table for this tile");
+      return launch_geom<F, KS, HS, NIN, W>(g, maxSegs, s, prm, io);
+    }
+    if (!ragged_geometry(t_ragged, KL * kWave, &g, &maxSegs))
+      return fail(CT_STATUS_UNSUPPORTED, "ragged: no table for this tile");
+    return launch_geom<F, KL, HL, NIN, W>(g, maxSegs, s, prm, io);
+  }
   if (len <= KS * kWave) {
     Geometry g = dense_geometry(len, rows, KS * kWave);
     return launch_geom<F, KS, HS, NIN, W>(g, g.segsPerRow, s, prm, io);
@@ -771,4 +784,85 @@ CTAPI_EXPORT ctStatus_t ct_minus_dm_batch(const float* hh, const float* hl, floa
   });
 }
 
+}  // extern "C"
+
+// ------------------------------------------------------------- ragged --
+namespace {
+template <class Fn>
+ctStatus_t ragged_call(ctxRagged layout, const char* fn, Fn&& body) {
+  return guarded([&]() -> ctStatus_t {
+    const Ragged* r = ragged_of(layout);
+    if (!r) return bad(fn, "null layout");
+    RaggedScope scope(r);
+    return body(r->maxLen, r->rows);
+  });
+}
+}  // namespace
+
+extern "C" {
+#define RAG1(name)                                                                            \
+  CTAPI_EXPORT ctStatus_t ctx_##name##_ragged_d(const float* x, float* y, ctxRagged lay, int p, \
+                                                hipStream_t s) {                              \
+    return ragged_call(lay, #name, [&](int n, int rows) { return dev_##name(x, y, n, rows, p, s); }); \
+  }
+RAG1(sma)
+RAG1(ema)
+RAG1(wma)
+RAG1(dema)
+RAG1(tema)
+RAG1(rsi)
+RAG1(var)
+RAG1(stddev)
+RAG1(sum)
+RAG1(max)
+RAG1(min)
+#undef RAG1
+#define RAGHLC(name)                                                                         \
+  CTAPI_EXPORT ctStatus_t ctx_##name##_ragged_d(const float* h, const float* l, const float* c, \
+                                                float* y, ctxRagged lay, int p, hipStream_t s) { \
+    return ragged_call(lay, #name,                                                          \
+                       [&](int n, int rows) { return dev_##name(h, l, c, y, n, rows, p, s); }); \
+  }
+RAGHLC(atr)
+RAGHLC(natr)
+RAGHLC(adx)
+RAGHLC(dx)
+RAGHLC(plus_di)
+RAGHLC(minus_di)
+RAGHLC(willr)
+RAGHLC(cci)
+#undef RAGHLC
+CTAPI_EXPORT ctStatus_t ctx_t3_ragged_d(const float* x, float* y, ctxRagged lay, int p, float v,
+                                        hipStream_t s) {
+  return ragged_call(lay, "t3", [&](int n, int rows) { return dev_t3(x, y, n, rows, p, v, s); });
+}
+CTAPI_EXPORT ctStatus_t ctx_kama_ragged_d(const float* x, float* y, ctxRagged lay, int p, int fast,
+                                          int slow, hipStream_t s) {
+  return ragged_call(lay, "kama",
+                     [&](int n, int rows) { return dev_kama(x, y, n, rows, p, fast, slow, s); });
+}
+CTAPI_EXPORT ctStatus_t ctx_macd_ragged_d(const float* x, float* m, float* sg, float* h,
+                                          ctxRagged lay, int fast, int slow, int sig,
+                                          hipStream_t s) {
+  return ragged_call(lay, "macd", [&](int n, int rows) {
+    return dev_macd(x, m, sg, h, n, rows, fast, slow, sig, CT_MA_EMA, s);
+  });
+}
+CTAPI_EXPORT ctStatus_t ctx_bbands_ragged_d(const float* x, float* u, float* m, float* l,
+                                            ctxRagged lay, int p, float up, float dn,
+                                            hipStream_t s) {
+  return ragged_call(lay, "bbands",
+                     [&](int n, int rows) { return dev_bbands(x, u, m, l, n, rows, p, up, dn, s); });
+}
+CTAPI_EXPORT ctStatus_t ctx_stochastic_ragged_d(const float* h, const float* l, const float* c,
+                                                float* k, float* d, ctxRagged lay, int kp, int dp,
+                                                hipStream_t s) {
+  return ragged_call(lay, "stochastic",
+                     [&](int n, int rows) { return dev_stoch(h, l, c, k, d, n, rows, kp, dp, s); });
+}
+CTAPI_EXPORT ctStatus_t ctx_trange_ragged_d(const float* h, const float* l, const float* c,
+                                            float* y, ctxRagged lay, hipStream_t s) {
+  return ragged_call(lay, "trange",
+                     [&](int n, int rows) { return dev_trange(h, l, c, y, n, rows, s); });
+}
 }  // extern "C"

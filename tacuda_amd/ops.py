@@ -176,3 +176,148 @@ def pack(high, low, close, outputs=PACK_OUTPUTS, **periods):
     call("ctx_pack_batch_d", high.data_ptr(), low.data_ptr(), close.data_ptr(), n, rows, prm, outs,
          _stream())
     return res
+
+
+class Ragged:
+    """Ragged batch layout (ctx_ragged_create): series r occupies
+    [offsets[r], offsets[r+1]) of 1-D concatenated tensors."""
+
+    def __init__(self, offsets):
+        import ctypes
+
+        import numpy as np
+
+        from . import lib
+
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        self.offsets = off
+        self.rows = len(off) - 1
+        h = ctypes.c_void_p()
+        call("ctx_ragged_create", off.ctypes.data, self.rows, ctypes.byref(h))
+        self._h = h
+        self._lib = lib()
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            self._lib.ctx_ragged_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def one(self, name, x, *params):
+        """Single-input family (sma, ema, ..., t3, kama) over the layout."""
+        _check(x)
+        y = torch.empty_like(x)
+        call(f"ctx_{name}_ragged_d", x.data_ptr(), y.data_ptr(), self._h, *params, _stream())
+        return y
+
+    def hlc(self, name, high, low, close, *params):
+        _check(high, low, close)
+        y = torch.empty_like(close)
+        call(f"ctx_{name}_ragged_d", high.data_ptr(), low.data_ptr(), close.data_ptr(),
+             y.data_ptr(), self._h, *params, _stream())
+        return y
+
+    def macd(self, x, fast=12, slow=26, signal=9):
+        _check(x)
+        m, s, h = (torch.empty_like(x) for _ in range(3))
+        call("ctx_macd_ragged_d", x.data_ptr(), m.data_ptr(), s.data_ptr(), h.data_ptr(), self._h,
+             fast, slow, signal, _stream())
+        return m, s, h
+
+    def bbands(self, x, period=20, up=2.0, dn=2.0):
+        _check(x)
+        u, m, l = (torch.empty_like(x) for _ in range(3))
+        call("ctx_bbands_ragged_d", x.data_ptr(), u.data_ptr(), m.data_ptr(), l.data_ptr(),
+             self._h, period, up, dn, _stream())
+        return u, m, l
+
+    def stochastic(self, high, low, close, k_period=14, d_period=3):
+        _check(high, low, close)
+        k, d = torch.empty_like(close), torch.empty_like(close)
+        call("ctx_stochastic_ragged_d", high.data_ptr(), low.data_ptr(), close.data_ptr(),
+             k.data_ptr(), d.data_ptr(), self._h, k_period, d_period, _stream())
+        return k, d
+
+    def trange(self, high, low, close):
+        _check(high, low, close)
+        y = torch.empty_like(close)
+        call("ctx_trange_ragged_d", high.data_ptr(), low.data_ptr(), close.data_ptr(),
+             y.data_ptr(), self._h, _stream())
+        return y
+
+    def pack(self, high, low, close, outputs=PACK_OUTPUTS, **periods):
+        """The fused multi-indicator launch over the layout (ctx_pack_ragged_d)."""
+        _check(close)
+        if high is None:
+            high = low = close
+        _check(high, low)
+        prm = PackParams(**{**PACK_DEFAULTS, **periods})
+        res = {k: torch.empty_like(close) for k in outputs}
+        outs = PackOutputs(**{k: v.data_ptr() for k, v in res.items()})
+        call("ctx_pack_ragged_d", high.data_ptr(), low.data_ptr(), close.data_ptr(), self._h, prm,
+             outs, _stream())
+        return res
+
+
+class Plan:
+    """A tuned, graph-captured fused pass over fixed buffers
+    (ctx_pack_plan_create).  Output tensors are allocated once and rewritten
+    by every launch(); inputs are read from the given tensors at launch time."""
+
+    def __init__(self, high, low, close, outputs=PACK_OUTPUTS, layout=None, tune=True,
+                 **periods):
+        import ctypes
+
+        from . import lib
+
+        _check(close)
+        if high is None:
+            high = low = close
+        _check(high, low)
+        self.inputs = (high, low, close)
+        self.outputs = {k: torch.empty_like(close) for k in outputs}
+        prm = PackParams(**{**PACK_DEFAULTS, **periods})
+        outs = PackOutputs(**{k: v.data_ptr() for k, v in self.outputs.items()})
+        n, rows = _shape(close) if layout is None else (0, 0)
+        h = ctypes.c_void_p()
+        torch.cuda.current_stream().synchronize()
+        call("ctx_pack_plan_create", high.data_ptr(), low.data_ptr(), close.data_ptr(), n, rows,
+             layout.handle if layout is not None else None, prm, outs, int(tune),
+             ctypes.byref(h))
+        self._h = h
+        self._lib = lib()
+        self.layout = layout
+
+    def launch(self):
+        call("ctx_plan_launch", self._h, _stream())
+        return self.outputs
+
+    def info(self):
+        import ctypes
+
+        k, w = ctypes.c_int(), ctypes.c_int()
+        tu, du = ctypes.c_float(), ctypes.c_float()
+        call("ctx_plan_info", self._h, ctypes.byref(k), ctypes.byref(w), ctypes.byref(tu),
+             ctypes.byref(du))
+        return {"bars_per_lane": k.value, "waves_per_group": w.value, "tuned_us": tu.value,
+                "default_us": du.value}
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            self._lib.ctx_plan_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

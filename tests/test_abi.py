@@ -18,7 +18,8 @@ def _declared():
     for h in HEADERS:
         text = open(h).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-        text = "\n".join(l for l in text.splitlines() if not l.lstrip().startswith("#define"))
+        # drop macro definitions, continuation lines included
+        text = re.sub(r"^\s*#define[^\n]*(?:\\\n[^\n]*)*", "", text, flags=re.M)
         # plain declarations
         for m in re.finditer(r"\b(ct_\w+|ctx_\w+)\s*\(", text):
             names.add(m.group(1))
@@ -27,6 +28,9 @@ def _declared():
             names.update({f"ct_{m.group(1)}_batch", f"ct_{m.group(1)}_batch_d"})
         for m in re.finditer(r"CT_BATCH_HLC\((\w+)\)", text):
             names.update({f"ct_{m.group(1)}_batch", f"ct_{m.group(1)}_batch_d"})
+        # ragged macros: CTX_RAGGED1(name) and CTX_RAGGED_HLC(name)
+        for m in re.finditer(r"CTX_RAGGED(?:1|_HLC)\((\w+)\)", text):
+            names.add(f"ctx_{m.group(1)}_ragged_d")
     names.discard("ct_")
     return sorted(n for n in names if not n.endswith("_") and "##" not in n)
 
@@ -120,3 +124,22 @@ def test_pack_arguments_are_validated_without_the_device():
     assert L.ctx_pack_batch_d(p, p, p, 8, 1, wide, ta.PackOutputs(sma=p), None) == 6
     # high/low are required only by the outputs that read them
     assert L.ctx_pack_batch_d(None, None, p, 8, 1, prm, ta.PackOutputs(atr=p), None) == 4
+
+
+def test_ragged_layout_arguments_are_validated_without_the_device():
+    import ctypes
+
+    import tacuda_amd as ta
+
+    L = ta.lib()
+    h = ctypes.c_void_p()
+    off = np.array([0, 10, 5], dtype=np.int64)
+    assert L.ctx_ragged_create(off.ctypes.data, 2, ctypes.byref(h)) == 4
+    assert b"decrease" in L.ct_get_last_error()
+    assert L.ctx_ragged_create(off.ctypes.data, 0, ctypes.byref(h)) == 4
+    assert L.ctx_ragged_create(None, 2, ctypes.byref(h)) == 4
+    empty = np.array([0, 0, 0], dtype=np.int64)
+    assert L.ctx_ragged_create(empty.ctypes.data, 2, ctypes.byref(h)) == 4
+    assert L.ctx_ragged_destroy(None) == 0
+    buf = np.zeros(4, np.float32)
+    assert L.ctx_ema_ragged_d(buf.ctypes.data, buf.ctypes.data, None, 5, None) == 4

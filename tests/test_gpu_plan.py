@@ -1,0 +1,91 @@
+"""Persistent tuned plans (ctx_pack_plan_create): the autotuner's choice is
+one of the tile configurations, graph replays equal each other bit for bit
+and the oracle within tolerance, with eager work and input rewrites between
+replays (the graph reads the live buffers)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _check_all(got, d, params):
+    from oracle import ORC
+    from tests.parity import compare
+    from tests.test_gpu_pack import EXACT, _want
+
+    h, l, c = d["high"], d["low"], d["close"]
+    for name, t in got.items():
+        g = t.cpu().numpy()
+        if c.ndim == 1:
+            w = _want(ORC, params, h, l, c, name)
+        else:
+            w = np.stack([_want(ORC, params, h[r], l[r], c[r], name) for r in range(c.shape[0])])
+        if name in EXACT:
+            compare(g, w, tol=0.0, min_exact=1.0, name=name)
+        else:
+            compare(g, w, tol=1e-5, min_exact=0.999, name=name)
+
+
+@pytest.mark.parametrize("shape", [(4000, 234), (1, 200000), (8, 30000)])
+def test_plan_replays_match_oracle(dev, shape):
+    import tacuda_amd as ta
+    from tests.test_gpu_pack import DEFAULT
+
+    rows, n = shape
+    print(f"plan {rows}x{n}", flush=True)
+    d = ta.synth_ohlcv(n, rows=rows, seed=41)
+    T = [torch.from_numpy(np.ascontiguousarray(d[k])).to(dev) for k in ("high", "low", "close")]
+    plan = ta.ops.Plan(*T, tune=True, **DEFAULT)
+    info = plan.info()
+    assert (info["bars_per_lane"], info["waves_per_group"]) in {
+        (4, 4), (4, 2), (4, 1), (4, 8), (8, 2), (8, 1), (8, 4)}
+    assert info["tuned_us"] > 0 and info["tuned_us"] <= info["default_us"] * 1.0001
+    first = {k: v.clone() for k, v in plan.launch().items()}
+    torch.cuda.synchronize()
+    _check_all(first, d, DEFAULT)
+    for _ in range(3):
+        junk = torch.randn(1 << 20, device=dev).sum()  # eager work between replays
+        out = plan.launch()
+        torch.cuda.synchronize()
+        for k in first:
+            assert torch.equal(out[k], first[k]), k
+    # new data in the same input buffers: the next replay reads it
+    d2 = ta.synth_ohlcv(n, rows=rows, seed=43)
+    for t, k in zip(T, ("high", "low", "close")):
+        t.copy_(torch.from_numpy(np.ascontiguousarray(d2[k])).to(dev))
+    out = plan.launch()
+    torch.cuda.synchronize()
+    _check_all(out, d2, DEFAULT)
+    plan.close()
+    del junk
+
+
+def test_plan_over_ragged_layout(dev):
+    import tacuda_amd as ta
+    from tests.test_gpu_pack import DEFAULT
+    from tests.test_gpu_ragged import _layout, _rows
+    from oracle import ORC
+    from tests.parity import compare
+    from tests.test_gpu_pack import EXACT, _want
+
+    print("plan ragged", flush=True)
+    lengths = list(np.random.default_rng(3).integers(64, 3000, size=300))
+    off, cat = _layout(lengths, seed=5)
+    lay = ta.ops.Ragged(off)
+    T = [torch.from_numpy(cat[k]).to(dev) for k in ("high", "low", "close")]
+    plan = ta.ops.Plan(*T, layout=lay, tune=True, **DEFAULT)
+    out = plan.launch()
+    torch.cuda.synchronize()
+    for name, t in out.items():
+        g = t.cpu().numpy()
+        for r in range(len(lengths)):
+            sl = slice(off[r], off[r + 1])
+            w = _want(ORC, DEFAULT, cat["high"][sl], cat["low"][sl], cat["close"][sl], name)
+            if name in EXACT:
+                compare(g[sl], w, tol=0.0, min_exact=1.0, name=f"{name} row {r}")
+            else:
+                compare(g[sl], w, tol=1e-5, min_exact=0.999, name=f"{name} row {r}")
+    plan.close()
+    lay.close()
