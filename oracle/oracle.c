@@ -516,8 +516,20 @@ static int atr_core(const float* h, const float* l, const float* c, int n, int p
   return 1;
 }
 
+/* Period 1 of ATR and NATR is TRANGE: unnormalized, NaN at bar 0, and a NaN
+ * bar does not poison later bars (reference docs/BATCH_API.md:155-159,
+ * NATR.cu:171-174, ATR.cu:163; classic TA-Lib's period-1 TRANGE arm). */
+static void trange_p1(const float* h, const float* l, const float* c, float* out, int n) {
+  out[0] = NAN;
+  for (int t = 1; t < n; ++t) out[t] = (float)true_range(h[t], l[t], c[t - 1]);
+}
+
 API int orc_atr(const float* h, const float* l, const float* c, float* out, int n, int p) {
   if (!h || !l || !c || !out || n <= 0 || p <= 0) return ORC_BAD;
+  if (p == 1) {
+    trange_p1(h, l, c, out, n);
+    return ORC_OK;
+  }
   double* tmp = (double*)malloc(sizeof(double) * (size_t)n);
   atr_core(h, l, c, n, p, tmp);
   for (int i = 0; i < n; ++i) out[i] = (float)tmp[i];
@@ -530,6 +542,10 @@ API int orc_atr(const float* h, const float* l, const float* c, float* out, int 
  * from the pinned ATR by an independent division in the test. */
 API int orc_natr(const float* h, const float* l, const float* c, float* out, int n, int p) {
   if (!h || !l || !c || !out || n <= 0 || p <= 0) return ORC_BAD;
+  if (p == 1) {
+    trange_p1(h, l, c, out, n);
+    return ORC_OK;
+  }
   double* tmp = (double*)malloc(sizeof(double) * (size_t)n);
   atr_core(h, l, c, n, p, tmp);
   for (int i = 0; i < n; ++i) {

@@ -637,6 +637,18 @@ struct FAtrT {
     const int lane = lane_id();
     const int a = c.s0 + lane * T::K;
     const int p = pr.p;
+    if (p == 1) {
+      // period 1 is TRANGE for ATR and NATR alike (unnormalized; NaN at bar
+      // 0; recovers after a NaN bar): reference docs/BATCH_API.md:155-159
+      double pc1 = (double)at<T>(lc, io.in[2], c, a - 1);
+#pragma unroll
+      for (int i = 0; i < T::K; ++i) {
+        ov[i] = (a + i < 1) ? qnanf_() : (float)true_range(hv[i], lv[i], pc1);
+        pc1 = cv[i];
+      }
+      store_lane<T>(lo, ov, io.out[0], c);
+      return;
+    }
     const int L = p;
     const double wb = (double)(p - 1) / (double)p;
     const double wa = 1.0 - wb;

@@ -531,7 +531,7 @@ __device__ __forceinline__ void pack_hlc(const Seg& c, float* li, float* lo, con
       dxv[i] = qnan_();
       if (t == La) ya = sA;
       else if (t > La) ya = fma(wb, ya, wa * tr[i]);
-      if (t >= La) ov[i] = (float)ya;
+      if (t >= La) ov[i] = (float)(pa == 1 ? tr[i] : ya);
       if (t == Lx) {
         yp = sP;
         yq = sM;

@@ -324,3 +324,19 @@ def test_invalid_arguments():
         with pytest.raises(ValueError):
             bad()
     assert np.isnan(ORC.ema(x, 20)).all()
+
+
+def test_period_one_atr_and_natr_are_trange():
+    """Reference contract (docs/BATCH_API.md:155-159): ATR(1) and NATR(1) are
+    the unnormalized true range, NaN at bar 0, recovering after a NaN bar.
+    On finite bars this is also TA-Lib 0.8.3's fma(0, prev, 1*TR)."""
+    o, h, l, c = DATASETS["rw0"]
+    h, l, c = h.copy(), l.copy(), c.copy()
+    tr = ORC.trange(h, l, c)
+    assert np.array_equal(ORC.atr(h, l, c, 1), tr, equal_nan=True)
+    assert np.array_equal(ORC.natr(h, l, c, 1), tr, equal_nan=True)
+    h[100] = np.nan
+    tr = ORC.trange(h, l, c)
+    atr = ORC.atr(h, l, c, 1)
+    assert np.isnan(atr[100]) and np.isfinite(atr[101:]).all()
+    assert np.array_equal(atr, tr, equal_nan=True)
