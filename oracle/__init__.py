@@ -182,6 +182,12 @@ class _Family:
     def adx(self, h, l, c, p):
         return _call(self._fn("adx", 3, 1, [_I]), [h, l, c], 1, len(c), p)
 
+    def obv(self, c, v):
+        return _call(self._fn("obv", 2, 1, []), [c, v], 1, len(c))
+
+    def ad(self, h, l, c, v):
+        return _call(self._fn("ad", 4, 1, []), [h, l, c, v], 1, len(c))
+
     def candle(self, name, o, h, l, c):
         """int32 signal of one of the 61 TA-Lib patterns (see candles.c)."""
         L = self._get()

@@ -948,6 +948,48 @@ API int orc_adx(const float* h, const float* l, const float* c, float* out, int 
   return directional(h, l, c, out, n, p, DIR_ADX);
 }
 
+/* ------------------------------------------------------------ OBV / AD ---
+ * Reference OBV.cu / AD.cu semantics as restated by its stream states
+ * (include/utils/StreamStates.h:442-505): OBV starts from bar 0's raw volume
+ * and adds sign(close_t - close_{t-1}) * volume after; AD adds
+ * (((c-l)-(h-c))/(h-l))*v for finite bars with a positive range, 0
+ * otherwise.  Both accumulate in double from 0.0.  parity vs TA-Lib
+ * unpinned (not vendored); pinned by brute force and sign invariants. */
+API int orc_obv(const float* c, const float* v, float* out, int n) {
+  if (!c || !v || !out || n <= 0) return ORC_BAD;
+  double acc = 0.0;
+  for (int t = 0; t < n; ++t) {
+    if (t == 0) {
+      acc += (double)v[0];
+    } else {
+      const double p = (double)c[t], prev = (double)c[t - 1];
+      double s = 0.0;
+      if (p > prev) s = 1.0;
+      else if (p < prev) s = -1.0;
+      acc += s * (double)v[t];
+    }
+    out[t] = (float)acc;
+  }
+  return ORC_OK;
+}
+
+API int orc_ad(const float* h, const float* l, const float* c, const float* v, float* out,
+               int n) {
+  if (!h || !l || !c || !v || !out || n <= 0) return ORC_BAD;
+  double acc = 0.0;
+  for (int t = 0; t < n; ++t) {
+    const double hi = h[t], lo = l[t], cl = c[t], vo = v[t];
+    double term = 0.0;
+    if (isfinite(hi) && isfinite(lo) && isfinite(cl) && isfinite(vo)) {
+      const double range = hi - lo;
+      if (range > 0.0) term = (((cl - lo) - (hi - cl)) / range) * vo;
+    }
+    acc += term;
+    out[t] = (float)acc;
+  }
+  return ORC_OK;
+}
+
 /* ------------------------------------------------- batch convenience ---
  * Row loop over a series-major [rows x len] matrix, used by the tests and
  * the CPU baseline to time the oracle over many series. */

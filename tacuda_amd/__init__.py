@@ -125,6 +125,40 @@ _sig("ctx_plan_launch", c_void_p, c_void_p, stream=False)
 _sig("ctx_plan_info", c_void_p, ctypes.POINTER(c_int), ctypes.POINTER(c_int),
      ctypes.POINTER(c_float), ctypes.POINTER(c_float), stream=False)
 _sig("ctx_plan_destroy", c_void_p, stream=False)
+# streaming (include/tacuda_stream.h)
+_PP = ctypes.POINTER(c_void_p)
+for _n in ("sma", "ema", "wma", "rsi", "atr", "natr"):
+    _sig(f"ct_{_n}_stream_create", c_int, _PP, stream=False)
+    _sig(f"ct_{_n}_stream_batch_create", c_int, c_int, _PP, stream=False)
+_sig("ct_obv_stream_create", _PP, stream=False)
+_sig("ct_ad_stream_create", _PP, stream=False)
+_sig("ct_macd_stream_create", c_int, c_int, c_int, _PP, stream=False)
+_sig("ct_bbands_stream_create", c_int, c_float, c_float, _PP, stream=False)
+_sig("ct_obv_stream_batch_create", c_int, _PP, stream=False)
+_sig("ct_ad_stream_batch_create", c_int, _PP, stream=False)
+_sig("ct_macd_stream_batch_create", c_int, c_int, c_int, c_int, _PP, stream=False)
+_sig("ct_bbands_stream_batch_create", c_int, c_float, c_float, c_int, _PP, stream=False)
+for _pre in ("ct_{}_stream_append", "ct_{}_stream_batch_append"):
+    for _n in ("sma", "ema", "wma", "rsi"):
+        _sig(_pre.format(_n), c_void_p, P, c_int, P, stream=False)
+    for _n in ("atr", "natr"):
+        _sig(_pre.format(_n), c_void_p, P, P, P, c_int, P, stream=False)
+    _sig(_pre.format("obv"), c_void_p, P, P, c_int, P, stream=False)
+    _sig(_pre.format("ad"), c_void_p, P, P, P, P, c_int, P, stream=False)
+    _sig(_pre.format("macd"), c_void_p, P, c_int, P, P, P, stream=False)
+    _sig(_pre.format("bbands"), c_void_p, P, c_int, P, P, P, stream=False)
+for _n in ("destroy", "reset"):
+    _sig(f"ct_stream_{_n}", c_void_p, stream=False)
+    _sig(f"ct_stream_batch_{_n}", c_void_p, stream=False)
+_sig("ct_stream_lookback", c_void_p, ctypes.POINTER(c_int), stream=False)
+_sig("ct_stream_outputs", c_void_p, ctypes.POINTER(c_int), stream=False)
+_sig("ct_stream_bars", c_void_p, ctypes.POINTER(ctypes.c_longlong), stream=False)
+_sig("ct_stream_batch_lookback", c_void_p, ctypes.POINTER(c_int), stream=False)
+_sig("ct_stream_batch_outputs", c_void_p, ctypes.POINTER(c_int), stream=False)
+_sig("ct_stream_batch_series", c_void_p, ctypes.POINTER(c_int), stream=False)
+_sig("ct_stream_batch_bars", c_void_p, ctypes.POINTER(ctypes.c_longlong), stream=False)
+_sig("ct_stream_batch_set_exec", c_void_p, c_int, stream=False)
+_sig("ctx_stream_batch_append_d", c_void_p, P, P, P, P, c_int, P, P, P)
 
 
 class TacudaError(RuntimeError):
@@ -174,3 +208,4 @@ def synth_ohlcv(n: int, rows: int = 1, seed: int = 0, first: int = 0, threads: i
 
 
 from . import ops  # noqa: E402,F401
+from . import stream  # noqa: E402,F401

@@ -1,0 +1,148 @@
+"""Streaming API (include/tacuda_stream.h) over numpy arrays, mirroring the
+reference's binding (bindings/python/tacuda/__init__.py streaming classes):
+Stream for one series on the host, StreamBatch for a portfolio whose
+appends run on the host or on the GPU."""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import call, lib
+
+# family -> (inputs read, number of outputs)
+FAMILIES = {
+    "sma": ("c", 1), "ema": ("c", 1), "wma": ("c", 1), "rsi": ("c", 1),
+    "atr": ("hlc", 1), "natr": ("hlc", 1), "obv": ("cv", 1), "ad": ("hlcv", 1),
+    "macd": ("c", 3), "bbands": ("c", 3),
+}
+EXEC = {"auto": 0, "cpu": 1, "gpu": 2}
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _params(family, params):
+    if family == "bbands":
+        p, up, dn = (list(params) + [20, 2.0, 2.0][len(params):])[:3]
+        return [int(p), float(up), float(dn)]
+    return [int(v) for v in params]
+
+
+class _Base:
+    _destroy = ""
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            getattr(lib(), self._destroy)(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _q(self, name, ctype=ctypes.c_int):
+        v = ctype()
+        call(name, self._h, ctypes.byref(v))
+        return v.value
+
+
+class Stream(_Base):
+    """Incremental state of one indicator over one series (host, FP64)."""
+
+    _destroy = "ct_stream_destroy"
+
+    def __init__(self, family, *params):
+        self.family = family
+        self.inputs, self.nout = FAMILIES[family]
+        self._h = ctypes.c_void_p()
+        call(f"ct_{family}_stream_create", *_params(family, params), ctypes.byref(self._h))
+
+    def append(self, high=None, low=None, close=None, volume=None):
+        arrs = {"h": high, "l": low, "c": close, "v": volume}
+        ins = [_f32(arrs[k]) for k in self.inputs]
+        n = len(ins[0])
+        outs = [np.empty(n, np.float32) for _ in range(self.nout)]
+        args = [self._h] + [a.ctypes.data for a in ins] + [n] + [o.ctypes.data for o in outs]
+        call(f"ct_{self.family}_stream_append", *args)
+        return outs[0] if self.nout == 1 else tuple(outs)
+
+    def reset(self):
+        call("ct_stream_reset", self._h)
+
+    @property
+    def lookback(self):
+        return self._q("ct_stream_lookback")
+
+    @property
+    def outputs(self):
+        return self._q("ct_stream_outputs")
+
+    @property
+    def bars(self):
+        return self._q("ct_stream_bars", ctypes.c_longlong)
+
+
+class StreamBatch(_Base):
+    """numSeries independent streams of one indicator; inputs and outputs
+    are series-major [series, n] matrices."""
+
+    _destroy = "ct_stream_batch_destroy"
+
+    def __init__(self, family, num_series, *params, exec="auto"):
+        self.family = family
+        self.inputs, self.nout = FAMILIES[family]
+        self.rows = int(num_series)
+        self._h = ctypes.c_void_p()
+        call(f"ct_{family}_stream_batch_create", *_params(family, params), self.rows,
+             ctypes.byref(self._h))
+        self.set_exec(exec)
+
+    def set_exec(self, mode):
+        call("ct_stream_batch_set_exec", self._h, EXEC[mode])
+
+    def append(self, high=None, low=None, close=None, volume=None):
+        arrs = {"h": high, "l": low, "c": close, "v": volume}
+        ins = [_f32(arrs[k]).reshape(self.rows, -1) for k in self.inputs]
+        n = ins[0].shape[1]
+        outs = [np.empty((self.rows, n), np.float32) for _ in range(self.nout)]
+        args = [self._h] + [a.ctypes.data for a in ins] + [n] + [o.ctypes.data for o in outs]
+        call(f"ct_{self.family}_stream_batch_append", *args)
+        return outs[0] if self.nout == 1 else tuple(outs)
+
+    def append_d(self, high=None, low=None, close=None, volume=None, outs=None):
+        """Device tensors in and out (ctx_stream_batch_append_d) on the
+        current stream; the state stays in HBM."""
+        import torch
+
+        ref = next(t for t in (close, high, volume) if t is not None)
+        n = ref.numel() // self.rows
+        if outs is None:
+            outs = [torch.empty_like(ref) for _ in range(self.nout)]
+        ptr = [t.data_ptr() if t is not None else None for t in (high, low, close, volume)]
+        op = [o.data_ptr() for o in outs] + [None] * (3 - len(outs))
+        call("ctx_stream_batch_append_d", self._h, *ptr, n, *op,
+             torch.cuda.current_stream().cuda_stream)
+        return outs[0] if self.nout == 1 else tuple(outs)
+
+    def reset(self):
+        call("ct_stream_batch_reset", self._h)
+
+    @property
+    def lookback(self):
+        return self._q("ct_stream_batch_lookback")
+
+    @property
+    def outputs(self):
+        return self._q("ct_stream_batch_outputs")
+
+    @property
+    def series(self):
+        return self._q("ct_stream_batch_series")
+
+    @property
+    def bars(self):
+        return self._q("ct_stream_batch_bars", ctypes.c_longlong)

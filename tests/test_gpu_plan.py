@@ -78,14 +78,14 @@ def test_plan_over_ragged_layout(dev):
     plan = ta.ops.Plan(*T, layout=lay, tune=True, **DEFAULT)
     out = plan.launch()
     torch.cuda.synchronize()
+    from tests.test_gpu_ragged import _cat_want
+
     for name, t in out.items():
-        g = t.cpu().numpy()
-        for r in range(len(lengths)):
-            sl = slice(off[r], off[r + 1])
-            w = _want(ORC, DEFAULT, cat["high"][sl], cat["low"][sl], cat["close"][sl], name)
-            if name in EXACT:
-                compare(g[sl], w, tol=0.0, min_exact=1.0, name=f"{name} row {r}")
-            else:
-                compare(g[sl], w, tol=1e-5, min_exact=0.999, name=f"{name} row {r}")
+        want = _cat_want(off, lambda a, b, d: _want(ORC, DEFAULT, a, b, d, name),
+                         cat["high"], cat["low"], cat["close"])
+        if name in EXACT:
+            compare(t.cpu().numpy(), want, tol=0.0, min_exact=1.0, name=name)
+        else:
+            compare(t.cpu().numpy(), want, tol=1e-5, min_exact=0.999, name=name)
     plan.close()
     lay.close()

@@ -46,6 +46,18 @@ def _cmp(g, w, exact, name):
         compare(g, w, tol=1e-5, min_exact=0.999, name=name)
 
 
+def _cat_want(off, fn, *arrs):
+    """Oracle per series, concatenated in layout order (empty series skipped)."""
+    parts = []
+    for r in range(len(off) - 1):
+        sl = slice(off[r], off[r + 1])
+        if off[r + 1] > off[r]:
+            parts.append(fn(*[a[sl] for a in arrs]))
+    if isinstance(parts[0], tuple):
+        return tuple(np.concatenate([p[j] for p in parts]) for j in range(len(parts[0])))
+    return np.concatenate(parts)
+
+
 @pytest.mark.parametrize("case", sorted(LENGTHS))
 def test_ragged_families(dev, case):
     from oracle import ORC
@@ -55,38 +67,31 @@ def test_ragged_families(dev, case):
     off, cat = _layout(LENGTHS[case])
     lay = Ragged(off)
     T = {k: torch.from_numpy(v).to(dev) for k, v in cat.items()}
+    h, l, c = cat["high"], cat["low"], cat["close"]
     for name, (prm, exact) in ONE.items():
         got = lay.one(name, T["close"], *prm).cpu().numpy()
-        for r, (g, x) in enumerate(zip(_rows(got, off), _rows(cat["close"], off))):
-            if len(x):
-                _cmp(g, getattr(ORC, name)(x, *prm), exact, f"{name} row {r}")
+        want = _cat_want(off, lambda x: getattr(ORC, name)(x, *prm), c)
+        _cmp(got, want, exact, name)
     for name, (prm, exact) in HLC.items():
         got = lay.hlc(name, T["high"], T["low"], T["close"], *prm).cpu().numpy()
-        rows = zip(_rows(got, off), _rows(cat["high"], off), _rows(cat["low"], off),
-                   _rows(cat["close"], off))
-        for r, (g, h, l, c) in enumerate(rows):
-            if len(c):
-                _cmp(g, getattr(ORC, name)(h, l, c, *prm), exact, f"{name} row {r}")
-    m, s, h = lay.macd(T["close"], 12, 26, 9)
-    k, d = lay.stochastic(T["high"], T["low"], T["close"], 14, 3)
-    for r, c in enumerate(_rows(cat["close"], off)):
-        if not len(c):
-            continue
-        sl = slice(off[r], off[r + 1])
-        want = ORC.macd(c, 12, 26, 9)
-        for j, g in enumerate((m, s, h)):
-            _cmp(g[sl].cpu().numpy(), want[j], False, f"macd[{j}] row {r}")
-        want = ORC.stoch(cat["high"][sl], cat["low"][sl], c, 14, 3)
-        _cmp(k[sl].cpu().numpy(), want[0], True, f"stoch k row {r}")
-        _cmp(d[sl].cpu().numpy(), want[1], True, f"stoch d row {r}")
+        want = _cat_want(off, lambda a, b, d: getattr(ORC, name)(a, b, d, *prm), h, l, c)
+        _cmp(got, want, exact, name)
+    got = [t.cpu().numpy() for t in lay.macd(T["close"], 12, 26, 9)]
+    want = _cat_want(off, lambda x: ORC.macd(x, 12, 26, 9), c)
+    for j in range(3):
+        _cmp(got[j], want[j], False, f"macd[{j}]")
+    got = [t.cpu().numpy() for t in lay.stochastic(T["high"], T["low"], T["close"], 14, 3)]
+    want = _cat_want(off, lambda a, b, d: ORC.stoch(a, b, d, 14, 3), h, l, c)
+    for j in range(2):
+        _cmp(got[j], want[j], True, f"stoch[{j}]")
     lay.close()
 
 
 @pytest.mark.parametrize("case", sorted(LENGTHS))
 def test_ragged_pack(dev, case):
+    from oracle import ORC
     from tacuda_amd.ops import Ragged
     from tests.test_gpu_pack import DEFAULT, EXACT, _want
-    from oracle import ORC
 
     print(f"ragged pack {case}", flush=True)
     off, cat = _layout(LENGTHS[case], seed=9)
@@ -95,11 +100,7 @@ def test_ragged_pack(dev, case):
     got = lay.pack(T["high"], T["low"], T["close"], **DEFAULT)
     torch.cuda.synchronize()
     for name, t in got.items():
-        g = t.cpu().numpy()
-        for r in range(len(off) - 1):
-            sl = slice(off[r], off[r + 1])
-            if off[r + 1] == off[r]:
-                continue
-            w = _want(ORC, DEFAULT, cat["high"][sl], cat["low"][sl], cat["close"][sl], name)
-            _cmp(g[sl], w, name in EXACT, f"{name} row {r}")
+        want = _cat_want(off, lambda a, b, d: _want(ORC, DEFAULT, a, b, d, name),
+                         cat["high"], cat["low"], cat["close"])
+        _cmp(t.cpu().numpy(), want, name in EXACT, name)
     lay.close()
