@@ -50,7 +50,8 @@ def test_plan_replays_match_oracle(dev, shape):
         out = plan.launch()
         torch.cuda.synchronize()
         for k in first:
-            assert torch.equal(out[k], first[k]), k
+            # bit patterns: NaN warm-ups compare equal
+            assert torch.equal(out[k].view(torch.int32), first[k].view(torch.int32)), k
     # new data in the same input buffers: the next replay reads it
     d2 = ta.synth_ohlcv(n, rows=rows, seed=43)
     for t, k in zip(T, ("high", "low", "close")):
