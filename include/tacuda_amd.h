@@ -151,6 +151,46 @@ CTAPI_EXPORT ctStatus_t ctx_plan_info(ctxPlan plan, int* barsPerLane, int* waves
                                       float* tunedUs, float* defaultUs);
 CTAPI_EXPORT ctStatus_t ctx_plan_destroy(ctxPlan plan);
 
+/* ---------------------------------------------------------------------------
+ * Differentiable indicators: forward and adjoint kernels used by the torch
+ * autograd binding.  Dense series-major rows (numSeries x seriesLen, at most
+ * 16384 bars per row), device pointers, enqueued on `stream`.
+ *
+ * EMA_alpha / RSI_alpha take the smoothing weight alpha as a continuous
+ * parameter and keep the TA-Lib seed of `period` samples (EMA: mean of bars
+ * 0..p-1 at bar p-1; RSI: (sum of the first p gains/losses)/p at bar p);
+ * alpha = 2/(p+1) (EMA) or 1/p (RSI) gives the TA-Lib indicator.  The
+ * forward writes a checkpoint of the recurrence state at 64 points per row
+ * (EMA: 64 doubles per row, RSI: 128); the adjoint recomputes the rest.
+ * Adjoints write the gradient w.r.t. the prices (gx) and, per row, the
+ * gradient w.r.t. alpha (galpha, doubles; sum over rows for a shared alpha).
+ * MACD (ta_MACD.c, EMA type) and BBANDS (SMA middle) adjoints take the
+ * output gradients; their forwards are ct_macd_batch_d / ct_bbands_batch_d.
+ * Inputs must be finite.
+ * ------------------------------------------------------------------------- */
+CTAPI_EXPORT ctStatus_t ctx_ema_alpha_fwd_d(const float* x, float* y, double* ckpt,
+                                            int seriesLen, int numSeries, int period,
+                                            double alpha, hipStream_t stream);
+CTAPI_EXPORT ctStatus_t ctx_ema_alpha_bwd_d(const float* x, const float* gy, const double* ckpt,
+                                            float* gx, double* galpha, int seriesLen,
+                                            int numSeries, int period, double alpha,
+                                            hipStream_t stream);
+CTAPI_EXPORT ctStatus_t ctx_rsi_alpha_fwd_d(const float* x, float* y, double* ckpt,
+                                            int seriesLen, int numSeries, int period,
+                                            double alpha, hipStream_t stream);
+CTAPI_EXPORT ctStatus_t ctx_rsi_alpha_bwd_d(const float* x, const float* gy, const double* ckpt,
+                                            float* gx, double* galpha, int seriesLen,
+                                            int numSeries, int period, double alpha,
+                                            hipStream_t stream);
+CTAPI_EXPORT ctStatus_t ctx_macd_bwd_d(const float* gMacd, const float* gSignal,
+                                       const float* gHist, float* gx, int seriesLen,
+                                       int numSeries, int fastPeriod, int slowPeriod,
+                                       int signalPeriod, hipStream_t stream);
+CTAPI_EXPORT ctStatus_t ctx_bbands_bwd_d(const float* x, const float* gUpper,
+                                         const float* gMiddle, const float* gLower, float* gx,
+                                         int seriesLen, int numSeries, int period,
+                                         float nbDevUp, float nbDevDn, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

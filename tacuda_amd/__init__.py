@@ -159,6 +159,13 @@ _sig("ct_stream_batch_series", c_void_p, ctypes.POINTER(c_int), stream=False)
 _sig("ct_stream_batch_bars", c_void_p, ctypes.POINTER(ctypes.c_longlong), stream=False)
 _sig("ct_stream_batch_set_exec", c_void_p, c_int, stream=False)
 _sig("ctx_stream_batch_append_d", c_void_p, P, P, P, P, c_int, P, P, P)
+# differentiable indicators (include/tacuda_amd.h)
+_D = ctypes.c_double
+for _n in ("ema", "rsi"):
+    _sig(f"ctx_{_n}_alpha_fwd_d", P, P, P, c_int, c_int, c_int, _D)
+    _sig(f"ctx_{_n}_alpha_bwd_d", P, P, P, P, P, c_int, c_int, c_int, _D)
+_sig("ctx_macd_bwd_d", P, P, P, P, c_int, c_int, c_int, c_int, c_int)
+_sig("ctx_bbands_bwd_d", P, P, P, P, P, c_int, c_int, c_int, c_float, c_float)
 
 
 class TacudaError(RuntimeError):
@@ -209,3 +216,4 @@ def synth_ohlcv(n: int, rows: int = 1, seed: int = 0, first: int = 0, threads: i
 
 from . import ops  # noqa: E402,F401
 from . import stream  # noqa: E402,F401
+from . import autograd  # noqa: E402,F401

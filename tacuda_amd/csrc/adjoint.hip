@@ -1,0 +1,617 @@
+// Differentiable indicators (no counterpart in the reference): forward and
+// adjoint (reverse-mode) kernels behind torch.autograd.Function.
+//
+// One wave per row.  Lane l owns the contiguous chunk [l*C, l*C + C) of the
+// row (C = ceil(n/64)).  A recurrence y_t = beta*y_{t-1} + u_t is folded per
+// chunk, composed across lanes by an offset scan with the known multiplier
+// beta^C, and replayed.  The adjoint of such a recurrence is the same
+// recurrence run backwards, lambda_t = g_t + beta*lambda_{t+1}: the chunk
+// folds from its end and the lane scan runs from lane 63 down.  Forward
+// passes save only the state entering each chunk (64 doubles per row and
+// recurrence); adjoint passes recompute each chunk's forward values from it
+// into LDS (as float32) before walking the chunk backwards.
+//
+// Operators and their seeds follow the TA-Lib definitions of the families
+// (SMA-mean seed of the first p samples), with the smoothing weight alpha as
+// a continuous parameter:
+//   EMA_alpha   y_s = mean(x_0..x_{p-1}) at s = p-1, then y_t = (1-a)y + a x_t
+//   RSI_alpha   G, L Wilder averages of gains / losses: seed = (sum of the first
+//               p) * (1/p) at s = p, then G_t = (1-a)G + a*gain_t; out
+//               100*G/(G+L) (50 when G+L is 0)
+//   MACD        ta_MACD.c (fixed periods; linear in the prices)
+//   BBANDS      SMA middle, population deviation, mid +/- k*sd
+// Inputs must be finite (a NaN gives NaN gradients).
+#include <hip/hip_runtime.h>
+
+#include <utility>
+
+#include "../../include/tacuda_amd.h"
+#include "engine.hpp"
+#include "host.hpp"
+
+namespace tca {
+namespace {
+
+constexpr int kMaxChunk = 256;  // C <= 256: rows up to 16384 bars
+
+struct Chunk {
+  int lane, C, a, e;  // chunk [a, e) of a row of n bars
+};
+__device__ __forceinline__ Chunk chunk_of(int n) {
+  Chunk k;
+  k.lane = lane_id();
+  k.C = (n + kWave - 1) / kWave;
+  k.a = min(n, k.lane * k.C);
+  k.e = min(n, k.a + k.C);
+  return k;
+}
+
+// Per-lane float array in LDS, padded so lanes hit distinct banks.
+struct LdsCol {
+  float* base;
+  int stride;
+  __device__ float& at(int lane, int i) { return base[lane * stride + i]; }
+};
+
+// Forward offset scan: returns the state entering this lane's chunk.
+__device__ __forceinline__ double carry_fwd(double b, double beta, int C) {
+  double bb[1] = {b}, A[1] = {pow_n(beta, C)};
+  const double be[1] = {beta};
+  bscan_incl<1, true>(bb, A, be);
+  return dpp_d<kWaveShr1, 0xf, 0xf>(0.0, bb[0]);
+}
+
+// Reverse offset scan: `b` is the chunk's adjoint folded from its end (the
+// adjoint at its first bar given zero beyond it); returns the adjoint just
+// past the chunk's end.
+__device__ __forceinline__ double carry_bwd(double b, double beta, int C) {
+  double A = pow_clamp(pow_n(beta, C), beta);
+  const int lane = lane_id();
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const double ob = __shfl_down(b, d);
+    const double oA = __shfl_down(A, d);
+    if (lane + d < kWave) {
+      b = fma(A, ob, b);
+      A = pow_clamp(A * oA, beta);
+    }
+  }
+  const double nb = __shfl_down(b, 1);
+  return lane + 1 < kWave ? nb : 0.0;
+}
+
+// Sum over the wave (every lane receives it).
+__device__ __forceinline__ double wsum(double v) {
+  double t[1] = {v};
+  wave_total<1>(t);
+  return t[0];
+}
+
+// =================================================================== EMA ==
+__global__ __launch_bounds__(64) void k_ema_fwd(const float* x, float* y, double* ck, int rows,
+                                                int n, int p, double alpha) {
+  const int row = blockIdx.x;
+  if (row >= rows) return;
+  const Chunk k = chunk_of(n);
+  const float* xr = x + (size_t)row * n;
+  float* yr = y + (size_t)row * n;
+  const double beta = 1.0 - alpha;
+  const int s = p - 1;
+  double part = 0.0;
+  for (int t = k.a; t < k.e && t <= s; ++t) part += (double)xr[t];
+  const double seed = wsum(part) / p;
+  double b = 0.0;
+  for (int t = k.a; t < k.e; ++t) {
+    if (t == s) b = seed;
+    else if (t > s) b = fma(beta, b, alpha * (double)xr[t]);
+  }
+  double v = carry_fwd(b, beta, k.C);
+  ck[(size_t)row * kWave + k.lane] = v;
+  for (int t = k.a; t < k.e; ++t) {
+    if (t < s) {
+      yr[t] = __builtin_nanf("");
+      continue;
+    }
+    v = (t == s) ? seed : fma(beta, v, alpha * (double)xr[t]);
+    yr[t] = (float)v;
+  }
+}
+
+__global__ __launch_bounds__(64) void k_ema_bwd(const float* x, const float* g, const double* ck,
+                                                float* dx, double* dal, int rows, int n, int p,
+                                                double alpha) {
+  extern __shared__ float s_col[];
+  const int row = blockIdx.x;
+  if (row >= rows) return;
+  const Chunk k = chunk_of(n);
+  LdsCol r{s_col, k.C + 1};
+  const float* xr = x + (size_t)row * n;
+  const float* gr = g + (size_t)row * n;
+  float* dr = dx + (size_t)row * n;
+  const double beta = 1.0 - alpha;
+  const int s = p - 1;
+  // forward recompute: residual x_t - y_{t-1} of every bar after the seed
+  double v = ck[(size_t)row * kWave + k.lane];
+  double part = 0.0;
+  for (int t = k.a; t < k.e && t <= s; ++t) part += (double)xr[t];
+  const double seed = wsum(part) / p;
+  for (int t = k.a; t < k.e; ++t) {
+    if (t < s) continue;
+    if (t == s) {
+      v = seed;
+      continue;
+    }
+    const double xv = (double)xr[t];
+    r.at(k.lane, t - k.a) = (float)(xv - v);
+    v = fma(beta, v, alpha * xv);
+  }
+  // adjoint: lambda_t = g_t + beta*lambda_{t+1} for t >= s
+  double b = 0.0;
+  for (int t = k.e - 1; t >= k.a && t >= s; --t) b = fma(beta, b, (double)gr[t]);
+  double lam = carry_bwd(b, beta, k.C);
+  double da = 0.0, lamS = 0.0;
+  for (int t = k.e - 1; t >= k.a; --t) {
+    if (t < s) {
+      dr[t] = 0.0f;
+      continue;
+    }
+    lam = fma(beta, lam, (double)gr[t]);
+    if (t == s) {
+      lamS = lam;
+      dr[t] = 0.0f;
+    } else {
+      dr[t] = (float)(alpha * lam);
+      da += lam * (double)r.at(k.lane, t - k.a);
+    }
+  }
+  // the seed is the mean of bars 0..s
+  lamS = wsum(lamS);
+  for (int t = k.a; t < k.e && t <= s; ++t) dr[t] = (float)(lamS / p);
+  da = wsum(da);
+  if (k.lane == 0) dal[row] = da;
+}
+
+// =================================================================== RSI ==
+__global__ __launch_bounds__(64) void k_rsi_fwd(const float* x, float* y, double* ck, int rows,
+                                                int n, int p, double alpha) {
+  const int row = blockIdx.x;
+  if (row >= rows) return;
+  const Chunk k = chunk_of(n);
+  const float* xr = x + (size_t)row * n;
+  float* yr = y + (size_t)row * n;
+  const double beta = 1.0 - alpha, inv = 1.0 / (double)p;
+  const int s = p;
+  double pg = 0.0, pl = 0.0;
+  for (int t = max(k.a, 1); t < k.e && t <= s; ++t) {
+    const double d = (double)xr[t] - (double)xr[t - 1];
+    const double gd = d > 0.0 ? d : 0.0;
+    pg += gd;
+    pl += gd - d;
+  }
+  const double sG = wsum(pg) * inv, sL = wsum(pl) * inv;
+  double bg = 0.0, bl = 0.0;
+  for (int t = max(k.a, 1); t < k.e; ++t) {
+    const double d = (double)xr[t] - (double)xr[t - 1];
+    const double gd = d > 0.0 ? d : 0.0, ld = gd - d;
+    if (t == s) {
+      bg = sG;
+      bl = sL;
+    } else if (t > s) {
+      bg = fma(beta, bg, alpha * gd);
+      bl = fma(beta, bl, alpha * ld);
+    }
+  }
+  double G = carry_fwd(bg, beta, k.C), L = carry_fwd(bl, beta, k.C);
+  ck[((size_t)row * kWave + k.lane) * 2] = G;
+  ck[((size_t)row * kWave + k.lane) * 2 + 1] = L;
+  for (int t = k.a; t < k.e; ++t) {
+    if (t < s) {
+      yr[t] = __builtin_nanf("");
+      continue;
+    }
+    const double d = (double)xr[t] - (double)xr[t - 1];
+    const double gd = d > 0.0 ? d : 0.0, ld = gd - d;
+    if (t == s) {
+      G = sG;
+      L = sL;
+    } else {
+      G = fma(beta, G, alpha * gd);
+      L = fma(beta, L, alpha * ld);
+    }
+    const double sum = G + L;
+    yr[t] = (float)(sum > 0.0 ? 100.0 * (G / sum) : 50.0);
+  }
+}
+
+__global__ __launch_bounds__(64) void k_rsi_bwd(const float* x, const float* g, const double* ck,
+                                                float* dx, double* dal, int rows, int n, int p,
+                                                double alpha) {
+  extern __shared__ float s_col[];
+  const int row = blockIdx.x;
+  if (row >= rows) return;
+  const Chunk k = chunk_of(n);
+  LdsCol cG{s_col, k.C + 1}, cL{s_col + kWave * (k.C + 1), k.C + 1};
+  const float* xr = x + (size_t)row * n;
+  const float* gr = g + (size_t)row * n;
+  float* dr = dx + (size_t)row * n;
+  const double beta = 1.0 - alpha, inv = 1.0 / (double)p;
+  const int s = p;
+  // forward recompute: G_{t-1}, L_{t-1} per bar (float32 in LDS)
+  double pg = 0.0, pl = 0.0;
+  for (int t = max(k.a, 1); t < k.e && t <= s; ++t) {
+    const double d = (double)xr[t] - (double)xr[t - 1];
+    const double gd = d > 0.0 ? d : 0.0;
+    pg += gd;
+    pl += gd - d;
+  }
+  const double sG = wsum(pg) * inv, sL = wsum(pl) * inv;
+  double G = ck[((size_t)row * kWave + k.lane) * 2], L = ck[((size_t)row * kWave + k.lane) * 2 + 1];
+  for (int t = k.a; t < k.e; ++t) {
+    if (t < s) continue;
+    cG.at(k.lane, t - k.a) = (float)G;  // state before bar t
+    cL.at(k.lane, t - k.a) = (float)L;
+    const double d = (double)xr[t] - (double)xr[t - 1];
+    const double gd = d > 0.0 ? d : 0.0, ld = gd - d;
+    if (t == s) {
+      G = sG;
+      L = sL;
+    } else {
+      G = fma(beta, G, alpha * gd);
+      L = fma(beta, L, alpha * ld);
+    }
+  }
+  // adjoint of the output map at bar t needs G_t, L_t: the value before the
+  // next bar, i.e. the recomputed entry of t+1 (or the carried end state)
+  auto after = [&](int t, double& Gt, double& Lt, double Gend, double Lend) {
+    if (t + 1 < k.e) {
+      Gt = cG.at(k.lane, t + 1 - k.a);
+      Lt = cL.at(k.lane, t + 1 - k.a);
+    } else {
+      Gt = Gend;
+      Lt = Lend;
+    }
+  };
+  const double Gend = G, Lend = L;
+  auto coef = [&](double Gt, double Lt, double& cg, double& cl) {
+    const double sum = Gt + Lt;
+    if (sum > 0.0) {
+      const double q = 100.0 / (sum * sum);
+      cg = q * Lt;
+      cl = -q * Gt;
+    } else {
+      cg = cl = 0.0;
+    }
+  };
+  // lambdaG_t = cG_t g_t + beta*lambdaG_{t+1} (t >= s), same for L
+  double bg = 0.0, bl = 0.0;
+  for (int t = k.e - 1; t >= k.a && t >= s; --t) {
+    double Gt, Lt, cg, cl;
+    after(t, Gt, Lt, Gend, Lend);
+    coef(Gt, Lt, cg, cl);
+    const double gv = (double)gr[t];
+    bg = fma(beta, bg, cg * gv);
+    bl = fma(beta, bl, cl * gv);
+  }
+  double lg = carry_bwd(bg, beta, k.C), lq = carry_bwd(bl, beta, k.C);
+  // d(gain)_t and d(loss)_t adjoints, then through d_t = x_t - x_{t-1}.
+  // dd_t (the adjoint of d_t) goes to this lane's own part of dx first.
+  double da = 0.0, lgS = 0.0, lqS = 0.0;
+  for (int t = k.e - 1; t >= k.a; --t) {
+    double ddt = 0.0;
+    if (t >= s) {
+      double Gt, Lt, cg, cl;
+      after(t, Gt, Lt, Gend, Lend);
+      coef(Gt, Lt, cg, cl);
+      const double gv = (double)gr[t];
+      lg = fma(beta, lg, cg * gv);
+      lq = fma(beta, lq, cl * gv);
+      const double d = (double)xr[t] - (double)xr[t - 1];
+      const double gd = d > 0.0 ? d : 0.0, ld = gd - d;
+      if (t == s) {
+        lgS = lg;
+        lqS = lq;
+      } else {
+        ddt = (d > 0.0 ? alpha * lg : 0.0) - (d < 0.0 ? alpha * lq : 0.0);
+        da += lg * (gd - (double)cG.at(k.lane, t - k.a)) +
+              lq * (ld - (double)cL.at(k.lane, t - k.a));
+      }
+    }
+    dr[t] = (float)ddt;
+  }
+  lgS = wsum(lgS);
+  lqS = wsum(lqS);
+  // the seeds are (sum of the first p gains / losses) * (1/p)
+  for (int t = max(k.a, 1); t < k.e && t <= s; ++t) {
+    const double d = (double)xr[t] - (double)xr[t - 1];
+    dr[t] = (float)((d > 0.0 ? lgS * inv : 0.0) - (d < 0.0 ? lqS * inv : 0.0));
+  }
+  // dx_t = dd_t - dd_{t+1}; the next chunk's first dd comes from lane+1
+  const float first = k.a < k.e ? dr[k.a] : 0.0f;
+  const float nextFirst = __shfl_down(first, 1);
+  float cur = first;
+  for (int t = k.a; t < k.e; ++t) {
+    const float nxt = (t + 1 < k.e) ? dr[t + 1]
+                                    : (k.lane + 1 < kWave && t + 1 < n ? nextFirst : 0.0f);
+    dr[t] = (t == 0 ? 0.0f : cur) - nxt;
+    cur = nxt;
+  }
+  da = wsum(da);
+  if (k.lane == 0) dal[row] = da;
+}
+
+// ================================================================== MACD ==
+// Adjoint of ta_MACD.c (linear in the prices).  gm, gs, gh: gradients of the
+// MACD line, signal and histogram outputs.
+__global__ __launch_bounds__(64) void k_macd_bwd(const float* gm, const float* gs,
+                                                 const float* gh, float* dx, int rows, int n,
+                                                 int fast, int slow, int sig) {
+  extern __shared__ float s_col[];
+  const int row = blockIdx.x;
+  if (row >= rows) return;
+  const Chunk k = chunk_of(n);
+  LdsCol dm{s_col, k.C + 1};
+  const size_t ro = (size_t)row * n;
+  double fb, fk, sb, sk, gb, gk;
+  {
+    auto coef = [](int p, double& b, double& kk) {
+      double bb = (double)(p - 1) / (double)(p + 1);
+      const double q = 1.0 - bb;
+      if (bb < 0.5) bb = 1.0 - q;
+      b = bb;
+      kk = q;
+    };
+    coef(fast, fb, fk);
+    coef(slow, sb, sk);
+    coef(sig, gb, gk);
+  }
+  if (sig == 1) {
+    gb = 0.0;
+    gk = 1.0;
+  }
+  const int L1 = slow - 1, L2 = L1 + sig - 1;
+  // signal adjoint mu_t = (gs - gh)_t + gb*mu_{t+1} for t >= L2
+  double b = 0.0;
+  for (int t = k.e - 1; t >= k.a && t >= L2; --t)
+    b = fma(gb, b, (double)gs[ro + t] - (double)gh[ro + t]);
+  double mu = carry_bwd(b, gb, k.C), muS = 0.0;
+  for (int t = k.e - 1; t >= k.a; --t) {
+    double d = 0.0;
+    if (t >= L2) {
+      mu = fma(gb, mu, (double)gs[ro + t] - (double)gh[ro + t]);
+      d = (double)gm[ro + t] + (double)gh[ro + t];
+      if (t == L2) muS = mu;
+      else d += gk * mu;
+    }
+    dm.at(k.lane, t - k.a) = (float)d;
+  }
+  muS = wsum(muS);
+  for (int t = max(k.a, L1); t < k.e && t <= L2; ++t)
+    dm.at(k.lane, t - k.a) += (float)(muS / sig);
+  // fast / slow adjoints from dm (macd = fast - slow) for t >= L1
+  double bf = 0.0, bs = 0.0;
+  for (int t = k.e - 1; t >= k.a && t >= L1; --t) {
+    const double d = dm.at(k.lane, t - k.a);
+    bf = fma(fb, bf, d);
+    bs = fma(sb, bs, -d);
+  }
+  double nf = carry_bwd(bf, fb, k.C), ns = carry_bwd(bs, sb, k.C), nfS = 0.0, nsS = 0.0;
+  for (int t = k.e - 1; t >= k.a; --t) {
+    double out = 0.0;
+    if (t >= L1) {
+      const double d = dm.at(k.lane, t - k.a);
+      nf = fma(fb, nf, d);
+      ns = fma(sb, ns, -d);
+      if (t == L1) {
+        nfS = nf;
+        nsS = ns;
+      } else {
+        out = fk * nf + sk * ns;
+      }
+    }
+    dx[ro + t] = (float)out;
+  }
+  nfS = wsum(nfS);
+  nsS = wsum(nsS);
+  for (int t = k.a; t < k.e && t <= L1; ++t) {
+    double add = nsS / slow;
+    if (t > L1 - fast) add += nfS / fast;
+    dx[ro + t] = (float)((double)dx[ro + t] + add);
+  }
+}
+
+// ================================================================ BBANDS ==
+// Window adjoint: out_t = mid_t + ku*sd_t (upper), mid_t, mid_t - kd*sd_t,
+// mid = window mean, sd = population deviation.  dL/dx_j sums, over the
+// windows t = j..j+p-1 holding bar j, (gu+gm+gl)_t/p + (ku*gu - kd*gl)_t *
+// (x_j - mid_t)/(p*sd_t).  One thread per bar of a 256-bar tile; the tile's
+// per-window coefficients are built once in LDS.
+__global__ __launch_bounds__(256) void k_bbands_bwd(const float* x, const float* gu,
+                                                    const float* gm, const float* gl, float* dx,
+                                                    int rows, int n, int p, double ku,
+                                                    double kd) {
+  extern __shared__ double s_bb[];
+  const int tiles = (n + 255) / 256;
+  const int row = blockIdx.x / tiles, tile = blockIdx.x % tiles;
+  if (row >= rows) return;
+  const int j0 = tile * 256;
+  const int T0 = j0, T1 = min(n, j0 + 256 + p - 1);  // windows used by this tile
+  const int X0 = max(0, T0 - p + 1);                  // bars those windows read
+  float* xs = reinterpret_cast<float*>(s_bb + 3 * (256 + p));
+  const size_t ro = (size_t)row * n;
+  for (int i = threadIdx.x; X0 + i < T1; i += 256) xs[i] = x[ro + X0 + i];
+  __syncthreads();
+  double* A = s_bb;            // (gu+gm+gl)/p
+  double* B = s_bb + 256 + p;  // (ku*gu - kd*gl)/(p*sd)
+  double* M = s_bb + 2 * (256 + p);
+  for (int i = threadIdx.x; T0 + i < T1; i += 256) {
+    const int t = T0 + i;
+    double a = 0.0, bcoef = 0.0, mid = 0.0;
+    if (t >= p - 1) {
+      double sum = 0.0, s1 = 0.0, s2 = 0.0;
+      const double shift = xs[t - p + 1 - X0];
+      for (int q = t - p + 1; q <= t; ++q) {
+        const double v = xs[q - X0];
+        sum += v;
+        const double dd = v - shift;
+        s1 += dd;
+        s2 += dd * dd;
+      }
+      mid = sum / p;
+      const double inv = 1.0 / (double)p, mean = s1 * inv, e2 = s2 * inv;
+      double var = e2 - mean * mean;
+      if (var < 1e-12 * e2) var = 0.0;
+      const double sd = var > 0.0 ? sqrt(var) : 0.0;
+      const double u = gu[ro + t], m = gm[ro + t], l = gl[ro + t];
+      a = (u + m + l) / p;
+      bcoef = sd > 0.0 ? (ku * u - kd * l) / (p * sd) : 0.0;
+    }
+    A[i] = a;
+    B[i] = bcoef;
+    M[i] = mid;
+  }
+  __syncthreads();
+  const int j = j0 + threadIdx.x;
+  if (j >= n) return;
+  const double xj = xs[j - X0];
+  double acc = 0.0;
+  for (int t = max(j, p - 1); t <= min(n - 1, j + p - 1); ++t) {
+    const int i = t - T0;
+    acc += A[i] + B[i] * (xj - M[i]);
+  }
+  dx[ro + j] = (float)acc;
+}
+
+ctStatus_t check(const void* a, const void* b, int rows, int n, int p, const char* fn) {
+  if (!a || !b) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: null pointer", fn);
+  if (rows <= 0 || n <= 0 || p <= 0) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: invalid shape or period", fn);
+  if ((n + kWave - 1) / kWave > kMaxChunk)
+    return fail(CT_STATUS_UNSUPPORTED, "%s: rows longer than %d bars are not provided", fn,
+                kWave * kMaxChunk);
+  return CT_STATUS_SUCCESS;
+}
+
+size_t col_bytes(int n, int arrays) {
+  const int C = (n + kWave - 1) / kWave;
+  return (size_t)arrays * kWave * (C + 1) * sizeof(float);
+}
+
+template <class K>
+ctStatus_t allow_lds(K kernel, size_t bytes) {
+  if (bytes > 64 * 1024)
+    TCA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  return CT_STATUS_SUCCESS;
+}
+
+}  // namespace
+}  // namespace tca
+
+using namespace tca;
+
+extern "C" {
+
+CTAPI_EXPORT ctStatus_t ctx_ema_alpha_fwd_d(const float* x, float* y, double* ckpt, int n,
+                                            int rows, int period, double alpha, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    ctStatus_t r = check(x, y, rows, n, period, "ema_alpha");
+    if (r != CT_STATUS_SUCCESS) return r;
+    if (!ckpt) return fail(CT_STATUS_INVALID_ARGUMENT, "ema_alpha: null checkpoint");
+    hipLaunchKernelGGL(k_ema_fwd, dim3(rows), dim3(kWave), 0, s, x, y, ckpt, rows, n, period,
+                       alpha);
+    TCA_HIP(hipGetLastError());
+    return CT_STATUS_SUCCESS;
+  });
+}
+
+CTAPI_EXPORT ctStatus_t ctx_ema_alpha_bwd_d(const float* x, const float* gy, const double* ckpt,
+                                            float* gx, double* galpha, int n, int rows,
+                                            int period, double alpha, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    ctStatus_t r = check(x, gy, rows, n, period, "ema_alpha backward");
+    if (r != CT_STATUS_SUCCESS) return r;
+    if (!ckpt || !gx || !galpha)
+      return fail(CT_STATUS_INVALID_ARGUMENT, "ema_alpha backward: null pointer");
+    const size_t lds = col_bytes(n, 1);
+    r = allow_lds(k_ema_bwd, lds);
+    if (r != CT_STATUS_SUCCESS) return r;
+    hipLaunchKernelGGL(k_ema_bwd, dim3(rows), dim3(kWave), lds, s, x, gy, ckpt, gx, galpha, rows,
+                       n, period, alpha);
+    TCA_HIP(hipGetLastError());
+    return CT_STATUS_SUCCESS;
+  });
+}
+
+CTAPI_EXPORT ctStatus_t ctx_rsi_alpha_fwd_d(const float* x, float* y, double* ckpt, int n,
+                                            int rows, int period, double alpha, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    ctStatus_t r = check(x, y, rows, n, period, "rsi_alpha");
+    if (r != CT_STATUS_SUCCESS) return r;
+    if (!ckpt) return fail(CT_STATUS_INVALID_ARGUMENT, "rsi_alpha: null checkpoint");
+    hipLaunchKernelGGL(k_rsi_fwd, dim3(rows), dim3(kWave), 0, s, x, y, ckpt, rows, n, period,
+                       alpha);
+    TCA_HIP(hipGetLastError());
+    return CT_STATUS_SUCCESS;
+  });
+}
+
+CTAPI_EXPORT ctStatus_t ctx_rsi_alpha_bwd_d(const float* x, const float* gy, const double* ckpt,
+                                            float* gx, double* galpha, int n, int rows,
+                                            int period, double alpha, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    ctStatus_t r = check(x, gy, rows, n, period, "rsi_alpha backward");
+    if (r != CT_STATUS_SUCCESS) return r;
+    if (!ckpt || !gx || !galpha)
+      return fail(CT_STATUS_INVALID_ARGUMENT, "rsi_alpha backward: null pointer");
+    const size_t lds = col_bytes(n, 2);
+    r = allow_lds(k_rsi_bwd, lds);
+    if (r != CT_STATUS_SUCCESS) return r;
+    hipLaunchKernelGGL(k_rsi_bwd, dim3(rows), dim3(kWave), lds, s, x, gy, ckpt, gx, galpha, rows,
+                       n, period, alpha);
+    TCA_HIP(hipGetLastError());
+    return CT_STATUS_SUCCESS;
+  });
+}
+
+CTAPI_EXPORT ctStatus_t ctx_macd_bwd_d(const float* gMacd, const float* gSignal,
+                                       const float* gHist, float* gx, int n, int rows, int fast,
+                                       int slow, int signal, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    ctStatus_t r = check(gMacd, gx, rows, n, 1, "macd backward");
+    if (r != CT_STATUS_SUCCESS) return r;
+    if (!gSignal || !gHist) return fail(CT_STATUS_INVALID_ARGUMENT, "macd backward: null pointer");
+    if (fast <= 0 || slow <= 0 || signal <= 0)
+      return fail(CT_STATUS_INVALID_ARGUMENT, "macd backward: invalid periods");
+    if (slow < fast) std::swap(slow, fast);
+    const size_t lds = col_bytes(n, 1);
+    r = allow_lds(k_macd_bwd, lds);
+    if (r != CT_STATUS_SUCCESS) return r;
+    hipLaunchKernelGGL(k_macd_bwd, dim3(rows), dim3(kWave), lds, s, gMacd, gSignal, gHist, gx,
+                       rows, n, fast, slow, signal);
+    TCA_HIP(hipGetLastError());
+    return CT_STATUS_SUCCESS;
+  });
+}
+
+CTAPI_EXPORT ctStatus_t ctx_bbands_bwd_d(const float* x, const float* gUpper,
+                                         const float* gMiddle, const float* gLower, float* gx,
+                                         int n, int rows, int period, float nbDevUp,
+                                         float nbDevDn, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    if (!x || !gUpper || !gMiddle || !gLower || !gx)
+      return fail(CT_STATUS_INVALID_ARGUMENT, "bbands backward: null pointer");
+    if (rows <= 0 || n <= 0 || period <= 0)
+      return fail(CT_STATUS_INVALID_ARGUMENT, "bbands backward: invalid shape or period");
+    if (period > 4096) return fail(CT_STATUS_UNSUPPORTED, "bbands backward: period above 4096");
+    const size_t lds = (size_t)3 * (256 + period) * sizeof(double) +
+                       (size_t)(256 + 2 * period) * sizeof(float);
+    ctStatus_t r = allow_lds(k_bbands_bwd, lds);
+    if (r != CT_STATUS_SUCCESS) return r;
+    const int tiles = (n + 255) / 256;
+    hipLaunchKernelGGL(k_bbands_bwd, dim3((unsigned)rows * tiles), dim3(256), lds, s, x, gUpper,
+                       gMiddle, gLower, gx, rows, n, period, (double)nbDevUp, (double)nbDevDn);
+    TCA_HIP(hipGetLastError());
+    return CT_STATUS_SUCCESS;
+  });
+}
+
+}  // extern "C"
