@@ -8,6 +8,8 @@
 #ifndef TACUDA_AMD_EXT_H
 #define TACUDA_AMD_EXT_H
 
+#include <stdint.h>
+
 #include "tacuda.h"
 
 #ifdef __cplusplus
@@ -190,6 +192,23 @@ CTAPI_EXPORT ctStatus_t ctx_bbands_bwd_d(const float* x, const float* gUpper,
                                          const float* gMiddle, const float* gLower, float* gx,
                                          int seriesLen, int numSeries, int period,
                                          float nbDevUp, float nbDevDn, hipStream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * Candlestick patterns with int32 signals.  Patterns are numbered 0..60 in
+ * the order ctx_cdl_name() reports (the reference's ct_cdl_<name>_d names).
+ * ctx_cdl_all_batch_d evaluates all 61 in one launch, reading OHLC once and
+ * writing plane k (numSeries x seriesLen int32) at out + k*numSeries*seriesLen.
+ * ------------------------------------------------------------------------- */
+CTAPI_EXPORT int ctx_cdl_count(void);
+CTAPI_EXPORT const char* ctx_cdl_name(int pattern);
+CTAPI_EXPORT ctStatus_t ctx_cdl_batch_d(int pattern, const float* d_open, const float* d_high,
+                                        const float* d_low, const float* d_close,
+                                        int32_t* d_output, int seriesLen, int numSeries,
+                                        hipStream_t stream);
+CTAPI_EXPORT ctStatus_t ctx_cdl_all_batch_d(const float* d_open, const float* d_high,
+                                            const float* d_low, const float* d_close,
+                                            int32_t* d_outputs, int seriesLen, int numSeries,
+                                            hipStream_t stream);
 
 #ifdef __cplusplus
 }

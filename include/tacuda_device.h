@@ -70,6 +70,84 @@ CTAPI_EXPORT ctStatus_t ct_min_d(const float* d_input, float* d_output, int size
 CTAPI_EXPORT ctStatus_t ct_aroon_d(const float* d_high, const float* d_low, float* d_aroonDown, float* d_aroonUp, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
 CTAPI_EXPORT ctStatus_t ct_aroonosc_d(const float* d_high, const float* d_low, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
 
+/* The 61 TA-Lib candlestick patterns (tacuda_device.h:438+, tacuda.h:949+):
+ * legacy float signals {0, +-80, +-100, +-200}, 0 during each pattern's
+ * TA-Lib look-back.  ct_cdl_<name>_batch_d runs numSeries series-major rows
+ * (maxConcurrency is accepted and unused).  int32 outputs and the all-61
+ * launch: ctx_cdl_batch_d / ctx_cdl_all_batch_d (tacuda_amd.h). */
+#define CT_CDL_DECL(name)                                                                      \
+  CTAPI_EXPORT ctStatus_t ct_cdl_##name##_d(const float* d_open, const float* d_high,         \
+                                            const float* d_low, const float* d_close,          \
+                                            float* d_output, int size,                         \
+                                            hipStream_t stream CTAPI_STREAM_DEFAULT);          \
+  CTAPI_EXPORT ctStatus_t ct_cdl_##name##_batch_d(const float* d_opens, const float* d_highs,  \
+                                                  const float* d_lows, const float* d_closes,  \
+                                                  float* d_outputs, int seriesLen,             \
+                                                  int numSeries, int maxConcurrency,           \
+                                                  hipStream_t stream CTAPI_STREAM_DEFAULT);
+CT_CDL_DECL(two_crows)
+CT_CDL_DECL(three_black_crows)
+CT_CDL_DECL(three_inside)
+CT_CDL_DECL(three_line_strike)
+CT_CDL_DECL(three_outside)
+CT_CDL_DECL(three_stars_in_south)
+CT_CDL_DECL(three_white_soldiers)
+CT_CDL_DECL(abandoned_baby)
+CT_CDL_DECL(advance_block)
+CT_CDL_DECL(belt_hold)
+CT_CDL_DECL(breakaway)
+CT_CDL_DECL(closing_marubozu)
+CT_CDL_DECL(conceal_baby_swallow)
+CT_CDL_DECL(counterattack)
+CT_CDL_DECL(dark_cloud_cover)
+CT_CDL_DECL(doji)
+CT_CDL_DECL(doji_star)
+CT_CDL_DECL(dragonfly_doji)
+CT_CDL_DECL(engulfing)
+CT_CDL_DECL(evening_doji_star)
+CT_CDL_DECL(evening_star)
+CT_CDL_DECL(gap_side_side_white)
+CT_CDL_DECL(gravestone_doji)
+CT_CDL_DECL(hammer)
+CT_CDL_DECL(hanging_man)
+CT_CDL_DECL(harami)
+CT_CDL_DECL(harami_cross)
+CT_CDL_DECL(high_wave)
+CT_CDL_DECL(hikkake)
+CT_CDL_DECL(hikkake_mod)
+CT_CDL_DECL(homing_pigeon)
+CT_CDL_DECL(identical_three_crows)
+CT_CDL_DECL(in_neck)
+CT_CDL_DECL(inverted_hammer)
+CT_CDL_DECL(kicking)
+CT_CDL_DECL(kicking_by_length)
+CT_CDL_DECL(ladder_bottom)
+CT_CDL_DECL(long_legged_doji)
+CT_CDL_DECL(long_line)
+CT_CDL_DECL(marubozu)
+CT_CDL_DECL(matching_low)
+CT_CDL_DECL(mat_hold)
+CT_CDL_DECL(morning_doji_star)
+CT_CDL_DECL(morning_star)
+CT_CDL_DECL(on_neck)
+CT_CDL_DECL(piercing)
+CT_CDL_DECL(rickshaw_man)
+CT_CDL_DECL(rise_fall3_methods)
+CT_CDL_DECL(separating_lines)
+CT_CDL_DECL(shooting_star)
+CT_CDL_DECL(short_line)
+CT_CDL_DECL(spinning_top)
+CT_CDL_DECL(stalled_pattern)
+CT_CDL_DECL(stick_sandwich)
+CT_CDL_DECL(takuri)
+CT_CDL_DECL(tasuki_gap)
+CT_CDL_DECL(thrusting)
+CT_CDL_DECL(tristar)
+CT_CDL_DECL(unique_3_river)
+CT_CDL_DECL(upside_gap_2_crows)
+CT_CDL_DECL(xside_gap_3_methods)
+#undef CT_CDL_DECL
+
 #ifdef __cplusplus
 }
 #endif

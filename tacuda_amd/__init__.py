@@ -159,6 +159,11 @@ _sig("ct_stream_batch_series", c_void_p, ctypes.POINTER(c_int), stream=False)
 _sig("ct_stream_batch_bars", c_void_p, ctypes.POINTER(ctypes.c_longlong), stream=False)
 _sig("ct_stream_batch_set_exec", c_void_p, c_int, stream=False)
 _sig("ctx_stream_batch_append_d", c_void_p, P, P, P, P, c_int, P, P, P)
+# candlestick patterns (include/tacuda_device.h, include/tacuda_amd.h)
+_sig("ctx_cdl_count", stream=False)
+_sig("ctx_cdl_name", c_int, stream=False, restype=ctypes.c_char_p)
+_sig("ctx_cdl_batch_d", c_int, P, P, P, P, P, c_int, c_int)
+_sig("ctx_cdl_all_batch_d", P, P, P, P, P, c_int, c_int)
 # differentiable indicators (include/tacuda_amd.h)
 _D = ctypes.c_double
 for _n in ("ema", "rsi"):

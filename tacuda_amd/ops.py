@@ -9,7 +9,11 @@ from __future__ import annotations
 
 import torch
 
+import ctypes
+
 from . import PACK_OUTPUTS, PackOutputs, PackParams, call
+
+P = ctypes.c_void_p
 
 _ONE_IN = ("sma", "ema", "wma", "dema", "tema", "rsi", "stddev", "var", "sum", "max", "min")
 _HLC = ("natr", "adx", "dx", "plus_di", "minus_di", "willr", "cci")
@@ -321,3 +325,56 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+def cdl_names():
+    """The 61 TA-Lib pattern names, in the library's pattern order."""
+    from . import lib
+
+    L = lib()
+    return [L.ctx_cdl_name(k).decode() for k in range(L.ctx_cdl_count())]
+
+
+def cdl(name, open_, high, low, close):
+    """int32 signal of one pattern (ctx_cdl_batch_d); 1-D or [series, bars]."""
+    _check(open_, high, low, close)
+    k = cdl_names().index(name)
+    n, rows = _shape(close)
+    out = torch.empty(close.shape, dtype=torch.int32, device=close.device)
+    call("ctx_cdl_batch_d", k, open_.data_ptr(), high.data_ptr(), low.data_ptr(),
+         close.data_ptr(), out.data_ptr(), n, rows, _stream())
+    return out
+
+
+def cdl_all(open_, high, low, close):
+    """All 61 patterns in one launch: int32 tensor [61, *close.shape]."""
+    _check(open_, high, low, close)
+    n, rows = _shape(close)
+    out = torch.empty((61,) + tuple(close.shape), dtype=torch.int32, device=close.device)
+    call("ctx_cdl_all_batch_d", open_.data_ptr(), high.data_ptr(), low.data_ptr(),
+         close.data_ptr(), out.data_ptr(), n, rows, _stream())
+    return out
+
+
+def cdl_legacy(name, open_, high, low, close):
+    """The reference's float-signal call ct_cdl_<name>_d / _batch_d."""
+    _check(open_, high, low, close)
+    from . import lib
+
+    L = lib()
+    n, rows = _shape(close)
+    out = torch.empty_like(close)
+    args = [open_.data_ptr(), high.data_ptr(), low.data_ptr(), close.data_ptr(), out.data_ptr()]
+    if close.dim() == 1:
+        fn = getattr(L, f"ct_cdl_{name}_d")
+        fn.argtypes = [P] * 5 + [ctypes.c_int, ctypes.c_void_p]
+        rc = fn(*args, n, _stream())
+    else:
+        fn = getattr(L, f"ct_cdl_{name}_batch_d")
+        fn.argtypes = [P] * 5 + [ctypes.c_int] * 3 + [ctypes.c_void_p]
+        rc = fn(*args, n, rows, 0, _stream())
+    if rc != 0:
+        from . import TacudaError, last_error
+
+        raise TacudaError(f"ct_cdl_{name}", rc, last_error())
+    return out

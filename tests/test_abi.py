@@ -28,6 +28,8 @@ def _declared():
             names.update({f"ct_{m.group(1)}_batch", f"ct_{m.group(1)}_batch_d"})
         for m in re.finditer(r"CT_BATCH_HLC\((\w+)\)", text):
             names.update({f"ct_{m.group(1)}_batch", f"ct_{m.group(1)}_batch_d"})
+        for m in re.finditer(r"CT_CDL_DECL\((\w+)\)", text):
+            names.update({f"ct_cdl_{m.group(1)}_d", f"ct_cdl_{m.group(1)}_batch_d"})
         # ragged macros: CTX_RAGGED1(name) and CTX_RAGGED_HLC(name)
         for m in re.finditer(r"CTX_RAGGED(?:1|_HLC)\((\w+)\)", text):
             names.add(f"ctx_{m.group(1)}_ragged_d")
