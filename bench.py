@@ -59,7 +59,8 @@ def _args():
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="time budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", default="", help="comma list of workloads: pipeline7,batch13,scan256")
+    ap.add_argument("--only", default="",
+                    help="comma list of workloads: pipeline7,batch13,scan256,ragged,ticks,autograd")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="also run a plain device copy of this many MB (counter calibration)")
     return ap.parse_args()
@@ -227,6 +228,152 @@ def _scan256(a, torch, ops, T, dev):
             "shape": [rows, n], "kernels": per}
 
 
+def _ragged(a, torch, ops, T, dev):
+    """4096 series, lengths uniform in [64, 8192] (seed 1), the 13-indicator
+    fused pass through a tuned, graph-captured plan."""
+    import numpy as np
+
+    import tacuda_amd as ta
+
+    lengths = np.random.default_rng(1).integers(64, 8193, size=4096)
+    off = np.concatenate([[0], np.cumsum(lengths)]).astype(np.int64)
+    d = ta.synth_ohlcv(int(lengths.max()), rows=len(lengths), seed=0)
+    cat = {k: np.concatenate([d[k][r, :m] for r, m in enumerate(lengths)]) for k in
+           ("high", "low", "close")}
+    del d
+    Tn = [torch.from_numpy(cat[k]).to(dev) for k in ("high", "low", "close")]
+    lay = ops.Ragged(off)
+    with torch.cuda.stream(T.stream):
+        plan = ops.Plan(*Tn, layout=lay, tune=True)
+        info = plan.info()
+        for _ in range(3):
+            plan.launch()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        T.stream.synchronize()
+        reps = max(5, a.steps // 5)
+        e0.record(T.stream)
+        for _ in range(reps):
+            plan.launch()
+        e1.record(T.stream)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    bars = int(lengths.sum())
+    alg = (3 + 18) * 4 * bars
+    print(f"ragged 4096 x U[64,8192]: {ms * 1e3:.1f} us/pass {info}", file=sys.stderr, flush=True)
+    plan.close()
+    lay.close()
+    return {"ms": ms, "bars": bars, "bars_per_s": bars / (ms * 1e-3), "alg_bytes": alg,
+            "GBs": _gbs(alg, ms), "frac_hbm": _gbs(alg, ms) / PEAK_HBM_GBS, "plan": info}
+
+
+def _ticks(a, torch, dev):
+    """Streaming tick: 65,536 symbols' EMA(20), RSI(14) and ATR(14) states
+    advanced one bar per tick (three batch handles, GPU execution)."""
+    import numpy as np
+
+    import tacuda_amd as ta
+    from tacuda_amd.stream import StreamBatch
+
+    S, hist, ticks = 65536, 64, 200
+    d = ta.synth_ohlcv(hist + ticks, rows=S, seed=0)
+    bs = [StreamBatch("ema", S, 20, exec="gpu"), StreamBatch("rsi", S, 14, exec="gpu"),
+          StreamBatch("atr", S, 14, exec="gpu")]
+
+    def step(b, sl):
+        kw = dict(high=np.ascontiguousarray(d["high"][:, sl]),
+                  low=np.ascontiguousarray(d["low"][:, sl]),
+                  close=np.ascontiguousarray(d["close"][:, sl]))
+        return b.append(**({"close": kw["close"]} if b.family != "atr" else kw))
+
+    for b in bs:
+        step(b, slice(0, hist))
+    lat = []
+    for t in range(hist, hist + ticks):
+        t0 = time.perf_counter()
+        for b in bs:
+            step(b, slice(t, t + 1))
+        lat.append(time.perf_counter() - t0)
+    lat = np.array(lat) * 1e6
+    # the same ticks from device memory (states stay in HBM, one sync per tick)
+    dd = {k: torch.from_numpy(np.ascontiguousarray(d[k][:, hist:])).to(dev)
+          for k in ("high", "low", "close")}
+    cols = [{k: dd[k][:, i].contiguous() for k in dd} for i in range(ticks)]
+    outs = [torch.empty(S, device=dev) for _ in range(3)]
+    dlat = []
+    for b in bs:
+        b.reset()
+    for i in range(ticks):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        bs[0].append_d(close=cols[i]["close"], outs=[outs[0]])
+        bs[1].append_d(close=cols[i]["close"], outs=[outs[1]])
+        bs[2].append_d(high=cols[i]["high"], low=cols[i]["low"], close=cols[i]["close"],
+                       outs=[outs[2]])
+        torch.cuda.synchronize()
+        dlat.append(time.perf_counter() - t0)
+    dlat = np.array(dlat[5:]) * 1e6
+    for b in bs:
+        b.close()
+    res = {"symbols": S, "indicators": ["EMA20", "RSI14", "ATR14"], "ticks": ticks,
+           "host_api_us_p50": float(np.percentile(lat, 50)),
+           "host_api_us_p99": float(np.percentile(lat, 99)),
+           "host_api_ticks_per_s": float(1e6 / np.median(lat)),
+           "device_api_us_p50": float(np.percentile(dlat, 50)),
+           "device_api_us_p99": float(np.percentile(dlat, 99)),
+           "device_api_ticks_per_s": float(1e6 / np.median(dlat))}
+    print(f"ticks 65536 x (EMA,RSI,ATR): host p50 {res['host_api_us_p50']:.0f} us, "
+          f"device p50 {res['device_api_us_p50']:.0f} us", file=sys.stderr, flush=True)
+    return res
+
+
+def _autograd_ratio(a, torch, dev):
+    """1024 x 8192 close: EMA(alpha), MACD(12,26,9), RSI(alpha, 14),
+    BBANDS(20, 2); loss = sum(out * w), w ~ N(0,1) seed 2.  Forward-only
+    time vs forward+backward time."""
+    import tacuda_amd as ta
+    from tacuda_amd import autograd as ag
+
+    rows, n = 1024, 8192
+    c = torch.from_numpy(ta.synth_ohlcv(n, rows=rows, seed=0)["close"]).to(dev)
+    g = torch.Generator(device=dev)
+    g.manual_seed(2)
+    w = torch.randn(8, rows, n, device=dev, generator=g)
+    ae = torch.tensor(2.0 / 21.0, device=dev, requires_grad=True)
+    ar = torch.tensor(1.0 / 14.0, device=dev, requires_grad=True)
+
+    def loss_of(x):
+        outs = [ag.ema(x, ae, 20), *ag.macd(x, 12, 26, 9), ag.rsi(x, ar, 14),
+                *ag.bbands(x, 20, 2.0, 2.0)]
+        return sum((torch.nan_to_num(o, nan=0.0) * w[i]).sum() for i, o in enumerate(outs))
+
+    def run(backward):
+        x = c.detach().requires_grad_(backward)
+        if backward:
+            loss_of(x).backward()
+        else:
+            with torch.no_grad():
+                loss_of(x)
+
+    res = {}
+    for name, bw in (("fwd", False), ("fwd_bwd", True)):
+        for _ in range(3):
+            run(bw)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = 10
+        e0.record()
+        for _ in range(reps):
+            run(bw)
+        e1.record()
+        e1.synchronize()
+        res[name + "_ms"] = e0.elapsed_time(e1) / reps
+    res["ratio"] = res["fwd_bwd_ms"] / res["fwd_ms"]
+    res["shape"] = [rows, n]
+    print(f"autograd 1024x8192: fwd {res['fwd_ms']:.3f} ms, fwd+bwd {res['fwd_bwd_ms']:.3f} ms, "
+          f"ratio {res['ratio']:.2f}", file=sys.stderr, flush=True)
+    return res
+
+
 def main():
     a = _args()
     if a.gpus != 1:
@@ -238,7 +385,8 @@ def main():
     from tacuda_amd import ops
 
     dev = torch.device("cuda:0")
-    only = set(filter(None, a.only.split(","))) or {"pipeline7", "batch13", "scan256"}
+    only = set(filter(None, a.only.split(","))) or {"pipeline7", "batch13", "scan256", "ragged",
+                                                   "ticks", "autograd"}
     T = Timer(torch)
     work = {}
     d = p7 = None
@@ -250,6 +398,14 @@ def main():
     torch.cuda.empty_cache()
     if "scan256" in only:
         work["scan256_ema200_kama30_t3_sma200"] = _scan256(a, torch, ops, T, dev)
+    if "ragged" in only:
+        work["ragged_4096_U64_8192"] = _ragged(a, torch, ops, T, dev)
+        torch.cuda.empty_cache()
+    if "ticks" in only:
+        work["stream_ticks_65536"] = _ticks(a, torch, dev)
+    if "autograd" in only:
+        work["autograd_1024x8192"] = _autograd_ratio(a, torch, dev)
+        torch.cuda.empty_cache()
     if a.copy_mb:
         x = torch.empty(a.copy_mb << 18, dtype=torch.float32, device=dev)
         y = torch.empty_like(x)
