@@ -340,3 +340,26 @@ def test_period_one_atr_and_natr_are_trange():
     atr = ORC.atr(h, l, c, 1)
     assert np.isnan(atr[100]) and np.isfinite(atr[101:]).all()
     assert np.array_equal(atr, tr, equal_nan=True)
+
+
+def test_obv_and_ad_brute_force():
+    """OBV / AD (reference OBV.cu / AD.cu semantics): numpy cumulative sums
+    in float64, independent of the C loop."""
+    import tacuda_amd as ta
+
+    d = ta.synth_ohlcv(3000, seed=21)
+    h, l, c, v = (d[k].astype(np.float64) for k in ("high", "low", "close", "volume"))
+    sgn = np.sign(np.diff(c))
+    obv = np.concatenate([[v[0]], v[0] + np.cumsum(sgn * v[1:])])
+    assert np.allclose(ORC.obv(d["close"], d["volume"]), obv, rtol=1e-6)
+    rng = h - l
+    term = np.where(rng > 0, ((c - l) - (h - c)) / np.where(rng > 0, rng, 1) * v, 0.0)
+    assert np.allclose(ORC.ad(d["high"], d["low"], d["close"], d["volume"]), np.cumsum(term),
+                       rtol=1e-6, atol=1e-3)
+    # non-finite bars contribute nothing to AD
+    h2 = d["high"].copy()
+    h2[100] = np.nan
+    a1 = ORC.ad(h2, d["low"], d["close"], d["volume"])
+    t2 = term.copy()
+    t2[100] = 0.0
+    assert np.allclose(a1, np.cumsum(t2), rtol=1e-6, atol=1e-3)
