@@ -43,6 +43,11 @@ struct PackCfg {
   int W;  // segments (waves) per workgroup
 };
 
+// Tiles of the two recurrence kernels, chosen independently.
+struct PackTiles {
+  PackCfg close, hlc;
+};
+
 struct PackIO {
   const float* h;
   const float* l;
@@ -859,9 +864,10 @@ static ctStatus_t launch_windows(int len, int rows, hipStream_t s, const PackP& 
   return launch_win_h<kPackMaxWindow, 1>(len, rows, s, prm, io);
 }
 
-// Recurrence kernels of one tile configuration: K bars per lane (the scan
-// chunk), W waves (segments) per workgroup.
-template <int K, int W>
+// One recurrence kernel (PART 1 close, 2 high/low/close) of one tile
+// configuration: K bars per lane (the scan chunk), W waves (segments) per
+// workgroup.
+template <int K, int W, int PART>
 static ctStatus_t launch_rec(int len, int rows, hipStream_t s, const PackP& prm,
                              const PackIO& io) {
   constexpr int SEG = K * kWave;
@@ -879,31 +885,20 @@ static ctStatus_t launch_rec(int len, int rows, hipStream_t s, const PackP& prm,
     ctStatus_t st = workspace_for(s, bytes, &ws);
     if (st != CT_STATUS_SUCCESS) return st;
   }
-  if (prm.mask & kCloseRecMask) {
-    hipLaunchKernelGGL((k_pack<K, K, W, 1>), dim3(blocks), dim3(W * kWave), 0, s, g, ws,
-                       useLB ? 1 : 0, prm, io);
-    TCA_HIP(hipGetLastError());
-  }
-  if (prm.mask & kHlcRecMask) {
-    hipLaunchKernelGGL((k_pack<K, K, W, 2>), dim3(blocks), dim3(W * kWave), 0, s, g, ws,
-                       useLB ? 1 : 0, prm, io);
-    TCA_HIP(hipGetLastError());
-  }
+  hipLaunchKernelGGL((k_pack<K, K, W, PART>), dim3(blocks), dim3(W * kWave), 0, s, g, ws,
+                     useLB ? 1 : 0, prm, io);
+  TCA_HIP(hipGetLastError());
   return CT_STATUS_SUCCESS;
 }
 
 // Tile configurations the autotuner chooses from.
 const PackCfg kPackCfgs[] = {{4, 4}, {4, 2}, {4, 1}, {4, 8}, {8, 2}, {8, 1}, {8, 4}};
 
-ctStatus_t pack_launch(int len, int rows, hipStream_t s, const PackP& p, const PackIO& io,
-                       PackCfg cfg) {
-  if (p.mask & kWinMask) {
-    ctStatus_t st = launch_windows(len, rows, s, p, io);
-    if (st != CT_STATUS_SUCCESS) return st;
-  }
-  if (!(p.mask & ~kWinMask)) return CT_STATUS_SUCCESS;
+template <int PART>
+static ctStatus_t launch_part(PackCfg cfg, int len, int rows, hipStream_t s, const PackP& p,
+                              const PackIO& io) {
 #define TCA_CFG(KK, WW) \
-  if (cfg.K == KK && cfg.W == WW) return launch_rec<KK, WW>(len, rows, s, p, io);
+  if (cfg.K == KK && cfg.W == WW) return launch_rec<KK, WW, PART>(len, rows, s, p, io);
   TCA_CFG(4, 4)
   TCA_CFG(4, 2)
   TCA_CFG(4, 1)
@@ -915,10 +910,35 @@ ctStatus_t pack_launch(int len, int rows, hipStream_t s, const PackP& p, const P
   return fail(CT_STATUS_INVALID_ARGUMENT, "pack: unknown tile configuration %dx%d", cfg.K, cfg.W);
 }
 
-// Dense or single-series pack with the default tile (4 bars per lane, one
-// wave covers a 256-bar row; longer rows chain segments by look-back).
+// The fused pass: the window kernel and the two recurrence kernels read the
+// same inputs and write disjoint outputs.  They run in stream order: each
+// fills the GPU on its own and is bound by its vector ALU work, so running
+// them concurrently gains nothing (measured).
+ctStatus_t pack_launch(int len, int rows, hipStream_t s, const PackP& p, const PackIO& io,
+                       PackTiles t) {
+  ctStatus_t st = CT_STATUS_SUCCESS;
+  if (p.mask & kWinMask) st = launch_windows(len, rows, s, p, io);
+  if (st == CT_STATUS_SUCCESS && (p.mask & kCloseRecMask))
+    st = launch_part<1>(t.close, len, rows, s, p, io);
+  if (st == CT_STATUS_SUCCESS && (p.mask & kHlcRecMask))
+    st = launch_part<2>(t.hlc, len, rows, s, p, io);
+  return st;
+}
+
+// Default tiles by row length: rows that fit one 256-bar wave use 4 bars per
+// lane (most waves for short rows); longer rows use 8 bars per lane, which
+// halves the look-back segments -- one wave per workgroup for the close
+// kernel (fewest waves waiting on one workgroup's slowest segment), four
+// for the high/low/close kernel.
+PackTiles default_tiles(int len) {
+  if (len <= 4 * kWave) return PackTiles{{4, 4}, {4, 4}};
+  return PackTiles{{8, 1}, {8, 4}};
+}
+
+// Dense, single-series or ragged (maximum row length) pack with the default
+// tiles.
 ctStatus_t pack_dense(int len, int rows, hipStream_t s, const PackP& p, const PackIO& io) {
-  return pack_launch(len, rows, s, p, io, PackCfg{4, 4});
+  return pack_launch(len, rows, s, p, io, default_tiles(len));
 }
 
 }  // namespace tca
@@ -1059,13 +1079,14 @@ struct ctxPlan_st {
   hipStream_t stream = nullptr;  // private stream the graph was captured on
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  PackCfg cfg{4, 4};
+  PackTiles tiles{{4, 4}, {4, 4}};
+  unsigned mask = 0;
   float tunedUs = 0.0f, defaultUs = 0.0f;
 };
 
 namespace {
 
-// Tuned tile per shape class, so a second plan of the same shape skips the
+// Tuned tiles per shape class, so a second plan of the same shape skips the
 // measurement.
 struct TuneKey {
   int device, rows, len, ragged;
@@ -1075,14 +1096,18 @@ struct TuneKey {
            std::tie(o.device, o.rows, o.len, o.ragged, o.mask);
   }
 };
+struct Tuned {
+  PackTiles tiles;
+  float tunedUs, defaultUs;
+};
 std::mutex g_tune_mu;
-std::map<TuneKey, std::pair<PackCfg, std::pair<float, float>>> g_tuned;
+std::map<TuneKey, Tuned> g_tuned;
 
-// Average microseconds of `reps` passes with tile `cfg` on `st`.
-ctStatus_t time_cfg(int len, int rows, hipStream_t st, const PackP& P, const PackIO& io,
-                    PackCfg cfg, int reps, float* us) {
+// Average microseconds of `reps` passes with tiles `t` on `st`.
+ctStatus_t time_tiles(int len, int rows, hipStream_t st, const PackP& P, const PackIO& io,
+                      PackTiles t, int reps, float* us) {
   for (int i = 0; i < 2; ++i) {
-    ctStatus_t r = pack_launch(len, rows, st, P, io, cfg);
+    ctStatus_t r = pack_launch(len, rows, st, P, io, t);
     if (r != CT_STATUS_SUCCESS) return r;
   }
   hipEvent_t e0, e1;
@@ -1090,7 +1115,7 @@ ctStatus_t time_cfg(int len, int rows, hipStream_t st, const PackP& P, const Pac
   TCA_HIP(hipEventCreate(&e1));
   ctStatus_t r = CT_STATUS_SUCCESS;
   (void)hipEventRecord(e0, st);
-  for (int i = 0; i < reps && r == CT_STATUS_SUCCESS; ++i) r = pack_launch(len, rows, st, P, io, cfg);
+  for (int i = 0; i < reps && r == CT_STATUS_SUCCESS; ++i) r = pack_launch(len, rows, st, P, io, t);
   (void)hipEventRecord(e1, st);
   hipError_t e = hipEventSynchronize(e1);
   float ms = 0.0f;
@@ -1100,6 +1125,25 @@ ctStatus_t time_cfg(int len, int rows, hipStream_t st, const PackP& P, const Pac
   if (r != CT_STATUS_SUCCESS) return r;
   if (e != hipSuccess) return hip_fail(e, "plan: timing");
   *us = ms * 1000.0f / reps;
+  return CT_STATUS_SUCCESS;
+}
+
+// Best tile of one recurrence kernel (outputs `part` of P), each candidate
+// timed alone.
+ctStatus_t tune_part(int len, int rows, hipStream_t st, const PackP& P, const PackIO& io,
+                     unsigned part, PackCfg* best) {
+  PackP Q = P;
+  Q.mask = P.mask & part;
+  float bestUs = 0.0f;
+  for (const PackCfg& cf : kPackCfgs) {
+    float us = 0.0f;
+    ctStatus_t r = time_tiles(len, rows, st, Q, io, PackTiles{cf, cf}, 10, &us);
+    if (r != CT_STATUS_SUCCESS) return r;
+    if (bestUs == 0.0f || us < bestUs) {
+      bestUs = us;
+      *best = cf;
+    }
+  }
   return CT_STATUS_SUCCESS;
 }
 
@@ -1147,42 +1191,46 @@ CTAPI_EXPORT ctStatus_t ctx_pack_plan_create(const float* h, const float* l, con
     (void)hipGetDevice(&dev);
     const TuneKey key{dev, rows, rg ? (int)(rg->total & 0x7fffffff) : len, rg ? 1 : 0, P.mask};
     bool cached = false;
+    const PackTiles dflt = default_tiles(len);
+    p->tiles = dflt;
+    p->mask = P.mask;
     {
       std::lock_guard<std::mutex> lk(g_tune_mu);
       auto it = g_tuned.find(key);
       if (it != g_tuned.end()) {
-        p->cfg = it->second.first;
-        p->tunedUs = it->second.second.first;
-        p->defaultUs = it->second.second.second;
+        p->tiles = it->second.tiles;
+        p->tunedUs = it->second.tunedUs;
+        p->defaultUs = it->second.defaultUs;
         cached = true;
       }
     }
     // every candidate runs eagerly before capture: each reserves the
     // look-back workspace its tile needs on the plan's stream
     if (tune && !cached) {
-      float best = 0.0f;
-      for (const PackCfg& cf : kPackCfgs) {
-        float us = 0.0f;
-        ctStatus_t r = time_cfg(len, rows, p->stream, P, io, cf, 10, &us);
-        if (r != CT_STATUS_SUCCESS) return bail(r);
-        if (cf.K == 4 && cf.W == 4) p->defaultUs = us;
-        if (best == 0.0f || us < best) {
-          best = us;
-          p->cfg = cf;
-        }
-      }
-      p->tunedUs = best;
+      PackTiles t = dflt;
+      ctStatus_t r = CT_STATUS_SUCCESS;
+      if (P.mask & kCloseRecMask) r = tune_part(len, rows, p->stream, P, io, kCloseRecMask, &t.close);
+      if (r == CT_STATUS_SUCCESS && (P.mask & kHlcRecMask))
+        r = tune_part(len, rows, p->stream, P, io, kHlcRecMask, &t.hlc);
+      float tu = 0.0f, du = 0.0f;
+      if (r == CT_STATUS_SUCCESS) r = time_tiles(len, rows, p->stream, P, io, t, 10, &tu);
+      if (r == CT_STATUS_SUCCESS) r = time_tiles(len, rows, p->stream, P, io, dflt, 10, &du);
+      if (r != CT_STATUS_SUCCESS) return bail(r);
+      // the whole pass decides, against the default tiles
+      p->tiles = tu <= du ? t : dflt;
+      p->tunedUs = tu <= du ? tu : du;
+      p->defaultUs = du;
       std::lock_guard<std::mutex> lk(g_tune_mu);
-      g_tuned[key] = {p->cfg, {p->tunedUs, p->defaultUs}};
+      g_tuned[key] = Tuned{p->tiles, p->tunedUs, p->defaultUs};
     } else {
       float us = 0.0f;
-      ctStatus_t r = time_cfg(len, rows, p->stream, P, io, p->cfg, 1, &us);
+      ctStatus_t r = time_tiles(len, rows, p->stream, P, io, p->tiles, 1, &us);
       if (r != CT_STATUS_SUCCESS) return bail(r);
       if (!cached) p->tunedUs = p->defaultUs = us;
     }
     e = hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return bail(hip_fail(e, "plan: begin capture"));
-    ctStatus_t r = pack_launch(len, rows, p->stream, P, io, p->cfg);
+    ctStatus_t r = pack_launch(len, rows, p->stream, P, io, p->tiles);
     e = hipStreamEndCapture(p->stream, &p->graph);
     if (r != CT_STATUS_SUCCESS) return bail(r);
     if (e != hipSuccess) return bail(hip_fail(e, "plan: end capture"));
@@ -1205,8 +1253,11 @@ CTAPI_EXPORT ctStatus_t ctx_plan_info(ctxPlan plan, int* barsPerLane, int* waves
                                       float* tunedUs, float* defaultUs) {
   return guarded([&]() -> ctStatus_t {
     if (!plan) return fail(CT_STATUS_INVALID_ARGUMENT, "plan: null plan");
-    if (barsPerLane) *barsPerLane = plan->cfg.K;
-    if (wavesPerGroup) *wavesPerGroup = plan->cfg.W;
+    // the close kernel's tile, or the high/low/close kernel's when the
+    // plan has no close recurrences
+    const PackCfg& t = (plan->mask & kCloseRecMask) ? plan->tiles.close : plan->tiles.hlc;
+    if (barsPerLane) *barsPerLane = t.K;
+    if (wavesPerGroup) *wavesPerGroup = t.W;
     if (tunedUs) *tunedUs = plan->tunedUs;
     if (defaultUs) *defaultUs = plan->defaultUs;
     return CT_STATUS_SUCCESS;

@@ -90,3 +90,35 @@ def test_plan_over_ragged_layout(dev):
             compare(t.cpu().numpy(), want, tol=1e-5, min_exact=0.999, name=name)
     plan.close()
     lay.close()
+
+
+@pytest.mark.parametrize("shape", [(300, 234), (1, 100000), (3, 40000)])
+def test_user_graph_capture_of_the_fused_pass(dev, shape):
+    """ctx_pack_batch_d captured by the caller on a stream whose look-back
+    workspace an eager call reserved: every replay equals the eager pass bit
+    for bit, with eager work and cleared outputs between replays."""
+    import tacuda_amd as ta
+    from tests.test_gpu_pack import DEFAULT
+
+    rows, n = shape
+    print(f"user graph {rows}x{n}", flush=True)
+    d = ta.synth_ohlcv(n, rows=rows, seed=7)
+    T = [torch.from_numpy(np.ascontiguousarray(d[k])).to(dev) for k in ("high", "low", "close")]
+    T = [t if rows > 1 else t.reshape(-1) for t in T]
+    st = torch.cuda.Stream()
+    with torch.cuda.stream(st):
+        eager = {k: v.clone() for k, v in ta.ops.pack(*T, **DEFAULT).items()}
+    st.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=st):
+        cap = ta.ops.pack(*T, **DEFAULT)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        junk = torch.randn(1 << 20, device=dev).sum()
+        for v in cap.values():
+            v.fill_(0.0)
+        g.replay()
+        torch.cuda.synchronize()
+        for k in eager:
+            assert torch.equal(cap[k].view(torch.int32), eager[k].view(torch.int32)), k
+    del junk, g
