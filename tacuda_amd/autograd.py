@@ -9,7 +9,9 @@ ctx_bbands_bwd_d).  Inputs: float32 CUDA tensors of shape [series, bars]
   bbands(x, period, up, dn)    Bollinger bands (upper, middle, lower)
 
 Backward passes save only block-boundary carries of the recurrences
-(64 per series) and recompute the rest on the GPU.
+(64 per series) and recompute the rest on the GPU.  The adjoint kernels
+never read the incoming gradient at warm-up bars (where the outputs are
+NaN), so a loss that leaves NaN gradients there needs no masking.
 """
 from __future__ import annotations
 
@@ -58,7 +60,6 @@ class _AlphaRecurrence(torch.autograd.Function):
         x2, ck = ctx.saved_tensors
         a, period, squeeze, alpha_is_tensor = ctx.meta
         g2 = (gy.unsqueeze(0) if squeeze else gy).contiguous().float()
-        g2 = torch.nan_to_num(g2, nan=0.0)
         n, rows = _shape(x2)
         gx = torch.empty_like(x2)
         ga = torch.empty(rows, dtype=torch.float64, device=x2.device)
@@ -100,7 +101,7 @@ class Macd(torch.autograd.Function):
             if g is None:
                 return torch.zeros(shape, dtype=torch.float32, device=dev)
             g = g.unsqueeze(0) if squeeze else g
-            return torch.nan_to_num(g.contiguous().float(), nan=0.0)
+            return g.contiguous().float()
 
         gm2, gs2, gh2 = prep(gm), prep(gs), prep(gh)
         gx = torch.empty(shape, dtype=torch.float32, device=dev)
@@ -132,7 +133,7 @@ class Bbands(torch.autograd.Function):
             if g is None:
                 return torch.zeros_like(x2)
             g = g.unsqueeze(0) if squeeze else g
-            return torch.nan_to_num(g.contiguous().float(), nan=0.0)
+            return g.contiguous().float()
 
         gu2, gm2, gl2 = prep(gu), prep(gm), prep(gl)
         gx = torch.empty_like(x2)

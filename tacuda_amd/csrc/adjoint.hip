@@ -87,6 +87,64 @@ __device__ __forceinline__ double wsum(double v) {
   return t[0];
 }
 
+// Lanes walk their chunks in blocks of kB bars: every load of a block is
+// issued before the block's arithmetic (one memory round trip per kB bars,
+// not per bar), as two float4 accesses when the block lies inside [lo, hi)
+// and is 16-byte aligned.  Bars outside [lo, hi) read as 0 and are not
+// written.
+constexpr int kB = 8;
+
+__device__ __forceinline__ bool whole8(const float* r, int t, int lo, int hi) {
+  return t >= lo && t + kB <= hi && ((reinterpret_cast<size_t>(r + t) & 15) == 0);
+}
+
+__device__ __forceinline__ void ld8(const float* r, int t, int lo, int hi, float (&v)[kB]) {
+  if (whole8(r, t, lo, hi)) {
+    const float4 a = *reinterpret_cast<const float4*>(r + t);
+    const float4 b = *reinterpret_cast<const float4*>(r + t + 4);
+    v[0] = a.x;
+    v[1] = a.y;
+    v[2] = a.z;
+    v[3] = a.w;
+    v[4] = b.x;
+    v[5] = b.y;
+    v[6] = b.z;
+    v[7] = b.w;
+    return;
+  }
+#pragma unroll
+  for (int i = 0; i < kB; ++i) {
+    const int u = t + i;
+    v[i] = (u >= lo && u < hi) ? r[u] : 0.0f;
+  }
+}
+
+__device__ __forceinline__ void st8(float* r, int t, int lo, int hi, const float (&v)[kB]) {
+  if (whole8(r, t, lo, hi)) {
+    *reinterpret_cast<float4*>(r + t) = make_float4(v[0], v[1], v[2], v[3]);
+    *reinterpret_cast<float4*>(r + t + 4) = make_float4(v[4], v[5], v[6], v[7]);
+    return;
+  }
+#pragma unroll
+  for (int i = 0; i < kB; ++i) {
+    const int u = t + i;
+    if (u >= lo && u < hi) r[u] = v[i];
+  }
+}
+
+// Sum of bars [lo, hi) of a lane's chunk, in bar order.
+__device__ __forceinline__ double chunk_sum(const float* r, int lo, int hi) {
+  double part = 0.0;
+  for (int t0 = lo; t0 < hi; t0 += kB) {
+    float v[kB];
+    ld8(r, t0, lo, hi, v);
+#pragma unroll
+    for (int i = 0; i < kB; ++i)
+      if (t0 + i < hi) part += (double)v[i];
+  }
+  return part;
+}
+
 // =================================================================== EMA ==
 __global__ __launch_bounds__(64) void k_ema_fwd(const float* x, float* y, double* ck, int rows,
                                                 int n, int p, double alpha) {
@@ -97,23 +155,33 @@ __global__ __launch_bounds__(64) void k_ema_fwd(const float* x, float* y, double
   float* yr = y + (size_t)row * n;
   const double beta = 1.0 - alpha;
   const int s = p - 1;
-  double part = 0.0;
-  for (int t = k.a; t < k.e && t <= s; ++t) part += (double)xr[t];
-  const double seed = wsum(part) / p;
+  const double seed = wsum(chunk_sum(xr, k.a, min(k.e, s + 1))) / p;
   double b = 0.0;
-  for (int t = k.a; t < k.e; ++t) {
-    if (t == s) b = seed;
-    else if (t > s) b = fma(beta, b, alpha * (double)xr[t]);
-  }
-  double v = carry_fwd(b, beta, k.C);
-  ck[(size_t)row * kWave + k.lane] = v;
-  for (int t = k.a; t < k.e; ++t) {
-    if (t < s) {
-      yr[t] = __builtin_nanf("");
-      continue;
+  for (int t0 = max(k.a, s); t0 < k.e; t0 += kB) {
+    float v[kB];
+    ld8(xr, t0, k.a, k.e, v);
+#pragma unroll
+    for (int i = 0; i < kB; ++i) {
+      const int t = t0 + i;
+      if (t >= k.e) break;
+      b = (t == s) ? seed : fma(beta, b, alpha * (double)v[i]);
     }
-    v = (t == s) ? seed : fma(beta, v, alpha * (double)xr[t]);
-    yr[t] = (float)v;
+  }
+  double w = carry_fwd(b, beta, k.C);
+  ck[(size_t)row * kWave + k.lane] = w;
+  for (int t0 = k.a; t0 < k.e; t0 += kB) {
+    float v[kB], o[kB];
+    ld8(xr, t0, k.a, k.e, v);
+#pragma unroll
+    for (int i = 0; i < kB; ++i) {
+      const int t = t0 + i;
+      o[i] = __builtin_nanf("");
+      if (t >= s && t < k.e) {
+        w = (t == s) ? seed : fma(beta, w, alpha * (double)v[i]);
+        o[i] = (float)w;
+      }
+    }
+    st8(yr, t0, k.a, k.e, o);
   }
 }
 
@@ -131,38 +199,55 @@ __global__ __launch_bounds__(64) void k_ema_bwd(const float* x, const float* g, 
   const double beta = 1.0 - alpha;
   const int s = p - 1;
   // forward recompute: residual x_t - y_{t-1} of every bar after the seed
-  double v = ck[(size_t)row * kWave + k.lane];
-  double part = 0.0;
-  for (int t = k.a; t < k.e && t <= s; ++t) part += (double)xr[t];
-  const double seed = wsum(part) / p;
-  for (int t = k.a; t < k.e; ++t) {
-    if (t < s) continue;
-    if (t == s) {
-      v = seed;
-      continue;
+  double w = ck[(size_t)row * kWave + k.lane];
+  const double seed = wsum(chunk_sum(xr, k.a, min(k.e, s + 1))) / p;
+  for (int t0 = max(k.a, s); t0 < k.e; t0 += kB) {
+    float v[kB];
+    ld8(xr, t0, k.a, k.e, v);
+#pragma unroll
+    for (int i = 0; i < kB; ++i) {
+      const int t = t0 + i;
+      if (t >= k.e) break;
+      if (t == s) {
+        w = seed;
+        continue;
+      }
+      const double xv = (double)v[i];
+      r.at(k.lane, t - k.a) = (float)(xv - w);
+      w = fma(beta, w, alpha * xv);
     }
-    const double xv = (double)xr[t];
-    r.at(k.lane, t - k.a) = (float)(xv - v);
-    v = fma(beta, v, alpha * xv);
   }
   // adjoint: lambda_t = g_t + beta*lambda_{t+1} for t >= s
+  const int lo = max(k.a, s);
   double b = 0.0;
-  for (int t = k.e - 1; t >= k.a && t >= s; --t) b = fma(beta, b, (double)gr[t]);
+  for (int t1 = k.e - 1; t1 >= lo; t1 -= kB) {
+    float v[kB];
+    ld8(gr, t1 - (kB - 1), lo, k.e, v);
+#pragma unroll
+    for (int i = 0; i < kB; ++i) {
+      if (t1 - i < lo) break;
+      b = fma(beta, b, (double)v[kB - 1 - i]);
+    }
+  }
   double lam = carry_bwd(b, beta, k.C);
   double da = 0.0, lamS = 0.0;
-  for (int t = k.e - 1; t >= k.a; --t) {
-    if (t < s) {
-      dr[t] = 0.0f;
-      continue;
+  for (int t1 = k.e - 1; t1 >= k.a; t1 -= kB) {
+    float v[kB], o[kB];
+    ld8(gr, t1 - (kB - 1), k.a, k.e, v);
+#pragma unroll
+    for (int i = 0; i < kB; ++i) {
+      const int t = t1 - i;
+      o[kB - 1 - i] = 0.0f;
+      if (t < s || t < k.a) continue;
+      lam = fma(beta, lam, (double)v[kB - 1 - i]);
+      if (t == s) {
+        lamS = lam;
+      } else {
+        o[kB - 1 - i] = (float)(alpha * lam);
+        da += lam * (double)r.at(k.lane, t - k.a);
+      }
     }
-    lam = fma(beta, lam, (double)gr[t]);
-    if (t == s) {
-      lamS = lam;
-      dr[t] = 0.0f;
-    } else {
-      dr[t] = (float)(alpha * lam);
-      da += lam * (double)r.at(k.lane, t - k.a);
-    }
+    st8(dr, t1 - (kB - 1), k.a, k.e, o);
   }
   // the seed is the mean of bars 0..s
   lamS = wsum(lamS);
@@ -172,6 +257,28 @@ __global__ __launch_bounds__(64) void k_ema_bwd(const float* x, const float* g, 
 }
 
 // =================================================================== RSI ==
+// Gains and losses of the bars [lo, hi) of a lane's chunk (bar t uses x_{t-1}),
+// in bar order: f(t, gain, loss, d) for every t >= 1.
+template <class F>
+__device__ __forceinline__ void walk_moves(const float* xr, int lo, int hi, F&& f) {
+  float prev = lo >= 1 ? xr[lo - 1] : 0.0f;
+  for (int t0 = lo; t0 < hi; t0 += kB) {
+    float v[kB];
+    ld8(xr, t0, lo, hi, v);
+#pragma unroll
+    for (int i = 0; i < kB; ++i) {
+      const int t = t0 + i;
+      if (t >= hi) break;
+      if (t >= 1) {
+        const double d = (double)v[i] - (double)prev;
+        const double gd = d > 0.0 ? d : 0.0;
+        f(t, gd, gd - d, d);
+      }
+      prev = v[i];
+    }
+  }
+}
+
 __global__ __launch_bounds__(64) void k_rsi_fwd(const float* x, float* y, double* ck, int rows,
                                                 int n, int p, double alpha) {
   const int row = blockIdx.x;
@@ -182,17 +289,13 @@ __global__ __launch_bounds__(64) void k_rsi_fwd(const float* x, float* y, double
   const double beta = 1.0 - alpha, inv = 1.0 / (double)p;
   const int s = p;
   double pg = 0.0, pl = 0.0;
-  for (int t = max(k.a, 1); t < k.e && t <= s; ++t) {
-    const double d = (double)xr[t] - (double)xr[t - 1];
-    const double gd = d > 0.0 ? d : 0.0;
+  walk_moves(xr, k.a, min(k.e, s + 1), [&](int, double gd, double ld, double) {
     pg += gd;
-    pl += gd - d;
-  }
+    pl += ld;
+  });
   const double sG = wsum(pg) * inv, sL = wsum(pl) * inv;
   double bg = 0.0, bl = 0.0;
-  for (int t = max(k.a, 1); t < k.e; ++t) {
-    const double d = (double)xr[t] - (double)xr[t - 1];
-    const double gd = d > 0.0 ? d : 0.0, ld = gd - d;
+  walk_moves(xr, k.a, k.e, [&](int t, double gd, double ld, double) {
     if (t == s) {
       bg = sG;
       bl = sL;
@@ -200,123 +303,189 @@ __global__ __launch_bounds__(64) void k_rsi_fwd(const float* x, float* y, double
       bg = fma(beta, bg, alpha * gd);
       bl = fma(beta, bl, alpha * ld);
     }
-  }
+  });
   double G = carry_fwd(bg, beta, k.C), L = carry_fwd(bl, beta, k.C);
   ck[((size_t)row * kWave + k.lane) * 2] = G;
   ck[((size_t)row * kWave + k.lane) * 2 + 1] = L;
-  for (int t = k.a; t < k.e; ++t) {
-    if (t < s) {
-      yr[t] = __builtin_nanf("");
-      continue;
+  float prev = k.a >= 1 ? xr[k.a - 1] : 0.0f;
+  for (int t0 = k.a; t0 < k.e; t0 += kB) {
+    float v[kB], o[kB];
+    ld8(xr, t0, k.a, k.e, v);
+#pragma unroll
+    for (int i = 0; i < kB; ++i) {
+      const int t = t0 + i;
+      o[i] = __builtin_nanf("");
+      if (t >= k.e) break;
+      if (t >= s) {
+        const double d = (double)v[i] - (double)prev;
+        const double gd = d > 0.0 ? d : 0.0, ld = gd - d;
+        if (t == s) {
+          G = sG;
+          L = sL;
+        } else {
+          G = fma(beta, G, alpha * gd);
+          L = fma(beta, L, alpha * ld);
+        }
+        const double sum = G + L;
+        o[i] = (float)(sum > 0.0 ? 100.0 * (G / sum) : 50.0);
+      }
+      prev = v[i];
     }
-    const double d = (double)xr[t] - (double)xr[t - 1];
-    const double gd = d > 0.0 ? d : 0.0, ld = gd - d;
-    if (t == s) {
-      G = sG;
-      L = sL;
-    } else {
-      G = fma(beta, G, alpha * gd);
-      L = fma(beta, L, alpha * ld);
-    }
-    const double sum = G + L;
-    yr[t] = (float)(sum > 0.0 ? 100.0 * (G / sum) : 50.0);
+    st8(yr, t0, k.a, k.e, o);
   }
 }
 
+// Wilder step of RSI_alpha at bar t (gd, ld: the bar's gain and loss).
+__device__ __forceinline__ void rsi_step(int t, int s, double sG, double sL, double beta,
+                                         double alpha, double gd, double ld, double& G,
+                                         double& L) {
+  if (t == s) {
+    G = sG;
+    L = sL;
+  } else if (t > s) {
+    G = fma(beta, G, alpha * gd);
+    L = fma(beta, L, alpha * ld);
+  }
+}
+
+// d(100 G/(G+L)) / dG and / dL at the state (G, L).
+__device__ __forceinline__ void rsi_coef(double G, double L, double& cg, double& cl) {
+  const double sum = G + L;
+  if (sum > 0.0) {
+    const double q = 100.0 / (sum * sum);
+    cg = q * L;
+    cl = -q * G;
+  } else {
+    cg = cl = 0.0;
+  }
+}
+
+// The adjoint walks each lane's chunk backwards in kB-bar blocks.  The
+// forward states inside a block are recomputed from the state entering the
+// block, kept in LDS for every block (2 doubles per lane and block), so a
+// row needs 16 KB of LDS at 8192 bars instead of its whole state history.
 __global__ __launch_bounds__(64) void k_rsi_bwd(const float* x, const float* g, const double* ck,
                                                 float* dx, double* dal, int rows, int n, int p,
                                                 double alpha) {
-  extern __shared__ float s_col[];
+  extern __shared__ double s_ck[];
   const int row = blockIdx.x;
   if (row >= rows) return;
   const Chunk k = chunk_of(n);
-  LdsCol cG{s_col, k.C + 1}, cL{s_col + kWave * (k.C + 1), k.C + 1};
+  const int nb = (k.C + kB - 1) / kB;
+  double* ckG = s_ck;               // [block][lane]
+  double* ckL = s_ck + nb * kWave;  // [block][lane]
   const float* xr = x + (size_t)row * n;
   const float* gr = g + (size_t)row * n;
   float* dr = dx + (size_t)row * n;
   const double beta = 1.0 - alpha, inv = 1.0 / (double)p;
   const int s = p;
-  // forward recompute: G_{t-1}, L_{t-1} per bar (float32 in LDS)
   double pg = 0.0, pl = 0.0;
-  for (int t = max(k.a, 1); t < k.e && t <= s; ++t) {
-    const double d = (double)xr[t] - (double)xr[t - 1];
-    const double gd = d > 0.0 ? d : 0.0;
+  walk_moves(xr, k.a, min(k.e, s + 1), [&](int, double gd, double ld, double) {
     pg += gd;
-    pl += gd - d;
-  }
+    pl += ld;
+  });
   const double sG = wsum(pg) * inv, sL = wsum(pl) * inv;
-  double G = ck[((size_t)row * kWave + k.lane) * 2], L = ck[((size_t)row * kWave + k.lane) * 2 + 1];
-  for (int t = k.a; t < k.e; ++t) {
-    if (t < s) continue;
-    cG.at(k.lane, t - k.a) = (float)G;  // state before bar t
-    cL.at(k.lane, t - k.a) = (float)L;
-    const double d = (double)xr[t] - (double)xr[t - 1];
-    const double gd = d > 0.0 ? d : 0.0, ld = gd - d;
-    if (t == s) {
-      G = sG;
-      L = sL;
-    } else {
-      G = fma(beta, G, alpha * gd);
-      L = fma(beta, L, alpha * ld);
+  // forward: the state entering every block of the chunk
+  {
+    double G = ck[((size_t)row * kWave + k.lane) * 2], L = ck[((size_t)row * kWave + k.lane) * 2 + 1];
+    float prev = k.a >= 1 ? xr[k.a - 1] : 0.0f;
+    for (int j = 0; j < nb; ++j) {
+      const int t0 = k.a + j * kB;
+      ckG[j * kWave + k.lane] = G;
+      ckL[j * kWave + k.lane] = L;
+      if (t0 >= k.e) continue;
+      float v[kB];
+      ld8(xr, t0, k.a, k.e, v);
+#pragma unroll
+      for (int i = 0; i < kB; ++i) {
+        const int t = t0 + i;
+        if (t >= k.e) break;
+        if (t >= 1) {
+          const double d = (double)v[i] - (double)prev;
+          const double gd = d > 0.0 ? d : 0.0;
+          rsi_step(t, s, sG, sL, beta, alpha, gd, gd - d, G, L);
+        }
+        prev = v[i];
+      }
     }
   }
-  // adjoint of the output map at bar t needs G_t, L_t: the value before the
-  // next bar, i.e. the recomputed entry of t+1 (or the carried end state)
-  auto after = [&](int t, double& Gt, double& Lt, double Gend, double Lend) {
-    if (t + 1 < k.e) {
-      Gt = cG.at(k.lane, t + 1 - k.a);
-      Lt = cL.at(k.lane, t + 1 - k.a);
-    } else {
-      Gt = Gend;
-      Lt = Lend;
-    }
-  };
-  const double Gend = G, Lend = L;
-  auto coef = [&](double Gt, double Lt, double& cg, double& cl) {
-    const double sum = Gt + Lt;
-    if (sum > 0.0) {
-      const double q = 100.0 / (sum * sum);
-      cg = q * Lt;
-      cl = -q * Gt;
-    } else {
-      cg = cl = 0.0;
+  // Recomputes block j: Ga[i], La[i] = state after bar t0+i; gd/ld of its
+  // bars; xv the block's prices.
+  auto block = [&](int j, float (&xv)[kB], double (&Ga)[kB], double (&La)[kB], double (&gdv)[kB],
+                   double (&dv)[kB]) {
+    const int t0 = k.a + j * kB;
+    ld8(xr, t0, 0, n, xv);
+    float prev = t0 >= 1 ? xr[t0 - 1] : 0.0f;
+    double G = ckG[j * kWave + k.lane], L = ckL[j * kWave + k.lane];
+#pragma unroll
+    for (int i = 0; i < kB; ++i) {
+      const int t = t0 + i;
+      double d = 0.0;
+      if (t >= 1 && t < k.e) d = (double)xv[i] - (double)prev;
+      const double gd = d > 0.0 ? d : 0.0;
+      if (t >= 1 && t < k.e) rsi_step(t, s, sG, sL, beta, alpha, gd, gd - d, G, L);
+      Ga[i] = G;
+      La[i] = L;
+      gdv[i] = gd;
+      dv[i] = d;
+      prev = xv[i];
     }
   };
   // lambdaG_t = cG_t g_t + beta*lambdaG_{t+1} (t >= s), same for L
+  const int lo = max(k.a, s);
   double bg = 0.0, bl = 0.0;
-  for (int t = k.e - 1; t >= k.a && t >= s; --t) {
-    double Gt, Lt, cg, cl;
-    after(t, Gt, Lt, Gend, Lend);
-    coef(Gt, Lt, cg, cl);
-    const double gv = (double)gr[t];
-    bg = fma(beta, bg, cg * gv);
-    bl = fma(beta, bl, cl * gv);
+  for (int j = nb - 1; j >= 0; --j) {
+    const int t0 = k.a + j * kB;
+    if (t0 >= k.e || t0 + kB <= lo) continue;
+    float xv[kB], gv[kB];
+    double Ga[kB], La[kB], gdv[kB], dv[kB];
+    block(j, xv, Ga, La, gdv, dv);
+    ld8(gr, t0, lo, k.e, gv);
+#pragma unroll
+    for (int i = kB - 1; i >= 0; --i) {
+      const int t = t0 + i;
+      if (t < lo || t >= k.e) continue;
+      double cg, cl;
+      rsi_coef(Ga[i], La[i], cg, cl);
+      bg = fma(beta, bg, cg * (double)gv[i]);
+      bl = fma(beta, bl, cl * (double)gv[i]);
+    }
   }
   double lg = carry_bwd(bg, beta, k.C), lq = carry_bwd(bl, beta, k.C);
   // d(gain)_t and d(loss)_t adjoints, then through d_t = x_t - x_{t-1}.
   // dd_t (the adjoint of d_t) goes to this lane's own part of dx first.
   double da = 0.0, lgS = 0.0, lqS = 0.0;
-  for (int t = k.e - 1; t >= k.a; --t) {
-    double ddt = 0.0;
-    if (t >= s) {
-      double Gt, Lt, cg, cl;
-      after(t, Gt, Lt, Gend, Lend);
-      coef(Gt, Lt, cg, cl);
-      const double gv = (double)gr[t];
-      lg = fma(beta, lg, cg * gv);
-      lq = fma(beta, lq, cl * gv);
-      const double d = (double)xr[t] - (double)xr[t - 1];
-      const double gd = d > 0.0 ? d : 0.0, ld = gd - d;
-      if (t == s) {
-        lgS = lg;
-        lqS = lq;
-      } else {
-        ddt = (d > 0.0 ? alpha * lg : 0.0) - (d < 0.0 ? alpha * lq : 0.0);
-        da += lg * (gd - (double)cG.at(k.lane, t - k.a)) +
-              lq * (ld - (double)cL.at(k.lane, t - k.a));
+  for (int j = nb - 1; j >= 0; --j) {
+    const int t0 = k.a + j * kB;
+    if (t0 >= k.e) continue;
+    float xv[kB], gv[kB], o[kB];
+    double Ga[kB], La[kB], gdv[kB], dv[kB];
+    block(j, xv, Ga, La, gdv, dv);
+    ld8(gr, t0, lo, k.e, gv);
+    const double G0 = ckG[j * kWave + k.lane], L0 = ckL[j * kWave + k.lane];
+#pragma unroll
+    for (int i = kB - 1; i >= 0; --i) {
+      const int t = t0 + i;
+      double ddt = 0.0;
+      if (t >= lo && t < k.e) {
+        double cg, cl;
+        rsi_coef(Ga[i], La[i], cg, cl);
+        const double gvv = (double)gv[i];
+        lg = fma(beta, lg, cg * gvv);
+        lq = fma(beta, lq, cl * gvv);
+        if (t == s) {
+          lgS = lg;
+          lqS = lq;
+        } else {
+          const double d = dv[i], gd = gdv[i], ld = gd - d;
+          const double Gp = i > 0 ? Ga[i - 1] : G0, Lp = i > 0 ? La[i - 1] : L0;
+          ddt = (d > 0.0 ? alpha * lg : 0.0) - (d < 0.0 ? alpha * lq : 0.0);
+          da += lg * (gd - Gp) + lq * (ld - Lp);
+        }
       }
+      o[i] = (float)ddt;
     }
-    dr[t] = (float)ddt;
+    st8(dr, t0, k.a, k.e, o);
   }
   lgS = wsum(lgS);
   lqS = wsum(lqS);
@@ -328,12 +497,19 @@ __global__ __launch_bounds__(64) void k_rsi_bwd(const float* x, const float* g, 
   // dx_t = dd_t - dd_{t+1}; the next chunk's first dd comes from lane+1
   const float first = k.a < k.e ? dr[k.a] : 0.0f;
   const float nextFirst = __shfl_down(first, 1);
+  const float tail = (k.lane + 1 < kWave && k.e < n) ? nextFirst : 0.0f;
   float cur = first;
-  for (int t = k.a; t < k.e; ++t) {
-    const float nxt = (t + 1 < k.e) ? dr[t + 1]
-                                    : (k.lane + 1 < kWave && t + 1 < n ? nextFirst : 0.0f);
-    dr[t] = (t == 0 ? 0.0f : cur) - nxt;
-    cur = nxt;
+  for (int t0 = k.a; t0 < k.e; t0 += kB) {
+    float nx[kB], o[kB];
+    ld8(dr, t0 + 1, k.a, k.e, nx);  // dd_{t+1} inside the chunk
+#pragma unroll
+    for (int i = 0; i < kB; ++i) {
+      const int t = t0 + i;
+      const float nxt = (t + 1 < k.e) ? nx[i] : tail;
+      o[i] = (t == 0 ? 0.0f : cur) - nxt;
+      cur = nxt;
+    }
+    st8(dr, t0, k.a, k.e, o);
   }
   da = wsum(da);
   if (k.lane == 0) dal[row] = da;
@@ -370,19 +546,39 @@ __global__ __launch_bounds__(64) void k_macd_bwd(const float* gm, const float* g
   }
   const int L1 = slow - 1, L2 = L1 + sig - 1;
   // signal adjoint mu_t = (gs - gh)_t + gb*mu_{t+1} for t >= L2
+  const float *gmr = gm + ro, *gsr = gs + ro, *ghr = gh + ro;
+  const int lo2 = max(k.a, L2);
   double b = 0.0;
-  for (int t = k.e - 1; t >= k.a && t >= L2; --t)
-    b = fma(gb, b, (double)gs[ro + t] - (double)gh[ro + t]);
-  double mu = carry_bwd(b, gb, k.C), muS = 0.0;
-  for (int t = k.e - 1; t >= k.a; --t) {
-    double d = 0.0;
-    if (t >= L2) {
-      mu = fma(gb, mu, (double)gs[ro + t] - (double)gh[ro + t]);
-      d = (double)gm[ro + t] + (double)gh[ro + t];
-      if (t == L2) muS = mu;
-      else d += gk * mu;
+  for (int t1 = k.e - 1; t1 >= lo2; t1 -= kB) {
+    float vs[kB], vh[kB];
+    ld8(gsr, t1 - (kB - 1), lo2, k.e, vs);
+    ld8(ghr, t1 - (kB - 1), lo2, k.e, vh);
+#pragma unroll
+    for (int i = 0; i < kB; ++i) {
+      if (t1 - i < lo2) break;
+      b = fma(gb, b, (double)vs[kB - 1 - i] - (double)vh[kB - 1 - i]);
     }
-    dm.at(k.lane, t - k.a) = (float)d;
+  }
+  double mu = carry_bwd(b, gb, k.C), muS = 0.0;
+  for (int t1 = k.e - 1; t1 >= k.a; t1 -= kB) {
+    float vs[kB], vh[kB], vm[kB];
+    ld8(gsr, t1 - (kB - 1), lo2, k.e, vs);
+    ld8(ghr, t1 - (kB - 1), lo2, k.e, vh);
+    ld8(gmr, t1 - (kB - 1), lo2, k.e, vm);
+#pragma unroll
+    for (int i = 0; i < kB; ++i) {
+      const int t = t1 - i;
+      if (t < k.a) break;
+      const int j = kB - 1 - i;
+      double d = 0.0;
+      if (t >= L2) {
+        mu = fma(gb, mu, (double)vs[j] - (double)vh[j]);
+        d = (double)vm[j] + (double)vh[j];
+        if (t == L2) muS = mu;
+        else d += gk * mu;
+      }
+      dm.at(k.lane, t - k.a) = (float)d;
+    }
   }
   muS = wsum(muS);
   for (int t = max(k.a, L1); t < k.e && t <= L2; ++t)
@@ -395,20 +591,26 @@ __global__ __launch_bounds__(64) void k_macd_bwd(const float* gm, const float* g
     bs = fma(sb, bs, -d);
   }
   double nf = carry_bwd(bf, fb, k.C), ns = carry_bwd(bs, sb, k.C), nfS = 0.0, nsS = 0.0;
-  for (int t = k.e - 1; t >= k.a; --t) {
-    double out = 0.0;
-    if (t >= L1) {
-      const double d = dm.at(k.lane, t - k.a);
-      nf = fma(fb, nf, d);
-      ns = fma(sb, ns, -d);
-      if (t == L1) {
-        nfS = nf;
-        nsS = ns;
-      } else {
-        out = fk * nf + sk * ns;
+  for (int t1 = k.e - 1; t1 >= k.a; t1 -= kB) {
+    float o[kB];
+#pragma unroll
+    for (int i = 0; i < kB; ++i) {
+      const int t = t1 - i;
+      double out = 0.0;
+      if (t >= L1 && t >= k.a) {
+        const double d = dm.at(k.lane, t - k.a);
+        nf = fma(fb, nf, d);
+        ns = fma(sb, ns, -d);
+        if (t == L1) {
+          nfS = nf;
+          nsS = ns;
+        } else {
+          out = fk * nf + sk * ns;
+        }
       }
+      o[kB - 1 - i] = (float)out;
     }
-    dx[ro + t] = (float)out;
+    st8(dx + ro, t1 - (kB - 1), k.a, k.e, o);
   }
   nfS = wsum(nfS);
   nsS = wsum(nsS);
@@ -562,7 +764,8 @@ CTAPI_EXPORT ctStatus_t ctx_rsi_alpha_bwd_d(const float* x, const float* gy, con
     if (r != CT_STATUS_SUCCESS) return r;
     if (!ckpt || !gx || !galpha)
       return fail(CT_STATUS_INVALID_ARGUMENT, "rsi_alpha backward: null pointer");
-    const size_t lds = col_bytes(n, 2);
+    // per lane and kB-bar block: the entering G and L (doubles)
+    const size_t lds = (size_t)2 * kWave * (((n + kWave - 1) / kWave + kB - 1) / kB) * sizeof(double);
     r = allow_lds(k_rsi_bwd, lds);
     if (r != CT_STATUS_SUCCESS) return r;
     hipLaunchKernelGGL(k_rsi_bwd, dim3(rows), dim3(kWave), lds, s, x, gy, ckpt, gx, galpha, rows,

@@ -82,3 +82,29 @@ def test_full_size_forward_backward_runs(dev):
     loss = sum(torch.nan_to_num(o, nan=0.0).sum() for o in outs)
     loss.backward()
     assert torch.isfinite(c.grad).all() and torch.isfinite(a.grad)
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_warmup_gradients_are_not_read(dev, name):
+    """An incoming gradient that is NaN at the warm-up bars (where the
+    outputs are NaN) gives the same price and alpha gradients as one that is
+    zero there."""
+    print(f"autograd warm-up {name}", flush=True)
+    spec = CASES[name]
+    c = _close(4, 1000, seed=3)
+    res = []
+    for fill in (0.0, float("nan")):
+        xg = torch.tensor(c, device=dev, requires_grad=True)
+        ag_ = torch.tensor(ALPHA0[name], dtype=torch.float32, device=dev, requires_grad=True)
+        out = _gpu_fn(name)(xg, ag_ if spec["alpha"] else None)
+        outs = out if isinstance(out, tuple) else (out,)
+        grads = []
+        for i, o in enumerate(outs):
+            w = torch.tensor(np.random.default_rng(10 + i).standard_normal(tuple(o.shape)),
+                             dtype=torch.float32, device=dev)
+            grads.append(torch.where(torch.isnan(o), torch.full_like(w, fill), w))
+        torch.autograd.backward(outs, grads)
+        res.append((xg.grad.cpu().numpy(), ag_.grad.item() if spec["alpha"] else 0.0))
+    assert np.isfinite(res[1][0]).all()
+    assert np.array_equal(res[0][0], res[1][0])
+    assert res[0][1] == res[1][1]
