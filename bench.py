@@ -205,10 +205,20 @@ def _batch13(a, torch, ops, T, dev):
 
 
 def _scan256(a, torch, ops, T, dev):
+    """256 x 2^20 close: EMA(200), KAMA(30), T3(5, 0.7), SMA(200).  The fused
+    two-pass launch (ctx_scan_batch_d) is the workload's figure; the four
+    drop-in family calls are timed beside it."""
     import tacuda_amd as ta
 
     rows, n = 256, 1 << 20
     c = torch.from_numpy(ta.synth_ohlcv(n, rows=rows, seed=0)["close"]).to(dev)
+    bars = rows * n
+    alg = (1 + 4) * 4 * bars
+    g = T.graphs([c], lambda x: ops.scan(x))
+    fused = T.time(g, max(5, a.steps // 5), 2)
+    T.keep.clear()
+    torch.cuda.empty_cache()
+    print(f"scan256 fused: {fused:.3f} ms/pass", file=sys.stderr, flush=True)
     calls = {"ema200": lambda x: ops.ema(x, 200), "kama30": lambda x: ops.kama(x, 30),
              "t3_5": lambda x: ops.t3(x, 5, 0.7), "sma200": lambda x: ops.sma(x, 200)}
     per = {}
@@ -220,12 +230,11 @@ def _scan256(a, torch, ops, T, dev):
         total += kms
         T.keep.clear()
         torch.cuda.empty_cache()
-    bars = rows * n
-    alg = (1 + 4) * 4 * bars
-    print(f"scan256 256x2^20: {total:.3f} ms/pass", file=sys.stderr, flush=True)
-    return {"ms": total, "bars_per_s": bars / (total * 1e-3), "alg_bytes": alg,
-            "GBs": _gbs(alg, total), "frac_hbm": _gbs(alg, total) / PEAK_HBM_GBS,
-            "shape": [rows, n], "kernels": per}
+    print(f"scan256 drop-in 4 calls: {total:.3f} ms/pass", file=sys.stderr, flush=True)
+    return {"ms": fused, "bars_per_s": bars / (fused * 1e-3), "alg_bytes": alg,
+            "GBs": _gbs(alg, fused), "frac_hbm": _gbs(alg, fused) / PEAK_HBM_GBS,
+            "shape": [rows, n], "graph": True, "drop_in_ms": total,
+            "drop_in_frac_hbm": _gbs(alg, total) / PEAK_HBM_GBS, "drop_in_kernels": per}
 
 
 def _ragged(a, torch, ops, T, dev):

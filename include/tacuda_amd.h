@@ -84,6 +84,28 @@ CTAPI_EXPORT ctStatus_t ctx_pack_ragged_d(const float* d_high, const float* d_lo
                                           const ctxPackParams* params,
                                           const ctxPackOutputs* outputs, hipStream_t stream);
 
+/* Fused long-series scan (no counterpart in the reference, whose families run
+ * as separate calls ct_ema_batch_d, ct_kama_batch_d, ct_t3_batch_d,
+ * ct_sma_batch_d at include/tacuda.h and src/api_batch.cu): EMA, KAMA, T3
+ * and SMA of the same close rows from one read-twice, write-once launch
+ * sequence.  Each output equals the family call's; a null output is skipped.
+ * Window look-backs (SMA period - 1, KAMA period + 1) up to 1024 bars. */
+typedef struct ctxScanParams {
+  int ema_period;
+  int kama_period, kama_fast, kama_slow;
+  int t3_period;
+  float t3_vfactor;
+  int sma_period;
+} ctxScanParams;
+
+typedef struct ctxScanOutputs {
+  float *ema, *kama, *t3, *sma;
+} ctxScanOutputs;
+
+CTAPI_EXPORT ctStatus_t ctx_scan_batch_d(const float* d_close, int seriesLen, int numSeries,
+                                         const ctxScanParams* params,
+                                         const ctxScanOutputs* outputs, hipStream_t stream);
+
 /* Single-family ragged calls; arguments as the family's ct_*_d call. */
 #define CTX_RAGGED1(name) \
   CTAPI_EXPORT ctStatus_t ctx_##name##_ragged_d(const float* in, float* out, ctxRagged layout, \

@@ -108,6 +108,24 @@ _sig("ctx_ragged_create", P, c_int, ctypes.POINTER(c_void_p), stream=False)
 _sig("ctx_ragged_destroy", c_void_p, stream=False)
 _sig("ctx_pack_ragged_d", P, P, P, c_void_p, ctypes.POINTER(PackParams),
      ctypes.POINTER(PackOutputs))
+
+
+class ScanParams(ctypes.Structure):
+    """ctxScanParams of include/tacuda_amd.h."""
+    _fields_ = [("ema_period", c_int), ("kama_period", c_int), ("kama_fast", c_int),
+                ("kama_slow", c_int), ("t3_period", c_int), ("t3_vfactor", c_float),
+                ("sma_period", c_int)]
+
+
+SCAN_OUTPUTS = ("ema", "kama", "t3", "sma")
+
+
+class ScanOutputs(ctypes.Structure):
+    """ctxScanOutputs of include/tacuda_amd.h (device pointers, NULL = skip)."""
+    _fields_ = [(n, c_void_p) for n in SCAN_OUTPUTS]
+
+
+_sig("ctx_scan_batch_d", P, c_int, c_int, ctypes.POINTER(ScanParams), ctypes.POINTER(ScanOutputs))
 for _n in ("sma", "ema", "wma", "dema", "tema", "rsi", "var", "stddev", "sum", "max", "min"):
     _sig(f"ctx_{_n}_ragged_d", P, P, c_void_p, c_int)
 for _n in ("atr", "natr", "adx", "dx", "plus_di", "minus_di", "willr", "cci"):

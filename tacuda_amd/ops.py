@@ -11,7 +11,7 @@ import torch
 
 import ctypes
 
-from . import PACK_OUTPUTS, PackOutputs, PackParams, call
+from . import PACK_OUTPUTS, SCAN_OUTPUTS, PackOutputs, PackParams, ScanOutputs, ScanParams, call
 
 P = ctypes.c_void_p
 
@@ -179,6 +179,27 @@ def pack(high, low, close, outputs=PACK_OUTPUTS, **periods):
     n, rows = _shape(close)
     call("ctx_pack_batch_d", high.data_ptr(), low.data_ptr(), close.data_ptr(), n, rows, prm, outs,
          _stream())
+    return res
+
+
+SCAN_DEFAULTS = dict(ema_period=200, kama_period=30, kama_fast=2, kama_slow=30, t3_period=5,
+                     t3_vfactor=0.7, sma_period=200)
+
+
+def scan(close, outputs=SCAN_OUTPUTS, **periods):
+    """EMA, KAMA, T3 and SMA of the same close rows from one fused launch
+    sequence (ctx_scan_batch_d); ``outputs`` names the ones to compute,
+    ``periods`` overrides SCAN_DEFAULTS.  Returns a dict of tensors shaped
+    like close."""
+    _check(close)
+    unknown = set(outputs) - set(SCAN_OUTPUTS)
+    if unknown:
+        raise ValueError(f"unknown scan outputs {sorted(unknown)}")
+    prm = ScanParams(**{**SCAN_DEFAULTS, **periods})
+    res = {k: torch.empty_like(close) for k in outputs}
+    outs = ScanOutputs(**{k: v.data_ptr() for k, v in res.items()})
+    n, rows = _shape(close)
+    call("ctx_scan_batch_d", close.data_ptr(), n, rows, prm, outs, _stream())
     return res
 
 

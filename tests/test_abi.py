@@ -128,6 +128,28 @@ def test_pack_arguments_are_validated_without_the_device():
     assert L.ctx_pack_batch_d(None, None, p, 8, 1, prm, ta.PackOutputs(atr=p), None) == 4
 
 
+def test_scan_arguments_are_validated_without_the_device():
+    import tacuda_amd as ta
+    from tacuda_amd.ops import SCAN_DEFAULTS
+
+    L = ta.lib()
+    buf = np.zeros(8, np.float32)
+    p = buf.ctypes.data
+    prm = ta.ScanParams(**SCAN_DEFAULTS)
+    assert L.ctx_scan_batch_d(p, 8, 1, prm, ta.ScanOutputs(), None) == 4
+    assert b"no output" in L.ct_get_last_error()
+    assert L.ctx_scan_batch_d(p, 0, 1, prm, ta.ScanOutputs(ema=p), None) == 4
+    assert L.ctx_scan_batch_d(None, 8, 1, prm, ta.ScanOutputs(ema=p), None) == 4
+    t3 = ta.ScanParams(**{**SCAN_DEFAULTS, "t3_period": 1})
+    assert L.ctx_scan_batch_d(p, 8, 1, t3, ta.ScanOutputs(t3=p), None) == 4
+    bad = ta.ScanParams(**{**SCAN_DEFAULTS, "kama_fast": 0})
+    assert L.ctx_scan_batch_d(p, 8, 1, bad, ta.ScanOutputs(kama=p), None) == 4
+    wide = ta.ScanParams(**{**SCAN_DEFAULTS, "sma_period": 1026})
+    assert L.ctx_scan_batch_d(p, 8, 1, wide, ta.ScanOutputs(sma=p), None) == 6
+    wide = ta.ScanParams(**{**SCAN_DEFAULTS, "kama_period": 1024})
+    assert L.ctx_scan_batch_d(p, 8, 1, wide, ta.ScanOutputs(kama=p), None) == 6
+
+
 def test_ragged_layout_arguments_are_validated_without_the_device():
     import ctypes
 
