@@ -14,7 +14,13 @@ Also measured in the same run (reported under "workloads"):
            TEMA, RSI, MACD, BBANDS, ATR, ADX, STOCH, WILLR, CCI (18 outputs)
            in one fused call, hipGraph replay, five rotated buffer sets;
   scan256  256 series x 2^20 bars, EMA(200), KAMA(30), T3(5, 0.7), SMA(200)
-           through the dense batch calls (look-back scan across segments).
+           through the fused two-pass scan (ctx_scan_batch_d), with the four
+           dense drop-in batch calls (look-back scan) timed beside it;
+  candles  all 61 candlestick patterns, int32 signals, 1024 x 16,384 OHLC;
+  ragged   4096 series of lengths U[64, 8192], the 13-indicator pass through
+           a tuned, graph-captured plan;
+  ticks    one streaming update of 65,536 symbols' EMA/RSI/ATR states;
+  autograd 1024 x 8192, forward+backward vs forward-only time.
 
 Prints one JSON line: bars/s of the headline pass, the roofline of its
 dominant kernel (algorithmic bytes per launch / event-timed launch duration
@@ -60,7 +66,8 @@ def _args():
                     help="time budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", default="",
-                    help="comma list of workloads: pipeline7,batch13,scan256,ragged,ticks,autograd")
+                    help="comma list of workloads: pipeline7,batch13,scan256,candles,ragged,ticks,"
+                         "autograd")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="also run a plain device copy of this many MB (counter calibration)")
     return ap.parse_args()
@@ -237,6 +244,26 @@ def _scan256(a, torch, ops, T, dev):
             "drop_in_frac_hbm": _gbs(alg, total) / PEAK_HBM_GBS, "drop_in_kernels": per}
 
 
+def _candles(a, torch, ops, T, dev):
+    """All 61 candlestick patterns, int32 signals, 1024 x 16,384 OHLC in one
+    launch (ctx_cdl_all_batch_d)."""
+    import tacuda_amd as ta
+
+    rows, n = 1024, 16384
+    d = ta.synth_ohlcv(n, rows=rows, seed=0)
+    x = {k: torch.from_numpy(d[k]).to(dev) for k in ("open", "high", "low", "close")}
+    g = T.graphs([x], lambda s: ops.cdl_all(s["open"], s["high"], s["low"], s["close"]))
+    ms = T.time(g, max(5, a.steps // 5), 2)
+    T.keep.clear()
+    torch.cuda.empty_cache()
+    bars = rows * n
+    alg = (4 + 61) * 4 * bars
+    print(f"candles 61 x 1024x16384: {ms:.3f} ms/pass", file=sys.stderr, flush=True)
+    return {"ms": ms, "bars_per_s": bars / (ms * 1e-3), "alg_bytes": alg, "GBs": _gbs(alg, ms),
+            "frac_hbm": _gbs(alg, ms) / PEAK_HBM_GBS, "shape": [rows, n], "graph": True,
+            "patterns": 61, "dtype": "int32"}
+
+
 def _ragged(a, torch, ops, T, dev):
     """4096 series, lengths uniform in [64, 8192] (seed 1), the 13-indicator
     fused pass through a tuned, graph-captured plan."""
@@ -394,8 +421,8 @@ def main():
     from tacuda_amd import ops
 
     dev = torch.device("cuda:0")
-    only = set(filter(None, a.only.split(","))) or {"pipeline7", "batch13", "scan256", "ragged",
-                                                   "ticks", "autograd"}
+    only = set(filter(None, a.only.split(","))) or {"pipeline7", "batch13", "scan256", "candles",
+                                                    "ragged", "ticks", "autograd"}
     T = Timer(torch)
     work = {}
     d = p7 = None
@@ -407,6 +434,8 @@ def main():
     torch.cuda.empty_cache()
     if "scan256" in only:
         work["scan256_ema200_kama30_t3_sma200"] = _scan256(a, torch, ops, T, dev)
+    if "candles" in only:
+        work["candles_61_1024x16384"] = _candles(a, torch, ops, T, dev)
     if "ragged" in only:
         work["ragged_4096_U64_8192"] = _ragged(a, torch, ops, T, dev)
         torch.cuda.empty_cache()

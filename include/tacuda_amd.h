@@ -158,11 +158,15 @@ CTAPI_EXPORT ctStatus_t ctx_trange_ragged_d(const float* high, const float* low,
  * Persistent tuned plans: one fused pack pass over fixed device buffers
  * (dense seriesLen x numSeries when `layout` is NULL, else the ragged
  * layout), captured into a hipGraph.  With `tune` set, creation times every
- * tile configuration (bars per lane of the scan, segments per workgroup) on
- * these buffers and keeps the fastest; the choice is cached per shape for
- * the process.  Creation runs the pass several times (outputs are
- * overwritten).  A plan's launches must be serialized (they share the
- * plan's look-back workspace); stream 0 blocks until the pass is done.
+ * tile configuration (bars per lane of the scan, segments per workgroup) of
+ * each recurrence kernel on these buffers, then the whole pass as three
+ * launches with the best tiles against one launch of 4 or 8 bars per lane,
+ * and keeps the fastest; the choice is cached per shape for the process.
+ * ctx_plan_info reports the tile of the launch that runs (the close
+ * kernel's when there are several).  Creation runs the pass several times
+ * (outputs are overwritten).  A plan's launches must be serialized (they
+ * share the plan's look-back workspace); stream 0 blocks until the pass is
+ * done.
  * ------------------------------------------------------------------------- */
 typedef struct ctxPlan_st* ctxPlan;
 CTAPI_EXPORT ctStatus_t ctx_pack_plan_create(const float* d_high, const float* d_low,

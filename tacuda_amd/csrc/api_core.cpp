@@ -4,6 +4,7 @@
 
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <vector>
 
 #include "../../include/tacuda_device.h"
@@ -20,7 +21,7 @@ struct WsEntry {
   int device = -1;
 };
 std::mutex g_ws_mu;
-std::map<std::pair<int, hipStream_t>, WsEntry> g_ws;
+std::map<std::tuple<int, hipStream_t, int>, WsEntry> g_ws;
 size_t g_ws_allocs = 0;
 }  // namespace
 
@@ -65,11 +66,11 @@ ctStatus_t hip_fail(hipError_t e, const char* what) {
   return fail(st, "%s: %s", what, hipGetErrorString(e));
 }
 
-ctStatus_t workspace_for(hipStream_t stream, size_t treeBytes, Ws* out) {
+ctStatus_t workspace_for(hipStream_t stream, size_t treeBytes, Ws* out, int part) {
   int dev = 0;
   TCA_HIP(hipGetDevice(&dev));
   std::lock_guard<std::mutex> lk(g_ws_mu);
-  WsEntry& w = g_ws[{dev, stream}];
+  WsEntry& w = g_ws[std::make_tuple(dev, stream, part)];
   if (w.base == nullptr || w.bytes < treeBytes) {
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (stream != nullptr) TCA_HIP(hipStreamIsCapturing(stream, &cs));
@@ -103,13 +104,15 @@ void workspace_release(hipStream_t stream) {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return;
   std::lock_guard<std::mutex> lk(g_ws_mu);
-  auto it = g_ws.find({dev, stream});
-  if (it == g_ws.end()) return;
-  if (it->second.base) {
-    (void)hipStreamSynchronize(stream);
-    (void)hipFree(it->second.base);
+  for (int part = 0; part < kWsParts; ++part) {
+    auto it = g_ws.find(std::make_tuple(dev, stream, part));
+    if (it == g_ws.end()) continue;
+    if (it->second.base) {
+      (void)hipStreamSynchronize(stream);
+      (void)hipFree(it->second.base);
+    }
+    g_ws.erase(it);
   }
-  g_ws.erase(it);
 }
 
 void workspace_release_all() {

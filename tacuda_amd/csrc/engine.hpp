@@ -587,12 +587,13 @@ __device__ __forceinline__ unsigned read_epoch(const Ws& ws) {
   return (unsigned)__builtin_amdgcn_readfirstlane((int)e);
 }
 
-// Block epilogue (thread 0, after the block's waves are done): the block
+// Block epilogue (thread 0, after the block's waves are done; b is the
+// block's index among the nblocks blocks that share the workspace): the block
 // completing the last stripe resets the counters and bumps the epoch, so the
 // next launch on this workspace (ordered after this one by the stream)
 // starts clean.  Relaxed atomics suffice: kernel end publishes every write.
-__device__ __forceinline__ void finish_block(const Ws& ws, unsigned nblocks) {
-  const unsigned b = blockIdx.x;
+__device__ __forceinline__ void finish_block(const Ws& ws, unsigned nblocks,
+                                             unsigned b = blockIdx.x) {
   const unsigned st = b % kStripes;
   const unsigned inStripe = (nblocks - st + kStripes - 1) / kStripes;
   const unsigned d = __hip_atomic_fetch_add(&ws.hdr->stripe[st][0], 1u, __ATOMIC_RELAXED,

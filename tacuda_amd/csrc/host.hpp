@@ -45,10 +45,13 @@ ctStatus_t guarded(F&& body) {
 }
 
 // ---------------------------------------------------------- workspace --
-// Look-back tree area plus the ticket/epoch header, one per HIP stream so that
-// launches ordered on one stream share it safely.  Growing it is refused
-// while the stream is being captured (the graph would keep a stale pointer).
-ctStatus_t workspace_for(hipStream_t stream, size_t treeBytes, Ws* out);
+// Look-back tree area plus the ticket/epoch header, one per HIP stream and
+// part so that launches ordered on one stream share it safely; one launch
+// whose blocks run two independent look-back scans gives each its own part
+// (kWsParts per stream).  Growing it is refused while the stream is being
+// captured (the graph would keep a stale pointer).
+constexpr int kWsParts = 2;
+ctStatus_t workspace_for(hipStream_t stream, size_t treeBytes, Ws* out, int part = 0);
 void workspace_release_all();
 void workspace_release(hipStream_t stream);
 size_t workspace_allocations();

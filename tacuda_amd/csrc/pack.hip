@@ -43,9 +43,11 @@ struct PackCfg {
   int W;  // segments (waves) per workgroup
 };
 
-// Tiles of the two recurrence kernels, chosen independently.
+// Tiles of the two recurrence kernels, chosen independently, and whether the
+// pass runs as one launch (one = bars per lane of that launch, 0 = three).
 struct PackTiles {
   PackCfg close, hlc;
+  int one;
 };
 
 struct PackIO {
@@ -630,7 +632,11 @@ __device__ __forceinline__ void pack_windows(const Seg& c, ColLds<C::E>& s, cons
   col_stage<C>(s.c, io.c, c);
   col_sync();
 
-  if (m & (1u << kOSma)) {
+  // SMA and the BBANDS middle band of one period are the same window mean:
+  // the BBANDS loop then writes both
+  const bool bbAll = m & ((1u << kOBbU) | (1u << kOBbM) | (1u << kOBbL));
+  const bool smaInBb = (m & (1u << kOSma)) && bbAll && P.smaP == P.bbP;
+  if ((m & (1u << kOSma)) && !smaInBb) {
     const int p = P.smaP;
     for (int j = C::NH; j < C::NC; ++j) {
       const int e = j * kWave + lane;
@@ -657,7 +663,7 @@ __device__ __forceinline__ void pack_windows(const Seg& c, ColLds<C::E>& s, cons
       col_store<C>(io.out[kOWma], c, j, (base + e >= p - 1) ? v : qnanf_());
     }
   }
-  if (m & ((1u << kOBbU) | (1u << kOBbM) | (1u << kOBbL))) {
+  if (bbAll) {
     const int p = P.bbP;
     const double inv = 1.0 / (double)p;
     for (int j = C::NH; j < C::NC; ++j) {
@@ -684,6 +690,7 @@ __device__ __forceinline__ void pack_windows(const Seg& c, ColLds<C::E>& s, cons
         lo = (float)(mean - sd * P.bbDn);
       }
       const bool ok = base + e >= p - 1;
+      if (smaInBb) col_store<C>(io.out[kOSma], c, j, ok ? (float)(sum / (double)p) : qnanf_());
       col_store<C>(io.out[kOBbU], c, j, ok ? up : qnanf_());
       col_store<C>(io.out[kOBbM], c, j, ok ? (float)mean : qnanf_());
       col_store<C>(io.out[kOBbL], c, j, ok ? lo : qnanf_());
@@ -910,12 +917,108 @@ static ctStatus_t launch_part(PackCfg cfg, int len, int rows, hipStream_t s, con
   return fail(CT_STATUS_INVALID_ARGUMENT, "pack: unknown tile configuration %dx%d", cfg.K, cfg.W);
 }
 
+// ------------------------------------------------------- one launch --
+// The window kernel's and both recurrence kernels' workgroups in one grid,
+// [windows | close | high/low/close], kFuseW waves each: recurrence waves
+// that wait on look-back leave the ALUs to the window waves.  Each
+// recurrence part scans through its own workspace part; tickets within a
+// part follow block order, so every predecessor a wave waits for was
+// dispatched before it.  Window look-backs up to 64 bars.
+constexpr int kFuseW = 4;
+
+template <int K>
+struct FuseLds {
+  using C = ColTile<64>;
+  using T = Tile<K, K>;
+  union {
+    ColLds<C::E> win[kFuseW];
+    struct {
+      float in[kFuseW][3 * T::PLEN];
+      float out[kFuseW][T::POUT];
+    } rec;
+  };
+};
+
+template <int K>
+__global__ __launch_bounds__(kFuseW* kWave) void k_pack_one(Geometry gw, Geometry gr, Ws wsC,
+                                                            Ws wsH, int useLB, int nbW, int nbC,
+                                                            int nbH, PackP prm, PackIO io) {
+  using C = ColTile<64>;
+  using T = Tile<K, K>;
+  __shared__ __align__(16) FuseLds<K> sm;
+  const int w = threadIdx.x / kWave;
+  int b = blockIdx.x;
+  if (b < nbW) {
+    Seg c;
+    c.tree = false;
+    c.epoch = 0;
+    if (resolve(gw, C::SEG, b * kFuseW + w, c)) pack_windows<C>(c, sm.win[w], prm, io);
+    return;
+  }
+  b -= nbW;
+  const bool cl = b < nbC;
+  if (!cl) b -= nbC;
+  const Ws ws = cl ? wsC : wsH;
+  Seg c;
+  c.tree = useLB != 0;
+  c.epoch = useLB ? read_epoch(ws) : 0u;
+  if (resolve(gr, T::SEG, b * kFuseW + w, c)) {
+    c.ws = ws;
+    if (cl)
+      pack_close<T>(c, sm.rec.in[w], sm.rec.out[w], prm, io);
+    else
+      pack_hlc<T>(c, sm.rec.in[w], sm.rec.out[w], prm, io);
+  }
+  if (useLB) {
+    __syncthreads();
+    if (threadIdx.x == 0) finish_block(ws, cl ? nbC : nbH, b);
+  }
+}
+
+template <int K>
+static ctStatus_t launch_one(int len, int rows, hipStream_t s, const PackP& prm,
+                             const PackIO& io) {
+  using C = ColTile<64>;
+  constexpr int SEG = K * kWave;
+  const bool wW = prm.mask & kWinMask, wC = prm.mask & kCloseRecMask, wH = prm.mask & kHlcRecMask;
+  Geometry gw = dense_geometry(len, rows, C::SEG);
+  Geometry gr = dense_geometry(len, rows, SEG);
+  int maxW = 0, maxSegs = gr.segsPerRow;
+  if (t_ragged && (!ragged_geometry(t_ragged, C::SEG, &gw, &maxW) ||
+                   !ragged_geometry(t_ragged, SEG, &gr, &maxSegs)))
+    return fail(CT_STATUS_UNSUPPORTED, "pack: no ragged table for the tile");
+  const int nbW = wW ? div_up(gw.totalSegs, kFuseW) : 0;
+  const int nbR = gr.totalSegs > 0 ? div_up(gr.totalSegs, kFuseW) : 0;
+  const int nbC = wC ? nbR : 0, nbH = wH ? nbR : 0;
+  if (nbW + nbC + nbH == 0) return CT_STATUS_SUCCESS;
+  const bool useLB = maxSegs > 1;
+  Ws wsC{nullptr, nullptr, 1 + 5}, wsH{nullptr, nullptr, 1 + 5};
+  if (useLB) {
+    const size_t bytes = (size_t)3 * ((size_t)gr.totalSegs + gr.totalSegs / kGroup) * (1 + 5) * 8;
+    ctStatus_t st = CT_STATUS_SUCCESS;
+    if (wC) st = workspace_for(s, bytes, &wsC, 0);
+    if (st == CT_STATUS_SUCCESS && wH) st = workspace_for(s, bytes, &wsH, 1);
+    if (st != CT_STATUS_SUCCESS) return st;
+  }
+  hipLaunchKernelGGL((k_pack_one<K>), dim3(nbW + nbC + nbH), dim3(kFuseW * kWave), 0, s, gw, gr,
+                     wsC, wsH, useLB ? 1 : 0, nbW, nbC, nbH, prm, io);
+  TCA_HIP(hipGetLastError());
+  return CT_STATUS_SUCCESS;
+}
+
 // The fused pass: the window kernel and the two recurrence kernels read the
-// same inputs and write disjoint outputs.  They run in stream order: each
-// fills the GPU on its own and is bound by its vector ALU work, so running
-// them concurrently gains nothing (measured).
+// same inputs and write disjoint outputs -- as one launch (tiles.one) when
+// the window look-backs fit 64 bars and at least two parts are requested,
+// else as up to three launches in stream order.
 ctStatus_t pack_launch(int len, int rows, hipStream_t s, const PackP& p, const PackIO& io,
                        PackTiles t) {
+  const int parts = ((p.mask & kWinMask) ? 1 : 0) + ((p.mask & kCloseRecMask) ? 1 : 0) +
+                    ((p.mask & kHlcRecMask) ? 1 : 0);
+  if (t.one && parts >= 2 && window_halo(p) <= 64) {
+    if (t.one == 4) return launch_one<4>(len, rows, s, p, io);
+    if (t.one == 8) return launch_one<8>(len, rows, s, p, io);
+    return fail(CT_STATUS_INVALID_ARGUMENT, "pack: unknown single-launch tile %d", t.one);
+  }
   ctStatus_t st = CT_STATUS_SUCCESS;
   if (p.mask & kWinMask) st = launch_windows(len, rows, s, p, io);
   if (st == CT_STATUS_SUCCESS && (p.mask & kCloseRecMask))
@@ -929,10 +1032,10 @@ ctStatus_t pack_launch(int len, int rows, hipStream_t s, const PackP& p, const P
 // lane (most waves for short rows); longer rows use 8 bars per lane, which
 // halves the look-back segments -- one wave per workgroup for the close
 // kernel (fewest waves waiting on one workgroup's slowest segment), four
-// for the high/low/close kernel.
+// for the high/low/close kernel.  The pass runs as one launch.
 PackTiles default_tiles(int len) {
-  if (len <= 4 * kWave) return PackTiles{{4, 4}, {4, 4}};
-  return PackTiles{{8, 1}, {8, 4}};
+  if (len <= 4 * kWave) return PackTiles{{4, 4}, {4, 4}, 4};
+  return PackTiles{{8, 1}, {8, 4}, 8};
 }
 
 // Dense, single-series or ragged (maximum row length) pack with the default
@@ -1079,8 +1182,9 @@ struct ctxPlan_st {
   hipStream_t stream = nullptr;  // private stream the graph was captured on
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  PackTiles tiles{{4, 4}, {4, 4}};
+  PackTiles tiles{{4, 4}, {4, 4}, 4};
   unsigned mask = 0;
+  int runK = 4, runW = 4;  // tile of the launch that runs (the close kernel's if several)
   float tunedUs = 0.0f, defaultUs = 0.0f;
 };
 
@@ -1137,7 +1241,7 @@ ctStatus_t tune_part(int len, int rows, hipStream_t st, const PackP& P, const Pa
   float bestUs = 0.0f;
   for (const PackCfg& cf : kPackCfgs) {
     float us = 0.0f;
-    ctStatus_t r = time_tiles(len, rows, st, Q, io, PackTiles{cf, cf}, 10, &us);
+    ctStatus_t r = time_tiles(len, rows, st, Q, io, PackTiles{cf, cf, 0}, 10, &us);
     if (r != CT_STATUS_SUCCESS) return r;
     if (bestUs == 0.0f || us < bestUs) {
       bestUs = us;
@@ -1208,17 +1312,31 @@ CTAPI_EXPORT ctStatus_t ctx_pack_plan_create(const float* h, const float* l, con
     // look-back workspace its tile needs on the plan's stream
     if (tune && !cached) {
       PackTiles t = dflt;
+      t.one = 0;
       ctStatus_t r = CT_STATUS_SUCCESS;
       if (P.mask & kCloseRecMask) r = tune_part(len, rows, p->stream, P, io, kCloseRecMask, &t.close);
       if (r == CT_STATUS_SUCCESS && (P.mask & kHlcRecMask))
         r = tune_part(len, rows, p->stream, P, io, kHlcRecMask, &t.hlc);
-      float tu = 0.0f, du = 0.0f;
-      if (r == CT_STATUS_SUCCESS) r = time_tiles(len, rows, p->stream, P, io, t, 10, &tu);
+      // the whole pass decides: three launches with the tuned tiles, or one
+      // launch of 4 or 8 bars per lane
+      float du = 0.0f;
       if (r == CT_STATUS_SUCCESS) r = time_tiles(len, rows, p->stream, P, io, dflt, 10, &du);
+      PackTiles best = dflt;
+      float bu = du;
+      for (int one : {0, 4, 8}) {
+        if (r != CT_STATUS_SUCCESS) break;
+        PackTiles c = t;
+        c.one = one;
+        float us = 0.0f;
+        r = time_tiles(len, rows, p->stream, P, io, c, 10, &us);
+        if (r == CT_STATUS_SUCCESS && us < bu) {
+          bu = us;
+          best = c;
+        }
+      }
       if (r != CT_STATUS_SUCCESS) return bail(r);
-      // the whole pass decides, against the default tiles
-      p->tiles = tu <= du ? t : dflt;
-      p->tunedUs = tu <= du ? tu : du;
+      p->tiles = best;
+      p->tunedUs = bu;
       p->defaultUs = du;
       std::lock_guard<std::mutex> lk(g_tune_mu);
       g_tuned[key] = Tuned{p->tiles, p->tunedUs, p->defaultUs};
@@ -1227,6 +1345,14 @@ CTAPI_EXPORT ctStatus_t ctx_pack_plan_create(const float* h, const float* l, con
       ctStatus_t r = time_tiles(len, rows, p->stream, P, io, p->tiles, 1, &us);
       if (r != CT_STATUS_SUCCESS) return bail(r);
       if (!cached) p->tunedUs = p->defaultUs = us;
+    }
+    {
+      const int parts = ((P.mask & kWinMask) ? 1 : 0) + ((P.mask & kCloseRecMask) ? 1 : 0) +
+                        ((P.mask & kHlcRecMask) ? 1 : 0);
+      const PackCfg& t = (P.mask & kCloseRecMask) ? p->tiles.close : p->tiles.hlc;
+      const bool one = p->tiles.one && parts >= 2 && window_halo(P) <= 64;
+      p->runK = one ? p->tiles.one : t.K;
+      p->runW = one ? kFuseW : t.W;
     }
     e = hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return bail(hip_fail(e, "plan: begin capture"));
@@ -1253,11 +1379,8 @@ CTAPI_EXPORT ctStatus_t ctx_plan_info(ctxPlan plan, int* barsPerLane, int* waves
                                       float* tunedUs, float* defaultUs) {
   return guarded([&]() -> ctStatus_t {
     if (!plan) return fail(CT_STATUS_INVALID_ARGUMENT, "plan: null plan");
-    // the close kernel's tile, or the high/low/close kernel's when the
-    // plan has no close recurrences
-    const PackCfg& t = (plan->mask & kCloseRecMask) ? plan->tiles.close : plan->tiles.hlc;
-    if (barsPerLane) *barsPerLane = t.K;
-    if (wavesPerGroup) *wavesPerGroup = t.W;
+    if (barsPerLane) *barsPerLane = plan->runK;
+    if (wavesPerGroup) *wavesPerGroup = plan->runW;
     if (tunedUs) *tunedUs = plan->tunedUs;
     if (defaultUs) *defaultUs = plan->defaultUs;
     return CT_STATUS_SUCCESS;

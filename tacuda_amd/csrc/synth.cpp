@@ -94,6 +94,10 @@ extern "C" CTAPI_EXPORT ctStatus_t ctx_synth_ohlcv(unsigned long long seed, int 
 extern "C" CTAPI_EXPORT ctStatus_t ctx_workspace_reserve(hipStream_t stream, size_t bytes) {
   return tca::guarded([&]() -> ctStatus_t {
     tca::Ws ws;
-    return tca::workspace_for(stream, bytes, &ws);
+    for (int part = 0; part < tca::kWsParts; ++part) {
+      ctStatus_t st = tca::workspace_for(stream, bytes, &ws, part);
+      if (st != CT_STATUS_SUCCESS) return st;
+    }
+    return CT_STATUS_SUCCESS;
   });
 }
