@@ -99,6 +99,20 @@ struct RaggedScope {
 bool ragged_geometry(const Ragged* r, int seg, Geometry* g, int* maxSegsPerRow);
 const Ragged* ragged_of(struct ctxRagged_st* layout);
 
+// ------------------------------------------------------- fused scan --
+// The fused long-series scan (EMA, KAMA, T3, SMA) without the stream-0
+// synchronization.  scan_serves: whether the family calls of those
+// indicators route a dense batch to it -- at least kScanMinRows rows of at
+// least two 1024-bar segments, kScanMinSegments segments in all (batches
+// that fill the GPU, where its two streaming passes beat the look-back
+// scan; a few long rows leave its one-wave-per-row carry pass
+// latency-bound, measured slower), and a window look-back within its halo.
+constexpr long long kScanMinSegments = 512;
+constexpr int kScanMinRows = 16;
+ctStatus_t scan_run(const float* close, int seriesLen, int numSeries, const ctxScanParams* params,
+                    const ctxScanOutputs* outputs, hipStream_t stream);
+bool scan_serves(int len, int rows, int windowLookback);
+
 // Synchronize when the caller used the legacy default stream: batch calls
 // on stream 0 return once the batch is done (reference api_batch.cu:377-386).
 inline ctStatus_t finish_batch(hipStream_t stream) {

@@ -221,22 +221,30 @@ __device__ __forceinline__ Fold fold_lane(const float* lds, const float* x, cons
   return F;
 }
 
-// Inclusive lane scan of the folds (lane 0 earliest).
+// Inclusive lane scan of the folds (lane 0 earliest), of the requested
+// recurrences only.
 __device__ __forceinline__ void scan_lanes(const ScanP& P, Fold& F) {
   const int lane = lane_id();
+  const bool wE = P.mask & kLEma, wT = P.mask & kLT3, wK = P.mask & kLKama;
 #pragma unroll
   for (int s = 0; s < 6; ++s) {
     const int d = 1 << s;
-    const double be = __shfl_up(F.be, d);
-    double bt[6];
+    if (wE) {
+      const double be = __shfl_up(F.be, d);
+      if (lane >= d) F.be = fma(P.ePow[s], be, F.be);
+    }
+    if (wT) {
+      double bt[6];
 #pragma unroll
-    for (int q = 0; q < 6; ++q) bt[q] = __shfl_up(F.bt[q], d);
-    const double Ak = __shfl_up(F.Ak, d), bk = __shfl_up(F.bk, d);
-    if (lane >= d) {
-      F.be = fma(P.ePow[s], be, F.be);
-      toe_mv_add(P.tPow[s], bt, F.bt);
-      F.bk = fma(F.Ak, bk, F.bk);
-      F.Ak *= Ak;
+      for (int q = 0; q < 6; ++q) bt[q] = __shfl_up(F.bt[q], d);
+      if (lane >= d) toe_mv_add(P.tPow[s], bt, F.bt);
+    }
+    if (wK) {
+      const double Ak = __shfl_up(F.Ak, d), bk = __shfl_up(F.bk, d);
+      if (lane >= d) {
+        F.bk = fma(F.Ak, bk, F.bk);
+        F.Ak *= Ak;
+      }
     }
   }
 }
@@ -591,9 +599,13 @@ __global__ __launch_bounds__(kLW* kWave) void k_scan_emit(const float* x, const 
     F.bk = fma(F.Ak, ck, F.bk);
   }
   scan_lanes(P, F);
-  double ye = __shfl_up(F.be, 1), yk = __shfl_up(F.bk, 1), yt[6];
+  double ye = 0.0, yk = 0.0, yt[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (P.mask & kLEma) ye = __shfl_up(F.be, 1);
+  if (P.mask & kLKama) yk = __shfl_up(F.bk, 1);
+  if (P.mask & kLT3) {
 #pragma unroll
-  for (int q = 0; q < 6; ++q) yt[q] = __shfl_up(F.bt[q], 1);
+    for (int q = 0; q < 6; ++q) yt[q] = __shfl_up(F.bt[q], 1);
+  }
   if (lane == 0) {
     ye = ce;
     yk = ck;
@@ -790,6 +802,87 @@ ctStatus_t launch_scan(const float* x, int* lead, double* agg, double* carry, co
 }
 
 }  // namespace
+
+ctStatus_t scan_run(const float* close, int seriesLen, int numSeries, const ctxScanParams* params,
+                    const ctxScanOutputs* outputs, hipStream_t stream) {
+  if (!close || !params || !outputs)
+    return fail(CT_STATUS_INVALID_ARGUMENT, "scan: null pointer");
+  if (seriesLen <= 0 || numSeries <= 0)
+    return fail(CT_STATUS_INVALID_ARGUMENT, "scan: seriesLen and numSeries must be positive");
+  ScanP P{};
+  P.mask = (outputs->ema ? kLEma : 0u) | (outputs->kama ? kLKama : 0u) |
+           (outputs->t3 ? kLT3 : 0u) | (outputs->sma ? kLSma : 0u);
+  if (!P.mask) return fail(CT_STATUS_INVALID_ARGUMENT, "scan: no output requested");
+  P.pe = (P.mask & kLEma) ? params->ema_period : 2;
+  P.pk = (P.mask & kLKama) ? params->kama_period : 2;
+  P.pt = (P.mask & kLT3) ? params->t3_period : 2;
+  P.ps = (P.mask & kLSma) ? params->sma_period : 1;
+  const int fast = (P.mask & kLKama) ? params->kama_fast : 2;
+  const int slow = (P.mask & kLKama) ? params->kama_slow : 30;
+  if (P.pe < 1 || P.pk < 1 || P.ps < 1 || fast < 1 || slow < 1)
+    return fail(CT_STATUS_INVALID_ARGUMENT, "scan: invalid period");
+  if (P.pt < 2) return fail(CT_STATUS_INVALID_ARGUMENT, "scan: T3 needs period >= 2");
+  const int halo = std::max(P.ps - 1, P.pk + 1);
+  if (halo > kLMaxHalo)
+    return fail(CT_STATUS_UNSUPPORTED, "scan: window look-back above %d bars is not provided",
+                kLMaxHalo);
+  P.n = seriesLen;
+  P.rows = numSeries;
+  P.segsPerRow = div_up(seriesLen, kLSeg);
+  if ((long long)P.segsPerRow * numSeries > (1ll << 30))
+    return fail(CT_STATUS_UNSUPPORTED, "scan: batch too large");
+  P.cpl = div_up(P.segsPerRow, kWave);
+  P.useCarry = P.segsPerRow > 1 && (P.mask & (kLEma | kLKama | kLT3)) ? 1 : 0;
+  {
+    double b = (double)(P.pe - 1) / (double)(P.pe + 1);
+    const double kk = 1.0 - b;
+    if (b < 0.5) b = 1.0 - kk;
+    P.eb = b;
+    P.ek = kk;
+  }
+  P.kmax = 2.0 / (slow + 1.0);
+  P.kdiff = 2.0 / (fast + 1.0) - P.kmax;
+  P.tk = 2.0 / (P.pt + 1.0);
+  P.tomk = 1.0 - P.tk;
+  {
+    const double v = (double)params->t3_vfactor, tmp = v * v;
+    P.c1 = -(tmp * v);
+    P.c2 = 3.0 * (tmp - P.c1);
+    P.c3 = -6.0 * tmp - 3.0 * (v - P.c1);
+    P.c4 = std::fma(3.0, tmp, std::fma(3.0, v, 1.0) - P.c1);
+  }
+  Toe M;
+  for (int i = 0; i < 6; ++i) M.c[i] = P.tomk * std::pow(P.tk, (double)i);
+  for (int s = 0; s < 6; ++s) {
+    const long long L = (long long)kLK << s, R = (long long)kLSeg * P.cpl << s;
+    P.ePow[s] = hpow(P.eb, L);
+    P.eRow[s] = hpow(P.eb, R);
+    P.tPow[s] = toe_pow(M, L);
+    P.tRow[s] = toe_pow(M, R);
+  }
+  P.eSeg = hpow(P.eb, kLSeg);
+  P.tSeg = toe_pow(M, kLSeg);
+  const size_t segs = (size_t)P.segsPerRow * numSeries;
+  const size_t leadB = ((size_t)numSeries * sizeof(int) + 255) & ~(size_t)255;
+  const size_t bytes = leadB + segs * (9 + 8) * sizeof(double);
+  Ws ws;
+  ctStatus_t st = workspace_for(stream, bytes, &ws);
+  if (st != CT_STATUS_SUCCESS) return st;
+  int* lead = reinterpret_cast<int*>(ws.area);
+  double* agg = reinterpret_cast<double*>(reinterpret_cast<char*>(ws.area) + leadB);
+  double* carry = agg + segs * 9;
+  const ScanOut o{outputs->ema, outputs->kama, outputs->t3, outputs->sma};
+  if (halo <= 32) return launch_scan<32>(close, lead, agg, carry, P, o, stream);
+  if (halo <= 256) return launch_scan<256>(close, lead, agg, carry, P, o, stream);
+  return launch_scan<kLMaxHalo>(close, lead, agg, carry, P, o, stream);
+}
+
+bool scan_serves(int len, int rows, int windowLookback) {
+  const long long segsPerRow = (len + kLSeg - 1) / kLSeg;
+  return rows >= kScanMinRows && segsPerRow >= 2 && segsPerRow * rows >= kScanMinSegments &&
+         windowLookback <= kLMaxHalo;
+}
+
 }  // namespace tca
 
 using namespace tca;
@@ -800,76 +893,7 @@ CTAPI_EXPORT ctStatus_t ctx_scan_batch_d(const float* close, int seriesLen, int 
                                          const ctxScanParams* params,
                                          const ctxScanOutputs* outputs, hipStream_t stream) {
   return guarded([&]() -> ctStatus_t {
-    if (!close || !params || !outputs)
-      return fail(CT_STATUS_INVALID_ARGUMENT, "scan: null pointer");
-    if (seriesLen <= 0 || numSeries <= 0)
-      return fail(CT_STATUS_INVALID_ARGUMENT, "scan: seriesLen and numSeries must be positive");
-    ScanP P{};
-    P.mask = (outputs->ema ? kLEma : 0u) | (outputs->kama ? kLKama : 0u) |
-             (outputs->t3 ? kLT3 : 0u) | (outputs->sma ? kLSma : 0u);
-    if (!P.mask) return fail(CT_STATUS_INVALID_ARGUMENT, "scan: no output requested");
-    P.pe = (P.mask & kLEma) ? params->ema_period : 2;
-    P.pk = (P.mask & kLKama) ? params->kama_period : 2;
-    P.pt = (P.mask & kLT3) ? params->t3_period : 2;
-    P.ps = (P.mask & kLSma) ? params->sma_period : 1;
-    const int fast = (P.mask & kLKama) ? params->kama_fast : 2;
-    const int slow = (P.mask & kLKama) ? params->kama_slow : 30;
-    if (P.pe < 1 || P.pk < 1 || P.ps < 1 || fast < 1 || slow < 1)
-      return fail(CT_STATUS_INVALID_ARGUMENT, "scan: invalid period");
-    if (P.pt < 2) return fail(CT_STATUS_INVALID_ARGUMENT, "scan: T3 needs period >= 2");
-    const int halo = std::max(P.ps - 1, P.pk + 1);
-    if (halo > kLMaxHalo)
-      return fail(CT_STATUS_UNSUPPORTED, "scan: window look-back above %d bars is not provided",
-                  kLMaxHalo);
-    P.n = seriesLen;
-    P.rows = numSeries;
-    P.segsPerRow = div_up(seriesLen, kLSeg);
-    if ((long long)P.segsPerRow * numSeries > (1ll << 30))
-      return fail(CT_STATUS_UNSUPPORTED, "scan: batch too large");
-    P.cpl = div_up(P.segsPerRow, kWave);
-    P.useCarry = P.segsPerRow > 1 && (P.mask & (kLEma | kLKama | kLT3)) ? 1 : 0;
-    {
-      double b = (double)(P.pe - 1) / (double)(P.pe + 1);
-      const double kk = 1.0 - b;
-      if (b < 0.5) b = 1.0 - kk;
-      P.eb = b;
-      P.ek = kk;
-    }
-    P.kmax = 2.0 / (slow + 1.0);
-    P.kdiff = 2.0 / (fast + 1.0) - P.kmax;
-    P.tk = 2.0 / (P.pt + 1.0);
-    P.tomk = 1.0 - P.tk;
-    {
-      const double v = (double)params->t3_vfactor, tmp = v * v;
-      P.c1 = -(tmp * v);
-      P.c2 = 3.0 * (tmp - P.c1);
-      P.c3 = -6.0 * tmp - 3.0 * (v - P.c1);
-      P.c4 = std::fma(3.0, tmp, std::fma(3.0, v, 1.0) - P.c1);
-    }
-    Toe M;
-    for (int i = 0; i < 6; ++i) M.c[i] = P.tomk * std::pow(P.tk, (double)i);
-    for (int s = 0; s < 6; ++s) {
-      const long long L = (long long)kLK << s, R = (long long)kLSeg * P.cpl << s;
-      P.ePow[s] = hpow(P.eb, L);
-      P.eRow[s] = hpow(P.eb, R);
-      P.tPow[s] = toe_pow(M, L);
-      P.tRow[s] = toe_pow(M, R);
-    }
-    P.eSeg = hpow(P.eb, kLSeg);
-    P.tSeg = toe_pow(M, kLSeg);
-    const size_t segs = (size_t)P.segsPerRow * numSeries;
-    const size_t leadB = ((size_t)numSeries * sizeof(int) + 255) & ~(size_t)255;
-    const size_t bytes = leadB + segs * (9 + 8) * sizeof(double);
-    Ws ws;
-    ctStatus_t st = workspace_for(stream, bytes, &ws);
-    if (st != CT_STATUS_SUCCESS) return st;
-    int* lead = reinterpret_cast<int*>(ws.area);
-    double* agg = reinterpret_cast<double*>(reinterpret_cast<char*>(ws.area) + leadB);
-    double* carry = agg + segs * 9;
-    const ScanOut o{outputs->ema, outputs->kama, outputs->t3, outputs->sma};
-    if (halo <= 32) st = launch_scan<32>(close, lead, agg, carry, P, o, stream);
-    else if (halo <= 256) st = launch_scan<256>(close, lead, agg, carry, P, o, stream);
-    else st = launch_scan<kLMaxHalo>(close, lead, agg, carry, P, o, stream);
+    ctStatus_t st = scan_run(close, seriesLen, numSeries, params, outputs, stream);
     if (st != CT_STATUS_SUCCESS) return st;
     return finish_batch(stream);
   });

@@ -919,8 +919,12 @@ static ctStatus_t launch_part(PackCfg cfg, int len, int rows, hipStream_t s, con
 
 // ------------------------------------------------------- one launch --
 // The window kernel's and both recurrence kernels' workgroups in one grid,
-// [windows | close | high/low/close], kFuseW waves each: recurrence waves
-// that wait on look-back leave the ALUs to the window waves.  Each
+// kFuseW waves each.  Rows of several segments put the recurrence blocks
+// first, [close | high/low/close | windows]: their look-back chains set the
+// pass's length, and their waves waiting on look-back leave the ALUs to the
+// window waves.  Rows of one segment (no look-back) put the heavier window
+// blocks first, [windows | close | high/low/close], which shortens the
+// tail.  Each
 // recurrence part scans through its own workspace part; tickets within a
 // part follow block order, so every predecessor a wave waits for was
 // dispatched before it.  Window look-backs up to 64 bars.
@@ -948,14 +952,16 @@ __global__ __launch_bounds__(kFuseW* kWave) void k_pack_one(Geometry gw, Geometr
   __shared__ __align__(16) FuseLds<K> sm;
   const int w = threadIdx.x / kWave;
   int b = blockIdx.x;
-  if (b < nbW) {
+  // windows-first grids rotate the index so the same tests below apply
+  if (!useLB) b = b >= nbW ? b - nbW : b + nbC + nbH;
+  if (b >= nbC + nbH) {
+    b -= nbC + nbH;
     Seg c;
     c.tree = false;
     c.epoch = 0;
     if (resolve(gw, C::SEG, b * kFuseW + w, c)) pack_windows<C>(c, sm.win[w], prm, io);
     return;
   }
-  b -= nbW;
   const bool cl = b < nbC;
   if (!cl) b -= nbC;
   const Ws ws = cl ? wsC : wsH;

@@ -181,10 +181,37 @@ namespace tca {
     if (st_ != CT_STATUS_SUCCESS) return st_;   \
   } while (0)
 
+// Dense batches of long rows of EMA, KAMA, T3 and SMA run through the fused
+// scan's two streaming passes (scan_serves): the same oracle-following
+// results without a look-back, which batches of thousands of waiting
+// segments pay for in latency.
+ctStatus_t via_scan(const float* x, int len, int rows, hipStream_t s, ctxScanOutputs so,
+                    ctxScanParams sp) {
+  return scan_run(x, len, rows, &sp, &so, s);
+}
+ctxScanParams scan_params() {
+  ctxScanParams sp{};
+  sp.ema_period = 2;
+  sp.kama_period = 2;
+  sp.kama_fast = 2;
+  sp.kama_slow = 30;
+  sp.t3_period = 2;
+  sp.t3_vfactor = 0.7f;
+  sp.sma_period = 1;
+  return sp;
+}
+
 ctStatus_t dev_sma(const float* x, float* y, int len, int rows, int p, hipStream_t s) {
   CHECK_PTR(x, "sma"); CHECK_PTR(y, "sma");
   CHECK_RET(check_shape("sma", len, rows));
   if (p <= 0) return bad("sma", "invalid period");
+  if (!t_ragged && scan_serves(len, rows, p - 1)) {
+    ctxScanParams sp = scan_params();
+    sp.sma_period = p;
+    ctxScanOutputs so{};
+    so.sma = y;
+    return via_scan(x, len, rows, s, so, sp);
+  }
   return run_Sma(len, rows, s, {p}, io1(x, y));
 }
 ctStatus_t dev_sum(const float* x, float* y, int len, int rows, int p, hipStream_t s) {
@@ -203,6 +230,13 @@ ctStatus_t dev_ema(const float* x, float* y, int len, int rows, int p, hipStream
   CHECK_PTR(x, "ema"); CHECK_PTR(y, "ema");
   CHECK_RET(check_shape("ema", len, rows));
   if (p <= 0) return bad("ema", "invalid period");
+  if (!t_ragged && scan_serves(len, rows, 0)) {
+    ctxScanParams sp = scan_params();
+    sp.ema_period = p;
+    ctxScanOutputs so{};
+    so.ema = y;
+    return via_scan(x, len, rows, s, so, sp);
+  }
   return run_Ema(len, rows, s, {p}, io1(x, y));
 }
 ctStatus_t dev_dema(const float* x, float* y, int len, int rows, int p, hipStream_t s) {
@@ -221,6 +255,14 @@ ctStatus_t dev_t3(const float* x, float* y, int len, int rows, int p, float v, h
   CHECK_PTR(x, "t3"); CHECK_PTR(y, "t3");
   CHECK_RET(check_shape("t3", len, rows));
   if (p < 2) return bad("t3", "invalid period (T3 needs period >= 2)");
+  if (!t_ragged && scan_serves(len, rows, 0)) {
+    ctxScanParams sp = scan_params();
+    sp.t3_period = p;
+    sp.t3_vfactor = v;
+    ctxScanOutputs so{};
+    so.t3 = y;
+    return via_scan(x, len, rows, s, so, sp);
+  }
   return run_T3(len, rows, s, {p, (double)v}, io1(x, y));
 }
 ctStatus_t dev_kama(const float* x, float* y, int len, int rows, int p, int fast, int slow,
@@ -228,6 +270,15 @@ ctStatus_t dev_kama(const float* x, float* y, int len, int rows, int p, int fast
   CHECK_PTR(x, "kama"); CHECK_PTR(y, "kama");
   CHECK_RET(check_shape("kama", len, rows));
   if (p <= 0 || fast <= 0 || slow <= 0) return bad("kama", "invalid period");
+  if (!t_ragged && scan_serves(len, rows, p + 1)) {
+    ctxScanParams sp = scan_params();
+    sp.kama_period = p;
+    sp.kama_fast = fast;
+    sp.kama_slow = slow;
+    ctxScanOutputs so{};
+    so.kama = y;
+    return via_scan(x, len, rows, s, so, sp);
+  }
   return run_Kama(len, rows, s, {p, fast, slow}, io1(x, y));
 }
 ctStatus_t dev_rsi(const float* x, float* y, int len, int rows, int p, hipStream_t s) {

@@ -133,3 +133,24 @@ def test_scan_full_shape_sampled_rows(dev):
         assert torch.equal(got[k][100:104].view(torch.int32), part[k].view(torch.int32)), k
     del got, part, x
     torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("shape", [(128, 5000), (16, 40000), (1, 600000)])
+def test_family_calls_routed_through_the_scan(dev, shape):
+    """ct_ema/kama/t3/sma batch calls on batches the fused scan serves (at
+    least 16 rows of two or more 1024-bar segments, 512 segments in all) keep
+    the family contract -- the oracle within tolerance, NaN masks equal -- as
+    does a single long row, which stays on the look-back kernels."""
+    import tacuda_amd as ta
+
+    rows, n = shape
+    print(f"scan-routed family calls {rows}x{n}", flush=True)
+    c = ta.synth_ohlcv(n, rows=rows, seed=21)["close"].copy()
+    c2 = c.reshape(-1, n)
+    c2[0, :700] = np.nan
+    c2[-1, n // 2] = np.nan
+    x = torch.from_numpy(np.ascontiguousarray(c)).to(dev)
+    got = {"ema": ta.ops.ema(x, 200), "kama": ta.ops.kama(x, 30), "t3": ta.ops.t3(x, 5, 0.7),
+           "sma": ta.ops.sma(x, 200)}
+    torch.cuda.synchronize()
+    _check(got, c, DEFAULT)
