@@ -367,10 +367,13 @@ __global__ __launch_bounds__(64) void k_scan_lead(const float* x, int n, int row
   if (lane_id() == 0) lead[row] = f;
 }
 
-template <int H>
+// FM: the requested families when fixed at compile time (a single-family
+// call), so the others' code and registers drop out; 0 = P.mask at run time.
+template <int H, unsigned FM>
 __global__ __launch_bounds__(kLW* kWave) void k_scan_fold(const float* x, const int* lead, ScanP P,
                                                           double* agg) {
   using T = Tile<kLK, H>;
+  if (FM) P.mask = FM;
   __shared__ __align__(16) float s_in[kLW][T::PLEN];
   const int w = threadIdx.x / kWave;
   const int seg = blockIdx.x * kLW + w;
@@ -561,10 +564,11 @@ __device__ __forceinline__ double sma_window(const float* lds, const double* pc,
   return s;
 }
 
-template <int H>
+template <int H, unsigned FM>
 __global__ __launch_bounds__(kLW* kWave) void k_scan_emit(const float* x, const int* lead, ScanP P,
                                                           const double* carry, ScanOut out) {
   using T = Tile<kLK, H>;
+  if (FM) P.mask = FM;
   __shared__ __align__(16) float s_in[kLW][T::PLEN];
   __shared__ double s_pc[kLW][T::LEN / kLK + 1];
   const int w = threadIdx.x / kWave;
@@ -783,7 +787,7 @@ Toe toe_pow(const Toe& m, long long e) {
   return r;
 }
 
-template <int H>
+template <int H, unsigned FM>
 ctStatus_t launch_scan(const float* x, int* lead, double* agg, double* carry, const ScanP& P,
                        const ScanOut& o, hipStream_t s) {
   const int segs = P.rows * P.segsPerRow;
@@ -791,12 +795,14 @@ ctStatus_t launch_scan(const float* x, int* lead, double* agg, double* carry, co
   hipLaunchKernelGGL(k_scan_lead, dim3(P.rows), dim3(kWave), 0, s, x, P.n, P.rows, lead);
   TCA_HIP(hipGetLastError());
   if (P.useCarry) {
-    hipLaunchKernelGGL((k_scan_fold<H>), dim3(blocks), dim3(kLW * kWave), 0, s, x, lead, P, agg);
+    hipLaunchKernelGGL((k_scan_fold<H, FM>), dim3(blocks), dim3(kLW * kWave), 0, s, x, lead, P,
+                       agg);
     TCA_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_scan_carry, dim3(P.rows), dim3(kWave), 0, s, P, agg, carry);
     TCA_HIP(hipGetLastError());
   }
-  hipLaunchKernelGGL((k_scan_emit<H>), dim3(blocks), dim3(kLW * kWave), 0, s, x, lead, P, carry, o);
+  hipLaunchKernelGGL((k_scan_emit<H, FM>), dim3(blocks), dim3(kLW * kWave), 0, s, x, lead, P,
+                     carry, o);
   TCA_HIP(hipGetLastError());
   return CT_STATUS_SUCCESS;
 }
@@ -872,9 +878,22 @@ ctStatus_t scan_run(const float* close, int seriesLen, int numSeries, const ctxS
   double* agg = reinterpret_cast<double*>(reinterpret_cast<char*>(ws.area) + leadB);
   double* carry = agg + segs * 9;
   const ScanOut o{outputs->ema, outputs->kama, outputs->t3, outputs->sma};
-  if (halo <= 32) return launch_scan<32>(close, lead, agg, carry, P, o, stream);
-  if (halo <= 256) return launch_scan<256>(close, lead, agg, carry, P, o, stream);
-  return launch_scan<kLMaxHalo>(close, lead, agg, carry, P, o, stream);
+  // single-family calls run kernels compiled for that family alone
+  if (P.mask == kLEma) return launch_scan<32, kLEma>(close, lead, agg, carry, P, o, stream);
+  if (P.mask == kLT3) return launch_scan<32, kLT3>(close, lead, agg, carry, P, o, stream);
+  if (P.mask == kLKama) {
+    if (halo <= 32) return launch_scan<32, kLKama>(close, lead, agg, carry, P, o, stream);
+    if (halo <= 256) return launch_scan<256, kLKama>(close, lead, agg, carry, P, o, stream);
+    return launch_scan<kLMaxHalo, kLKama>(close, lead, agg, carry, P, o, stream);
+  }
+  if (P.mask == kLSma) {
+    if (halo <= 32) return launch_scan<32, kLSma>(close, lead, agg, carry, P, o, stream);
+    if (halo <= 256) return launch_scan<256, kLSma>(close, lead, agg, carry, P, o, stream);
+    return launch_scan<kLMaxHalo, kLSma>(close, lead, agg, carry, P, o, stream);
+  }
+  if (halo <= 32) return launch_scan<32, 0>(close, lead, agg, carry, P, o, stream);
+  if (halo <= 256) return launch_scan<256, 0>(close, lead, agg, carry, P, o, stream);
+  return launch_scan<kLMaxHalo, 0>(close, lead, agg, carry, P, o, stream);
 }
 
 bool scan_serves(int len, int rows, int windowLookback) {
