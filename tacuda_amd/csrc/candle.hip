@@ -26,6 +26,8 @@ constexpr int kHalo = 15;  // bars read before a tile: 5 candles back + 10-bar t
 constexpr int kSumHalo = 5;  // averages are taken up to 5 candles back (HIKKAKEMOD)
 
 // LDS tile accessor: bars from `base`, window totals from `sbase`.
+// rb10, hl10, sh10, hl5 hold the window means (total / avgPeriod), divided
+// once per bar when the tile is built rather than at every use.
 struct CdlCtx {
   const float *so, *sh, *sl, *sc;
   const double *rb10, *hl10, *sh10, *hl5;
@@ -38,17 +40,17 @@ struct CdlCtx {
   __device__ double avg(int s, int j) const {
     const int k = j - sbase;
     switch (s) {
-      case BodyLong: return 1.0 * (rb10[k] / 10) / 1.0;
-      case BodyVeryLong: return 3.0 * (rb10[k] / 10) / 1.0;
-      case BodyShort: return 1.0 * (rb10[k] / 10) / 1.0;
-      case BodyDoji: return 0.1 * (hl10[k] / 10) / 1.0;
+      case BodyLong: return 1.0 * rb10[k] / 1.0;
+      case BodyVeryLong: return 3.0 * rb10[k] / 1.0;
+      case BodyShort: return 1.0 * rb10[k] / 1.0;
+      case BodyDoji: return 0.1 * hl10[k] / 1.0;
       case ShadowLong: return 1.0 * fabs(c(j) - o(j)) / 1.0;
       case ShadowVeryLong: return 2.0 * fabs(c(j) - o(j)) / 1.0;
-      case ShadowShort: return 1.0 * (sh10[k] / 10) / 2.0;
-      case ShadowVeryShort: return 0.1 * (hl10[k] / 10) / 1.0;
-      case Near: return 0.2 * (hl5[k] / 5) / 1.0;
-      case Far: return 0.6 * (hl5[k] / 5) / 1.0;
-      default: return 0.05 * (hl5[k] / 5) / 1.0;  // Equal
+      case ShadowShort: return 1.0 * sh10[k] / 2.0;
+      case ShadowVeryShort: return 0.1 * hl10[k] / 1.0;
+      case Near: return 0.2 * hl5[k] / 1.0;
+      case Far: return 0.6 * hl5[k] / 1.0;
+      default: return 0.05 * hl5[k] / 1.0;  // Equal
     }
   }
 };
@@ -124,7 +126,9 @@ __device__ __forceinline__ int eval(int k, const CdlCtx& X, int i) {
 
 // out: plane k of pattern k at out + k*planeStride (only set bits of mask
 // are written; a single pattern uses planeStride 0).
-template <bool I32>
+// ALL: every pattern, as straight-line code (shared subexpressions of the
+// rules are computed once); otherwise the patterns of `mask`, one at a time.
+template <bool I32, bool ALL>
 __global__ __launch_bounds__(kTile) void k_cdl(const float* o, const float* h, const float* l,
                                                const float* c, void* out, size_t planeStride,
                                                unsigned long long mask, int n, int rows,
@@ -159,25 +163,33 @@ __global__ __launch_bounds__(kTile) void k_cdl(const float* o, const float* h, c
       s += (hu - fmax(ou, cu)) + (fmin(ou, cu) - lu);
       if (u >= j - 5) h5 += hu - lu;
     }
-    rb10[e] = r;
-    hl10[e] = hl;
-    sh10[e] = s;
-    hl5[e] = h5;
+    rb10[e] = r / 10;
+    hl10[e] = hl / 10;
+    sh10[e] = s / 10;
+    hl5[e] = h5 / 5;
   }
   __syncthreads();
   const int i = j0 + threadIdx.x;
   if (i >= n) return;
   const CdlCtx X{so, sh, sl, sc, rb10, hl10, sh10, hl5, base, sbase};
-  unsigned long long m = mask;
-  while (m) {
-    const int k = __builtin_ctzll(m);
-    m &= m - 1;
-    const int v = i < c_lookback[k] ? 0 : eval(k, X, i);
+  auto put = [&](int k, int v) {
     const size_t idx = (size_t)k * planeStride + ro + i;
     if (I32)
       __builtin_nontemporal_store(v, static_cast<int32_t*>(out) + idx);
     else
       __builtin_nontemporal_store((float)v, static_cast<float*>(out) + idx);
+  };
+  if (ALL) {
+#define TCA_ALL(name, lb, fn) put(k_##name, i < lb ? 0 : fn(X, i));
+    TCA_CDL_LIST(TCA_ALL)
+#undef TCA_ALL
+    return;
+  }
+  unsigned long long m = mask;
+  while (m) {
+    const int k = __builtin_ctzll(m);
+    m &= m - 1;
+    put(k, i < c_lookback[k] ? 0 : eval(k, X, i));
   }
 }
 
@@ -190,12 +202,16 @@ ctStatus_t launch(const float* o, const float* h, const float* l, const float* c
   const int tiles = (n + kTile - 1) / kTile;
   const long long blocks = (long long)tiles * rows;
   if (blocks > 0x7fffffffLL) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: matrix too large", fn);
-  if (i32)
-    hipLaunchKernelGGL(k_cdl<true>, dim3((unsigned)blocks), dim3(kTile), 0, s, o, h, l, c, out,
-                       planeStride, mask, n, rows, tiles);
+  const bool all = mask == (~0ull >> (64 - kCount)) && planeStride != 0;
+  if (i32 && all)
+    hipLaunchKernelGGL((k_cdl<true, true>), dim3((unsigned)blocks), dim3(kTile), 0, s, o, h, l, c,
+                       out, planeStride, mask, n, rows, tiles);
+  else if (i32)
+    hipLaunchKernelGGL((k_cdl<true, false>), dim3((unsigned)blocks), dim3(kTile), 0, s, o, h, l,
+                       c, out, planeStride, mask, n, rows, tiles);
   else
-    hipLaunchKernelGGL(k_cdl<false>, dim3((unsigned)blocks), dim3(kTile), 0, s, o, h, l, c, out,
-                       planeStride, mask, n, rows, tiles);
+    hipLaunchKernelGGL((k_cdl<false, false>), dim3((unsigned)blocks), dim3(kTile), 0, s, o, h, l,
+                       c, out, planeStride, mask, n, rows, tiles);
   TCA_HIP(hipGetLastError());
   return CT_STATUS_SUCCESS;
 }
