@@ -48,6 +48,7 @@ struct PackCfg {
 struct PackTiles {
   PackCfg close, hlc;
   int one;
+  int twoPass;  // long dense rows may take the two-pass recurrences
 };
 
 struct PackIO {
@@ -1012,12 +1013,926 @@ static ctStatus_t launch_one(int len, int rows, hipStream_t s, const PackP& prm,
   return CT_STATUS_SUCCESS;
 }
 
+// ---------------------------------------------------------- two passes --
+// Long dense rows: the fused pass's recurrences -- EMA, RSI, MACD with its
+// signal, ATR -- in two streaming passes instead of look-back chains, like
+// the fused long scan: a fold pass writes every 512-bar segment's composite
+// (the affine map of its bars, from which the state after the segment
+// follows from the state before it), a wave per 64 segments scans them into
+// in-group prefixes and group composites, a wave per row scans those into
+// the state entering every group, and an emit pass
+// replays the bars from there with the families' TA-Lib arithmetic, its
+// window workgroups (the window part of the pass) alongside.
+//
+// Every recurrence maps y -> A y + b over a run of bars: identity before its
+// seed bar, a reset to the seed value at it, the family's step after it.
+// The MACD signal g also reads the MACD line, so its row is
+// g -> agf f + ags s + agg g + bgg over the fast and slow EMAs f and s.
+// Products of nonzero multipliers that underflow are clamped to the
+// smallest normal double, so a carried Inf survives them as in the
+// sequential recurrence.
+constexpr int kPK = 8;               // bars per lane
+constexpr int kPSeg = kPK * kWave;   // bars per segment
+constexpr int kPW = 4;               // waves per workgroup
+constexpr int kPMinLen = 16 * kPSeg; // shortest row taking the two passes
+constexpr unsigned kPUnsupported = (1u << kODema) | (1u << kOTema) | (1u << kOAdx);
+
+// Multiplier fields of a composite, in the order of PipeP::q: Ae, Ag, Al,
+// Af, As, agf, ags, agg, Aa.
+constexpr int kQF = 9;
+
+struct PipeP {
+  int n, rows, segsPerRow;
+  int wE, wR, wM, wA;
+  int pe, pr, fast, slow, sig, pa;
+  double eb, ek, inv, pm1, br, fb, fk, sb, sk, gb, gk, wb, wa;
+  // Multipliers of a steady run of 8 * 2^s bars (every bar past every seed):
+  // they do not depend on the bars, so steady segments fold and scan only
+  // the offsets.  q[0] is one lane's run, q[s + 1] = q[s] after q[s].
+  double q[7][kQF];
+  // Per-lane multipliers of the two row-broadcast scan steps (lane l of a
+  // steady wave after the row-shift steps, then after the first broadcast):
+  // qt[(k * kQF + field) * 64 + lane], written by k_pipe_lead.
+  double* qt;
+};
+
+struct PC {
+  double Ae, be, Ag, bg, Al, bl, Af, bf, As, bs, agf, ags, agg, bgg, Aa, ba;
+};
+
+// Recurrence groups of a composite: EMA (e), RSI (G, L), MACD (f, s, g), ATR (a).
+constexpr unsigned kGE = 1, kGR = 2, kGM = 4, kGA = 8, kGAll = 15;
+
+__device__ __forceinline__ void pc_id(PC& p) {
+  p.Ae = p.Ag = p.Al = p.Af = p.As = p.Aa = p.agg = 1.0;
+  p.be = p.bg = p.bl = p.bf = p.bs = p.ba = p.agf = p.ags = p.bgg = 0.0;
+}
+
+__device__ __forceinline__ double mulc(double a, double m) {
+  const double r = a * m;
+  return (r == 0.0 && a != 0.0 && m != 0.0) ? 2.2250738585072014e-308 : r;
+}
+
+// l o e: the run e followed by the run l.
+__device__ __forceinline__ PC pc_after(const PC& e, const PC& l) {
+  PC r;
+  r.be = fma(l.Ae, e.be, l.be);
+  r.Ae = mulc(l.Ae, e.Ae);
+  r.bg = fma(l.Ag, e.bg, l.bg);
+  r.Ag = mulc(l.Ag, e.Ag);
+  r.bl = fma(l.Al, e.bl, l.bl);
+  r.Al = mulc(l.Al, e.Al);
+  r.bf = fma(l.Af, e.bf, l.bf);
+  r.Af = mulc(l.Af, e.Af);
+  r.bs = fma(l.As, e.bs, l.bs);
+  r.As = mulc(l.As, e.As);
+  r.ba = fma(l.Aa, e.ba, l.ba);
+  r.Aa = mulc(l.Aa, e.Aa);
+  r.agf = fma(l.agf, e.Af, l.agg * e.agf);
+  r.ags = fma(l.ags, e.As, l.agg * e.ags);
+  r.agg = l.agg * e.agg;
+  r.bgg = fma(l.agf, e.bf, fma(l.ags, e.bs, fma(l.agg, e.bgg, l.bgg)));
+  return r;
+}
+
+// The composite of the lane before (identity in lane 0).
+__device__ __forceinline__ PC pc_prev_lane(const PC& p) {
+  PC r;
+  r.Ae = dpp_d<kWaveShr1, 0xf, 0xf>(1.0, p.Ae); r.be = dpp_d<kWaveShr1, 0xf, 0xf>(0.0, p.be);
+  r.Ag = dpp_d<kWaveShr1, 0xf, 0xf>(1.0, p.Ag); r.bg = dpp_d<kWaveShr1, 0xf, 0xf>(0.0, p.bg);
+  r.Al = dpp_d<kWaveShr1, 0xf, 0xf>(1.0, p.Al); r.bl = dpp_d<kWaveShr1, 0xf, 0xf>(0.0, p.bl);
+  r.Af = dpp_d<kWaveShr1, 0xf, 0xf>(1.0, p.Af); r.bf = dpp_d<kWaveShr1, 0xf, 0xf>(0.0, p.bf);
+  r.As = dpp_d<kWaveShr1, 0xf, 0xf>(1.0, p.As); r.bs = dpp_d<kWaveShr1, 0xf, 0xf>(0.0, p.bs);
+  r.agf = dpp_d<kWaveShr1, 0xf, 0xf>(0.0, p.agf); r.ags = dpp_d<kWaveShr1, 0xf, 0xf>(0.0, p.ags);
+  r.agg = dpp_d<kWaveShr1, 0xf, 0xf>(1.0, p.agg); r.bgg = dpp_d<kWaveShr1, 0xf, 0xf>(0.0, p.bgg);
+  r.Aa = dpp_d<kWaveShr1, 0xf, 0xf>(1.0, p.Aa); r.ba = dpp_d<kWaveShr1, 0xf, 0xf>(0.0, p.ba);
+  return r;
+}
+
+// One DPP step of the wave scan, one recurrence group at a time (the same
+// composition as pc_after).  Lanes the step does not reach read the identity,
+// whose composition is exact, so no lane branches.
+template <int CTRL, int RM, int BM, unsigned GM>
+__device__ __forceinline__ void pc_step(PC& F) {
+  if constexpr (GM & kGE) {
+    const double A = dpp_d<CTRL, RM, BM>(1.0, F.Ae), b = dpp_d<CTRL, RM, BM>(0.0, F.be);
+    F.be = fma(F.Ae, b, F.be);
+    F.Ae = mulc(F.Ae, A);
+  }
+  if constexpr (GM & kGR) {
+    const double Ag = dpp_d<CTRL, RM, BM>(1.0, F.Ag), bg = dpp_d<CTRL, RM, BM>(0.0, F.bg);
+    const double Al = dpp_d<CTRL, RM, BM>(1.0, F.Al), bl = dpp_d<CTRL, RM, BM>(0.0, F.bl);
+    F.bg = fma(F.Ag, bg, F.bg);
+    F.Ag = mulc(F.Ag, Ag);
+    F.bl = fma(F.Al, bl, F.bl);
+    F.Al = mulc(F.Al, Al);
+  }
+  if constexpr (GM & kGM) {
+    const double Af = dpp_d<CTRL, RM, BM>(1.0, F.Af), bf = dpp_d<CTRL, RM, BM>(0.0, F.bf);
+    const double As = dpp_d<CTRL, RM, BM>(1.0, F.As), bs = dpp_d<CTRL, RM, BM>(0.0, F.bs);
+    const double agf = dpp_d<CTRL, RM, BM>(0.0, F.agf), ags = dpp_d<CTRL, RM, BM>(0.0, F.ags);
+    const double agg = dpp_d<CTRL, RM, BM>(1.0, F.agg), bgg = dpp_d<CTRL, RM, BM>(0.0, F.bgg);
+    F.bgg = fma(F.agf, bf, fma(F.ags, bs, fma(F.agg, bgg, F.bgg)));
+    F.agf = fma(F.agf, Af, F.agg * agf);
+    F.ags = fma(F.ags, As, F.agg * ags);
+    F.agg = F.agg * agg;
+    F.bf = fma(F.Af, bf, F.bf);
+    F.Af = mulc(F.Af, Af);
+    F.bs = fma(F.As, bs, F.bs);
+    F.As = mulc(F.As, As);
+  }
+  if constexpr (GM & kGA) {
+    const double A = dpp_d<CTRL, RM, BM>(1.0, F.Aa), b = dpp_d<CTRL, RM, BM>(0.0, F.ba);
+    F.ba = fma(F.Aa, b, F.ba);
+    F.Aa = mulc(F.Aa, A);
+  }
+}
+
+// Inclusive scan over the wave's lanes (lane 0 earliest).
+template <unsigned GM = kGAll>
+__device__ __forceinline__ void pc_scan(PC& F) {
+  pc_step<kRowShr1, 0xf, 0xf, GM>(F);
+  pc_step<kRowShr2, 0xf, 0xf, GM>(F);
+  pc_step<kRowShr4, 0xf, 0xf, GM>(F);
+  pc_step<kRowShr8, 0xf, 0xf, GM>(F);
+  pc_step<kRowBcast15, 0xa, 0xf, GM>(F);
+  pc_step<kRowBcast31, 0xc, 0xf, GM>(F);
+}
+
+__device__ __forceinline__ void pc_set_a(PC& F, const double (&q)[kQF]) {
+  F.Ae = q[0]; F.Ag = q[1]; F.Al = q[2]; F.Af = q[3]; F.As = q[4];
+  F.agf = q[5]; F.ags = q[6]; F.agg = q[7]; F.Aa = q[8];
+}
+
+// One scan step of a steady wave: the later run's multipliers A are known,
+// so only the offsets move (the same arithmetic as pc_step).
+template <int CTRL, int RM, int BM, unsigned GM, class Q>
+__device__ __forceinline__ void pc_bstep(PC& F, const Q& A) {
+  if constexpr (GM & kGE) {
+    const double b = dpp_d<CTRL, RM, BM>(0.0, F.be);
+    F.be = fma(A[0], b, F.be);
+  }
+  if constexpr (GM & kGR) {
+    const double bg = dpp_d<CTRL, RM, BM>(0.0, F.bg), bl = dpp_d<CTRL, RM, BM>(0.0, F.bl);
+    F.bg = fma(A[1], bg, F.bg);
+    F.bl = fma(A[2], bl, F.bl);
+  }
+  if constexpr (GM & kGM) {
+    const double bf = dpp_d<CTRL, RM, BM>(0.0, F.bf), bs = dpp_d<CTRL, RM, BM>(0.0, F.bs);
+    const double bgg = dpp_d<CTRL, RM, BM>(0.0, F.bgg);
+    F.bgg = fma(A[5], bf, fma(A[6], bs, fma(A[7], bgg, F.bgg)));
+    F.bf = fma(A[3], bf, F.bf);
+    F.bs = fma(A[4], bs, F.bs);
+  }
+  if constexpr (GM & kGA) {
+    const double b = dpp_d<CTRL, RM, BM>(0.0, F.ba);
+    F.ba = fma(A[8], b, F.ba);
+  }
+}
+
+// The lane's multipliers of the two broadcast steps (fields of GM only).
+struct PQLane {
+  double a4[kQF], a5[kQF];
+};
+template <unsigned GM>
+__device__ __forceinline__ void pq_lane(const PipeP& P, PQLane& L) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int f = 0; f < kQF; ++f) {
+    const unsigned g = f == 0 ? kGE : f <= 2 ? kGR : f <= 7 ? kGM : kGA;
+    if (GM & g) {
+      L.a4[f] = P.qt[f * kWave + lane];
+      L.a5[f] = P.qt[(kQF + f) * kWave + lane];
+    } else {
+      L.a4[f] = L.a5[f] = 1.0;
+    }
+  }
+}
+
+// Inclusive scan of a steady wave's offsets.  Multipliers are left as they
+// were.
+template <unsigned GM>
+__device__ __forceinline__ void pc_scan_steady(PC& F, const PipeP& P, const PQLane& L) {
+  pc_bstep<kRowShr1, 0xf, 0xf, GM>(F, P.q[0]);
+  pc_bstep<kRowShr2, 0xf, 0xf, GM>(F, P.q[1]);
+  pc_bstep<kRowShr4, 0xf, 0xf, GM>(F, P.q[2]);
+  pc_bstep<kRowShr8, 0xf, 0xf, GM>(F, P.q[3]);
+  pc_bstep<kRowBcast15, 0xa, 0xf, GM>(F, L.a4);
+  pc_bstep<kRowBcast31, 0xc, 0xf, GM>(F, L.a5);
+}
+
+// State (e, G, L, f, s, g, atr) after a run, from the state before it.
+struct PState {
+  double e, G, L, f, s, g, a;
+};
+__device__ __forceinline__ PState pc_apply(const PC& p, const PState& y) {
+  PState r;
+  r.e = fma(p.Ae, y.e, p.be);
+  r.G = fma(p.Ag, y.G, p.bg);
+  r.L = fma(p.Al, y.L, p.bl);
+  r.f = fma(p.Af, y.f, p.bf);
+  r.s = fma(p.As, y.s, p.bs);
+  r.g = fma(p.agf, y.f, fma(p.ags, y.s, fma(p.agg, y.g, p.bgg)));
+  r.a = fma(p.Aa, y.a, p.ba);
+  return r;
+}
+
+struct PSeeds {
+  int f, Le, Lg, Lm, Ls, La;
+  double sE, sG, sL, sF, sS, sSig, sA;
+};
+
+__device__ __forceinline__ float lane_f(float x, int j) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), j));
+}
+
+// body(t, x_t) for t = t0..t1 in order, uniform over the wave: the lanes load
+// 64 bars at a time, the sequence reads them back lane by lane.  Every t lies
+// before the end of the staged segment.
+template <class T, class Body>
+__device__ __forceinline__ void pipe_walk(const float* lds, const float* src, const Seg& c, int t0,
+                                          int t1, Body&& body) {
+  for (int b = t0; b <= t1; b += kWave) {
+    const int t = b + lane_id();
+    const float x = t <= t1 ? at<T>(lds, src, c, t) : 0.0f;
+    const int m = min(kWave, t1 - b + 1);
+    for (int j = 0; j < m; ++j) body(b + j, lane_f(x, j));
+  }
+}
+
+// Seeds of the recurrences whose seed bar lies in this segment, in the
+// oracle's order (TA-Lib's warm-up sums), evaluated by every lane.
+template <class T>
+__device__ __forceinline__ PSeeds pipe_seeds(const float* lc, const float* lh, const float* ll,
+                                             const PackIO& io, const Seg& c, const PipeP& P,
+                                             int f) {
+  PSeeds S;
+  S.f = f;
+  S.Le = sat_add(f, P.pe - 1);
+  S.Lg = sat_add(f, P.pr);
+  S.Lm = sat_add(f, P.slow - 1);
+  S.Ls = sat_add(S.Lm, P.sig - 1);
+  S.La = P.pa;
+  S.sE = S.sG = S.sL = S.sF = S.sS = S.sSig = S.sA = 0.0;
+  if (P.wE && here(c, S.Le)) {
+    double t0 = 0.0;
+    pipe_walk<T>(lc, io.c, c, f, S.Le, [&](int, float x) { t0 += (double)x; });
+    S.sE = t0 / P.pe;
+  }
+  if (P.wR && here(c, S.Lg)) {
+    double prev = (double)at<T>(lc, io.c, c, f), g = 0.0, l = 0.0;
+    pipe_walk<T>(lc, io.c, c, f + 1, S.Lg, [&](int, float x) {
+      const double v = (double)x;
+      const double d = v - prev;
+      prev = v;
+      const double gd = d > 0.0 ? d : 0.0;
+      g += gd;
+      l += gd - d;
+    });
+    S.sG = g * P.inv;
+    S.sL = l * P.inv;
+  }
+  // the signal's seed (at Ls) needs the fast and slow seeds (at Lm <= Ls);
+  // every bar read lies at or before a seed bar of this segment
+  if (P.wM && (here(c, S.Lm) || here(c, S.Ls))) {
+    double ts = 0.0, tf = 0.0;
+    const int ff = S.Lm - P.fast + 1;
+    pipe_walk<T>(lc, io.c, c, f, S.Lm, [&](int t, float x) {
+      ts += (double)x;
+      if (t >= ff) tf += (double)x;
+    });
+    S.sS = ts / P.slow;
+    S.sF = tf / P.fast;
+    if (here(c, S.Ls)) {
+      double ps = S.sS, pf = S.sF, psig = 0.0;
+      psig += pf - ps;
+      pipe_walk<T>(lc, io.c, c, S.Lm + 1, S.Ls, [&](int, float x) {
+        const double v = (double)x;
+        pf = fma(P.fb, pf, P.fk * v);
+        ps = fma(P.sb, ps, P.sk * v);
+        psig += pf - ps;
+      });
+      S.sSig = psig / P.sig;
+    }
+  }
+  if (P.wA && here(c, S.La)) {
+    double tot = 0.0;
+    for (int b = 1; b <= P.pa; b += kWave) {
+      const int t = b + lane_id();
+      const double tr = t <= P.pa ? true_range(at<T>(lh, io.h, c, t), at<T>(ll, io.l, c, t),
+                                               at<T>(lc, io.c, c, t - 1))
+                                  : 0.0;
+      const int m = min(kWave, P.pa - b + 1);
+      for (int j = 0; j < m; ++j) tot += bcast_d(tr, j);
+    }
+    S.sA = tot / P.pa;
+  }
+  return S;
+}
+
+// The lane's bars (close, high, low) and the close before the first.
+struct PBars {
+  float cv[kPK], hv[kPK], lv[kPK];
+  float pc;
+  __device__ float prev(int i) const { return i == 0 ? pc : cv[i - 1]; }
+  __device__ double d(int i) const { return (double)cv[i] - (double)prev(i); }
+  __device__ double tr(int i) const { return true_range(hv[i], lv[i], prev(i)); }
+};
+
+template <class T>
+__device__ __forceinline__ void pipe_bars(const float* lc, const float* lh, const float* ll,
+                                          const PackIO& io, const Seg& c, const PipeP& P, int a,
+                                          PBars& B) {
+  lane_vals<T>(lc, B.cv);
+  B.pc = at<T>(lc, io.c, c, a - 1);
+  // high and low are staged only for ATR; without it these values are unread
+  lane_vals<T>(lh, B.hv);
+  lane_vals<T>(ll, B.lv);
+}
+
+// Composite of the lane's bars from a zero state.  STEADY: every bar is past
+// every seed (no per-bar seed tests).
+template <bool STEADY, unsigned GM = kGAll>
+__device__ __forceinline__ PC pipe_fold(const PipeP& P, const PSeeds& S, const PBars& B, int a,
+                                        int s1) {
+  PC F;
+  pc_id(F);
+#pragma unroll
+  for (int i = 0; i < kPK; ++i) {
+    const int t = a + i;
+    if (!STEADY && t >= s1) break;
+    const double v = (double)B.cv[i];
+    if ((GM & kGE) && P.wE) {
+      if (!STEADY && t == S.Le) {
+        F.Ae = 0.0;
+        F.be = S.sE;
+      } else if (STEADY || t > S.Le) {
+        if (!STEADY) F.Ae = mulc(F.Ae, P.eb);
+        F.be = fma(P.eb, F.be, P.ek * v);
+      }
+    }
+    if ((GM & kGR) && P.wR) {
+      if (!STEADY && t == S.Lg) {
+        F.Ag = F.Al = 0.0;
+        F.bg = S.sG;
+        F.bl = S.sL;
+      } else if (STEADY || t > S.Lg) {
+        const double di = B.d(i), gd = di > 0.0 ? di : 0.0;
+        if (!STEADY) {
+          F.Ag = mulc(F.Ag, P.br);
+          F.Al = mulc(F.Al, P.br);
+        }
+        F.bg = (F.bg * P.pm1 + gd) * P.inv;
+        F.bl = (F.bl * P.pm1 + (gd - di)) * P.inv;
+      }
+    }
+    if ((GM & kGM) && P.wM) {
+      if (!STEADY && t == S.Lm) {
+        F.Af = F.As = 0.0;
+        F.bf = S.sF;
+        F.bs = S.sS;
+      } else if (STEADY || t > S.Lm) {
+        if (!STEADY) {
+          F.Af = mulc(F.Af, P.fb);
+          F.As = mulc(F.As, P.sb);
+        }
+        F.bf = fma(P.fb, F.bf, P.fk * v);
+        F.bs = fma(P.sb, F.bs, P.sk * v);
+      }
+      if (!STEADY && t == S.Ls) {
+        F.agf = F.ags = F.agg = 0.0;
+        F.bgg = S.sSig;
+      } else if (STEADY || t > S.Ls) {
+        if (!STEADY) {
+          F.agf = fma(P.gb, F.agf, P.gk * F.Af);
+          F.ags = fma(P.gb, F.ags, -P.gk * F.As);
+          F.agg = P.gb * F.agg;
+        }
+        F.bgg = fma(P.gb, F.bgg, P.gk * (F.bf - F.bs));
+      }
+    }
+    if ((GM & kGA) && P.wA) {
+      if (!STEADY && t == S.La) {
+        F.Aa = 0.0;
+        F.ba = S.sA;
+      } else if (STEADY || t > S.La) {
+        if (!STEADY) F.Aa = mulc(F.Aa, P.wb);
+        F.ba = fma(P.wb, F.ba, P.wa * B.tr(i));
+      }
+    }
+  }
+  if (STEADY) pc_set_a(F, P.q[0]);
+  return F;
+}
+
+__device__ __forceinline__ bool pipe_steady(const Seg& c, const PipeP& P, const PSeeds& S) {
+  if (c.s1 - c.s0 != kPSeg) return false;
+  int last = -1;
+  if (P.wE) last = max(last, S.Le);
+  if (P.wR) last = max(last, S.Lg);
+  if (P.wM) last = max(last, S.Ls);
+  if (P.wA) last = max(last, S.La);
+  return c.s0 > last;
+}
+
+__device__ __forceinline__ Seg pipe_seg(const PipeP& P, int seg, int& row) {
+  Seg c;
+  row = seg / P.segsPerRow;
+  c.seg = seg - row * P.segsPerRow;
+  c.nseg = P.segsPerRow;
+  c.gseg = seg;
+  c.rowOff = (long long)row * P.n;
+  c.n = P.n;
+  c.s0 = c.seg * kPSeg;
+  c.s1 = min(P.n, c.s0 + kPSeg);
+  c.totalSegs = P.rows * P.segsPerRow;
+  c.tree = false;
+  c.epoch = 0;
+  return c;
+}
+
+// One wave per row: the row's first non-NaN close.  Row 0's wave also
+// writes the per-lane multipliers of steady waves (the general scan run on
+// q[0] in every lane, recorded before each broadcast step).
+__global__ __launch_bounds__(64) void k_pipe_lead(PipeP P, const float* x, int* lead) {
+  const int row = blockIdx.x;
+  if (row >= P.rows) return;
+  const int n = P.n;
+  const float* r = x + (size_t)row * n;
+  int f = n;
+  for (int t0 = 0; t0 < n; t0 += kWave) {
+    const int t = t0 + lane_id();
+    const unsigned long long m = __ballot(t < n && !__builtin_isnan(r[t]));
+    if (m) {
+      f = t0 + __builtin_ctzll(m);
+      break;
+    }
+  }
+  if (lane_id() == 0) lead[row] = f;
+  if (row != 0) return;
+  PC F;
+  pc_id(F);
+  pc_set_a(F, P.q[0]);
+  pc_step<kRowShr1, 0xf, 0xf, kGAll>(F);
+  pc_step<kRowShr2, 0xf, 0xf, kGAll>(F);
+  pc_step<kRowShr4, 0xf, 0xf, kGAll>(F);
+  pc_step<kRowShr8, 0xf, 0xf, kGAll>(F);
+  const int lane = lane_id();
+  auto put = [&](int k) {
+    const double v[kQF] = {F.Ae, F.Ag, F.Al, F.Af, F.As, F.agf, F.ags, F.agg, F.Aa};
+#pragma unroll
+    for (int q = 0; q < kQF; ++q) P.qt[(k * kQF + q) * kWave + lane] = v[q];
+  };
+  put(0);
+  pc_step<kRowBcast15, 0xa, 0xf, kGAll>(F);
+  put(1);
+}
+
+using PTile = Tile<kPK, kPK>;
+
+struct PLds {
+  float c[PTile::PLEN], h[PTile::PLEN], l[PTile::PLEN];
+};
+
+template <class T>
+__device__ __forceinline__ void pipe_stage(PLds& L, const PackIO& io, const Seg& c,
+                                           const PipeP& P) {
+  stage<T>(L.c, io.c, c);
+  if (P.wA) {
+    stage<T>(L.h, io.h, c);
+    stage<T>(L.l, io.l, c);
+  }
+  wave_sync();
+}
+
+// One group's segment composite (lane 63's inclusive scan) into agg; a group
+// the pass does not compute stores the identity.
+template <unsigned GM>
+__device__ __forceinline__ void pipe_fold_group(const PipeP& P, const PSeeds& S, const PBars& B,
+                                                int a, int s1, bool steady, double* o) {
+  PC F;
+  if (steady) {
+    PQLane L;
+    pq_lane<GM>(P, L);
+    F = pipe_fold<true, GM>(P, S, B, a, s1);
+    pc_scan_steady<GM>(F, P, L);
+    pc_set_a(F, P.q[6]);
+  } else {
+    F = pipe_fold<false, GM>(P, S, B, a, s1);
+    pc_scan<GM>(F);
+  }
+  if (lane_id() != kWave - 1) return;
+  if constexpr (GM == kGE) {
+    o[0] = F.Ae; o[1] = F.be;
+  } else if constexpr (GM == kGR) {
+    o[2] = F.Ag; o[3] = F.bg; o[4] = F.Al; o[5] = F.bl;
+  } else if constexpr (GM == kGM) {
+    o[6] = F.Af; o[7] = F.bf; o[8] = F.As; o[9] = F.bs;
+    o[10] = F.agf; o[11] = F.ags; o[12] = F.agg; o[13] = F.bgg;
+  } else {
+    o[14] = F.Aa; o[15] = F.ba;
+  }
+}
+
+__global__ __launch_bounds__(kPW* kWave) void k_pipe_fold(PipeP P, PackIO io, const int* lead,
+                                                          double* agg) {
+  using T = PTile;
+  __shared__ __align__(16) PLds sm[kPW];
+  const int w = threadIdx.x / kWave;
+  const int seg = blockIdx.x * kPW + w;
+  if (seg >= P.rows * P.segsPerRow) return;
+  int row;
+  const Seg c = pipe_seg(P, seg, row);
+  pipe_stage<T>(sm[w], io, c, P);
+  const int a = c.s0 + lane_id() * kPK;
+  PBars B;
+  pipe_bars<T>(sm[w].c, sm[w].h, sm[w].l, io, c, P, a, B);
+  const PSeeds S = pipe_seeds<T>(sm[w].c, sm[w].h, sm[w].l, io, c, P, lead[row]);
+  const bool steady = pipe_steady(c, P, S);
+  double* o = agg + (size_t)seg * 16;
+  pipe_fold_group<kGE>(P, S, B, a, c.s1, steady, o);
+  pipe_fold_group<kGR>(P, S, B, a, c.s1, steady, o);
+  pipe_fold_group<kGM>(P, S, B, a, c.s1, steady, o);
+  pipe_fold_group<kGA>(P, S, B, a, c.s1, steady, o);
+}
+
+__device__ __forceinline__ PC pc_load(const double* g) {
+  PC p;
+  p.Ae = g[0]; p.be = g[1]; p.Ag = g[2]; p.bg = g[3]; p.Al = g[4]; p.bl = g[5];
+  p.Af = g[6]; p.bf = g[7]; p.As = g[8]; p.bs = g[9]; p.agf = g[10]; p.ags = g[11];
+  p.agg = g[12]; p.bgg = g[13]; p.Aa = g[14]; p.ba = g[15];
+  return p;
+}
+
+__device__ __forceinline__ void pc_store(double* g, const PC& p) {
+  const double v[16] = {p.Ae, p.be, p.Ag, p.bg, p.Al, p.bl, p.Af, p.bf,
+                        p.As, p.bs, p.agf, p.ags, p.agg, p.bgg, p.Aa, p.ba};
+#pragma unroll
+  for (int q = 0; q < 16; ++q) g[q] = v[q];
+}
+
+__device__ __forceinline__ PC pc_lane63(const PC& p) {
+  PC r;
+  r.Ae = bcast_d(p.Ae, 63); r.be = bcast_d(p.be, 63);
+  r.Ag = bcast_d(p.Ag, 63); r.bg = bcast_d(p.bg, 63);
+  r.Al = bcast_d(p.Al, 63); r.bl = bcast_d(p.bl, 63);
+  r.Af = bcast_d(p.Af, 63); r.bf = bcast_d(p.bf, 63);
+  r.As = bcast_d(p.As, 63); r.bs = bcast_d(p.bs, 63);
+  r.agf = bcast_d(p.agf, 63); r.ags = bcast_d(p.ags, 63);
+  r.agg = bcast_d(p.agg, 63); r.bgg = bcast_d(p.bgg, 63);
+  r.Aa = bcast_d(p.Aa, 63); r.ba = bcast_d(p.ba, 63);
+  return r;
+}
+
+__device__ __forceinline__ int pipe_groups(const PipeP& P) { return (P.segsPerRow + kWave - 1) / kWave; }
+
+// One wave per group of 64 consecutive segments of a row: every segment's
+// composite becomes the composite of the group's segments before it (in
+// place), and the group's composite goes to gtot.
+__global__ __launch_bounds__(kPW* kWave) void k_pipe_group(PipeP P, double* agg, double* gtot) {
+  const int G = pipe_groups(P);
+  const int gid = blockIdx.x * kPW + threadIdx.x / kWave;
+  if (gid >= P.rows * G) return;
+  const int row = gid / G, g = gid - row * G;
+  const int j = g * kWave + lane_id();
+  const bool on = j < P.segsPerRow;
+  double* slot = agg + ((size_t)row * P.segsPerRow + j) * 16;
+  PC F;
+  if (on) F = pc_load(slot);
+  else pc_id(F);
+  pc_scan(F);
+  const PC E = pc_prev_lane(F);
+  if (on) pc_store(slot, E);
+  if (lane_id() == kWave - 1) pc_store(gtot + (size_t)gid * 16, F);
+}
+
+// One wave per row: the state entering every group (7 doubles), 64 groups
+// at a time.
+__global__ __launch_bounds__(kPW* kWave) void k_pipe_rows(PipeP P, const double* gtot,
+                                                          double* genter) {
+  const int G = pipe_groups(P);
+  const int row = blockIdx.x * kPW + threadIdx.x / kWave;
+  if (row >= P.rows) return;
+  PState y{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int g0 = 0; g0 < G; g0 += kWave) {
+    const int g = g0 + lane_id();
+    const size_t k = (size_t)row * G + g;
+    PC F;
+    if (g < G) F = pc_load(gtot + k * 16);
+    else pc_id(F);
+    pc_scan(F);
+    if (g < G) {
+      const PState v = pc_apply(pc_prev_lane(F), y);
+      double* o = genter + k * 7;
+      o[0] = v.e; o[1] = v.G; o[2] = v.L; o[3] = v.f; o[4] = v.s; o[5] = v.g; o[6] = v.a;
+    }
+    y = pc_apply(pc_lane63(F), y);
+  }
+}
+
+__device__ __forceinline__ void pipe_put(float* dst, const Seg& c, int a, const float (&v)[kPK]) {
+  if (!dst) return;
+  float* p = dst + c.rowOff + a;
+  if (a + kPK <= c.s1 && ((reinterpret_cast<size_t>(p) & 15) == 0)) {
+    *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+    *reinterpret_cast<float4*>(p + 4) = make_float4(v[4], v[5], v[6], v[7]);
+    return;
+  }
+#pragma unroll
+  for (int i = 0; i < kPK; ++i)
+    if (a + i < c.s1) p[i] = v[i];
+}
+
+// Replay of one group for one lane from the state entering its bars; the
+// group's outputs are stored together (whole 32-byte runs per lane).
+template <bool STEADY, unsigned GM>
+__device__ __forceinline__ void pipe_replay(const PipeP& P, const PSeeds& S, const PBars& B,
+                                            const Seg& c, int a, PState y, const PackIO& io) {
+  const float qn = __builtin_nanf("");
+  float o0[kPK], o1[kPK], o2[kPK];
+#pragma unroll
+  for (int i = 0; i < kPK; ++i) {
+    const int t = a + i;
+    o0[i] = o1[i] = o2[i] = qn;
+    if (!STEADY && t >= c.s1) continue;
+    const double v = (double)B.cv[i];
+    if constexpr (GM == kGE) {
+      if (!STEADY && t == S.Le) y.e = S.sE;
+      else if (STEADY || t > S.Le) y.e = fma(P.eb, y.e, P.ek * v);
+      if (STEADY || t >= S.Le) o0[i] = (float)y.e;
+    } else if constexpr (GM == kGR) {
+      if (!STEADY && t == S.Lg) {
+        y.G = S.sG;
+        y.L = S.sL;
+      } else if (STEADY || t > S.Lg) {
+        const double d = B.d(i), gd = d > 0.0 ? d : 0.0;
+        y.L *= P.pm1;
+        y.G *= P.pm1;
+        y.G += gd;
+        y.L += gd - d;
+        y.L *= P.inv;
+        y.G *= P.inv;
+      }
+      if (STEADY || t >= S.Lg) {
+        const double sm = y.G + y.L;
+        o0[i] = (float)(sm > 0.0 ? 100.0 * (y.G / sm) : 50.0);
+      }
+    } else if constexpr (GM == kGM) {
+      if (!STEADY && t == S.Lm) {
+        y.f = S.sF;
+        y.s = S.sS;
+      } else if (STEADY || t > S.Lm) {
+        y.f = fma(P.fb, y.f, P.fk * v);
+        y.s = fma(P.sb, y.s, P.sk * v);
+      }
+      const double mv = y.f - y.s;
+      if (!STEADY && t == S.Ls) y.g = S.sSig;
+      else if (STEADY || t > S.Ls) y.g = fma(P.gb, y.g, P.gk * mv);
+      if (STEADY || t >= S.Ls) {
+        o0[i] = (float)mv;
+        o1[i] = (float)y.g;
+        o2[i] = (float)(mv - y.g);
+      }
+    } else {
+      if (!STEADY && t == S.La) y.a = S.sA;
+      else if (STEADY || t > S.La) y.a = fma(P.wb, y.a, P.wa * B.tr(i));
+      if (STEADY || t >= S.La) o0[i] = (float)y.a;
+    }
+  }
+  if constexpr (GM == kGE) {
+    pipe_put(io.out[kOEma], c, a, o0);
+  } else if constexpr (GM == kGR) {
+    pipe_put(io.out[kORsi], c, a, o0);
+  } else if constexpr (GM == kGM) {
+    pipe_put(io.out[kOMacd], c, a, o0);
+    pipe_put(io.out[kOMacdSig], c, a, o1);
+    pipe_put(io.out[kOMacdHist], c, a, o2);
+  } else {
+    pipe_put(io.out[kOAtr], c, a, o0);
+  }
+}
+
+// One group of the emit pass: the lane's composite, the wave scan seeded by
+// the state entering the segment (lane 0), then the replay.
+template <unsigned GM>
+__device__ __forceinline__ void pipe_emit_group(const PipeP& P, const PSeeds& S, const PBars& B,
+                                                const Seg& c, int a, bool steady,
+                                                const PState& y0, const PackIO& io) {
+  const int lane = lane_id();
+  PQLane Lq;
+  if (steady) pq_lane<GM>(P, Lq);
+  PC F = steady ? pipe_fold<true, GM>(P, S, B, a, c.s1) : pipe_fold<false, GM>(P, S, B, a, c.s1);
+  // lane 0 starts from y0 (a composite with zero multipliers); the scans
+  // below read only the offsets of the lanes they combine it with
+  if (lane == 0) {
+    PC C;
+    pc_id(C);
+    C.Ae = C.Ag = C.Al = C.Af = C.As = C.Aa = C.agf = C.ags = C.agg = 0.0;
+    C.be = y0.e; C.bg = y0.G; C.bl = y0.L; C.bf = y0.f; C.bs = y0.s; C.bgg = y0.g; C.ba = y0.a;
+    F = pc_after(C, F);
+  }
+  if (steady) pc_scan_steady<GM>(F, P, Lq);
+  else pc_scan<GM>(F);
+  PState y = y0;
+  if constexpr (GM == kGE) {
+    const double e = dpp_d<kWaveShr1, 0xf, 0xf>(0.0, F.be);
+    if (lane > 0) y.e = e;
+  } else if constexpr (GM == kGR) {
+    const double G = dpp_d<kWaveShr1, 0xf, 0xf>(0.0, F.bg);
+    const double L = dpp_d<kWaveShr1, 0xf, 0xf>(0.0, F.bl);
+    if (lane > 0) {
+      y.G = G;
+      y.L = L;
+    }
+  } else if constexpr (GM == kGM) {
+    const double f = dpp_d<kWaveShr1, 0xf, 0xf>(0.0, F.bf);
+    const double sl = dpp_d<kWaveShr1, 0xf, 0xf>(0.0, F.bs);
+    const double g = dpp_d<kWaveShr1, 0xf, 0xf>(0.0, F.bgg);
+    if (lane > 0) {
+      y.f = f;
+      y.s = sl;
+      y.g = g;
+    }
+  } else {
+    const double v = dpp_d<kWaveShr1, 0xf, 0xf>(0.0, F.ba);
+    if (lane > 0) y.a = v;
+  }
+  if (steady) pipe_replay<true, GM>(P, S, B, c, a, y, io);
+  else pipe_replay<false, GM>(P, S, B, c, a, y, io);
+}
+
+union PipeEmitLds {
+  PLds rec[kPW];
+  ColLds<ColTile<64>::E> win[kPW];
+};
+
+// [recurrence segments | window segments]: the window part of the pass
+// runs in the same grid.
+__global__ __launch_bounds__(kPW* kWave) void k_pipe_emit(PipeP P, PackP prm, PackIO io,
+                                                          const int* lead, const double* agg,
+                                                          const double* genter, Geometry gw,
+                                                          int nbR) {
+  __shared__ __align__(16) PipeEmitLds sm;
+  const int w = threadIdx.x / kWave;
+  if ((int)blockIdx.x >= nbR) {
+    Seg c;
+    c.tree = false;
+    c.epoch = 0;
+    if (resolve(gw, ColTile<64>::SEG, ((int)blockIdx.x - nbR) * kPW + w, c))
+      pack_windows<ColTile<64>>(c, sm.win[w], prm, io);
+    return;
+  }
+  using T = PTile;
+  const int seg = blockIdx.x * kPW + w;
+  if (seg >= P.rows * P.segsPerRow) return;
+  int row;
+  const Seg c = pipe_seg(P, seg, row);
+  pipe_stage<T>(sm.rec[w], io, c, P);
+  const int lane = lane_id();
+  const int a = c.s0 + lane * kPK;
+  PBars B;
+  pipe_bars<T>(sm.rec[w].c, sm.rec[w].h, sm.rec[w].l, io, c, P, a, B);
+  const PSeeds S = pipe_seeds<T>(sm.rec[w].c, sm.rec[w].h, sm.rec[w].l, io, c, P, lead[row]);
+  const bool steady = pipe_steady(c, P, S);
+  // state entering the segment: its group's entering state, then the
+  // group's segments before it
+  const double* g = genter + ((size_t)row * pipe_groups(P) + c.seg / kWave) * 7;
+  const PState y0 = pc_apply(pc_load(agg + (size_t)seg * 16),
+                             PState{g[0], g[1], g[2], g[3], g[4], g[5], g[6]});
+  if (P.wE) pipe_emit_group<kGE>(P, S, B, c, a, steady, y0, io);
+  if (P.wR) pipe_emit_group<kGR>(P, S, B, c, a, steady, y0, io);
+  if (P.wM) pipe_emit_group<kGM>(P, S, B, c, a, steady, y0, io);
+  if (P.wA) pipe_emit_group<kGA>(P, S, B, c, a, steady, y0, io);
+}
+
+// Whether the two passes serve this pass: dense rows of at least kPMinLen
+// bars, recurrences among EMA, RSI, MACD, ATR with periods of 2 or more (the
+// one-bar contracts and DEMA, TEMA, ADX keep the look-back kernels), and
+// window look-backs within the window blocks' 64-bar halo.
+bool pipe_serves(int len, const PackP& p) {
+  if (t_ragged || len < kPMinLen) return false;
+  if (p.mask & kPUnsupported) return false;
+  if (!(p.mask & (kCloseRecMask | kHlcRecMask))) return false;
+  if (window_halo(p) > 64) return false;
+  const bool wE = p.mask & (1u << kOEma), wR = p.mask & (1u << kORsi);
+  const bool wM = p.mask & ((1u << kOMacd) | (1u << kOMacdSig) | (1u << kOMacdHist));
+  const bool wA = p.mask & (1u << kOAtr);
+  if ((wE && p.emaP < 2) || (wR && p.rsiP < 2) || (wA && p.atrP < 2)) return false;
+  if (wM && (p.mFast < 2 || p.mSlow < 2 || p.mSig < 2)) return false;
+  return true;
+}
+
+// q[0]: the multipliers of one lane's 8 steady bars, in pipe_fold's order;
+// q[s + 1] = q[s] after q[s], in pc_step's order (IEEE double on both sides,
+// so they equal what the general scan computes).
+static double mulc_h(double a, double m) {
+  const double r = a * m;
+  return (r == 0.0 && a != 0.0 && m != 0.0) ? 2.2250738585072014e-308 : r;
+}
+static void pipe_steady_q(PipeP& P) {
+  double Ae = 1.0, Ag = 1.0, Al = 1.0, Af = 1.0, As = 1.0, agf = 0.0, ags = 0.0, agg = 1.0,
+         Aa = 1.0;
+  for (int i = 0; i < kPK; ++i) {
+    Ae = mulc_h(Ae, P.eb);
+    Ag = mulc_h(Ag, P.br);
+    Al = mulc_h(Al, P.br);
+    Af = mulc_h(Af, P.fb);
+    As = mulc_h(As, P.sb);
+    agf = std::fma(P.gb, agf, P.gk * Af);
+    ags = std::fma(P.gb, ags, -P.gk * As);
+    agg = P.gb * agg;
+    Aa = mulc_h(Aa, P.wb);
+  }
+  double* q = P.q[0];
+  q[0] = Ae; q[1] = Ag; q[2] = Al; q[3] = Af; q[4] = As; q[5] = agf; q[6] = ags; q[7] = agg;
+  q[8] = Aa;
+  for (int s = 0; s < 6; ++s) {
+    const double* a = P.q[s];
+    double* r = P.q[s + 1];
+    r[0] = mulc_h(a[0], a[0]);
+    r[1] = mulc_h(a[1], a[1]);
+    r[2] = mulc_h(a[2], a[2]);
+    r[5] = std::fma(a[5], a[3], a[7] * a[5]);
+    r[6] = std::fma(a[6], a[4], a[7] * a[6]);
+    r[7] = a[7] * a[7];
+    r[3] = mulc_h(a[3], a[3]);
+    r[4] = mulc_h(a[4], a[4]);
+    r[8] = mulc_h(a[8], a[8]);
+  }
+}
+
+ctStatus_t launch_pipe(int len, int rows, hipStream_t s, const PackP& prm, const PackIO& io) {
+  PipeP P{};
+  P.n = len;
+  P.rows = rows;
+  P.segsPerRow = div_up(len, kPSeg);
+  P.wE = (prm.mask & (1u << kOEma)) ? 1 : 0;
+  P.wR = (prm.mask & (1u << kORsi)) ? 1 : 0;
+  P.wM = (prm.mask & ((1u << kOMacd) | (1u << kOMacdSig) | (1u << kOMacdHist))) ? 1 : 0;
+  P.wA = (prm.mask & (1u << kOAtr)) ? 1 : 0;
+  P.pe = prm.emaP;
+  P.pr = prm.rsiP;
+  P.fast = prm.mFast;
+  P.slow = prm.mSlow;
+  P.sig = prm.mSig;
+  P.pa = prm.atrP;
+  auto coef = [](int p, double& b, double& k) {
+    double bb = (double)(p - 1) / (double)(p + 1);
+    const double kk = 1.0 - bb;
+    if (bb < 0.5) bb = 1.0 - kk;
+    b = bb;
+    k = kk;
+  };
+  coef(P.pe, P.eb, P.ek);
+  coef(P.fast, P.fb, P.fk);
+  coef(P.slow, P.sb, P.sk);
+  coef(P.sig, P.gb, P.gk);
+  P.inv = 1.0 / (double)P.pr;
+  P.pm1 = (double)(P.pr - 1);
+  P.br = P.pm1 * P.inv;
+  P.wb = (double)(P.pa - 1) / (double)P.pa;
+  P.wa = 1.0 - P.wb;
+  const long long segs = (long long)P.segsPerRow * rows;
+  if (segs > (1ll << 30)) return fail(CT_STATUS_UNSUPPORTED, "pack: batch too large");
+  pipe_steady_q(P);
+  const long long groups = (long long)div_up(P.segsPerRow, kWave) * rows;
+  const size_t leadB = ((size_t)rows * sizeof(int) + 255) & ~(size_t)255;
+  const size_t bytes =
+      leadB + ((size_t)segs * 16 + (size_t)groups * (16 + 7) + 2 * kQF * kWave) * sizeof(double);
+  Ws ws;
+  ctStatus_t st = workspace_for(s, bytes, &ws, 0);
+  if (st != CT_STATUS_SUCCESS) return st;
+  int* lead = reinterpret_cast<int*>(ws.area);
+  double* agg = reinterpret_cast<double*>(reinterpret_cast<char*>(ws.area) + leadB);
+  double* gtot = agg + (size_t)segs * 16;
+  double* genter = gtot + (size_t)groups * 16;
+  P.qt = genter + (size_t)groups * 7;
+  Geometry gw = dense_geometry(len, rows, ColTile<64>::SEG);
+  const int nbR = div_up(segs, kPW);
+  const int nbW = (prm.mask & kWinMask) ? div_up(gw.totalSegs, kPW) : 0;
+  hipLaunchKernelGGL(k_pipe_lead, dim3(rows), dim3(kWave), 0, s, P, io.c, lead);
+  TCA_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_pipe_fold, dim3(nbR), dim3(kPW * kWave), 0, s, P, io, lead, agg);
+  TCA_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_pipe_group, dim3(div_up(groups, kPW)), dim3(kPW * kWave), 0, s, P, agg,
+                     gtot);
+  TCA_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_pipe_rows, dim3(div_up(rows, kPW)), dim3(kPW * kWave), 0, s, P, gtot,
+                     genter);
+  TCA_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_pipe_emit, dim3(nbR + nbW), dim3(kPW * kWave), 0, s, P, prm, io, lead,
+                     agg, genter, gw, nbR);
+  TCA_HIP(hipGetLastError());
+  return CT_STATUS_SUCCESS;
+}
+
 // The fused pass: the window kernel and the two recurrence kernels read the
 // same inputs and write disjoint outputs -- as one launch (tiles.one) when
 // the window look-backs fit 64 bars and at least two parts are requested,
 // else as up to three launches in stream order.
 ctStatus_t pack_launch(int len, int rows, hipStream_t s, const PackP& p, const PackIO& io,
                        PackTiles t) {
+  if (t.twoPass && pipe_serves(len, p)) return launch_pipe(len, rows, s, p, io);
   const int parts = ((p.mask & kWinMask) ? 1 : 0) + ((p.mask & kCloseRecMask) ? 1 : 0) +
                     ((p.mask & kHlcRecMask) ? 1 : 0);
   if (t.one && parts >= 2 && window_halo(p) <= 64) {
@@ -1040,8 +1955,8 @@ ctStatus_t pack_launch(int len, int rows, hipStream_t s, const PackP& p, const P
 // kernel (fewest waves waiting on one workgroup's slowest segment), four
 // for the high/low/close kernel.  The pass runs as one launch.
 PackTiles default_tiles(int len) {
-  if (len <= 4 * kWave) return PackTiles{{4, 4}, {4, 4}, 4};
-  return PackTiles{{8, 1}, {8, 4}, 8};
+  if (len <= 4 * kWave) return PackTiles{{4, 4}, {4, 4}, 4, 1};
+  return PackTiles{{8, 1}, {8, 4}, 8, 1};
 }
 
 // Dense, single-series or ragged (maximum row length) pack with the default
@@ -1188,7 +2103,7 @@ struct ctxPlan_st {
   hipStream_t stream = nullptr;  // private stream the graph was captured on
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  PackTiles tiles{{4, 4}, {4, 4}, 4};
+  PackTiles tiles{{4, 4}, {4, 4}, 4, 1};
   unsigned mask = 0;
   int runK = 4, runW = 4;  // tile of the launch that runs (the close kernel's if several)
   float tunedUs = 0.0f, defaultUs = 0.0f;
@@ -1247,7 +2162,7 @@ ctStatus_t tune_part(int len, int rows, hipStream_t st, const PackP& P, const Pa
   float bestUs = 0.0f;
   for (const PackCfg& cf : kPackCfgs) {
     float us = 0.0f;
-    ctStatus_t r = time_tiles(len, rows, st, Q, io, PackTiles{cf, cf, 0}, 10, &us);
+    ctStatus_t r = time_tiles(len, rows, st, Q, io, PackTiles{cf, cf, 0, 0}, 10, &us);
     if (r != CT_STATUS_SUCCESS) return r;
     if (bestUs == 0.0f || us < bestUs) {
       bestUs = us;
@@ -1323,8 +2238,9 @@ CTAPI_EXPORT ctStatus_t ctx_pack_plan_create(const float* h, const float* l, con
       if (P.mask & kCloseRecMask) r = tune_part(len, rows, p->stream, P, io, kCloseRecMask, &t.close);
       if (r == CT_STATUS_SUCCESS && (P.mask & kHlcRecMask))
         r = tune_part(len, rows, p->stream, P, io, kHlcRecMask, &t.hlc);
-      // the whole pass decides: three launches with the tuned tiles, or one
-      // launch of 4 or 8 bars per lane
+      // the whole pass decides: the default (two passes where they serve,
+      // else one launch), three launches with the tuned tiles, or one launch
+      // of 4 or 8 bars per lane
       float du = 0.0f;
       if (r == CT_STATUS_SUCCESS) r = time_tiles(len, rows, p->stream, P, io, dflt, 10, &du);
       PackTiles best = dflt;
@@ -1333,6 +2249,7 @@ CTAPI_EXPORT ctStatus_t ctx_pack_plan_create(const float* h, const float* l, con
         if (r != CT_STATUS_SUCCESS) break;
         PackTiles c = t;
         c.one = one;
+        c.twoPass = 0;
         float us = 0.0f;
         r = time_tiles(len, rows, p->stream, P, io, c, 10, &us);
         if (r == CT_STATUS_SUCCESS && us < bu) {
@@ -1357,8 +2274,9 @@ CTAPI_EXPORT ctStatus_t ctx_pack_plan_create(const float* h, const float* l, con
                         ((P.mask & kHlcRecMask) ? 1 : 0);
       const PackCfg& t = (P.mask & kCloseRecMask) ? p->tiles.close : p->tiles.hlc;
       const bool one = p->tiles.one && parts >= 2 && window_halo(P) <= 64;
-      p->runK = one ? p->tiles.one : t.K;
-      p->runW = one ? kFuseW : t.W;
+      const bool two = p->tiles.twoPass && !rg && pipe_serves(len, P);
+      p->runK = two ? kPK : one ? p->tiles.one : t.K;
+      p->runW = two ? kPW : one ? kFuseW : t.W;
     }
     e = hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return bail(hip_fail(e, "plan: begin capture"));

@@ -172,3 +172,43 @@ def test_pack_repeat_launches_agree(dev):
     for k in first:
         a, b = first[k].cpu().numpy(), again[k].cpu().numpy()
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), k
+
+
+# Long dense rows whose recurrences are EMA, RSI, MACD and ATR (periods of 2
+# or more) with window look-backs within 64 bars run as two passes: per
+# segment composites, a per-row carry, then the replay.
+PIPE_OUT = ("sma", "ema", "rsi", "macd", "macd_signal", "macd_hist", "bb_upper", "bb_middle",
+            "bb_lower", "atr", "stoch_k", "stoch_d", "willr", "cci", "wma")
+PIPE_REC = ("ema", "rsi", "macd", "macd_signal", "macd_hist", "atr")
+
+
+@pytest.mark.parametrize("shape", [(1, 8192), (1, 8193), (1, 70001), (3, 20037), (40, 9000)])
+def test_pack_two_pass_rows(dev, shape):
+    rows, n = shape
+    print(f"pack two-pass {rows}x{n}", flush=True)
+    _run(dev, _data(n, rows=rows, seed=37), DEFAULT, outputs=PIPE_OUT)
+
+
+@pytest.mark.parametrize("pname", ["long", "ema2"])
+def test_pack_two_pass_periods(dev, pname):
+    params = {"long": LONG, "ema2": dict(DEFAULT, ema_period=2, rsi_period=2, atr_period=2,
+                                         macd_fast=2, macd_slow=3, macd_signal=2)}[pname]
+    print(f"pack two-pass periods {pname}", flush=True)
+    _run(dev, _data(30000, rows=2, seed=41), params, outputs=PIPE_REC)
+
+
+@pytest.mark.parametrize("subset", [("ema",), ("rsi",), ("macd_signal",), ("atr",),
+                                    ("ema", "atr", "sma"), ("rsi", "macd_hist", "cci")])
+def test_pack_two_pass_subsets(dev, subset):
+    print(f"pack two-pass subset {subset}", flush=True)
+    _run(dev, _data(17000, rows=2, seed=43), DEFAULT, outputs=subset)
+
+
+@pytest.mark.parametrize("case", ["prefix1500", "prefix9000", "hole", "inf", "edge"])
+def test_pack_two_pass_non_finite(dev, case):
+    print(f"pack two-pass non-finite {case}", flush=True)
+    d = _data(20000, seed=47)
+    kw = {"prefix1500": dict(prefix=1500), "prefix9000": dict(prefix=9000),
+          "hole": dict(holes=(10000,)), "inf": dict(infs=(12000,)),
+          "edge": dict(holes=(511, 512, 4095))}[case]
+    _run(dev, _poison(d, **kw), DEFAULT, outputs=PIPE_OUT)
