@@ -1262,7 +1262,7 @@ __device__ __forceinline__ void pipe_walk(const float* lds, const float* src, co
 
 // Seeds of the recurrences whose seed bar lies in this segment, in the
 // oracle's order (TA-Lib's warm-up sums), evaluated by every lane.
-template <class T>
+template <class T, unsigned GM = kGAll>
 __device__ __forceinline__ PSeeds pipe_seeds(const float* lc, const float* lh, const float* ll,
                                              const PackIO& io, const Seg& c, const PipeP& P,
                                              int f) {
@@ -1274,12 +1274,12 @@ __device__ __forceinline__ PSeeds pipe_seeds(const float* lc, const float* lh, c
   S.Ls = sat_add(S.Lm, P.sig - 1);
   S.La = P.pa;
   S.sE = S.sG = S.sL = S.sF = S.sS = S.sSig = S.sA = 0.0;
-  if (P.wE && here(c, S.Le)) {
+  if ((GM & kGE) && P.wE && here(c, S.Le)) {
     double t0 = 0.0;
     pipe_walk<T>(lc, io.c, c, f, S.Le, [&](int, float x) { t0 += (double)x; });
     S.sE = t0 / P.pe;
   }
-  if (P.wR && here(c, S.Lg)) {
+  if ((GM & kGR) && P.wR && here(c, S.Lg)) {
     double prev = (double)at<T>(lc, io.c, c, f), g = 0.0, l = 0.0;
     pipe_walk<T>(lc, io.c, c, f + 1, S.Lg, [&](int, float x) {
       const double v = (double)x;
@@ -1294,7 +1294,7 @@ __device__ __forceinline__ PSeeds pipe_seeds(const float* lc, const float* lh, c
   }
   // the signal's seed (at Ls) needs the fast and slow seeds (at Lm <= Ls);
   // every bar read lies at or before a seed bar of this segment
-  if (P.wM && (here(c, S.Lm) || here(c, S.Ls))) {
+  if ((GM & kGM) && P.wM && (here(c, S.Lm) || here(c, S.Ls))) {
     double ts = 0.0, tf = 0.0;
     const int ff = S.Lm - P.fast + 1;
     pipe_walk<T>(lc, io.c, c, f, S.Lm, [&](int t, float x) {
@@ -1315,7 +1315,7 @@ __device__ __forceinline__ PSeeds pipe_seeds(const float* lc, const float* lh, c
       S.sSig = psig / P.sig;
     }
   }
-  if (P.wA && here(c, S.La)) {
+  if ((GM & kGA) && P.wA && here(c, S.La)) {
     double tot = 0.0;
     for (int b = 1; b <= P.pa; b += kWave) {
       const int t = b + lane_id();
@@ -1425,13 +1425,15 @@ __device__ __forceinline__ PC pipe_fold(const PipeP& P, const PSeeds& S, const P
   return F;
 }
 
+// Whether every bar of a full segment lies past the seeds of the groups GM.
+template <unsigned GM = kGAll>
 __device__ __forceinline__ bool pipe_steady(const Seg& c, const PipeP& P, const PSeeds& S) {
   if (c.s1 - c.s0 != kPSeg) return false;
   int last = -1;
-  if (P.wE) last = max(last, S.Le);
-  if (P.wR) last = max(last, S.Lg);
-  if (P.wM) last = max(last, S.Ls);
-  if (P.wA) last = max(last, S.La);
+  if ((GM & kGE) && P.wE) last = max(last, S.Le);
+  if ((GM & kGR) && P.wR) last = max(last, S.Lg);
+  if ((GM & kGM) && P.wM) last = max(last, S.Ls);
+  if ((GM & kGA) && P.wA) last = max(last, S.La);
   return c.s0 > last;
 }
 
@@ -1505,6 +1507,21 @@ __device__ __forceinline__ void pipe_stage(PLds& L, const PackIO& io, const Seg&
   wave_sync();
 }
 
+// The fields of group GM of a composite.
+template <unsigned GM>
+__device__ __forceinline__ void pc_store_group(double* o, const PC& F) {
+  if constexpr (GM == kGE) {
+    o[0] = F.Ae; o[1] = F.be;
+  } else if constexpr (GM == kGR) {
+    o[2] = F.Ag; o[3] = F.bg; o[4] = F.Al; o[5] = F.bl;
+  } else if constexpr (GM == kGM) {
+    o[6] = F.Af; o[7] = F.bf; o[8] = F.As; o[9] = F.bs;
+    o[10] = F.agf; o[11] = F.ags; o[12] = F.agg; o[13] = F.bgg;
+  } else {
+    o[14] = F.Aa; o[15] = F.ba;
+  }
+}
+
 // One group's segment composite (lane 63's inclusive scan) into agg; a group
 // the pass does not compute stores the identity.
 template <unsigned GM>
@@ -1521,39 +1538,59 @@ __device__ __forceinline__ void pipe_fold_group(const PipeP& P, const PSeeds& S,
     F = pipe_fold<false, GM>(P, S, B, a, s1);
     pc_scan<GM>(F);
   }
-  if (lane_id() != kWave - 1) return;
-  if constexpr (GM == kGE) {
-    o[0] = F.Ae; o[1] = F.be;
-  } else if constexpr (GM == kGR) {
-    o[2] = F.Ag; o[3] = F.bg; o[4] = F.Al; o[5] = F.bl;
-  } else if constexpr (GM == kGM) {
-    o[6] = F.Af; o[7] = F.bf; o[8] = F.As; o[9] = F.bs;
-    o[10] = F.agf; o[11] = F.ags; o[12] = F.agg; o[13] = F.bgg;
-  } else {
-    o[14] = F.Aa; o[15] = F.ba;
-  }
+  if (lane_id() == kWave - 1) pc_store_group<GM>(o, F);
 }
 
+// The block's segment staged by its waves: close, then high and low for ATR.
+template <class T>
+__device__ __forceinline__ void pipe_stage_block(PLds& L, const PackIO& io, const Seg& c,
+                                                 const PipeP& P, int w) {
+  if (w == 0) stage<T>(L.c, io.c, c);
+  else if (w == 1 && P.wA) stage<T>(L.h, io.h, c);
+  else if (w == 2 && P.wA) stage<T>(L.l, io.l, c);
+  __syncthreads();
+}
+
+template <unsigned GM>
+__device__ __forceinline__ bool pipe_wants(const PipeP& P) {
+  return GM == kGE ? P.wE : GM == kGR ? P.wR : GM == kGM ? P.wM : P.wA;
+}
+
+// One wave of the fold: group GM of the block's segment.
+template <unsigned GM>
+__device__ __forceinline__ void pipe_fold_wave(const PipeP& P, const PLds& L, const PackIO& io,
+                                               const Seg& c, int f, double* o) {
+  if (!pipe_wants<GM>(P)) {
+    if (lane_id() == kWave - 1) {
+      PC F;
+      pc_id(F);
+      pc_store_group<GM>(o, F);
+    }
+    return;
+  }
+  using T = PTile;
+  const int a = c.s0 + lane_id() * kPK;
+  PBars B;
+  pipe_bars<T>(L.c, L.h, L.l, io, c, P, a, B);
+  const PSeeds S = pipe_seeds<T, GM>(L.c, L.h, L.l, io, c, P, f);
+  pipe_fold_group<GM>(P, S, B, a, c.s1, pipe_steady<GM>(c, P, S), o);
+}
+
+// One workgroup per segment, one wave per recurrence group.
 __global__ __launch_bounds__(kPW* kWave) void k_pipe_fold(PipeP P, PackIO io, const int* lead,
                                                           double* agg) {
-  using T = PTile;
-  __shared__ __align__(16) PLds sm[kPW];
-  const int w = threadIdx.x / kWave;
-  const int seg = blockIdx.x * kPW + w;
+  __shared__ __align__(16) PLds L;
+  const int seg = blockIdx.x, w = threadIdx.x / kWave;
   if (seg >= P.rows * P.segsPerRow) return;
   int row;
   const Seg c = pipe_seg(P, seg, row);
-  pipe_stage<T>(sm[w], io, c, P);
-  const int a = c.s0 + lane_id() * kPK;
-  PBars B;
-  pipe_bars<T>(sm[w].c, sm[w].h, sm[w].l, io, c, P, a, B);
-  const PSeeds S = pipe_seeds<T>(sm[w].c, sm[w].h, sm[w].l, io, c, P, lead[row]);
-  const bool steady = pipe_steady(c, P, S);
+  pipe_stage_block<PTile>(L, io, c, P, w);
+  const int f = lead[row];
   double* o = agg + (size_t)seg * 16;
-  pipe_fold_group<kGE>(P, S, B, a, c.s1, steady, o);
-  pipe_fold_group<kGR>(P, S, B, a, c.s1, steady, o);
-  pipe_fold_group<kGM>(P, S, B, a, c.s1, steady, o);
-  pipe_fold_group<kGA>(P, S, B, a, c.s1, steady, o);
+  if (w == 0) pipe_fold_wave<kGE>(P, L, io, c, f, o);
+  else if (w == 1) pipe_fold_wave<kGR>(P, L, io, c, f, o);
+  else if (w == 2) pipe_fold_wave<kGM>(P, L, io, c, f, o);
+  else pipe_fold_wave<kGA>(P, L, io, c, f, o);
 }
 
 __device__ __forceinline__ PC pc_load(const double* g) {
@@ -1589,7 +1626,22 @@ __device__ __forceinline__ int pipe_groups(const PipeP& P) { return (P.segsPerRo
 // One wave per group of 64 consecutive segments of a row: every segment's
 // composite becomes the composite of the group's segments before it (in
 // place), and the group's composite goes to gtot.
-__global__ __launch_bounds__(kPW* kWave) void k_pipe_group(PipeP P, double* agg, double* gtot) {
+//
+// [group waves | window segments]: the window part of the pass runs here,
+// beside the short group scans.
+__global__ __launch_bounds__(kPW* kWave) void k_pipe_group(PipeP P, double* agg, double* gtot,
+                                                           PackP prm, PackIO io, Geometry gw,
+                                                           int nbG) {
+  __shared__ __align__(16) ColLds<ColTile<64>::E> win[kPW];
+  if ((int)blockIdx.x >= nbG) {
+    Seg c;
+    c.tree = false;
+    c.epoch = 0;
+    const int w = threadIdx.x / kWave;
+    if (resolve(gw, ColTile<64>::SEG, ((int)blockIdx.x - nbG) * kPW + w, c))
+      pack_windows<ColTile<64>>(c, win[w], prm, io);
+    return;
+  }
   const int G = pipe_groups(P);
   const int gid = blockIdx.x * kPW + threadIdx.x / kWave;
   if (gid >= P.rows * G) return;
@@ -1761,48 +1813,40 @@ __device__ __forceinline__ void pipe_emit_group(const PipeP& P, const PSeeds& S,
   else pipe_replay<false, GM>(P, S, B, c, a, y, io);
 }
 
-union PipeEmitLds {
-  PLds rec[kPW];
-  ColLds<ColTile<64>::E> win[kPW];
-};
-
-// [recurrence segments | window segments]: the window part of the pass
-// runs in the same grid.
-__global__ __launch_bounds__(kPW* kWave) void k_pipe_emit(PipeP P, PackP prm, PackIO io,
-                                                          const int* lead, const double* agg,
-                                                          const double* genter, Geometry gw,
-                                                          int nbR) {
-  __shared__ __align__(16) PipeEmitLds sm;
-  const int w = threadIdx.x / kWave;
-  if ((int)blockIdx.x >= nbR) {
-    Seg c;
-    c.tree = false;
-    c.epoch = 0;
-    if (resolve(gw, ColTile<64>::SEG, ((int)blockIdx.x - nbR) * kPW + w, c))
-      pack_windows<ColTile<64>>(c, sm.win[w], prm, io);
-    return;
-  }
+// One wave of the emit pass: group GM of the block's segment.
+template <unsigned GM>
+__device__ __forceinline__ void pipe_emit_wave(const PipeP& P, const PLds& L, const PackIO& io,
+                                               const Seg& c, int row, int seg, const int* lead,
+                                               const double* agg, const double* genter) {
+  if (!pipe_wants<GM>(P)) return;
   using T = PTile;
-  const int seg = blockIdx.x * kPW + w;
-  if (seg >= P.rows * P.segsPerRow) return;
-  int row;
-  const Seg c = pipe_seg(P, seg, row);
-  pipe_stage<T>(sm.rec[w], io, c, P);
-  const int lane = lane_id();
-  const int a = c.s0 + lane * kPK;
+  const int a = c.s0 + lane_id() * kPK;
   PBars B;
-  pipe_bars<T>(sm.rec[w].c, sm.rec[w].h, sm.rec[w].l, io, c, P, a, B);
-  const PSeeds S = pipe_seeds<T>(sm.rec[w].c, sm.rec[w].h, sm.rec[w].l, io, c, P, lead[row]);
-  const bool steady = pipe_steady(c, P, S);
+  pipe_bars<T>(L.c, L.h, L.l, io, c, P, a, B);
+  const PSeeds S = pipe_seeds<T, GM>(L.c, L.h, L.l, io, c, P, lead[row]);
   // state entering the segment: its group's entering state, then the
   // group's segments before it
   const double* g = genter + ((size_t)row * pipe_groups(P) + c.seg / kWave) * 7;
   const PState y0 = pc_apply(pc_load(agg + (size_t)seg * 16),
                              PState{g[0], g[1], g[2], g[3], g[4], g[5], g[6]});
-  if (P.wE) pipe_emit_group<kGE>(P, S, B, c, a, steady, y0, io);
-  if (P.wR) pipe_emit_group<kGR>(P, S, B, c, a, steady, y0, io);
-  if (P.wM) pipe_emit_group<kGM>(P, S, B, c, a, steady, y0, io);
-  if (P.wA) pipe_emit_group<kGA>(P, S, B, c, a, steady, y0, io);
+  pipe_emit_group<GM>(P, S, B, c, a, pipe_steady<GM>(c, P, S), y0, io);
+}
+
+// One workgroup per segment, one wave per recurrence group: the state
+// entering the segment, then the replay of its bars.
+__global__ __launch_bounds__(kPW* kWave) void k_pipe_emit(PipeP P, PackIO io, const int* lead,
+                                                          const double* agg,
+                                                          const double* genter) {
+  __shared__ __align__(16) PLds L;
+  const int seg = blockIdx.x, w = threadIdx.x / kWave;
+  if (seg >= P.rows * P.segsPerRow) return;
+  int row;
+  const Seg c = pipe_seg(P, seg, row);
+  pipe_stage_block<PTile>(L, io, c, P, w);
+  if (w == 0) pipe_emit_wave<kGE>(P, L, io, c, row, seg, lead, agg, genter);
+  else if (w == 1) pipe_emit_wave<kGR>(P, L, io, c, row, seg, lead, agg, genter);
+  else if (w == 2) pipe_emit_wave<kGM>(P, L, io, c, row, seg, lead, agg, genter);
+  else pipe_emit_wave<kGA>(P, L, io, c, row, seg, lead, agg, genter);
 }
 
 // Whether the two passes serve this pass: dense rows of at least kPMinLen
@@ -1908,20 +1952,19 @@ ctStatus_t launch_pipe(int len, int rows, hipStream_t s, const PackP& prm, const
   double* genter = gtot + (size_t)groups * 16;
   P.qt = genter + (size_t)groups * 7;
   Geometry gw = dense_geometry(len, rows, ColTile<64>::SEG);
-  const int nbR = div_up(segs, kPW);
   const int nbW = (prm.mask & kWinMask) ? div_up(gw.totalSegs, kPW) : 0;
   hipLaunchKernelGGL(k_pipe_lead, dim3(rows), dim3(kWave), 0, s, P, io.c, lead);
   TCA_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_pipe_fold, dim3(nbR), dim3(kPW * kWave), 0, s, P, io, lead, agg);
+  hipLaunchKernelGGL(k_pipe_fold, dim3(segs), dim3(kPW * kWave), 0, s, P, io, lead, agg);
   TCA_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_pipe_group, dim3(div_up(groups, kPW)), dim3(kPW * kWave), 0, s, P, agg,
-                     gtot);
+  const int nbG = div_up(groups, kPW);
+  hipLaunchKernelGGL(k_pipe_group, dim3(nbG + nbW), dim3(kPW * kWave), 0, s, P, agg, gtot, prm, io,
+                     gw, nbG);
   TCA_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_pipe_rows, dim3(div_up(rows, kPW)), dim3(kPW * kWave), 0, s, P, gtot,
                      genter);
   TCA_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_pipe_emit, dim3(nbR + nbW), dim3(kPW * kWave), 0, s, P, prm, io, lead,
-                     agg, genter, gw, nbR);
+  hipLaunchKernelGGL(k_pipe_emit, dim3(segs), dim3(kPW * kWave), 0, s, P, io, lead, agg, genter);
   TCA_HIP(hipGetLastError());
   return CT_STATUS_SUCCESS;
 }
