@@ -617,26 +617,25 @@ __device__ __forceinline__ void pack_hlc(const Seg& c, float* li, float* lo, con
 }
 
 // ================================================================ windows ==
-// SMA, WMA, BBANDS, STOCH, WILLR and CCI over a column tile (one bar per lane
-// per 64-bar chunk), each window evaluated directly in the oracle's order.
+// Window families over a column tile (one bar per lane per 64-bar chunk),
+// each window evaluated directly in the oracle's order.  The sections read
+// the staged h, l, c; the extrema section uses d0 and w, CCI its own d.
+
+// SMA and the BBANDS middle band of one period are the same window mean:
+// the BBANDS loop then writes both.
+__device__ __forceinline__ bool win_sma_in_bb(const PackP& P) {
+  const unsigned m = P.mask;
+  const bool bbAll = m & ((1u << kOBbU) | (1u << kOBbM) | (1u << kOBbL));
+  return (m & (1u << kOSma)) && bbAll && P.smaP == P.bbP;
+}
+
 template <class C>
-__device__ __forceinline__ void pack_windows(const Seg& c, ColLds<C::E>& s, const PackP& P,
-                                             const PackIO& io) {
+__device__ __forceinline__ void win_means(const Seg& c, ColLds<C::E>& s, const PackP& P,
+                                          const PackIO& io) {
   const unsigned m = P.mask;
   const int lane = lane_id();
   const int base = c.s0 - C::H;  // bar of e = 0
-  const bool needHL = m & ((1u << kOStK) | (1u << kOStD) | (1u << kOWillr) | (1u << kOCci));
-  if (needHL) {
-    col_stage<C>(s.h, io.h, c);
-    col_stage<C>(s.l, io.l, c);
-  }
-  col_stage<C>(s.c, io.c, c);
-  col_sync();
-
-  // SMA and the BBANDS middle band of one period are the same window mean:
-  // the BBANDS loop then writes both
-  const bool bbAll = m & ((1u << kOBbU) | (1u << kOBbM) | (1u << kOBbL));
-  const bool smaInBb = (m & (1u << kOSma)) && bbAll && P.smaP == P.bbP;
+  const bool smaInBb = win_sma_in_bb(P);
   if ((m & (1u << kOSma)) && !smaInBb) {
     const int p = P.smaP;
     for (int j = C::NH; j < C::NC; ++j) {
@@ -664,6 +663,16 @@ __device__ __forceinline__ void pack_windows(const Seg& c, ColLds<C::E>& s, cons
       col_store<C>(io.out[kOWma], c, j, (base + e >= p - 1) ? v : qnanf_());
     }
   }
+}
+
+template <class C>
+__device__ __forceinline__ void win_bbands(const Seg& c, ColLds<C::E>& s, const PackP& P,
+                                           const PackIO& io) {
+  const unsigned m = P.mask;
+  const int lane = lane_id();
+  const int base = c.s0 - C::H;
+  const bool bbAll = m & ((1u << kOBbU) | (1u << kOBbM) | (1u << kOBbL));
+  const bool smaInBb = win_sma_in_bb(P);
   if (bbAll) {
     const int p = P.bbP;
     const double inv = 1.0 / (double)p;
@@ -697,6 +706,14 @@ __device__ __forceinline__ void pack_windows(const Seg& c, ColLds<C::E>& s, cons
       col_store<C>(io.out[kOBbL], c, j, ok ? lo : qnanf_());
     }
   }
+}
+
+template <class C>
+__device__ __forceinline__ void win_extrema(const Seg& c, ColLds<C::E>& s, const PackP& P,
+                                            const PackIO& io) {
+  const unsigned m = P.mask;
+  const int lane = lane_id();
+  const int base = c.s0 - C::H;
   const bool wantSt = m & ((1u << kOStK) | (1u << kOStD));
   const bool wantW = m & (1u << kOWillr);
   if (wantSt || wantW) {
@@ -761,30 +778,84 @@ __device__ __forceinline__ void pack_windows(const Seg& c, ColLds<C::E>& s, cons
       willr();
     }
   }
+}
+
+template <class C>
+__device__ __forceinline__ void win_cci(const Seg& c, ColLds<C::E>& s, double* d, const PackP& P,
+                                        const PackIO& io) {
+  const unsigned m = P.mask;
+  const int lane = lane_id();
+  const int base = c.s0 - C::H;
   if (m & (1u << kOCci)) {
     const int p = P.cciP;
-    col_sync();
     const int e0 = C::H - (p - 1);
     for (int j = e0 / kWave; j < C::NC; ++j) {
       const int e = j * kWave + lane;
       const double tp =
           ((double)lds_f(s.h, e) + (double)lds_f(s.l, e) + (double)lds_f(s.c, e)) / 3.0;
-      lds_wd(s.d0, e, tp);
+      lds_wd(d, e, tp);
     }
     col_sync();
     for (int j = C::NH; j < C::NC; ++j) {
       const int e = j * kWave + lane;
       double sum = 0.0;
-      for (int q = e; q > e - p; --q) sum += lds_d(s.d0, q);
+      for (int q = e; q > e - p; --q) sum += lds_d(d, q);
       const double sma = sum / p;
       double dev = 0.0;
-      for (int q = e; q > e - p; --q) dev += fabs(lds_d(s.d0, q) - sma);
+      for (int q = e; q > e - p; --q) dev += fabs(lds_d(d, q) - sma);
       const double md = dev / p;
-      const double num = lds_d(s.d0, e) - sma;
+      const double num = lds_d(d, e) - sma;
       const float v = (num != 0.0 && md != 0.0) ? (float)(num / (0.015 * md)) : 0.0f;
       col_store<C>(io.out[kOCci], c, j, (base + e >= p - 1) ? v : qnanf_());
     }
   }
+}
+
+template <class C>
+__device__ __forceinline__ bool win_need_hl(const PackP& P) {
+  return P.mask & ((1u << kOStK) | (1u << kOStD) | (1u << kOWillr) | (1u << kOCci));
+}
+
+// Every window family of the pass, one wave.
+template <class C>
+__device__ __forceinline__ void pack_windows(const Seg& c, ColLds<C::E>& s, const PackP& P,
+                                             const PackIO& io) {
+  if (win_need_hl<C>(P)) {
+    col_stage<C>(s.h, io.h, c);
+    col_stage<C>(s.l, io.l, c);
+  }
+  col_stage<C>(s.c, io.c, c);
+  col_sync();
+  win_means<C>(c, s, P, io);
+  win_bbands<C>(c, s, P, io);
+  win_extrema<C>(c, s, P, io);
+  col_sync();
+  win_cci<C>(c, s, s.d0, P, io);
+}
+
+// Per-block LDS of the split window pass: one staged tile, the extrema
+// section's d0 and w, CCI's own array.
+template <int E>
+struct ColSplitLds {
+  ColLds<E> t;
+  double d[E];
+};
+
+// The window families of one segment over a workgroup of four waves, one
+// family group per wave: BBANDS (and SMA of its period); STOCH and WILLR;
+// CCI; SMA and WMA.
+template <class C>
+__device__ __forceinline__ void pack_windows_split(const Seg& c, ColSplitLds<C::E>& L,
+                                                   const PackP& P, const PackIO& io, int w) {
+  const bool hl = win_need_hl<C>(P);
+  if (w == 0) col_stage<C>(L.t.c, io.c, c);
+  else if (w == 1 && hl) col_stage<C>(L.t.h, io.h, c);
+  else if (w == 2 && hl) col_stage<C>(L.t.l, io.l, c);
+  __syncthreads();
+  if (w == 0) win_bbands<C>(c, L.t, P, io);
+  else if (w == 1) win_extrema<C>(c, L.t, P, io);
+  else if (w == 2) win_cci<C>(c, L.t, L.d, P, io);
+  else win_means<C>(c, L.t, P, io);
 }
 
 template <int H, int W>
@@ -1453,6 +1524,11 @@ __device__ __forceinline__ Seg pipe_seg(const PipeP& P, int seg, int& row) {
   return c;
 }
 
+// Groups of 64 consecutive segments in a row (the carry's first level).
+__device__ __forceinline__ int pipe_groups(const PipeP& P) {
+  return (P.segsPerRow + kWave - 1) / kWave;
+}
+
 // One wave per row: the row's first non-NaN close.  Row 0's wave also
 // writes the per-lane multipliers of steady waves (the general scan run on
 // q[0] in every lane, recorded before each broadcast step).
@@ -1576,23 +1652,6 @@ __device__ __forceinline__ void pipe_fold_wave(const PipeP& P, const PLds& L, co
   pipe_fold_group<GM>(P, S, B, a, c.s1, pipe_steady<GM>(c, P, S), o);
 }
 
-// One workgroup per segment, one wave per recurrence group.
-__global__ __launch_bounds__(kPW* kWave) void k_pipe_fold(PipeP P, PackIO io, const int* lead,
-                                                          double* agg) {
-  __shared__ __align__(16) PLds L;
-  const int seg = blockIdx.x, w = threadIdx.x / kWave;
-  if (seg >= P.rows * P.segsPerRow) return;
-  int row;
-  const Seg c = pipe_seg(P, seg, row);
-  pipe_stage_block<PTile>(L, io, c, P, w);
-  const int f = lead[row];
-  double* o = agg + (size_t)seg * 16;
-  if (w == 0) pipe_fold_wave<kGE>(P, L, io, c, f, o);
-  else if (w == 1) pipe_fold_wave<kGR>(P, L, io, c, f, o);
-  else if (w == 2) pipe_fold_wave<kGM>(P, L, io, c, f, o);
-  else pipe_fold_wave<kGA>(P, L, io, c, f, o);
-}
-
 __device__ __forceinline__ PC pc_load(const double* g) {
   PC p;
   p.Ae = g[0]; p.be = g[1]; p.Ag = g[2]; p.bg = g[3]; p.Al = g[4]; p.bl = g[5];
@@ -1621,27 +1680,11 @@ __device__ __forceinline__ PC pc_lane63(const PC& p) {
   return r;
 }
 
-__device__ __forceinline__ int pipe_groups(const PipeP& P) { return (P.segsPerRow + kWave - 1) / kWave; }
 
 // One wave per group of 64 consecutive segments of a row: every segment's
 // composite becomes the composite of the group's segments before it (in
 // place), and the group's composite goes to gtot.
-//
-// [group waves | window segments]: the window part of the pass runs here,
-// beside the short group scans.
-__global__ __launch_bounds__(kPW* kWave) void k_pipe_group(PipeP P, double* agg, double* gtot,
-                                                           PackP prm, PackIO io, Geometry gw,
-                                                           int nbG) {
-  __shared__ __align__(16) ColLds<ColTile<64>::E> win[kPW];
-  if ((int)blockIdx.x >= nbG) {
-    Seg c;
-    c.tree = false;
-    c.epoch = 0;
-    const int w = threadIdx.x / kWave;
-    if (resolve(gw, ColTile<64>::SEG, ((int)blockIdx.x - nbG) * kPW + w, c))
-      pack_windows<ColTile<64>>(c, win[w], prm, io);
-    return;
-  }
+__global__ __launch_bounds__(kPW* kWave) void k_pipe_group(PipeP P, double* agg, double* gtot) {
   const int G = pipe_groups(P);
   const int gid = blockIdx.x * kPW + threadIdx.x / kWave;
   if (gid >= P.rows * G) return;
@@ -1813,6 +1856,41 @@ __device__ __forceinline__ void pipe_emit_group(const PipeP& P, const PSeeds& S,
   else pipe_replay<false, GM>(P, S, B, c, a, y, io);
 }
 
+union PipeFoldLds {
+  PLds rec;
+  ColSplitLds<ColTile<64>::E> win;
+};
+
+// [window segments | recurrence segments], one segment per workgroup: the
+// window part of the pass (one family group per wave) first, then the fold
+// (one wave per recurrence group).
+__global__ __launch_bounds__(kPW* kWave) void k_pipe_fold(PipeP P, PackP prm, PackIO io,
+                                                          const int* lead, double* agg,
+                                                          Geometry gw, int nbW) {
+  __shared__ __align__(16) PipeFoldLds sm;
+  const int w = threadIdx.x / kWave;
+  if ((int)blockIdx.x < nbW) {
+    Seg c;
+    c.tree = false;
+    c.epoch = 0;
+    if (resolve(gw, ColTile<64>::SEG, (int)blockIdx.x, c))
+      pack_windows_split<ColTile<64>>(c, sm.win, prm, io, w);
+    return;
+  }
+  const int seg = (int)blockIdx.x - nbW;
+  if (seg >= P.rows * P.segsPerRow) return;
+  int row;
+  const Seg c = pipe_seg(P, seg, row);
+  PLds& L = sm.rec;
+  pipe_stage_block<PTile>(L, io, c, P, w);
+  const int f = lead[row];
+  double* o = agg + (size_t)seg * 16;
+  if (w == 0) pipe_fold_wave<kGE>(P, L, io, c, f, o);
+  else if (w == 1) pipe_fold_wave<kGR>(P, L, io, c, f, o);
+  else if (w == 2) pipe_fold_wave<kGM>(P, L, io, c, f, o);
+  else pipe_fold_wave<kGA>(P, L, io, c, f, o);
+}
+
 // One wave of the emit pass: group GM of the block's segment.
 template <unsigned GM>
 __device__ __forceinline__ void pipe_emit_wave(const PipeP& P, const PLds& L, const PackIO& io,
@@ -1952,14 +2030,14 @@ ctStatus_t launch_pipe(int len, int rows, hipStream_t s, const PackP& prm, const
   double* genter = gtot + (size_t)groups * 16;
   P.qt = genter + (size_t)groups * 7;
   Geometry gw = dense_geometry(len, rows, ColTile<64>::SEG);
-  const int nbW = (prm.mask & kWinMask) ? div_up(gw.totalSegs, kPW) : 0;
+  const int nbW = (prm.mask & kWinMask) ? gw.totalSegs : 0;
   hipLaunchKernelGGL(k_pipe_lead, dim3(rows), dim3(kWave), 0, s, P, io.c, lead);
   TCA_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_pipe_fold, dim3(segs), dim3(kPW * kWave), 0, s, P, io, lead, agg);
+  hipLaunchKernelGGL(k_pipe_fold, dim3(nbW + segs), dim3(kPW * kWave), 0, s, P, prm, io, lead,
+                     agg, gw, nbW);
   TCA_HIP(hipGetLastError());
-  const int nbG = div_up(groups, kPW);
-  hipLaunchKernelGGL(k_pipe_group, dim3(nbG + nbW), dim3(kPW * kWave), 0, s, P, agg, gtot, prm, io,
-                     gw, nbG);
+  hipLaunchKernelGGL(k_pipe_group, dim3(div_up(groups, kPW)), dim3(kPW * kWave), 0, s, P, agg,
+                     gtot);
   TCA_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_pipe_rows, dim3(div_up(rows, kPW)), dim3(kPW * kWave), 0, s, P, gtot,
                      genter);
