@@ -23,9 +23,9 @@ Also measured in the same run (reported under "workloads"):
   autograd 1024 x 8192, forward+backward vs forward-only time.
 
 Prints one JSON line: bars/s of the headline pass, the roofline of its
-kernel (algorithmic bytes per launch / event-timed launch duration
-against MI355X's 8 TB/s), and the reference's TA-Lib 0.8.3 C timed on the
-host.
+dominant kernel (algorithmic bytes per launch / that launch's duration,
+timed with HIP events around the kernel on its stream, against MI355X's
+8 TB/s), and the reference's TA-Lib 0.8.3 C timed on the host.
 """
 from __future__ import annotations
 
@@ -46,13 +46,25 @@ PIPE7 = dict(sma_period=20, ema_period=20, rsi_period=14, macd_fast=12, macd_slo
              stoch_d=3)
 PIPE7_OUT = ("sma", "ema", "rsi", "macd", "macd_signal", "macd_hist", "bb_upper", "bb_middle",
              "bb_lower", "atr", "stoch_k", "stoch_d")
-# the fused pass runs as one kernel holding three parts; each part is also
-# timed alone, as its own kernel, through the output subset that needs only
-# it: (inputs read, outputs)
+# algorithmic HBM bytes of each kernel of the two-pass fused pipeline over n
+# bars: every input read once, every output written once.  The fold launch
+# reads high/low/close and writes the six window outputs (SMA, BBANDS x3,
+# STOCH x2); the emit launch reads them again and writes the six recurrence
+# outputs (EMA, RSI, MACD x3, ATR); lead, group and rows move a few KB.
+PIPE7_KERNEL_BYTES = {
+    "k_pipe_fold": lambda n: (3 + 6) * 4 * n,
+    "k_pipe_emit": lambda n: (3 + 6) * 4 * n,
+    "k_pack_one<8>": lambda n: (3 + 12) * 4 * n,
+    "k_pack_one<4>": lambda n: (3 + 12) * 4 * n,
+}
+# the pass's three parts, each also timed alone as the whole fused call for
+# the output subset that needs only it: (inputs read, outputs).  The window
+# part alone is one k_pack_win launch; a recurrence part alone runs the two
+# passes.
 PACK_KERNELS = {
-    "k_pack_win": (3, ("sma", "bb_upper", "bb_middle", "bb_lower", "stoch_k", "stoch_d")),
-    "k_pack<close recurrences>": (1, ("ema", "rsi", "macd", "macd_signal", "macd_hist")),
-    "k_pack<hlc recurrences>": (3, ("atr",)),
+    "windows only": (3, ("sma", "bb_upper", "bb_middle", "bb_lower", "stoch_k", "stoch_d")),
+    "close recurrences only": (1, ("ema", "rsi", "macd", "macd_signal", "macd_hist")),
+    "ATR only": (3, ("atr",)),
 }
 
 
@@ -158,6 +170,32 @@ def _gbs(nbytes, ms):
     return nbytes / (ms * 1e-3) / 1e9
 
 
+def kernel_launch_ms(T, sets, fn, passes):
+    """Average event-timed duration (ms) of every kernel the library
+    launches in one call: `passes` eager calls on the timer's stream with
+    the library's kernel timing on (HIP events recorded on the stream each
+    kernel runs on, around that kernel alone)."""
+    import tacuda_amd as ta
+
+    torch = T.torch
+    with torch.cuda.stream(T.stream):
+        for i in range(3):
+            fn(sets[i % len(sets)])
+        T.stream.synchronize()
+        ta.kernel_times()
+        ta.kernel_timing(True)
+        try:
+            for i in range(passes):
+                fn(sets[i % len(sets)])
+            T.stream.synchronize()
+        finally:
+            ta.kernel_timing(False)
+    acc = {}
+    for name, ms in ta.kernel_times():
+        acc.setdefault(name, []).append(ms)
+    return {k: sum(v) / len(v) for k, v in acc.items()}
+
+
 def _upload(torch, d, dev, nsets):
     return [{k: torch.from_numpy(d[v]).to(dev) for k, v in
              (("h", "high"), ("l", "low"), ("c", "close"))} for _ in range(nsets)]
@@ -172,15 +210,15 @@ def _pipeline7(a, torch, ops, T, dev):
     fused = T.graphs(sets, lambda s: ops.pack(s["h"], s["l"], s["c"], outputs=PIPE7_OUT, **PIPE7))
     ms = T.time(fused, a.steps, a.warmup)
     print(f"pipeline7 fused: {ms * 1e3:.1f} us/pass", file=sys.stderr, flush=True)
-    # the pass is one kernel launch (k_pack_one<8>): its average duration,
-    # ten launches per graph replay so the replay's own overhead is amortized
-    g = T.graphs(sets, lambda s: ops.pack(s["h"], s["l"], s["c"], outputs=PIPE7_OUT, **PIPE7),
-                 reps=10)
-    one_ms = T.time(g, max(a.steps // 5, 10), 2) / 10
-    print(f"  k_pack_one<8>: {one_ms * 1e3:.1f} us", file=sys.stderr, flush=True)
+    # the pass's kernels (two passes for a long dense row: k_pipe_lead,
+    # k_pipe_fold with the window part, k_pipe_group, k_pipe_rows,
+    # k_pipe_emit), each timed alone with events around its launch
+    launch = kernel_launch_ms(T, sets, lambda s: ops.pack(s["h"], s["l"], s["c"],
+                                                          outputs=PIPE7_OUT, **PIPE7), 20)
+    print("  kernels: " + ", ".join(f"{k} {v * 1e3:.1f} us" for k, v in launch.items()),
+          file=sys.stderr, flush=True)
     kernels = {}
-    # ten launches per graph replay: the replay's own overhead is amortized
-    # and the figure is the kernel's average launch duration
+    # ten calls per graph replay: the replay's own overhead is amortized
     for name, (nin, outs) in PACK_KERNELS.items():
         g = T.graphs(sets, lambda s, outs=outs: ops.pack(s["h"], s["l"], s["c"], outputs=outs,
                                                         **PIPE7), reps=10)
@@ -199,7 +237,7 @@ def _pipeline7(a, torch, ops, T, dev):
     print(f"pipeline7 drop-in: {drop_ms * 1e3:.1f} us/pass {drop}", file=sys.stderr, flush=True)
     alg_total = (3 + 12) * 4 * n
     return d, {"ms": ms, "bars": n, "alg_bytes": alg_total, "kernels": kernels,
-               "one_ms": one_ms, "drop_in_ms": drop_ms, "drop_in_kernels_us": drop}
+               "launch_ms": launch, "drop_in_ms": drop_ms, "drop_in_kernels_us": drop}
 
 
 def _batch13(a, torch, ops, T, dev):
@@ -233,6 +271,16 @@ def _scan256(a, torch, ops, T, dev):
     T.keep.clear()
     torch.cuda.empty_cache()
     print(f"scan256 fused: {fused:.3f} ms/pass", file=sys.stderr, flush=True)
+    # per kernel: the fold reads the close rows, the emit reads them again
+    # and writes the four outputs
+    launch = kernel_launch_ms(T, [c], lambda x: ops.scan(x), 3)
+    kbytes = {"k_scan_fold": 4 * bars, "k_scan_emit": (1 + 4) * 4 * bars}
+    kern = {k: {"ms": v, **({"alg_bytes": kbytes[k], "GBs": _gbs(kbytes[k], v),
+                             "frac_hbm": _gbs(kbytes[k], v) / PEAK_HBM_GBS}
+                            if k in kbytes else {})} for k, v in launch.items()}
+    print("  kernels: " + ", ".join(f"{k} {v:.3f} ms" for k, v in launch.items()),
+          file=sys.stderr, flush=True)
+    torch.cuda.empty_cache()
     calls = {"ema200": lambda x: ops.ema(x, 200), "kama30": lambda x: ops.kama(x, 30),
              "t3_5": lambda x: ops.t3(x, 5, 0.7), "sma200": lambda x: ops.sma(x, 200)}
     per = {}
@@ -247,7 +295,7 @@ def _scan256(a, torch, ops, T, dev):
     print(f"scan256 drop-in 4 calls: {total:.3f} ms/pass", file=sys.stderr, flush=True)
     return {"ms": fused, "bars_per_s": bars / (fused * 1e-3), "alg_bytes": alg,
             "GBs": _gbs(alg, fused), "frac_hbm": _gbs(alg, fused) / PEAK_HBM_GBS,
-            "shape": [rows, n], "graph": True, "drop_in_ms": total,
+            "shape": [rows, n], "graph": True, "kernels": kern, "drop_in_ms": total,
             "drop_in_frac_hbm": _gbs(alg, total) / PEAK_HBM_GBS, "drop_in_kernels": per}
 
 
@@ -265,10 +313,17 @@ def _candles(a, torch, ops, T, dev):
     torch.cuda.empty_cache()
     bars = rows * n
     alg = (4 + 61) * 4 * bars
-    print(f"candles 61 x 1024x16384: {ms:.3f} ms/pass", file=sys.stderr, flush=True)
+    launch = kernel_launch_ms(T, [x], lambda s: ops.cdl_all(s["open"], s["high"], s["low"],
+                                                            s["close"]), 5)
+    torch.cuda.empty_cache()
+    kname, kms = max(launch.items(), key=lambda kv: kv[1])
+    print(f"candles 61 x 1024x16384: {ms:.3f} ms/pass, {kname} {kms:.3f} ms", file=sys.stderr,
+          flush=True)
     return {"ms": ms, "bars_per_s": bars / (ms * 1e-3), "alg_bytes": alg, "GBs": _gbs(alg, ms),
             "frac_hbm": _gbs(alg, ms) / PEAK_HBM_GBS, "shape": [rows, n], "graph": True,
-            "patterns": 61, "dtype": "int32"}
+            "patterns": 61, "dtype": "int32",
+            "kernel": {"name": kname, "ms": kms, "GBs": _gbs(alg, kms),
+                       "frac_hbm": _gbs(alg, kms) / PEAK_HBM_GBS}}
 
 
 def _ragged(a, torch, ops, T, dev):
@@ -461,8 +516,10 @@ def main():
         print(json.dumps({"workloads": work}))
         return
     n, ms = p7["bars"], p7["ms"]
-    dom = "k_pack_one<8>"
-    dk = {"GBs": _gbs(p7["alg_bytes"], p7["one_ms"])}
+    launch = p7["launch_ms"]
+    dom = max(launch, key=launch.get)
+    dk = {"GBs": _gbs(PIPE7_KERNEL_BYTES.get(dom, lambda n: 0)(n), launch[dom]),
+          "alg_bytes": PIPE7_KERNEL_BYTES.get(dom, lambda n: 0)(n)}
     traffic = None
     tpath = os.path.join(HERE, "profiles", "traffic.json")
     if os.path.exists(tpath):
@@ -486,13 +543,15 @@ def main():
                    "pass_alg_bytes": p7["alg_bytes"],
                    "pass_GBs": _gbs(p7["alg_bytes"], ms),
                    "pass_frac_hbm": _gbs(p7["alg_bytes"], ms) / PEAK_HBM_GBS,
-                   "kernel_us": round(p7["one_ms"] * 1e3, 2),
+                   "kernels_us": {k: round(v * 1e3, 2) for k, v in launch.items()},
+                   "kernels_sum_us": round(sum(launch.values()) * 1e3, 2),
                    "parts_alone_us": {k: round(v["us"], 2) for k, v in p7["kernels"].items()},
                    "drop_in_7call_ms": p7["drop_in_ms"],
                    "drop_in_bars_per_s": n / (p7["drop_in_ms"] * 1e-3),
                    "drop_in_kernels_us": p7["drop_in_kernels_us"]},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": dk["GBs"], "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": dk["GBs"] / PEAK_HBM_GBS, "traffic": traffic},
+                     "unit": "GB/s", "frac": dk["GBs"] / PEAK_HBM_GBS, "traffic": traffic,
+                     "alg_bytes": dk["alg_bytes"], "launch_us": round(launch[dom] * 1e3, 2)},
         "workloads": work,
     }
     if not a.no_cpu:

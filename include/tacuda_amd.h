@@ -21,6 +21,17 @@ extern "C" {
  * shape on the stream reserves it too). */
 CTAPI_EXPORT ctStatus_t ctx_workspace_reserve(hipStream_t stream, size_t bytes);
 
+/* Kernel timing (no counterpart in the reference; for benchmarks and
+ * tuning).  While enabled, the fused-pass, long-scan and candle launches
+ * record a pair of HIP events around each kernel on the caller's stream;
+ * launches on a stream that is being captured record nothing.
+ * ctx_kernel_times waits for the recorded launches, writes up to maxN of
+ * them (kernel names separated by '\n' into `names`, elapsed milliseconds
+ * into `ms`), sets *count, and clears the list. */
+CTAPI_EXPORT ctStatus_t ctx_kernel_timing(int enable);
+CTAPI_EXPORT ctStatus_t ctx_kernel_times(char* names, int namesLen, float* ms, int maxN,
+                                         int* count);
+
 /* Seeded synthetic OHLCV, float32, series-major [numSeries x n]:
  * close_0 = 100, close_t = close_{t-1} * exp(0.01*z_r); open_t = close_{t-1};
  * high = max(o,c)*(1+|0.005*z_h|), low = min(o,c)*(1-|0.005*z_l|),

@@ -79,6 +79,9 @@ _sig("ct_version", stream=False, restype=ctypes.c_char_p)
 # extensions (include/tacuda_amd.h)
 _sig("ctx_synth_ohlcv", ctypes.c_ulonglong, c_int, c_int, c_int, P, P, P, P, P, c_int, stream=False)
 _sig("ctx_workspace_reserve", c_void_p, ctypes.c_size_t, stream=False)
+_sig("ctx_kernel_timing", c_int, stream=False)
+_sig("ctx_kernel_times", ctypes.c_char_p, c_int, ctypes.POINTER(c_float), c_int,
+     ctypes.POINTER(c_int), stream=False)
 
 
 class PackParams(ctypes.Structure):
@@ -235,6 +238,23 @@ def synth_ohlcv(n: int, rows: int = 1, seed: int = 0, first: int = 0, threads: i
     if rows == 1:
         arrs = [a[0] for a in arrs]
     return dict(zip(("open", "high", "low", "close", "volume"), arrs))
+
+
+def kernel_timing(enable: bool) -> None:
+    """Record HIP events around the fused-pass, long-scan and candle kernels
+    launched from now on (ctx_kernel_timing)."""
+    call("ctx_kernel_timing", 1 if enable else 0)
+
+
+def kernel_times(max_n: int = 4096):
+    """[(kernel name, milliseconds)] of the launches recorded since the last
+    call, in launch order (ctx_kernel_times waits for them)."""
+    names = ctypes.create_string_buffer(max_n * 32)
+    ms = (c_float * max_n)()
+    n = c_int(0)
+    call("ctx_kernel_times", names, len(names), ms, max_n, ctypes.byref(n))
+    keys = names.value.decode().split("\n") if n.value else []
+    return list(zip(keys, [ms[i] for i in range(n.value)]))
 
 
 from . import ops  # noqa: E402,F401

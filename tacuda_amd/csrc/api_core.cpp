@@ -2,6 +2,7 @@
 // per-stream workspace cache.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstring>
 #include <map>
 #include <tuple>
@@ -222,6 +223,87 @@ CTAPI_EXPORT ctStatus_t ct_device_sync(hipStream_t s) {
   return guarded([&]() -> ctStatus_t {
     TCA_HIP(hipStreamSynchronize(s));
     return CT_STATUS_SUCCESS;
+  });
+}
+
+// -------------------------------------------------------- kernel timer --
+namespace tca {
+namespace {
+struct KRec {
+  const char* name;
+  hipEvent_t a, b;
+};
+std::atomic<bool> g_kt_on{false};
+std::mutex g_kt_mu;
+std::vector<KRec> g_kt;
+
+bool kt_capturing(hipStream_t s) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(s, &st) != hipSuccess || st != hipStreamCaptureStatusNone;
+}
+}  // namespace
+
+KTime::KTime(hipStream_t stream, const char* name) : s_(stream), name_(name) {
+  if (!g_kt_on.load(std::memory_order_relaxed) || kt_capturing(s_)) return;
+  if (hipEventCreate(&a_) != hipSuccess) {
+    a_ = nullptr;
+    return;
+  }
+  if (hipEventRecord(a_, s_) != hipSuccess) {
+    (void)hipEventDestroy(a_);
+    a_ = nullptr;
+  }
+}
+
+KTime::~KTime() {
+  if (!a_) return;
+  hipEvent_t b = nullptr;
+  if (hipEventCreate(&b) != hipSuccess || hipEventRecord(b, s_) != hipSuccess) {
+    if (b) (void)hipEventDestroy(b);
+    (void)hipEventDestroy(a_);
+    return;
+  }
+  std::lock_guard<std::mutex> lk(g_kt_mu);
+  g_kt.push_back(KRec{name_, a_, b});
+}
+}  // namespace tca
+
+CTAPI_EXPORT ctStatus_t ctx_kernel_timing(int enable) {
+  tca::g_kt_on.store(enable != 0);
+  return CT_STATUS_SUCCESS;
+}
+
+CTAPI_EXPORT ctStatus_t ctx_kernel_times(char* names, int namesLen, float* ms, int maxN,
+                                         int* count) {
+  return guarded([&]() -> ctStatus_t {
+    if (!count || maxN < 0 || (maxN > 0 && (!ms || !names || namesLen <= 0)))
+      return fail(CT_STATUS_INVALID_ARGUMENT, "kernel_times: bad arguments");
+    std::vector<tca::KRec> recs;
+    {
+      std::lock_guard<std::mutex> lk(tca::g_kt_mu);
+      recs.swap(tca::g_kt);
+    }
+    int n = 0;
+    size_t used = 0;
+    ctStatus_t st = CT_STATUS_SUCCESS;
+    for (const auto& r : recs) {
+      float t = 0.0f;
+      hipError_t e = hipEventSynchronize(r.b);
+      if (e == hipSuccess) e = hipEventElapsedTime(&t, r.a, r.b);
+      if (e != hipSuccess && st == CT_STATUS_SUCCESS) st = hip_fail(e, "kernel_times");
+      const size_t len = std::strlen(r.name) + 1;
+      if (st == CT_STATUS_SUCCESS && n < maxN && used + len <= (size_t)namesLen) {
+        std::memcpy(names + used, r.name, len - 1);
+        names[used + len - 1] = '\n';
+        used += len;
+        ms[n++] = t;
+      }
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
+    if (maxN > 0) names[used > 0 ? used - 1 : 0] = '\0';
+    *count = n;
+    return st;
   });
 }
 

@@ -203,6 +203,7 @@ ctStatus_t launch(const float* o, const float* h, const float* l, const float* c
   const long long blocks = (long long)tiles * rows;
   if (blocks > 0x7fffffffLL) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: matrix too large", fn);
   const bool all = mask == (~0ull >> (64 - kCount)) && planeStride != 0;
+  KTime kt(s, i32 && all ? "k_cdl<I32,ALL>" : i32 ? "k_cdl<I32>" : "k_cdl");
   if (i32 && all)
     hipLaunchKernelGGL((k_cdl<true, true>), dim3((unsigned)blocks), dim3(kTile), 0, s, o, h, l, c,
                        out, planeStride, mask, n, rows, tiles);

@@ -56,6 +56,23 @@ void workspace_release_all();
 void workspace_release(hipStream_t stream);
 size_t workspace_allocations();
 
+// -------------------------------------------------------- kernel timer --
+// While ctx_kernel_timing is on, a KTime scope records a pair of HIP events
+// around the launches it encloses on `stream` (none while the stream is
+// being captured).
+class KTime {
+ public:
+  KTime(hipStream_t stream, const char* name);
+  ~KTime();
+  KTime(const KTime&) = delete;
+  KTime& operator=(const KTime&) = delete;
+
+ private:
+  hipStream_t s_;
+  const char* name_;
+  hipEvent_t a_ = nullptr;
+};
+
 // ------------------------------------------------------------ geometry --
 inline int div_up(long long a, long long b) { return (int)((a + b - 1) / b); }
 

@@ -792,17 +792,29 @@ ctStatus_t launch_scan(const float* x, int* lead, double* agg, double* carry, co
                        const ScanOut& o, hipStream_t s) {
   const int segs = P.rows * P.segsPerRow;
   const int blocks = div_up(segs, kLW);
-  hipLaunchKernelGGL(k_scan_lead, dim3(P.rows), dim3(kWave), 0, s, x, P.n, P.rows, lead);
+  {
+    KTime kt(s, "k_scan_lead");
+    hipLaunchKernelGGL(k_scan_lead, dim3(P.rows), dim3(kWave), 0, s, x, P.n, P.rows, lead);
+  }
   TCA_HIP(hipGetLastError());
   if (P.useCarry) {
-    hipLaunchKernelGGL((k_scan_fold<H, FM>), dim3(blocks), dim3(kLW * kWave), 0, s, x, lead, P,
-                       agg);
+    {
+      KTime kt(s, "k_scan_fold");
+      hipLaunchKernelGGL((k_scan_fold<H, FM>), dim3(blocks), dim3(kLW * kWave), 0, s, x, lead, P,
+                         agg);
+    }
     TCA_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_scan_carry, dim3(P.rows), dim3(kWave), 0, s, P, agg, carry);
+    {
+      KTime kt(s, "k_scan_carry");
+      hipLaunchKernelGGL(k_scan_carry, dim3(P.rows), dim3(kWave), 0, s, P, agg, carry);
+    }
     TCA_HIP(hipGetLastError());
   }
-  hipLaunchKernelGGL((k_scan_emit<H, FM>), dim3(blocks), dim3(kLW * kWave), 0, s, x, lead, P,
-                     carry, o);
+  {
+    KTime kt(s, "k_scan_emit");
+    hipLaunchKernelGGL((k_scan_emit<H, FM>), dim3(blocks), dim3(kLW * kWave), 0, s, x, lead, P,
+                       carry, o);
+  }
   TCA_HIP(hipGetLastError());
   return CT_STATUS_SUCCESS;
 }

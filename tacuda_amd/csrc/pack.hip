@@ -1078,6 +1078,7 @@ static ctStatus_t launch_one(int len, int rows, hipStream_t s, const PackP& prm,
     if (st == CT_STATUS_SUCCESS && wH) st = workspace_for(s, bytes, &wsH, 1);
     if (st != CT_STATUS_SUCCESS) return st;
   }
+  KTime kt(s, K == 4 ? "k_pack_one<4>" : "k_pack_one<8>");
   hipLaunchKernelGGL((k_pack_one<K>), dim3(nbW + nbC + nbH), dim3(kFuseW * kWave), 0, s, gw, gr,
                      wsC, wsH, useLB ? 1 : 0, nbW, nbC, nbH, prm, io);
   TCA_HIP(hipGetLastError());
@@ -2031,18 +2032,34 @@ ctStatus_t launch_pipe(int len, int rows, hipStream_t s, const PackP& prm, const
   P.qt = genter + (size_t)groups * 7;
   Geometry gw = dense_geometry(len, rows, ColTile<64>::SEG);
   const int nbW = (prm.mask & kWinMask) ? gw.totalSegs : 0;
-  hipLaunchKernelGGL(k_pipe_lead, dim3(rows), dim3(kWave), 0, s, P, io.c, lead);
+  {
+    KTime kt(s, "k_pipe_lead");
+    hipLaunchKernelGGL(k_pipe_lead, dim3(rows), dim3(kWave), 0, s, P, io.c, lead);
+  }
   TCA_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_pipe_fold, dim3(nbW + segs), dim3(kPW * kWave), 0, s, P, prm, io, lead,
-                     agg, gw, nbW);
+  {
+    KTime kt(s, "k_pipe_fold");
+    hipLaunchKernelGGL(k_pipe_fold, dim3(nbW + segs), dim3(kPW * kWave), 0, s, P, prm, io, lead,
+                       agg, gw, nbW);
+  }
   TCA_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_pipe_group, dim3(div_up(groups, kPW)), dim3(kPW * kWave), 0, s, P, agg,
-                     gtot);
+  {
+    KTime kt(s, "k_pipe_group");
+    hipLaunchKernelGGL(k_pipe_group, dim3(div_up(groups, kPW)), dim3(kPW * kWave), 0, s, P, agg,
+                       gtot);
+  }
   TCA_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_pipe_rows, dim3(div_up(rows, kPW)), dim3(kPW * kWave), 0, s, P, gtot,
-                     genter);
+  {
+    KTime kt(s, "k_pipe_rows");
+    hipLaunchKernelGGL(k_pipe_rows, dim3(div_up(rows, kPW)), dim3(kPW * kWave), 0, s, P, gtot,
+                       genter);
+  }
   TCA_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_pipe_emit, dim3(segs), dim3(kPW * kWave), 0, s, P, io, lead, agg, genter);
+  {
+    KTime kt(s, "k_pipe_emit");
+    hipLaunchKernelGGL(k_pipe_emit, dim3(segs), dim3(kPW * kWave), 0, s, P, io, lead, agg,
+                       genter);
+  }
   TCA_HIP(hipGetLastError());
   return CT_STATUS_SUCCESS;
 }

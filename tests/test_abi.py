@@ -167,3 +167,15 @@ def test_ragged_layout_arguments_are_validated_without_the_device():
     assert L.ctx_ragged_destroy(None) == 0
     buf = np.zeros(4, np.float32)
     assert L.ctx_ema_ragged_d(buf.ctypes.data, buf.ctypes.data, None, 5, None) == 4
+
+
+def test_kernel_timing_without_the_device():
+    import tacuda_amd as ta
+
+    L = ta.lib()
+    n = ctypes.c_int(-1)
+    assert L.ctx_kernel_times(None, 0, None, 0, None) == 4
+    assert L.ctx_kernel_times(None, 0, None, 1, ctypes.byref(n)) == 4
+    assert L.ctx_kernel_times(None, 0, None, 0, ctypes.byref(n)) == 0 and n.value == 0
+    assert L.ctx_kernel_timing(1) == 0 and L.ctx_kernel_timing(0) == 0
+    assert ta.kernel_times() == []
