@@ -629,43 +629,61 @@ __device__ __forceinline__ bool win_sma_in_bb(const PackP& P) {
   return (m & (1u << kOSma)) && bbAll && P.smaP == P.bbP;
 }
 
-template <class C>
+template <class C, int IL = 4>
 __device__ __forceinline__ void win_means(const Seg& c, ColLds<C::E>& s, const PackP& P,
                                           const PackIO& io) {
   const unsigned m = P.mask;
   const int lane = lane_id();
   const int base = c.s0 - C::H;  // bar of e = 0
   const bool smaInBb = win_sma_in_bb(P);
+  // the output chunks in groups of IL, evaluated side by side
+  constexpr int NJ = C::NC - C::NH;
+  static_assert(NJ % IL == 0, "chunk groups");
   if ((m & (1u << kOSma)) && !smaInBb) {
     const int p = P.smaP;
-    for (int j = C::NH; j < C::NC; ++j) {
-      const int e = j * kWave + lane;
-      double sum = 0.0;
-      for (int q = e - p + 1; q <= e; ++q) sum += (double)lds_f(s.c, q);
-      const float v = (base + e >= p - 1) ? (float)(sum / (double)p) : qnanf_();
-      col_store<C>(io.out[kOSma], c, j, v);
+    for (int g = 0; g < NJ; g += IL) {
+      double sum[IL];
+#pragma unroll
+      for (int u = 0; u < IL; ++u) sum[u] = 0.0;
+      const int q0 = (C::NH + g) * kWave + lane - p + 1;
+      for (int k = 0; k < p; ++k) {
+#pragma unroll
+        for (int u = 0; u < IL; ++u) sum[u] += (double)lds_f(s.c, q0 + u * kWave + k);
+      }
+#pragma unroll
+      for (int u = 0; u < IL; ++u) {
+        const int j = C::NH + g + u, e = j * kWave + lane;
+        const float v = (base + e >= p - 1) ? (float)(sum[u] / (double)p) : qnanf_();
+        col_store<C>(io.out[kOSma], c, j, v);
+      }
     }
   }
   if (m & (1u << kOWma)) {
     const int p = P.wmaP;
     const double divider = (double)p * (p + 1) / 2.0;
-    for (int j = C::NH; j < C::NC; ++j) {
-      const int e = j * kWave + lane;
-      float v;
-      if (p == 1) {
-        v = lds_f(s.c, e);
-      } else {
-        double sum = 0.0;
+    for (int g = 0; g < NJ; g += IL) {
+      double sum[IL];
+#pragma unroll
+      for (int u = 0; u < IL; ++u) sum[u] = 0.0;
+      if (p > 1) {
+        const int q0 = (C::NH + g) * kWave + lane - p + 1;
         double w = 1.0;
-        for (int q = e - p + 1; q <= e; ++q, w += 1.0) sum += (double)lds_f(s.c, q) * w;
-        v = (float)(sum / divider);
+        for (int k = 0; k < p; ++k, w += 1.0) {
+#pragma unroll
+          for (int u = 0; u < IL; ++u) sum[u] += (double)lds_f(s.c, q0 + u * kWave + k) * w;
+        }
       }
-      col_store<C>(io.out[kOWma], c, j, (base + e >= p - 1) ? v : qnanf_());
+#pragma unroll
+      for (int u = 0; u < IL; ++u) {
+        const int j = C::NH + g + u, e = j * kWave + lane;
+        const float v = p == 1 ? lds_f(s.c, e) : (float)(sum[u] / divider);
+        col_store<C>(io.out[kOWma], c, j, (base + e >= p - 1) ? v : qnanf_());
+      }
     }
   }
 }
 
-template <class C>
+template <class C, int IL = 4>
 __device__ __forceinline__ void win_bbands(const Seg& c, ColLds<C::E>& s, const PackP& P,
                                            const PackIO& io) {
   const unsigned m = P.mask;
@@ -676,19 +694,32 @@ __device__ __forceinline__ void win_bbands(const Seg& c, ColLds<C::E>& s, const 
   if (bbAll) {
     const int p = P.bbP;
     const double inv = 1.0 / (double)p;
-    for (int j = C::NH; j < C::NC; ++j) {
-      const int e = j * kWave + lane;
-      const double shift = (double)lds_f(s.c, e - p + 1);
-      double sum = 0.0, s1 = 0.0, s2 = 0.0;
-      for (int q = e - p + 1; q <= e; ++q) {
-        const double x = (double)lds_f(s.c, q);
-        sum += x;
-        const double d = x - shift;
-        s1 += d;
-        s2 += d * d;
+    // the output chunks in groups of IL, evaluated side by side
+    constexpr int NJ = C::NC - C::NH;
+    static_assert(NJ % IL == 0, "chunk groups");
+    for (int g = 0; g < NJ; g += IL) {
+    const int q0 = (C::NH + g) * kWave + lane - p + 1;
+    double shift[IL], sum[IL], s1[IL], s2[IL];
+#pragma unroll
+    for (int u = 0; u < IL; ++u) {
+      shift[u] = (double)lds_f(s.c, q0 + u * kWave);
+      sum[u] = s1[u] = s2[u] = 0.0;
+    }
+    for (int k = 0; k < p; ++k) {
+#pragma unroll
+      for (int u = 0; u < IL; ++u) {
+        const double x = (double)lds_f(s.c, q0 + u * kWave + k);
+        sum[u] += x;
+        const double d = x - shift[u];
+        s1[u] += d;
+        s2[u] += d * d;
       }
-      const double mean = sum / p;
-      const double var = col_var_finish(s1, s2, inv);
+    }
+#pragma unroll
+    for (int u = 0; u < IL; ++u) {
+      const int j = C::NH + g + u, e = j * kWave + lane;
+      const double mean = sum[u] / p;
+      const double var = col_var_finish(s1[u], s2[u], inv);
       const double sd = (var != 0.0) ? sqrt(var) : 0.0;
       float up, lo;
       if (P.bbUp == P.bbDn) {
@@ -700,10 +731,11 @@ __device__ __forceinline__ void win_bbands(const Seg& c, ColLds<C::E>& s, const 
         lo = (float)(mean - sd * P.bbDn);
       }
       const bool ok = base + e >= p - 1;
-      if (smaInBb) col_store<C>(io.out[kOSma], c, j, ok ? (float)(sum / (double)p) : qnanf_());
+      if (smaInBb) col_store<C>(io.out[kOSma], c, j, ok ? (float)(sum[u] / (double)p) : qnanf_());
       col_store<C>(io.out[kOBbU], c, j, ok ? up : qnanf_());
       col_store<C>(io.out[kOBbM], c, j, ok ? (float)mean : qnanf_());
       col_store<C>(io.out[kOBbL], c, j, ok ? lo : qnanf_());
+    }
     }
   }
 }
@@ -780,7 +812,7 @@ __device__ __forceinline__ void win_extrema(const Seg& c, ColLds<C::E>& s, const
   }
 }
 
-template <class C>
+template <class C, int IL = 4>
 __device__ __forceinline__ void win_cci(const Seg& c, ColLds<C::E>& s, double* d, const PackP& P,
                                         const PackIO& io) {
   const unsigned m = P.mask;
@@ -796,17 +828,32 @@ __device__ __forceinline__ void win_cci(const Seg& c, ColLds<C::E>& s, double* d
       lds_wd(d, e, tp);
     }
     col_sync();
-    for (int j = C::NH; j < C::NC; ++j) {
-      const int e = j * kWave + lane;
-      double sum = 0.0;
-      for (int q = e; q > e - p; --q) sum += lds_d(d, q);
-      const double sma = sum / p;
-      double dev = 0.0;
-      for (int q = e; q > e - p; --q) dev += fabs(lds_d(d, q) - sma);
-      const double md = dev / p;
-      const double num = lds_d(d, e) - sma;
-      const float v = (num != 0.0 && md != 0.0) ? (float)(num / (0.015 * md)) : 0.0f;
-      col_store<C>(io.out[kOCci], c, j, (base + e >= p - 1) ? v : qnanf_());
+    // the output chunks in groups of IL, evaluated side by side
+    constexpr int NJ = C::NC - C::NH;
+    static_assert(NJ % IL == 0, "chunk groups");
+    for (int g = 0; g < NJ; g += IL) {
+      const int eo = (C::NH + g) * kWave + lane;
+      double sum[IL], sma[IL], dev[IL];
+#pragma unroll
+      for (int u = 0; u < IL; ++u) sum[u] = dev[u] = 0.0;
+      for (int k = 0; k < p; ++k) {
+#pragma unroll
+        for (int u = 0; u < IL; ++u) sum[u] += lds_d(d, eo + u * kWave - k);
+      }
+#pragma unroll
+      for (int u = 0; u < IL; ++u) sma[u] = sum[u] / p;
+      for (int k = 0; k < p; ++k) {
+#pragma unroll
+        for (int u = 0; u < IL; ++u) dev[u] += fabs(lds_d(d, eo + u * kWave - k) - sma[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < IL; ++u) {
+        const int j = C::NH + g + u, e = j * kWave + lane;
+        const double md = dev[u] / p;
+        const double num = lds_d(d, e) - sma[u];
+        const float v = (num != 0.0 && md != 0.0) ? (float)(num / (0.015 * md)) : 0.0f;
+        col_store<C>(io.out[kOCci], c, j, (base + e >= p - 1) ? v : qnanf_());
+      }
     }
   }
 }
@@ -815,6 +862,10 @@ template <class C>
 __device__ __forceinline__ bool win_need_hl(const PackP& P) {
   return P.mask & ((1u << kOStK) | (1u << kOStD) | (1u << kOWillr) | (1u << kOCci));
 }
+
+// Window chunks evaluated side by side when one wave computes every window
+// family (four independent accumulator chains per family loop).
+constexpr int kWinIL = 4;
 
 // Every window family of the pass, one wave.
 template <class C>
@@ -826,11 +877,11 @@ __device__ __forceinline__ void pack_windows(const Seg& c, ColLds<C::E>& s, cons
   }
   col_stage<C>(s.c, io.c, c);
   col_sync();
-  win_means<C>(c, s, P, io);
-  win_bbands<C>(c, s, P, io);
+  win_means<C, kWinIL>(c, s, P, io);
+  win_bbands<C, kWinIL>(c, s, P, io);
   win_extrema<C>(c, s, P, io);
   col_sync();
-  win_cci<C>(c, s, s.d0, P, io);
+  win_cci<C, kWinIL>(c, s, s.d0, P, io);
 }
 
 // Per-block LDS of the split window pass: one staged tile, the extrema
@@ -844,7 +895,7 @@ struct ColSplitLds {
 // The window families of one segment over a workgroup of four waves, one
 // family group per wave: BBANDS (and SMA of its period); STOCH and WILLR;
 // CCI; SMA and WMA.
-template <class C>
+template <class C, int IL>
 __device__ __forceinline__ void pack_windows_split(const Seg& c, ColSplitLds<C::E>& L,
                                                    const PackP& P, const PackIO& io, int w) {
   const bool hl = win_need_hl<C>(P);
@@ -852,10 +903,10 @@ __device__ __forceinline__ void pack_windows_split(const Seg& c, ColSplitLds<C::
   else if (w == 1 && hl) col_stage<C>(L.t.h, io.h, c);
   else if (w == 2 && hl) col_stage<C>(L.t.l, io.l, c);
   __syncthreads();
-  if (w == 0) win_bbands<C>(c, L.t, P, io);
+  if (w == 0) win_bbands<C, IL>(c, L.t, P, io);
   else if (w == 1) win_extrema<C>(c, L.t, P, io);
-  else if (w == 2) win_cci<C>(c, L.t, L.d, P, io);
-  else win_means<C>(c, L.t, P, io);
+  else if (w == 2) win_cci<C, IL>(c, L.t, L.d, P, io);
+  else win_means<C, IL>(c, L.t, P, io);
 }
 
 template <int H, int W>
@@ -1108,6 +1159,7 @@ constexpr int kPSeg = kPK * kWave;   // bars per segment
 constexpr int kPW = 4;               // waves per workgroup
 constexpr int kPMinLen = 16 * kPSeg; // shortest row taking the two passes
 constexpr unsigned kPUnsupported = (1u << kODema) | (1u << kOTema) | (1u << kOAdx);
+constexpr int kPWinIL = 1;  // window chunks side by side in the fold launch's window waves
 
 // Multiplier fields of a composite, in the order of PipeP::q: Ae, Ag, Al,
 // Af, As, agf, ags, agg, Aa.
@@ -1875,7 +1927,7 @@ __global__ __launch_bounds__(kPW* kWave) void k_pipe_fold(PipeP P, PackP prm, Pa
     c.tree = false;
     c.epoch = 0;
     if (resolve(gw, ColTile<64>::SEG, (int)blockIdx.x, c))
-      pack_windows_split<ColTile<64>>(c, sm.win, prm, io, w);
+      pack_windows_split<ColTile<64>, kPWinIL>(c, sm.win, prm, io, w);
     return;
   }
   const int seg = (int)blockIdx.x - nbW;
