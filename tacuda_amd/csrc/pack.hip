@@ -740,7 +740,7 @@ __device__ __forceinline__ void win_bbands(const Seg& c, ColLds<C::E>& s, const 
   }
 }
 
-template <class C>
+template <class C, int IL = 4>
 __device__ __forceinline__ void win_extrema(const Seg& c, ColLds<C::E>& s, const PackP& P,
                                             const PackIO& io) {
   const unsigned m = P.mask;
@@ -755,6 +755,7 @@ __device__ __forceinline__ void win_extrema(const Seg& c, ColLds<C::E>& s, const
     int k = col_extrema_table<C>(s, kp);
     auto willr = [&]() {
       const int p = P.willrP;
+#pragma unroll
       for (int j = C::NH; j < C::NC; ++j) {
         const int e = j * kWave + lane;
         float hh, ll;
@@ -772,36 +773,91 @@ __device__ __forceinline__ void win_extrema(const Seg& c, ColLds<C::E>& s, const
       const int lb = kp - 1 + 2 * (dp - 1);
       // fast %K from bar s0 - 2(d-1), slow %K from s0 - (d-1)
       const int eK = C::H - 2 * (dp - 1), eS = C::H - (dp - 1);
-      for (int j = eK / kWave; j < C::NC; ++j) {
-        const int e = j * kWave + lane;
-        float hh, ll;
-        col_hl<C>(s, e < eK ? eK : e, kp, k, hh, ll);
-        const double range = (double)hh - (double)ll;
-        const double cl = (double)lds_f(s.c, e);
-        const double fk = (fabs(range) <= 1e-14 * (fabs((double)hh) + fabs((double)ll)))
-                              ? 0.0
-                              : (cl - (double)ll) / range * 100.0;
-        if (e >= eK) lds_wd(s.d0, e, fk);
+      if constexpr (IL == 1) {
+        for (int j = eK / kWave; j < C::NC; ++j) {
+          const int e = j * kWave + lane;
+          float hh, ll;
+          col_hl<C>(s, e < eK ? eK : e, kp, k, hh, ll);
+          const double range = (double)hh - (double)ll;
+          const double cl = (double)lds_f(s.c, e);
+          const double fk = (fabs(range) <= 1e-14 * (fabs((double)hh) + fabs((double)ll)))
+                                ? 0.0
+                                : (cl - (double)ll) / range * 100.0;
+          if (e >= eK) lds_wd(s.d0, e, fk);
+        }
+        col_sync();
+        double* d1 = s.d1();  // the extrema table is no longer read
+        for (int j = eS / kWave; j < C::NC; ++j) {
+          const int e = j * kWave + lane;
+          double sum = 0.0;
+          if (e >= eS)
+            for (int q = e - dp + 1; q <= e; ++q) sum += lds_d(s.d0, q);
+          if (e >= eS) lds_wd(d1, e, sum / dp);
+        }
+        col_sync();
+        for (int j = C::NH; j < C::NC; ++j) {
+          const int e = j * kWave + lane;
+          double sum = 0.0;
+          for (int q = e - dp + 1; q <= e; ++q) sum += lds_d(d1, q);
+          const bool ok = base + e >= lb;
+          col_store<C>(io.out[kOStK], c, j, ok ? (float)lds_d(d1, e) : qnanf_());
+          col_store<C>(io.out[kOStD], c, j, ok ? (float)(sum / dp) : qnanf_());
+        }
+        col_sync();
+      } else {
+        // every chunk side by side (chunks before eK or eS compute values
+        // that are not stored)
+  #pragma unroll
+        for (int j = 0; j < C::NC; ++j) {
+          const int e = j * kWave + lane;
+          float hh, ll;
+          col_hl<C>(s, e < eK ? eK : e, kp, k, hh, ll);
+          const double range = (double)hh - (double)ll;
+          const double cl = (double)lds_f(s.c, e);
+          const double fk = (fabs(range) <= 1e-14 * (fabs((double)hh) + fabs((double)ll)))
+                                ? 0.0
+                                : (cl - (double)ll) / range * 100.0;
+          if (e >= eK) lds_wd(s.d0, e, fk);
+        }
+        col_sync();
+        double* d1 = s.d1();  // the extrema table is no longer read
+        {
+          double sum[C::NC];
+  #pragma unroll
+          for (int j = 0; j < C::NC; ++j) sum[j] = 0.0;
+          for (int q = 0; q < dp; ++q) {
+  #pragma unroll
+            for (int j = 0; j < C::NC; ++j) {
+              const int i = j * kWave + lane - dp + 1 + q;
+              sum[j] += lds_d(s.d0, i < 0 ? 0 : i);
+            }
+          }
+  #pragma unroll
+          for (int j = 0; j < C::NC; ++j) {
+            const int e = j * kWave + lane;
+            if (e >= eS) lds_wd(d1, e, sum[j] / dp);
+          }
+        }
+        col_sync();
+        {
+          constexpr int NJ = C::NC - C::NH;
+          double sum[NJ];
+  #pragma unroll
+          for (int u = 0; u < NJ; ++u) sum[u] = 0.0;
+          for (int q = 0; q < dp; ++q) {
+  #pragma unroll
+            for (int u = 0; u < NJ; ++u) sum[u] += lds_d(d1, (C::NH + u) * kWave + lane - dp + 1 + q);
+          }
+  #pragma unroll
+          for (int u = 0; u < NJ; ++u) {
+            const int j = C::NH + u, e = j * kWave + lane;
+            const bool ok = base + e >= lb;
+            col_store<C>(io.out[kOStK], c, j, ok ? (float)lds_d(d1, e) : qnanf_());
+            col_store<C>(io.out[kOStD], c, j, ok ? (float)(sum[u] / dp) : qnanf_());
+          }
+        }
+        col_sync();
       }
-      col_sync();
-      double* d1 = s.d1();  // the extrema table is no longer read
-      for (int j = eS / kWave; j < C::NC; ++j) {
-        const int e = j * kWave + lane;
-        double sum = 0.0;
-        if (e >= eS)
-          for (int q = e - dp + 1; q <= e; ++q) sum += lds_d(s.d0, q);
-        if (e >= eS) lds_wd(d1, e, sum / dp);
-      }
-      col_sync();
-      for (int j = C::NH; j < C::NC; ++j) {
-        const int e = j * kWave + lane;
-        double sum = 0.0;
-        for (int q = e - dp + 1; q <= e; ++q) sum += lds_d(d1, q);
-        const bool ok = base + e >= lb;
-        col_store<C>(io.out[kOStK], c, j, ok ? (float)lds_d(d1, e) : qnanf_());
-        col_store<C>(io.out[kOStD], c, j, ok ? (float)(sum / dp) : qnanf_());
-      }
-      col_sync();
       if (wantW && P.willrP != kp) {
         k = col_extrema_table<C>(s, P.willrP);
         willr();
@@ -879,7 +935,7 @@ __device__ __forceinline__ void pack_windows(const Seg& c, ColLds<C::E>& s, cons
   col_sync();
   win_means<C, kWinIL>(c, s, P, io);
   win_bbands<C, kWinIL>(c, s, P, io);
-  win_extrema<C>(c, s, P, io);
+  win_extrema<C, kWinIL>(c, s, P, io);
   col_sync();
   win_cci<C, kWinIL>(c, s, s.d0, P, io);
 }
@@ -904,7 +960,7 @@ __device__ __forceinline__ void pack_windows_split(const Seg& c, ColSplitLds<C::
   else if (w == 2 && hl) col_stage<C>(L.t.l, io.l, c);
   __syncthreads();
   if (w == 0) win_bbands<C, IL>(c, L.t, P, io);
-  else if (w == 1) win_extrema<C>(c, L.t, P, io);
+  else if (w == 1) win_extrema<C, IL>(c, L.t, P, io);
   else if (w == 2) win_cci<C, IL>(c, L.t, L.d, P, io);
   else win_means<C, IL>(c, L.t, P, io);
 }
