@@ -15,7 +15,8 @@ Also measured in the same run (reported under "workloads"):
            in one fused call, hipGraph replay, five rotated buffer sets;
   scan256  256 series x 2^20 bars, EMA(200), KAMA(30), T3(5, 0.7), SMA(200)
            through the fused two-pass scan (ctx_scan_batch_d), with the four
-           dense drop-in batch calls (look-back scan) timed beside it;
+           dense drop-in batch calls timed beside it;
+  scan1024 the same scan on 1024 series x 2^20 bars;
   candles  all 61 candlestick patterns, int32 signals, 1024 x 16,384 OHLC;
   ragged   4096 series of lengths U[64, 8192], the 13-indicator pass through
            a tuned, graph-captured plan;
@@ -79,8 +80,8 @@ def _args():
                     help="time budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", default="",
-                    help="comma list of workloads: pipeline7,batch13,scan256,candles,ragged,ticks,"
-                         "autograd")
+                    help="comma list of workloads: pipeline7,batch13,scan256,scan1024,candles,ragged,"
+                         "ticks,autograd")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="also run a plain device copy of this many MB (counter calibration)")
     return ap.parse_args()
@@ -299,6 +300,36 @@ def _scan256(a, torch, ops, T, dev):
             "drop_in_frac_hbm": _gbs(alg, total) / PEAK_HBM_GBS, "drop_in_kernels": per}
 
 
+def _scan1024(a, torch, ops, T, dev):
+    """1024 x 2^20 close through the fused scan (EMA(200), KAMA(30),
+    T3(5, 0.7), SMA(200)): the 256 seeded rows of scan256 stacked four
+    times (rows are independent, so the cost is that of 1024 distinct
+    rows)."""
+    import tacuda_amd as ta
+
+    rows, n = 1024, 1 << 20
+    base = torch.from_numpy(ta.synth_ohlcv(n, rows=256, seed=0)["close"]).to(dev)
+    c = base.repeat(rows // 256, 1).contiguous()
+    del base
+    bars = rows * n
+    alg = (1 + 4) * 4 * bars
+    g = T.graphs([c], lambda x: ops.scan(x))
+    ms = T.time(g, max(3, a.steps // 10), 1)
+    T.keep.clear()
+    torch.cuda.empty_cache()
+    launch = kernel_launch_ms(T, [c], lambda x: ops.scan(x), 2)
+    torch.cuda.empty_cache()
+    kbytes = {"k_scan_fold": 4 * bars, "k_scan_emit": (1 + 4) * 4 * bars}
+    kern = {k: {"ms": v, **({"alg_bytes": kbytes[k], "GBs": _gbs(kbytes[k], v),
+                             "frac_hbm": _gbs(kbytes[k], v) / PEAK_HBM_GBS}
+                            if k in kbytes else {})} for k, v in launch.items()}
+    print(f"scan1024 fused: {ms:.3f} ms/pass; kernels: " +
+          ", ".join(f"{k} {v:.3f} ms" for k, v in launch.items()), file=sys.stderr, flush=True)
+    return {"ms": ms, "bars_per_s": bars / (ms * 1e-3), "alg_bytes": alg, "GBs": _gbs(alg, ms),
+            "frac_hbm": _gbs(alg, ms) / PEAK_HBM_GBS, "shape": [rows, n], "graph": True,
+            "data": "the 256 seeded scan256 rows stacked 4x", "kernels": kern}
+
+
 def _candles(a, torch, ops, T, dev):
     """All 61 candlestick patterns, int32 signals, 1024 x 16,384 OHLC in one
     launch (ctx_cdl_all_batch_d)."""
@@ -483,8 +514,8 @@ def main():
     from tacuda_amd import ops
 
     dev = torch.device("cuda:0")
-    only = set(filter(None, a.only.split(","))) or {"pipeline7", "batch13", "scan256", "candles",
-                                                    "ragged", "ticks", "autograd"}
+    only = set(filter(None, a.only.split(","))) or {"pipeline7", "batch13", "scan256", "scan1024",
+                                                    "candles", "ragged", "ticks", "autograd"}
     T = Timer(torch)
     work = {}
     d = p7 = None
@@ -496,6 +527,9 @@ def main():
     torch.cuda.empty_cache()
     if "scan256" in only:
         work["scan256_ema200_kama30_t3_sma200"] = _scan256(a, torch, ops, T, dev)
+    if "scan1024" in only:
+        work["scan1024_ema200_kama30_t3_sma200"] = _scan1024(a, torch, ops, T, dev)
+        torch.cuda.empty_cache()
     if "candles" in only:
         work["candles_61_1024x16384"] = _candles(a, torch, ops, T, dev)
     if "ragged" in only:
