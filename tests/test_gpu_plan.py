@@ -122,3 +122,42 @@ def test_user_graph_capture_of_the_fused_pass(dev, shape):
         for k in eager:
             assert torch.equal(cap[k].view(torch.int32), eager[k].view(torch.int32)), k
     del junk, g
+
+
+@pytest.mark.parametrize("shape", [(1, 200000), (6, 20000)])
+def test_two_pass_plan_and_user_capture(dev, shape):
+    """The two-pass fused pass (long dense rows, EMA/RSI/MACD/ATR with short
+    windows) as a tuned plan and under the caller's own graph capture: every
+    replay equals the oracle, and the eager call bit for bit."""
+    import tacuda_amd as ta
+    from tests.test_gpu_pack import DEFAULT, PIPE_OUT
+
+    rows, n = shape
+    print(f"two-pass plan/capture {rows}x{n}", flush=True)
+    d = ta.synth_ohlcv(n, rows=rows, seed=53)
+    T = [torch.from_numpy(np.ascontiguousarray(d[k])).to(dev) for k in ("high", "low", "close")]
+    T = [t if rows > 1 else t.reshape(-1) for t in T]
+    plan = ta.ops.Plan(*T, outputs=PIPE_OUT, tune=True, **DEFAULT)
+    first = {k: v.clone() for k, v in plan.launch().items()}
+    torch.cuda.synchronize()
+    _check_all(first, d, DEFAULT)
+    st = torch.cuda.Stream()
+    with torch.cuda.stream(st):
+        eager = {k: v.clone() for k, v in ta.ops.pack(*T, outputs=PIPE_OUT, **DEFAULT).items()}
+    st.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=st):
+        cap = ta.ops.pack(*T, outputs=PIPE_OUT, **DEFAULT)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        junk = torch.randn(1 << 20, device=dev).sum()
+        for v in cap.values():
+            v.fill_(0.0)
+        g.replay()
+        out = plan.launch()
+        torch.cuda.synchronize()
+        for k in eager:
+            assert torch.equal(cap[k].view(torch.int32), eager[k].view(torch.int32)), k
+            assert torch.equal(out[k].view(torch.int32), first[k].view(torch.int32)), k
+    plan.close()
+    del junk, g
