@@ -1199,9 +1199,10 @@ static ctStatus_t launch_one(int len, int rows, hipStream_t s, const PackP& prm,
 // (the affine map of its bars, from which the state after the segment
 // follows from the state before it), a wave per 64 segments scans them into
 // in-group prefixes and group composites, a wave per row scans those into
-// the state entering every group, and an emit pass
-// replays the bars from there with the families' TA-Lib arithmetic, its
-// window workgroups (the window part of the pass) alongside.
+// the state entering every group, and an emit pass replays the bars from
+// there with the families' TA-Lib arithmetic.  Fold and emit give each
+// 512-bar segment a workgroup with one wave per recurrence group; the
+// window part of the pass runs in the fold launch.
 //
 // Every recurrence maps y -> A y + b over a run of bars: identity before its
 // seed bar, a reset to the seed value at it, the family's step after it.
