@@ -512,8 +512,17 @@ struct ScanOut {
 // SMA window sums of a wave whose staged in-row bars are all finite: prefix
 // sums of the staged range's 16-bar chunks (pc[k] = bars of chunks < k),
 // so a lane's first window costs its two partial chunks plus a difference.
+// Lane l holds the prefix of chunks 2l and 2l + 1 in registers; other lanes
+// read them with a cross-lane shuffle, which keeps them out of LDS (and the
+// emit launch's LDS at one staged tile per wave).
+struct SmaPrefix {
+  double excl;   // pc[2 * lane]
+  double c0;     // bars of chunk 2 * lane: pc[2 * lane + 1] = excl + c0
+  double total;  // pc of one past the last chunk
+};
+
 template <class T>
-__device__ __forceinline__ bool sma_prefix(const float* lds, const Seg& c, double* pc,
+__device__ __forceinline__ bool sma_prefix(const float* lds, const Seg& c, SmaPrefix& pr,
                                            bool check) {
   constexpr int NCH = T::LEN / kLK;
   static_assert(NCH <= 2 * kWave, "two chunks per lane");
@@ -540,18 +549,35 @@ __device__ __forceinline__ bool sma_prefix(const float* lds, const Seg& c, doubl
     const double o = __shfl_up(incl, d);
     if (lane >= d) incl += o;
   }
-  const double excl = incl - (cs[0] + cs[1]);
-  if (2 * lane < NCH) pc[2 * lane] = excl;
-  if (2 * lane + 1 < NCH) pc[2 * lane + 1] = excl + cs[0];
-  if (lane == kWave - 1) pc[NCH] = incl;
-  wave_sync();
+  pr.excl = incl - (cs[0] + cs[1]);
+  pr.c0 = cs[0];
+  pr.total = bcast_d(incl, kWave - 1);
   return true;
+}
+
+// pc[k] for this lane's k (0 <= k <= the chunk count).
+__device__ __forceinline__ double sma_pc(const SmaPrefix& pr, int k) {
+  const int src = k >> 1;
+  const double ex = __shfl(pr.excl, src < kWave ? src : 0);
+  const double c0 = __shfl(pr.c0, src < kWave ? src : 0);
+  return src >= kWave ? pr.total : (k & 1) ? ex + c0 : ex;
+}
+
+// The chunk prefixes a window e_lo..e_hi needs, read by every lane of the
+// wave together (the shuffles need all lanes).
+struct SmaEnds {
+  double lo, hi;
+};
+__device__ __forceinline__ SmaEnds sma_ends(const SmaPrefix& pr, int e_lo, int e_hi) {
+  const int k_lo = (e_lo + kLK - 1) / kLK, k_hi = (e_hi + 1) / kLK;
+  return SmaEnds{sma_pc(pr, k_lo), sma_pc(pr, k_hi)};
 }
 
 // Sum of staged bars e_lo..e_hi (inclusive) from the chunk prefix sums and
 // the partial chunks at both ends.
 template <class T>
-__device__ __forceinline__ double sma_window(const float* lds, const double* pc, int e_lo, int e_hi) {
+__device__ __forceinline__ double sma_window(const float* lds, const SmaEnds& pc, int e_lo,
+                                             int e_hi) {
   const int k_lo = (e_lo + kLK - 1) / kLK, k_hi = (e_hi + 1) / kLK;
   double s = 0.0;
   if (k_lo >= k_hi) {
@@ -559,7 +585,7 @@ __device__ __forceinline__ double sma_window(const float* lds, const double* pc,
     return s;
   }
   for (int e = e_lo; e < k_lo * kLK; ++e) s += (double)lds_ld(lds, T::pos(e));
-  s += pc[k_hi] - pc[k_lo];
+  s += pc.hi - pc.lo;
   for (int e = k_hi * kLK; e <= e_hi; ++e) s += (double)lds_ld(lds, T::pos(e));
   return s;
 }
@@ -570,7 +596,6 @@ __global__ __launch_bounds__(kLW* kWave) void k_scan_emit(const float* x, const 
   using T = Tile<kLK, H>;
   if (FM) P.mask = FM;
   __shared__ __align__(16) float s_in[kLW][T::PLEN];
-  __shared__ double s_pc[kLW][T::LEN / kLK + 1];
   const int w = threadIdx.x / kWave;
   const int seg = blockIdx.x * kLW + w;
   if (seg >= P.rows * P.segsPerRow) return;
@@ -634,9 +659,10 @@ __global__ __launch_bounds__(kLW* kWave) void k_scan_emit(const float* x, const 
     }
     if (wS) {
       // slides from the chunk prefix sums (all staged bars are finite here)
-      sma_prefix<T>(lds, c, s_pc[w], false);
+      SmaPrefix pr;
+      sma_prefix<T>(lds, c, pr, false);
       const int e = a - (c.s0 - T::H);
-      double ssum = sma_window<T>(lds, s_pc[w], e - P.ps + 1, e);
+      double ssum = sma_window<T>(lds, sma_ends(pr, e - P.ps + 1, e), e - P.ps + 1, e);
       const int eo = e - P.ps;  // staged index of x[a - ps]
 #pragma unroll
       for (int j = 0; j < kLK; ++j) {
@@ -665,63 +691,108 @@ __global__ __launch_bounds__(kLW* kWave) void k_scan_emit(const float* x, const 
     }
     return;
   }
-  // replay with the families' arithmetic
+  // replay with the families' arithmetic, one family at a time (each
+  // family's state and outputs only are live in its loop)
   const float qn = __builtin_nanf("");
-  KamaWin kw{0.0, -1};
-  // SMA window of the lane: finite-part sum and counts of NaN, +Inf, -Inf
-  // (the counts stay 0 on the all-finite path)
-  const bool sfast = wS && sma_prefix<T>(lds, c, s_pc[w], true);
-  double ssum = 0.0;
-  int snan = 0, spos = 0, sneg = 0, st = -1;
-  auto sadd = [&](float v, int sgn) {
-    if (__builtin_isnan(v)) snan += sgn;
-    else if (__builtin_isinf(v)) (v > 0.0f ? spos : sneg) += sgn;
-    else ssum += sgn > 0 ? (double)v : -(double)v;
-  };
+  if (wE) {
 #pragma unroll
-  for (int h = 0; h < kLK / 8; ++h) {
-    float oe[8], ok[8], ot[8], os[8];
+    for (int h = 0; h < kLK / 8; ++h) {
+      float o[8];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const int t = a + 8 * h + i;
-      const float xf = xv[8 * h + i];
-      const double v = (double)xf;
-      oe[i] = ok[i] = ot[i] = os[i] = qn;
-      if (t >= c.s1) continue;
-      if (wE) {
+      for (int i = 0; i < 8; ++i) {
+        const int t = a + 8 * h + i;
+        const float xf = xv[8 * h + i];
+        o[i] = qn;
+        if (t >= c.s1) continue;
         if (P.pe == 1) {
-          oe[i] = xf;
+          o[i] = xf;
         } else if (t >= S.Le) {
-          ye = (t == S.Le) ? S.se : fma(P.eb, ye, P.ek * v);
-          oe[i] = (float)ye;
+          ye = (t == S.Le) ? S.se : fma(P.eb, ye, P.ek * (double)xf);
+          o[i] = (float)ye;
         }
       }
-      if (wT && t >= S.L6) {
+      if (a + 8 * h < c.s1) put8(out.ema, c, a + 8 * h, o);
+    }
+  }
+  if (wT) {
+#pragma unroll
+    for (int h = 0; h < kLK / 8; ++h) {
+      float o[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int t = a + 8 * h + i;
+        o[i] = qn;
+        if (t >= c.s1 || t < S.L6) continue;
         if (t == S.L6) {
 #pragma unroll
           for (int q = 0; q < 6; ++q) yt[q] = S.st[q];
         } else {
-          yt[0] = fma(P.tomk, yt[0], P.tk * v);
+          yt[0] = fma(P.tomk, yt[0], P.tk * (double)xv[8 * h + i]);
 #pragma unroll
           for (int q = 1; q < 6; ++q) yt[q] = fma(P.tomk, yt[q], P.tk * yt[q - 1]);
         }
-        ot[i] = (float)fma(P.c4, yt[2], fma(P.c3, yt[3], fma(P.c1, yt[5], P.c2 * yt[4])));
+        o[i] = (float)fma(P.c4, yt[2], fma(P.c3, yt[3], fma(P.c1, yt[5], P.c2 * yt[4])));
       }
-      if (wK) {
+      if (a + 8 * h < c.s1) put8(out.t3, c, a + 8 * h, o);
+    }
+  }
+  if (wK) {
+    KamaWin kw{0.0, -1};
+#pragma unroll
+    for (int h = 0; h < kLK / 8; ++h) {
+      float o[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int t = a + 8 * h + i;
+        const float xf = xv[8 * h + i];
+        const double v = (double)xf;
+        o[i] = qn;
+        if (t >= c.s1) continue;
         if (P.pk == 1) {
-          ok[i] = xf;
+          o[i] = xf;
         } else if (t == S.Lk) {
           yk = v;
         } else if (t > S.Lk) {
           const double sc = kama_sc<T>(lds, x, c, P, kw, t, v);
           yk = fma(v - yk, sc, yk);
-          ok[i] = (float)yk;
+          o[i] = (float)yk;
         }
       }
-      if (wS && t >= P.ps - 1) {
+      if (a + 8 * h < c.s1) put8(out.kama, c, a + 8 * h, o);
+    }
+  }
+  if (wS) {
+    // SMA window of the lane: finite-part sum and counts of NaN, +Inf, -Inf
+    // (the counts stay 0 on the all-finite path)
+    SmaPrefix spr;
+    const bool sfast = sma_prefix<T>(lds, c, spr, true);
+    // the prefixes of the lane's first window (its first bar at or past the
+    // SMA warm-up), read before the lanes' bar loops diverge
+    SmaEnds sends{0.0, 0.0};
+    if (sfast) {
+      const int tf = max(a, P.ps - 1);
+      const int ef = tf - (c.s0 - T::H);
+      sends = sma_ends(spr, max(ef - P.ps + 1, 0), max(ef, 0));
+    }
+    double ssum = 0.0;
+    int snan = 0, spos = 0, sneg = 0, st = -1;
+    auto sadd = [&](float v, int sgn) {
+      if (__builtin_isnan(v)) snan += sgn;
+      else if (__builtin_isinf(v)) (v > 0.0f ? spos : sneg) += sgn;
+      else ssum += sgn > 0 ? (double)v : -(double)v;
+    };
+#pragma unroll
+    for (int h = 0; h < kLK / 8; ++h) {
+      float o[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int t = a + 8 * h + i;
+        const float xf = xv[8 * h + i];
+        o[i] = qn;
+        if (t >= c.s1 || t < P.ps - 1) continue;
         const int e = t - (c.s0 - T::H);
         if (sfast) {
-          ssum = st < 0 ? sma_window<T>(lds, s_pc[w], e - P.ps + 1, e)
+          ssum = st < 0 ? sma_window<T>(lds, sends, e - P.ps + 1, e)
                         : ssum + ((double)xf - (double)lds_ld(lds, T::pos(e - P.ps)));
         } else if (st < 0) {
           for (int j = t - P.ps + 1; j <= t; ++j) sadd(xat<T>(lds, x, c, j), 1);
@@ -735,15 +806,9 @@ __global__ __launch_bounds__(kLW* kWave) void k_scan_emit(const float* x, const 
         else if (spos > 0) r = __builtin_inff();
         else if (sneg > 0) r = -__builtin_inff();
         else r = (float)(ssum / (double)P.ps);
-        os[i] = r;
+        o[i] = r;
       }
-    }
-    const int t0 = a + 8 * h;
-    if (t0 < c.s1) {
-      if (wE) put8(out.ema, c, t0, oe);
-      if (wK) put8(out.kama, c, t0, ok);
-      if (wT) put8(out.t3, c, t0, ot);
-      if (wS) put8(out.sma, c, t0, os);
+      if (a + 8 * h < c.s1) put8(out.sma, c, a + 8 * h, o);
     }
   }
 }
