@@ -1159,6 +1159,49 @@ __device__ __forceinline__ double window_var(const float* li, const float* x, co
   return var;
 }
 
+// window_var of the lane's bars a+i for i in [i0, i0+IG): the IG windows are
+// evaluated side by side (independent accumulator chains, each window in
+// the same order as window_var).  Every bar of the windows is staged.
+template <class T, int IG>
+__device__ __forceinline__ void window_var_group(const float* li, const Seg& c, int a, int i0, int p,
+                                                 double (&var)[IG], double (&mean)[IG]) {
+  const int e0 = a + i0 - p + 1 - (c.s0 - T::H);  // staged index of the first window's oldest bar
+  const double inv = 1.0 / (double)p;
+  double shift[IG], s1[IG], s2[IG], sm[IG];
+#pragma unroll
+  for (int u = 0; u < IG; ++u) {
+    shift[u] = (double)lds_ld(li, T::pos(e0 + u));
+    s1[u] = s2[u] = sm[u] = 0.0;
+  }
+  for (int k = 0; k < p; ++k) {
+#pragma unroll
+    for (int u = 0; u < IG; ++u) {
+      const double v = (double)lds_ld(li, T::pos(e0 + u + k));
+      const double d = v - shift[u];
+      sm[u] += v;
+      s1[u] += d;
+      s2[u] += d * d;
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < IG; ++u) {
+    const double m = s1[u] * inv;
+    const double e2 = s2[u] * inv;
+    double v = e2 - m * m;
+    if (v < 1e-12 * e2) v = 0.0;
+    if (!(v >= 0.0) && !isnan(v)) v = 0.0;
+    var[u] = v;
+    mean[u] = sm[u] / p;
+  }
+}
+
+// Whether every window of the lane's bars lies inside the staged range and
+// the row (the side-by-side path), else the per-bar path.
+template <class T>
+__device__ __forceinline__ bool windows_staged(const Seg& c, int a, int p) {
+  return a - p + 1 >= c.s0 - T::H && a - p + 1 >= 0 && a + T::K <= c.n;
+}
+
 template <int MODE>  // 0 = VAR, 1 = STDDEV
 struct FVarT {
   static constexpr bool kScan = false;
@@ -1174,15 +1217,23 @@ struct FVarT {
     wave_sync();
     float ov[T::K];
     const int a = c.s0 + lane_id() * T::K;
+    constexpr int IG = 4;
+    static_assert(T::K % IG == 0, "bar groups");
+    const bool grouped = windows_staged<T>(c, a, pr.p);
 #pragma unroll
-    for (int i = 0; i < T::K; ++i) {
-      const int t = a + i;
-      if (t < pr.p - 1 || t >= c.n) {
-        ov[i] = qnanf_();
-        continue;
+    for (int i0 = 0; i0 < T::K; i0 += IG) {
+      double gv[IG] = {}, gm[IG] = {};
+      if (grouped) window_var_group<T, IG>(li, c, a, i0, pr.p, gv, gm);
+#pragma unroll
+      for (int u = 0; u < IG; ++u) {
+        const int i = i0 + u, t = a + i;
+        if (t < pr.p - 1 || t >= c.n) {
+          ov[i] = qnanf_();
+          continue;
+        }
+        const double v = grouped ? gv[u] : window_var<T>(li, x, c, t, pr.p, nullptr);
+        ov[i] = MODE == 0 ? (float)v : (float)sqrt(v);
       }
-      const double v = window_var<T>(li, x, c, t, pr.p, nullptr);
-      ov[i] = MODE == 0 ? (float)v : (float)sqrt(v);
     }
     store_lane<T>(lo, ov, io.out[0], c);
   }
@@ -1206,15 +1257,22 @@ struct FBbands {
     wave_sync();
     float ou[T::K], om[T::K], ol[T::K];
     const int a = c.s0 + lane_id() * T::K;
+    constexpr int IG = 4;
+    static_assert(T::K % IG == 0, "bar groups");
+    const bool grouped = windows_staged<T>(c, a, pr.p);
 #pragma unroll
-    for (int i = 0; i < T::K; ++i) {
-      const int t = a + i;
+    for (int i0 = 0; i0 < T::K; i0 += IG) {
+      double gv[IG] = {}, gm[IG] = {};
+      if (grouped) window_var_group<T, IG>(li, c, a, i0, pr.p, gv, gm);
+#pragma unroll
+    for (int u = 0; u < IG; ++u) {
+      const int i = i0 + u, t = a + i;
       if (t < pr.p - 1 || t >= c.n) {
         ou[i] = om[i] = ol[i] = qnanf_();
         continue;
       }
-      double m;
-      const double var = window_var<T>(li, x, c, t, pr.p, &m);
+      double m = gm[u];
+      const double var = grouped ? gv[u] : window_var<T>(li, x, c, t, pr.p, &m);
       const double sd = (var != 0.0) ? sqrt(var) : 0.0;
       om[i] = (float)m;
       if (pr.up == pr.dn) {
@@ -1225,6 +1283,7 @@ struct FBbands {
         ou[i] = (float)fma(sd, pr.up, m);
         ol[i] = (float)(m - sd * pr.dn);
       }
+    }
     }
     store_lane<T>(lo, ou, io.out[0], c);
     store_lane<T>(lo, om, io.out[1], c);
