@@ -1195,11 +1195,12 @@ __device__ __forceinline__ void window_var_group(const float* li, const Seg& c, 
   }
 }
 
-// Whether every window of the lane's bars lies inside the staged range and
-// the row (the side-by-side path), else the per-bar path.
+// Whether every window the segment outputs lies inside the staged range:
+// bars before the row (staged as NaN) only enter windows whose output is
+// NaN anyway, and bars past the row are never output.
 template <class T>
-__device__ __forceinline__ bool windows_staged(const Seg& c, int a, int p) {
-  return a - p + 1 >= c.s0 - T::H && a - p + 1 >= 0 && a + T::K <= c.n;
+__device__ __forceinline__ bool windows_staged(int p) {
+  return p - 1 <= T::H;
 }
 
 template <int MODE>  // 0 = VAR, 1 = STDDEV
@@ -1219,7 +1220,7 @@ struct FVarT {
     const int a = c.s0 + lane_id() * T::K;
     constexpr int IG = 4;
     static_assert(T::K % IG == 0, "bar groups");
-    const bool grouped = windows_staged<T>(c, a, pr.p);
+    const bool grouped = windows_staged<T>(pr.p);
 #pragma unroll
     for (int i0 = 0; i0 < T::K; i0 += IG) {
       double gv[IG] = {}, gm[IG] = {};
@@ -1259,7 +1260,7 @@ struct FBbands {
     const int a = c.s0 + lane_id() * T::K;
     constexpr int IG = 4;
     static_assert(T::K % IG == 0, "bar groups");
-    const bool grouped = windows_staged<T>(c, a, pr.p);
+    const bool grouped = windows_staged<T>(pr.p);
 #pragma unroll
     for (int i0 = 0; i0 < T::K; i0 += IG) {
       double gv[IG] = {}, gm[IG] = {};
@@ -1408,6 +1409,48 @@ struct FCci {
     float ov[T::K];
     const int a = c.s0 + lane_id() * T::K;
     const int p = pr.p;
+    if (windows_staged<T>(p)) {
+      // every window is staged: the typical price of each staged bar once,
+      // in doubles over the high/low area (2 PLEN floats hold LEN doubles)
+      constexpr int NE = (T::LEN + kWave - 1) / kWave;
+      double tv[NE];
+#pragma unroll
+      for (int q = 0; q < NE; ++q) {
+        const int e = lane_id() + q * kWave;
+        tv[q] = 0.0;
+        if (e < T::LEN)
+          tv[q] = ((double)lds_ld(lh, T::pos(e)) + (double)lds_ld(ll, T::pos(e)) +
+                   (double)lds_ld(lc, T::pos(e))) /
+                  3.0;
+      }
+      wave_sync();
+      double* tpd = reinterpret_cast<double*>(lh);
+#pragma unroll
+      for (int q = 0; q < NE; ++q) {
+        const int e = lane_id() + q * kWave;
+        if (e < T::LEN) tpd[e] = tv[q];
+      }
+      wave_sync();
+      const int e0 = a - (c.s0 - T::H);  // staged index of bar a
+#pragma unroll
+      for (int i = 0; i < T::K; ++i) {
+        const int t = a + i, e = e0 + i;
+        if (t < p - 1 || t >= c.n) {
+          ov[i] = qnanf_();
+          continue;
+        }
+        double s = 0.0;
+        for (int j = 0; j < p; ++j) s += tpd[e - j];
+        const double sma = s / p;
+        double dev = 0.0;
+        for (int j = 0; j < p; ++j) dev += fabs(tpd[e - j] - sma);
+        const double md = dev / p;
+        const double num = tpd[e] - sma;
+        ov[i] = (num != 0.0 && md != 0.0) ? (float)(num / (0.015 * md)) : 0.0f;
+      }
+      store_lane<T>(lo, ov, io.out[0], c);
+      return;
+    }
 #pragma unroll
     for (int i = 0; i < T::K; ++i) {
       const int t = a + i;
