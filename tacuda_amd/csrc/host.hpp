@@ -130,6 +130,20 @@ ctStatus_t scan_run(const float* close, int seriesLen, int numSeries, const ctxS
                     const ctxScanOutputs* outputs, hipStream_t stream);
 bool scan_serves(int len, int rows, int windowLookback);
 
+// ------------------------------------------------------ fused pass --
+// The family calls of RSI, MACD and ATR on dense batches of at least
+// kPipeMinRows long rows, kPipeMinSegments 512-bar segments in all, run the
+// fused pass's two streaming passes (the pack's outputs for that family,
+// the same oracle-following arithmetic): *served says whether they did;
+// when not, nothing was launched and the caller takes its own path.  A few
+// long rows or a small batch stay on the one-launch look-back kernels (the
+// passes' five launches are latency-bound there).
+constexpr int kPipeMinRows = 16;
+constexpr long long kPipeMinSegments = 512;
+ctStatus_t pipe_family(const ctxPackParams& params, const ctxPackOutputs& outputs,
+                       const float* h, const float* l, const float* c, int len, int rows,
+                       hipStream_t stream, bool* served);
+
 // Synchronize when the caller used the legacy default stream: batch calls
 // on stream 0 return once the batch is done (reference api_batch.cu:377-386).
 inline ctStatus_t finish_batch(hipStream_t stream) {

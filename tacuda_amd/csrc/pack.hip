@@ -2306,6 +2306,28 @@ ctStatus_t pack_validate(const ctxPackParams* p, const ctxPackOutputs* o, PackP&
 
 }  // namespace
 
+namespace tca {
+
+ctStatus_t pipe_family(const ctxPackParams& params, const ctxPackOutputs& outputs,
+                       const float* h, const float* l, const float* c, int len, int rows,
+                       hipStream_t stream, bool* served) {
+  *served = false;
+  if (t_ragged || rows < kPipeMinRows || len < kPMinLen ||
+      (long long)rows * div_up(len, kPSeg) < kPipeMinSegments)
+    return CT_STATUS_SUCCESS;
+  PackP P;
+  PackIO io;
+  if (pack_validate(&params, &outputs, P, io, h, l, c) != CT_STATUS_SUCCESS) {
+    clear_error();  // the family path reports its own errors
+    return CT_STATUS_SUCCESS;
+  }
+  if (!pipe_serves(len, P)) return CT_STATUS_SUCCESS;
+  *served = true;
+  return launch_pipe(len, rows, stream, P, io);
+}
+
+}  // namespace tca
+
 extern "C" {
 
 CTAPI_EXPORT ctStatus_t ctx_pack_batch_d(const float* h, const float* l, const float* c,

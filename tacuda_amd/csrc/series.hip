@@ -189,6 +189,23 @@ ctStatus_t via_scan(const float* x, int len, int rows, hipStream_t s, ctxScanOut
                     ctxScanParams sp) {
   return scan_run(x, len, rows, &sp, &so, s);
 }
+// Dense batches of long rows of RSI, MACD and ATR likewise run through the
+// fused pass's two streaming passes (pipe_family), with every other pack
+// period at a valid placeholder.
+ctxPackParams pipe_params() {
+  ctxPackParams p{};
+  p.sma_period = p.ema_period = p.wma_period = p.dema_period = p.tema_period = 14;
+  p.rsi_period = p.atr_period = p.adx_period = p.willr_period = p.cci_period = 14;
+  p.macd_fast = 12;
+  p.macd_slow = 26;
+  p.macd_signal = 9;
+  p.bb_period = 20;
+  p.bb_up = p.bb_dn = 2.0f;
+  p.stoch_k = 14;
+  p.stoch_d = 3;
+  return p;
+}
+
 ctxScanParams scan_params() {
   ctxScanParams sp{};
   sp.ema_period = 2;
@@ -285,6 +302,15 @@ ctStatus_t dev_rsi(const float* x, float* y, int len, int rows, int p, hipStream
   CHECK_PTR(x, "rsi"); CHECK_PTR(y, "rsi");
   CHECK_RET(check_shape("rsi", len, rows));
   if (p <= 0) return bad("rsi", "invalid period");
+  if (rows >= kPipeMinRows) {
+    ctxPackParams pp = pipe_params();
+    pp.rsi_period = p;
+    ctxPackOutputs po{};
+    po.rsi = y;
+    bool served = false;
+    CHECK_RET(pipe_family(pp, po, x, x, x, len, rows, s, &served));
+    if (served) return CT_STATUS_SUCCESS;
+  }
   return run_Rsi(len, rows, s, {p}, io1(x, y));
 }
 ctStatus_t dev_macd(const float* x, float* m, float* sg, float* h, int len, int rows, int fast,
@@ -295,6 +321,19 @@ ctStatus_t dev_macd(const float* x, float* m, float* sg, float* h, int len, int 
   if (maType != CT_MA_EMA && maType != CT_MA_DEFAULT)
     return fail(CT_STATUS_UNSUPPORTED, "macd: only the EMA moving-average type is provided");
   if (slow < fast) std::swap(slow, fast);
+  if (rows >= kPipeMinRows) {
+    ctxPackParams pp = pipe_params();
+    pp.macd_fast = fast;
+    pp.macd_slow = slow;
+    pp.macd_signal = sig;
+    ctxPackOutputs po{};
+    po.macd = m;
+    po.macd_signal = sg;
+    po.macd_hist = h;
+    bool served = false;
+    CHECK_RET(pipe_family(pp, po, x, x, x, len, rows, s, &served));
+    if (served) return CT_STATUS_SUCCESS;
+  }
   return run_Macd(len, rows, s, {fast, slow, sig}, io1(x, m, sg, h));
 }
 ctStatus_t dev_bbands(const float* x, float* u, float* m, float* l, int len, int rows, int p,
@@ -333,6 +372,15 @@ ctStatus_t dev_atr(const float* h, const float* l, const float* c, float* y, int
   CHECK_PTR(h, "atr"); CHECK_PTR(l, "atr"); CHECK_PTR(c, "atr"); CHECK_PTR(y, "atr");
   CHECK_RET(check_shape("atr", len, rows));
   if (p <= 0) return bad("atr", "invalid period");
+  if (rows >= kPipeMinRows) {
+    ctxPackParams pp = pipe_params();
+    pp.atr_period = p;
+    ctxPackOutputs po{};
+    po.atr = y;
+    bool served = false;
+    CHECK_RET(pipe_family(pp, po, h, l, c, len, rows, s, &served));
+    if (served) return CT_STATUS_SUCCESS;
+  }
   return run_Atr(len, rows, s, {p}, io3(h, l, c, y));
 }
 ctStatus_t dev_natr(const float* h, const float* l, const float* c, float* y, int len, int rows,

@@ -212,3 +212,45 @@ def test_pack_two_pass_non_finite(dev, case):
           "hole": dict(holes=(10000,)), "inf": dict(infs=(12000,)),
           "edge": dict(holes=(511, 512, 4095))}[case]
     _run(dev, _poison(d, **kw), DEFAULT, outputs=PIPE_OUT)
+
+
+@pytest.mark.parametrize("case", ["plain", "prefix", "hole", "period1"])
+@pytest.mark.parametrize("shape", [(16, 16500), (20, 30011)])
+def test_family_batches_routed_through_two_passes(dev, case, shape):
+    """ct_rsi_batch_d / ct_macd_batch_d / ct_atr_batch_d on dense batches of
+    at least 16 long rows and 512 segments take the fused pass's two passes
+    (period 1 keeps the look-back kernels): the oracle's results either
+    way."""
+    import tacuda_amd as ta
+    from oracle import ORC
+
+    rows, n = shape
+    print(f"routed families {case} {rows}x{n}", flush=True)
+    d = _data(n, rows=rows, seed=59)
+    if case == "prefix":
+        d = _poison(d, prefix=600)
+    elif case == "hole":
+        d = _poison(d, holes=(n // 2,))
+    p = 1 if case == "period1" else 14
+    h, l, c = (torch.from_numpy(np.ascontiguousarray(d[k])).to(dev) for k in ("high", "low", "close"))
+    torch.cuda.synchronize()
+    ta.kernel_times()
+    ta.kernel_timing(True)
+    try:
+        rsi = ta.ops.rsi(c, p)
+        atr = ta.ops.atr(h, l, c, p)
+        m = ta.ops.macd(c, 12, 26, 9)
+    finally:
+        ta.kernel_timing(False)
+    torch.cuda.synchronize()
+    folds = sum(1 for k, _ in ta.kernel_times() if k == "k_pipe_fold")
+    assert folds == (1 if case == "period1" else 3), folds  # MACD's periods are >= 2
+    H, L, C = d["high"], d["low"], d["close"]
+    for r in range(rows):
+        compare(rsi[r].cpu().numpy(), ORC.rsi(C[r], p), tol=1e-5, min_exact=0.999,
+                name=f"rsi row {r}")
+        compare(atr[r].cpu().numpy(), ORC.atr(H[r], L[r], C[r], p), tol=1e-5, min_exact=0.999,
+                name=f"atr row {r}")
+        w = ORC.macd(C[r], 12, 26, 9)
+        for i in range(3):
+            compare(m[i][r].cpu().numpy(), w[i], tol=1e-5, min_exact=0.999, name=f"macd{i} row {r}")
