@@ -264,7 +264,10 @@ def _scan256(a, torch, ops, T, dev):
     import tacuda_amd as ta
 
     rows, n = 256, 1 << 20
-    c = torch.from_numpy(ta.synth_ohlcv(n, rows=rows, seed=0)["close"]).to(dev)
+    dd = ta.synth_ohlcv(n, rows=rows, seed=0)
+    c = torch.from_numpy(dd["close"]).to(dev)
+    hl = [torch.from_numpy(dd[k]).to(dev) for k in ("high", "low")]
+    del dd
     bars = rows * n
     alg = (1 + 4) * 4 * bars
     g = T.graphs([c], lambda x: ops.scan(x))
@@ -294,10 +297,26 @@ def _scan256(a, torch, ops, T, dev):
         T.keep.clear()
         torch.cuda.empty_cache()
     print(f"scan256 drop-in 4 calls: {total:.3f} ms/pass", file=sys.stderr, flush=True)
+    # the Wilder/MACD family calls on the same rows (dense batches of long
+    # rows take the fused pass's two passes)
+    fam = {"rsi14": (1, 1, lambda x: ops.rsi(x, 14)),
+           "macd12_26_9": (1, 3, lambda x: ops.macd(x, 12, 26, 9)),
+           "atr14": (3, 1, lambda x: ops.atr(hl[0], hl[1], x, 14))}
+    famr = {}
+    for name, (nin, nout, f) in fam.items():
+        g = T.graphs([c], f)
+        kms = T.time(g, max(5, a.steps // 10), 2)
+        famr[name] = {"ms": kms, "GBs": _gbs((nin + nout) * 4 * bars, kms)}
+        T.keep.clear()
+        torch.cuda.empty_cache()
+    print("scan256 family calls: " + ", ".join(f"{k} {v['ms']:.3f} ms" for k, v in famr.items()),
+          file=sys.stderr, flush=True)
+    del hl
     return {"ms": fused, "bars_per_s": bars / (fused * 1e-3), "alg_bytes": alg,
             "GBs": _gbs(alg, fused), "frac_hbm": _gbs(alg, fused) / PEAK_HBM_GBS,
             "shape": [rows, n], "graph": True, "kernels": kern, "drop_in_ms": total,
-            "drop_in_frac_hbm": _gbs(alg, total) / PEAK_HBM_GBS, "drop_in_kernels": per}
+            "drop_in_frac_hbm": _gbs(alg, total) / PEAK_HBM_GBS, "drop_in_kernels": per,
+            "family_calls": famr}
 
 
 def _scan1024(a, torch, ops, T, dev):
