@@ -1215,6 +1215,9 @@ constexpr int kPK = 8;               // bars per lane
 constexpr int kPSeg = kPK * kWave;   // bars per segment
 constexpr int kPW = 4;               // waves per workgroup
 constexpr int kPMinLen = 16 * kPSeg; // shortest row taking the two passes
+// Passes whose segments x groups-per-row stay within this many scan the
+// row's group composites inside the emit launch (no k_pipe_rows launch).
+constexpr long long kPipeRowsInEmit = 1ll << 17;
 constexpr unsigned kPUnsupported = (1u << kODema) | (1u << kOTema) | (1u << kOAdx);
 constexpr int kPWinIL = 1;  // window chunks side by side in the fold launch's window waves
 
@@ -1777,18 +1780,21 @@ __device__ __forceinline__ void pc_store(double* g, const PC& p) {
   for (int q = 0; q < 16; ++q) g[q] = v[q];
 }
 
-__device__ __forceinline__ PC pc_lane63(const PC& p) {
+// Lane j's composite in every lane (j uniform over the wave).
+__device__ __forceinline__ PC pc_lane(const PC& p, int j) {
   PC r;
-  r.Ae = bcast_d(p.Ae, 63); r.be = bcast_d(p.be, 63);
-  r.Ag = bcast_d(p.Ag, 63); r.bg = bcast_d(p.bg, 63);
-  r.Al = bcast_d(p.Al, 63); r.bl = bcast_d(p.bl, 63);
-  r.Af = bcast_d(p.Af, 63); r.bf = bcast_d(p.bf, 63);
-  r.As = bcast_d(p.As, 63); r.bs = bcast_d(p.bs, 63);
-  r.agf = bcast_d(p.agf, 63); r.ags = bcast_d(p.ags, 63);
-  r.agg = bcast_d(p.agg, 63); r.bgg = bcast_d(p.bgg, 63);
-  r.Aa = bcast_d(p.Aa, 63); r.ba = bcast_d(p.ba, 63);
+  r.Ae = bcast_d(p.Ae, j); r.be = bcast_d(p.be, j);
+  r.Ag = bcast_d(p.Ag, j); r.bg = bcast_d(p.bg, j);
+  r.Al = bcast_d(p.Al, j); r.bl = bcast_d(p.bl, j);
+  r.Af = bcast_d(p.Af, j); r.bf = bcast_d(p.bf, j);
+  r.As = bcast_d(p.As, j); r.bs = bcast_d(p.bs, j);
+  r.agf = bcast_d(p.agf, j); r.ags = bcast_d(p.ags, j);
+  r.agg = bcast_d(p.agg, j); r.bgg = bcast_d(p.bgg, j);
+  r.Aa = bcast_d(p.Aa, j); r.ba = bcast_d(p.ba, j);
   return r;
 }
+
+__device__ __forceinline__ PC pc_lane63(const PC& p) { return pc_lane(p, kWave - 1); }
 
 
 // One wave per group of 64 consecutive segments of a row: every segment's
@@ -1831,6 +1837,27 @@ __global__ __launch_bounds__(kPW* kWave) void k_pipe_rows(PipeP P, const double*
       double* o = genter + k * 7;
       o[0] = v.e; o[1] = v.G; o[2] = v.L; o[3] = v.f; o[4] = v.s; o[5] = v.g; o[6] = v.a;
     }
+    y = pc_apply(pc_lane63(F), y);
+  }
+}
+
+// The state entering group g of a row, for one recurrence group, computed
+// by the emit wave itself from the row's group composites: the lane scans
+// and compositions of k_pipe_rows restricted to the groups before g, so the
+// same doubles come out.  Used where the row scans are short (few segments
+// in all), which saves the k_pipe_rows launch.
+template <unsigned GM>
+__device__ __forceinline__ PState pipe_group_enter(const PipeP& P, const double* gtot, int row,
+                                                   int g) {
+  const int G = pipe_groups(P);
+  PState y{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int g0 = 0;; g0 += kWave) {
+    const int j = g0 + lane_id();
+    PC F;
+    if (j < g) F = pc_load(gtot + ((size_t)row * G + j) * 16);
+    else pc_id(F);
+    pc_scan<GM>(F);
+    if (g < g0 + kWave) return pc_apply(pc_lane(pc_prev_lane(F), g - g0), y);
     y = pc_apply(pc_lane63(F), y);
   }
 }
@@ -2005,25 +2032,31 @@ __global__ __launch_bounds__(kPW* kWave) void k_pipe_fold(PipeP P, PackP prm, Pa
 template <unsigned GM>
 __device__ __forceinline__ void pipe_emit_wave(const PipeP& P, const PLds& L, const PackIO& io,
                                                const Seg& c, int row, int seg, const int* lead,
-                                               const double* agg, const double* genter) {
+                                               const double* agg, const double* gtot,
+                                               const double* genter) {
   if (!pipe_wants<GM>(P)) return;
   using T = PTile;
   const int a = c.s0 + lane_id() * kPK;
   PBars B;
   pipe_bars<T>(L.c, L.h, L.l, io, c, P, a, B);
   const PSeeds S = pipe_seeds<T, GM>(L.c, L.h, L.l, io, c, P, lead[row]);
-  // state entering the segment: its group's entering state, then the
-  // group's segments before it
-  const double* g = genter + ((size_t)row * pipe_groups(P) + c.seg / kWave) * 7;
-  const PState y0 = pc_apply(pc_load(agg + (size_t)seg * 16),
-                             PState{g[0], g[1], g[2], g[3], g[4], g[5], g[6]});
+  // state entering the segment: its group's entering state (from k_pipe_rows,
+  // or scanned here when genter is null), then the group's segments before it
+  PState ge;
+  if (genter) {
+    const double* g = genter + ((size_t)row * pipe_groups(P) + c.seg / kWave) * 7;
+    ge = PState{g[0], g[1], g[2], g[3], g[4], g[5], g[6]};
+  } else {
+    ge = pipe_group_enter<GM>(P, gtot, row, c.seg / kWave);
+  }
+  const PState y0 = pc_apply(pc_load(agg + (size_t)seg * 16), ge);
   pipe_emit_group<GM>(P, S, B, c, a, pipe_steady<GM>(c, P, S), y0, io);
 }
 
 // One workgroup per segment, one wave per recurrence group: the state
 // entering the segment, then the replay of its bars.
 __global__ __launch_bounds__(kPW* kWave) void k_pipe_emit(PipeP P, PackIO io, const int* lead,
-                                                          const double* agg,
+                                                          const double* agg, const double* gtot,
                                                           const double* genter) {
   __shared__ __align__(16) PLds L;
   const int seg = blockIdx.x, w = threadIdx.x / kWave;
@@ -2031,10 +2064,10 @@ __global__ __launch_bounds__(kPW* kWave) void k_pipe_emit(PipeP P, PackIO io, co
   int row;
   const Seg c = pipe_seg(P, seg, row);
   pipe_stage_block<PTile>(L, io, c, P, w);
-  if (w == 0) pipe_emit_wave<kGE>(P, L, io, c, row, seg, lead, agg, genter);
-  else if (w == 1) pipe_emit_wave<kGR>(P, L, io, c, row, seg, lead, agg, genter);
-  else if (w == 2) pipe_emit_wave<kGM>(P, L, io, c, row, seg, lead, agg, genter);
-  else pipe_emit_wave<kGA>(P, L, io, c, row, seg, lead, agg, genter);
+  if (w == 0) pipe_emit_wave<kGE>(P, L, io, c, row, seg, lead, agg, gtot, genter);
+  else if (w == 1) pipe_emit_wave<kGR>(P, L, io, c, row, seg, lead, agg, gtot, genter);
+  else if (w == 2) pipe_emit_wave<kGM>(P, L, io, c, row, seg, lead, agg, gtot, genter);
+  else pipe_emit_wave<kGA>(P, L, io, c, row, seg, lead, agg, gtot, genter);
 }
 
 // Whether the two passes serve this pass: dense rows of at least kPMinLen
@@ -2158,7 +2191,11 @@ ctStatus_t launch_pipe(int len, int rows, hipStream_t s, const PackP& prm, const
                        gtot);
   }
   TCA_HIP(hipGetLastError());
-  {
+  // Few segments in all: each emit wave scans its row's group composites
+  // itself (at most segs * groups-per-row composites read, from L2) instead
+  // of a k_pipe_rows launch.
+  const bool rowsInEmit = segs * div_up(P.segsPerRow, kWave) <= kPipeRowsInEmit;
+  if (!rowsInEmit) {
     KTime kt(s, "k_pipe_rows");
     hipLaunchKernelGGL(k_pipe_rows, dim3(div_up(rows, kPW)), dim3(kPW * kWave), 0, s, P, gtot,
                        genter);
@@ -2166,8 +2203,8 @@ ctStatus_t launch_pipe(int len, int rows, hipStream_t s, const PackP& prm, const
   TCA_HIP(hipGetLastError());
   {
     KTime kt(s, "k_pipe_emit");
-    hipLaunchKernelGGL(k_pipe_emit, dim3(segs), dim3(kPW * kWave), 0, s, P, io, lead, agg,
-                       genter);
+    hipLaunchKernelGGL(k_pipe_emit, dim3(segs), dim3(kPW * kWave), 0, s, P, io, lead, agg, gtot,
+                       rowsInEmit ? nullptr : genter);
   }
   TCA_HIP(hipGetLastError());
   return CT_STATUS_SUCCESS;

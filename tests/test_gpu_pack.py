@@ -189,6 +189,24 @@ def test_pack_two_pass_rows(dev, shape):
     _run(dev, _data(n, rows=rows, seed=37), DEFAULT, outputs=PIPE_OUT)
 
 
+def test_pack_two_pass_row_carry_launch(dev):
+    """Many segments (8 rows of 2^20 + 3 bars, 33 groups of 64 segments per
+    row): the state entering each group comes from the k_pipe_rows launch
+    instead of the emit waves' own scans."""
+    import tacuda_amd as ta
+
+    print("pack two-pass 8x1048579 (row carry launch)", flush=True)
+    d = _data((1 << 20) + 3, rows=8, seed=39)
+    ta.kernel_times()
+    ta.kernel_timing(True)
+    try:
+        _run(dev, d, DEFAULT, outputs=PIPE_REC)
+    finally:
+        ta.kernel_timing(False)
+    names = [k for k, _ in ta.kernel_times()]
+    assert names == ["k_pipe_lead", "k_pipe_fold", "k_pipe_group", "k_pipe_rows", "k_pipe_emit"]
+
+
 @pytest.mark.parametrize("pname", ["long", "ema2"])
 def test_pack_two_pass_periods(dev, pname):
     params = {"long": LONG, "ema2": dict(DEFAULT, ema_period=2, rsi_period=2, atr_period=2,

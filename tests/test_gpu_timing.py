@@ -34,7 +34,9 @@ def test_two_pass_kernels_are_timed(dev):
         ta.kernel_timing(False)
     rec = ta.kernel_times()
     names = [k for k, _ in rec]
-    one = ["k_pipe_lead", "k_pipe_fold", "k_pipe_group", "k_pipe_rows", "k_pipe_emit"]
+    # one row of 40 segments: the emit waves scan the row's group composites
+    # themselves, so there is no k_pipe_rows launch
+    one = ["k_pipe_lead", "k_pipe_fold", "k_pipe_group", "k_pipe_emit"]
     assert names == one + one
     assert all(ms > 0.0 for _, ms in rec)
     ta.ops.pack(h, l, c, outputs=outs)
