@@ -216,7 +216,9 @@ def test_pack_two_pass_periods(dev, pname):
 
 
 @pytest.mark.parametrize("subset", [("ema",), ("rsi",), ("macd_signal",), ("atr",),
-                                    ("ema", "atr", "sma"), ("rsi", "macd_hist", "cci")])
+                                    ("ema", "atr", "sma"), ("rsi", "macd_hist", "cci"),
+                                    ("sma", "bb_upper", "bb_lower", "stoch_k", "ema"),
+                                    ("bb_middle", "cci", "rsi"), ("bb_lower", "wma", "atr")])
 def test_pack_two_pass_subsets(dev, subset):
     print(f"pack two-pass subset {subset}", flush=True)
     _run(dev, _data(17000, rows=2, seed=43), DEFAULT, outputs=subset)
