@@ -865,8 +865,14 @@ ctStatus_t launch_scan(const float* x, int* lead, double* agg, double* carry, co
   if (P.useCarry) {
     {
       KTime kt(s, "k_scan_fold");
-      hipLaunchKernelGGL((k_scan_fold<H, FM>), dim3(blocks), dim3(kLW * kWave), 0, s, x, lead, P,
-                         agg);
+      // the fold's only look-back is KAMA's window (pk + 1 bars), not SMA's:
+      // a 32-bar halo stages 1056 bars per segment instead of 1280
+      if (H > 32 && P.pk + 1 <= 32)
+        hipLaunchKernelGGL((k_scan_fold<32, FM>), dim3(blocks), dim3(kLW * kWave), 0, s, x, lead,
+                           P, agg);
+      else
+        hipLaunchKernelGGL((k_scan_fold<H, FM>), dim3(blocks), dim3(kLW * kWave), 0, s, x, lead,
+                           P, agg);
     }
     TCA_HIP(hipGetLastError());
     {
