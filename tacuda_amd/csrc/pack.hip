@@ -2028,6 +2028,32 @@ __global__ __launch_bounds__(kPW* kWave) void k_pipe_fold(PipeP P, PackP prm, Pa
   else pipe_fold_wave<kGA>(P, L, io, c, f, o);
 }
 
+// Passes with one recurrence group and no window part (the RSI, MACD and
+// ATR family calls): four segments per workgroup, one per wave, so no wave
+// of the block idles.  The other groups' fields of the composite are the
+// identity.
+template <unsigned GM>
+__global__ __launch_bounds__(kPW* kWave) void k_pipe_fold1(PipeP P, PackIO io, const int* lead,
+                                                           double* agg) {
+  __shared__ __align__(16) PLds sm[kPW];
+  const int w = threadIdx.x / kWave;
+  const int seg = blockIdx.x * kPW + w;
+  if (seg >= P.rows * P.segsPerRow) return;
+  int row;
+  const Seg c = pipe_seg(P, seg, row);
+  pipe_stage<PTile>(sm[w], io, c, P);
+  double* o = agg + (size_t)seg * 16;
+  if (lane_id() == kWave - 1) {
+    PC I;
+    pc_id(I);
+    if constexpr (GM != kGE) pc_store_group<kGE>(o, I);
+    if constexpr (GM != kGR) pc_store_group<kGR>(o, I);
+    if constexpr (GM != kGM) pc_store_group<kGM>(o, I);
+    if constexpr (GM != kGA) pc_store_group<kGA>(o, I);
+  }
+  pipe_fold_wave<GM>(P, sm[w], io, c, lead[row], o);
+}
+
 // One wave of the emit pass: group GM of the block's segment.
 template <unsigned GM>
 __device__ __forceinline__ void pipe_emit_wave(const PipeP& P, const PLds& L, const PackIO& io,
@@ -2068,6 +2094,21 @@ __global__ __launch_bounds__(kPW* kWave) void k_pipe_emit(PipeP P, PackIO io, co
   else if (w == 1) pipe_emit_wave<kGR>(P, L, io, c, row, seg, lead, agg, gtot, genter);
   else if (w == 2) pipe_emit_wave<kGM>(P, L, io, c, row, seg, lead, agg, gtot, genter);
   else pipe_emit_wave<kGA>(P, L, io, c, row, seg, lead, agg, gtot, genter);
+}
+
+// The emit pass of k_pipe_fold1's passes: one segment per wave.
+template <unsigned GM>
+__global__ __launch_bounds__(kPW* kWave) void k_pipe_emit1(PipeP P, PackIO io, const int* lead,
+                                                           const double* agg, const double* gtot,
+                                                           const double* genter) {
+  __shared__ __align__(16) PLds sm[kPW];
+  const int w = threadIdx.x / kWave;
+  const int seg = blockIdx.x * kPW + w;
+  if (seg >= P.rows * P.segsPerRow) return;
+  int row;
+  const Seg c = pipe_seg(P, seg, row);
+  pipe_stage<PTile>(sm[w], io, c, P);
+  pipe_emit_wave<GM>(P, sm[w], io, c, row, seg, lead, agg, gtot, genter);
 }
 
 // Whether the two passes serve this pass: dense rows of at least kPMinLen
@@ -2179,10 +2220,19 @@ ctStatus_t launch_pipe(int len, int rows, hipStream_t s, const PackP& prm, const
     hipLaunchKernelGGL(k_pipe_lead, dim3(rows), dim3(kWave), 0, s, P, io.c, lead);
   }
   TCA_HIP(hipGetLastError());
+  // one recurrence group and no window part: the one-segment-per-wave kernels
+  const unsigned one = nbW ? 0u
+                           : (P.wE ? kGE : 0u) | (P.wR ? kGR : 0u) | (P.wM ? kGM : 0u) |
+                                 (P.wA ? kGA : 0u);
+  const dim3 g1(div_up(segs, kPW)), blk(kPW * kWave);
   {
     KTime kt(s, "k_pipe_fold");
-    hipLaunchKernelGGL(k_pipe_fold, dim3(nbW + segs), dim3(kPW * kWave), 0, s, P, prm, io, lead,
-                       agg, gw, nbW);
+    if (one == kGE) hipLaunchKernelGGL(k_pipe_fold1<kGE>, g1, blk, 0, s, P, io, lead, agg);
+    else if (one == kGR) hipLaunchKernelGGL(k_pipe_fold1<kGR>, g1, blk, 0, s, P, io, lead, agg);
+    else if (one == kGM) hipLaunchKernelGGL(k_pipe_fold1<kGM>, g1, blk, 0, s, P, io, lead, agg);
+    else if (one == kGA) hipLaunchKernelGGL(k_pipe_fold1<kGA>, g1, blk, 0, s, P, io, lead, agg);
+    else
+      hipLaunchKernelGGL(k_pipe_fold, dim3(nbW + segs), blk, 0, s, P, prm, io, lead, agg, gw, nbW);
   }
   TCA_HIP(hipGetLastError());
   {
@@ -2203,8 +2253,12 @@ ctStatus_t launch_pipe(int len, int rows, hipStream_t s, const PackP& prm, const
   TCA_HIP(hipGetLastError());
   {
     KTime kt(s, "k_pipe_emit");
-    hipLaunchKernelGGL(k_pipe_emit, dim3(segs), dim3(kPW * kWave), 0, s, P, io, lead, agg, gtot,
-                       rowsInEmit ? nullptr : genter);
+    const double* ge = rowsInEmit ? nullptr : genter;
+    if (one == kGE) hipLaunchKernelGGL(k_pipe_emit1<kGE>, g1, blk, 0, s, P, io, lead, agg, gtot, ge);
+    else if (one == kGR) hipLaunchKernelGGL(k_pipe_emit1<kGR>, g1, blk, 0, s, P, io, lead, agg, gtot, ge);
+    else if (one == kGM) hipLaunchKernelGGL(k_pipe_emit1<kGM>, g1, blk, 0, s, P, io, lead, agg, gtot, ge);
+    else if (one == kGA) hipLaunchKernelGGL(k_pipe_emit1<kGA>, g1, blk, 0, s, P, io, lead, agg, gtot, ge);
+    else hipLaunchKernelGGL(k_pipe_emit, dim3(segs), blk, 0, s, P, io, lead, agg, gtot, ge);
   }
   TCA_HIP(hipGetLastError());
   return CT_STATUS_SUCCESS;
