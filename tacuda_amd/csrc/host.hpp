@@ -143,6 +143,14 @@ constexpr long long kPipeMinSegments = 512;
 ctStatus_t pipe_family(const ctxPackParams& params, const ctxPackOutputs& outputs,
                        const float* h, const float* l, const float* c, int len, int rows,
                        hipStream_t stream, bool* served);
+// The family calls of SMA, WMA, BBANDS, STOCH, WILLR and CCI (dense rows of
+// any length, window look-backs within the pack's 1024 bars) run the fused
+// pass's window kernel: one wave per 256-bar column tile, every window
+// evaluated directly in the oracle's order, so the outputs are those of the
+// family kernels.  *served as for pipe_family.
+ctStatus_t win_family(const ctxPackParams& params, const ctxPackOutputs& outputs,
+                      const float* h, const float* l, const float* c, int len, int rows,
+                      hipStream_t stream, bool* served);
 
 // Synchronize when the caller used the legacy default stream: batch calls
 // on stream 0 return once the batch is done (reference api_batch.cu:377-386).

@@ -2417,6 +2417,22 @@ ctStatus_t pipe_family(const ctxPackParams& params, const ctxPackOutputs& output
   return launch_pipe(len, rows, stream, P, io);
 }
 
+ctStatus_t win_family(const ctxPackParams& params, const ctxPackOutputs& outputs,
+                      const float* h, const float* l, const float* c, int len, int rows,
+                      hipStream_t stream, bool* served) {
+  *served = false;
+  if (t_ragged) return CT_STATUS_SUCCESS;
+  PackP P;
+  PackIO io;
+  if (pack_validate(&params, &outputs, P, io, h, l, c) != CT_STATUS_SUCCESS) {
+    clear_error();  // the family path reports its own errors
+    return CT_STATUS_SUCCESS;
+  }
+  if (!(P.mask & kWinMask) || (P.mask & ~kWinMask)) return CT_STATUS_SUCCESS;
+  *served = true;
+  return launch_windows(len, rows, stream, P, io);
+}
+
 }  // namespace tca
 
 extern "C" {

@@ -229,6 +229,15 @@ ctStatus_t dev_sma(const float* x, float* y, int len, int rows, int p, hipStream
     so.sma = y;
     return via_scan(x, len, rows, s, so, sp);
   }
+  {
+    ctxPackParams pp = pipe_params();
+    pp.sma_period = p;
+    ctxPackOutputs po{};
+    po.sma = y;
+    bool served = false;
+    CHECK_RET(win_family(pp, po, x, x, x, len, rows, s, &served));
+    if (served) return CT_STATUS_SUCCESS;
+  }
   return run_Sma(len, rows, s, {p}, io1(x, y));
 }
 ctStatus_t dev_sum(const float* x, float* y, int len, int rows, int p, hipStream_t s) {
@@ -241,6 +250,15 @@ ctStatus_t dev_wma(const float* x, float* y, int len, int rows, int p, hipStream
   CHECK_PTR(x, "wma"); CHECK_PTR(y, "wma");
   CHECK_RET(check_shape("wma", len, rows));
   if (p <= 0) return bad("wma", "invalid period");
+  {
+    ctxPackParams pp = pipe_params();
+    pp.wma_period = p;
+    ctxPackOutputs po{};
+    po.wma = y;
+    bool served = false;
+    CHECK_RET(win_family(pp, po, x, x, x, len, rows, s, &served));
+    if (served) return CT_STATUS_SUCCESS;
+  }
   return run_Wma(len, rows, s, {p}, io1(x, y));
 }
 ctStatus_t dev_ema(const float* x, float* y, int len, int rows, int p, hipStream_t s) {
@@ -341,6 +359,19 @@ ctStatus_t dev_bbands(const float* x, float* u, float* m, float* l, int len, int
   CHECK_PTR(x, "bbands"); CHECK_PTR(u, "bbands"); CHECK_PTR(m, "bbands"); CHECK_PTR(l, "bbands");
   CHECK_RET(check_shape("bbands", len, rows));
   if (p <= 0) return bad("bbands", "invalid period");
+  {
+    ctxPackParams pp = pipe_params();
+    pp.bb_period = p;
+    pp.bb_up = up;
+    pp.bb_dn = dn;
+    ctxPackOutputs po{};
+    po.bb_upper = u;
+    po.bb_middle = m;
+    po.bb_lower = l;
+    bool served = false;
+    CHECK_RET(win_family(pp, po, x, x, x, len, rows, s, &served));
+    if (served) return CT_STATUS_SUCCESS;
+  }
   return run_Bbands(len, rows, s, {p, (double)up, (double)dn}, io1(x, u, m, l));
 }
 ctStatus_t dev_var(const float* x, float* y, int len, int rows, int p, hipStream_t s) {
@@ -404,6 +435,17 @@ ctStatus_t dev_stoch(const float* h, const float* l, const float* c, float* k, f
   if (kp <= 0 || dp <= 0) return bad("stochastic", "invalid period");
   if (dp > kStochMaxD)
     return fail(CT_STATUS_UNSUPPORTED, "stochastic: dPeriod above %d is not provided", kStochMaxD);
+  {
+    ctxPackParams pp = pipe_params();
+    pp.stoch_k = kp;
+    pp.stoch_d = dp;
+    ctxPackOutputs po{};
+    po.stoch_k = k;
+    po.stoch_d = d;
+    bool served = false;
+    CHECK_RET(win_family(pp, po, h, l, c, len, rows, s, &served));
+    if (served) return CT_STATUS_SUCCESS;
+  }
   return run_Stoch(len, rows, s, {kp, dp}, io3(h, l, c, k, d));
 }
 ctStatus_t dev_willr(const float* h, const float* l, const float* c, float* y, int len, int rows,
@@ -411,6 +453,15 @@ ctStatus_t dev_willr(const float* h, const float* l, const float* c, float* y, i
   CHECK_PTR(h, "willr"); CHECK_PTR(l, "willr"); CHECK_PTR(c, "willr"); CHECK_PTR(y, "willr");
   CHECK_RET(check_shape("willr", len, rows));
   if (p <= 0) return bad("willr", "invalid period");
+  {
+    ctxPackParams pp = pipe_params();
+    pp.willr_period = p;
+    ctxPackOutputs po{};
+    po.willr = y;
+    bool served = false;
+    CHECK_RET(win_family(pp, po, h, l, c, len, rows, s, &served));
+    if (served) return CT_STATUS_SUCCESS;
+  }
   return run_Willr(len, rows, s, {p}, io3(h, l, c, y));
 }
 ctStatus_t dev_cci(const float* h, const float* l, const float* c, float* y, int len, int rows,
@@ -418,6 +469,15 @@ ctStatus_t dev_cci(const float* h, const float* l, const float* c, float* y, int
   CHECK_PTR(h, "cci"); CHECK_PTR(l, "cci"); CHECK_PTR(c, "cci"); CHECK_PTR(y, "cci");
   CHECK_RET(check_shape("cci", len, rows));
   if (p <= 0) return bad("cci", "invalid period");
+  {
+    ctxPackParams pp = pipe_params();
+    pp.cci_period = p;
+    ctxPackOutputs po{};
+    po.cci = y;
+    bool served = false;
+    CHECK_RET(win_family(pp, po, h, l, c, len, rows, s, &served));
+    if (served) return CT_STATUS_SUCCESS;
+  }
   return run_Cci(len, rows, s, {p}, io3(h, l, c, y));
 }
 ctStatus_t dev_aroon(const float* h, const float* l, float* dn, float* up, int len, int rows,
