@@ -131,18 +131,23 @@ ctStatus_t scan_run(const float* close, int seriesLen, int numSeries, const ctxS
 bool scan_serves(int len, int rows, int windowLookback);
 
 // ------------------------------------------------------ fused pass --
-// The family calls of RSI, MACD and ATR on dense batches of at least
-// kPipeMinRows long rows, kPipeMinSegments 512-bar segments in all, run the
-// fused pass's two streaming passes (the pack's outputs for that family,
-// the same oracle-following arithmetic): *served says whether they did;
-// when not, nothing was launched and the caller takes its own path.  A few
-// long rows or a small batch stay on the one-launch look-back kernels (the
-// passes' five launches are latency-bound there).
+// The family calls of EMA, RSI, MACD and ATR on rows of at least 8192 bars run
+// the fused pass's two streaming passes (the pack's outputs for that family,
+// the same oracle-following arithmetic) for dense batches of at least
+// kPipeMinRows rows and kPipeMinSegments 512-bar segments in all, or for
+// fewer rows holding at least fewRowsMinSegs segments (the look-back kernels'
+// chains grow with the row length, the passes' four launches do not):
+// *served says whether they did; when not, nothing was launched and the
+// caller takes its own path.
 constexpr int kPipeMinRows = 16;
 constexpr long long kPipeMinSegments = 512;
+constexpr long long kPipeFewRowsSegs = 4096;      // ATR
+constexpr long long kPipeFewRowsSegsRsi = 2048;
+constexpr long long kPipeFewRowsSegsEma = 2048;   // below the long scan's batches
+constexpr long long kPipeFewRowsSegsMacd = 1024;  // MACD's three-level look-back
 ctStatus_t pipe_family(const ctxPackParams& params, const ctxPackOutputs& outputs,
                        const float* h, const float* l, const float* c, int len, int rows,
-                       hipStream_t stream, bool* served);
+                       hipStream_t stream, long long fewRowsMinSegs, bool* served);
 // The family calls of SMA, WMA, BBANDS, STOCH, WILLR and CCI (dense rows of
 // any length, window look-backs within the pack's 1024 bars) run the fused
 // pass's window kernel: one wave per 256-bar column tile, every window

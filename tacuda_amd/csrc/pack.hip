@@ -2401,11 +2401,11 @@ namespace tca {
 
 ctStatus_t pipe_family(const ctxPackParams& params, const ctxPackOutputs& outputs,
                        const float* h, const float* l, const float* c, int len, int rows,
-                       hipStream_t stream, bool* served) {
+                       hipStream_t stream, long long fewRowsMinSegs, bool* served) {
   *served = false;
-  if (t_ragged || rows < kPipeMinRows || len < kPMinLen ||
-      (long long)rows * div_up(len, kPSeg) < kPipeMinSegments)
-    return CT_STATUS_SUCCESS;
+  const long long segs = (long long)rows * div_up(len, kPSeg);
+  const bool dense = rows >= kPipeMinRows && segs >= kPipeMinSegments;
+  if (t_ragged || len < kPMinLen || !(dense || segs >= fewRowsMinSegs)) return CT_STATUS_SUCCESS;
   PackP P;
   PackIO io;
   if (pack_validate(&params, &outputs, P, io, h, l, c) != CT_STATUS_SUCCESS) {

@@ -272,6 +272,15 @@ ctStatus_t dev_ema(const float* x, float* y, int len, int rows, int p, hipStream
     so.ema = y;
     return via_scan(x, len, rows, s, so, sp);
   }
+  {
+    ctxPackParams pp = pipe_params();
+    pp.ema_period = p;
+    ctxPackOutputs po{};
+    po.ema = y;
+    bool served = false;
+    CHECK_RET(pipe_family(pp, po, x, x, x, len, rows, s, kPipeFewRowsSegsEma, &served));
+    if (served) return CT_STATUS_SUCCESS;
+  }
   return run_Ema(len, rows, s, {p}, io1(x, y));
 }
 ctStatus_t dev_dema(const float* x, float* y, int len, int rows, int p, hipStream_t s) {
@@ -320,13 +329,13 @@ ctStatus_t dev_rsi(const float* x, float* y, int len, int rows, int p, hipStream
   CHECK_PTR(x, "rsi"); CHECK_PTR(y, "rsi");
   CHECK_RET(check_shape("rsi", len, rows));
   if (p <= 0) return bad("rsi", "invalid period");
-  if (rows >= kPipeMinRows) {
+  {
     ctxPackParams pp = pipe_params();
     pp.rsi_period = p;
     ctxPackOutputs po{};
     po.rsi = y;
     bool served = false;
-    CHECK_RET(pipe_family(pp, po, x, x, x, len, rows, s, &served));
+    CHECK_RET(pipe_family(pp, po, x, x, x, len, rows, s, kPipeFewRowsSegsRsi, &served));
     if (served) return CT_STATUS_SUCCESS;
   }
   return run_Rsi(len, rows, s, {p}, io1(x, y));
@@ -339,7 +348,7 @@ ctStatus_t dev_macd(const float* x, float* m, float* sg, float* h, int len, int 
   if (maType != CT_MA_EMA && maType != CT_MA_DEFAULT)
     return fail(CT_STATUS_UNSUPPORTED, "macd: only the EMA moving-average type is provided");
   if (slow < fast) std::swap(slow, fast);
-  if (rows >= kPipeMinRows) {
+  {
     ctxPackParams pp = pipe_params();
     pp.macd_fast = fast;
     pp.macd_slow = slow;
@@ -349,7 +358,7 @@ ctStatus_t dev_macd(const float* x, float* m, float* sg, float* h, int len, int 
     po.macd_signal = sg;
     po.macd_hist = h;
     bool served = false;
-    CHECK_RET(pipe_family(pp, po, x, x, x, len, rows, s, &served));
+    CHECK_RET(pipe_family(pp, po, x, x, x, len, rows, s, kPipeFewRowsSegsMacd, &served));
     if (served) return CT_STATUS_SUCCESS;
   }
   return run_Macd(len, rows, s, {fast, slow, sig}, io1(x, m, sg, h));
@@ -403,13 +412,13 @@ ctStatus_t dev_atr(const float* h, const float* l, const float* c, float* y, int
   CHECK_PTR(h, "atr"); CHECK_PTR(l, "atr"); CHECK_PTR(c, "atr"); CHECK_PTR(y, "atr");
   CHECK_RET(check_shape("atr", len, rows));
   if (p <= 0) return bad("atr", "invalid period");
-  if (rows >= kPipeMinRows) {
+  {
     ctxPackParams pp = pipe_params();
     pp.atr_period = p;
     ctxPackOutputs po{};
     po.atr = y;
     bool served = false;
-    CHECK_RET(pipe_family(pp, po, h, l, c, len, rows, s, &served));
+    CHECK_RET(pipe_family(pp, po, h, l, c, len, rows, s, kPipeFewRowsSegs, &served));
     if (served) return CT_STATUS_SUCCESS;
   }
   return run_Atr(len, rows, s, {p}, io3(h, l, c, y));

@@ -235,12 +235,12 @@ def test_pack_two_pass_non_finite(dev, case):
 
 
 @pytest.mark.parametrize("case", ["plain", "prefix", "hole", "period1"])
-@pytest.mark.parametrize("shape", [(16, 16500), (20, 30011)])
+@pytest.mark.parametrize("shape", [(16, 16500), (20, 30011), (2, (1 << 20) + 5)])
 def test_family_batches_routed_through_two_passes(dev, case, shape):
     """ct_rsi_batch_d / ct_macd_batch_d / ct_atr_batch_d on dense batches of
-    at least 16 long rows and 512 segments take the fused pass's two passes
-    (period 1 keeps the look-back kernels): the oracle's results either
-    way."""
+    at least 16 long rows and 512 segments, or on fewer rows holding 4096
+    segments, take the fused pass's two passes (period 1 keeps the look-back
+    kernels): the oracle's results either way."""
     import tacuda_amd as ta
     from oracle import ORC
 
@@ -274,3 +274,58 @@ def test_family_batches_routed_through_two_passes(dev, case, shape):
         w = ORC.macd(C[r], 12, 26, 9)
         for i in range(3):
             compare(m[i][r].cpu().numpy(), w[i], tol=1e-5, min_exact=0.999, name=f"macd{i} row {r}")
+
+
+def test_single_long_row_macd_routed(dev):
+    """One row of 600,000 bars (1172 segments): MACD takes the two passes
+    (1024 segments suffice for its three-level look-back), RSI and ATR keep
+    the look-back kernels; the oracle's results either way."""
+    import tacuda_amd as ta
+    from oracle import ORC
+
+    print("routed families 1x600000", flush=True)
+    d = _data(600000, seed=61)
+    h, l, c = (torch.from_numpy(np.ascontiguousarray(d[k])).to(dev) for k in ("high", "low", "close"))
+    torch.cuda.synchronize()
+    ta.kernel_times()
+    ta.kernel_timing(True)
+    try:
+        m = ta.ops.macd(c, 12, 26, 9)
+        rsi = ta.ops.rsi(c, 14)
+        atr = ta.ops.atr(h, l, c, 14)
+    finally:
+        ta.kernel_timing(False)
+    torch.cuda.synchronize()
+    assert sum(1 for k, _ in ta.kernel_times() if k == "k_pipe_fold") == 1
+    w = ORC.macd(d["close"], 12, 26, 9)
+    for i in range(3):
+        compare(m[i].cpu().numpy(), w[i], tol=1e-5, min_exact=0.999, name=f"macd{i}")
+    compare(rsi.cpu().numpy(), ORC.rsi(d["close"], 14), tol=1e-5, min_exact=0.999, name="rsi")
+    compare(atr.cpu().numpy(), ORC.atr(d["high"], d["low"], d["close"], 14), tol=1e-5,
+            min_exact=0.999, name="atr")
+
+
+def test_single_long_row_ema_rsi_routed(dev):
+    """One row of 2^20 bars (2048 segments): EMA and RSI take the two passes,
+    ATR keeps the look-back kernels; the oracle's results either way."""
+    import tacuda_amd as ta
+    from oracle import ORC
+
+    print("routed families 1x1048576", flush=True)
+    d = _data(1 << 20, seed=67)
+    h, l, c = (torch.from_numpy(np.ascontiguousarray(d[k])).to(dev) for k in ("high", "low", "close"))
+    torch.cuda.synchronize()
+    ta.kernel_times()
+    ta.kernel_timing(True)
+    try:
+        ema = ta.ops.ema(c, 20)
+        rsi = ta.ops.rsi(c, 14)
+        atr = ta.ops.atr(h, l, c, 14)
+    finally:
+        ta.kernel_timing(False)
+    torch.cuda.synchronize()
+    assert sum(1 for k, _ in ta.kernel_times() if k == "k_pipe_fold") == 2
+    compare(ema.cpu().numpy(), ORC.ema(d["close"], 20), tol=1e-5, min_exact=0.999, name="ema")
+    compare(rsi.cpu().numpy(), ORC.rsi(d["close"], 14), tol=1e-5, min_exact=0.999, name="rsi")
+    compare(atr.cpu().numpy(), ORC.atr(d["high"], d["low"], d["close"], 14), tol=1e-5,
+            min_exact=0.999, name="atr")
