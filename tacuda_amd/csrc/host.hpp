@@ -142,7 +142,7 @@ bool scan_serves(int len, int rows, int windowLookback);
 constexpr int kPipeMinRows = 16;
 constexpr long long kPipeMinSegments = 512;
 constexpr long long kPipeFewRowsSegs = 4096;      // ATR
-constexpr long long kPipeFewRowsSegsRsi = 2048;
+constexpr long long kPipeFewRowsSegsRsi = 1900;  // one row of 10^6 bars
 constexpr long long kPipeFewRowsSegsEma = 2048;   // below the long scan's batches
 constexpr long long kPipeFewRowsSegsMacd = 1024;  // MACD's three-level look-back
 ctStatus_t pipe_family(const ctxPackParams& params, const ctxPackOutputs& outputs,
