@@ -82,7 +82,9 @@ typedef enum ctDataType {
 
 /* ---------------------------------------------------------------------------
  * Host calls: upload, run the device kernel, download, synchronize the stream
- * (reference api.cpp:230-287).  Lines cite tacuda.h.
+ * (reference api.cpp:230-287).  Device buffers come from a pool and arrays of
+ * 1 MB or more are copied through pinned staging chunks, so a call allocates
+ * nothing once the pool holds buffers of its sizes.  Lines cite tacuda.h.
  * ------------------------------------------------------------------------- */
 /* :98 */ CTAPI_EXPORT ctStatus_t ct_sma(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
 /* :109 */ CTAPI_EXPORT ctStatus_t ct_wma(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
@@ -114,8 +116,44 @@ typedef enum ctDataType {
 /* :419 */ CTAPI_EXPORT ctStatus_t ct_willr(const float* host_high, const float* host_low, const float* host_close, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
 /* :739 */ CTAPI_EXPORT ctStatus_t ct_dx(const float* host_high, const float* host_low, const float* host_close, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
 
+/* :103 -- moving average selected by type: SMA, EMA, WMA, DEMA, TEMA, KAMA
+ * (fast 2, slow 30) and T3 (vFactor 0.7); other types return
+ * CT_STATUS_UNSUPPORTED. */
+CTAPI_EXPORT ctStatus_t ct_ma(const float* host_input, float* host_output, int size, int period, ctMaType_t type, hipStream_t stream CTAPI_STREAM_DEFAULT);
+
+/* Typed host variants (:100, :125, :161, :177, :303, :312): float32 storage
+ * with float32 or float64 compute; other storage types return
+ * CT_STATUS_UNSUPPORTED (see the _ex_d calls in tacuda_device.h). */
+CTAPI_EXPORT ctStatus_t ct_sma_ex(const void* host_input, void* host_output, int size, int period, ctDataType_t storageType, ctDataType_t computeType, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_ema_ex(const void* host_input, void* host_output, int size, int period, ctDataType_t storageType, ctDataType_t computeType, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_rsi_ex(const void* host_input, void* host_output, int size, int period, ctDataType_t storageType, ctDataType_t computeType, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_macd_ex(const void* host_input, void* host_macd, void* host_signal, void* host_hist, int size, int fastPeriod, int slowPeriod, int signalPeriod, ctMaType_t type, ctDataType_t storageType, ctDataType_t computeType, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_bbands_ex(const void* host_input, void* host_upper, void* host_middle, void* host_lower, int size, int period, float upperMul, float lowerMul, ctDataType_t storageType, ctDataType_t computeType, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_atr_ex(const void* host_high, const void* host_low, const void* host_close, void* host_output, int size, int period, float initial, ctDataType_t storageType, ctDataType_t computeType, hipStream_t stream CTAPI_STREAM_DEFAULT);
+
+/* ---------------------------------------------------------------------------
+ * Device-pointer variants (:202-290): the same calls as ct_<name>_d
+ * (tacuda_device.h), enqueued on the stream without waiting.
+ * ------------------------------------------------------------------------- */
+CTAPI_EXPORT ctStatus_t ct_sma_device(const float* device_input, float* device_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_ma_device(const float* device_input, float* device_output, int size, int period, ctMaType_t type, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_wma_device(const float* device_input, float* device_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_ema_device(const float* device_input, float* device_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_dema_device(const float* device_input, float* device_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_tema_device(const float* device_input, float* device_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_t3_device(const float* device_input, float* device_output, int size, int period, float vFactor, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_max_device(const float* device_input, float* device_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_min_device(const float* device_input, float* device_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_stddev_device(const float* device_input, float* device_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_var_device(const float* device_input, float* device_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_sum_device(const float* device_input, float* device_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_rsi_device(const float* device_input, float* device_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_kama_device(const float* device_input, float* device_output, int size, int period, int fastPeriod, int slowPeriod, hipStream_t stream CTAPI_STREAM_DEFAULT);
+
 /* tacuda.h:824-827 -- the workspace cache of this build stands in for the
- * reference's DeviceBufferPool. */
+ * reference's DeviceBufferPool.  Cleanup frees every look-back workspace,
+ * including those that graphs captured earlier still reference: call it only
+ * when no such graph will be replayed again. */
 CTAPI_EXPORT void ct_device_buffer_pool_cleanup(void);
 CTAPI_EXPORT size_t ct_device_buffer_pool_allocation_count(void);
 CTAPI_EXPORT size_t ct_device_buffer_pool_set_max_cache_bytes(size_t bytes);

@@ -18,8 +18,23 @@ extern "C" {
 
 /* Reserve `bytes` of the per-stream look-back workspace so a later call can
  * be captured into a hipGraph without allocating (an eager call of the same
- * shape on the stream reserves it too). */
+ * shape on the stream reserves it too).  A graph captured on a stream keeps
+ * the workspace it was captured with: a later, larger call on that stream
+ * moves the stream to a new workspace and leaves the old one allocated until
+ * ct_device_buffer_pool_cleanup, after which such graphs must not be
+ * replayed.  Replays of one captured graph, and eager calls on its capture
+ * stream, share that workspace and must not overlap: replay such a graph on
+ * one stream at a time (ctx_plan_launch orders the replays of a plan
+ * itself). */
 CTAPI_EXPORT ctStatus_t ctx_workspace_reserve(hipStream_t stream, size_t bytes);
+
+/* How the host calls (ct_<name>, ct_<name>_batch) move host arrays (no
+ * counterpart in the reference, which page-locks host matrices of 1 MB or
+ * more): 0 = by size (arrays of 1 MiB or more through a pair of pinned 4 MiB
+ * staging chunks whose transfers overlap the host-side copies, smaller ones
+ * copied directly; the default), 1 = always direct copies of pageable
+ * memory, 2 = always pinned staging. */
+CTAPI_EXPORT ctStatus_t ctx_host_staging(int mode);
 
 /* Kernel timing (no counterpart in the reference; for benchmarks and
  * tuning).  While enabled, the fused-pass, long-scan and candle launches

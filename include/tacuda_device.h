@@ -21,6 +21,9 @@ CTAPI_EXPORT void ct_device_free(float* d_ptr);
 CTAPI_EXPORT void ct_device_free_ex(void* d_ptr);
 CTAPI_EXPORT ctStatus_t ct_device_upload(float* d_dst, const float* h_src, int count, hipStream_t stream CTAPI_STREAM_DEFAULT);
 CTAPI_EXPORT ctStatus_t ct_device_download(float* h_dst, const float* d_src, int count, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :101, :115 -- element size from the storage type (2, 4 or 8 bytes) */
+CTAPI_EXPORT ctStatus_t ct_device_upload_ex(void* d_dst, const void* h_src, int count, ctDataType_t storageType, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_device_download_ex(void* h_dst, const void* d_src, int count, ctDataType_t storageType, hipStream_t stream CTAPI_STREAM_DEFAULT);
 CTAPI_EXPORT ctStatus_t ct_device_sync(hipStream_t stream CTAPI_STREAM_DEFAULT);
 
 /* :130-171 moving averages */
@@ -59,6 +62,18 @@ CTAPI_EXPORT ctStatus_t ct_cci_d(const float* d_high, const float* d_low, const 
 /* :280 bands */
 CTAPI_EXPORT ctStatus_t ct_bbands_d(const float* d_input, float* d_upper, float* d_middle, float* d_lower, int size, int period, float nbDevUp, float nbDevDn, hipStream_t stream CTAPI_STREAM_DEFAULT);
 
+/* Typed variants (:132, :138, :167, :197, :244, :283).  Float32 storage with
+ * float32 or float64 compute runs the float32 path (its recurrences, window
+ * sums and ratios are evaluated in float64 and rounded once per output);
+ * half and double storage return CT_STATUS_UNSUPPORTED, as do the pairs the
+ * reference rejects. */
+CTAPI_EXPORT ctStatus_t ct_sma_ex_d(const void* d_input, void* d_output, int size, int period, ctDataType_t storageType, ctDataType_t computeType, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_ema_ex_d(const void* d_input, void* d_output, int size, int period, ctDataType_t storageType, ctDataType_t computeType, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_rsi_ex_d(const void* d_input, void* d_output, int size, int period, ctDataType_t storageType, ctDataType_t computeType, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_atr_ex_d(const void* d_high, const void* d_low, const void* d_close, void* d_output, int size, int period, float percentile, ctDataType_t storageType, ctDataType_t computeType, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_macd_ex_d(const void* d_input, void* d_macd, void* d_signal, void* d_hist, int size, int fastPeriod, int slowPeriod, int signalPeriod, int maType, ctDataType_t storageType, ctDataType_t computeType, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_bbands_ex_d(const void* d_input, void* d_upper, void* d_middle, void* d_lower, int size, int period, float nbDevUp, float nbDevDn, ctDataType_t storageType, ctDataType_t computeType, hipStream_t stream CTAPI_STREAM_DEFAULT);
+
 /* :313-326 statistical */
 CTAPI_EXPORT ctStatus_t ct_stddev_d(const float* d_input, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
 CTAPI_EXPORT ctStatus_t ct_var_d(const float* d_input, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
@@ -73,8 +88,11 @@ CTAPI_EXPORT ctStatus_t ct_aroonosc_d(const float* d_high, const float* d_low, f
 /* The 61 TA-Lib candlestick patterns (tacuda_device.h:438+, tacuda.h:949+):
  * legacy float signals {0, +-80, +-100, +-200}, 0 during each pattern's
  * TA-Lib look-back.  ct_cdl_<name>_batch_d runs numSeries series-major rows
- * (maxConcurrency is accepted and unused).  int32 outputs and the all-61
- * launch: ctx_cdl_batch_d / ctx_cdl_all_batch_d (tacuda_amd.h). */
+ * (maxConcurrency is accepted and unused); the host forms ct_cdl_<name>
+ * (tacuda.h:432+) and ct_cdl_<name>_batch (tacuda.h:1870+) stage host
+ * arrays through the host-call buffers and wait for the stream.  int32
+ * outputs and the all-61 launch: ctx_cdl_batch_d / ctx_cdl_all_batch_d
+ * (tacuda_amd.h). */
 #define CT_CDL_DECL(name)                                                                      \
   CTAPI_EXPORT ctStatus_t ct_cdl_##name##_d(const float* d_open, const float* d_high,         \
                                             const float* d_low, const float* d_close,          \
@@ -84,7 +102,18 @@ CTAPI_EXPORT ctStatus_t ct_aroonosc_d(const float* d_high, const float* d_low, f
                                                   const float* d_lows, const float* d_closes,  \
                                                   float* d_outputs, int seriesLen,             \
                                                   int numSeries, int maxConcurrency,           \
-                                                  hipStream_t stream CTAPI_STREAM_DEFAULT);
+                                                  hipStream_t stream CTAPI_STREAM_DEFAULT);    \
+  CTAPI_EXPORT ctStatus_t ct_cdl_##name(const float* host_open, const float* host_high,       \
+                                        const float* host_low, const float* host_close,        \
+                                        float* host_output, int size,                          \
+                                        hipStream_t stream CTAPI_STREAM_DEFAULT);              \
+  CTAPI_EXPORT ctStatus_t ct_cdl_##name##_batch(const float* host_opens,                      \
+                                                const float* host_highs,                      \
+                                                const float* host_lows,                       \
+                                                const float* host_closes,                     \
+                                                float* host_outputs, int seriesLen,           \
+                                                int numSeries, int maxConcurrency,            \
+                                                hipStream_t stream CTAPI_STREAM_DEFAULT);
 CT_CDL_DECL(two_crows)
 CT_CDL_DECL(three_black_crows)
 CT_CDL_DECL(three_inside)
@@ -146,6 +175,12 @@ CT_CDL_DECL(tristar)
 CT_CDL_DECL(unique_3_river)
 CT_CDL_DECL(upside_gap_2_crows)
 CT_CDL_DECL(xside_gap_3_methods)
+/* The reference's legacy engulfing names (tacuda.h:447-452, :1917-1952;
+ * tacuda_device.h:571-582): both return TA-Lib's signed engulfing signal
+ * (+-100, or +-80 when an open equals the other candle's close), written
+ * from bar 1 on (0 at bar 0). */
+CT_CDL_DECL(bullish_engulfing)
+CT_CDL_DECL(bearish_engulfing)
 #undef CT_CDL_DECL
 
 #ifdef __cplusplus

@@ -63,10 +63,57 @@ _sig("ct_aroonosc_batch_d", P, P, P, c_int, c_int, c_int, c_int)
 _sig("ct_aroon_batch_d", P, P, P, P, c_int, c_int, c_int, c_int)
 _sig("ct_plus_dm_batch_d", P, P, P, c_int, c_int, c_int, c_int)
 _sig("ct_minus_dm_batch_d", P, P, P, c_int, c_int, c_int, c_int)
-# host calls
-_sig("ct_sma", P, P, c_int, c_int)
-_sig("ct_ema", P, P, c_int, c_int)
-_sig("ct_rsi", P, P, c_int, c_int)
+# host calls (include/tacuda.h): host arrays in and out, the call waits
+for _n in ("sma", "ema", "wma", "dema", "tema", "rsi", "stddev", "var", "sum", "max", "min"):
+    _sig(f"ct_{_n}", P, P, c_int, c_int)
+    _sig(f"ct_{_n}_batch", P, P, c_int, c_int, c_int, c_int)
+_sig("ct_t3", P, P, c_int, c_int, c_float)
+_sig("ct_kama", P, P, c_int, c_int, c_int, c_int)
+_sig("ct_ma", P, P, c_int, c_int, c_int)
+_sig("ct_macd", P, P, P, P, c_int, c_int, c_int, c_int, c_int)
+_sig("ct_bbands", P, P, P, P, c_int, c_int, c_float, c_float)
+_sig("ct_atr", P, P, P, P, c_int, c_int, c_float)
+for _n in ("natr", "adx", "dx", "plus_di", "minus_di", "willr", "cci"):
+    _sig(f"ct_{_n}", P, P, P, P, c_int, c_int)
+    _sig(f"ct_{_n}_batch", P, P, P, P, c_int, c_int, c_int, c_int)
+_sig("ct_trange", P, P, P, P, c_int)
+_sig("ct_stochastic", P, P, P, P, P, c_int, c_int, c_int)
+_sig("ct_plus_dm", P, P, P, c_int, c_int)
+_sig("ct_minus_dm", P, P, P, c_int, c_int)
+_sig("ct_aroon", P, P, P, P, P, c_int, c_int, c_int)
+_sig("ct_aroonosc", P, P, P, c_int, c_int)
+_sig("ct_atr_batch", P, P, P, P, c_int, c_int, c_int, c_float, c_int)
+_sig("ct_macd_batch", P, P, P, P, c_int, c_int, c_int, c_int, c_int, c_int, c_int)
+_sig("ct_bbands_batch", P, P, P, P, c_int, c_int, c_int, c_float, c_float, c_int)
+_sig("ct_stochastic_batch", P, P, P, P, P, c_int, c_int, c_int, c_int, c_int)
+_sig("ct_kama_batch", P, P, c_int, c_int, c_int, c_int, c_int, c_int)
+_sig("ct_t3_batch", P, P, c_int, c_int, c_int, c_float, c_int)
+_sig("ct_trange_batch", P, P, P, P, c_int, c_int, c_int)
+_sig("ct_aroonosc_batch", P, P, P, c_int, c_int, c_int, c_int)
+_sig("ct_aroon_batch", P, P, P, P, c_int, c_int, c_int, c_int)
+_sig("ct_plus_dm_batch", P, P, P, c_int, c_int, c_int, c_int)
+_sig("ct_minus_dm_batch", P, P, P, c_int, c_int, c_int, c_int)
+# device aliases and the moving-average selector (include/tacuda.h)
+for _n in ("sma", "ema", "wma", "dema", "tema", "rsi", "stddev", "var", "sum", "max", "min"):
+    _sig(f"ct_{_n}_device", P, P, c_int, c_int)
+_sig("ct_t3_device", P, P, c_int, c_int, c_float)
+_sig("ct_kama_device", P, P, c_int, c_int, c_int, c_int)
+_sig("ct_ma_device", P, P, c_int, c_int, c_int)
+# typed variants: storage type, compute type (include/tacuda.h, tacuda_device.h)
+for _n in ("sma", "ema", "rsi"):
+    _sig(f"ct_{_n}_ex", P, P, c_int, c_int, c_int, c_int)
+    _sig(f"ct_{_n}_ex_d", P, P, c_int, c_int, c_int, c_int)
+for _sfx in ("_ex", "_ex_d"):
+    _sig(f"ct_atr{_sfx}", P, P, P, P, c_int, c_int, c_float, c_int, c_int)
+    _sig(f"ct_macd{_sfx}", P, P, P, P, c_int, c_int, c_int, c_int, c_int, c_int, c_int)
+    _sig(f"ct_bbands{_sfx}", P, P, P, P, c_int, c_int, c_float, c_float, c_int, c_int)
+_sig("ct_device_upload_ex", P, P, c_int, c_int)
+_sig("ct_device_download_ex", P, P, c_int, c_int)
+_sig("ctx_host_staging", c_int, stream=False)
+_sig("ct_device_buffer_pool_cleanup", stream=False, restype=None)
+_sig("ct_device_buffer_pool_allocation_count", stream=False, restype=ctypes.c_size_t)
+_sig("ct_device_buffer_pool_set_max_cache_bytes", ctypes.c_size_t, stream=False,
+     restype=ctypes.c_size_t)
 # memory / errors
 _sig("ct_device_alloc", c_int, stream=False, restype=c_void_p)
 _sig("ct_device_free", c_void_p, stream=False, restype=None)
@@ -185,6 +232,17 @@ _sig("ctx_cdl_count", stream=False)
 _sig("ctx_cdl_name", c_int, stream=False, restype=ctypes.c_char_p)
 _sig("ctx_cdl_batch_d", c_int, P, P, P, P, P, c_int, c_int)
 _sig("ctx_cdl_all_batch_d", P, P, P, P, P, c_int, c_int)
+
+
+def _cdl_sigs(names):
+    for n in names:
+        _sig(f"ct_cdl_{n}_d", P, P, P, P, P, c_int)
+        _sig(f"ct_cdl_{n}", P, P, P, P, P, c_int)
+        _sig(f"ct_cdl_{n}_batch_d", P, P, P, P, P, c_int, c_int, c_int)
+        _sig(f"ct_cdl_{n}_batch", P, P, P, P, P, c_int, c_int, c_int)
+
+
+_cdl_sigs(("bullish_engulfing", "bearish_engulfing"))
 # differentiable indicators (include/tacuda_amd.h)
 _D = ctypes.c_double
 for _n in ("ema", "rsi"):
@@ -209,6 +267,10 @@ def lib() -> ctypes.CDLL:
                 f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
                 "(tacuda_amd has no CPU fallback)")
         L = ctypes.CDLL(LIB_PATH)
+        fn = L.ctx_cdl_name
+        fn.restype = ctypes.c_char_p
+        fn.argtypes = [c_int]
+        _cdl_sigs([L.ctx_cdl_name(k).decode() for k in range(L.ctx_cdl_count())])
         for name, (args, res) in _SIGS.items():
             fn = getattr(L, name)
             fn.argtypes = args

@@ -10,20 +10,16 @@
 
 #include "../../include/tacuda_device.h"
 #include "host.hpp"
+#include "hostio.hpp"
+#include "ws_registry.hpp"
 
 namespace tca {
 
 namespace {
 thread_local std::string g_error;
 
-struct WsEntry {
-  void* base = nullptr;
-  size_t bytes = 0;
-  int device = -1;
-};
 std::mutex g_ws_mu;
-std::map<std::tuple<int, hipStream_t, int>, WsEntry> g_ws;
-size_t g_ws_allocs = 0;
+WsRegistry g_ws;
 }  // namespace
 
 thread_local const Ragged* t_ragged = nullptr;
@@ -71,8 +67,9 @@ ctStatus_t workspace_for(hipStream_t stream, size_t treeBytes, Ws* out, int part
   int dev = 0;
   TCA_HIP(hipGetDevice(&dev));
   std::lock_guard<std::mutex> lk(g_ws_mu);
-  WsEntry& w = g_ws[std::make_tuple(dev, stream, part)];
-  if (w.base == nullptr || w.bytes < treeBytes) {
+  const WsRegistry::Key key{dev, stream, part};
+  void* base = g_ws.find(key, treeBytes);
+  if (base == nullptr) {
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (stream != nullptr) TCA_HIP(hipStreamIsCapturing(stream, &cs));
     if (cs != hipStreamCaptureStatusNone)
@@ -80,57 +77,60 @@ ctStatus_t workspace_for(hipStream_t stream, size_t treeBytes, Ws* out, int part
                   "look-back workspace of %zu bytes is not reserved for this stream; run the "
                   "call once eagerly (or call ctx_workspace_reserve) before capturing",
                   treeBytes);
-    size_t want = treeBytes < (1u << 20) ? (1u << 20) : treeBytes;
-    if (w.base) {
-      TCA_HIP(hipStreamSynchronize(stream));
-      TCA_HIP(hipFree(w.base));
-      w.base = nullptr;
-    }
+    // The buffer this replaces stays allocated (retired): a graph captured
+    // on this stream may still hold its address.
+    const size_t want = g_ws.grow_size(key, treeBytes);
     const size_t bytes = sizeof(WsHeader) + want;
-    TCA_HIP(hipMalloc(&w.base, bytes));
-    ++g_ws_allocs;
-    TCA_HIP(hipMemset(w.base, 0, bytes));
+    TCA_HIP(hipMalloc(&base, bytes));
+    hipError_t e = hipMemsetAsync(base, 0, bytes, stream);
     WsHeader h{};
     h.epoch = 1;
-    TCA_HIP(hipMemcpy(w.base, &h, sizeof h, hipMemcpyHostToDevice));
-    w.bytes = want;
-    w.device = dev;
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(base, &h, sizeof h, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) {
+      (void)hipFree(base);
+      return hip_fail(e, "workspace init");
+    }
+    g_ws.install(key, base, want);
   }
-  out->hdr = static_cast<WsHeader*>(w.base);
-  out->area = reinterpret_cast<double*>(static_cast<char*>(w.base) + sizeof(WsHeader));
+  out->hdr = static_cast<WsHeader*>(base);
+  out->area = reinterpret_cast<double*>(static_cast<char*>(base) + sizeof(WsHeader));
   return CT_STATUS_SUCCESS;
 }
 
 void workspace_release(hipStream_t stream) {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return;
-  std::lock_guard<std::mutex> lk(g_ws_mu);
-  for (int part = 0; part < kWsParts; ++part) {
-    auto it = g_ws.find(std::make_tuple(dev, stream, part));
-    if (it == g_ws.end()) continue;
-    if (it->second.base) {
-      (void)hipStreamSynchronize(stream);
-      (void)hipFree(it->second.base);
-    }
-    g_ws.erase(it);
+  std::vector<WsRegistry::Buf> bufs;
+  {
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    bufs = g_ws.release_stream(dev, stream);
   }
+  if (bufs.empty()) return;
+  (void)hipStreamSynchronize(stream);
+  for (const auto& b : bufs) (void)hipFree(b.base);
 }
 
 void workspace_release_all() {
-  std::lock_guard<std::mutex> lk(g_ws_mu);
-  for (auto& kv : g_ws) {
-    if (kv.second.base) {
-      (void)hipSetDevice(kv.second.device);
-      (void)hipDeviceSynchronize();
-      (void)hipFree(kv.second.base);
-    }
+  std::vector<WsRegistry::Buf> bufs;
+  {
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    bufs = g_ws.release_all();
   }
-  g_ws.clear();
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  for (const auto& b : bufs) {
+    (void)hipSetDevice(b.device);
+    (void)hipDeviceSynchronize();
+    (void)hipFree(b.base);
+  }
+  (void)hipSetDevice(cur);
 }
 
 size_t workspace_allocations() {
   std::lock_guard<std::mutex> lk(g_ws_mu);
-  return g_ws_allocs;
+  return g_ws.allocations();
 }
 
 }  // namespace tca
@@ -307,8 +307,18 @@ CTAPI_EXPORT ctStatus_t ctx_kernel_times(char* names, int namesLen, float* ms, i
   });
 }
 
-CTAPI_EXPORT void ct_device_buffer_pool_cleanup(void) { workspace_release_all(); }
-CTAPI_EXPORT size_t ct_device_buffer_pool_allocation_count(void) { return workspace_allocations(); }
-CTAPI_EXPORT size_t ct_device_buffer_pool_set_max_cache_bytes(size_t bytes) { return bytes; }
+// The reference's buffer-pool controls (tacuda.h:824-827) cover this
+// build's two caches: the host calls' device buffers and pinned staging, and
+// the per-stream look-back workspaces.
+CTAPI_EXPORT void ct_device_buffer_pool_cleanup(void) {
+  workspace_release_all();
+  pool_cleanup();
+}
+CTAPI_EXPORT size_t ct_device_buffer_pool_allocation_count(void) {
+  return workspace_allocations() + pool_allocations();
+}
+CTAPI_EXPORT size_t ct_device_buffer_pool_set_max_cache_bytes(size_t bytes) {
+  return pool_set_max_cache(bytes);
+}
 
 }  // extern "C"

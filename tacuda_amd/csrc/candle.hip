@@ -14,6 +14,7 @@
 
 #include "../../include/tacuda_amd.h"
 #include "host.hpp"
+#include "hostio.hpp"
 
 namespace tca {
 namespace cdl {
@@ -217,6 +218,60 @@ ctStatus_t launch(const float* o, const float* h, const float* l, const float* c
   return CT_STATUS_SUCCESS;
 }
 
+// Legacy engulfing signal (reference CandleUtils.h:233-260 as used by
+// BullishEngulfing.cu:10-23 and BearishEngulfing.cu): TA-Lib's engulfing
+// rule on bars t-1, t, written from bar 1 of every row (0 at bar 0).
+__global__ void k_engulf_legacy(const float* __restrict__ o, const float* __restrict__ c,
+                                float* __restrict__ out, int n, long long total) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  if (i % n == 0) {
+    out[i] = 0.0f;
+    return;
+  }
+  const float co = o[i], cc = c[i], po = o[i - 1], pc = c[i - 1];
+  const int cur = cc >= co ? 1 : -1, prev = pc >= po ? 1 : -1;
+  const bool bull = cur == 1 && prev == -1 && ((cc >= po && co < pc) || (cc > po && co <= pc));
+  const bool bear = cur == -1 && prev == 1 && ((co >= pc && cc < po) || (co > pc && cc <= po));
+  out[i] = (bull || bear) ? (float)(cur * ((co != pc && cc != po) ? 100 : 80)) : 0.0f;
+}
+
+ctStatus_t engulf_legacy(const float* o, const float* h, const float* l, const float* c,
+                         float* out, int n, int rows, hipStream_t s, const char* fn) {
+  if (!o || !h || !l || !c || !out) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: null pointer", fn);
+  if (n <= 0 || rows <= 0)
+    return fail(CT_STATUS_INVALID_ARGUMENT, "%s: size/seriesLen and numSeries must be positive", fn);
+  const long long total = (long long)n * rows;
+  const long long blocks = (total + 255) / 256;
+  if (blocks > 0x7fffffffLL) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: matrix too large", fn);
+  hipLaunchKernelGGL(k_engulf_legacy, dim3((unsigned)blocks), dim3(256), 0, s, o, c, out, n, total);
+  TCA_HIP(hipGetLastError());
+  return CT_STATUS_SUCCESS;
+}
+
+// Host form of a float-signal call: mask selects one pattern of the list, 0
+// the legacy engulfing alias.
+ctStatus_t host_run(const float* ho, const float* hh, const float* hl, const float* hc, float* hy,
+                    int n, int rows, unsigned long long mask, hipStream_t s, const char* fn) {
+  if (!ho || !hh || !hl || !hc || !hy) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: null pointer", fn);
+  if (n <= 0 || rows <= 0)
+    return fail(CT_STATUS_INVALID_ARGUMENT, "%s: size/seriesLen and numSeries must be positive", fn);
+  const size_t e = (size_t)n * rows;
+  HostRun r(s);
+  const float *o, *h, *l, *c;
+  float* y;
+  ctStatus_t st = r.in(ho, e, &o);
+  if (st == CT_STATUS_SUCCESS) st = r.in(hh, e, &h);
+  if (st == CT_STATUS_SUCCESS) st = r.in(hl, e, &l);
+  if (st == CT_STATUS_SUCCESS) st = r.in(hc, e, &c);
+  if (st == CT_STATUS_SUCCESS) st = r.out(hy, e, &y);
+  if (st == CT_STATUS_SUCCESS)
+    st = mask ? launch(o, h, l, c, y, 0, mask, n, rows, false, s, fn)
+              : engulf_legacy(o, h, l, c, y, n, rows, s, fn);
+  if (st == CT_STATUS_SUCCESS) st = r.finish();
+  return st;
+}
+
 }  // namespace cdl
 }  // namespace tca
 
@@ -277,5 +332,65 @@ CTAPI_EXPORT ctStatus_t ctx_cdl_all_batch_d(const float* o, const float* h, cons
   }
 TCA_CDL_LIST(TCA_ABI)
 #undef TCA_ABI
+
+// Host forms ct_cdl_<name> and ct_cdl_<name>_batch (tacuda.h:432+,
+// :1870+): staged through the host-call buffers, the float signals of the
+// device call.
+#define TCA_HOST(name, lb, fn)                                                                  \
+  CTAPI_EXPORT ctStatus_t ct_cdl_##name(const float* ho, const float* hh, const float* hl,     \
+                                        const float* hc, float* hy, int size, hipStream_t s) { \
+    return guarded([&]() -> ctStatus_t {                                                       \
+      return cdl::host_run(ho, hh, hl, hc, hy, size, 1, 1ull << cdl::k_##name, s,              \
+                           "cdl_" #name);                                                      \
+    });                                                                                        \
+  }                                                                                            \
+  CTAPI_EXPORT ctStatus_t ct_cdl_##name##_batch(const float* ho, const float* hh,              \
+                                                const float* hl, const float* hc, float* hy,   \
+                                                int seriesLen, int numSeries, int,             \
+                                                hipStream_t s) {                               \
+    return guarded([&]() -> ctStatus_t {                                                       \
+      return cdl::host_run(ho, hh, hl, hc, hy, seriesLen, numSeries, 1ull << cdl::k_##name, s, \
+                           "cdl_" #name);                                                      \
+    });                                                                                        \
+  }
+TCA_CDL_LIST(TCA_HOST)
+#undef TCA_HOST
+
+// The reference's legacy engulfing names (tacuda.h:447-452, :1917-1952;
+// tacuda_device.h:571-582): bullish and bearish both return the signed
+// engulfing signal from bar 1 on.
+#define TCA_ENGULF(name)                                                                        \
+  CTAPI_EXPORT ctStatus_t ct_cdl_##name##_d(const float* o, const float* h, const float* l,    \
+                                            const float* c, float* out, int size,              \
+                                            hipStream_t s) {                                   \
+    return guarded([&] { return cdl::engulf_legacy(o, h, l, c, out, size, 1, s, #name); });    \
+  }                                                                                            \
+  CTAPI_EXPORT ctStatus_t ct_cdl_##name##_batch_d(const float* o, const float* h,              \
+                                                  const float* l, const float* c, float* out,  \
+                                                  int seriesLen, int numSeries, int,           \
+                                                  hipStream_t s) {                             \
+    return guarded([&]() -> ctStatus_t {                                                       \
+      ctStatus_t r = cdl::engulf_legacy(o, h, l, c, out, seriesLen, numSeries, s, #name);      \
+      if (r != CT_STATUS_SUCCESS) return r;                                                    \
+      return finish_batch(s);                                                                  \
+    });                                                                                        \
+  }                                                                                            \
+  CTAPI_EXPORT ctStatus_t ct_cdl_##name(const float* ho, const float* hh, const float* hl,     \
+                                        const float* hc, float* hy, int size, hipStream_t s) { \
+    return guarded([&]() -> ctStatus_t {                                                       \
+      return cdl::host_run(ho, hh, hl, hc, hy, size, 1, 0, s, #name);                          \
+    });                                                                                        \
+  }                                                                                            \
+  CTAPI_EXPORT ctStatus_t ct_cdl_##name##_batch(const float* ho, const float* hh,              \
+                                                const float* hl, const float* hc, float* hy,   \
+                                                int seriesLen, int numSeries, int,             \
+                                                hipStream_t s) {                               \
+    return guarded([&]() -> ctStatus_t {                                                       \
+      return cdl::host_run(ho, hh, hl, hc, hy, seriesLen, numSeries, 0, s, #name);             \
+    });                                                                                        \
+  }
+TCA_ENGULF(bullish_engulfing)
+TCA_ENGULF(bearish_engulfing)
+#undef TCA_ENGULF
 
 }  // extern "C"

@@ -1559,8 +1559,10 @@ struct FTrange {
 };
 
 // STOCH (ta_STOCH.c:36-322): fastK over kP, SMA(dP) twice; both outputs
-// start at kP-1 + 2(dP-1).  fastK values of the lane's bars and the
-// 2(dP-1) before them are kept in registers (dP <= kStochMaxD).
+// start at kP-1 + 2(dP-1).  For dP <= kStochMaxD the fastK values of the
+// lane's bars and the 2(dP-1) before them are kept in registers; longer
+// smoothing re-evaluates them per output (any dP > 0, as the reference's
+// StochasticFused.cu:184-220 accepts).
 constexpr int kStochMaxD = 9;
 struct FStoch {
   static constexpr bool kScan = false;
@@ -1583,20 +1585,44 @@ struct FStoch {
     const int kp = pr.kp, dp = pr.dp;
     const int lb = kp - 1 + 2 * (dp - 1);
     const int ex = 2 * (dp - 1);
+    auto fast_k = [&](int t) -> double {
+      float hh, lw;
+      window_hl<T>(lh, ll, io, c, t, kp, hh, lw);
+      const double range = (double)hh - (double)lw;
+      const double cl = (double)at<T>(lc, io.in[2], c, t);
+      return (fabs(range) <= 1e-14 * (fabs((double)hh) + fabs((double)lw)))
+                 ? 0.0
+                 : (cl - (double)lw) / range * 100.0;
+    };
+    if (dp > kStochMaxD) {
+      // long smoothing: every output re-evaluates the fast %K windows it
+      // averages, each sum in ascending bar order (the oracle's doubles)
+      float ok[T::K], od[T::K];
+      for (int i = 0; i < T::K; ++i) {
+        const int t = a + i;
+        ok[i] = od[i] = qnanf_();
+        if (t < lb || t >= c.n) continue;
+        double sd = 0.0, skt = 0.0;
+        for (int q = dp - 1; q >= 0; --q) {
+          double sum = 0.0;
+          for (int r = dp - 1; r >= 0; --r) sum += fast_k(t - q - r);
+          const double sk = sum / dp;
+          sd += sk;
+          skt = sk;
+        }
+        ok[i] = (float)skt;
+        od[i] = (float)(sd / dp);
+      }
+      store_lane<T>(lo, ok, io.out[0], c);
+      store_lane<T>(lo, od, io.out[1], c);
+      return;
+    }
     double fk[T::K + EX];
 #pragma unroll
     for (int j = 0; j < T::K + EX; ++j) {
       const int t = a - EX + j;
       fk[j] = qnan_();
-      if (j >= EX - ex && t >= kp - 1 && t < c.n) {
-        float hh, lw;
-        window_hl<T>(lh, ll, io, c, t, kp, hh, lw);
-        const double range = (double)hh - (double)lw;
-        const double cl = (double)at<T>(lc, io.in[2], c, t);
-        fk[j] = (fabs(range) <= 1e-14 * (fabs((double)hh) + fabs((double)lw)))
-                    ? 0.0
-                    : (cl - (double)lw) / range * 100.0;
-      }
+      if (j >= EX - ex && t >= kp - 1 && t < c.n) fk[j] = fast_k(t);
     }
     double sk[T::K + EX];
 #pragma unroll

@@ -49,7 +49,9 @@ ctStatus_t guarded(F&& body) {
 // part so that launches ordered on one stream share it safely; one launch
 // whose blocks run two independent look-back scans gives each its own part
 // (kWsParts per stream).  Growing it is refused while the stream is being
-// captured (the graph would keep a stale pointer).
+// captured; growth outside capture retires the old buffer instead of freeing
+// it (graphs captured earlier keep its address), and retired buffers are
+// freed when the stream's workspaces are released or at pool cleanup.
 constexpr int kWsParts = 2;
 ctStatus_t workspace_for(hipStream_t stream, size_t treeBytes, Ws* out, int part = 0);
 void workspace_release_all();

@@ -2384,8 +2384,6 @@ ctStatus_t pack_validate(const ctxPackParams* p, const ctxPackOutputs* o, PackP&
     P.mSlow = 3;
     P.mSig = 2;
   }
-  if ((P.mask & ((1u << kOStK) | (1u << kOStD))) && P.stD > kStochMaxD)
-    return fail(CT_STATUS_UNSUPPORTED, "pack: STOCH dPeriod above %d is not provided", kStochMaxD);
   if (window_halo(P) > kPackMaxWindow)
     return fail(CT_STATUS_UNSUPPORTED, "pack: window look-back above %d bars is not provided",
                 kPackMaxWindow);
@@ -2479,6 +2477,10 @@ struct ctxPlan_st {
   hipStream_t stream = nullptr;  // private stream the graph was captured on
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // Recorded after every replay; the next replay's stream waits on it, so
+  // replays on different streams never share the plan's workspace at once.
+  hipEvent_t done = nullptr;
+  std::mutex mu;
   PackTiles tiles{{4, 4}, {4, 4}, 4, 1};
   unsigned mask = 0;
   int runK = 4, runW = 4;  // tile of the launch that runs (the close kernel's if several)
@@ -2551,8 +2553,10 @@ ctStatus_t tune_part(int len, int rows, hipStream_t st, const PackP& P, const Pa
 void plan_free(ctxPlan p) {
   if (!p) return;
   if (p->stream) (void)hipStreamSynchronize(p->stream);
+  if (p->done) (void)hipEventSynchronize(p->done);
   if (p->exec) (void)hipGraphExecDestroy(p->exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
+  if (p->done) (void)hipEventDestroy(p->done);
   if (p->stream) {
     workspace_release(p->stream);
     (void)hipStreamDestroy(p->stream);
@@ -2662,6 +2666,9 @@ CTAPI_EXPORT ctStatus_t ctx_pack_plan_create(const float* h, const float* l, con
     if (e != hipSuccess) return bail(hip_fail(e, "plan: end capture"));
     e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
     if (e != hipSuccess) return bail(hip_fail(e, "plan: instantiate"));
+    e = hipEventCreateWithFlags(&p->done, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventRecord(p->done, p->stream);
+    if (e != hipSuccess) return bail(hip_fail(e, "plan: event"));
     *out = p;
     return CT_STATUS_SUCCESS;
   });
@@ -2670,7 +2677,18 @@ CTAPI_EXPORT ctStatus_t ctx_pack_plan_create(const float* h, const float* l, con
 CTAPI_EXPORT ctStatus_t ctx_plan_launch(ctxPlan plan, hipStream_t stream) {
   return guarded([&]() -> ctStatus_t {
     if (!plan || !plan->exec) return fail(CT_STATUS_INVALID_ARGUMENT, "plan: null plan");
+    // The graph's look-back slots, epoch and carries live in the plan's
+    // workspace: a replay starts after the previous one ended, whatever
+    // stream either was launched on.
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (stream != nullptr) TCA_HIP(hipStreamIsCapturing(stream, &cs));
+    if (cs != hipStreamCaptureStatusNone)
+      return fail(CT_STATUS_UNSUPPORTED,
+                  "plan: a plan is a captured graph already; launch it outside stream capture");
+    std::lock_guard<std::mutex> lk(plan->mu);
+    TCA_HIP(hipStreamWaitEvent(stream, plan->done, 0));
     TCA_HIP(hipGraphLaunch(plan->exec, stream));
+    TCA_HIP(hipEventRecord(plan->done, stream));
     return finish_batch(stream);
   });
 }

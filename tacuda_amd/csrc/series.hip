@@ -442,8 +442,6 @@ ctStatus_t dev_stoch(const float* h, const float* l, const float* c, float* k, f
   CHECK_PTR(k, "stochastic"); CHECK_PTR(d, "stochastic");
   CHECK_RET(check_shape("stochastic", len, rows));
   if (kp <= 0 || dp <= 0) return bad("stochastic", "invalid period");
-  if (dp > kStochMaxD)
-    return fail(CT_STATUS_UNSUPPORTED, "stochastic: dPeriod above %d is not provided", kStochMaxD);
   {
     ctxPackParams pp = pipe_params();
     pp.stoch_k = kp;
@@ -521,49 +519,6 @@ DIR_DEV(adx, run_Adx, true)
 #undef DIR_DEV
 
 }  // namespace tca
-
-// ------------------------------------------------------ host staging --
-namespace {
-
-// Device staging for the host entry points: upload the inputs, run, download
-// the outputs, synchronize (reference api.cpp:230-287).
-struct HostRun {
-  hipStream_t s;
-  std::vector<void*> bufs;
-  explicit HostRun(hipStream_t st) : s(st) {}
-  ~HostRun() {
-    for (void* b : bufs) (void)hipFree(b);
-  }
-  ctStatus_t in(const float* h, size_t n, const float** d) {
-    if (!h) return bad("host call", "null pointer");
-    void* p = nullptr;
-    TCA_HIP(hipMalloc(&p, n * 4));
-    bufs.push_back(p);
-    TCA_HIP(hipMemcpyAsync(p, h, n * 4, hipMemcpyHostToDevice, s));
-    *d = static_cast<const float*>(p);
-    return CT_STATUS_SUCCESS;
-  }
-  ctStatus_t out(float* h, size_t n, float** d) {
-    if (!h) return bad("host call", "null pointer");
-    void* p = nullptr;
-    TCA_HIP(hipMalloc(&p, n * 4));
-    bufs.push_back(p);
-    *d = static_cast<float*>(p);
-    return CT_STATUS_SUCCESS;
-  }
-  ctStatus_t back(float* h, const float* d, size_t n) {
-    TCA_HIP(hipMemcpyAsync(h, d, n * 4, hipMemcpyDeviceToHost, s));
-    return CT_STATUS_SUCCESS;
-  }
-  ctStatus_t sync() {
-    TCA_HIP(hipStreamSynchronize(s));
-    return CT_STATUS_SUCCESS;
-  }
-};
-
-size_t elems(int len, int rows) { return (size_t)(len > 0 ? len : 0) * (rows > 0 ? rows : 0); }
-
-}  // namespace
 
 #define R_(x) CHECK_RET(x)
 
@@ -656,300 +611,6 @@ CTAPI_EXPORT ctStatus_t ct_plus_dm_batch_d(const float* h, const float* l, float
 }
 CTAPI_EXPORT ctStatus_t ct_minus_dm_batch_d(const float* h, const float* l, float* y, int len, int rows, int p, int, hipStream_t s) {
   return guarded([&]() -> ctStatus_t { R_(dev_minus_dm(h, l, nullptr, y, len, rows, p, s)); return finish_batch(s); });
-}
-
-// ----------------------------------------------------- host single + batch --
-// Host variants upload, run the device path and download synchronously.
-#define H1(NAME, DEVCALL)                                                                          \
-  CTAPI_EXPORT ctStatus_t ct_##NAME(const float* hx, float* hy, int n, int p, hipStream_t s) {     \
-    return guarded([&]() -> ctStatus_t {                                                           \
-      if (n <= 0) return bad(#NAME, "size must be positive");                                      \
-      HostRun r(s);                                                                                \
-      const float* x;                                                                              \
-      float* y;                                                                                    \
-      R_(r.in(hx, n, &x));                                                                         \
-      R_(r.out(hy, n, &y));                                                                        \
-      R_(DEVCALL);                                                                                 \
-      R_(r.back(hy, y, n));                                                                        \
-      return r.sync();                                                                             \
-    });                                                                                            \
-  }
-H1(sma, dev_sma(x, y, n, 1, p, s))
-H1(wma, dev_wma(x, y, n, 1, p, s))
-H1(ema, dev_ema(x, y, n, 1, p, s))
-H1(dema, dev_dema(x, y, n, 1, p, s))
-H1(tema, dev_tema(x, y, n, 1, p, s))
-H1(max, dev_max(x, y, n, 1, p, s))
-H1(min, dev_min(x, y, n, 1, p, s))
-H1(stddev, dev_stddev(x, y, n, 1, p, s))
-H1(var, dev_var(x, y, n, 1, p, s))
-H1(sum, dev_sum(x, y, n, 1, p, s))
-H1(rsi, dev_rsi(x, y, n, 1, p, s))
-#undef H1
-
-CTAPI_EXPORT ctStatus_t ct_t3(const float* hx, float* hy, int n, int p, float v, hipStream_t s) {
-  return guarded([&]() -> ctStatus_t {
-    if (n <= 0) return bad("t3", "size must be positive");
-    HostRun r(s); const float* x; float* y;
-    R_(r.in(hx, n, &x)); R_(r.out(hy, n, &y)); R_(dev_t3(x, y, n, 1, p, v, s)); R_(r.back(hy, y, n));
-    return r.sync();
-  });
-}
-CTAPI_EXPORT ctStatus_t ct_kama(const float* hx, float* hy, int n, int p, int fast, int slow, hipStream_t s) {
-  return guarded([&]() -> ctStatus_t {
-    if (n <= 0) return bad("kama", "size must be positive");
-    HostRun r(s); const float* x; float* y;
-    R_(r.in(hx, n, &x)); R_(r.out(hy, n, &y)); R_(dev_kama(x, y, n, 1, p, fast, slow, s)); R_(r.back(hy, y, n));
-    return r.sync();
-  });
-}
-CTAPI_EXPORT ctStatus_t ct_macd(const float* hx, float* hm, float* hs, float* hh, int n, int fast, int slow, int sig, ctMaType_t t, hipStream_t s) {
-  return guarded([&]() -> ctStatus_t {
-    if (n <= 0) return bad("macd", "size must be positive");
-    HostRun r(s); const float* x; float *m, *g, *h;
-    R_(r.in(hx, n, &x)); R_(r.out(hm, n, &m)); R_(r.out(hs, n, &g)); R_(r.out(hh, n, &h));
-    R_(dev_macd(x, m, g, h, n, 1, fast, slow, sig, (int)t, s));
-    R_(r.back(hm, m, n)); R_(r.back(hs, g, n)); R_(r.back(hh, h, n));
-    return r.sync();
-  });
-}
-CTAPI_EXPORT ctStatus_t ct_bbands(const float* hx, float* hu, float* hm, float* hl, int n, int p, float up, float dn, hipStream_t s) {
-  return guarded([&]() -> ctStatus_t {
-    if (n <= 0) return bad("bbands", "size must be positive");
-    HostRun r(s); const float* x; float *u, *m, *l;
-    R_(r.in(hx, n, &x)); R_(r.out(hu, n, &u)); R_(r.out(hm, n, &m)); R_(r.out(hl, n, &l));
-    R_(dev_bbands(x, u, m, l, n, 1, p, up, dn, s));
-    R_(r.back(hu, u, n)); R_(r.back(hm, m, n)); R_(r.back(hl, l, n));
-    return r.sync();
-  });
-}
-
-#define HHLC(NAME, DEVCALL)                                                                        \
-  CTAPI_EXPORT ctStatus_t ct_##NAME(const float* hh, const float* hl, const float* hc, float* hy,  \
-                                    int n, int p, hipStream_t s) {                                 \
-    return guarded([&]() -> ctStatus_t {                                                           \
-      if (n <= 0) return bad(#NAME, "size must be positive");                                      \
-      HostRun r(s);                                                                                \
-      const float *h, *l, *c;                                                                      \
-      float* y;                                                                                    \
-      R_(r.in(hh, n, &h)); R_(r.in(hl, n, &l)); R_(r.in(hc, n, &c)); R_(r.out(hy, n, &y));         \
-      R_(DEVCALL);                                                                                 \
-      R_(r.back(hy, y, n));                                                                        \
-      return r.sync();                                                                             \
-    });                                                                                            \
-  }
-HHLC(natr, dev_natr(h, l, c, y, n, 1, p, s))
-HHLC(cci, dev_cci(h, l, c, y, n, 1, p, s))
-HHLC(adx, dev_adx(h, l, c, y, n, 1, p, s))
-HHLC(dx, dev_dx(h, l, c, y, n, 1, p, s))
-HHLC(plus_di, dev_plus_di(h, l, c, y, n, 1, p, s))
-HHLC(minus_di, dev_minus_di(h, l, c, y, n, 1, p, s))
-HHLC(willr, dev_willr(h, l, c, y, n, 1, p, s))
-#undef HHLC
-
-CTAPI_EXPORT ctStatus_t ct_atr(const float* hh, const float* hl, const float* hc, float* hy, int n, int p, float, hipStream_t s) {
-  return guarded([&]() -> ctStatus_t {
-    if (n <= 0) return bad("atr", "size must be positive");
-    HostRun r(s); const float *h, *l, *c; float* y;
-    R_(r.in(hh, n, &h)); R_(r.in(hl, n, &l)); R_(r.in(hc, n, &c)); R_(r.out(hy, n, &y));
-    R_(dev_atr(h, l, c, y, n, 1, p, s)); R_(r.back(hy, y, n));
-    return r.sync();
-  });
-}
-CTAPI_EXPORT ctStatus_t ct_trange(const float* hh, const float* hl, const float* hc, float* hy, int n, hipStream_t s) {
-  return guarded([&]() -> ctStatus_t {
-    if (n <= 0) return bad("trange", "size must be positive");
-    HostRun r(s); const float *h, *l, *c; float* y;
-    R_(r.in(hh, n, &h)); R_(r.in(hl, n, &l)); R_(r.in(hc, n, &c)); R_(r.out(hy, n, &y));
-    R_(dev_trange(h, l, c, y, n, 1, s)); R_(r.back(hy, y, n));
-    return r.sync();
-  });
-}
-CTAPI_EXPORT ctStatus_t ct_stochastic(const float* hh, const float* hl, const float* hc, float* hk, float* hd, int n, int kp, int dp, hipStream_t s) {
-  return guarded([&]() -> ctStatus_t {
-    if (n <= 0) return bad("stochastic", "size must be positive");
-    HostRun r(s); const float *h, *l, *c; float *k, *d;
-    R_(r.in(hh, n, &h)); R_(r.in(hl, n, &l)); R_(r.in(hc, n, &c)); R_(r.out(hk, n, &k)); R_(r.out(hd, n, &d));
-    R_(dev_stoch(h, l, c, k, d, n, 1, kp, dp, s)); R_(r.back(hk, k, n)); R_(r.back(hd, d, n));
-    return r.sync();
-  });
-}
-CTAPI_EXPORT ctStatus_t ct_plus_dm(const float* hh, const float* hl, float* hy, int n, int p, hipStream_t s) {
-  return guarded([&]() -> ctStatus_t {
-    if (n <= 0) return bad("plus_dm", "size must be positive");
-    HostRun r(s); const float *h, *l; float* y;
-    R_(r.in(hh, n, &h)); R_(r.in(hl, n, &l)); R_(r.out(hy, n, &y));
-    R_(dev_plus_dm(h, l, nullptr, y, n, 1, p, s)); R_(r.back(hy, y, n));
-    return r.sync();
-  });
-}
-CTAPI_EXPORT ctStatus_t ct_minus_dm(const float* hh, const float* hl, float* hy, int n, int p, hipStream_t s) {
-  return guarded([&]() -> ctStatus_t {
-    if (n <= 0) return bad("minus_dm", "size must be positive");
-    HostRun r(s); const float *h, *l; float* y;
-    R_(r.in(hh, n, &h)); R_(r.in(hl, n, &l)); R_(r.out(hy, n, &y));
-    R_(dev_minus_dm(h, l, nullptr, y, n, 1, p, s)); R_(r.back(hy, y, n));
-    return r.sync();
-  });
-}
-CTAPI_EXPORT ctStatus_t ct_aroonosc(const float* hh, const float* hl, float* hy, int n, int p, hipStream_t s) {
-  return guarded([&]() -> ctStatus_t {
-    if (n <= 0) return bad("aroonosc", "size must be positive");
-    HostRun r(s); const float *h, *l; float* y;
-    R_(r.in(hh, n, &h)); R_(r.in(hl, n, &l)); R_(r.out(hy, n, &y));
-    R_(dev_aroonosc(h, l, y, n, 1, p, s)); R_(r.back(hy, y, n));
-    return r.sync();
-  });
-}
-// Host AROON keeps the reference's host order (up, down, osc) and two periods
-// (tacuda.h:385-388); the oscillator uses the up period's window.
-CTAPI_EXPORT ctStatus_t ct_aroon(const float* hh, const float* hl, float* hup, float* hdn, float* hosc, int n, int upP, int dnP, hipStream_t s) {
-  return guarded([&]() -> ctStatus_t {
-    if (n <= 0) return bad("aroon", "size must be positive");
-    if (upP != dnP) return fail(CT_STATUS_UNSUPPORTED, "aroon: distinct up/down periods are not provided");
-    HostRun r(s); const float *h, *l; float *u, *d, *o;
-    R_(r.in(hh, n, &h)); R_(r.in(hl, n, &l)); R_(r.out(hup, n, &u)); R_(r.out(hdn, n, &d)); R_(r.out(hosc, n, &o));
-    R_(dev_aroon(h, l, d, u, n, 1, upP, s)); R_(dev_aroonosc(h, l, o, n, 1, upP, s));
-    R_(r.back(hup, u, n)); R_(r.back(hdn, d, n)); R_(r.back(hosc, o, n));
-    return r.sync();
-  });
-}
-
-#define HB1(NAME)                                                                                   \
-  CTAPI_EXPORT ctStatus_t ct_##NAME##_batch(const float* hx, float* hy, int len, int rows, int p,   \
-                                            int, hipStream_t s) {                                  \
-    return guarded([&]() -> ctStatus_t {                                                            \
-      R_(check_shape(#NAME, len, rows));                                                            \
-      const size_t n = elems(len, rows);                                                            \
-      HostRun r(s);                                                                                 \
-      const float* x;                                                                               \
-      float* y;                                                                                     \
-      R_(r.in(hx, n, &x)); R_(r.out(hy, n, &y));                                                    \
-      R_(dev_##NAME(x, y, len, rows, p, s));                                                        \
-      R_(r.back(hy, y, n));                                                                         \
-      return r.sync();                                                                              \
-    });                                                                                             \
-  }
-HB1(sma) HB1(ema) HB1(rsi) HB1(wma) HB1(dema) HB1(tema) HB1(stddev) HB1(var) HB1(sum) HB1(max) HB1(min)
-#undef HB1
-
-#define HBHLC(NAME)                                                                                 \
-  CTAPI_EXPORT ctStatus_t ct_##NAME##_batch(const float* hh, const float* hl, const float* hc,      \
-                                            float* hy, int len, int rows, int p, int,              \
-                                            hipStream_t s) {                                       \
-    return guarded([&]() -> ctStatus_t {                                                            \
-      R_(check_shape(#NAME, len, rows));                                                            \
-      const size_t n = elems(len, rows);                                                            \
-      HostRun r(s);                                                                                 \
-      const float *h, *l, *c;                                                                       \
-      float* y;                                                                                     \
-      R_(r.in(hh, n, &h)); R_(r.in(hl, n, &l)); R_(r.in(hc, n, &c)); R_(r.out(hy, n, &y));          \
-      R_(dev_##NAME(h, l, c, y, len, rows, p, s));                                                  \
-      R_(r.back(hy, y, n));                                                                         \
-      return r.sync();                                                                              \
-    });                                                                                             \
-  }
-HBHLC(natr) HBHLC(adx) HBHLC(dx) HBHLC(plus_di) HBHLC(minus_di) HBHLC(willr) HBHLC(cci)
-#undef HBHLC
-
-CTAPI_EXPORT ctStatus_t ct_atr_batch(const float* hh, const float* hl, const float* hc, float* hy, int len, int rows, int p, float, int, hipStream_t s) {
-  return guarded([&]() -> ctStatus_t {
-    R_(check_shape("atr", len, rows)); const size_t n = elems(len, rows);
-    HostRun r(s); const float *h, *l, *c; float* y;
-    R_(r.in(hh, n, &h)); R_(r.in(hl, n, &l)); R_(r.in(hc, n, &c)); R_(r.out(hy, n, &y));
-    R_(dev_atr(h, l, c, y, len, rows, p, s)); R_(r.back(hy, y, n));
-    return r.sync();
-  });
-}
-CTAPI_EXPORT ctStatus_t ct_macd_batch(const float* hx, float* hm, float* hs, float* hh, int len, int rows, int fast, int slow, int sig, int maType, int, hipStream_t s) {
-  return guarded([&]() -> ctStatus_t {
-    R_(check_shape("macd", len, rows)); const size_t n = elems(len, rows);
-    HostRun r(s); const float* x; float *m, *g, *h;
-    R_(r.in(hx, n, &x)); R_(r.out(hm, n, &m)); R_(r.out(hs, n, &g)); R_(r.out(hh, n, &h));
-    R_(dev_macd(x, m, g, h, len, rows, fast, slow, sig, maType, s));
-    R_(r.back(hm, m, n)); R_(r.back(hs, g, n)); R_(r.back(hh, h, n));
-    return r.sync();
-  });
-}
-CTAPI_EXPORT ctStatus_t ct_bbands_batch(const float* hx, float* hu, float* hm, float* hl, int len, int rows, int p, float up, float dn, int, hipStream_t s) {
-  return guarded([&]() -> ctStatus_t {
-    R_(check_shape("bbands", len, rows)); const size_t n = elems(len, rows);
-    HostRun r(s); const float* x; float *u, *m, *l;
-    R_(r.in(hx, n, &x)); R_(r.out(hu, n, &u)); R_(r.out(hm, n, &m)); R_(r.out(hl, n, &l));
-    R_(dev_bbands(x, u, m, l, len, rows, p, up, dn, s));
-    R_(r.back(hu, u, n)); R_(r.back(hm, m, n)); R_(r.back(hl, l, n));
-    return r.sync();
-  });
-}
-CTAPI_EXPORT ctStatus_t ct_stochastic_batch(const float* hh, const float* hl, const float* hc, float* hk, float* hd, int len, int rows, int kp, int dp, int, hipStream_t s) {
-  return guarded([&]() -> ctStatus_t {
-    R_(check_shape("stochastic", len, rows)); const size_t n = elems(len, rows);
-    HostRun r(s); const float *h, *l, *c; float *k, *d;
-    R_(r.in(hh, n, &h)); R_(r.in(hl, n, &l)); R_(r.in(hc, n, &c)); R_(r.out(hk, n, &k)); R_(r.out(hd, n, &d));
-    R_(dev_stoch(h, l, c, k, d, len, rows, kp, dp, s)); R_(r.back(hk, k, n)); R_(r.back(hd, d, n));
-    return r.sync();
-  });
-}
-CTAPI_EXPORT ctStatus_t ct_kama_batch(const float* hx, float* hy, int len, int rows, int p, int fast, int slow, int, hipStream_t s) {
-  return guarded([&]() -> ctStatus_t {
-    R_(check_shape("kama", len, rows)); const size_t n = elems(len, rows);
-    HostRun r(s); const float* x; float* y;
-    R_(r.in(hx, n, &x)); R_(r.out(hy, n, &y)); R_(dev_kama(x, y, len, rows, p, fast, slow, s)); R_(r.back(hy, y, n));
-    return r.sync();
-  });
-}
-CTAPI_EXPORT ctStatus_t ct_t3_batch(const float* hx, float* hy, int len, int rows, int p, float v, int, hipStream_t s) {
-  return guarded([&]() -> ctStatus_t {
-    R_(check_shape("t3", len, rows)); const size_t n = elems(len, rows);
-    HostRun r(s); const float* x; float* y;
-    R_(r.in(hx, n, &x)); R_(r.out(hy, n, &y)); R_(dev_t3(x, y, len, rows, p, v, s)); R_(r.back(hy, y, n));
-    return r.sync();
-  });
-}
-CTAPI_EXPORT ctStatus_t ct_trange_batch(const float* hh, const float* hl, const float* hc, float* hy, int len, int rows, int, hipStream_t s) {
-  return guarded([&]() -> ctStatus_t {
-    R_(check_shape("trange", len, rows)); const size_t n = elems(len, rows);
-    HostRun r(s); const float *h, *l, *c; float* y;
-    R_(r.in(hh, n, &h)); R_(r.in(hl, n, &l)); R_(r.in(hc, n, &c)); R_(r.out(hy, n, &y));
-    R_(dev_trange(h, l, c, y, len, rows, s)); R_(r.back(hy, y, n));
-    return r.sync();
-  });
-}
-CTAPI_EXPORT ctStatus_t ct_aroonosc_batch(const float* hh, const float* hl, float* hy, int len, int rows, int p, int, hipStream_t s) {
-  return guarded([&]() -> ctStatus_t {
-    R_(check_shape("aroonosc", len, rows)); const size_t n = elems(len, rows);
-    HostRun r(s); const float *h, *l; float* y;
-    R_(r.in(hh, n, &h)); R_(r.in(hl, n, &l)); R_(r.out(hy, n, &y));
-    R_(dev_aroonosc(h, l, y, len, rows, p, s)); R_(r.back(hy, y, n));
-    return r.sync();
-  });
-}
-CTAPI_EXPORT ctStatus_t ct_aroon_batch(const float* hh, const float* hl, float* hd, float* hu, int len, int rows, int p, int, hipStream_t s) {
-  return guarded([&]() -> ctStatus_t {
-    R_(check_shape("aroon", len, rows)); const size_t n = elems(len, rows);
-    HostRun r(s); const float *h, *l; float *d, *u;
-    R_(r.in(hh, n, &h)); R_(r.in(hl, n, &l)); R_(r.out(hd, n, &d)); R_(r.out(hu, n, &u));
-    R_(dev_aroon(h, l, d, u, len, rows, p, s)); R_(r.back(hd, d, n)); R_(r.back(hu, u, n));
-    return r.sync();
-  });
-}
-CTAPI_EXPORT ctStatus_t ct_plus_dm_batch(const float* hh, const float* hl, float* hy, int len, int rows, int p, int, hipStream_t s) {
-  return guarded([&]() -> ctStatus_t {
-    R_(check_shape("plus_dm", len, rows)); const size_t n = elems(len, rows);
-    HostRun r(s); const float *h, *l; float* y;
-    R_(r.in(hh, n, &h)); R_(r.in(hl, n, &l)); R_(r.out(hy, n, &y));
-    R_(dev_plus_dm(h, l, nullptr, y, len, rows, p, s)); R_(r.back(hy, y, n));
-    return r.sync();
-  });
-}
-CTAPI_EXPORT ctStatus_t ct_minus_dm_batch(const float* hh, const float* hl, float* hy, int len, int rows, int p, int, hipStream_t s) {
-  return guarded([&]() -> ctStatus_t {
-    R_(check_shape("minus_dm", len, rows)); const size_t n = elems(len, rows);
-    HostRun r(s); const float *h, *l; float* y;
-    R_(r.in(hh, n, &h)); R_(r.in(hl, n, &l)); R_(r.out(hy, n, &y));
-    R_(dev_minus_dm(h, l, nullptr, y, len, rows, p, s)); R_(r.back(hy, y, n));
-    return r.sync();
-  });
 }
 
 }  // extern "C"

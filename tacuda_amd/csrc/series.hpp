@@ -52,4 +52,6 @@ ctStatus_t dev_dx(const float* h, const float* l, const float* c, float* y, int 
                   int p, hipStream_t s);
 ctStatus_t dev_adx(const float* h, const float* l, const float* c, float* y, int len, int rows,
                    int p, hipStream_t s);
+// The moving average `type` (ctMaType_t) of the families above.
+ctStatus_t dev_ma(const float* x, float* y, int len, int rows, int p, int type, hipStream_t s);
 }  // namespace tca

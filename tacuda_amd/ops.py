@@ -24,11 +24,31 @@ def _stream() -> int:
 
 
 def _check(*ts):
+    """Inputs of one call: contiguous float32 GPU tensors of one shape (the
+    kernels index every input with the shape of the last one, so a shorter
+    high or low would be read past its end; the reference binding rejects
+    mismatched lengths too, bindings/python/tacuda/__init__.py:852,914)."""
+    for t in ts:
+        if t.dim() not in (1, 2):
+            raise ValueError("expected a 1-D series or a 2-D [series, bars] matrix")
+    for t in ts[1:]:
+        if t.shape != ts[0].shape:
+            raise ValueError(f"input shapes differ: {tuple(ts[0].shape)} vs {tuple(t.shape)}")
     for t in ts:
         if not (t.is_cuda and t.dtype == torch.float32 and t.is_contiguous()):
             raise ValueError("expected contiguous float32 tensors on the GPU")
-        if t.dim() not in (1, 2):
-            raise ValueError("expected a 1-D series or a 2-D [series, bars] matrix")
+        if t.device != ts[0].device:
+            raise ValueError("inputs are on different devices")
+
+
+def _check_ragged(layout, *ts):
+    """Ragged inputs: 1-D tensors holding exactly offsets[-1] bars."""
+    total = int(layout.offsets[-1])
+    for t in ts:
+        if t.dim() != 1 or t.numel() != total:
+            raise ValueError(f"ragged inputs must be 1-D with offsets[-1] = {total} bars, "
+                             f"got shape {tuple(t.shape)}")
+    _check(*ts)
 
 
 def _shape(x):
@@ -166,10 +186,9 @@ def pack(high, low, close, outputs=PACK_OUTPUTS, **periods):
     overrides fields of ctxPackParams (defaults: PACK_DEFAULTS).  EMA, DEMA
     and TEMA share one period.  Returns a dict of tensors shaped like close.
     """
-    _check(close)
     if high is None:
         high = low = close
-    _check(high, low)
+    _check(high, low, close)
     unknown = set(outputs) - set(PACK_OUTPUTS)
     if unknown:
         raise ValueError(f"unknown pack outputs {sorted(unknown)}")
@@ -239,41 +258,41 @@ class Ragged:
 
     def one(self, name, x, *params):
         """Single-input family (sma, ema, ..., t3, kama) over the layout."""
-        _check(x)
+        _check_ragged(self, x)
         y = torch.empty_like(x)
         call(f"ctx_{name}_ragged_d", x.data_ptr(), y.data_ptr(), self._h, *params, _stream())
         return y
 
     def hlc(self, name, high, low, close, *params):
-        _check(high, low, close)
+        _check_ragged(self, high, low, close)
         y = torch.empty_like(close)
         call(f"ctx_{name}_ragged_d", high.data_ptr(), low.data_ptr(), close.data_ptr(),
              y.data_ptr(), self._h, *params, _stream())
         return y
 
     def macd(self, x, fast=12, slow=26, signal=9):
-        _check(x)
+        _check_ragged(self, x)
         m, s, h = (torch.empty_like(x) for _ in range(3))
         call("ctx_macd_ragged_d", x.data_ptr(), m.data_ptr(), s.data_ptr(), h.data_ptr(), self._h,
              fast, slow, signal, _stream())
         return m, s, h
 
     def bbands(self, x, period=20, up=2.0, dn=2.0):
-        _check(x)
+        _check_ragged(self, x)
         u, m, l = (torch.empty_like(x) for _ in range(3))
         call("ctx_bbands_ragged_d", x.data_ptr(), u.data_ptr(), m.data_ptr(), l.data_ptr(),
              self._h, period, up, dn, _stream())
         return u, m, l
 
     def stochastic(self, high, low, close, k_period=14, d_period=3):
-        _check(high, low, close)
+        _check_ragged(self, high, low, close)
         k, d = torch.empty_like(close), torch.empty_like(close)
         call("ctx_stochastic_ragged_d", high.data_ptr(), low.data_ptr(), close.data_ptr(),
              k.data_ptr(), d.data_ptr(), self._h, k_period, d_period, _stream())
         return k, d
 
     def trange(self, high, low, close):
-        _check(high, low, close)
+        _check_ragged(self, high, low, close)
         y = torch.empty_like(close)
         call("ctx_trange_ragged_d", high.data_ptr(), low.data_ptr(), close.data_ptr(),
              y.data_ptr(), self._h, _stream())
@@ -281,10 +300,9 @@ class Ragged:
 
     def pack(self, high, low, close, outputs=PACK_OUTPUTS, **periods):
         """The fused multi-indicator launch over the layout (ctx_pack_ragged_d)."""
-        _check(close)
         if high is None:
             high = low = close
-        _check(high, low)
+        _check_ragged(self, high, low, close)
         prm = PackParams(**{**PACK_DEFAULTS, **periods})
         res = {k: torch.empty_like(close) for k in outputs}
         outs = PackOutputs(**{k: v.data_ptr() for k, v in res.items()})
@@ -304,10 +322,12 @@ class Plan:
 
         from . import lib
 
-        _check(close)
         if high is None:
             high = low = close
-        _check(high, low)
+        if layout is None:
+            _check(high, low, close)
+        else:
+            _check_ragged(layout, high, low, close)
         self.inputs = (high, low, close)
         self.outputs = {k: torch.empty_like(close) for k in outputs}
         prm = PackParams(**{**PACK_DEFAULTS, **periods})

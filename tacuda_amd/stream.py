@@ -23,6 +23,19 @@ def _f32(a):
     return np.ascontiguousarray(a, dtype=np.float32)
 
 
+def _host_inputs(family, inputs, arrs):
+    """The family's host input arrays as contiguous float32, all of one
+    length (the library reads every input for the length of the first)."""
+    missing = [k for k in inputs if arrs[k] is None]
+    if missing:
+        raise ValueError(f"{family}: missing inputs {missing}")
+    ins = [_f32(arrs[k]) for k in inputs]
+    for a in ins[1:]:
+        if a.shape != ins[0].shape:
+            raise ValueError(f"{family}: input shapes differ: {ins[0].shape} vs {a.shape}")
+    return ins
+
+
 def _params(family, params):
     if family == "bbands":
         p, up, dn = (list(params) + [20, 2.0, 2.0][len(params):])[:3]
@@ -63,7 +76,7 @@ class Stream(_Base):
 
     def append(self, high=None, low=None, close=None, volume=None):
         arrs = {"h": high, "l": low, "c": close, "v": volume}
-        ins = [_f32(arrs[k]) for k in self.inputs]
+        ins = [a.ravel() for a in _host_inputs(self.family, self.inputs, arrs)]
         n = len(ins[0])
         outs = [np.empty(n, np.float32) for _ in range(self.nout)]
         args = [self._h] + [a.ctypes.data for a in ins] + [n] + [o.ctypes.data for o in outs]
@@ -106,7 +119,11 @@ class StreamBatch(_Base):
 
     def append(self, high=None, low=None, close=None, volume=None):
         arrs = {"h": high, "l": low, "c": close, "v": volume}
-        ins = [_f32(arrs[k]).reshape(self.rows, -1) for k in self.inputs]
+        ins = _host_inputs(self.family, self.inputs, arrs)
+        if ins[0].size % self.rows:
+            raise ValueError(f"{self.family}: {ins[0].size} values do not split into "
+                             f"{self.rows} series")
+        ins = [a.reshape(self.rows, -1) for a in ins]
         n = ins[0].shape[1]
         outs = [np.empty((self.rows, n), np.float32) for _ in range(self.nout)]
         args = [self._h] + [a.ctypes.data for a in ins] + [n] + [o.ctypes.data for o in outs]
@@ -118,10 +135,29 @@ class StreamBatch(_Base):
         current stream; the state stays in HBM."""
         import torch
 
-        ref = next(t for t in (close, high, volume) if t is not None)
+        arrs = {"h": high, "l": low, "c": close, "v": volume}
+        ins = []
+        for k in self.inputs:
+            t = arrs[k]
+            if t is None:
+                raise ValueError(f"{self.family}: missing input {k!r}")
+            ins.append(t)
+        ref = ins[0]
+        if ref.numel() == 0 or ref.numel() % self.rows:
+            raise ValueError(f"{self.family}: {ref.numel()} values do not split into "
+                             f"{self.rows} series")
+        for t in ins[1:] + list(outs or []):
+            if t.numel() != ref.numel() or t.device != ref.device:
+                raise ValueError(f"{self.family}: inputs and outputs must hold {ref.numel()} "
+                                 "values on one device")
+        for t in ins + list(outs or []):
+            if not (t.is_cuda and t.dtype == torch.float32 and t.is_contiguous()):
+                raise ValueError("append_d expects contiguous float32 tensors on the GPU")
         n = ref.numel() // self.rows
         if outs is None:
             outs = [torch.empty_like(ref) for _ in range(self.nout)]
+        elif len(outs) != self.nout:
+            raise ValueError(f"{self.family}: expected {self.nout} output tensors")
         ptr = [t.data_ptr() if t is not None else None for t in (high, low, close, volume)]
         op = [o.data_ptr() for o in outs] + [None] * (3 - len(outs))
         call("ctx_stream_batch_append_d", self._h, *ptr, n, *op,

@@ -24,6 +24,8 @@ def compare(got, want, tol=1e-5, min_exact=0.999, name=""):
     w = want[ok].astype(np.float64)
     inf_mismatch = np.isinf(g) != np.isinf(w)
     assert not inf_mismatch.any(), f"{name}: Inf mask differs"
+    winf = np.isinf(w)
+    assert np.array_equal(g[winf], w[winf]), f"{name}: Inf sign differs"
     fin = np.isfinite(w)
     rel = np.abs(g[fin] - w[fin]) / np.maximum(1.0, np.maximum(np.abs(g[fin]), np.abs(w[fin])))
     exact = float(np.mean(got[ok].view(np.uint32) == want[ok].view(np.uint32)))

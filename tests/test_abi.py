@@ -29,7 +29,8 @@ def _declared():
         for m in re.finditer(r"CT_BATCH_HLC\((\w+)\)", text):
             names.update({f"ct_{m.group(1)}_batch", f"ct_{m.group(1)}_batch_d"})
         for m in re.finditer(r"CT_CDL_DECL\((\w+)\)", text):
-            names.update({f"ct_cdl_{m.group(1)}_d", f"ct_cdl_{m.group(1)}_batch_d"})
+            names.update({f"ct_cdl_{m.group(1)}_d", f"ct_cdl_{m.group(1)}_batch_d",
+                          f"ct_cdl_{m.group(1)}", f"ct_cdl_{m.group(1)}_batch"})
         # ragged macros: CTX_RAGGED1(name) and CTX_RAGGED_HLC(name)
         for m in re.finditer(r"CTX_RAGGED(?:1|_HLC)\((\w+)\)", text):
             names.add(f"ctx_{m.group(1)}_ragged_d")
@@ -80,7 +81,32 @@ def test_invalid_arguments_fail_before_touching_the_device():
     assert L.ct_sma_d(p, p, 4, 0, None) == 4
     assert L.ct_t3_d(p, p, 4, 1, 0.7, None) == 4
     assert L.ct_macd_d(p, p, p, p, 4, 12, 26, 9, 0, None) == 6
-    assert L.ct_stochastic_d(p, p, p, p, p, 4, 14, 20, None) == 6
+    assert L.ct_stochastic_d(p, p, p, p, p, 4, 14, 0, None) == 4
+    # moving-average types outside the hot path, and storage types other
+    # than float32, are refused before any device work
+    assert L.ct_ma_device(p, p, 4, 3, 7, None) == 6  # MAMA
+    assert L.ct_ma_device(p, p, 4, 3, 99, None) == 6
+    assert L.ct_sma_ex_d(p, p, 4, 3, 2, 2, None) == 6  # float64 storage
+    assert L.ct_rsi_ex_d(p, p, 4, 3, 0, 2, None) == 6  # half storage needs float32 compute
+    assert L.ct_ema_ex_d(p, p, 4, 3, 7, 1, None) == 6  # not a data type
+    assert L.ct_sma_ex(None, p, 4, 3, 1, 1, None) == 4
+    assert L.ctx_host_staging(3) == 4
+
+
+def test_host_aroon_short_series_is_all_nan_without_the_device():
+    """Host AROON with a period beyond the series: TA-Lib's zero-count
+    convention, every line NaN (reference Aroon.cu:60-65), and invalid
+    periods are rejected."""
+    import tacuda_amd as ta
+
+    L = ta.lib()
+    h = np.arange(5, dtype=np.float32) + 10
+    l = h - 1
+    up, dn, osc = (np.zeros(5, np.float32) for _ in range(3))
+    args = [h.ctypes.data, l.ctypes.data, up.ctypes.data, dn.ctypes.data, osc.ctypes.data, 5]
+    assert L.ct_aroon(*args, 3, 6, None) == 0
+    assert np.isnan(up).all() and np.isnan(dn).all() and np.isnan(osc).all()
+    assert L.ct_aroon(*args, 0, 3, None) == 4
 
 
 def test_synthetic_generator_is_seeded_and_well_formed():
@@ -120,7 +146,8 @@ def test_pack_arguments_are_validated_without_the_device():
     assert L.ctx_pack_batch_d(p, p, p, 8, 1, mixed, ta.PackOutputs(ema=p, dema=p), None) == 6
     bad = ta.PackParams(**{**PACK_DEFAULTS, "rsi_period": 0})
     assert L.ctx_pack_batch_d(p, p, p, 8, 1, bad, ta.PackOutputs(rsi=p), None) == 4
-    deep = ta.PackParams(**{**PACK_DEFAULTS, "stoch_d": 10})
+    # STOCH smoothing of any length, up to the pack's 1024-bar window halo
+    deep = ta.PackParams(**{**PACK_DEFAULTS, "stoch_d": 510})
     assert L.ctx_pack_batch_d(p, p, p, 8, 1, deep, ta.PackOutputs(stoch_k=p), None) == 6
     wide = ta.PackParams(**{**PACK_DEFAULTS, "sma_period": 1026})
     assert L.ctx_pack_batch_d(p, p, p, 8, 1, wide, ta.PackOutputs(sma=p), None) == 6

@@ -161,3 +161,73 @@ def test_two_pass_plan_and_user_capture(dev, shape):
             assert torch.equal(out[k].view(torch.int32), first[k].view(torch.int32)), k
     plan.close()
     del junk, g
+
+
+def test_graph_replay_after_workspace_growth(dev):
+    """A graph captured on a stream keeps the look-back workspace it was
+    captured with: a larger eager call on the same stream afterwards moves
+    the stream to a bigger workspace (the old one stays allocated), and the
+    graph's replays still equal the eager pass bit for bit; the large call
+    matches the oracle on sampled rows."""
+    import tacuda_amd as ta
+    from tests.test_gpu_pack import DEFAULT
+
+    print("workspace growth after capture", flush=True)
+    L = ta.lib()
+    d = ta.synth_ohlcv(234, rows=300, seed=71)
+    T = [torch.from_numpy(np.ascontiguousarray(d[k])).to(dev) for k in ("high", "low", "close")]
+    st = torch.cuda.Stream()
+    with torch.cuda.stream(st):
+        eager = {k: v.clone() for k, v in ta.ops.pack(*T, **DEFAULT).items()}
+    st.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=st):
+        cap = ta.ops.pack(*T, **DEFAULT)
+    torch.cuda.synchronize()
+    before = L.ct_device_buffer_pool_allocation_count()
+    big = ta.synth_ohlcv(2340, rows=4000, seed=72)
+    B = [torch.from_numpy(np.ascontiguousarray(big[k])).to(dev) for k in ("high", "low", "close")]
+    with torch.cuda.stream(st):
+        out = ta.ops.pack(*B, **DEFAULT)
+    st.synchronize()
+    assert L.ct_device_buffer_pool_allocation_count() > before, "workspace did not grow"
+    for _ in range(3):
+        for v in cap.values():
+            v.fill_(0.0)
+        g.replay()
+        torch.cuda.synchronize()
+        for k in eager:
+            assert torch.equal(cap[k].view(torch.int32), eager[k].view(torch.int32)), k
+    sub = {k: v[[0, 1999, 3999]] for k, v in big.items()}
+    _check_all({k: v[[0, 1999, 3999]] for k, v in out.items()}, sub, DEFAULT)
+    del g
+
+
+def test_plan_launches_on_two_streams(dev):
+    """Replays of one plan on two streams, issued back to back without host
+    synchronization, are ordered by the plan (they share its look-back
+    workspace): every replay's outputs equal the first replay's and the
+    oracle."""
+    import tacuda_amd as ta
+    from tests.test_gpu_pack import DEFAULT
+
+    print("plan on two streams", flush=True)
+    d = ta.synth_ohlcv(234, rows=4000, seed=73)
+    T = [torch.from_numpy(np.ascontiguousarray(d[k])).to(dev) for k in ("high", "low", "close")]
+    plan = ta.ops.Plan(*T, tune=False, **DEFAULT)
+    first = {k: v.clone() for k, v in plan.launch().items()}
+    torch.cuda.synchronize()
+    sub = {k: v[[0, 17, 3999]] for k, v in d.items()}
+    _check_all({k: v[[0, 17, 3999]] for k, v in first.items()}, sub, DEFAULT)
+    sa, sb = torch.cuda.Stream(), torch.cuda.Stream()
+    snaps = []
+    for i in range(6):
+        s = sa if i % 2 == 0 else sb
+        with torch.cuda.stream(s):
+            out = plan.launch()
+            snaps.append({k: v.clone() for k, v in out.items()})
+    torch.cuda.synchronize()
+    for snap in snaps:
+        for k in first:
+            assert torch.equal(snap[k].view(torch.int32), first[k].view(torch.int32)), k
+    plan.close()
