@@ -203,5 +203,43 @@ class _Family:
         return out
 
 
+    def aroon_host(self, h, l, up_p, dn_p):
+        """The host AROON call with two periods (reference api.cpp:2511-2570,
+        Aroon.cu:12-66): (up, down, osc); up from orc_aroon over up_p, down
+        over dn_p, osc = up - down (float32) from max(up_p, dn_p); every line
+        NaN when either period exceeds the series.  pinned: each line equals
+        the one-period oracle for its own period (tests/test_oracle.py)."""
+        n = len(h)
+        if up_p > n or dn_p > n:
+            nan = np.full(n, np.nan, np.float32)
+            return nan, nan.copy(), nan.copy()
+        up = self.aroon(h, l, up_p)[1]
+        dn = self.aroon(h, l, dn_p)[0]
+        osc = np.full(n, np.nan, np.float32)
+        m = max(up_p, dn_p)
+        osc[m:] = up[m:] - dn[m:]
+        return up, dn, osc
+
+    def engulfing_legacy(self, o, h, l, c):
+        """The reference's ct_cdl_bullish_engulfing / bearish_engulfing
+        (CandleUtils.h:233-260 through BullishEngulfing.cu:10-23): TA-Lib's
+        signed engulfing signal, from bar 1 on (the 61-pattern engulfing
+        starts at bar 2).  From bar 2 it is the engulfing oracle itself; bar
+        1 applies the same rule.  pinned: the engulfing oracle's own checks
+        (tests/test_oracle_candles.py)."""
+        o, c = _f32(o), _f32(c)
+        out = self.candle("engulfing", o, h, l, c).astype(np.float32)
+        out[:2] = 0.0
+        if len(c) > 1:
+            co, cc, po, pc = o[1], c[1], o[0], c[0]
+            cur = 1 if cc >= co else -1
+            prev = 1 if pc >= po else -1
+            bull = cur == 1 and prev == -1 and ((cc >= po and co < pc) or (cc > po and co <= pc))
+            bear = cur == -1 and prev == 1 and ((co >= pc and cc < po) or (co > pc and cc <= po))
+            if bull or bear:
+                out[1] = cur * (100 if (co != pc and cc != po) else 80)
+        return out
+
+
 ORC = _Family(lib, "orc_")
 REF = _Family(ref, "ref_")

@@ -363,3 +363,47 @@ def test_obv_and_ad_brute_force():
     t2 = term.copy()
     t2[100] = 0.0
     assert np.allclose(a1, np.cumsum(t2), rtol=1e-6, atol=1e-3)
+
+
+def test_host_aroon_two_periods_brute_force():
+    """Host AROON with distinct periods: up and down each follow their own
+    window (newest extremum wins ties), the oscillator is up - down from the
+    longer period, and a period beyond the series gives three NaN lines."""
+    o, h, l, c = DATASETS["rw1"]
+    h, l = h[:400], l[:400]
+    up_p, dn_p = 9, 25
+    up, dn, osc = ORC.aroon_host(h, l, up_p, dn_p)
+    assert np.isnan(up[:up_p]).all() and np.isnan(dn[:dn_p]).all()
+    assert np.isnan(osc[:dn_p]).all()
+    for t in range(up_p, len(h)):
+        wh = h[t - up_p: t + 1][::-1]
+        assert up[t] == np.float32(100.0 / up_p * (up_p - int(np.argmax(wh))))
+    for t in range(dn_p, len(h)):
+        wl = l[t - dn_p: t + 1][::-1]
+        assert dn[t] == np.float32(100.0 / dn_p * (dn_p - int(np.argmin(wl))))
+        assert osc[t] == np.float32(up[t] - dn[t])
+    for line in ORC.aroon_host(h[:20], l[:20], 5, 21):
+        assert np.isnan(line).all()
+
+
+def test_engulfing_legacy_alias_bars():
+    """The legacy engulfing names: +100 for a white body engulfing the black
+    body before it, -100 for the mirror, +-80 when an open equals the other
+    candle's close; the signal is written from bar 1 (the 61-pattern
+    engulfing starts at bar 2) and equals it from bar 2 on."""
+    # bar0 black 10->9, bar1 white 8.5->10.5 (engulfs): +100 at bar 1
+    o = np.array([10.0, 8.5, 11.0, 10.0, 9.0, 9.0], np.float32)
+    c = np.array([9.0, 10.5, 10.0, 11.5, 9.5, 8.0], np.float32)
+    h = np.maximum(o, c) + 0.1
+    l = np.minimum(o, c) - 0.1
+    out = ORC.engulfing_legacy(o, h, l, c)
+    assert out[0] == 0 and out[1] == 100
+    # bar2 black 11->10 after white 8.5->10.5: open 11 > 10.5, close 10 > 8.5: no
+    assert out[2] == 0
+    # bar3 white 10->11.5 after black 11->10: open 10 == prev close -> +80
+    assert out[3] == 80
+    # bar5 black 9->8 after white 9->9.5: open 9 < 9.5 fails the strict rule,
+    # but open >= ... no engulfing (close 8 < 9, open 9 < prev close 9.5)
+    assert out[5] == 0
+    full = ORC.candle("engulfing", o, h, l, c)
+    np.testing.assert_array_equal(out[2:], full[2:].astype(np.float32))
