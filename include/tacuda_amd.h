@@ -92,6 +92,21 @@ CTAPI_EXPORT ctStatus_t ctx_pack_batch_d(const float* d_high, const float* d_low
                                          const ctxPackParams* params,
                                          const ctxPackOutputs* outputs, hipStream_t stream);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The same pass and outputs as ctx_pack_batch_d, with
+ * its look-back scratch in a fixed array of 2 x 8 MiB per GPU held until
+ * the process ends, instead of the per-stream workspace cache: the call
+ * allocates nothing.  A pass whose scratch exceeds 8 MiB per part (about
+ * 3 x 10^7 bars of long rows) returns CT_STATUS_INVALID_ARGUMENT (outputs
+ * undefined).  Every call on one GPU shares that GPU's array: calls that
+ * overlap in time on one GPU (several streams or threads) race. */
+CTAPI_EXPORT ctStatus_t ctx_pack_batch_noalloc_d(const float* d_high, const float* d_low,
+                                                 const float* d_close, int seriesLen,
+                                                 int numSeries, const ctxPackParams* params,
+                                                 const ctxPackOutputs* outputs,
+                                                 hipStream_t stream);
+
 /* ---------------------------------------------------------------------------
  * Ragged batches: series r occupies elements [offsets[r], offsets[r+1]) of
  * concatenated input and output arrays (offsets are int64, host memory,

@@ -63,7 +63,20 @@ ctStatus_t hip_fail(hipError_t e, const char* what) {
   return fail(st, "%s: %s", what, hipGetErrorString(e));
 }
 
+thread_local const WsFixed* t_ws_fixed = nullptr;
+
 ctStatus_t workspace_for(hipStream_t stream, size_t treeBytes, Ws* out, int part) {
+  if (const WsFixed* f = t_ws_fixed) {
+    if (part < 0 || part >= kWsParts) return fail(CT_STATUS_INVALID_ARGUMENT, "workspace part");
+    if (treeBytes > f->bytes)
+      return fail(CT_STATUS_INVALID_ARGUMENT,
+                  "this call needs %zu bytes of look-back scratch; the allocation-free entry "
+                  "point holds %zu per part",
+                  treeBytes, f->bytes);
+    out->hdr = reinterpret_cast<WsHeader*>(f->part[part]);
+    out->area = reinterpret_cast<double*>(f->part[part] + sizeof(WsHeader));
+    return CT_STATUS_SUCCESS;
+  }
   int dev = 0;
   TCA_HIP(hipGetDevice(&dev));
   std::lock_guard<std::mutex> lk(g_ws_mu);
@@ -244,7 +257,8 @@ bool kt_capturing(hipStream_t s) {
 }  // namespace
 
 KTime::KTime(hipStream_t stream, const char* name) : s_(stream), name_(name) {
-  if (!g_kt_on.load(std::memory_order_relaxed) || kt_capturing(s_)) return;
+  // the allocation-free entry point creates no events either
+  if (!g_kt_on.load(std::memory_order_relaxed) || t_ws_fixed || kt_capturing(s_)) return;
   if (hipEventCreate(&a_) != hipSuccess) {
     a_ = nullptr;
     return;

@@ -54,6 +54,18 @@ ctStatus_t guarded(F&& body) {
 // freed when the stream's workspaces are released or at pool cleanup.
 constexpr int kWsParts = 2;
 ctStatus_t workspace_for(hipStream_t stream, size_t treeBytes, Ws* out, int part = 0);
+// While a WsFixedScope is alive on a thread, workspace_for hands out these
+// caller-provided buffers (each `bytes` of area after a WsHeader) instead of
+// the per-stream cache, and refuses requests larger than `bytes`.
+struct WsFixed {
+  char* part[kWsParts];
+  size_t bytes;
+};
+extern thread_local const WsFixed* t_ws_fixed;
+struct WsFixedScope {
+  explicit WsFixedScope(const WsFixed* f) { t_ws_fixed = f; }
+  ~WsFixedScope() { t_ws_fixed = nullptr; }
+};
 void workspace_release_all();
 void workspace_release(hipStream_t stream);
 size_t workspace_allocations();
@@ -158,6 +170,12 @@ ctStatus_t pipe_family(const ctxPackParams& params, const ctxPackOutputs& output
 ctStatus_t win_family(const ctxPackParams& params, const ctxPackOutputs& outputs,
                       const float* h, const float* l, const float* c, int len, int rows,
                       hipStream_t stream, bool* served);
+
+// The body of ctx_pack_batch_d (validation, the fused pass, the stream-0
+// wait) without the ABI guard.
+ctStatus_t pack_batch_body(const float* h, const float* l, const float* c, int seriesLen,
+                           int numSeries, const ctxPackParams* p, const ctxPackOutputs* o,
+                           hipStream_t s);
 
 // Synchronize when the caller used the legacy default stream: batch calls
 // on stream 0 return once the batch is done (reference api_batch.cu:377-386).

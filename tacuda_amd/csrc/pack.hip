@@ -2439,17 +2439,31 @@ CTAPI_EXPORT ctStatus_t ctx_pack_batch_d(const float* h, const float* l, const f
                                          int seriesLen, int numSeries, const ctxPackParams* p,
                                          const ctxPackOutputs* o, hipStream_t s) {
   return guarded([&]() -> ctStatus_t {
-    if (seriesLen <= 0 || numSeries <= 0)
-      return fail(CT_STATUS_INVALID_ARGUMENT, "pack: seriesLen and numSeries must be positive");
-    PackP P;
-    PackIO io;
-    ctStatus_t st = pack_validate(p, o, P, io, h, l, c);
-    if (st != CT_STATUS_SUCCESS) return st;
-    st = pack_dense(seriesLen, numSeries, s, P, io);
-    if (st != CT_STATUS_SUCCESS) return st;
-    return finish_batch(s);
+    return pack_batch_body(h, l, c, seriesLen, numSeries, p, o, s);
   });
 }
+
+}  // extern "C"
+
+namespace tca {
+// ctx_pack_batch_d without the error reset and exception guard, for the
+// allocation-free entry point that sets up its own scratch first.
+ctStatus_t pack_batch_body(const float* h, const float* l, const float* c, int seriesLen,
+                           int numSeries, const ctxPackParams* p, const ctxPackOutputs* o,
+                           hipStream_t s) {
+  if (seriesLen <= 0 || numSeries <= 0)
+    return fail(CT_STATUS_INVALID_ARGUMENT, "pack: seriesLen and numSeries must be positive");
+  PackP P;
+  PackIO io;
+  ctStatus_t st = pack_validate(p, o, P, io, h, l, c);
+  if (st != CT_STATUS_SUCCESS) return st;
+  st = pack_dense(seriesLen, numSeries, s, P, io);
+  if (st != CT_STATUS_SUCCESS) return st;
+  return finish_batch(s);
+}
+}  // namespace tca
+
+extern "C" {
 
 CTAPI_EXPORT ctStatus_t ctx_pack_ragged_d(const float* h, const float* l, const float* c,
                                           ctxRagged layout, const ctxPackParams* p,
