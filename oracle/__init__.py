@@ -241,5 +241,18 @@ class _Family:
         return out
 
 
+    # -- neighbours only the vendored library has (prefix "ref_") ----------
+    def donchian(self, h, l, p):
+        """(upper, lower) bands: highest high and lowest low over p bars."""
+        return _call(self._fn("donchian", 2, 2, [_I]), [h, l], 2, len(h), p)
+
+    def stochf(self, h, l, c, kp, dp):
+        """(fast %K, fast %D with SMA smoothing)."""
+        return _call(self._fn("stochf", 3, 2, [_I, _I]), [h, l, c], 2, len(c), kp, dp)
+
+    def ma(self, x, p, ma_type):
+        return _call(self._fn("ma", 1, 1, [_I, _I]), [x], 1, len(x), p, ma_type)
+
+
 ORC = _Family(lib, "orc_")
 REF = _Family(ref, "ref_")

@@ -12,9 +12,14 @@
  * Output: 0 for bars before the pattern's lookback (reference
  * src/indicators/<Pattern>.cu warm-ups), the signed TA-Lib signal after.
  *
- * parity vs TA-Lib unpinned (TA-Lib is not importable here): pinned by the
- * reference's hand-authored activation fixture (differential_corpus.py:
- * 73-142), signal-set and symmetry invariants, and GPU bit equality. */
+ * TA-Lib itself is not importable here.  Every pattern is pinned by value:
+ * 10 by the reference's hand-authored activation fixture
+ * (differential_corpus.py:73-142), 6 by independent numpy restatements
+ * (tests/test_oracle_candles.py) and the other 45 by constructive fixtures
+ * built from TA-Lib's rule statements, each a sequence that fires with a
+ * stated signal plus near misses that break one condition by one tick
+ * (tests/test_oracle_candle_fixtures.py); signal-set and symmetry
+ * invariants on random bars besides. */
 #include <math.h>
 #include <stdint.h>
 #include <string.h>

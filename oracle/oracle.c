@@ -682,8 +682,8 @@ API int orc_stoch(const float* h, const float* l, const float* c, float* k, floa
 /* ---------------------------------------------------------------- WILLR ---
  * Reference WILLR.cu:12-30 (TA-Lib WILLR not vendored): extrema in float,
  * ratio in double, 0 on a zero range, clamped to [-100, 0].
- * parity vs TA-Lib unpinned; pinned to the reference formula by range,
- * flat-window and brute-force checks. */
+ * pinned: fast %K - 100 of the vendored TA_STOCHF (_ref) within float
+ * rounding; range, flat-window and brute-force checks. */
 API int orc_willr(const float* h, const float* l, const float* c, float* out, int n, int p) {
   if (!h || !l || !c || !out || n <= 0 || p <= 0) return ORC_BAD;
   fill_nan(out, n);
@@ -781,7 +781,8 @@ API int orc_aroonosc(const float* h, const float* l, float* out, int n, int p) {
 /* ------------------------------------------------------------ MIN / MAX ---
  * Reference MAX.cu:12-30 / MIN.cu: NaN-ignoring window extremum (fmaxf),
  * except a window that starts inside the leading NaN prefix is NaN
- * (LeadingNaN.cuh:22-28).  pinned: numpy nanmax brute force. */
+ * (LeadingNaN.cuh:22-28).  pinned: bit-equal to the vendored TA_DONCHIAN
+ * bands (_ref) on finite data; numpy nanmax brute force for NaN rules. */
 static int in_lead_prefix(const float* x, int idx) {
   if (!isnan(x[0])) return 0;
   for (int i = idx - 1; i > 0; --i)

@@ -129,3 +129,57 @@ API int ref_stoch(const float* hi, const float* lo, const float* cl, float* k, f
   place(d, n, beg, nb, od.data());
   return 0;
 }
+
+// Neighbours that pin oracle functions the vendored library has no exact
+// counterpart for: DONCHIAN's bands are MAX(high) and MIN(low); STOCHF's
+// fast %K minus 100 is WILLR.
+API int ref_donchian(const float* hi, const float* lo, float* up, float* dn, int n, int p) {
+  std::vector<double> h = widen(hi, n), l = widen(lo, n), u(n), m(n), d(n);
+  Arena a = make_arena(n);
+  int beg = 0, nb = 0;
+  int rc = TA_DONCHIAN(a, 0, n - 1, h.data(), l.data(), p, &beg, &nb, u.data(), m.data(),
+                       d.data());
+  if (rc) return rc;
+  place(up, n, beg, nb, u.data());
+  place(dn, n, beg, nb, d.data());
+  return 0;
+}
+
+API int ref_stochf(const float* hi, const float* lo, const float* cl, float* k, float* d, int n,
+                   int kp, int dp) {
+  std::vector<double> h = widen(hi, n), l = widen(lo, n), c = widen(cl, n), ok(n), od(n);
+  Arena a = make_arena(n);
+  int beg = 0, nb = 0;
+  int rc = TA_STOCHF(a, 0, n - 1, h.data(), l.data(), c.data(), kp, dp, 0, &beg, &nb, ok.data(),
+                     od.data());
+  if (rc) return rc;
+  place(k, n, beg, nb, ok.data());
+  place(d, n, beg, nb, od.data());
+  return 0;
+}
+
+// TA_MACDEXT with a moving-average type per line (TA_MAType numbering:
+// 0 SMA, 1 EMA, 2 WMA, 3 DEMA, 4 TEMA, 5 TRIMA, 6 KAMA, 7 MAMA, 8 T3).
+API int ref_macdext(const float* x, float* m, float* s, float* h, int n, int fast, int ft,
+                    int slow, int st, int sig, int sgt) {
+  std::vector<double> in = widen(x, n), om(n), os(n), oh(n);
+  Arena a = make_arena(n);
+  int beg = 0, nb = 0;
+  int rc = TA_MACDEXT(a, 0, n - 1, in.data(), fast, ft, slow, st, sig, sgt, &beg, &nb, om.data(),
+                      os.data(), oh.data());
+  if (rc) return rc;
+  place(m, n, beg, nb, om.data());
+  place(s, n, beg, nb, os.data());
+  place(h, n, beg, nb, oh.data());
+  return 0;
+}
+
+API int ref_ma(const float* x, float* out, int n, int p, int type) {
+  std::vector<double> in = widen(x, n), o(n);
+  Arena a = make_arena(n);
+  int beg = 0, nb = 0;
+  int rc = TA_MA(a, 0, n - 1, in.data(), p, type, &beg, &nb, o.data());
+  if (rc) return rc;
+  place(out, n, beg, nb, o.data());
+  return 0;
+}

@@ -407,3 +407,29 @@ def test_engulfing_legacy_alias_bars():
     assert out[5] == 0
     full = ORC.candle("engulfing", o, h, l, c)
     np.testing.assert_array_equal(out[2:], full[2:].astype(np.float32))
+
+
+@needs_ref
+@pytest.mark.parametrize("ds", ["rw0", "rw1", "short", "zeros"])
+@pytest.mark.parametrize("p", [2, 14, 30])
+def test_min_max_pinned_by_donchian(ds, p):
+    """MAX(high) and MIN(low) equal the vendored TA_DONCHIAN bands bit for
+    bit (extrema are input values, so no rounding enters)."""
+    o, h, l, c = DATASETS[ds]
+    up, dn = REF.donchian(h, l, p)
+    _bits_equal(ORC.max(h, p), up)
+    _bits_equal(ORC.min(l, p), dn)
+
+
+@needs_ref
+@pytest.mark.parametrize("ds", ["rw0", "rw1", "short"])
+@pytest.mark.parametrize("p", [5, 14, 30])
+def test_willr_pinned_by_stochf(ds, p):
+    """WILLR = fast %K - 100 (TA_STOCHF of the vendored library): the same
+    NaN range and values within float rounding of the two forms."""
+    o, h, l, c = DATASETS[ds]
+    k, _ = REF.stochf(h, l, c, p, 1)
+    w = ORC.willr(h, l, c, p)
+    assert np.array_equal(np.isnan(w), np.isnan(k))
+    ok = ~np.isnan(k)
+    np.testing.assert_allclose(w[ok], k[ok].astype(np.float64) - 100.0, rtol=0, atol=3e-5)
