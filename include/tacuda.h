@@ -150,6 +150,11 @@ CTAPI_EXPORT ctStatus_t ct_sum_device(const float* device_input, float* device_o
 CTAPI_EXPORT ctStatus_t ct_rsi_device(const float* device_input, float* device_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
 CTAPI_EXPORT ctStatus_t ct_kama_device(const float* device_input, float* device_output, int size, int period, int fastPeriod, int slowPeriod, hipStream_t stream CTAPI_STREAM_DEFAULT);
 
+/* :134, :243 -- TRIMA; :770 -- MACDEXT (see ct_macdext_d in tacuda_device.h) */
+CTAPI_EXPORT ctStatus_t ct_trima(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_trima_device(const float* device_input, float* device_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_macdext(const float* host_input, float* host_macd, float* host_signal, float* host_hist, int size, int fastPeriod, ctMaType_t fastType, int slowPeriod, ctMaType_t slowType, int signalPeriod, ctMaType_t signalType, hipStream_t stream CTAPI_STREAM_DEFAULT);
+
 /* tacuda.h:824-827 -- the workspace cache of this build stands in for the
  * reference's DeviceBufferPool.  Cleanup frees every look-back workspace,
  * including those that graphs captured earlier still reference: call it only
@@ -233,6 +238,12 @@ CTAPI_EXPORT ctStatus_t ct_aroonosc_batch_d(const float* d_highs, const float* d
 /* :1356 / :1363 (down before up, as in the reference) */
 CTAPI_EXPORT ctStatus_t ct_aroon_batch(const float* host_highs, const float* host_lows, float* host_aroon_downs, float* host_aroon_ups, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
 CTAPI_EXPORT ctStatus_t ct_aroon_batch_d(const float* d_highs, const float* d_lows, float* d_aroon_downs, float* d_aroon_ups, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :982 / :987 */
+CTAPI_EXPORT ctStatus_t ct_trima_batch(const float* host_inputs, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_trima_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1387 / :1396 */
+CTAPI_EXPORT ctStatus_t ct_macdext_batch(const float* host_inputs, float* host_macds, float* host_signals, float* host_hists, int seriesLen, int numSeries, int fastPeriod, int fastType, int slowPeriod, int slowType, int signalPeriod, int signalType, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_macdext_batch_d(const float* d_inputs, float* d_macds, float* d_signals, float* d_hists, int seriesLen, int numSeries, int fastPeriod, int fastType, int slowPeriod, int slowType, int signalPeriod, int signalType, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
 /* :1733 / :1739, :1745 / :1751 */
 CTAPI_EXPORT ctStatus_t ct_plus_dm_batch(const float* host_highs, const float* host_lows, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
 CTAPI_EXPORT ctStatus_t ct_plus_dm_batch_d(const float* d_highs, const float* d_lows, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);

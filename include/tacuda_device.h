@@ -62,6 +62,16 @@ CTAPI_EXPORT ctStatus_t ct_cci_d(const float* d_high, const float* d_low, const 
 /* :280 bands */
 CTAPI_EXPORT ctStatus_t ct_bbands_d(const float* d_input, float* d_upper, float* d_middle, float* d_lower, int size, int period, float nbDevUp, float nbDevDn, hipStream_t stream CTAPI_STREAM_DEFAULT);
 
+/* :151 -- TA-Lib TRIMA (triangular weights), window semantics for NaN */
+CTAPI_EXPORT ctStatus_t ct_trima_d(const float* d_input, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :388 -- TA-Lib MACDEXT with a moving-average type per line (ctMaType_t:
+ * SMA, EMA, WMA, DEMA, TEMA, TRIMA, KAMA, T3; MAMA and the types above T3
+ * return CT_STATUS_UNSUPPORTED).  Fast and slow periods from 2, signal from
+ * 1; reversed periods swap with their types; a leading NaN prefix is
+ * skipped.  ct_macd_d with maType other than EMA is MACDEXT with that type
+ * on all three lines. */
+CTAPI_EXPORT ctStatus_t ct_macdext_d(const float* d_input, float* d_macd, float* d_signal, float* d_hist, int size, int fastPeriod, int fastType, int slowPeriod, int slowType, int signalPeriod, int signalType, hipStream_t stream CTAPI_STREAM_DEFAULT);
+
 /* Typed variants (:132, :138, :167, :197, :244, :283).  Float32 storage with
  * float32 or float64 compute runs the float32 path (its recurrences, window
  * sums and ratios are evaluated in float64 and rounded once per output);

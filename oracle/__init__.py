@@ -241,6 +241,17 @@ class _Family:
         return out
 
 
+    def trima(self, x, p):
+        if self._prefix == "ref_":
+            return self.ma(x, p, 5)
+        return _call(self._fn("trima", 1, 1, [_I]), [x], 1, len(x), p)
+
+    def macdext(self, x, fast, ft, slow, st, sig, sgt):
+        """(macd, signal, hist) with a moving-average type per line
+        (TA_MAType numbering)."""
+        return _call(self._fn("macdext", 1, 3, [_I] * 6), [x], 3, len(x), fast, ft, slow, st,
+                     sig, sgt)
+
     # -- neighbours only the vendored library has (prefix "ref_") ----------
     def donchian(self, h, l, p):
         """(upper, lower) bands: highest high and lowest low over p bars."""

@@ -1001,3 +1001,222 @@ API int orc_ema_rows(const float* x, float* out, int len, int rows, int p) {
   }
   return ORC_OK;
 }
+
+/* ---------------------------------------------------------------- TRIMA ---
+ * ta_TRIMA.c: triangular weights min(j+1, p-j) over the window, times
+ * 1/((p/2+1)^2) for odd p and 1/((p/2)(p/2+1)) for even p.  TA-Lib keeps the
+ * weighted total as add-then-subtract running sums of the float32 inputs
+ * (exact in double), so the direct weighted sum gives the same double.
+ * Window semantics for NaN.  pinned: _ref TA_MA(TRIMA) bit equality. */
+API int orc_trima(const float* x, float* out, int n, int p) {
+  if (!x || !out || n <= 0 || p <= 0) return ORC_BAD;
+  fill_nan(out, n);
+  const int i = p >> 1;
+  const double factor = (p % 2 == 1) ? 1.0 / ((double)(i + 1) * (i + 1))
+                                     : 1.0 / ((double)i * (i + 1));
+  for (int t = p - 1; t < n; ++t) {
+    double s = 0.0;
+    for (int j = 0; j < p; ++j) {
+      const int w = (j + 1 < p - j) ? j + 1 : p - j;
+      s += (double)w * (double)x[t - p + 1 + j];
+    }
+    out[t] = (float)(s * factor);
+  }
+  return ORC_OK;
+}
+
+/* TA_MAType numbering: 0 SMA, 1 EMA, 2 WMA, 3 DEMA, 4 TEMA, 5 TRIMA, 6 KAMA,
+ * 8 T3 (7 MAMA is not restated). */
+static int ma_lookback(int p, int type) {
+  if (p == 1) return 0;
+  switch (type) {
+    case 0: case 1: case 2: case 5: return p - 1;
+    case 3: return 2 * (p - 1);
+    case 4: return 3 * (p - 1);
+    case 6: return p;
+    case 8: return 6 * (p - 1);
+  }
+  return -1;
+}
+/* Moving averages in double over in[s..n) (the bars before s are not
+ * part of the series), for MACDEXT, whose macd line is the difference of
+ * two of them.  Each follows the float32 oracle function of its type above
+ * (TA-Lib's arithmetic), with double outputs; out[t] is NaN before s +
+ * lookback. */
+static void md_ma(const double* in, double* out, int s, int n, int p, int type) {
+  const double qnan = NAN;
+  const int lb = ma_lookback(p, type);
+  for (int t = 0; t < n; ++t) out[t] = qnan;
+  if (s + lb >= n) return;
+  if (p == 1) { /* TA_MA copies the input for a one-bar period */
+    for (int t = s; t < n; ++t) out[t] = in[t];
+    return;
+  }
+  double beta, k;
+  ema_coef(p, &beta, &k);
+  switch (type) {
+    case 0: /* SMA */
+      for (int t = s + p - 1; t < n; ++t) {
+        double a = 0.0;
+        for (int j = t - p + 1; j <= t; ++j) a += in[j];
+        out[t] = a / p;
+      }
+      return;
+    case 2: { /* WMA */
+      const double divider = (double)p * (p + 1) / 2.0;
+      for (int t = s + p - 1; t < n; ++t) {
+        double a = 0.0;
+        int w = 1;
+        for (int j = t - p + 1; j <= t; ++j, ++w) a += in[j] * w;
+        out[t] = a / divider;
+      }
+      return;
+    }
+    case 5: { /* TRIMA */
+      const int i = p >> 1;
+      const double factor = (p % 2 == 1) ? 1.0 / ((double)(i + 1) * (i + 1))
+                                         : 1.0 / ((double)i * (i + 1));
+      for (int t = s + p - 1; t < n; ++t) {
+        double a = 0.0;
+        for (int j = 0; j < p; ++j) {
+          const int w = (j + 1 < p - j) ? j + 1 : p - j;
+          a += (double)w * in[t - p + 1 + j];
+        }
+        out[t] = a * factor;
+      }
+      return;
+    }
+    case 1: case 3: case 4: { /* EMA, DEMA, TEMA: SMA-seeded EMA cascade */
+      const int levels = type == 1 ? 1 : type == 3 ? 2 : 3;
+      double e[3] = {0.0, 0.0, 0.0};
+      int today = s;
+      double tmp = 0.0;
+      for (int i = 0; i < p; ++i) tmp += in[today++];
+      e[0] = tmp / p;
+      for (int lev = 1; lev < levels; ++lev) {
+        tmp = e[lev - 1];
+        for (int i = 0; i < p - 1; ++i) {
+          const double xv = in[today++];
+          e[0] = fma(beta, e[0], k * xv);
+          for (int q = 1; q < lev; ++q) e[q] = fma(beta, e[q], k * e[q - 1]);
+          tmp += e[lev - 1];
+        }
+        e[lev] = tmp / p;
+      }
+      out[today - 1] = levels == 1 ? e[0] : levels == 2 ? 2.0 * e[0] - e[1]
+                                                        : e[2] + (3.0 * e[0] - 3.0 * e[1]);
+      for (; today < n; ++today) {
+        e[0] = fma(beta, e[0], k * in[today]);
+        for (int q = 1; q < levels; ++q) e[q] = fma(beta, e[q], k * e[q - 1]);
+        out[today] = levels == 1 ? e[0] : levels == 2 ? 2.0 * e[0] - e[1]
+                                                      : e[2] + (3.0 * e[0] - 3.0 * e[1]);
+      }
+      return;
+    }
+    case 6: { /* KAMA 2/30 */
+      const double cmax = 2.0 / 31.0, cdiff = 2.0 / 3.0 - cmax;
+      double prev = in[s + p - 1];
+      for (int t = s + p; t < n; ++t) {
+        double sum = 0.0;
+        for (int j = t - p + 1; j <= t; ++j) sum += fabs(in[j] - in[j - 1]);
+        const double roc = in[t] - in[t - p];
+        double er;
+        if (sum <= 0.0 || sum <= roc) {
+          er = 1.0;
+        } else {
+          er = fabs(roc / sum);
+          if (er > 1.0) er = 1.0;
+        }
+        double sc = fma(er, cdiff, cmax);
+        sc *= sc;
+        prev = fma(in[t] - prev, sc, prev);
+        out[t] = prev;
+      }
+      return;
+    }
+    case 8: { /* T3, volume factor 0.7 (as TA_MA passes it) */
+      const double v = (double)0.7f;
+      const double kk = 2.0 / (p + 1.0), omk = 1.0 - kk;
+      double e[6];
+      int today = s;
+      for (int lev = 0; lev < 6; ++lev) {
+        double tmp = 0.0;
+        for (int i = 0; i < p; ++i) {
+          if (lev == 0) {
+            tmp += in[today++];
+            continue;
+          }
+          if (i == 0) {
+            tmp = e[lev - 1];
+            continue;
+          }
+          const double xv = in[today++];
+          e[0] = fma(omk, e[0], kk * xv);
+          for (int q = 1; q < lev; ++q) e[q] = fma(omk, e[q], kk * e[q - 1]);
+          tmp += e[lev - 1];
+        }
+        e[lev] = tmp / p;
+      }
+      const double v2 = v * v;
+      const double c1 = -(v2 * v);
+      const double c2 = 3.0 * (v2 - c1);
+      const double c3 = -6.0 * v2 - 3.0 * (v - c1);
+      const double c4 = fma(3.0, v2, fma(3.0, v, 1.0) - c1);
+      out[today - 1] = fma(c4, e[2], fma(c3, e[3], fma(c1, e[5], c2 * e[4])));
+      for (; today < n; ++today) {
+        e[0] = fma(omk, e[0], kk * in[today]);
+        for (int q = 1; q < 6; ++q) e[q] = fma(omk, e[q], kk * e[q - 1]);
+        out[today] = fma(c4, e[2], fma(c3, e[3], fma(c1, e[5], c2 * e[4])));
+      }
+      return;
+    }
+  }
+}
+
+/* -------------------------------------------------------------- MACDEXT ---
+ * ta_MACDEXT.c with the reference's formula backend convention
+ * (src/compat/Talib08.cu:23-142: a leading NaN prefix is skipped):
+ * reversed periods swap with their types; L = max of the two MA look-backs;
+ * the fast and slow MAs are those TA_MA computes when asked to start at
+ * bar L -- for the recursive types the seed sits at L, which is the
+ * moving average of the bars from L - lookback(MA) on; macd = fast -
+ * slow from L, in double; signal = MA of the macd line; every output
+ * starts at L + lookback(signal MA).  pinned: _ref TA_MACDEXT. */
+API int orc_macdext(const float* x, float* m, float* s, float* h, int n, int fast, int ft,
+                    int slow, int st, int sig, int sgt) {
+  /* TA_MACDEXT accepts fast and slow periods from 2, the signal from 1 */
+  if (!x || !m || !s || !h || n <= 0 || fast < 2 || slow < 2 || sig < 1) return ORC_BAD;
+  if (slow < fast) {
+    int tp = fast;
+    fast = slow;
+    slow = tp;
+    tp = ft;
+    ft = st;
+    st = tp;
+  }
+  const int lbF = ma_lookback(fast, ft), lbS = ma_lookback(slow, st), lbG = ma_lookback(sig, sgt);
+  if (lbF < 0 || lbS < 0 || lbG < 0) return ORC_BAD;
+  fill_nan(m, n);
+  fill_nan(s, n);
+  fill_nan(h, n);
+  const int f = lead_nan(x, n);
+  const int L = lbF > lbS ? lbF : lbS;
+  if (f + L + lbG >= n) return ORC_OK;
+  double* in = (double*)malloc(sizeof(double) * (size_t)n);
+  double* a = (double*)malloc(sizeof(double) * (size_t)n);
+  double* b = (double*)malloc(sizeof(double) * (size_t)n);
+  for (int t = 0; t < n; ++t) in[t] = (double)x[t];
+  md_ma(in, a, f + L - lbF, n, fast, ft);
+  md_ma(in, b, f + L - lbS, n, slow, st);
+  for (int t = f + L; t < n; ++t) a[t] -= b[t]; /* macd line from L */
+  md_ma(a, b, f + L, n, sig, sgt);
+  for (int t = f + L + lbG; t < n; ++t) {
+    m[t] = (float)a[t];
+    s[t] = (float)b[t];
+    h[t] = (float)(a[t] - b[t]);
+  }
+  free(in);
+  free(a);
+  free(b);
+  return ORC_OK;
+}

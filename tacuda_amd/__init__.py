@@ -107,6 +107,17 @@ for _sfx in ("_ex", "_ex_d"):
     _sig(f"ct_atr{_sfx}", P, P, P, P, c_int, c_int, c_float, c_int, c_int)
     _sig(f"ct_macd{_sfx}", P, P, P, P, c_int, c_int, c_int, c_int, c_int, c_int, c_int)
     _sig(f"ct_bbands{_sfx}", P, P, P, P, c_int, c_int, c_float, c_float, c_int, c_int)
+# TRIMA and MACDEXT (a moving-average type per line)
+_sig("ct_trima_d", P, P, c_int, c_int)
+_sig("ct_trima_device", P, P, c_int, c_int)
+_sig("ct_trima", P, P, c_int, c_int)
+_sig("ct_trima_batch", P, P, c_int, c_int, c_int, c_int)
+_sig("ct_trima_batch_d", P, P, c_int, c_int, c_int, c_int)
+_sig("ct_macdext_d", P, P, P, P, c_int, c_int, c_int, c_int, c_int, c_int, c_int)
+_sig("ct_macdext", P, P, P, P, c_int, c_int, c_int, c_int, c_int, c_int, c_int)
+for _sfx in ("_batch", "_batch_d"):
+    _sig(f"ct_macdext{_sfx}", P, P, P, P, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
+         c_int)
 _sig("ct_device_upload_ex", P, P, c_int, c_int)
 _sig("ct_device_download_ex", P, P, c_int, c_int)
 _sig("ctx_host_staging", c_int, stream=False)

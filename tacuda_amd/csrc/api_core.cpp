@@ -20,6 +20,7 @@ thread_local std::string g_error;
 
 std::mutex g_ws_mu;
 WsRegistry g_ws;
+WsRegistry g_tmp;  // whole-row scratch, keyed like the workspaces
 }  // namespace
 
 thread_local const Ragged* t_ragged = nullptr;
@@ -112,6 +113,33 @@ ctStatus_t workspace_for(hipStream_t stream, size_t treeBytes, Ws* out, int part
   return CT_STATUS_SUCCESS;
 }
 
+ctStatus_t temp_for(hipStream_t stream, size_t bytes, int slot, void** out) {
+  *out = nullptr;
+  if (slot < 0 || slot >= kTmpSlots) return fail(CT_STATUS_INVALID_ARGUMENT, "temp slot");
+  if (t_ws_fixed)
+    return fail(CT_STATUS_UNSUPPORTED, "this call needs row scratch the allocation-free entry "
+                                       "point does not provide");
+  int dev = 0;
+  TCA_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  const WsRegistry::Key key{dev, stream, slot};
+  void* base = g_tmp.find(key, bytes);
+  if (base == nullptr) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (stream != nullptr) TCA_HIP(hipStreamIsCapturing(stream, &cs));
+    if (cs != hipStreamCaptureStatusNone)
+      return fail(CT_STATUS_UNSUPPORTED,
+                  "row scratch of %zu bytes is not reserved for this stream; run the call once "
+                  "eagerly before capturing",
+                  bytes);
+    const size_t want = g_tmp.grow_size(key, bytes);
+    TCA_HIP(hipMalloc(&base, want));
+    g_tmp.install(key, base, want);
+  }
+  *out = base;
+  return CT_STATUS_SUCCESS;
+}
+
 void workspace_release(hipStream_t stream) {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return;
@@ -119,6 +147,7 @@ void workspace_release(hipStream_t stream) {
   {
     std::lock_guard<std::mutex> lk(g_ws_mu);
     bufs = g_ws.release_stream(dev, stream);
+    for (const auto& b : g_tmp.release_stream(dev, stream)) bufs.push_back(b);
   }
   if (bufs.empty()) return;
   (void)hipStreamSynchronize(stream);
@@ -130,6 +159,7 @@ void workspace_release_all() {
   {
     std::lock_guard<std::mutex> lk(g_ws_mu);
     bufs = g_ws.release_all();
+    for (const auto& b : g_tmp.release_all()) bufs.push_back(b);
   }
   int cur = 0;
   (void)hipGetDevice(&cur);
@@ -143,7 +173,7 @@ void workspace_release_all() {
 
 size_t workspace_allocations() {
   std::lock_guard<std::mutex> lk(g_ws_mu);
-  return g_ws.allocations();
+  return g_ws.allocations() + g_tmp.allocations();
 }
 
 }  // namespace tca

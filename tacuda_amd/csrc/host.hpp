@@ -67,6 +67,11 @@ struct WsFixedScope {
   ~WsFixedScope() { t_ws_fixed = nullptr; }
 };
 void workspace_release_all();
+// Per-stream temporary device buffers (slot 0..kTmpSlots-1) for kernels that
+// need whole-row scratch; growth follows the workspace rules (refused under
+// capture, old buffers retired until the stream's release or pool cleanup).
+constexpr int kTmpSlots = 2;
+ctStatus_t temp_for(hipStream_t stream, size_t bytes, int slot, void** out);
 void workspace_release(hipStream_t stream);
 size_t workspace_allocations();
 

@@ -77,7 +77,7 @@ ctStatus_t ex_types(const char* fn, ctDataType_t storage, ctDataType_t compute) 
 }  // namespace
 
 // ct_ma / ct_ma_device (reference MA.cu:23-113): the selected moving average
-// of the hot-path families; KAMA takes TA-Lib's 2/30 and T3 its 0.7.
+// (SMA, EMA, WMA, DEMA, TEMA, TRIMA, KAMA with TA-Lib's 2/30, T3 with 0.7).
 ctStatus_t dev_ma(const float* x, float* y, int len, int rows, int p, int type, hipStream_t s) {
   switch (type) {
     case CT_MA_SMA: return dev_sma(x, y, len, rows, p, s);
@@ -85,6 +85,7 @@ ctStatus_t dev_ma(const float* x, float* y, int len, int rows, int p, int type, 
     case CT_MA_WMA: return dev_wma(x, y, len, rows, p, s);
     case CT_MA_DEMA: return dev_dema(x, y, len, rows, p, s);
     case CT_MA_TEMA: return dev_tema(x, y, len, rows, p, s);
+    case CT_MA_TRIMA: return dev_trima(x, y, len, rows, p, s);
     case CT_MA_KAMA: return dev_kama(x, y, len, rows, p, 2, 30, s);
     case CT_MA_T3: return dev_t3(x, y, len, rows, p, 0.7f, s);
     default: break;

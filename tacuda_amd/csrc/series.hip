@@ -345,8 +345,10 @@ ctStatus_t dev_macd(const float* x, float* m, float* sg, float* h, int len, int 
   CHECK_PTR(x, "macd"); CHECK_PTR(m, "macd"); CHECK_PTR(sg, "macd"); CHECK_PTR(h, "macd");
   CHECK_RET(check_shape("macd", len, rows));
   if (fast <= 0 || slow <= 0 || sig <= 0) return bad("macd", "invalid periods");
+  // other moving-average types: TA-Lib MACDEXT with that type on all three
+  // lines (reference MACDEXT.cu:164-235)
   if (maType != CT_MA_EMA && maType != CT_MA_DEFAULT)
-    return fail(CT_STATUS_UNSUPPORTED, "macd: only the EMA moving-average type is provided");
+    return dev_macdext(x, m, sg, h, len, rows, fast, maType, slow, maType, sig, maType, s);
   if (slow < fast) std::swap(slow, fast);
   {
     ctxPackParams pp = pipe_params();

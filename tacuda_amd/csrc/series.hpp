@@ -52,6 +52,10 @@ ctStatus_t dev_dx(const float* h, const float* l, const float* c, float* y, int 
                   int p, hipStream_t s);
 ctStatus_t dev_adx(const float* h, const float* l, const float* c, float* y, int len, int rows,
                    int p, hipStream_t s);
+ctStatus_t dev_trima(const float* x, float* y, int len, int rows, int p, hipStream_t s);
+// TA-Lib MACDEXT: a moving-average type (TA_MAType numbering) per line.
+ctStatus_t dev_macdext(const float* x, float* m, float* sg, float* h, int len, int rows, int fast,
+                       int ft, int slow, int st, int sig, int sgt, hipStream_t s);
 // The moving average `type` (ctMaType_t) of the families above.
 ctStatus_t dev_ma(const float* x, float* y, int len, int rows, int p, int type, hipStream_t s);
 }  // namespace tca

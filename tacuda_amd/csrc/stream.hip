@@ -14,9 +14,28 @@
 #include "../../include/tacuda_stream.h"
 #include "host.hpp"
 #include "stream_states.hpp"
+#ifdef TCA_STREAM_FAULT_INJECTION
+#include <atomic>
+
+#include "stream_faults.h"
+#endif
 
 namespace tca {
 namespace {
+
+// Controlled failures for the state-transition tests: only the
+// fault-injection build (libtacuda_amd_faults.so) has the hook; elsewhere
+// FAULT() is a constant false.
+#ifdef TCA_STREAM_FAULT_INJECTION
+std::atomic<int> g_next_fault{0};
+bool consume_fault(int site) {
+  int expected = site;
+  return g_next_fault.compare_exchange_strong(expected, 0);
+}
+#define FAULT(site) consume_fault(site)
+#else
+#define FAULT(site) false
+#endif
 
 using st::State;
 
@@ -43,7 +62,21 @@ struct Batch {
   size_t pinFloats = 0;
   float* dstage = nullptr;
   size_t dstageFloats = 0;
+  // A GPU append that failed from its kernel launch on may have advanced the
+  // device state past what the caller received: the handle then refuses
+  // work until reset (reference api_stream_batch.cu:187-189, 219-231).
+  bool failed = false;
 };
+
+ctStatus_t refuse_failed() {
+  return fail(CT_STATUS_DEVICE_ERROR,
+              "stream batch: an earlier GPU append failed after its launch and left the state "
+              "undefined; reset the handle");
+}
+ctStatus_t mark_failed(Batch* b, ctStatus_t st, const char* what) {
+  b->failed = true;
+  return fail(st, "stream batch: %s", what);
+}
 
 // Opaque tokens: increasing integers, never reused while loaded.
 std::mutex g_reg_mu;
@@ -166,6 +199,8 @@ ctStatus_t wait_user(Batch* b) {
   return CT_STATUS_SUCCESS;
 }
 
+// Failures here leave the host copy current and untouched: the caller may
+// retry.
 ctStatus_t to_device(Batch* b) {
   if (b->onDevice) return CT_STATUS_SUCCESS;
   if (!b->ds) {
@@ -173,6 +208,8 @@ ctStatus_t to_device(Batch* b) {
     if (b->ringSize) TCA_HIP(hipMalloc(&b->dring, sizeof(float) * b->rows * (size_t)b->ringSize));
   }
   if (!b->stream) TCA_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  if (FAULT(CT_STREAM_BATCH_FAULT_STATE_UPLOAD))
+    return fail(CT_STATUS_COPY_FAILED, "stream batch: state upload failed");
   TCA_HIP(hipMemcpy(b->ds, b->hs.data(), sizeof(State) * b->rows, hipMemcpyHostToDevice));
   if (b->ringSize)
     TCA_HIP(hipMemcpy(b->dring, b->hring.data(), sizeof(float) * b->hring.size(),
@@ -181,14 +218,23 @@ ctStatus_t to_device(Batch* b) {
   return CT_STATUS_SUCCESS;
 }
 
+// A failed state download leaves neither copy trustworthy (the host copy may
+// be partly overwritten): the handle refuses work until reset.
 ctStatus_t to_host(Batch* b) {
   if (!b->onDevice) return CT_STATUS_SUCCESS;
   ctStatus_t r = wait_user(b);
   if (r != CT_STATUS_SUCCESS) return r;
-  TCA_HIP(hipMemcpy(b->hs.data(), b->ds, sizeof(State) * b->rows, hipMemcpyDeviceToHost));
-  if (b->ringSize)
-    TCA_HIP(hipMemcpy(b->hring.data(), b->dring, sizeof(float) * b->hring.size(),
-                      hipMemcpyDeviceToHost));
+  hipError_t e = FAULT(CT_STREAM_BATCH_FAULT_STATE_DOWNLOAD)
+                     ? hipErrorUnknown
+                     : hipMemcpy(b->hs.data(), b->ds, sizeof(State) * b->rows,
+                                 hipMemcpyDeviceToHost);
+  if (e == hipSuccess && b->ringSize)
+    e = hipMemcpy(b->hring.data(), b->dring, sizeof(float) * b->hring.size(),
+                  hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return mark_failed(b, CT_STATUS_COPY_FAILED, "state download failed");
+  }
   b->onDevice = false;
   return CT_STATUS_SUCCESS;
 }
@@ -244,6 +290,7 @@ ctStatus_t batch_append(ctStreamBatch_t* h, int kind, const float* hi, const flo
   if (!o0 || (nout == 3 && (!o1 || !o2)))
     return fail(CT_STATUS_INVALID_ARGUMENT, "stream batch: null output");
   const size_t tot = (size_t)b->rows * n;
+  if (b->failed) return refuse_failed();
   if (!use_gpu(b)) {
     ctStatus_t r = to_host(b);
     if (r != CT_STATUS_SUCCESS) return r;
@@ -272,6 +319,8 @@ ctStatus_t batch_append(ctStreamBatch_t* h, int kind, const float* hi, const flo
   if (r != CT_STATUS_SUCCESS) return r;
   // planes: 4 inputs then 3 outputs
   const size_t need = 7 * tot;
+  if (FAULT(CT_STREAM_BATCH_FAULT_IO_ALLOC))
+    return fail(CT_STATUS_ALLOC_FAILED, "stream batch: staging allocation failed");
   if (b->pinFloats < need) {
     if (b->pin) TCA_HIP(hipHostFree(b->pin));
     b->pin = nullptr;
@@ -289,6 +338,11 @@ ctStatus_t batch_append(ctStreamBatch_t* h, int kind, const float* hi, const flo
   for (int q = 0; q < 4; ++q) {
     if (!ins[q]) continue;
     std::memcpy(b->pin + q * tot, ins[q], tot * sizeof(float));
+    if (FAULT(CT_STREAM_BATCH_FAULT_INPUT_UPLOAD)) {
+      // queued uploads of other inputs only touch staging
+      (void)hipStreamSynchronize(b->stream);
+      return fail(CT_STATUS_COPY_FAILED, "stream batch: input upload failed");
+    }
     TCA_HIP(hipMemcpyAsync(b->dstage + q * tot, b->pin + q * tot, tot * sizeof(float),
                            hipMemcpyHostToDevice, b->stream));
     dins[q] = b->dstage + q * tot;
@@ -296,13 +350,24 @@ ctStatus_t batch_append(ctStreamBatch_t* h, int kind, const float* hi, const flo
   float* douts[3] = {b->dstage + 4 * tot, nout == 3 ? b->dstage + 5 * tot : nullptr,
                      nout == 3 ? b->dstage + 6 * tot : nullptr};
   const int blocks = (b->rows + 255) / 256;
+  // From the launch on, the device state may advance past what the caller
+  // receives: any failure marks the handle.
   hipLaunchKernelGGL(k_stream_batch, dim3(blocks), dim3(256), 0, b->stream, b->ds, b->dring,
                      b->ringSize, b->rows, dins[0], dins[1], dins[2], dins[3], n, douts[0],
                      douts[1], douts[2]);
-  TCA_HIP(hipGetLastError());
-  TCA_HIP(hipMemcpyAsync(b->pin + 4 * tot, douts[0], nout * tot * sizeof(float),
-                         hipMemcpyDeviceToHost, b->stream));
-  TCA_HIP(hipStreamSynchronize(b->stream));
+  if (hipGetLastError() != hipSuccess)
+    return mark_failed(b, CT_STATUS_KERNEL_FAILED, "kernel launch failed");
+  if (hipStreamSynchronize(b->stream) != hipSuccess || FAULT(CT_STREAM_BATCH_FAULT_KERNEL)) {
+    (void)hipGetLastError();
+    return mark_failed(b, CT_STATUS_KERNEL_FAILED, "kernel execution failed");
+  }
+  if (FAULT(CT_STREAM_BATCH_FAULT_OUTPUT_DOWNLOAD) ||
+      hipMemcpyAsync(b->pin + 4 * tot, douts[0], nout * tot * sizeof(float),
+                     hipMemcpyDeviceToHost, b->stream) != hipSuccess ||
+      hipStreamSynchronize(b->stream) != hipSuccess) {
+    (void)hipGetLastError();
+    return mark_failed(b, CT_STATUS_COPY_FAILED, "output download failed");
+  }
   float* outs[3] = {o0, o1, o2};
   for (int q = 0; q < nout; ++q)
     std::memcpy(outs[q], b->pin + (4 + q) * tot, tot * sizeof(float));
@@ -500,6 +565,7 @@ CTAPI_EXPORT ctStatus_t ctx_stream_batch_append_d(ctStreamBatch_t* h, const floa
     const int nout = st::outputs(b->proto);
     if (!o0 || (nout == 3 && (!o1 || !o2)))
       return fail(CT_STATUS_INVALID_ARGUMENT, "stream batch: null output");
+    if (b->failed) return refuse_failed();
     if (b->userPending && b->lastUser != stream) {
       ctStatus_t r = wait_user(b);
       if (r != CT_STATUS_SUCCESS) return r;
@@ -512,7 +578,8 @@ CTAPI_EXPORT ctStatus_t ctx_stream_batch_append_d(ctStreamBatch_t* h, const floa
                        b->ringSize, b->rows, (need & 1) ? dh : nullptr, (need & 2) ? dl : nullptr,
                        (need & 4) ? dc : nullptr, (need & 8) ? dv : nullptr, n, o0,
                        nout == 3 ? o1 : nullptr, nout == 3 ? o2 : nullptr);
-    TCA_HIP(hipGetLastError());
+    if (hipGetLastError() != hipSuccess)
+      return mark_failed(b, CT_STATUS_KERNEL_FAILED, "kernel launch failed");
     b->lastUser = stream;
     b->userPending = true;
     b->bars += n;
@@ -592,6 +659,7 @@ CTAPI_EXPORT ctStatus_t ct_stream_batch_reset(ctStreamBatch_t* h) {
     std::fill(b->hs.begin(), b->hs.end(), b->proto);
     std::fill(b->hring.begin(), b->hring.end(), 0.0f);
     b->onDevice = false;
+    b->failed = false;
     b->bars = 0;
     return CT_STATUS_SUCCESS;
   });
@@ -634,10 +702,17 @@ CTAPI_EXPORT ctStatus_t ct_stream_batch_set_exec(ctStreamBatch_t* h, ctStreamExe
     if (!b) return fail(CT_STATUS_INVALID_ARGUMENT, "stream batch: invalid or destroyed handle");
     if (exec != CT_STREAM_EXEC_AUTO && exec != CT_STREAM_EXEC_CPU && exec != CT_STREAM_EXEC_GPU)
       return fail(CT_STATUS_INVALID_ARGUMENT, "stream batch: unknown execution mode");
+    // the states move with the next append (reference
+    // api_stream_batch.cu:464-482)
     b->exec = exec;
-    if (!use_gpu(b)) return to_host(b);
     return CT_STATUS_SUCCESS;
   });
 }
+
+#ifdef TCA_STREAM_FAULT_INJECTION
+CTAPI_EXPORT void ct_test_stream_batch_fail_next(ctStreamBatchFault_t site) {
+  g_next_fault.store(static_cast<int>(site));
+}
+#endif
 
 }  // extern "C"

@@ -116,6 +116,29 @@ def macd(x, fast: int = 12, slow: int = 26, signal: int = 9, ma_type: int = 1):
     return m, s, h
 
 
+def trima(x, period: int):
+    """TA-Lib TRIMA (ct_trima_d / ct_trima_batch_d)."""
+    _check(x)
+    y = torch.empty_like(x)
+    _dispatch("trima", x, [x], [y], int(period))
+    return y
+
+
+MA_TYPES = {"sma": 0, "ema": 1, "wma": 2, "dema": 3, "tema": 4, "trima": 5, "kama": 6, "t3": 8}
+
+
+def macdext(x, fast: int = 12, fast_type="ema", slow: int = 26, slow_type="ema", signal: int = 9,
+            signal_type="ema"):
+    """TA-Lib MACDEXT: a moving-average type per line (names of MA_TYPES or
+    ctMaType_t values), ct_macdext_d / ct_macdext_batch_d."""
+    _check(x)
+    t = [MA_TYPES.get(v, v) if isinstance(v, str) else int(v)
+         for v in (fast_type, slow_type, signal_type)]
+    m, s, h = (torch.empty_like(x) for _ in range(3))
+    _dispatch("macdext", x, [x], [m, s, h], int(fast), t[0], int(slow), t[1], int(signal), t[2])
+    return m, s, h
+
+
 def bbands(x, period: int = 20, up: float = 2.0, dn: float = 2.0):
     _check(x)
     u, m, l = (torch.empty_like(x) for _ in range(3))

@@ -80,7 +80,6 @@ def test_invalid_arguments_fail_before_touching_the_device():
     assert L.ct_sma_d(p, p, 0, 5, None) == 4
     assert L.ct_sma_d(p, p, 4, 0, None) == 4
     assert L.ct_t3_d(p, p, 4, 1, 0.7, None) == 4
-    assert L.ct_macd_d(p, p, p, p, 4, 12, 26, 9, 0, None) == 6
     assert L.ct_stochastic_d(p, p, p, p, p, 4, 14, 0, None) == 4
     # moving-average types outside the hot path, and storage types other
     # than float32, are refused before any device work
@@ -91,6 +90,14 @@ def test_invalid_arguments_fail_before_touching_the_device():
     assert L.ct_ema_ex_d(p, p, 4, 3, 7, 1, None) == 6  # not a data type
     assert L.ct_sma_ex(None, p, 4, 3, 1, 1, None) == 4
     assert L.ctx_host_staging(3) == 4
+    # MACDEXT: MAMA and the moving averages above T3 are not provided; TA-Lib's
+    # period ranges are enforced
+    assert L.ct_macdext_d(p, p, p, p, 4, 2, 7, 3, 0, 2, 0, None) == 6
+    assert L.ct_macdext_d(p, p, p, p, 4, 2, 0, 3, 12, 2, 0, None) == 6
+    assert L.ct_macdext_d(p, p, p, p, 4, 2, 0, 3, 0, 2, 99, None) == 4
+    assert L.ct_macdext_d(p, p, p, p, 4, 1, 0, 3, 0, 2, 0, None) == 4
+    assert L.ct_macd_d(p, p, p, p, 4, 12, 26, 9, 7, None) == 6
+    assert L.ct_trima_d(p, p, 4, 0, None) == 4
 
 
 def test_host_aroon_short_series_is_all_nan_without_the_device():
@@ -206,3 +213,15 @@ def test_kernel_timing_without_the_device():
     assert L.ctx_kernel_times(None, 0, None, 0, ctypes.byref(n)) == 0 and n.value == 0
     assert L.ctx_kernel_timing(1) == 0 and L.ctx_kernel_timing(0) == 0
     assert ta.kernel_times() == []
+
+
+def test_fault_hook_only_in_the_fault_injection_build():
+    """ct_test_stream_batch_fail_next exists only in the test build
+    libtacuda_amd_faults.so, not in the product library."""
+    prod = subprocess.run(["nm", "-D", "--defined-only", _lib_path()], capture_output=True,
+                          text=True, check=True).stdout
+    assert "ct_test_stream_batch_fail_next" not in prod
+    faults = os.path.join(ROOT, "tacuda_amd", "libtacuda_amd_faults.so")
+    test = subprocess.run(["nm", "-D", "--defined-only", faults], capture_output=True, text=True,
+                          check=True).stdout
+    assert "ct_test_stream_batch_fail_next" in test

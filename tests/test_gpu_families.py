@@ -176,8 +176,9 @@ def test_invalid_arguments(dev):
         with pytest.raises(ta.TacudaError) as e:
             bad()
         assert e.value.status == 4
+    # MAMA is the one TA-Lib moving average MACD does not provide
     with pytest.raises(ta.TacudaError) as e:
-        ops.macd(c, 12, 26, 9, ma_type=0)
+        ops.macd(c, 12, 26, 9, ma_type=7)
     assert e.value.status == 6
 
 

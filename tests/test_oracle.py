@@ -433,3 +433,58 @@ def test_willr_pinned_by_stochf(ds, p):
     assert np.array_equal(np.isnan(w), np.isnan(k))
     ok = ~np.isnan(k)
     np.testing.assert_allclose(w[ok], k[ok].astype(np.float64) - 100.0, rtol=0, atol=3e-5)
+
+
+@needs_ref
+@pytest.mark.parametrize("ds", ["rw0", "rw1", "short", "zeros"])
+@pytest.mark.parametrize("p", [1, 2, 3, 4, 5, 10, 11, 30])
+def test_trima_matches_reference_talib(ds, p):
+    o, h, l, c = DATASETS[ds]
+    _bits_equal(ORC.trima(c, p), REF.trima(c, p))
+
+
+MA_TYPES = [0, 1, 2, 3, 4, 5, 6, 8]  # TA_MAType without MAMA
+
+
+@needs_ref
+@pytest.mark.parametrize("ds", ["rw0", "rw1"])
+@pytest.mark.parametrize("sgt", MA_TYPES)
+def test_macdext_matches_reference_talib(ds, sgt):
+    """Every fast/slow/signal moving-average combination (and reversed
+    periods, and one-bar periods) against the vendored TA_MACDEXT: identical
+    NaN ranges, values within 1e-5 (5e-5 with a KAMA signal line, whose
+    efficiency ratio amplifies the float32 rounding of the macd line;
+    corpus tolerance 1e-4)."""
+    o, h, l, c = DATASETS[ds]
+    tol = 5e-5 if sgt == 6 else 1e-5
+    cases = [(12, ft, 26, st, 9, sgt) for ft in MA_TYPES for st in MA_TYPES]
+    cases += [(26, 3, 12, 0, 9, sgt), (5, 8, 3, 6, 1, sgt), (2, 0, 7, 2, 4, sgt)]
+    for args in cases:
+        got, want = ORC.macdext(c, *args), REF.macdext(c, *args)
+        for j, (g, w) in enumerate(zip(got, want)):
+            assert np.array_equal(np.isnan(g), np.isnan(w)), (args, j)
+            ok = ~np.isnan(w)
+            rel = np.abs(g[ok].astype(np.float64) - w[ok]) / np.maximum(
+                1.0, np.maximum(np.abs(g[ok]), np.abs(w[ok])))
+            assert rel.size == 0 or rel.max() <= tol, (args, j, rel.max())
+
+
+@needs_ref
+def test_macdext_skips_a_leading_nan_prefix():
+    """The formula backend convention: a NaN prefix is skipped and the rest
+    is TA_MACDEXT of the remaining bars; one-bar fast or slow periods are
+    rejected as TA_MACDEXT rejects them."""
+    o, h, l, c = DATASETS["rw0"]
+    x = c.copy()
+    x[:37] = np.nan
+    with pytest.raises(ValueError):
+        ORC.macdext(c, 1, 0, 7, 0, 4, 0)
+    with pytest.raises(ValueError):
+        REF.macdext(c, 1, 0, 7, 0, 4, 0)
+    got = ORC.macdext(x, 10, 2, 21, 4, 7, 0)
+    want = REF.macdext(c[37:], 10, 2, 21, 4, 7, 0)
+    for g, w in zip(got, want):
+        assert np.isnan(g[:37]).all()
+        ok = ~np.isnan(w)
+        assert np.array_equal(np.isnan(g[37:]), np.isnan(w))
+        np.testing.assert_allclose(g[37:][ok], w[ok], rtol=0, atol=1e-5)
