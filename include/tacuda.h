@@ -238,6 +238,9 @@ CTAPI_EXPORT ctStatus_t ct_aroonosc_batch_d(const float* d_highs, const float* d
 /* :1356 / :1363 (down before up, as in the reference) */
 CTAPI_EXPORT ctStatus_t ct_aroon_batch(const float* host_highs, const float* host_lows, float* host_aroon_downs, float* host_aroon_ups, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
 CTAPI_EXPORT ctStatus_t ct_aroon_batch_d(const float* d_highs, const float* d_lows, float* d_aroon_downs, float* d_aroon_ups, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1581 / :1586 -- moving average selected by type (see ct_ma) */
+CTAPI_EXPORT ctStatus_t ct_ma_batch(const float* host_inputs, float* host_outputs, int seriesLen, int numSeries, int period, int maType, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_ma_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int period, int maType, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
 /* :982 / :987 */
 CTAPI_EXPORT ctStatus_t ct_trima_batch(const float* host_inputs, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
 CTAPI_EXPORT ctStatus_t ct_trima_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);

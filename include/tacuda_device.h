@@ -62,6 +62,8 @@ CTAPI_EXPORT ctStatus_t ct_cci_d(const float* d_high, const float* d_low, const 
 /* :280 bands */
 CTAPI_EXPORT ctStatus_t ct_bbands_d(const float* d_input, float* d_upper, float* d_middle, float* d_lower, int size, int period, float nbDevUp, float nbDevDn, hipStream_t stream CTAPI_STREAM_DEFAULT);
 
+/* :468 -- moving average selected by type (see ct_ma in tacuda.h) */
+CTAPI_EXPORT ctStatus_t ct_ma_d(const float* d_input, float* d_output, int size, int period, int maType, hipStream_t stream CTAPI_STREAM_DEFAULT);
 /* :151 -- TA-Lib TRIMA (triangular weights), window semantics for NaN */
 CTAPI_EXPORT ctStatus_t ct_trima_d(const float* d_input, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
 /* :388 -- TA-Lib MACDEXT with a moving-average type per line (ctMaType_t:

@@ -99,6 +99,9 @@ for _n in ("sma", "ema", "wma", "dema", "tema", "rsi", "stddev", "var", "sum", "
 _sig("ct_t3_device", P, P, c_int, c_int, c_float)
 _sig("ct_kama_device", P, P, c_int, c_int, c_int, c_int)
 _sig("ct_ma_device", P, P, c_int, c_int, c_int)
+_sig("ct_ma_d", P, P, c_int, c_int, c_int)
+_sig("ct_ma_batch", P, P, c_int, c_int, c_int, c_int, c_int)
+_sig("ct_ma_batch_d", P, P, c_int, c_int, c_int, c_int, c_int)
 # typed variants: storage type, compute type (include/tacuda.h, tacuda_device.h)
 for _n in ("sma", "ema", "rsi"):
     _sig(f"ct_{_n}_ex", P, P, c_int, c_int, c_int, c_int)

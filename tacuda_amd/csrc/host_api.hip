@@ -454,6 +454,28 @@ CTAPI_EXPORT ctStatus_t ct_t3_device(const float* x, float* y, int n, int p, flo
 CTAPI_EXPORT ctStatus_t ct_kama_device(const float* x, float* y, int n, int p, int fast, int slow, hipStream_t s) {
   return guarded([&] { return dev_kama(x, y, n, 1, p, fast, slow, s); });
 }
+CTAPI_EXPORT ctStatus_t ct_ma_d(const float* x, float* y, int n, int p, int type, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    R_(check_shape("ma", n, 1));
+    return dev_ma(x, y, n, 1, p, type, s);
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_ma_batch_d(const float* x, float* y, int len, int rows, int p, int type, int, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    R_(check_shape("ma", len, rows));
+    R_(dev_ma(x, y, len, rows, p, type, s));
+    return finish_batch(s);
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_ma_batch(const float* hx, float* hy, int len, int rows, int p, int type, int, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    R_(check_shape("ma", len, rows));
+    const size_t n = elems(len, rows);
+    HostRun r(s); const float* x; float* y;
+    R_(r.in(hx, n, &x)); R_(r.out(hy, n, &y)); R_(dev_ma(x, y, len, rows, p, type, s));
+    return r.finish();
+  });
+}
 CTAPI_EXPORT ctStatus_t ct_ma_device(const float* x, float* y, int n, int p, ctMaType_t type, hipStream_t s) {
   return guarded([&]() -> ctStatus_t {
     R_(check_shape("ma", n, 1));

@@ -225,3 +225,34 @@ def test_fault_hook_only_in_the_fault_injection_build():
     test = subprocess.run(["nm", "-D", "--defined-only", faults], capture_output=True, text=True,
                           check=True).stdout
     assert "ct_test_stream_batch_fail_next" in test
+
+
+def test_reference_abi_of_the_hot_path():
+    """Every hot-path entry point the reference's public headers declare
+    (tests/golden/reference_abi.json, written by make_abi_fixture.py from
+    include/tacuda.h, tacuda_device.h and tacuda_stream.h: the indicator
+    families' host, device, batch and typed forms, the 61 candle calls and
+    the legacy engulfing names, the streaming surface, memory and error
+    calls) is declared here with the same return and argument types
+    (hipStream_t for cudaStream_t) and exported by the library."""
+    import json
+    import sys
+
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    from make_abi_fixture import prototypes
+
+    fx = json.load(open(os.path.join(ROOT, "tests", "golden", "reference_abi.json")))
+    assert len(fx) > 400
+    ours = {}
+    for h in HEADERS:
+        txt = subprocess.run(["gcc", "-E", "-P", "-x", "c", h], capture_output=True, text=True,
+                             check=True).stdout
+        ours.update(prototypes(txt.replace('__attribute__((visibility("default")))',
+                                           "CTAPI_EXPORT")))
+    missing = sorted(n for n in fx if n not in ours)
+    assert not missing, f"not declared: {missing}"
+    differ = {n: (fx[n], ours[n]) for n in fx if fx[n] != ours[n]}
+    assert not differ, f"prototypes differ: {differ}"
+    lib = ctypes.CDLL(_lib_path())
+    unexported = [n for n in fx if not hasattr(lib, n)]
+    assert not unexported, f"not exported: {unexported}"

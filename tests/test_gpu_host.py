@@ -346,3 +346,32 @@ def test_stoch_long_smoothing(dev, kd):
     _exact(k.cpu().numpy(), want[0], f"stoch ragged {kd} k")
     _exact(dd.cpu().numpy(), want[1], f"stoch ragged {kd} d")
     lay.close()
+
+
+def test_ma_d_and_batch_forms(dev):
+    """ct_ma_d, ct_ma_batch_d and ct_ma_batch pick the same family by type."""
+    import tacuda_amd as ta
+    from oracle import ORC
+
+    print("ma batch forms", flush=True)
+    L = ta.lib()
+    rows, n = 11, 600
+    b = ta.synth_ohlcv(n, rows=rows, seed=41)["close"]
+    b = np.ascontiguousarray(b)
+    B = torch.from_numpy(b).to(dev)
+    s = torch.cuda.current_stream().cuda_stream
+    for t, fn in ((0, lambda x: ORC.sma(x, 9)), (3, lambda x: ORC.dema(x, 9)),
+                  (5, lambda x: ORC.trima(x, 9)), (6, lambda x: ORC.kama(x, 9))):
+        y = torch.empty_like(B)
+        assert L.ct_ma_batch_d(B.data_ptr(), y.data_ptr(), n, rows, 9, t, 0, s) == 0
+        torch.cuda.synchronize()
+        g = y.cpu().numpy()
+        for r in range(rows):
+            (_exact if t in (0, 5) else _scan)(g[r], fn(b[r]), f"ma batch type {t} row {r}")
+        hy = np.empty_like(b)
+        assert L.ct_ma_batch(_p(b), _p(hy), n, rows, 9, t, 0, None) == 0
+        assert np.array_equal(hy.view(np.uint32), g.view(np.uint32))
+        y1 = torch.empty_like(B[0])
+        assert L.ct_ma_d(B[0].data_ptr(), y1.data_ptr(), n, 9, t, s) == 0
+        torch.cuda.synchronize()
+        assert np.array_equal(y1.cpu().numpy().view(np.uint32), g[0].view(np.uint32))
