@@ -276,10 +276,10 @@ def _scan256(a, torch, ops, T, dev):
     T.keep.clear()
     torch.cuda.empty_cache()
     print(f"scan256 fused: {fused:.3f} ms/pass", file=sys.stderr, flush=True)
-    # per kernel: the fold reads the close rows, the emit reads them again
-    # and writes the four outputs
+    # per kernel: the one-pass scan reads the close rows once and writes the
+    # four outputs
     launch = kernel_launch_ms(T, [c], lambda x: ops.scan(x), 3)
-    kbytes = {"k_scan_fold": 4 * bars, "k_scan_emit": (1 + 4) * 4 * bars}
+    kbytes = {"k_scan_one": (1 + 4) * 4 * bars}
     kern = {k: {"ms": v, **({"alg_bytes": kbytes[k], "GBs": _gbs(kbytes[k], v),
                              "frac_hbm": _gbs(kbytes[k], v) / PEAK_HBM_GBS}
                             if k in kbytes else {})} for k, v in launch.items()}
@@ -339,7 +339,7 @@ def _scan1024(a, torch, ops, T, dev):
     torch.cuda.empty_cache()
     launch = kernel_launch_ms(T, [c], lambda x: ops.scan(x), 2)
     torch.cuda.empty_cache()
-    kbytes = {"k_scan_fold": 4 * bars, "k_scan_emit": (1 + 4) * 4 * bars}
+    kbytes = {"k_scan_one": (1 + 4) * 4 * bars}
     kern = {k: {"ms": v, **({"alg_bytes": kbytes[k], "GBs": _gbs(kbytes[k], v),
                              "frac_hbm": _gbs(kbytes[k], v) / PEAK_HBM_GBS}
                             if k in kbytes else {})} for k, v in launch.items()}

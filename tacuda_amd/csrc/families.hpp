@@ -113,32 +113,34 @@ __device__ __forceinline__ void lane_vals(const float* lds, float (&v)[T::K]) {
   }
 }
 
-// Writes the lane's K outputs.  Full, 16-byte aligned segments store
-// straight from registers as float4 (each lane owns K contiguous bars);
-// other segments go through LDS so the global stores coalesce.
+// Writes the lane's K outputs.  The wave transposes its segment through
+// ldo so that consecutive lanes store consecutive addresses: float4 runs
+// when the segment is full and 16-byte aligned, dwords otherwise.  Stores
+// of K contiguous bars per lane reach a fraction of that bandwidth on
+// MI355X (a quarter when nontemporal).
 template <class T>
 __device__ __forceinline__ void store_lane(float* ldo, const float (&v)[T::K], float* dst,
                                            const Seg& c) {
-  const long long g0 = c.rowOff + c.s0;
-  if (((g0 & 3) == 0) && (c.s0 + T::SEG <= c.n)) {
-    v4f* p = reinterpret_cast<v4f*>(dst + g0 + lane_id() * T::K);
-#pragma unroll
-    for (int q = 0; q < T::K / 4; ++q) {
-      v4f w = {v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]};
-      __builtin_nontemporal_store(w, p + q);
-    }
-    return;
-  }
   wave_sync();
   float* p = ldo + T::pos(lane_id() * T::K);
 #pragma unroll
   for (int q = 0; q < T::K / 4; ++q)
     *reinterpret_cast<float4*>(p + 4 * q) = make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
   wave_sync();
+  const long long g0 = c.rowOff + c.s0;
+  if (((g0 & 3) == 0) && (c.s0 + T::SEG <= c.n)) {
+    v4f* row = reinterpret_cast<v4f*>(dst + g0);
+#pragma unroll
+    for (int i = 0; i < T::K / 4; ++i) {
+      const int q = lane_id() + i * kWave;
+      __builtin_nontemporal_store(*reinterpret_cast<const v4f*>(ldo + T::pos(4 * q)), row + q);
+    }
+    return;
+  }
   float* row = dst + c.rowOff;
   for (int e = lane_id(); e < T::SEG; e += kWave) {
     const int t = c.s0 + e;
-    if (t < c.s1) row[t] = ldo[T::pos(e)];
+    if (t < c.s1) __builtin_nontemporal_store(ldo[T::pos(e)], row + t);
   }
 }
 

@@ -140,9 +140,9 @@ const Ragged* ragged_of(struct ctxRagged_st* layout);
 // synchronization.  scan_serves: whether the family calls of those
 // indicators route a dense batch to it -- at least kScanMinRows rows of at
 // least two 1024-bar segments, kScanMinSegments segments in all (batches
-// that fill the GPU, where its two streaming passes beat the look-back
-// scan; a few long rows leave its one-wave-per-row carry pass
-// latency-bound, measured slower), and a window look-back within its halo.
+// that fill the GPU; with a few long rows its per-row look-back chains leave
+// the GPU short of independent work), and a window look-back within its
+// halo.
 constexpr long long kScanMinSegments = 512;
 constexpr int kScanMinRows = 16;
 ctStatus_t scan_run(const float* close, int seriesLen, int numSeries, const ctxScanParams* params,

@@ -3,25 +3,29 @@
 // run as separate calls: EMA.cu, KAMA.cu, T3.cu, SMA.cu).  Results follow the
 // per-family definitions (ta_EMA.c, ta_KAMA.c, ta_T3.c, the reference SMA).
 //
-// Long rows are scanned in two streaming passes instead of a decoupled
-// look-back, so no workgroup ever waits on another:
-//   1. k_scan_lead  -- first non-NaN bar of every row (the seeds start there);
-//   2. k_scan_fold  -- every 1024-bar segment folds its bars from a zero state
-//                      and writes its composite (EMA offset, T3 6-vector
-//                      offset, KAMA affine pair);
-//   3. k_scan_carry -- one wave per row scans the segment composites into the
-//                      state entering every segment;
-//   4. k_scan_emit  -- every segment re-reads its bars, rebuilds the states
-//                      entering each lane from its carry and replays the bars
-//                      with the families' own sequential arithmetic.
-// The input is read twice and each output written once.
+// Two launches:
+//   1. k_scan_lead -- first non-NaN bar of every row (the seeds start there);
+//                     block 0 also opens the launch (epoch tag, stall flag)
+//                     and writes the power tables of the EMA and T3
+//                     transition maps;
+//   2. k_scan_one  -- one wave per 1024-bar segment reads its bars once and
+//                     folds them from a zero state; the workgroup's kLW
+//                     segments (a tile of one row) combine their composites
+//                     in LDS, publish the tile's composite, obtain the state
+//                     entering the tile by a decoupled look-back along the
+//                     row and publish the state leaving it; every wave then
+//                     replays its bars from its segment's entering state
+//                     with the families' own sequential arithmetic.
+// Workgroups take tiles column-major (the j-th tiles of a set of rows before
+// any (j+1)-th), so a workgroup only ever waits on tiles of workgroups
+// started before it (scan_tile_of), and each wait is bounded (kSpinLimit).
 //
 // Every recurrence here starts at a seed bar with everything before it
 // contributing exactly 0, so a run of L bars maps the state s to
-// M^L s + o: EMA has M = beta, T3's six cascaded EMAs form one linear
-// recurrence whose matrix is lower-triangular Toeplitz (entry (i, j) =
-// (1-k) k^(i-j)), so its powers are Toeplitz too (six numbers each) and are
-// computed on the host.  KAMA's multiplier (1 - sc_t) varies per bar and is
+// M^L s + o: EMA has M = beta; T3's six cascaded EMAs form one linear
+// recurrence, carried in the scaled state h_q = e_q / k^(q+1) whose matrix
+// is lower-triangular Toeplitz with every entry 1 - k, so its powers are
+// Toeplitz too (six numbers each) and are computed on the host.  KAMA's multiplier (1 - sc_t) varies per bar and is
 // carried as an (A, b) pair.  Powers of nonzero factors that underflow are
 // clamped to the smallest normal double, so a carried Inf survives them as
 // it does in the sequential recurrence.
@@ -49,17 +53,51 @@ struct Toe {
 };
 
 struct ScanP {
-  int n, rows, segsPerRow, cpl;  // cpl: segments per lane in the carry scan
+  int n, rows, segsPerRow, tilesPerRow;  // tile: kLW segments sharing a record
   unsigned mask;
   int useCarry;                  // rows span several segments (carries exist)
   int pe, pk, pt, ps;            // EMA, KAMA, T3, SMA periods
   double eb, ek;                 // EMA beta and weight (ta_EMA.c)
   double kmax, kdiff;            // KAMA constants (ta_KAMA.c)
   double tk, tomk;               // T3 weight and 1 - weight (ta_T3.c)
-  double c1, c2, c3, c4;         // T3 combination
-  double ePow[6], eSeg, eRow[6]; // beta^(K 2^s), beta^SEG, beta^(SEG cpl 2^s)
-  Toe tPow[6], tSeg, tRow[6];    // the same powers of the T3 matrix
+  double c1, c2, c3, c4;         // T3 combination of e3..e6, scaled to h3..h6
+  double tsc[6];                 // 1 / tk^(q+1): e_q -> h_q
+  double rps;                    // RN(1 / SMA period)
+  double ePow[6], eSeg, eTile;   // beta^(K 2^s), beta^SEG, beta^(SEG kLW)
+  Toe tPow[6], tSeg, tTile;      // the same powers of the T3 matrix
 };
+
+// Power tables written by k_scan_lead, kPowStride doubles an entry: entries
+// 0..63 hold beta^(K l) at [0] and the first column of M^(K l) at [1..6]
+// (entry 2^s is the multiplier of step s of the lane scan, entry l carries
+// a segment's entering state over lanes 0..l-1); entries 64..127 hold the
+// same for the tile map (kLW segments), beta^(SEG kLW l) and M^(SEG kLW l).
+constexpr int kPowStride = 8;
+constexpr int kPowEntries = 2 * 64;
+
+// Launch header of the scan's workspace: the tag of this launch's record
+// flags and the stall flag (both set by k_scan_lead).
+struct ScanHdr {
+  unsigned epoch;
+  unsigned stalled;  // set when a look-back wait gave up (see scan_lookback)
+  unsigned pad[62];
+};
+
+// Upper bound on the polls of one look-back wait (about a second of
+// s_sleep): the segment waited on is always running, so a wait this long
+// means a broken launch; the wave then carries NaN on and flags `stalled`
+// instead of hanging the device, and every other waiting wave gives up at
+// its next check of the flag.
+constexpr int kSpinLimit = 1 << 24;
+
+// One record per tile (kLW segments of a row, one workgroup): [0] flag (low
+// word: launch epoch; high word: kAgg once [1..9] holds the tile's
+// composite -- EMA offset, T3 6-vector offset, KAMA (A, b) -- then kIncl
+// once [10..17] holds the state leaving the tile -- EMA, T3 6-vector,
+// KAMA).  A row's first tile publishes kIncl only, so a look-back always
+// ends at a kIncl record.
+constexpr int kRec = 18;
+constexpr unsigned kAgg = 1u, kIncl = 2u;
 
 // y += T v
 __device__ __forceinline__ void toe_mv_add(const Toe& T, const double (&v)[6], double (&y)[6]) {
@@ -70,6 +108,26 @@ __device__ __forceinline__ void toe_mv_add(const Toe& T, const double (&v)[6], d
     for (int j = 0; j <= i; ++j) s = fma(T.c[i - j], v[j], s);
     y[i] = s;
   }
+}
+
+// x / d for a constant divisor d >= 1 with rd = RN(1 / d): Markstein's
+// correction of x * rd, the correctly rounded quotient for every finite x.
+__device__ __forceinline__ double div_by(double x, double d, double rd) {
+  const double q = x * rd;
+  return fma(fma(-q, d, x), rd, q);
+}
+
+// a / b for finite a and a normal b > 0: the hardware reciprocal refined
+// by two Newton steps, then the same correction; it equals the IEEE
+// quotient but for rare last-bit cases.
+__device__ __forceinline__ double div_nr(double a, double b) {
+  double y = __builtin_amdgcn_rcp(b);
+  double e = fma(-b, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-b, y, 1.0);
+  y = fma(y, e, y);
+  const double q = a * y;
+  return fma(fma(-q, b, a), y, q);
 }
 
 template <class T>
@@ -106,6 +164,9 @@ __device__ void t3_seed(const float* lds, const float* x, const Seg& c, int f, i
     }
     e[lev] = tmp / p;
   }
+  // to the scaled state h_q = e_q / tk^(q+1)
+#pragma unroll
+  for (int q = 0; q < 6; ++q) e[q] *= P.tsc[q];
 }
 
 // ------------------------------------------------------------ KAMA bar --
@@ -201,9 +262,9 @@ __device__ __forceinline__ Fold fold_lane(const float* lds, const float* x, cons
 #pragma unroll
         for (int q = 0; q < 6; ++q) F.bt[q] = S.st[q];
       } else if (t > S.L6) {
-        F.bt[0] = fma(P.tomk, F.bt[0], P.tk * v);
+        F.bt[0] = fma(P.tomk, F.bt[0], v);
 #pragma unroll
-        for (int q = 1; q < 6; ++q) F.bt[q] = fma(P.tomk, F.bt[q], P.tk * F.bt[q - 1]);
+        for (int q = 1; q < 6; ++q) F.bt[q] = fma(P.tomk, F.bt[q], F.bt[q - 1]);
       }
     }
     if (wK) {
@@ -221,9 +282,16 @@ __device__ __forceinline__ Fold fold_lane(const float* lds, const float* x, cons
   return F;
 }
 
+__device__ __forceinline__ Toe toe_at(const double* p) {
+  Toe t;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) t.c[i] = p[i];
+  return t;
+}
+
 // Inclusive lane scan of the folds (lane 0 earliest), of the requested
-// recurrences only.
-__device__ __forceinline__ void scan_lanes(const ScanP& P, Fold& F) {
+// recurrences only; pw is the per-lane power table.
+__device__ __forceinline__ void scan_lanes(const ScanP& P, const double* __restrict__ pw, Fold& F) {
   const int lane = lane_id();
   const bool wE = P.mask & kLEma, wT = P.mask & kLT3, wK = P.mask & kLKama;
 #pragma unroll
@@ -231,13 +299,13 @@ __device__ __forceinline__ void scan_lanes(const ScanP& P, Fold& F) {
     const int d = 1 << s;
     if (wE) {
       const double be = __shfl_up(F.be, d);
-      if (lane >= d) F.be = fma(P.ePow[s], be, F.be);
+      if (lane >= d) F.be = fma(pw[d * kPowStride], be, F.be);
     }
     if (wT) {
       double bt[6];
 #pragma unroll
       for (int q = 0; q < 6; ++q) bt[q] = __shfl_up(F.bt[q], d);
-      if (lane >= d) toe_mv_add(P.tPow[s], bt, F.bt);
+      if (lane >= d) toe_mv_add(toe_at(pw + d * kPowStride + 1), bt, F.bt);
     }
     if (wK) {
       const double Ak = __shfl_up(F.Ak, d), bk = __shfl_up(F.bk, d);
@@ -271,20 +339,63 @@ __device__ __forceinline__ Seg seg_of(const ScanP& P, int seg, int& row) {
 // lane loops carry no per-bar conditions.  Results equal the general path's
 // (the same operations in the same order).
 template <class T>
-__device__ __forceinline__ bool steady(const float* lds, const Seg& c, const ScanP& P,
-                                       const Seeds& S) {
-  if (c.s1 - c.s0 != kLSeg) return false;
+__device__ __forceinline__ bool steady(const Seg& c, const ScanP& P, const Seeds& S,
+                                       bool allFinite) {
+  if (!allFinite || c.s1 - c.s0 != kLSeg) return false;
   if (((P.mask & kLEma) && P.pe == 1) || ((P.mask & kLKama) && P.pk == 1)) return false;
   int last = -1;
   if (P.mask & kLEma) last = max(last, S.Le);
   if (P.mask & kLT3) last = max(last, S.L6);
   if (P.mask & kLKama) last = max(last, S.Lk);
   if (P.mask & kLSma) last = max(last, P.ps - 1);
-  if (c.s0 <= last) return false;
+  return c.s0 > last;
+}
+
+// Stages bars [s0-H, s0+SEG) of the row into LDS, as stage() does, and
+// returns whether every staged bar inside the row is finite (wave-uniform),
+// tested on the loaded registers.
+template <class T>
+__device__ __forceinline__ bool stage_finite(float* lds, const float* src, const Seg& c) {
+  const int lane = lane_id();
+  const long long g0 = c.rowOff + c.s0 - T::H;
   bool bad = false;
-  for (int e = lane_id(); e < T::LEN; e += kWave) {
+  if (((g0 & 3) == 0) && (c.s0 - T::H >= 0) && (c.s0 + T::SEG <= c.n)) {
+    constexpr int NV = (T::LEN / 4 + kWave - 1) / kWave;
+    v4f v[NV];
+    const v4f* p = reinterpret_cast<const v4f*>(src + g0);
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      const int q = lane + i * kWave;
+      if (q < T::LEN / 4) v[i] = p[q];
+    }
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      const int q = lane + i * kWave;
+      if (q < T::LEN / 4) {
+        *reinterpret_cast<v4f*>(lds + T::pos(4 * q)) = v[i];
+        bad |= !(__builtin_isfinite(v[i].x) && __builtin_isfinite(v[i].y) &&
+                 __builtin_isfinite(v[i].z) && __builtin_isfinite(v[i].w));
+      }
+    }
+    return __ballot(bad) == 0;
+  }
+  const float* row = src + c.rowOff;
+  constexpr int NS = (T::LEN + kWave - 1) / kWave;
+  float v[NS];
+#pragma unroll
+  for (int i = 0; i < NS; ++i) {
+    const int e = lane + i * kWave;
     const int t = c.s0 - T::H + e;
-    if (t >= 0 && t < c.n) bad |= !__builtin_isfinite(lds_ld(lds, T::pos(e)));
+    v[i] = qnanf_();
+    if (e < T::LEN && t >= 0 && t < c.n) {
+      v[i] = row[t];
+      bad |= !__builtin_isfinite(v[i]);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NS; ++i) {
+    const int e = lane + i * kWave;
+    if (e < T::LEN) lds[T::pos(e)] = v[i];
   }
   return __ballot(bad) == 0;
 }
@@ -300,15 +411,18 @@ __device__ __forceinline__ Fold fold_steady(const float* lds, const Seg& c, cons
   F.Ak = 1.0;
   F.bk = 0.0;
   if (P.mask & kLEma) {
+    // the offset scaled by 1 / weight: one fma a bar
+    double se = 0.0;
 #pragma unroll
-    for (int i = 0; i < kLK; ++i) F.be = fma(P.eb, F.be, P.ek * (double)xv[i]);
+    for (int i = 0; i < kLK; ++i) se = fma(P.eb, se, (double)xv[i]);
+    F.be = P.ek * se;
   }
   if (P.mask & kLT3) {
 #pragma unroll
     for (int i = 0; i < kLK; ++i) {
-      F.bt[0] = fma(P.tomk, F.bt[0], P.tk * (double)xv[i]);
+      F.bt[0] = fma(P.tomk, F.bt[0], (double)xv[i]);
 #pragma unroll
-      for (int q = 1; q < 6; ++q) F.bt[q] = fma(P.tomk, F.bt[q], P.tk * F.bt[q - 1]);
+      for (int q = 1; q < 6; ++q) F.bt[q] = fma(P.tomk, F.bt[q], F.bt[q - 1]);
     }
   }
   if (P.mask & kLKama) {
@@ -335,7 +449,7 @@ __device__ __forceinline__ Fold fold_steady(const float* lds, const Seg& c, cons
       if (sum <= 0.0 || sum <= roc) {
         er = 1.0;
       } else {
-        er = fabs(roc / sum);
+        er = fabs(div_nr(roc, sum));
         if (er > 1.0) er = 1.0;
       }
       double s2 = fma(er, P.kdiff, P.kmax);
@@ -350,9 +464,27 @@ __device__ __forceinline__ Fold fold_steady(const float* lds, const Seg& c, cons
 }
 
 // ------------------------------------------------------------- kernels --
-__global__ __launch_bounds__(64) void k_scan_lead(const float* x, int n, int rows, int* lead) {
+__device__ Toe toe_mul_d(const Toe& a, const Toe& b) {
+  Toe r;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j <= i; ++j) s = fma(a.c[j], b.c[i - j], s);
+    r.c[i] = s == 0.0 ? DBL_MIN : s;  // every entry of a power is positive
+  }
+  return r;
+}
+
+// First non-NaN bar of every row.  Block 0 also opens the launch: it takes
+// the stream workspace's epoch as this launch's record tag (and advances
+// it, so no later launch on the workspace reuses the tag), clears the stall
+// flag and writes the power tables.
+__global__ __launch_bounds__(64) void k_scan_lead(const float* x, ScanP P, int* lead, WsHeader* hdr,
+                                                  ScanHdr* sh, double* pw) {
   const int row = blockIdx.x;
-  if (row >= rows) return;
+  if (row >= P.rows) return;
+  const int n = P.n;
   const float* r = x + (size_t)row * n;
   int f = n;
   for (int t0 = 0; t0 < n; t0 += kWave) {
@@ -365,144 +497,220 @@ __global__ __launch_bounds__(64) void k_scan_lead(const float* x, int n, int row
     }
   }
   if (lane_id() == 0) lead[row] = f;
-}
-
-// FM: the requested families when fixed at compile time (a single-family
-// call), so the others' code and registers drop out; 0 = P.mask at run time.
-template <int H, unsigned FM>
-__global__ __launch_bounds__(kLW* kWave) void k_scan_fold(const float* x, const int* lead, ScanP P,
-                                                          double* agg) {
-  using T = Tile<kLK, H>;
-  if (FM) P.mask = FM;
-  __shared__ __align__(16) float s_in[kLW][T::PLEN];
-  const int w = threadIdx.x / kWave;
-  const int seg = blockIdx.x * kLW + w;
-  if (seg >= P.rows * P.segsPerRow) return;
-  int row;
-  const Seg c = seg_of(P, seg, row);
-  float* lds = s_in[w];
-  stage<T>(lds, x, c);
-  wave_sync();
-  float xv[kLK];
-  lane_vals<T>(lds, xv);
-  const int a = c.s0 + lane_id() * kLK;
-  const Seeds S = make_seeds<T>(lds, x, c, P, lead[row]);
-  Fold F;
-  if (steady<T>(lds, c, P, S)) {
-    double sc[kLK];
-    F = fold_steady<T>(lds, c, P, xv, a, sc);
-  } else {
-    F = fold_lane<T>(lds, x, c, P, S, xv, a);
+  if (row != 0) return;
+  const int l = lane_id();
+  if (l == 0) {
+    const unsigned e =
+        __hip_atomic_load(&hdr->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&hdr->epoch, e + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&sh->epoch, e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&sh->stalled, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
-  scan_lanes(P, F);
-  if (lane_id() == kWave - 1) {
-    double* o = agg + (size_t)seg * 9;
-    o[0] = F.be;
-#pragma unroll
-    for (int q = 0; q < 6; ++q) o[1 + q] = F.bt[q];
-    o[7] = F.Ak;
-    o[8] = F.bk;
-  }
-}
-
-// One wave per row: the state entering every segment of the row.
-__global__ __launch_bounds__(64) void k_scan_carry(ScanP P, const double* agg, double* carry) {
-  const int row = blockIdx.x;
-  if (row >= P.rows) return;
-  const int S = P.segsPerRow, cpl = P.cpl, lane = lane_id();
-  const size_t base = (size_t)row * S;
-  const int j0 = lane * cpl, j1 = min(S, j0 + cpl);
-  Fold F;
-  F.be = 0.0;
-#pragma unroll
-  for (int q = 0; q < 6; ++q) F.bt[q] = 0.0;
-  F.Ak = 1.0;
-  F.bk = 0.0;
-  for (int j = j0; j < j1; ++j) {
-    const double* g = agg + (base + j) * 9;
-    F.be = fma(P.eSeg, F.be, g[0]);
-    double v[6], y[6];
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-      v[q] = F.bt[q];
-      y[q] = g[1 + q];
-    }
-    toe_mv_add(P.tSeg, v, y);
-#pragma unroll
-    for (int q = 0; q < 6; ++q) F.bt[q] = y[q];
-    F.bk = fma(g[7], F.bk, g[8]);
-    F.Ak *= g[7];
-  }
-  // inclusive scan over lanes with cpl-segment multipliers
+  // beta^(K l) and M^(K l) by binary powering over the bits of l
+  double pe = 1.0;
+  Toe pt{{1.0, 0.0, 0.0, 0.0, 0.0, 0.0}};
+  bool first = true;
 #pragma unroll
   for (int s = 0; s < 6; ++s) {
-    const int d = 1 << s;
-    const double be = __shfl_up(F.be, d);
-    double bt[6];
-#pragma unroll
-    for (int q = 0; q < 6; ++q) bt[q] = __shfl_up(F.bt[q], d);
-    const double Ak = __shfl_up(F.Ak, d), bk = __shfl_up(F.bk, d);
-    if (lane >= d) {
-      F.be = fma(P.eRow[s], be, F.be);
-      toe_mv_add(P.tRow[s], bt, F.bt);
-      F.bk = fma(F.Ak, bk, F.bk);
-      F.Ak *= Ak;
+    if ((l >> s) & 1) {
+      const double q = pe * P.ePow[s];
+      pe = (q == 0.0 && P.eb != 0.0) ? DBL_MIN : q;
+      pt = first ? P.tPow[s] : toe_mul_d(pt, P.tPow[s]);
+      first = false;
     }
   }
-  // exclusive: the state entering this lane's first segment
-  double ce = __shfl_up(F.be, 1), ck = __shfl_up(F.bk, 1), ct[6];
+  double* o = pw + l * kPowStride;
+  o[0] = pe;
 #pragma unroll
-  for (int q = 0; q < 6; ++q) ct[q] = __shfl_up(F.bt[q], 1);
-  if (lane == 0) {
-    ce = ck = 0.0;
+  for (int q = 0; q < 6; ++q) o[1 + q] = pt.c[q];
+  o[7] = 0.0;
+  // powers of the tile map for the look-back
+  double sq = P.eTile, se = 1.0;
+  Toe tq = P.tTile, ts{{1.0, 0.0, 0.0, 0.0, 0.0, 0.0}};
+  bool f1 = true;
 #pragma unroll
-    for (int q = 0; q < 6; ++q) ct[q] = 0.0;
+  for (int s = 0; s < 6; ++s) {
+    if ((l >> s) & 1) {
+      const double q = se * sq;
+      se = (q == 0.0 && P.eb != 0.0) ? DBL_MIN : q;
+      ts = f1 ? tq : toe_mul_d(ts, tq);
+      f1 = false;
+    }
+    const double q2 = sq * sq;
+    sq = (q2 == 0.0 && P.eb != 0.0) ? DBL_MIN : q2;
+    if (s < 5) tq = toe_mul_d(tq, tq);
   }
-  for (int j = j0; j < j1; ++j) {
-    double* o = carry + (base + j) * 8;
-    const double* g = agg + (base + j) * 9;
-    o[0] = ce;
+  double* so = pw + (kWave + l) * kPowStride;
+  so[0] = se;
 #pragma unroll
-    for (int q = 0; q < 6; ++q) o[1 + q] = ct[q];
-    o[7] = ck;
-    ce = fma(P.eSeg, ce, g[0]);
-    double y[6];
-#pragma unroll
-    for (int q = 0; q < 6; ++q) y[q] = g[1 + q];
-    toe_mv_add(P.tSeg, ct, y);
-#pragma unroll
-    for (int q = 0; q < 6; ++q) ct[q] = y[q];
-    ck = fma(g[7], ck, g[8]);
+  for (int q = 0; q < 6; ++q) so[1 + q] = ts.c[q];
+  so[7] = 0.0;
+}
+
+// Lane 0 writes n values of a record at offset off, then the flag.
+__device__ __forceinline__ void rec_publish(double* rp, const double* v, int n, int off,
+                                            unsigned epoch, unsigned status) {
+  if (lane_id() == 0) {
+    for (int i = 0; i < n; ++i) st_rel_d(rp + off + i, v[i]);
+    vmem_order();
+    __hip_atomic_store(reinterpret_cast<unsigned long long*>(rp),
+                       ((unsigned long long)status << 32) | epoch, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
-// Eight outputs of a lane at bar a + 8h: float4 stores when whole and
-// aligned, else per bar.
+// Wave sum of v over lanes (result in every lane).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) v += __shfl_xor(v, d);
+  return v;
+}
+
+// State entering tile j of a row (j >= 1) from the row's records.  The
+// immediate predecessor is polled first; if it has only its composite, lane
+// l reads the record of tile j-1-l, so one poll covers 64 predecessors:
+// once the nearest published leaving state lies among them and every newer
+// tile has published its composite, the wave folds them in one pass -- the
+// EMA and T3 terms weighted by powers of the tile map (sp: entry l holds
+// beta^(SEG kLW l) and the first column of M^(SEG kLW l)), KAMA's by the
+// exclusive product of the newer composites' multipliers -- and sums over
+// lanes.  The tiles waited on belong to workgroups started before this one
+// (scan_tile_of), and the oldest unfinished tile of a row finds its
+// predecessor's leaving state, so the published frontier keeps moving.
+__device__ __forceinline__ void scan_lookback(const double* rowRec, int j, unsigned epoch,
+                                              const double* __restrict__ sp, ScanHdr* sh,
+                                              double& ye, double (&yt)[6], double& yk) {
+  const int lane = lane_id();
+  int polls = 0;
+  auto give_up = [&]() {
+    ++polls;
+    if (polls > kSpinLimit ||
+        ((polls & 1023) == 0 &&
+         __hip_atomic_load(&sh->stalled, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+      const double qn = qnan_d();
+      ye = yk = qn;
+#pragma unroll
+      for (int r = 0; r < 6; ++r) yt[r] = qn;
+      if (lane == 0)
+        __hip_atomic_store(&sh->stalled, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return true;
+    }
+    __builtin_amdgcn_s_sleep(2);
+    return false;
+  };
+  // the immediate predecessor first, one record per poll: usually it has
+  // published at least its composite by the time this wave looks
+  {
+    const double* r1 = rowRec + (size_t)(j - 1) * kRec;
+    unsigned long long f;
+    for (;;) {
+      unsigned lo = 0, hi = 0;
+      if (lane == 0) {
+        const unsigned long long g = ld_flag(r1);
+        lo = (unsigned)g;
+        hi = (unsigned)(g >> 32);
+      }
+      f = ((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)hi) << 32) |
+          (unsigned)__builtin_amdgcn_readfirstlane((int)lo);
+      if ((unsigned)f == epoch) break;
+      if (give_up()) return;
+    }
+    if ((unsigned)(f >> 32) == kIncl) {
+      vmem_order();
+      ye = ld_rel_d(r1 + 10);
+#pragma unroll
+      for (int r = 0; r < 6; ++r) yt[r] = ld_rel_d(r1 + 11 + r);
+      yk = ld_rel_d(r1 + 17);
+      return;
+    }
+  }
+  const int i = j - 1 - lane;
+  const double* rp = rowRec + (size_t)(i < 0 ? 0 : i) * kRec;
+  int lim;
+  for (;;) {
+    unsigned st = 0;
+    if (i >= 0) {
+      const unsigned long long f = ld_flag(rp);
+      if ((unsigned)f == epoch) st = (unsigned)(f >> 32);
+    }
+    const unsigned long long inc = __ballot(st == kIncl), rdy = __ballot(st != 0);
+    lim = inc ? __builtin_ctzll(inc) : kWave;
+    const unsigned long long need = (1ull << (lim & 63)) - 1ull;
+    if (lim < kWave && (rdy & need) == need) break;
+    if (give_up()) return;
+  }
+  vmem_order();
+  // lane l < lim: composite of tile j-1-l; lane lim: leaving state of tile
+  // j-1-lim
+  double e = 0.0, k = 0.0, m = 1.0, t[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (lane < lim) {
+    e = ld_rel_d(rp + 1);
+#pragma unroll
+    for (int r = 0; r < 6; ++r) t[r] = ld_rel_d(rp + 2 + r);
+    m = ld_rel_d(rp + 8);
+    k = ld_rel_d(rp + 9);
+  } else if (lane == lim) {
+    e = ld_rel_d(rp + 10);
+#pragma unroll
+    for (int r = 0; r < 6; ++r) t[r] = ld_rel_d(rp + 11 + r);
+    k = ld_rel_d(rp + 17);
+  }
+  // KAMA: weight of lane l = product of the multipliers of lanes 0..l-1
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const double o = __shfl_up(m, d);
+    if (lane >= d) m *= o;
+  }
+  double wx = __shfl_up(m, 1);
+  if (lane == 0) wx = 1.0;
+  const bool on = lane <= lim;
+  const double* pl = sp + lane * kPowStride;
+  ye = wave_sum(on ? pl[0] * e : 0.0);
+  yk = wave_sum(on ? wx * k : 0.0);
+#pragma unroll
+  for (int r = 0; r < 6; ++r) yt[r] = 0.0;
+  if (on) toe_mv_add(toe_at(pl + 1), t, yt);
+#pragma unroll
+  for (int r = 0; r < 6; ++r) yt[r] = wave_sum(yt[r]);
+}
+
+// Eight outputs of a lane at bar a + 8h (the general path: segments with a
+// seed, a non-finite bar or the row end), per bar.
 __device__ __forceinline__ void put8(float* dst, const Seg& c, int t0, const float (&v)[8]) {
   if (!dst) return;
   float* p = dst + c.rowOff + t0;
-  if (t0 + 8 <= c.s1 && ((reinterpret_cast<size_t>(p) & 15) == 0)) {
-    const v4f a = {v[0], v[1], v[2], v[3]}, b = {v[4], v[5], v[6], v[7]};
-    __builtin_nontemporal_store(a, reinterpret_cast<v4f*>(p));
-    __builtin_nontemporal_store(b, reinterpret_cast<v4f*>(p + 4));
-    return;
-  }
 #pragma unroll
   for (int i = 0; i < 8; ++i)
     if (t0 + i < c.s1) p[i] = v[i];
 }
 
-// A steady lane's 16 outputs (the segment is full: always in the row).
-__device__ __forceinline__ void put16(float* dst, const Seg& c, int a, const float (&v)[kLK]) {
-  float* p = dst + c.rowOff + a;
-  if ((reinterpret_cast<size_t>(p) & 15) == 0) {
+// A steady wave's outputs (a full segment), stored coalesced: the lanes'
+// 16 bars go through the staging area (no longer read) so consecutive
+// lanes store consecutive float4s; lane-contiguous stores reach a fraction
+// of that bandwidth.
+template <class T>
+__device__ __forceinline__ void put16(float* lds, float* dst, const Seg& c, const float (&v)[kLK]) {
+  wave_sync();
+  float* p = lds + T::pos(lane_id() * kLK);
 #pragma unroll
-    for (int q = 0; q < kLK / 4; ++q)
-      *reinterpret_cast<float4*>(p + 4 * q) = make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+  for (int q = 0; q < kLK / 4; ++q)
+    *reinterpret_cast<float4*>(p + 4 * q) = make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+  wave_sync();
+  float* row = dst + c.rowOff + c.s0;
+  if ((reinterpret_cast<size_t>(row) & 15) == 0) {
+#pragma unroll
+    for (int i = 0; i < kLK / 4; ++i) {
+      const int q = lane_id() + i * kWave;
+      __builtin_nontemporal_store(*reinterpret_cast<const v4f*>(lds + T::pos(4 * q)),
+                                  reinterpret_cast<v4f*>(row) + q);
+    }
     return;
   }
 #pragma unroll
-  for (int i = 0; i < kLK; ++i) p[i] = v[i];
+  for (int i = 0; i < kLK; ++i) {
+    const int e = lane_id() + i * kWave;
+    __builtin_nontemporal_store(lds_ld(lds, T::pos(e)), row + e);
+  }
 }
 
 struct ScanOut {
@@ -590,73 +798,193 @@ __device__ __forceinline__ double sma_window(const float* lds, const SmaEnds& pc
   return s;
 }
 
+// Tile of workgroup b: kLW consecutive segments of one row, one per wave,
+// sharing one look-back record.  Workgroups are dealt to the XCDs
+// round-robin (workgroup b runs on XCD b mod kXcd) and start in order within
+// each XCD, so with at least kXcd rows every XCD owns the rows r = XCD
+// (mod kXcd) and its workgroups take those rows' tiles column-major (the
+// j-th tiles of its rows before any (j+1)-th): the tiles a workgroup waits
+// on belong to workgroups started before it on its own XCD, and the halo it
+// reads was loaded through that XCD's L2.  With fewer rows the tiles are
+// taken column-major over all rows.
+constexpr int kXcd = 8;
+__device__ __forceinline__ bool scan_tile_of(const ScanP& P, int b, int& row, int& tile) {
+  int rowsX, rowStep, rowBase, u;
+  if (P.rows >= kXcd) {
+    const int xcd = b % kXcd;
+    rowsX = (P.rows - xcd + kXcd - 1) / kXcd;
+    rowStep = kXcd;
+    rowBase = xcd;
+    u = b / kXcd;
+  } else {
+    rowsX = P.rows;
+    rowStep = 1;
+    rowBase = 0;
+    u = b;
+  }
+  if ((long long)u >= (long long)rowsX * P.tilesPerRow) return false;
+  tile = u / rowsX;
+  row = rowBase + (u - tile * rowsX) * rowStep;
+  return true;
+}
+
+// Workgroups of a launch (the grid scan_tile_of expects).
+int scan_blocks(const ScanP& P) {
+  if (P.rows >= kXcd) return div_up(P.rows, kXcd) * P.tilesPerRow * kXcd;
+  return P.rows * P.tilesPerRow;
+}
+
+// y -> the state after a segment with composite g (EMA offset, T3 offset
+// 6-vector, KAMA multiplier and offset), every segment but a row's last
+// being full.
+__device__ __forceinline__ void seg_apply(const ScanP& P, const double* g, double& ye,
+                                          double (&yt)[6], double& yk) {
+  ye = fma(P.eSeg, ye, g[0]);
+  double n6[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) n6[q] = g[1 + q];
+  toe_mv_add(P.tSeg, yt, n6);
+#pragma unroll
+  for (int q = 0; q < 6; ++q) yt[q] = n6[q];
+  yk = fma(g[7], yk, g[8]);
+}
+
 template <int H, unsigned FM>
-__global__ __launch_bounds__(kLW* kWave) void k_scan_emit(const float* x, const int* lead, ScanP P,
-                                                          const double* carry, ScanOut out) {
+__global__ __launch_bounds__(kLW* kWave) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_scan_one(const float* x, const int* lead, ScanP P,
+                                                         ScanHdr* sh, const double* __restrict__ pw,
+                                                         double* rec, ScanOut out) {
   using T = Tile<kLK, H>;
   if (FM) P.mask = FM;
   __shared__ __align__(16) float s_in[kLW][T::PLEN];
+  __shared__ double s_tot[kLW][9];  // each wave's segment composite
+  __shared__ double s_ent[kLW][8];  // the state entering each wave's segment
+  int row, tile;
+  if (!scan_tile_of(P, blockIdx.x, row, tile)) return;  // uniform over the workgroup
   const int w = threadIdx.x / kWave;
-  const int seg = blockIdx.x * kLW + w;
-  if (seg >= P.rows * P.segsPerRow) return;
-  int row;
-  const Seg c = seg_of(P, seg, row);
+  const int sj = tile * kLW + w;
+  const bool active = sj < P.segsPerRow;  // a row's last tile may be partial
+  int rowChk;
+  const Seg c = seg_of(P, row * P.segsPerRow + (active ? sj : P.segsPerRow - 1), rowChk);
   float* lds = s_in[w];
-  stage<T>(lds, x, c);
-  wave_sync();
-  float xv[kLK];
-  lane_vals<T>(lds, xv);
   const int lane = lane_id();
   const int a = c.s0 + lane * kLK;
-  const Seeds S = make_seeds<T>(lds, x, c, P, lead[row]);
-  const bool fast = steady<T>(lds, c, P, S);
+  float xv[kLK];
   double sc[kLK];
-  Fold F = fast ? fold_steady<T>(lds, c, P, xv, a, sc) : fold_lane<T>(lds, x, c, P, S, xv, a);
-  // lane 0 absorbs the state entering the segment; the inclusive scan then
-  // gives the state after every lane's bars
+  Fold F;
+  F.be = 0.0;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) F.bt[q] = 0.0;
+  F.Ak = 1.0;
+  F.bk = 0.0;
+  Seeds S{};
+  bool fast = false;
+  if (active) {
+    const bool fin = stage_finite<T>(lds, x, c);
+    wave_sync();
+    lane_vals<T>(lds, xv);
+    S = make_seeds<T>(lds, x, c, P, lead[row]);
+    fast = steady<T>(c, P, S, fin);
+    F = fast ? fold_steady<T>(lds, c, P, xv, a, sc) : fold_lane<T>(lds, x, c, P, S, xv, a);
+    scan_lanes(P, pw, F);
+  }
+  // the state entering the segment: zero for a row's first segment (all
+  // contributions before the seeds are exactly 0), else carried from the
+  // tile's entering state, which the look-back provides
   double ce = 0.0, ck = 0.0, ct[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (P.useCarry) {
-    const double* g = carry + (size_t)seg * 8;
-    ce = g[0];
-    ck = g[7];
+    if (active && lane == kWave - 1) {
+      s_tot[w][0] = F.be;
 #pragma unroll
-    for (int q = 0; q < 6; ++q) ct[q] = g[1 + q];
+      for (int q = 0; q < 6; ++q) s_tot[w][1 + q] = F.bt[q];
+      s_tot[w][7] = F.Ak;
+      s_tot[w][8] = F.bk;
+    }
+    __syncthreads();
+    if (w == 0) {
+      const unsigned epoch =
+          (unsigned)__builtin_amdgcn_readfirstlane((int)__hip_atomic_load(
+              &sh->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      double* rowRec = rec + (size_t)row * P.tilesPerRow * kRec;
+      double* mine = rowRec + (size_t)tile * kRec;
+      const bool last = tile == P.tilesPerRow - 1;  // every earlier tile is full
+      if (tile > 0) {
+        if (!last) {
+          double g[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0};
+          double gt[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+          for (int v = 0; v < kLW; ++v) {
+            seg_apply(P, s_tot[v], g[0], gt, g[8]);
+            g[7] *= s_tot[v][7];
+          }
+#pragma unroll
+          for (int q = 0; q < 6; ++q) g[1 + q] = gt[q];
+          rec_publish(mine, g, 9, 1, epoch, kAgg);
+        }
+        scan_lookback(rowRec, tile, epoch, pw + kWave * kPowStride, sh, ce, ct, ck);
+      }
+      // entering states of the tile's segments, then the tile's leaving state
+      for (int v = 0; v < kLW; ++v) {
+        if (lane == 0) {
+          s_ent[v][0] = ce;
+#pragma unroll
+          for (int q = 0; q < 6; ++q) s_ent[v][1 + q] = ct[q];
+          s_ent[v][7] = ck;
+        }
+        if (tile * kLW + v < P.segsPerRow) seg_apply(P, s_tot[v], ce, ct, ck);
+      }
+      if (!last) {
+        double leave[8];
+        leave[0] = ce;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) leave[1 + q] = ct[q];
+        leave[7] = ck;
+        rec_publish(mine, leave, 8, 10, epoch, kIncl);
+      }
+    }
+    __syncthreads();
+    if (!active) return;
+    ce = s_ent[w][0];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) ct[q] = s_ent[w][1 + q];
+    ck = s_ent[w][7];
+  } else if (!active) {
+    return;
   }
-  if (lane == 0) {
-    F.be = fma(P.ePow[0], ce, F.be);
-    toe_mv_add(P.tPow[0], ct, F.bt);
-    F.bk = fma(F.Ak, ck, F.bk);
-  }
-  scan_lanes(P, F);
+  // state entering each lane: the exclusive lane prefix applied after the
+  // segment's entering state carried over the earlier lanes' bars
   double ye = 0.0, yk = 0.0, yt[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  if (P.mask & kLEma) ye = __shfl_up(F.be, 1);
-  if (P.mask & kLKama) yk = __shfl_up(F.bk, 1);
-  if (P.mask & kLT3) {
+  {
+    const double* lp = pw + lane * kPowStride;
+    if (P.mask & kLEma) {
+      double ex = __shfl_up(F.be, 1);
+      if (lane == 0) ex = 0.0;
+      ye = fma(lp[0], ce, ex);
+    }
+    if (P.mask & kLKama) {
+      double exA = __shfl_up(F.Ak, 1), exb = __shfl_up(F.bk, 1);
+      if (lane == 0) {
+        exA = 1.0;
+        exb = 0.0;
+      }
+      yk = fma(exA, ck, exb);
+    }
+    if (P.mask & kLT3) {
 #pragma unroll
-    for (int q = 0; q < 6; ++q) yt[q] = __shfl_up(F.bt[q], 1);
+      for (int q = 0; q < 6; ++q) {
+        yt[q] = __shfl_up(F.bt[q], 1);
+        if (lane == 0) yt[q] = 0.0;
+      }
+      toe_mv_add(toe_at(lp + 1), ct, yt);
+    }
   }
-  if (lane == 0) {
-    ye = ce;
-    yk = ck;
-#pragma unroll
-    for (int q = 0; q < 6; ++q) yt[q] = ct[q];
-  }
+  // the lane's bars again from LDS (not held in registers across the
+  // look-back)
+  lane_vals<T>(lds, xv);
   const bool wE = P.mask & kLEma, wT = P.mask & kLT3, wK = P.mask & kLKama, wS = P.mask & kLSma;
   if (fast) {
-    // steady replay: every bar takes the step.  One family at a time, so
-    // each lane writes its 16 outputs of a family as four back-to-back
-    // float4 stores (whole 64-byte runs); KAMA first, so its smoothing
-    // constants die early.
+    // steady replay: every bar takes the step, one family at a time.  SMA
+    // first: it is the only family reading the staged bars, after which
+    // the staging area carries each family's outputs to coalesced stores.
     float o[kLK];
-    if (wK) {
-#pragma unroll
-      for (int j = 0; j < kLK; ++j) {
-        const double v = (double)xv[j];
-        yk = fma(v - yk, sc[j], yk);
-        o[j] = (float)yk;
-      }
-      put16(out.kama, c, a, o);
-    }
     if (wS) {
       // slides from the chunk prefix sums (all staged bars are finite here)
       SmaPrefix pr;
@@ -667,9 +995,18 @@ __global__ __launch_bounds__(kLW* kWave) void k_scan_emit(const float* x, const 
 #pragma unroll
       for (int j = 0; j < kLK; ++j) {
         if (j > 0) ssum = ssum + ((double)xv[j] - (double)lds_ld(lds, T::pos(eo + j)));
-        o[j] = (float)(ssum / (double)P.ps);
+        o[j] = (float)div_by(ssum, (double)P.ps, P.rps);
       }
-      put16(out.sma, c, a, o);
+      put16<T>(lds, out.sma, c, o);
+    }
+    if (wK) {
+#pragma unroll
+      for (int j = 0; j < kLK; ++j) {
+        const double v = (double)xv[j];
+        yk = fma(v - yk, sc[j], yk);
+        o[j] = (float)yk;
+      }
+      put16<T>(lds, out.kama, c, o);
     }
     if (wE) {
 #pragma unroll
@@ -677,17 +1014,17 @@ __global__ __launch_bounds__(kLW* kWave) void k_scan_emit(const float* x, const 
         ye = fma(P.eb, ye, P.ek * (double)xv[j]);
         o[j] = (float)ye;
       }
-      put16(out.ema, c, a, o);
+      put16<T>(lds, out.ema, c, o);
     }
     if (wT) {
 #pragma unroll
       for (int j = 0; j < kLK; ++j) {
-        yt[0] = fma(P.tomk, yt[0], P.tk * (double)xv[j]);
+        yt[0] = fma(P.tomk, yt[0], (double)xv[j]);
 #pragma unroll
-        for (int q = 1; q < 6; ++q) yt[q] = fma(P.tomk, yt[q], P.tk * yt[q - 1]);
+        for (int q = 1; q < 6; ++q) yt[q] = fma(P.tomk, yt[q], yt[q - 1]);
         o[j] = (float)fma(P.c4, yt[2], fma(P.c3, yt[3], fma(P.c1, yt[5], P.c2 * yt[4])));
       }
-      put16(out.t3, c, a, o);
+      put16<T>(lds, out.t3, c, o);
     }
     return;
   }
@@ -727,9 +1064,9 @@ __global__ __launch_bounds__(kLW* kWave) void k_scan_emit(const float* x, const 
 #pragma unroll
           for (int q = 0; q < 6; ++q) yt[q] = S.st[q];
         } else {
-          yt[0] = fma(P.tomk, yt[0], P.tk * (double)xv[8 * h + i]);
+          yt[0] = fma(P.tomk, yt[0], (double)xv[8 * h + i]);
 #pragma unroll
-          for (int q = 1; q < 6; ++q) yt[q] = fma(P.tomk, yt[q], P.tk * yt[q - 1]);
+          for (int q = 1; q < 6; ++q) yt[q] = fma(P.tomk, yt[q], yt[q - 1]);
         }
         o[i] = (float)fma(P.c4, yt[2], fma(P.c3, yt[3], fma(P.c1, yt[5], P.c2 * yt[4])));
       }
@@ -805,7 +1142,7 @@ __global__ __launch_bounds__(kLW* kWave) void k_scan_emit(const float* x, const 
         if (snan > 0 || (spos > 0 && sneg > 0)) r = qn;
         else if (spos > 0) r = __builtin_inff();
         else if (sneg > 0) r = -__builtin_inff();
-        else r = (float)(ssum / (double)P.ps);
+        else r = (float)div_by(ssum, (double)P.ps, P.rps);
         o[i] = r;
       }
       if (a + 8 * h < c.s1) put8(out.sma, c, a + 8 * h, o);
@@ -853,38 +1190,18 @@ Toe toe_pow(const Toe& m, long long e) {
 }
 
 template <int H, unsigned FM>
-ctStatus_t launch_scan(const float* x, int* lead, double* agg, double* carry, const ScanP& P,
-                       const ScanOut& o, hipStream_t s) {
-  const int segs = P.rows * P.segsPerRow;
-  const int blocks = div_up(segs, kLW);
+ctStatus_t launch_scan(const float* x, const Ws& ws, int* lead, ScanHdr* sh, double* pw,
+                       double* rec, const ScanP& P, const ScanOut& o, hipStream_t s) {
+  const int blocks = scan_blocks(P);
   {
     KTime kt(s, "k_scan_lead");
-    hipLaunchKernelGGL(k_scan_lead, dim3(P.rows), dim3(kWave), 0, s, x, P.n, P.rows, lead);
+    hipLaunchKernelGGL(k_scan_lead, dim3(P.rows), dim3(kWave), 0, s, x, P, lead, ws.hdr, sh, pw);
   }
   TCA_HIP(hipGetLastError());
-  if (P.useCarry) {
-    {
-      KTime kt(s, "k_scan_fold");
-      // the fold's only look-back is KAMA's window (pk + 1 bars), not SMA's:
-      // a 32-bar halo stages 1056 bars per segment instead of 1280
-      if (H > 32 && P.pk + 1 <= 32)
-        hipLaunchKernelGGL((k_scan_fold<32, FM>), dim3(blocks), dim3(kLW * kWave), 0, s, x, lead,
-                           P, agg);
-      else
-        hipLaunchKernelGGL((k_scan_fold<H, FM>), dim3(blocks), dim3(kLW * kWave), 0, s, x, lead,
-                           P, agg);
-    }
-    TCA_HIP(hipGetLastError());
-    {
-      KTime kt(s, "k_scan_carry");
-      hipLaunchKernelGGL(k_scan_carry, dim3(P.rows), dim3(kWave), 0, s, P, agg, carry);
-    }
-    TCA_HIP(hipGetLastError());
-  }
   {
-    KTime kt(s, "k_scan_emit");
-    hipLaunchKernelGGL((k_scan_emit<H, FM>), dim3(blocks), dim3(kLW * kWave), 0, s, x, lead, P,
-                       carry, o);
+    KTime kt(s, "k_scan_one");
+    hipLaunchKernelGGL((k_scan_one<H, FM>), dim3(blocks), dim3(kLW * kWave), 0, s, x, lead, P,
+                       sh, pw, rec, o);
   }
   TCA_HIP(hipGetLastError());
   return CT_STATUS_SUCCESS;
@@ -918,9 +1235,9 @@ ctStatus_t scan_run(const float* close, int seriesLen, int numSeries, const ctxS
   P.n = seriesLen;
   P.rows = numSeries;
   P.segsPerRow = div_up(seriesLen, kLSeg);
+  P.tilesPerRow = div_up(P.segsPerRow, kLW);
   if ((long long)P.segsPerRow * numSeries > (1ll << 30))
     return fail(CT_STATUS_UNSUPPORTED, "scan: batch too large");
-  P.cpl = div_up(P.segsPerRow, kWave);
   P.useCarry = P.segsPerRow > 1 && (P.mask & (kLEma | kLKama | kLT3)) ? 1 : 0;
   {
     double b = (double)(P.pe - 1) / (double)(P.pe + 1);
@@ -940,43 +1257,58 @@ ctStatus_t scan_run(const float* close, int seriesLen, int numSeries, const ctxS
     P.c3 = -6.0 * tmp - 3.0 * (v - P.c1);
     P.c4 = std::fma(3.0, tmp, std::fma(3.0, v, 1.0) - P.c1);
   }
+  // T3 runs on the scaled cascade h_q = e_q / tk^(q+1), q = 0..5, whose step
+  // is h_0 <- (1-tk) h_0 + x, h_q <- (1-tk) h_q + h_{q-1} (one fma a stage);
+  // its matrix is lower-triangular Toeplitz with every entry 1 - tk
+  double kq[7] = {1.0};
+  for (int q = 1; q < 7; ++q) kq[q] = kq[q - 1] * P.tk;
+  for (int q = 0; q < 6; ++q) P.tsc[q] = 1.0 / kq[q + 1];
+  P.c1 *= kq[6];
+  P.c2 *= kq[5];
+  P.c3 *= kq[4];
+  P.c4 *= kq[3];
   Toe M;
-  for (int i = 0; i < 6; ++i) M.c[i] = P.tomk * std::pow(P.tk, (double)i);
+  for (int i = 0; i < 6; ++i) M.c[i] = P.tomk;
   for (int s = 0; s < 6; ++s) {
-    const long long L = (long long)kLK << s, R = (long long)kLSeg * P.cpl << s;
+    const long long L = (long long)kLK << s;
     P.ePow[s] = hpow(P.eb, L);
-    P.eRow[s] = hpow(P.eb, R);
     P.tPow[s] = toe_pow(M, L);
-    P.tRow[s] = toe_pow(M, R);
   }
   P.eSeg = hpow(P.eb, kLSeg);
   P.tSeg = toe_pow(M, kLSeg);
-  const size_t segs = (size_t)P.segsPerRow * numSeries;
+  P.eTile = hpow(P.eb, (long long)kLSeg * kLW);
+  P.tTile = toe_pow(M, (long long)kLSeg * kLW);
+  P.rps = 1.0 / (double)P.ps;
+  // workspace: lead bars, launch header, lane powers, segment records
+  const size_t tiles = (size_t)P.tilesPerRow * numSeries;
   const size_t leadB = ((size_t)numSeries * sizeof(int) + 255) & ~(size_t)255;
-  const size_t bytes = leadB + segs * (9 + 8) * sizeof(double);
+  const size_t powB = (((size_t)kPowEntries * kPowStride * sizeof(double)) + 255) & ~(size_t)255;
+  const size_t bytes = leadB + sizeof(ScanHdr) + powB + (P.useCarry ? tiles * kRec * sizeof(double) : 0);
   Ws ws;
   ctStatus_t st = workspace_for(stream, bytes, &ws);
   if (st != CT_STATUS_SUCCESS) return st;
-  int* lead = reinterpret_cast<int*>(ws.area);
-  double* agg = reinterpret_cast<double*>(reinterpret_cast<char*>(ws.area) + leadB);
-  double* carry = agg + segs * 9;
+  char* base = reinterpret_cast<char*>(ws.area);
+  int* lead = reinterpret_cast<int*>(base);
+  ScanHdr* sh = reinterpret_cast<ScanHdr*>(base + leadB);
+  double* pw = reinterpret_cast<double*>(base + leadB + sizeof(ScanHdr));
+  double* rec = reinterpret_cast<double*>(base + leadB + sizeof(ScanHdr) + powB);
   const ScanOut o{outputs->ema, outputs->kama, outputs->t3, outputs->sma};
   // single-family calls run kernels compiled for that family alone
-  if (P.mask == kLEma) return launch_scan<32, kLEma>(close, lead, agg, carry, P, o, stream);
-  if (P.mask == kLT3) return launch_scan<32, kLT3>(close, lead, agg, carry, P, o, stream);
+  if (P.mask == kLEma) return launch_scan<32, kLEma>(close, ws, lead, sh, pw, rec, P, o, stream);
+  if (P.mask == kLT3) return launch_scan<32, kLT3>(close, ws, lead, sh, pw, rec, P, o, stream);
   if (P.mask == kLKama) {
-    if (halo <= 32) return launch_scan<32, kLKama>(close, lead, agg, carry, P, o, stream);
-    if (halo <= 256) return launch_scan<256, kLKama>(close, lead, agg, carry, P, o, stream);
-    return launch_scan<kLMaxHalo, kLKama>(close, lead, agg, carry, P, o, stream);
+    if (halo <= 32) return launch_scan<32, kLKama>(close, ws, lead, sh, pw, rec, P, o, stream);
+    if (halo <= 256) return launch_scan<256, kLKama>(close, ws, lead, sh, pw, rec, P, o, stream);
+    return launch_scan<kLMaxHalo, kLKama>(close, ws, lead, sh, pw, rec, P, o, stream);
   }
   if (P.mask == kLSma) {
-    if (halo <= 32) return launch_scan<32, kLSma>(close, lead, agg, carry, P, o, stream);
-    if (halo <= 256) return launch_scan<256, kLSma>(close, lead, agg, carry, P, o, stream);
-    return launch_scan<kLMaxHalo, kLSma>(close, lead, agg, carry, P, o, stream);
+    if (halo <= 32) return launch_scan<32, kLSma>(close, ws, lead, sh, pw, rec, P, o, stream);
+    if (halo <= 256) return launch_scan<256, kLSma>(close, ws, lead, sh, pw, rec, P, o, stream);
+    return launch_scan<kLMaxHalo, kLSma>(close, ws, lead, sh, pw, rec, P, o, stream);
   }
-  if (halo <= 32) return launch_scan<32, 0>(close, lead, agg, carry, P, o, stream);
-  if (halo <= 256) return launch_scan<256, 0>(close, lead, agg, carry, P, o, stream);
-  return launch_scan<kLMaxHalo, 0>(close, lead, agg, carry, P, o, stream);
+  if (halo <= 32) return launch_scan<32, 0>(close, ws, lead, sh, pw, rec, P, o, stream);
+  if (halo <= 256) return launch_scan<256, 0>(close, ws, lead, sh, pw, rec, P, o, stream);
+  return launch_scan<kLMaxHalo, 0>(close, ws, lead, sh, pw, rec, P, o, stream);
 }
 
 bool scan_serves(int len, int rows, int windowLookback) {

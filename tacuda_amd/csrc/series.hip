@@ -182,9 +182,8 @@ namespace tca {
   } while (0)
 
 // Dense batches of long rows of EMA, KAMA, T3 and SMA run through the fused
-// scan's two streaming passes (scan_serves): the same oracle-following
-// results without a look-back, which batches of thousands of waiting
-// segments pay for in latency.
+// scan (scan_serves): the same oracle-following results from one pass that
+// reads each bar once, with one look-back record per four segments.
 ctStatus_t via_scan(const float* x, int len, int rows, hipStream_t s, ctxScanOutputs so,
                     ctxScanParams sp) {
   return scan_run(x, len, rows, &sp, &so, s);

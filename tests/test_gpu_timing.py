@@ -59,8 +59,8 @@ def test_scan_and_candles_are_timed(dev):
     finally:
         ta.kernel_timing(False)
     names = [k for k, _ in ta.kernel_times()]
-    assert names[:4] == ["k_scan_lead", "k_scan_fold", "k_scan_carry", "k_scan_emit"]
-    assert names[4:] == ["k_cdl<I32,ALL>"]
+    assert names[:2] == ["k_scan_lead", "k_scan_one"]
+    assert names[2:] == ["k_cdl<I32,ALL>"]
 
 
 def test_captured_launches_are_not_timed(dev):
