@@ -14,13 +14,15 @@ Also measured in the same run (reported under "workloads"):
            TEMA, RSI, MACD, BBANDS, ATR, ADX, STOCH, WILLR, CCI (18 outputs)
            in one fused call, hipGraph replay, five rotated buffer sets;
   scan256  256 series x 2^20 bars, EMA(200), KAMA(30), T3(5, 0.7), SMA(200)
-           through the fused two-pass scan (ctx_scan_batch_d), with the four
+           through the fused one-pass scan (ctx_scan_batch_d), with the four
            dense drop-in batch calls timed beside it;
   scan1024 the same scan on 1024 series x 2^20 bars;
   candles  all 61 candlestick patterns, int32 signals, 1024 x 16,384 OHLC;
   ragged   4096 series of lengths U[64, 8192], the 13-indicator pass through
            a tuned, graph-captured plan;
   ticks    one streaming update of 65,536 symbols' EMA/RSI/ATR states;
+  host0    ct_sma(20), ct_ema(30), ct_rsi(14) host calls on 1,000,000 bars,
+           wall time beside the device-only time (the PCIe share);
   autograd 1024 x 8192, forward+backward vs forward-only time.
 
 Prints one JSON line: bars/s of the headline pass, the roofline of its
@@ -81,7 +83,7 @@ def _args():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", default="",
                     help="comma list of workloads: pipeline7,batch13,scan256,scan1024,candles,ragged,"
-                         "ticks,autograd")
+                         "ticks,autograd,host0")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="also run a plain device copy of this many MB (counter calibration)")
     return ap.parse_args()
@@ -99,29 +101,71 @@ def _drop_in(ops):
     ]
 
 
-def _cpu_baseline(d, budget_s):
-    """TA-Lib 0.8.3 C (the reference's own host build) on one core."""
-    from oracle import ORC, REF, ref
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
-    fam = REF if ref() is not None else ORC
-    kind = "reference" if ref() is not None else "port"
+
+def _cpu_baseline(d, budget_s):
+    """TA-Lib 0.8.3 C (the reference's own host build, oracle/_ref) running
+    the 7-indicator pass over the same 1,000,000-bar series: one thread, then
+    one series per thread on all the host threads this job may use (the
+    hardware threads, capped by OMP_NUM_THREADS).  Inputs are widened and
+    outputs allocated before the clock starts (oracle/cpu_bench.cpp)."""
+    import ctypes
+
+    from oracle import ORC, ref
+
     h, l, c = d["high"], d["low"], d["close"]
-    calls = [lambda: fam.sma(c, 20), lambda: fam.ema(c, 20), lambda: fam.rsi(c, 14),
-             lambda: fam.macd(c, 12, 26, 9), lambda: fam.bbands(c, 20, 2.0, 2.0),
-             lambda: fam.atr(h, l, c, 14), lambda: fam.stoch(h, l, c, 14, 3)]
-    reps = 0
-    t0 = time.perf_counter()
-    while True:
-        for f in calls:
-            f()
-        reps += 1
+    n = len(c)
+    L = ref()
+    if L is None:
+        # the port (oracle/oracle.c) through ctypes, single thread
+        calls = [lambda: ORC.sma(c, 20), lambda: ORC.ema(c, 20), lambda: ORC.rsi(c, 14),
+                 lambda: ORC.macd(c, 12, 26, 9), lambda: ORC.bbands(c, 20, 2.0, 2.0),
+                 lambda: ORC.atr(h, l, c, 14), lambda: ORC.stoch(h, l, c, 14, 3)]
+        reps, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < budget_s:
+            for f in calls:
+                f()
+            reps += 1
         el = time.perf_counter() - t0
-        if el >= budget_s or reps >= 1000:
-            break
-    bars = reps * len(c)
-    return {"value": bars / el, "unit": "bars/s", "cores": 1, "kind": kind,
-            "sample": f"{reps} passes of the 7 indicators over the same {len(c)}-bar series "
-                      f"(seed 0), single thread, {el:.1f} s"}
+        return {"value": reps * n / el, "unit": "bars/s", "cores": 1, "kind": "port",
+                "cpu_model": _cpu_model(),
+                "sample": f"{reps} passes of the 7 indicators over one {n}-bar series, 1 thread"}
+    L.ref_bench7.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                                     ctypes.POINTER(ctypes.c_double),
+                                                     ctypes.POINTER(ctypes.c_longlong)]
+    L.ref_bench7.restype = ctypes.c_int
+
+    def run(threads):
+        bps, passes = ctypes.c_double(), ctypes.c_longlong()
+        rc = L.ref_bench7(h.ctypes.data, l.ctypes.data, c.ctypes.data, n, threads, budget_s,
+                          ctypes.byref(bps), ctypes.byref(passes))
+        if rc:
+            raise RuntimeError(f"ref_bench7 failed: {rc}")
+        return bps.value, passes.value
+
+    hw = int(L.ref_hardware_threads())
+    cap = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = max(1, min(hw, cap) if cap > 0 else hw)
+    one, p1 = run(1)
+    allc, pa = run(threads)
+    cpu = _cpu_model()
+    print(f"cpu baseline ({cpu}): 1 thread {one / 1e6:.1f} Mbar/s, {threads} threads "
+          f"{allc / 1e6:.1f} Mbar/s", file=sys.stderr, flush=True)
+    return {"value": one, "unit": "bars/s", "cores": 1, "kind": "reference", "cpu_model": cpu,
+            "sample": f"{p1} passes of the 7 indicators over the seed-0 {n}-bar series on one "
+                      f"thread, {budget_s:.0f} s (TA-Lib 0.8.3 C, oracle/cpu_bench.cpp)",
+            "all_cores": {"value": allc, "unit": "bars/s", "cores": threads,
+                          "hardware_threads": hw,
+                          "sample": f"{pa} passes, one series per thread, {budget_s:.0f} s"}}
 
 
 class Timer:
@@ -151,7 +195,10 @@ class Timer:
         return out
 
     def time(self, glist, steps, warm):
-        """ms per replay, events recorded on the stream the kernels run on."""
+        """ms per replay: `steps` back-to-back replays bracketed by events on
+        the stream the kernels run on.  A second run of `steps` replays with
+        an event between replays gives each replay's own duration (median
+        and minimum in self.last; those events add a little to each)."""
         torch = self.torch
         with torch.cuda.stream(self.stream):
             for i in range(warm):
@@ -163,12 +210,41 @@ class Timer:
             for i in range(steps):
                 glist[i % len(glist)].replay()
             e1.record(self.stream)
-        e1.synchronize()
-        return e0.elapsed_time(e1) / steps
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+            ev[0].record(self.stream)
+            for i in range(steps):
+                glist[i % len(glist)].replay()
+                ev[i + 1].record(self.stream)
+        ev[-1].synchronize()
+        per = sorted(ev[i].elapsed_time(ev[i + 1]) for i in range(steps))
+        mid = len(per) // 2
+        self.last = {"mean_ms": e0.elapsed_time(e1) / steps,
+                     "median_ms": per[mid] if len(per) % 2 else 0.5 * (per[mid - 1] + per[mid]),
+                     "min_ms": per[0], "replays": steps}
+        return self.last["mean_ms"]
 
 
 def _gbs(nbytes, ms):
     return nbytes / (ms * 1e-3) / 1e9
+
+
+def _traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` in `workload` from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/traffic.json), or
+    None when that kernel was not profiled."""
+    tpath = os.path.join(HERE, "profiles", "traffic.json")
+    if not os.path.exists(tpath):
+        return None
+    return json.load(open(tpath)).get(workload, {}).get(kernel)
+
+
+def _roofline(workload, kernel, alg_bytes, launch_ms):
+    """Roofline of one kernel: algorithmic bytes per launch over its
+    event-timed launch duration, against MI355X's 8 TB/s."""
+    ach = _gbs(alg_bytes, launch_ms)
+    return {"bound": "hbm", "kernel": kernel, "achieved": ach, "peak": PEAK_HBM_GBS,
+            "unit": "GB/s", "frac": ach / PEAK_HBM_GBS, "traffic": _traffic(workload, kernel),
+            "alg_bytes": alg_bytes, "launch_us": round(launch_ms * 1e3, 2)}
 
 
 def kernel_launch_ms(T, sets, fn, passes):
@@ -210,7 +286,9 @@ def _pipeline7(a, torch, ops, T, dev):
     sets = _upload(torch, d, dev, a.sets)
     fused = T.graphs(sets, lambda s: ops.pack(s["h"], s["l"], s["c"], outputs=PIPE7_OUT, **PIPE7))
     ms = T.time(fused, a.steps, a.warmup)
-    print(f"pipeline7 fused: {ms * 1e3:.1f} us/pass", file=sys.stderr, flush=True)
+    stats = dict(T.last)
+    print(f"pipeline7 fused: {ms * 1e3:.1f} us/pass (median {stats['median_ms'] * 1e3:.1f}, "
+          f"min {stats['min_ms'] * 1e3:.1f})", file=sys.stderr, flush=True)
     # the pass's kernels (two passes for a long dense row: k_pipe_lead,
     # k_pipe_fold with the window part, k_pipe_group, k_pipe_emit; one row
     # of 1954 segments needs no k_pipe_rows launch), each timed alone with
@@ -238,7 +316,7 @@ def _pipeline7(a, torch, ops, T, dev):
         drop[name] = round(kms * 1e3, 2)
     print(f"pipeline7 drop-in: {drop_ms * 1e3:.1f} us/pass {drop}", file=sys.stderr, flush=True)
     alg_total = (3 + 12) * 4 * n
-    return d, {"ms": ms, "bars": n, "alg_bytes": alg_total, "kernels": kernels,
+    return d, {"ms": ms, "stats": stats, "bars": n, "alg_bytes": alg_total, "kernels": kernels,
                "launch_ms": launch, "drop_in_ms": drop_ms, "drop_in_kernels_us": drop}
 
 
@@ -250,17 +328,26 @@ def _batch13(a, torch, ops, T, dev):
     sets = _upload(torch, d, dev, a.sets)
     g = T.graphs(sets, lambda s: ops.pack(s["h"], s["l"], s["c"]))
     ms = T.time(g, a.steps, a.warmup)
+    stats = dict(T.last)
     bars = rows * n
     alg = (3 + 18) * 4 * bars
-    print(f"batch13 4000x234: {ms * 1e3:.1f} us/pass", file=sys.stderr, flush=True)
-    return {"ms": ms, "bars_per_s": bars / (ms * 1e-3), "alg_bytes": alg,
+    launch = kernel_launch_ms(T, sets, lambda s: ops.pack(s["h"], s["l"], s["c"]), 20)
+    dom = max(launch, key=launch.get)
+    print(f"batch13 4000x234: {ms * 1e3:.1f} us/pass (median {stats['median_ms'] * 1e3:.1f}, "
+          f"min {stats['min_ms'] * 1e3:.1f}); " +
+          ", ".join(f"{k} {v * 1e3:.1f} us" for k, v in launch.items()), file=sys.stderr,
+          flush=True)
+    # one launch computes the whole pass: its algorithmic bytes are the pass's
+    return {"ms": ms, "stats": stats, "bars_per_s": bars / (ms * 1e-3), "alg_bytes": alg,
             "GBs": _gbs(alg, ms), "frac_hbm": _gbs(alg, ms) / PEAK_HBM_GBS,
-            "shape": [rows, n], "graph": True, "rotated_sets": a.sets}
+            "shape": [rows, n], "graph": True, "rotated_sets": a.sets,
+            "kernels_us": {k: round(v * 1e3, 2) for k, v in launch.items()},
+            "roofline": _roofline("batch13", dom, alg, launch[dom])}
 
 
 def _scan256(a, torch, ops, T, dev):
     """256 x 2^20 close: EMA(200), KAMA(30), T3(5, 0.7), SMA(200).  The fused
-    two-pass launch (ctx_scan_batch_d) is the workload's figure; the four
+    one-pass launch (ctx_scan_batch_d) is the workload's figure; the four
     drop-in family calls are timed beside it."""
     import tacuda_amd as ta
 
@@ -273,6 +360,7 @@ def _scan256(a, torch, ops, T, dev):
     alg = (1 + 4) * 4 * bars
     g = T.graphs([c], lambda x: ops.scan(x))
     fused = T.time(g, max(5, a.steps // 5), 2)
+    stats = dict(T.last)
     T.keep.clear()
     torch.cuda.empty_cache()
     print(f"scan256 fused: {fused:.3f} ms/pass", file=sys.stderr, flush=True)
@@ -313,9 +401,11 @@ def _scan256(a, torch, ops, T, dev):
     print("scan256 family calls: " + ", ".join(f"{k} {v['ms']:.3f} ms" for k, v in famr.items()),
           file=sys.stderr, flush=True)
     del hl
-    return {"ms": fused, "bars_per_s": bars / (fused * 1e-3), "alg_bytes": alg,
+    return {"ms": fused, "stats": stats, "bars_per_s": bars / (fused * 1e-3), "alg_bytes": alg,
             "GBs": _gbs(alg, fused), "frac_hbm": _gbs(alg, fused) / PEAK_HBM_GBS,
-            "shape": [rows, n], "graph": True, "kernels": kern, "drop_in_ms": total,
+            "shape": [rows, n], "graph": True, "kernels": kern,
+            "roofline": _roofline("scan256", "k_scan_one", alg, launch["k_scan_one"]),
+            "drop_in_ms": total,
             "drop_in_frac_hbm": _gbs(alg, total) / PEAK_HBM_GBS, "drop_in_kernels": per,
             "family_calls": famr}
 
@@ -334,7 +424,8 @@ def _scan1024(a, torch, ops, T, dev):
     bars = rows * n
     alg = (1 + 4) * 4 * bars
     g = T.graphs([c], lambda x: ops.scan(x))
-    ms = T.time(g, max(3, a.steps // 10), 1)
+    ms = T.time(g, max(5, a.steps // 10), 1)
+    stats = dict(T.last)
     T.keep.clear()
     torch.cuda.empty_cache()
     launch = kernel_launch_ms(T, [c], lambda x: ops.scan(x), 2)
@@ -345,9 +436,10 @@ def _scan1024(a, torch, ops, T, dev):
                             if k in kbytes else {})} for k, v in launch.items()}
     print(f"scan1024 fused: {ms:.3f} ms/pass; kernels: " +
           ", ".join(f"{k} {v:.3f} ms" for k, v in launch.items()), file=sys.stderr, flush=True)
-    return {"ms": ms, "bars_per_s": bars / (ms * 1e-3), "alg_bytes": alg, "GBs": _gbs(alg, ms),
-            "frac_hbm": _gbs(alg, ms) / PEAK_HBM_GBS, "shape": [rows, n], "graph": True,
-            "data": "the 256 seeded scan256 rows stacked 4x", "kernels": kern}
+    return {"ms": ms, "stats": stats, "bars_per_s": bars / (ms * 1e-3), "alg_bytes": alg,
+            "GBs": _gbs(alg, ms), "frac_hbm": _gbs(alg, ms) / PEAK_HBM_GBS, "shape": [rows, n],
+            "graph": True, "data": "the 256 seeded scan256 rows stacked 4x", "kernels": kern,
+            "roofline": _roofline("scan1024", "k_scan_one", alg, launch["k_scan_one"])}
 
 
 def _candles(a, torch, ops, T, dev):
@@ -413,6 +505,56 @@ def _ragged(a, torch, ops, T, dev):
     lay.close()
     return {"ms": ms, "bars": bars, "bars_per_s": bars / (ms * 1e-3), "alg_bytes": alg,
             "GBs": _gbs(alg, ms), "frac_hbm": _gbs(alg, ms) / PEAK_HBM_GBS, "plan": info}
+
+
+def _host_config0(a, torch, ops, dev):
+    """The reference's plain host-API case: ct_sma(20), ct_ema(30), ct_rsi(14)
+    on one host float32 series of 1,000,000 bars (seed 0), each call taking
+    host arrays (pooled device buffers, pinned staging).  Reported per call:
+    the wall time of the host call, the device-only time of the same
+    computation (ct_*_d on resident tensors, events), and their difference,
+    the host<->device copy and synchronization share."""
+    import numpy as np
+
+    import tacuda_amd as ta
+
+    L = ta.lib()
+    n = 1_000_000
+    c = np.ascontiguousarray(ta.synth_ohlcv(n, seed=0)["close"])
+    out = np.empty_like(c)
+    x = torch.from_numpy(c).to(dev)
+    y = torch.empty_like(x)
+    s = torch.cuda.current_stream()
+    res = {}
+    for name, p in (("sma", 20), ("ema", 30), ("rsi", 14)):
+        host = getattr(L, f"ct_{name}")
+        devf = getattr(L, f"ct_{name}_d")
+        for _ in range(3):
+            assert host(c.ctypes.data, out.ctypes.data, n, p, None) == 0
+        reps = 30
+        wall = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            assert host(c.ctypes.data, out.ctypes.data, n, p, None) == 0
+            wall.append(time.perf_counter() - t0)
+        for _ in range(3):
+            devf(x.data_ptr(), y.data_ptr(), n, p, s.cuda_stream)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        for _ in range(reps):
+            devf(x.data_ptr(), y.data_ptr(), n, p, s.cuda_stream)
+        e1.record(s)
+        e1.synchronize()
+        dms = e0.elapsed_time(e1) / reps
+        hms = float(np.median(wall)) * 1e3
+        res[f"{name}{p}"] = {"host_call_ms_median": hms, "host_call_ms_min": min(wall) * 1e3,
+                             "device_ms": dms, "copy_and_sync_ms": hms - dms,
+                             "host_bars_per_s": n / (hms * 1e-3)}
+    print("host config0 1M bars: " + ", ".join(
+        f"{k} host {v['host_call_ms_median']:.3f} ms (device {v['device_ms']:.3f})"
+        for k, v in res.items()), file=sys.stderr, flush=True)
+    return res
 
 
 def _ticks(a, torch, dev):
@@ -535,7 +677,8 @@ def main():
 
     dev = torch.device("cuda:0")
     only = set(filter(None, a.only.split(","))) or {"pipeline7", "batch13", "scan256", "scan1024",
-                                                    "candles", "ragged", "ticks", "autograd"}
+                                                    "candles", "ragged", "ticks", "autograd",
+                                                    "host0"}
     T = Timer(torch)
     work = {}
     d = p7 = None
@@ -555,6 +698,8 @@ def main():
     if "ragged" in only:
         work["ragged_4096_U64_8192"] = _ragged(a, torch, ops, T, dev)
         torch.cuda.empty_cache()
+    if "host0" in only:
+        work["host_api_1M_sma20_ema30_rsi14"] = _host_config0(a, torch, ops, dev)
     if "ticks" in only:
         work["stream_ticks_65536"] = _ticks(a, torch, dev)
     if "autograd" in only:
@@ -572,12 +717,6 @@ def main():
     n, ms = p7["bars"], p7["ms"]
     launch = p7["launch_ms"]
     dom = max(launch, key=launch.get)
-    dk = {"GBs": _gbs(PIPE7_KERNEL_BYTES.get(dom, lambda n: 0)(n), launch[dom]),
-          "alg_bytes": PIPE7_KERNEL_BYTES.get(dom, lambda n: 0)(n)}
-    traffic = None
-    tpath = os.path.join(HERE, "profiles", "traffic.json")
-    if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get(dom)
     line = {
         "metric": METRIC,
         "value": n / (ms * 1e-3),
@@ -594,6 +733,8 @@ def main():
         "config": {"workload": "7-indicator pipeline, 1 x 1,000,000 bars (SMA20, EMA20, RSI14, "
                                "MACD12/26/9, BBANDS20, ATR14, STOCH14/3/3), fused launch",
                    "bars": n, "graph": True, "rotated_sets": a.sets,
+                   "step_ms_median": p7["stats"]["median_ms"],
+                   "step_ms_min": p7["stats"]["min_ms"],
                    "pass_alg_bytes": p7["alg_bytes"],
                    "pass_GBs": _gbs(p7["alg_bytes"], ms),
                    "pass_frac_hbm": _gbs(p7["alg_bytes"], ms) / PEAK_HBM_GBS,
@@ -603,9 +744,8 @@ def main():
                    "drop_in_7call_ms": p7["drop_in_ms"],
                    "drop_in_bars_per_s": n / (p7["drop_in_ms"] * 1e-3),
                    "drop_in_kernels_us": p7["drop_in_kernels_us"]},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": dk["GBs"], "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": dk["GBs"] / PEAK_HBM_GBS, "traffic": traffic,
-                     "alg_bytes": dk["alg_bytes"], "launch_us": round(launch[dom] * 1e3, 2)},
+        "roofline": _roofline("pipeline7", dom, PIPE7_KERNEL_BYTES.get(dom, lambda n: 0)(n),
+                              launch[dom]),
         "workloads": work,
     }
     if not a.no_cpu:

@@ -114,8 +114,9 @@ __device__ __forceinline__ void lane_vals(const float* lds, float (&v)[T::K]) {
 }
 
 // Writes the lane's K outputs.  The wave transposes its segment through
-// ldo so that consecutive lanes store consecutive addresses: float4 runs
-// when the segment is full and 16-byte aligned, dwords otherwise.  Stores
+// ldo so that consecutive lanes store consecutive addresses: nontemporal
+// float4 runs when the segment is full and 16-byte aligned, dwords
+// otherwise.  Stores
 // of K contiguous bars per lane reach a fraction of that bandwidth on
 // MI355X (a quarter when nontemporal).
 template <class T>
@@ -137,10 +138,12 @@ __device__ __forceinline__ void store_lane(float* ldo, const float (&v)[T::K], f
     }
     return;
   }
+  // rows that start or end inside a cache line share it with the next
+  // row's wave: plain stores let L2 merge the two halves
   float* row = dst + c.rowOff;
   for (int e = lane_id(); e < T::SEG; e += kWave) {
     const int t = c.s0 + e;
-    if (t < c.s1) __builtin_nontemporal_store(ldo[T::pos(e)], row + t);
+    if (t < c.s1) row[t] = ldo[T::pos(e)];
   }
 }
 

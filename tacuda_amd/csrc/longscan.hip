@@ -850,7 +850,7 @@ __device__ __forceinline__ void seg_apply(const ScanP& P, const double* g, doubl
 }
 
 template <int H, unsigned FM>
-__global__ __launch_bounds__(kLW* kWave) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_scan_one(const float* x, const int* lead, ScanP P,
+__global__ __launch_bounds__(kLW* kWave) void k_scan_one(const float* x, const int* lead, ScanP P,
                                                          ScanHdr* sh, const double* __restrict__ pw,
                                                          double* rec, ScanOut out) {
   using T = Tile<kLK, H>;
