@@ -251,6 +251,23 @@ CTAPI_EXPORT ctStatus_t ctx_rsi_alpha_bwd_d(const float* x, const float* gy, con
                                             float* gx, double* galpha, int seriesLen,
                                             int numSeries, int period, double alpha,
                                             hipStream_t stream);
+/* The same four with alpha read on the device from d_alpha (one double):
+ * no host round trip for a learned alpha, so they can be captured into a
+ * graph.  d_alpha is not range-checked (a NaN alpha gives NaN outputs). */
+CTAPI_EXPORT ctStatus_t ctx_ema_alpha_fwd_dp(const float* x, float* y, double* ckpt,
+                                             int seriesLen, int numSeries, int period,
+                                             const double* d_alpha, hipStream_t stream);
+CTAPI_EXPORT ctStatus_t ctx_ema_alpha_bwd_dp(const float* x, const float* gy, const double* ckpt,
+                                             float* gx, double* galpha, int seriesLen,
+                                             int numSeries, int period, const double* d_alpha,
+                                             hipStream_t stream);
+CTAPI_EXPORT ctStatus_t ctx_rsi_alpha_fwd_dp(const float* x, float* y, double* ckpt,
+                                             int seriesLen, int numSeries, int period,
+                                             const double* d_alpha, hipStream_t stream);
+CTAPI_EXPORT ctStatus_t ctx_rsi_alpha_bwd_dp(const float* x, const float* gy, const double* ckpt,
+                                             float* gx, double* galpha, int seriesLen,
+                                             int numSeries, int period, const double* d_alpha,
+                                             hipStream_t stream);
 CTAPI_EXPORT ctStatus_t ctx_macd_bwd_d(const float* gMacd, const float* gSignal,
                                        const float* gHist, float* gx, int seriesLen,
                                        int numSeries, int fastPeriod, int slowPeriod,

@@ -262,6 +262,8 @@ _D = ctypes.c_double
 for _n in ("ema", "rsi"):
     _sig(f"ctx_{_n}_alpha_fwd_d", P, P, P, c_int, c_int, c_int, _D)
     _sig(f"ctx_{_n}_alpha_bwd_d", P, P, P, P, P, c_int, c_int, c_int, _D)
+    _sig(f"ctx_{_n}_alpha_fwd_dp", P, P, P, c_int, c_int, c_int, P)
+    _sig(f"ctx_{_n}_alpha_bwd_dp", P, P, P, P, P, c_int, c_int, c_int, P)
 _sig("ctx_macd_bwd_d", P, P, P, P, c_int, c_int, c_int, c_int, c_int)
 _sig("ctx_bbands_bwd_d", P, P, P, P, P, c_int, c_int, c_int, c_float, c_float)
 

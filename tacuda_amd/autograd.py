@@ -34,8 +34,15 @@ def _shape(x):
     return x.shape[1], x.shape[0]  # seriesLen, numSeries
 
 
-def _alpha_value(alpha):
-    return float(alpha.detach().double().item()) if torch.is_tensor(alpha) else float(alpha)
+def _alpha_arg(alpha, device):
+    """A tensor alpha stays on the device (a float64 copy the kernels read,
+    so no host synchronization and the call can be graph-captured); a
+    number is passed by value."""
+    if torch.is_tensor(alpha):
+        if alpha.numel() != 1:
+            raise ValueError("alpha must be a scalar tensor")
+        return alpha.detach().to(device=device, dtype=torch.float64).reshape(1).contiguous()
+    return float(alpha)
 
 
 class _AlphaRecurrence(torch.autograd.Function):
@@ -46,25 +53,36 @@ class _AlphaRecurrence(torch.autograd.Function):
     def forward(cls, ctx, x, alpha, period):
         x2, squeeze = _as2d(x.contiguous())
         n, rows = _shape(x2)
-        a = _alpha_value(alpha)
+        a = _alpha_arg(alpha, x.device)
         y = torch.empty_like(x2)
         ck = torch.empty(rows * 64 * cls.ck_per_lane, dtype=torch.float64, device=x.device)
-        call(f"ctx_{cls.fam}_alpha_fwd_d", x2.data_ptr(), y.data_ptr(), ck.data_ptr(), n, rows,
-             int(period), a, _stream())
-        ctx.save_for_backward(x2, ck)
-        ctx.meta = (a, int(period), squeeze, torch.is_tensor(alpha))
+        if torch.is_tensor(a):
+            call(f"ctx_{cls.fam}_alpha_fwd_dp", x2.data_ptr(), y.data_ptr(), ck.data_ptr(), n,
+                 rows, int(period), a.data_ptr(), _stream())
+            ctx.save_for_backward(x2, ck, a)
+        else:
+            call(f"ctx_{cls.fam}_alpha_fwd_d", x2.data_ptr(), y.data_ptr(), ck.data_ptr(), n,
+                 rows, int(period), a, _stream())
+            ctx.save_for_backward(x2, ck)
+        ctx.meta = (None if torch.is_tensor(a) else a, int(period), squeeze,
+                    torch.is_tensor(alpha))
         return y.squeeze(0) if squeeze else y
 
     @classmethod
     def backward(cls, ctx, gy):
-        x2, ck = ctx.saved_tensors
+        saved = ctx.saved_tensors
+        x2, ck = saved[0], saved[1]
         a, period, squeeze, alpha_is_tensor = ctx.meta
         g2 = (gy.unsqueeze(0) if squeeze else gy).contiguous().float()
         n, rows = _shape(x2)
         gx = torch.empty_like(x2)
         ga = torch.empty(rows, dtype=torch.float64, device=x2.device)
-        call(f"ctx_{cls.fam}_alpha_bwd_d", x2.data_ptr(), g2.data_ptr(), ck.data_ptr(),
-             gx.data_ptr(), ga.data_ptr(), n, rows, period, a, _stream())
+        if len(saved) == 3:
+            call(f"ctx_{cls.fam}_alpha_bwd_dp", x2.data_ptr(), g2.data_ptr(), ck.data_ptr(),
+                 gx.data_ptr(), ga.data_ptr(), n, rows, period, saved[2].data_ptr(), _stream())
+        else:
+            call(f"ctx_{cls.fam}_alpha_bwd_d", x2.data_ptr(), g2.data_ptr(), ck.data_ptr(),
+                 gx.data_ptr(), ga.data_ptr(), n, rows, period, a, _stream())
         galpha = ga.sum().to(torch.float32) if alpha_is_tensor else None
         return (gx.squeeze(0) if squeeze else gx), galpha, None
 

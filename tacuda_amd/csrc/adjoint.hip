@@ -147,7 +147,9 @@ __device__ __forceinline__ double chunk_sum(const float* r, int lo, int hi) {
 
 // =================================================================== EMA ==
 __global__ __launch_bounds__(64) void k_ema_fwd(const float* x, float* y, double* ck, int rows,
-                                                int n, int p, double alpha) {
+                                                int n, int p, double alphaArg,
+                                                const double* alphaPtr) {
+  const double alpha = alphaPtr ? *alphaPtr : alphaArg;
   const int row = blockIdx.x;
   if (row >= rows) return;
   const Chunk k = chunk_of(n);
@@ -187,7 +189,8 @@ __global__ __launch_bounds__(64) void k_ema_fwd(const float* x, float* y, double
 
 __global__ __launch_bounds__(64) void k_ema_bwd(const float* x, const float* g, const double* ck,
                                                 float* dx, double* dal, int rows, int n, int p,
-                                                double alpha) {
+                                                double alphaArg, const double* alphaPtr) {
+  const double alpha = alphaPtr ? *alphaPtr : alphaArg;
   extern __shared__ float s_col[];
   const int row = blockIdx.x;
   if (row >= rows) return;
@@ -280,7 +283,9 @@ __device__ __forceinline__ void walk_moves(const float* xr, int lo, int hi, F&& 
 }
 
 __global__ __launch_bounds__(64) void k_rsi_fwd(const float* x, float* y, double* ck, int rows,
-                                                int n, int p, double alpha) {
+                                                int n, int p, double alphaArg,
+                                                const double* alphaPtr) {
+  const double alpha = alphaPtr ? *alphaPtr : alphaArg;
   const int row = blockIdx.x;
   if (row >= rows) return;
   const Chunk k = chunk_of(n);
@@ -366,7 +371,8 @@ __device__ __forceinline__ void rsi_coef(double G, double L, double& cg, double&
 // row needs 16 KB of LDS at 8192 bars instead of its whole state history.
 __global__ __launch_bounds__(64) void k_rsi_bwd(const float* x, const float* g, const double* ck,
                                                 float* dx, double* dal, int rows, int n, int p,
-                                                double alpha) {
+                                                double alphaArg, const double* alphaPtr) {
+  const double alpha = alphaPtr ? *alphaPtr : alphaArg;
   extern __shared__ double s_ck[];
   const int row = blockIdx.x;
   if (row >= rows) return;
@@ -712,66 +718,139 @@ using namespace tca;
 
 extern "C" {
 
+}  // extern "C"
+
+namespace {
+
+ctStatus_t ema_alpha_fwd(const float* x, float* y, double* ckpt, int n, int rows, int period,
+                         double alpha, const double* alphaPtr, hipStream_t s) {
+  ctStatus_t r = check(x, y, rows, n, period, "ema_alpha");
+  if (r != CT_STATUS_SUCCESS) return r;
+  if (!ckpt) return fail(CT_STATUS_INVALID_ARGUMENT, "ema_alpha: null checkpoint");
+  hipLaunchKernelGGL(k_ema_fwd, dim3(rows), dim3(kWave), 0, s, x, y, ckpt, rows, n, period, alpha,
+                     alphaPtr);
+  TCA_HIP(hipGetLastError());
+  return CT_STATUS_SUCCESS;
+}
+
+ctStatus_t ema_alpha_bwd(const float* x, const float* gy, const double* ckpt, float* gx,
+                         double* galpha, int n, int rows, int period, double alpha,
+                         const double* alphaPtr, hipStream_t s) {
+  ctStatus_t r = check(x, gy, rows, n, period, "ema_alpha backward");
+  if (r != CT_STATUS_SUCCESS) return r;
+  if (!ckpt || !gx || !galpha)
+    return fail(CT_STATUS_INVALID_ARGUMENT, "ema_alpha backward: null pointer");
+  const size_t lds = col_bytes(n, 1);
+  r = allow_lds(k_ema_bwd, lds);
+  if (r != CT_STATUS_SUCCESS) return r;
+  hipLaunchKernelGGL(k_ema_bwd, dim3(rows), dim3(kWave), lds, s, x, gy, ckpt, gx, galpha, rows, n,
+                     period, alpha, alphaPtr);
+  TCA_HIP(hipGetLastError());
+  return CT_STATUS_SUCCESS;
+}
+
+ctStatus_t rsi_alpha_fwd(const float* x, float* y, double* ckpt, int n, int rows, int period,
+                         double alpha, const double* alphaPtr, hipStream_t s) {
+  ctStatus_t r = check(x, y, rows, n, period, "rsi_alpha");
+  if (r != CT_STATUS_SUCCESS) return r;
+  if (!ckpt) return fail(CT_STATUS_INVALID_ARGUMENT, "rsi_alpha: null checkpoint");
+  hipLaunchKernelGGL(k_rsi_fwd, dim3(rows), dim3(kWave), 0, s, x, y, ckpt, rows, n, period, alpha,
+                     alphaPtr);
+  TCA_HIP(hipGetLastError());
+  return CT_STATUS_SUCCESS;
+}
+
+ctStatus_t rsi_alpha_bwd(const float* x, const float* gy, const double* ckpt, float* gx,
+                         double* galpha, int n, int rows, int period, double alpha,
+                         const double* alphaPtr, hipStream_t s) {
+  ctStatus_t r = check(x, gy, rows, n, period, "rsi_alpha backward");
+  if (r != CT_STATUS_SUCCESS) return r;
+  if (!ckpt || !gx || !galpha)
+    return fail(CT_STATUS_INVALID_ARGUMENT, "rsi_alpha backward: null pointer");
+  // per lane and kB-bar block: the entering G and L (doubles)
+  const size_t lds = (size_t)2 * kWave * (((n + kWave - 1) / kWave + kB - 1) / kB) * sizeof(double);
+  r = allow_lds(k_rsi_bwd, lds);
+  if (r != CT_STATUS_SUCCESS) return r;
+  hipLaunchKernelGGL(k_rsi_bwd, dim3(rows), dim3(kWave), lds, s, x, gy, ckpt, gx, galpha, rows, n,
+                     period, alpha, alphaPtr);
+  TCA_HIP(hipGetLastError());
+  return CT_STATUS_SUCCESS;
+}
+
+ctStatus_t alpha_ptr_ok(const double* d_alpha, const char* what) {
+  if (!d_alpha) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: null device alpha", what);
+  return CT_STATUS_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
 CTAPI_EXPORT ctStatus_t ctx_ema_alpha_fwd_d(const float* x, float* y, double* ckpt, int n,
                                             int rows, int period, double alpha, hipStream_t s) {
-  return guarded([&]() -> ctStatus_t {
-    ctStatus_t r = check(x, y, rows, n, period, "ema_alpha");
-    if (r != CT_STATUS_SUCCESS) return r;
-    if (!ckpt) return fail(CT_STATUS_INVALID_ARGUMENT, "ema_alpha: null checkpoint");
-    hipLaunchKernelGGL(k_ema_fwd, dim3(rows), dim3(kWave), 0, s, x, y, ckpt, rows, n, period,
-                       alpha);
-    TCA_HIP(hipGetLastError());
-    return CT_STATUS_SUCCESS;
-  });
+  return guarded([&] { return ema_alpha_fwd(x, y, ckpt, n, rows, period, alpha, nullptr, s); });
 }
 
 CTAPI_EXPORT ctStatus_t ctx_ema_alpha_bwd_d(const float* x, const float* gy, const double* ckpt,
                                             float* gx, double* galpha, int n, int rows,
                                             int period, double alpha, hipStream_t s) {
-  return guarded([&]() -> ctStatus_t {
-    ctStatus_t r = check(x, gy, rows, n, period, "ema_alpha backward");
-    if (r != CT_STATUS_SUCCESS) return r;
-    if (!ckpt || !gx || !galpha)
-      return fail(CT_STATUS_INVALID_ARGUMENT, "ema_alpha backward: null pointer");
-    const size_t lds = col_bytes(n, 1);
-    r = allow_lds(k_ema_bwd, lds);
-    if (r != CT_STATUS_SUCCESS) return r;
-    hipLaunchKernelGGL(k_ema_bwd, dim3(rows), dim3(kWave), lds, s, x, gy, ckpt, gx, galpha, rows,
-                       n, period, alpha);
-    TCA_HIP(hipGetLastError());
-    return CT_STATUS_SUCCESS;
+  return guarded([&] {
+    return ema_alpha_bwd(x, gy, ckpt, gx, galpha, n, rows, period, alpha, nullptr, s);
   });
 }
 
 CTAPI_EXPORT ctStatus_t ctx_rsi_alpha_fwd_d(const float* x, float* y, double* ckpt, int n,
                                             int rows, int period, double alpha, hipStream_t s) {
-  return guarded([&]() -> ctStatus_t {
-    ctStatus_t r = check(x, y, rows, n, period, "rsi_alpha");
-    if (r != CT_STATUS_SUCCESS) return r;
-    if (!ckpt) return fail(CT_STATUS_INVALID_ARGUMENT, "rsi_alpha: null checkpoint");
-    hipLaunchKernelGGL(k_rsi_fwd, dim3(rows), dim3(kWave), 0, s, x, y, ckpt, rows, n, period,
-                       alpha);
-    TCA_HIP(hipGetLastError());
-    return CT_STATUS_SUCCESS;
-  });
+  return guarded([&] { return rsi_alpha_fwd(x, y, ckpt, n, rows, period, alpha, nullptr, s); });
 }
 
 CTAPI_EXPORT ctStatus_t ctx_rsi_alpha_bwd_d(const float* x, const float* gy, const double* ckpt,
                                             float* gx, double* galpha, int n, int rows,
                                             int period, double alpha, hipStream_t s) {
-  return guarded([&]() -> ctStatus_t {
-    ctStatus_t r = check(x, gy, rows, n, period, "rsi_alpha backward");
-    if (r != CT_STATUS_SUCCESS) return r;
-    if (!ckpt || !gx || !galpha)
-      return fail(CT_STATUS_INVALID_ARGUMENT, "rsi_alpha backward: null pointer");
-    // per lane and kB-bar block: the entering G and L (doubles)
-    const size_t lds = (size_t)2 * kWave * (((n + kWave - 1) / kWave + kB - 1) / kB) * sizeof(double);
-    r = allow_lds(k_rsi_bwd, lds);
-    if (r != CT_STATUS_SUCCESS) return r;
-    hipLaunchKernelGGL(k_rsi_bwd, dim3(rows), dim3(kWave), lds, s, x, gy, ckpt, gx, galpha, rows,
-                       n, period, alpha);
-    TCA_HIP(hipGetLastError());
-    return CT_STATUS_SUCCESS;
+  return guarded([&] {
+    return rsi_alpha_bwd(x, gy, ckpt, gx, galpha, n, rows, period, alpha, nullptr, s);
+  });
+}
+
+CTAPI_EXPORT ctStatus_t ctx_ema_alpha_fwd_dp(const float* x, float* y, double* ckpt, int n,
+                                             int rows, int period, const double* d_alpha,
+                                             hipStream_t s) {
+  return guarded([&] {
+    ctStatus_t r = alpha_ptr_ok(d_alpha, "ema_alpha");
+    return r != CT_STATUS_SUCCESS ? r
+                                  : ema_alpha_fwd(x, y, ckpt, n, rows, period, 0.0, d_alpha, s);
+  });
+}
+
+CTAPI_EXPORT ctStatus_t ctx_ema_alpha_bwd_dp(const float* x, const float* gy, const double* ckpt,
+                                             float* gx, double* galpha, int n, int rows,
+                                             int period, const double* d_alpha, hipStream_t s) {
+  return guarded([&] {
+    ctStatus_t r = alpha_ptr_ok(d_alpha, "ema_alpha backward");
+    return r != CT_STATUS_SUCCESS
+               ? r
+               : ema_alpha_bwd(x, gy, ckpt, gx, galpha, n, rows, period, 0.0, d_alpha, s);
+  });
+}
+
+CTAPI_EXPORT ctStatus_t ctx_rsi_alpha_fwd_dp(const float* x, float* y, double* ckpt, int n,
+                                             int rows, int period, const double* d_alpha,
+                                             hipStream_t s) {
+  return guarded([&] {
+    ctStatus_t r = alpha_ptr_ok(d_alpha, "rsi_alpha");
+    return r != CT_STATUS_SUCCESS ? r
+                                  : rsi_alpha_fwd(x, y, ckpt, n, rows, period, 0.0, d_alpha, s);
+  });
+}
+
+CTAPI_EXPORT ctStatus_t ctx_rsi_alpha_bwd_dp(const float* x, const float* gy, const double* ckpt,
+                                             float* gx, double* galpha, int n, int rows,
+                                             int period, const double* d_alpha, hipStream_t s) {
+  return guarded([&] {
+    ctStatus_t r = alpha_ptr_ok(d_alpha, "rsi_alpha backward");
+    return r != CT_STATUS_SUCCESS
+               ? r
+               : rsi_alpha_bwd(x, gy, ckpt, gx, galpha, n, rows, period, 0.0, d_alpha, s);
   });
 }
 

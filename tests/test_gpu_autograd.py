@@ -108,3 +108,45 @@ def test_warmup_gradients_are_not_read(dev, name):
     assert np.isfinite(res[1][0]).all()
     assert np.array_equal(res[0][0], res[1][0])
     assert res[0][1] == res[1][1]
+
+
+@pytest.mark.parametrize("name", ["ema", "rsi"])
+def test_device_alpha_matches_host_alpha_and_captures(dev, name):
+    """A tensor alpha is read on the device (ctx_*_alpha_*_dp): the same
+    outputs and gradients as the same alpha passed as a number, with no host
+    synchronization, so the forward can be captured into a graph; updating
+    the alpha tensor in place between replays changes the replayed outputs."""
+    from tacuda_amd import autograd as ag
+
+    print(f"autograd device alpha {name}", flush=True)
+    c = torch.tensor(_close(8, 3000, seed=4), device=dev)
+    a0 = float(np.float32(ALPHA0[name]))
+    fn = _gpu_fn(name)
+    xa = c.clone().requires_grad_(True)
+    at = torch.tensor(a0, dtype=torch.float32, device=dev, requires_grad=True)
+    ya = fn(xa, at)
+    xb = c.clone().requires_grad_(True)
+    yb = fn(xb, a0)
+    w = torch.tensor(np.random.default_rng(5).standard_normal(tuple(c.shape)),
+                     dtype=torch.float32, device=dev)
+    (torch.nan_to_num(ya, nan=0.0) * w).sum().backward()
+    (torch.nan_to_num(yb, nan=0.0) * w).sum().backward()
+    assert torch.equal(torch.isnan(ya), torch.isnan(yb))
+    assert torch.equal(torch.nan_to_num(ya), torch.nan_to_num(yb))
+    assert torch.equal(xa.grad, xb.grad)
+    # capture the forward with a tensor alpha; replay after changing alpha
+    alpha = torch.tensor(a0, dtype=torch.float32, device=dev)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        fn(c, alpha)  # eager first (allocations outside the capture)
+    s.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        yg = fn(c, alpha)
+    for a in (a0, float(np.float32(a0 * 0.5)), a0):
+        alpha.fill_(a)
+        g.replay()
+        torch.cuda.synchronize()
+        want = fn(c, a)
+        torch.cuda.synchronize()
+        assert torch.equal(torch.nan_to_num(yg), torch.nan_to_num(want)), a
