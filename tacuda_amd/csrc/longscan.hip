@@ -36,6 +36,7 @@
 #include <cmath>
 
 #include "../../include/tacuda_amd.h"
+#include "divide.hpp"
 #include "families.hpp"
 #include "host.hpp"
 
@@ -108,13 +109,6 @@ __device__ __forceinline__ void toe_mv_add(const Toe& T, const double (&v)[6], d
     for (int j = 0; j <= i; ++j) s = fma(T.c[i - j], v[j], s);
     y[i] = s;
   }
-}
-
-// x / d for a constant divisor d >= 1 with rd = RN(1 / d): Markstein's
-// correction of x * rd, the correctly rounded quotient for every finite x.
-__device__ __forceinline__ double div_by(double x, double d, double rd) {
-  const double q = x * rd;
-  return fma(fma(-q, d, x), rd, q);
 }
 
 // a / b for finite a and a normal b > 0: the hardware reciprocal refined
@@ -995,7 +989,7 @@ __global__ __launch_bounds__(kLW* kWave) void k_scan_one(const float* x, const i
 #pragma unroll
       for (int j = 0; j < kLK; ++j) {
         if (j > 0) ssum = ssum + ((double)xv[j] - (double)lds_ld(lds, T::pos(eo + j)));
-        o[j] = (float)div_by(ssum, (double)P.ps, P.rps);
+        o[j] = (float)div_by(ssum, DivBy{(double)P.ps, P.rps});
       }
       put16<T>(lds, out.sma, c, o);
     }
@@ -1142,7 +1136,7 @@ __global__ __launch_bounds__(kLW* kWave) void k_scan_one(const float* x, const i
         if (snan > 0 || (spos > 0 && sneg > 0)) r = qn;
         else if (spos > 0) r = __builtin_inff();
         else if (sneg > 0) r = -__builtin_inff();
-        else r = (float)div_by(ssum, (double)P.ps, P.rps);
+        else r = (float)div_by(ssum, DivBy{(double)P.ps, P.rps});
         o[i] = r;
       }
       if (a + 8 * h < c.s1) put8(out.sma, c, a + 8 * h, o);

@@ -20,6 +20,7 @@
 #include <tuple>
 
 #include "../../include/tacuda_amd.h"
+#include "divide.hpp"
 #include "families.hpp"
 #include "host.hpp"
 #include "series.hpp"
@@ -437,6 +438,7 @@ __device__ __forceinline__ void pack_hlc(const Seg& c, float* li, float* lo, con
   const int La = pa, Lx = px - 1, Lax = 2 * px - 1;
   const double wb = (double)(pa - 1) / (double)pa, wa = 1.0 - wb;
   const double pp = (double)px, am = 1.0 - 1.0 / pp, bx = (pp - 1.0) / pp;
+  const DivBy byPP = div_by_of(pp);
   auto tr_at = [&](int t) {
     return true_range(at<T>(lh, io.h, c, t), at<T>(ll, io.l, c, t), at<T>(lc, io.c, c, t - 1));
   };
@@ -522,9 +524,9 @@ __device__ __forceinline__ void pack_hlc(const Seg& c, float* li, float* lo, con
         b1[2] = sM;
         b1[3] = sT;
       } else if (t > Lx) {
-        b1[1] = b1[1] - b1[1] / pp + pdm[i];
-        b1[2] = b1[2] - b1[2] / pp + mdm[i];
-        b1[3] = b1[3] - b1[3] / pp + tr[i];
+        b1[1] = b1[1] - div_by(b1[1], byPP) + pdm[i];
+        b1[2] = b1[2] - div_by(b1[2], byPP) + mdm[i];
+        b1[3] = b1[3] - div_by(b1[3], byPP) + tr[i];
       }
     }
   }
@@ -545,9 +547,9 @@ __device__ __forceinline__ void pack_hlc(const Seg& c, float* li, float* lo, con
         yq = sM;
         yt = sT;
       } else if (t > Lx) {
-        yp = yp - yp / pp + pdm[i];
-        yq = yq - yq / pp + mdm[i];
-        yt = yt - yt / pp + tr[i];
+        yp = yp - div_by(yp, byPP) + pdm[i];
+        yq = yq - div_by(yq, byPP) + mdm[i];
+        yt = yt - div_by(yt, byPP) + tr[i];
         double pdi, mdi, dx;
         dir_outputs(yp, yq, yt, pdi, mdi, dx);
         dxv[i] = dx;
@@ -568,7 +570,7 @@ __device__ __forceinline__ void pack_hlc(const Seg& c, float* li, float* lo, con
         if (t >= px && t <= Lax) part[0] += dxv[i];
       }
       wave_total<1>(part);
-      sX = part[0] / pp;
+      sX = div_by(part[0], byPP);
     }
     if (hAx && !inAx) {
       double sp = 0.0, sm = 0.0, st = 0.0, sum = 0.0;
@@ -582,14 +584,14 @@ __device__ __forceinline__ void pack_hlc(const Seg& c, float* li, float* lo, con
       for (int t = px; t <= Lax; ++t) {
         double pd, md;
         dm_at(t, pd, md);
-        sp = sp - sp / pp + pd;
-        sm = sm - sm / pp + md;
-        st = st - st / pp + tr_at(t);
+        sp = sp - div_by(sp, byPP) + pd;
+        sm = sm - div_by(sm, byPP) + md;
+        st = st - div_by(st, byPP) + tr_at(t);
         double pdi, mdi, dx;
         dir_outputs(sp, sm, st, pdi, mdi, dx);
         sum += dx;
       }
-      sX = sum / pp;
+      sX = div_by(sum, byPP);
     }
   }
   const double beta2[1] = {bx};
@@ -599,7 +601,7 @@ __device__ __forceinline__ void pack_hlc(const Seg& c, float* li, float* lo, con
     const int t = a + i;
     if (t >= c.n) continue;
     if (t == Lax) b2[0] = sX;
-    else if (t > Lax) b2[0] = ((b2[0] * (pp - 1.0)) + dxv[i]) / pp;
+    else if (t > Lax) b2[0] = div_by((b2[0] * (pp - 1.0)) + dxv[i], byPP);
   }
   bcarry<1>(c, 1, b2, P2, beta2, y2);
   {
@@ -609,7 +611,7 @@ __device__ __forceinline__ void pack_hlc(const Seg& c, float* li, float* lo, con
       const int t = a + i;
       ov[i] = qnanf_();
       if (t == Lax) yx = sX;
-      else if (t > Lax) yx = ((yx * (pp - 1.0)) + dxv[i]) / pp;
+      else if (t > Lax) yx = div_by((yx * (pp - 1.0)) + dxv[i], byPP);
       if (t >= Lax) ov[i] = (float)yx;
     }
     pack_store<T>(c, lo, ov, io.out[kOAdx]);
@@ -641,6 +643,7 @@ __device__ __forceinline__ void win_means(const Seg& c, ColLds<C::E>& s, const P
   static_assert(NJ % IL == 0, "chunk groups");
   if ((m & (1u << kOSma)) && !smaInBb) {
     const int p = P.smaP;
+    const DivBy byP = div_by_of((double)p);
     for (int g = 0; g < NJ; g += IL) {
       double sum[IL];
 #pragma unroll
@@ -653,7 +656,7 @@ __device__ __forceinline__ void win_means(const Seg& c, ColLds<C::E>& s, const P
 #pragma unroll
       for (int u = 0; u < IL; ++u) {
         const int j = C::NH + g + u, e = j * kWave + lane;
-        const float v = (base + e >= p - 1) ? (float)(sum[u] / (double)p) : qnanf_();
+        const float v = (base + e >= p - 1) ? (float)div_by(sum[u], byP) : qnanf_();
         col_store<C>(io.out[kOSma], c, j, v);
       }
     }
@@ -661,6 +664,7 @@ __device__ __forceinline__ void win_means(const Seg& c, ColLds<C::E>& s, const P
   if (m & (1u << kOWma)) {
     const int p = P.wmaP;
     const double divider = (double)p * (p + 1) / 2.0;
+    const DivBy byDiv = div_by_of(divider);
     for (int g = 0; g < NJ; g += IL) {
       double sum[IL];
 #pragma unroll
@@ -676,7 +680,7 @@ __device__ __forceinline__ void win_means(const Seg& c, ColLds<C::E>& s, const P
 #pragma unroll
       for (int u = 0; u < IL; ++u) {
         const int j = C::NH + g + u, e = j * kWave + lane;
-        const float v = p == 1 ? lds_f(s.c, e) : (float)(sum[u] / divider);
+        const float v = p == 1 ? lds_f(s.c, e) : (float)div_by(sum[u], byDiv);
         col_store<C>(io.out[kOWma], c, j, (base + e >= p - 1) ? v : qnanf_());
       }
     }
@@ -694,6 +698,7 @@ __device__ __forceinline__ void win_bbands(const Seg& c, ColLds<C::E>& s, const 
   if (bbAll) {
     const int p = P.bbP;
     const double inv = 1.0 / (double)p;
+    const DivBy byP = div_by_of((double)p);
     // the output chunks in groups of IL, evaluated side by side
     constexpr int NJ = C::NC - C::NH;
     static_assert(NJ % IL == 0, "chunk groups");
@@ -718,7 +723,7 @@ __device__ __forceinline__ void win_bbands(const Seg& c, ColLds<C::E>& s, const 
 #pragma unroll
     for (int u = 0; u < IL; ++u) {
       const int j = C::NH + g + u, e = j * kWave + lane;
-      const double mean = sum[u] / p;
+      const double mean = div_by(sum[u], byP);
       const double var = col_var_finish(s1[u], s2[u], inv);
       const double sd = (var != 0.0) ? sqrt(var) : 0.0;
       float up, lo;
@@ -731,7 +736,7 @@ __device__ __forceinline__ void win_bbands(const Seg& c, ColLds<C::E>& s, const 
         lo = (float)(mean - sd * P.bbDn);
       }
       const bool ok = base + e >= p - 1;
-      if (smaInBb) col_store<C>(io.out[kOSma], c, j, ok ? (float)(sum[u] / (double)p) : qnanf_());
+      if (smaInBb) col_store<C>(io.out[kOSma], c, j, ok ? (float)mean : qnanf_());
       col_store<C>(io.out[kOBbU], c, j, ok ? up : qnanf_());
       col_store<C>(io.out[kOBbM], c, j, ok ? (float)mean : qnanf_());
       col_store<C>(io.out[kOBbL], c, j, ok ? lo : qnanf_());
@@ -770,6 +775,7 @@ __device__ __forceinline__ void win_extrema(const Seg& c, ColLds<C::E>& s, const
     if (wantW && P.willrP == kp) willr();
     if (wantSt) {
       const int dp = P.stD;
+      const DivBy byD = div_by_of((double)dp);
       const int lb = kp - 1 + 2 * (dp - 1);
       // fast %K from bar s0 - 2(d-1), slow %K from s0 - (d-1)
       const int eK = C::H - 2 * (dp - 1), eS = C::H - (dp - 1);
@@ -792,7 +798,7 @@ __device__ __forceinline__ void win_extrema(const Seg& c, ColLds<C::E>& s, const
           double sum = 0.0;
           if (e >= eS)
             for (int q = e - dp + 1; q <= e; ++q) sum += lds_d(s.d0, q);
-          if (e >= eS) lds_wd(d1, e, sum / dp);
+          if (e >= eS) lds_wd(d1, e, div_by(sum, byD));
         }
         col_sync();
         for (int j = C::NH; j < C::NC; ++j) {
@@ -801,7 +807,7 @@ __device__ __forceinline__ void win_extrema(const Seg& c, ColLds<C::E>& s, const
           for (int q = e - dp + 1; q <= e; ++q) sum += lds_d(d1, q);
           const bool ok = base + e >= lb;
           col_store<C>(io.out[kOStK], c, j, ok ? (float)lds_d(d1, e) : qnanf_());
-          col_store<C>(io.out[kOStD], c, j, ok ? (float)(sum / dp) : qnanf_());
+          col_store<C>(io.out[kOStD], c, j, ok ? (float)div_by(sum, byD) : qnanf_());
         }
         col_sync();
       } else {
@@ -835,7 +841,7 @@ __device__ __forceinline__ void win_extrema(const Seg& c, ColLds<C::E>& s, const
   #pragma unroll
           for (int j = 0; j < C::NC; ++j) {
             const int e = j * kWave + lane;
-            if (e >= eS) lds_wd(d1, e, sum[j] / dp);
+            if (e >= eS) lds_wd(d1, e, div_by(sum[j], byD));
           }
         }
         col_sync();
@@ -853,7 +859,7 @@ __device__ __forceinline__ void win_extrema(const Seg& c, ColLds<C::E>& s, const
             const int j = C::NH + u, e = j * kWave + lane;
             const bool ok = base + e >= lb;
             col_store<C>(io.out[kOStK], c, j, ok ? (float)lds_d(d1, e) : qnanf_());
-            col_store<C>(io.out[kOStD], c, j, ok ? (float)(sum[u] / dp) : qnanf_());
+            col_store<C>(io.out[kOStD], c, j, ok ? (float)div_by(sum[u], byD) : qnanf_());
           }
         }
         col_sync();
@@ -876,11 +882,12 @@ __device__ __forceinline__ void win_cci(const Seg& c, ColLds<C::E>& s, double* d
   const int base = c.s0 - C::H;
   if (m & (1u << kOCci)) {
     const int p = P.cciP;
+    const DivBy byP = div_by_of((double)p), by3 = div_by_of(3.0);
     const int e0 = C::H - (p - 1);
     for (int j = e0 / kWave; j < C::NC; ++j) {
       const int e = j * kWave + lane;
       const double tp =
-          ((double)lds_f(s.h, e) + (double)lds_f(s.l, e) + (double)lds_f(s.c, e)) / 3.0;
+          div_by((double)lds_f(s.h, e) + (double)lds_f(s.l, e) + (double)lds_f(s.c, e), by3);
       lds_wd(d, e, tp);
     }
     col_sync();
@@ -897,7 +904,7 @@ __device__ __forceinline__ void win_cci(const Seg& c, ColLds<C::E>& s, double* d
         for (int u = 0; u < IL; ++u) sum[u] += lds_d(d, eo + u * kWave - k);
       }
 #pragma unroll
-      for (int u = 0; u < IL; ++u) sma[u] = sum[u] / p;
+      for (int u = 0; u < IL; ++u) sma[u] = div_by(sum[u], byP);
       for (int k = 0; k < p; ++k) {
 #pragma unroll
         for (int u = 0; u < IL; ++u) dev[u] += fabs(lds_d(d, eo + u * kWave - k) - sma[u]);
@@ -905,7 +912,7 @@ __device__ __forceinline__ void win_cci(const Seg& c, ColLds<C::E>& s, double* d
 #pragma unroll
       for (int u = 0; u < IL; ++u) {
         const int j = C::NH + g + u, e = j * kWave + lane;
-        const double md = dev[u] / p;
+        const double md = div_by(dev[u], byP);
         const double num = lds_d(d, e) - sma[u];
         const float v = (num != 0.0 && md != 0.0) ? (float)(num / (0.015 * md)) : 0.0f;
         col_store<C>(io.out[kOCci], c, j, (base + e >= p - 1) ? v : qnanf_());
