@@ -289,10 +289,10 @@ def _pipeline7(a, torch, ops, T, dev):
     stats = dict(T.last)
     print(f"pipeline7 fused: {ms * 1e3:.1f} us/pass (median {stats['median_ms'] * 1e3:.1f}, "
           f"min {stats['min_ms'] * 1e3:.1f})", file=sys.stderr, flush=True)
-    # the pass's kernels (two passes for a long dense row: k_pipe_lead,
-    # k_pipe_fold with the window part, k_pipe_group, k_pipe_emit; one row
-    # of 1954 segments needs no k_pipe_rows launch), each timed alone with
-    # events around its launch
+    # the pass's kernels (two passes for a long dense row: k_pipe_fold with
+    # the window part, k_pipe_group, k_pipe_emit; one row of 1954 segments
+    # needs no k_pipe_rows launch), each timed alone with events around its
+    # launch
     launch = kernel_launch_ms(T, sets, lambda s: ops.pack(s["h"], s["l"], s["c"],
                                                           outputs=PIPE7_OUT, **PIPE7), 20)
     print("  kernels: " + ", ".join(f"{k} {v * 1e3:.1f} us" for k, v in launch.items()),

@@ -15,9 +15,12 @@
 // level 2 (DEMA/TEMA e2, MACD signal, ADX) and level 3 (TEMA e3) follow.
 #include <hip/hip_runtime.h>
 
+#include <array>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <vector>
 
 #include "../../include/tacuda_amd.h"
 #include "divide.hpp"
@@ -1243,7 +1246,7 @@ struct PipeP {
   double q[7][kQF];
   // Per-lane multipliers of the two row-broadcast scan steps (lane l of a
   // steady wave after the row-shift steps, then after the first broadcast):
-  // qt[(k * kQF + field) * 64 + lane], written by k_pipe_lead.
+  // qt[(k * kQF + field) * 64 + lane]: the host table or k_pipe_qt's.
   double* qt;
 };
 
@@ -1649,25 +1652,13 @@ __device__ __forceinline__ int pipe_groups(const PipeP& P) {
   return (P.segsPerRow + kWave - 1) / kWave;
 }
 
-// One wave per row: the row's first non-NaN close.  Row 0's wave also
-// writes the per-lane multipliers of steady waves (the general scan run on
-// q[0] in every lane, recorded before each broadcast step).
-__global__ __launch_bounds__(64) void k_pipe_lead(PipeP P, const float* x, int* lead) {
-  const int row = blockIdx.x;
-  if (row >= P.rows) return;
-  const int n = P.n;
-  const float* r = x + (size_t)row * n;
-  int f = n;
-  for (int t0 = 0; t0 < n; t0 += kWave) {
-    const int t = t0 + lane_id();
-    const unsigned long long m = __ballot(t < n && !__builtin_isnan(r[t]));
-    if (m) {
-      f = t0 + __builtin_ctzll(m);
-      break;
-    }
-  }
-  if (lane_id() == 0) lead[row] = f;
-  if (row != 0) return;
+// One wave: the per-lane multipliers of steady waves (the general scan run
+// on q[0] in every lane, recorded before each broadcast step).  The host
+// computes the same table (pipe_qt_host) and keeps it on the device per
+// parameter set; this kernel writes it where the host cannot upload it
+// (under graph capture before the parameters were seen eagerly, and in the
+// allocation-free entry point).
+__global__ __launch_bounds__(64) void k_pipe_qt(PipeP P) {
   PC F;
   pc_id(F);
   pc_set_a(F, P.q[0]);
@@ -2009,8 +2000,7 @@ union PipeFoldLds {
 // window part of the pass (one family group per wave) first, then the fold
 // (one wave per recurrence group).
 __global__ __launch_bounds__(kPW* kWave) void k_pipe_fold(PipeP P, PackP prm, PackIO io,
-                                                          const int* lead, double* agg,
-                                                          Geometry gw, int nbW) {
+                                                          double* agg, Geometry gw, int nbW) {
   __shared__ __align__(16) PipeFoldLds sm;
   const int w = threadIdx.x / kWave;
   if ((int)blockIdx.x < nbW) {
@@ -2026,8 +2016,9 @@ __global__ __launch_bounds__(kPW* kWave) void k_pipe_fold(PipeP P, PackP prm, Pa
   int row;
   const Seg c = pipe_seg(P, seg, row);
   PLds& L = sm.rec;
+  const float c0 = io.c[c.rowOff];  // in flight with the staging loads
   pipe_stage_block<PTile>(L, io, c, P, w);
-  const int f = lead[row];
+  const int f = __builtin_isnan(c0) ? first_valid(c, io.c) : 0;
   double* o = agg + (size_t)seg * 16;
   if (w == 0) pipe_fold_wave<kGE>(P, L, io, c, f, o);
   else if (w == 1) pipe_fold_wave<kGR>(P, L, io, c, f, o);
@@ -2040,15 +2031,16 @@ __global__ __launch_bounds__(kPW* kWave) void k_pipe_fold(PipeP P, PackP prm, Pa
 // of the block idles.  The other groups' fields of the composite are the
 // identity.
 template <unsigned GM>
-__global__ __launch_bounds__(kPW* kWave) void k_pipe_fold1(PipeP P, PackIO io, const int* lead,
-                                                           double* agg) {
+__global__ __launch_bounds__(kPW* kWave) void k_pipe_fold1(PipeP P, PackIO io, double* agg) {
   __shared__ __align__(16) PLds sm[kPW];
   const int w = threadIdx.x / kWave;
   const int seg = blockIdx.x * kPW + w;
   if (seg >= P.rows * P.segsPerRow) return;
   int row;
   const Seg c = pipe_seg(P, seg, row);
+  const float c0 = io.c[c.rowOff];  // in flight with the staging loads
   pipe_stage<PTile>(sm[w], io, c, P);
+  const int f = __builtin_isnan(c0) ? first_valid(c, io.c) : 0;
   double* o = agg + (size_t)seg * 16;
   if (lane_id() == kWave - 1) {
     PC I;
@@ -2058,13 +2050,13 @@ __global__ __launch_bounds__(kPW* kWave) void k_pipe_fold1(PipeP P, PackIO io, c
     if constexpr (GM != kGM) pc_store_group<kGM>(o, I);
     if constexpr (GM != kGA) pc_store_group<kGA>(o, I);
   }
-  pipe_fold_wave<GM>(P, sm[w], io, c, lead[row], o);
+  pipe_fold_wave<GM>(P, sm[w], io, c, f, o);
 }
 
 // One wave of the emit pass: group GM of the block's segment.
 template <unsigned GM>
 __device__ __forceinline__ void pipe_emit_wave(const PipeP& P, const PLds& L, const PackIO& io,
-                                               const Seg& c, int row, int seg, const int* lead,
+                                               const Seg& c, int row, int seg, int f,
                                                const double* agg, const double* gtot,
                                                const double* genter) {
   if (!pipe_wants<GM>(P)) return;
@@ -2072,7 +2064,7 @@ __device__ __forceinline__ void pipe_emit_wave(const PipeP& P, const PLds& L, co
   const int a = c.s0 + lane_id() * kPK;
   PBars B;
   pipe_bars<T>(L.c, L.h, L.l, io, c, P, a, B);
-  const PSeeds S = pipe_seeds<T, GM>(L.c, L.h, L.l, io, c, P, lead[row]);
+  const PSeeds S = pipe_seeds<T, GM>(L.c, L.h, L.l, io, c, P, f);
   // state entering the segment: its group's entering state (from k_pipe_rows,
   // or scanned here when genter is null), then the group's segments before it
   PState ge;
@@ -2088,25 +2080,27 @@ __device__ __forceinline__ void pipe_emit_wave(const PipeP& P, const PLds& L, co
 
 // One workgroup per segment, one wave per recurrence group: the state
 // entering the segment, then the replay of its bars.
-__global__ __launch_bounds__(kPW* kWave) void k_pipe_emit(PipeP P, PackIO io, const int* lead,
-                                                          const double* agg, const double* gtot,
+__global__ __launch_bounds__(kPW* kWave) void k_pipe_emit(PipeP P, PackIO io, const double* agg,
+                                                          const double* gtot,
                                                           const double* genter) {
   __shared__ __align__(16) PLds L;
   const int seg = blockIdx.x, w = threadIdx.x / kWave;
   if (seg >= P.rows * P.segsPerRow) return;
   int row;
   const Seg c = pipe_seg(P, seg, row);
+  const float c0 = io.c[c.rowOff];  // in flight with the staging loads
   pipe_stage_block<PTile>(L, io, c, P, w);
-  if (w == 0) pipe_emit_wave<kGE>(P, L, io, c, row, seg, lead, agg, gtot, genter);
-  else if (w == 1) pipe_emit_wave<kGR>(P, L, io, c, row, seg, lead, agg, gtot, genter);
-  else if (w == 2) pipe_emit_wave<kGM>(P, L, io, c, row, seg, lead, agg, gtot, genter);
-  else pipe_emit_wave<kGA>(P, L, io, c, row, seg, lead, agg, gtot, genter);
+  const int f = __builtin_isnan(c0) ? first_valid(c, io.c) : 0;
+  if (w == 0) pipe_emit_wave<kGE>(P, L, io, c, row, seg, f, agg, gtot, genter);
+  else if (w == 1) pipe_emit_wave<kGR>(P, L, io, c, row, seg, f, agg, gtot, genter);
+  else if (w == 2) pipe_emit_wave<kGM>(P, L, io, c, row, seg, f, agg, gtot, genter);
+  else pipe_emit_wave<kGA>(P, L, io, c, row, seg, f, agg, gtot, genter);
 }
 
 // The emit pass of k_pipe_fold1's passes: one segment per wave.
 template <unsigned GM>
-__global__ __launch_bounds__(kPW* kWave) void k_pipe_emit1(PipeP P, PackIO io, const int* lead,
-                                                           const double* agg, const double* gtot,
+__global__ __launch_bounds__(kPW* kWave) void k_pipe_emit1(PipeP P, PackIO io, const double* agg,
+                                                           const double* gtot,
                                                            const double* genter) {
   __shared__ __align__(16) PLds sm[kPW];
   const int w = threadIdx.x / kWave;
@@ -2114,8 +2108,10 @@ __global__ __launch_bounds__(kPW* kWave) void k_pipe_emit1(PipeP P, PackIO io, c
   if (seg >= P.rows * P.segsPerRow) return;
   int row;
   const Seg c = pipe_seg(P, seg, row);
+  const float c0 = io.c[c.rowOff];  // in flight with the staging loads
   pipe_stage<PTile>(sm[w], io, c, P);
-  pipe_emit_wave<GM>(P, sm[w], io, c, row, seg, lead, agg, gtot, genter);
+  const int f = __builtin_isnan(c0) ? first_valid(c, io.c) : 0;
+  pipe_emit_wave<GM>(P, sm[w], io, c, row, seg, f, agg, gtot, genter);
 }
 
 // Whether the two passes serve this pass: dense rows of at least kPMinLen
@@ -2174,6 +2170,97 @@ static void pipe_steady_q(PipeP& P) {
   }
 }
 
+// The table k_pipe_qt writes, on the host: every lane starts from q[0]
+// and takes the same DPP steps (row_shr 1, 2, 4, 8, then row_bcast 15 on
+// rows 1 and 3; a lane without a source reads the identity) with the same
+// products, fmas and clamps in the same order, so the doubles are the
+// kernel's.
+static void pipe_qt_host(const PipeP& P, double* out) {
+  struct Q {
+    double Ae, Ag, Al, Af, As, agf, ags, agg, Aa;
+  };
+  const double* q = P.q[0];
+  Q v[kWave];
+  for (int l = 0; l < kWave; ++l) v[l] = Q{q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], q[8]};
+  const Q id{1.0, 1.0, 1.0, 1.0, 1.0, 0.0, 0.0, 1.0, 1.0};
+  auto step = [&](auto src) {
+    Q nv[kWave];
+    for (int l = 0; l < kWave; ++l) {
+      const int j = src(l);
+      const Q o = j < 0 ? id : v[j];
+      Q f = v[l];
+      f.Ae = mulc_h(f.Ae, o.Ae);
+      f.Ag = mulc_h(f.Ag, o.Ag);
+      f.Al = mulc_h(f.Al, o.Al);
+      const double agf = std::fma(f.agf, o.Af, f.agg * o.agf);
+      const double ags = std::fma(f.ags, o.As, f.agg * o.ags);
+      const double agg = f.agg * o.agg;
+      f.agf = agf;
+      f.ags = ags;
+      f.agg = agg;
+      f.Af = mulc_h(f.Af, o.Af);
+      f.As = mulc_h(f.As, o.As);
+      f.Aa = mulc_h(f.Aa, o.Aa);
+      nv[l] = f;
+    }
+    for (int l = 0; l < kWave; ++l) v[l] = nv[l];
+  };
+  for (int d = 1; d <= 8; d *= 2) step([d](int l) { return (l % 16) >= d ? l - d : -1; });
+  auto put = [&](int k) {
+    for (int l = 0; l < kWave; ++l) {
+      const double f[kQF] = {v[l].Ae, v[l].Ag, v[l].Al, v[l].Af, v[l].As,
+                             v[l].agf, v[l].ags, v[l].agg, v[l].Aa};
+      for (int i = 0; i < kQF; ++i) out[(k * kQF + i) * kWave + l] = f[i];
+    }
+  };
+  put(0);
+  step([](int l) {
+    const int r = l / 16;
+    return r == 1 ? 15 : r == 3 ? 47 : -1;
+  });
+  put(1);
+}
+
+// Device copies of the multiplier table per (device, q[0]), uploaded once
+// outside capture; null where that cannot be done here (under capture,
+// inside the allocation-free entry point, or past the cache's bound), and
+// the caller then runs k_pipe_qt.  Entries live until the process ends:
+// captured graphs may hold their addresses.
+static ctStatus_t pipe_qt_cached(const PipeP& P, hipStream_t s, const double** out) {
+  *out = nullptr;
+  if (t_ws_fixed) return CT_STATUS_SUCCESS;
+  int dev = 0;
+  TCA_HIP(hipGetDevice(&dev));
+  std::array<unsigned long long, kQF + 1> key{};
+  key[0] = (unsigned long long)dev;
+  std::memcpy(key.data() + 1, P.q[0], sizeof(double) * kQF);
+  static std::mutex mu;
+  static std::map<std::array<unsigned long long, kQF + 1>, double*> cache;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = cache.find(key);
+  if (it != cache.end()) {
+    *out = it->second;
+    return CT_STATUS_SUCCESS;
+  }
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (s != nullptr) TCA_HIP(hipStreamIsCapturing(s, &cs));
+  if (cs != hipStreamCaptureStatusNone || cache.size() >= 256) return CT_STATUS_SUCCESS;
+  std::vector<double> host(2 * kQF * kWave);
+  pipe_qt_host(P, host.data());
+  double* d = nullptr;
+  const size_t bytes = host.size() * sizeof(double);
+  TCA_HIP(hipMalloc(&d, bytes));
+  hipError_t e = hipMemcpyAsync(d, host.data(), bytes, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    (void)hipFree(d);
+    return hip_fail(e, "pipe multiplier table upload");
+  }
+  cache[key] = d;
+  *out = d;
+  return CT_STATUS_SUCCESS;
+}
+
 ctStatus_t launch_pipe(int len, int rows, hipStream_t s, const PackP& prm, const PackIO& io) {
   PipeP P{};
   P.n = len;
@@ -2209,24 +2296,27 @@ ctStatus_t launch_pipe(int len, int rows, hipStream_t s, const PackP& prm, const
   if (segs > (1ll << 30)) return fail(CT_STATUS_UNSUPPORTED, "pack: batch too large");
   pipe_steady_q(P);
   const long long groups = (long long)div_up(P.segsPerRow, kWave) * rows;
-  const size_t leadB = ((size_t)rows * sizeof(int) + 255) & ~(size_t)255;
   const size_t bytes =
-      leadB + ((size_t)segs * 16 + (size_t)groups * (16 + 7) + 2 * kQF * kWave) * sizeof(double);
+      ((size_t)segs * 16 + (size_t)groups * (16 + 7) + 2 * kQF * kWave) * sizeof(double);
   Ws ws;
   ctStatus_t st = workspace_for(s, bytes, &ws, 0);
   if (st != CT_STATUS_SUCCESS) return st;
-  int* lead = reinterpret_cast<int*>(ws.area);
-  double* agg = reinterpret_cast<double*>(reinterpret_cast<char*>(ws.area) + leadB);
+  double* agg = ws.area;
   double* gtot = agg + (size_t)segs * 16;
   double* genter = gtot + (size_t)groups * 16;
-  P.qt = genter + (size_t)groups * 7;
+  const double* qt = nullptr;
+  st = pipe_qt_cached(P, s, &qt);
+  if (st != CT_STATUS_SUCCESS) return st;
+  if (qt) {
+    P.qt = const_cast<double*>(qt);
+  } else {
+    P.qt = genter + (size_t)groups * 7;
+    KTime kt(s, "k_pipe_qt");
+    hipLaunchKernelGGL(k_pipe_qt, dim3(1), dim3(kWave), 0, s, P);
+    TCA_HIP(hipGetLastError());
+  }
   Geometry gw = dense_geometry(len, rows, ColTile<64>::SEG);
   const int nbW = (prm.mask & kWinMask) ? gw.totalSegs : 0;
-  {
-    KTime kt(s, "k_pipe_lead");
-    hipLaunchKernelGGL(k_pipe_lead, dim3(rows), dim3(kWave), 0, s, P, io.c, lead);
-  }
-  TCA_HIP(hipGetLastError());
   // one recurrence group and no window part: the one-segment-per-wave kernels
   const unsigned one = nbW ? 0u
                            : (P.wE ? kGE : 0u) | (P.wR ? kGR : 0u) | (P.wM ? kGM : 0u) |
@@ -2234,12 +2324,12 @@ ctStatus_t launch_pipe(int len, int rows, hipStream_t s, const PackP& prm, const
   const dim3 g1(div_up(segs, kPW)), blk(kPW * kWave);
   {
     KTime kt(s, "k_pipe_fold");
-    if (one == kGE) hipLaunchKernelGGL(k_pipe_fold1<kGE>, g1, blk, 0, s, P, io, lead, agg);
-    else if (one == kGR) hipLaunchKernelGGL(k_pipe_fold1<kGR>, g1, blk, 0, s, P, io, lead, agg);
-    else if (one == kGM) hipLaunchKernelGGL(k_pipe_fold1<kGM>, g1, blk, 0, s, P, io, lead, agg);
-    else if (one == kGA) hipLaunchKernelGGL(k_pipe_fold1<kGA>, g1, blk, 0, s, P, io, lead, agg);
+    if (one == kGE) hipLaunchKernelGGL(k_pipe_fold1<kGE>, g1, blk, 0, s, P, io, agg);
+    else if (one == kGR) hipLaunchKernelGGL(k_pipe_fold1<kGR>, g1, blk, 0, s, P, io, agg);
+    else if (one == kGM) hipLaunchKernelGGL(k_pipe_fold1<kGM>, g1, blk, 0, s, P, io, agg);
+    else if (one == kGA) hipLaunchKernelGGL(k_pipe_fold1<kGA>, g1, blk, 0, s, P, io, agg);
     else
-      hipLaunchKernelGGL(k_pipe_fold, dim3(nbW + segs), blk, 0, s, P, prm, io, lead, agg, gw, nbW);
+      hipLaunchKernelGGL(k_pipe_fold, dim3(nbW + segs), blk, 0, s, P, prm, io, agg, gw, nbW);
   }
   TCA_HIP(hipGetLastError());
   {
@@ -2261,11 +2351,11 @@ ctStatus_t launch_pipe(int len, int rows, hipStream_t s, const PackP& prm, const
   {
     KTime kt(s, "k_pipe_emit");
     const double* ge = rowsInEmit ? nullptr : genter;
-    if (one == kGE) hipLaunchKernelGGL(k_pipe_emit1<kGE>, g1, blk, 0, s, P, io, lead, agg, gtot, ge);
-    else if (one == kGR) hipLaunchKernelGGL(k_pipe_emit1<kGR>, g1, blk, 0, s, P, io, lead, agg, gtot, ge);
-    else if (one == kGM) hipLaunchKernelGGL(k_pipe_emit1<kGM>, g1, blk, 0, s, P, io, lead, agg, gtot, ge);
-    else if (one == kGA) hipLaunchKernelGGL(k_pipe_emit1<kGA>, g1, blk, 0, s, P, io, lead, agg, gtot, ge);
-    else hipLaunchKernelGGL(k_pipe_emit, dim3(segs), blk, 0, s, P, io, lead, agg, gtot, ge);
+    if (one == kGE) hipLaunchKernelGGL(k_pipe_emit1<kGE>, g1, blk, 0, s, P, io, agg, gtot, ge);
+    else if (one == kGR) hipLaunchKernelGGL(k_pipe_emit1<kGR>, g1, blk, 0, s, P, io, agg, gtot, ge);
+    else if (one == kGM) hipLaunchKernelGGL(k_pipe_emit1<kGM>, g1, blk, 0, s, P, io, agg, gtot, ge);
+    else if (one == kGA) hipLaunchKernelGGL(k_pipe_emit1<kGA>, g1, blk, 0, s, P, io, agg, gtot, ge);
+    else hipLaunchKernelGGL(k_pipe_emit, dim3(segs), blk, 0, s, P, io, agg, gtot, ge);
   }
   TCA_HIP(hipGetLastError());
   return CT_STATUS_SUCCESS;

@@ -36,7 +36,7 @@ def test_two_pass_kernels_are_timed(dev):
     names = [k for k, _ in rec]
     # one row of 40 segments: the emit waves scan the row's group composites
     # themselves, so there is no k_pipe_rows launch
-    one = ["k_pipe_lead", "k_pipe_fold", "k_pipe_group", "k_pipe_emit"]
+    one = ["k_pipe_fold", "k_pipe_group", "k_pipe_emit"]
     assert names == one + one
     assert all(ms > 0.0 for _, ms in rec)
     ta.ops.pack(h, l, c, outputs=outs)
