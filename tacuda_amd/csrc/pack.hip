@@ -1996,23 +1996,38 @@ union PipeFoldLds {
   ColSplitLds<ColTile<64>::E> win;
 };
 
-// [window segments | recurrence segments], one segment per workgroup: the
-// window part of the pass (one family group per wave) first, then the fold
-// (one wave per recurrence group).
+// One segment per workgroup: a recurrence segment (one wave per recurrence
+// group) or a window segment of the pass's window part (one family group
+// per wave).  With a window part, the blocks come in runs of 24: the 8
+// recurrence segments u = 8m .. 8m+7 and then, in the same order, the two
+// 256-bar window segments over each one's bars, so that the three blocks
+// of segment u run on one XCD (workgroup b on XCD b mod 8) in consecutive
+// slots and the later ones read the bars through its L2.
+constexpr int kPipeRun = 8;
 __global__ __launch_bounds__(kPW* kWave) void k_pipe_fold(PipeP P, PackP prm, PackIO io,
                                                           double* agg, Geometry gw, int nbW) {
   __shared__ __align__(16) PipeFoldLds sm;
   const int w = threadIdx.x / kWave;
-  if ((int)blockIdx.x < nbW) {
-    Seg c;
-    c.tree = false;
-    c.epoch = 0;
-    if (resolve(gw, ColTile<64>::SEG, (int)blockIdx.x, c))
-      pack_windows_split<ColTile<64>, kPWinIL>(c, sm.win, prm, io, w);
-    return;
+  const int segs = P.rows * P.segsPerRow;
+  int seg = (int)blockIdx.x;
+  if (nbW > 0) {
+    const int m = (int)blockIdx.x / (3 * kPipeRun), in = (int)blockIdx.x % (3 * kPipeRun);
+    const int kind = in / kPipeRun, u = m * kPipeRun + in % kPipeRun;
+    if (u >= segs) return;
+    if (kind > 0) {
+      const int row = u / P.segsPerRow, j = u - row * P.segsPerRow;
+      const int wj = 2 * j + (kind - 1);
+      if (wj >= gw.segsPerRow) return;
+      Seg c;
+      c.tree = false;
+      c.epoch = 0;
+      if (resolve(gw, ColTile<64>::SEG, row * gw.segsPerRow + wj, c))
+        pack_windows_split<ColTile<64>, kPWinIL>(c, sm.win, prm, io, w);
+      return;
+    }
+    seg = u;
   }
-  const int seg = (int)blockIdx.x - nbW;
-  if (seg >= P.rows * P.segsPerRow) return;
+  if (seg >= segs) return;
   int row;
   const Seg c = pipe_seg(P, seg, row);
   PLds& L = sm.rec;
@@ -2329,7 +2344,9 @@ ctStatus_t launch_pipe(int len, int rows, hipStream_t s, const PackP& prm, const
     else if (one == kGM) hipLaunchKernelGGL(k_pipe_fold1<kGM>, g1, blk, 0, s, P, io, agg);
     else if (one == kGA) hipLaunchKernelGGL(k_pipe_fold1<kGA>, g1, blk, 0, s, P, io, agg);
     else
-      hipLaunchKernelGGL(k_pipe_fold, dim3(nbW + segs), blk, 0, s, P, prm, io, agg, gw, nbW);
+      hipLaunchKernelGGL(k_pipe_fold,
+                         dim3(nbW ? 3 * kPipeRun * div_up(segs, kPipeRun) : segs), blk, 0, s,
+                         P, prm, io, agg, gw, nbW);
   }
   TCA_HIP(hipGetLastError());
   {
