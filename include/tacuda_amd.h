@@ -95,7 +95,7 @@ CTAPI_EXPORT ctStatus_t ctx_pack_batch_d(const float* d_high, const float* d_low
 /* No counterpart in the reference: for callers that cannot allocate device
  * memory (code under hipGraph capture, tight serving loops, harnesses that
  * forbid hipMalloc).  The same pass and outputs as ctx_pack_batch_d, with
- * its look-back scratch in a fixed array of 2 x 8 MiB per GPU held until
+ * its look-back scratch in a fixed array of 3 x 8 MiB per GPU held until
  * the process ends, instead of the per-stream workspace cache: the call
  * allocates nothing.  A pass whose scratch exceeds 8 MiB per part (about
  * 3 x 10^7 bars of long rows) returns CT_STATUS_INVALID_ARGUMENT (outputs

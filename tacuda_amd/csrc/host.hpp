@@ -47,12 +47,13 @@ ctStatus_t guarded(F&& body) {
 // ---------------------------------------------------------- workspace --
 // Look-back tree area plus the ticket/epoch header, one per HIP stream and
 // part so that launches ordered on one stream share it safely; one launch
-// whose blocks run two independent look-back scans gives each its own part
-// (kWsParts per stream).  Growing it is refused while the stream is being
+// whose blocks run two independent look-back scans gives each its own part,
+// and the fused pass's one-launch form keeps its tagged records in a part of
+// their own (kWsParts per stream).  Growing it is refused while the stream is being
 // captured; growth outside capture retires the old buffer instead of freeing
 // it (graphs captured earlier keep its address), and retired buffers are
 // freed when the stream's workspaces are released or at pool cleanup.
-constexpr int kWsParts = 2;
+constexpr int kWsParts = 3;
 ctStatus_t workspace_for(hipStream_t stream, size_t treeBytes, Ws* out, int part = 0);
 // While a WsFixedScope is alive on a thread, workspace_for hands out these
 // caller-provided buffers (each `bytes` of area after a WsHeader) instead of
@@ -151,13 +152,15 @@ bool scan_serves(int len, int rows, int windowLookback);
 
 // ------------------------------------------------------ fused pass --
 // The family calls of EMA, RSI, MACD and ATR on rows of at least 8192 bars run
-// the fused pass's two streaming passes (the pack's outputs for that family,
-// the same oracle-following arithmetic) for dense batches of at least
-// kPipeMinRows rows and kPipeMinSegments 512-bar segments in all, or for
-// fewer rows holding at least fewRowsMinSegs segments (the look-back kernels'
-// chains grow with the row length, the passes' four launches do not):
-// *served says whether they did; when not, nothing was launched and the
-// caller takes its own path.
+// the fused pass (the pack's outputs for that family, the same
+// oracle-following arithmetic): its one-launch form wherever that serves (at
+// most 1024 rows, 16384 512-bar segments and 1,050,624 bars per row; as fast
+// as or faster than the look-back kernels at every measured shape), else its
+// two passes for dense batches of at least kPipeMinRows rows and
+// kPipeMinSegments segments in all, or for fewer rows holding at least
+// fewRowsMinSegs segments (the look-back kernels' chains grow with the row
+// length, the passes' launches do not): *served says whether it did; when
+// not, nothing was launched and the caller takes its own path.
 constexpr int kPipeMinRows = 16;
 constexpr long long kPipeMinSegments = 512;
 constexpr long long kPipeFewRowsSegs = 4096;      // ATR

@@ -14,8 +14,9 @@
 namespace tca {
 namespace {
 
-// Two parts (the pass's close and high/low/close recurrence groups), each a
-// WsHeader plus kNoAllocPartBytes of look-back slots and carries.
+// Three parts (the pass's close and high/low/close recurrence groups, and
+// the records of its one-launch form), each a WsHeader plus
+// kNoAllocPartBytes of look-back slots and carries.
 constexpr size_t kNoAllocPartBytes = size_t(8) << 20;
 constexpr size_t kPartStride = (sizeof(WsHeader) + kNoAllocPartBytes + 255) / 256 * 256;
 static_assert(kPartStride % 256 == 0, "parts keep 256-byte alignment");

@@ -20,6 +20,7 @@
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/tacuda_amd.h"
@@ -1945,13 +1946,11 @@ __device__ __forceinline__ void pipe_replay(const PipeP& P, const PSeeds& S, con
 // One group of the emit pass: the lane's composite, the wave scan seeded by
 // the state entering the segment (lane 0), then the replay.
 template <unsigned GM>
-__device__ __forceinline__ void pipe_emit_group(const PipeP& P, const PSeeds& S, const PBars& B,
-                                                const Seg& c, int a, bool steady,
-                                                const PState& y0, const PackIO& io) {
+__device__ __forceinline__ void pipe_emit_scan(const PipeP& P, const PSeeds& S, const PBars& B,
+                                               const Seg& c, int a, bool steady,
+                                               const PQLane& Lq, PC F, const PState& y0,
+                                               const PackIO& io) {
   const int lane = lane_id();
-  PQLane Lq;
-  if (steady) pq_lane<GM>(P, Lq);
-  PC F = steady ? pipe_fold<true, GM>(P, S, B, a, c.s1) : pipe_fold<false, GM>(P, S, B, a, c.s1);
   // lane 0 starts from y0 (a composite with zero multipliers); the scans
   // below read only the offsets of the lanes they combine it with
   if (lane == 0) {
@@ -1989,6 +1988,17 @@ __device__ __forceinline__ void pipe_emit_group(const PipeP& P, const PSeeds& S,
   }
   if (steady) pipe_replay<true, GM>(P, S, B, c, a, y, io);
   else pipe_replay<false, GM>(P, S, B, c, a, y, io);
+}
+
+template <unsigned GM>
+__device__ __forceinline__ void pipe_emit_group(const PipeP& P, const PSeeds& S, const PBars& B,
+                                                const Seg& c, int a, bool steady,
+                                                const PState& y0, const PackIO& io) {
+  PQLane Lq;
+  if (steady) pq_lane<GM>(P, Lq);
+  const PC F =
+      steady ? pipe_fold<true, GM>(P, S, B, a, c.s1) : pipe_fold<false, GM>(P, S, B, a, c.s1);
+  pipe_emit_scan<GM>(P, S, B, c, a, steady, Lq, F, y0, io);
 }
 
 union PipeFoldLds {
@@ -2127,6 +2137,275 @@ __global__ __launch_bounds__(kPW* kWave) void k_pipe_emit1(PipeP P, PackIO io, c
   pipe_stage<PTile>(sm[w], io, c, P);
   const int f = __builtin_isnan(c0) ? first_valid(c, io.c) : 0;
   pipe_emit_wave<GM>(P, sm[w], io, c, row, seg, f, agg, gtot, genter);
+}
+
+// ------------------------------------------------------- one-launch pass --
+// Passes of one recurrence group and no window part (the EMA, RSI, MACD
+// and ATR family calls) on few rows run as one launch.  A tile is kPW
+// consecutive segments of a row (2048 bars) and one workgroup, a wave per
+// segment: the waves fold their bars, the workgroup publishes the tile's
+// composite, one wave obtains the state entering the tile from the
+// composites of every tile before it in the row (up to kSoloM records per
+// lane, composed in the lane, then across the wave), and the waves replay
+// their segments from it.  Every wait is on a tile of a lower index in its
+// row, whose workgroup starts earlier (workgroup u holds tile u), so the
+// launch always completes; each wait is bounded in time all the same
+// (kSoloWaitTicks, then the row's stall word and NaN states).  (Several
+// groups per pass or a window part multiply the workgroups that wait at
+// once beyond what stays resident, and those passes keep the two passes.)
+//
+// Records carry a tag (row, group and the launch epoch of that row and
+// group).  The epoch word is read by each tile and bumped by the row's last
+// tile once its look-back has seen every other tile's record, which proves
+// that every tile of the row has read it; the next launch on the workspace
+// (stream-ordered after this one) so tags its records anew, with no reset
+// launch and no end-of-launch counter.  The workspace part holds only these
+// records, so no other kernel rewrites an epoch word.
+constexpr int kSoloMaxRows = 1024;
+constexpr long long kSoloMaxSegs = 1ll << 14;
+constexpr int kSoloM = 8;                          // records per lane of a look-back
+constexpr int kSoloMaxTiles = kSoloM * kWave + 1;  // tiles per row
+constexpr int kSR = 9;  // doubles per record: tag, then a group's composite fields
+constexpr unsigned long long kSoloWaitTicks = 20000000ull;  // 0.2 s of the 100 MHz clock
+constexpr int kSoloPart = 2;
+
+struct SoloWs {
+  unsigned* epoch;  // [kSoloMaxRows * 4]: row-major, one word per group
+  unsigned* stall;  // [kSoloMaxRows * 4]: epoch | 2^31 once a wait of that launch gave up
+  double* rec;      // [rows * tilesPerRow * 4 * kSR]
+};
+template <unsigned GM>
+constexpr int pipe_gi() {
+  return GM == kGE ? 0 : GM == kGR ? 1 : GM == kGM ? 2 : 3;
+}
+
+__device__ __forceinline__ unsigned long long solo_tag(int row, int gi, unsigned e) {
+  return (1ull << 62) | ((unsigned long long)(row * 4 + gi) << 32) | e;
+}
+
+// Group GM's fields of lane j's composite in every lane (j uniform); the
+// other fields are the identity.
+template <unsigned GM>
+__device__ __forceinline__ PC pc_lane_g(const PC& p, int j) {
+  PC r;
+  pc_id(r);
+  if constexpr (GM == kGE) {
+    r.Ae = bcast_d(p.Ae, j); r.be = bcast_d(p.be, j);
+  } else if constexpr (GM == kGR) {
+    r.Ag = bcast_d(p.Ag, j); r.bg = bcast_d(p.bg, j);
+    r.Al = bcast_d(p.Al, j); r.bl = bcast_d(p.bl, j);
+  } else if constexpr (GM == kGM) {
+    r.Af = bcast_d(p.Af, j); r.bf = bcast_d(p.bf, j);
+    r.As = bcast_d(p.As, j); r.bs = bcast_d(p.bs, j);
+    r.agf = bcast_d(p.agf, j); r.ags = bcast_d(p.ags, j);
+    r.agg = bcast_d(p.agg, j); r.bgg = bcast_d(p.bgg, j);
+  } else {
+    r.Aa = bcast_d(p.Aa, j); r.ba = bcast_d(p.ba, j);
+  }
+  return r;
+}
+
+// Publish group GM's fields of the (wave-uniform) composite C: lane 0
+// stores the fields, drains its stores, then stores the tag.
+template <unsigned GM>
+__device__ __forceinline__ void solo_put(double* r, const PC& C, unsigned long long tag) {
+  if (lane_id() != 0) return;
+  if constexpr (GM == kGE) {
+    st_rel_d(r + 1, C.Ae); st_rel_d(r + 2, C.be);
+  } else if constexpr (GM == kGR) {
+    st_rel_d(r + 1, C.Ag); st_rel_d(r + 2, C.bg); st_rel_d(r + 3, C.Al); st_rel_d(r + 4, C.bl);
+  } else if constexpr (GM == kGM) {
+    st_rel_d(r + 1, C.Af); st_rel_d(r + 2, C.bf); st_rel_d(r + 3, C.As); st_rel_d(r + 4, C.bs);
+    st_rel_d(r + 5, C.agf); st_rel_d(r + 6, C.ags); st_rel_d(r + 7, C.agg);
+    st_rel_d(r + 8, C.bgg);
+  } else {
+    st_rel_d(r + 1, C.Aa); st_rel_d(r + 2, C.ba);
+  }
+  vmem_order();
+  __hip_atomic_store(reinterpret_cast<unsigned long long*>(r), tag, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <unsigned GM>
+__device__ __forceinline__ void solo_get(const double* r, PC& F) {
+  if constexpr (GM == kGE) {
+    F.Ae = ld_rel_d(r + 1); F.be = ld_rel_d(r + 2);
+  } else if constexpr (GM == kGR) {
+    F.Ag = ld_rel_d(r + 1); F.bg = ld_rel_d(r + 2); F.Al = ld_rel_d(r + 3); F.bl = ld_rel_d(r + 4);
+  } else if constexpr (GM == kGM) {
+    F.Af = ld_rel_d(r + 1); F.bf = ld_rel_d(r + 2); F.As = ld_rel_d(r + 3); F.bs = ld_rel_d(r + 4);
+    F.agf = ld_rel_d(r + 5); F.ags = ld_rel_d(r + 6); F.agg = ld_rel_d(r + 7);
+    F.bgg = ld_rel_d(r + 8);
+  } else {
+    F.Aa = ld_rel_d(r + 1); F.ba = ld_rel_d(r + 2);
+  }
+}
+
+// The composite of records 0 .. cnt-1 (1 <= cnt <= kSoloM * 64) at `base`,
+// `stride` doubles apart, uniform: lane l waits for records l*m .. l*m+m-1
+// (m = ceil(cnt / 64)) to carry `tag`, composes them in order, and the
+// wave scan composes the lanes.  A wait longer than kSoloWaitTicks, or one
+// of a launch whose row already gave up, clears `ok` (the caller then
+// carries NaN).
+template <unsigned GM>
+__device__ __forceinline__ PC solo_gather(const double* base, size_t stride, int cnt,
+                                          unsigned long long tag, unsigned* stall, bool& ok) {
+  const int lane = lane_id();
+  const int m = (cnt + kWave - 1) / kWave;
+  const int k0 = min(cnt, lane * m), n = min(cnt, k0 + m) - k0;
+  // every load is issued unconditionally (lanes past the count reread
+  // record 0), so the lane's loads are in flight together
+  const double* rk[kSoloM];
+#pragma unroll
+  for (int k = 0; k < kSoloM; ++k) rk[k] = base + (size_t)(k < n ? k0 + k : 0) * stride;
+  if (ok) {
+    const unsigned stallTag = (unsigned)tag | 0x80000000u;
+    const unsigned long long t0 = wall_clock64();
+    for (int polls = 1;; ++polls) {
+      unsigned long long fl[kSoloM];
+#pragma unroll
+      for (int k = 0; k < kSoloM; ++k) fl[k] = ld_flag(rk[k]);
+      bool rdy = true;
+#pragma unroll
+      for (int k = 0; k < kSoloM; ++k) rdy = rdy & (k >= n || fl[k] == tag);
+      if (__ballot(!rdy) == 0ull) break;
+      if ((polls & 63) == 0 &&
+          (wall_clock64() - t0 > kSoloWaitTicks ||
+           __hip_atomic_load(stall, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == stallTag)) {
+        ok = false;
+        if (lane == 0)
+          __hip_atomic_store(stall, stallTag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+  }
+  vmem_order();
+  // the fields in two halves of kSoloM / 2 records (the second only when a
+  // lane holds more), each half's loads in flight together
+  PC acc;
+  pc_id(acc);
+#pragma unroll
+  for (int h = 0; h < kSoloM; h += kSoloM / 2) {
+    if (h > 0 && m <= h) break;
+    PC r[kSoloM / 2];
+#pragma unroll
+    for (int k = 0; k < kSoloM / 2; ++k) {
+      pc_id(r[k]);
+      solo_get<GM>(rk[h + k], r[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < kSoloM / 2; ++k)
+      if (ok && h + k < n) acc = pc_after(acc, r[k]);
+  }
+  pc_scan<GM>(acc);
+  return pc_lane_g<GM>(acc, kWave - 1);
+}
+
+struct SoloTileLds {
+  PLds seg[kPW];
+  double comp[kPW][16];  // the segments' composites
+  double ent[kPW][7];    // the states entering the segments
+};
+
+// Group GM of tile t of a row: one wave per segment.
+template <unsigned GM>
+__device__ __forceinline__ void solo_tile(const PipeP& P, SoloTileLds& L, const PackIO& io,
+                                          const SoloWs& W, int row, int t, int tilesPerRow) {
+  using T = PTile;
+  constexpr int gi = pipe_gi<GM>();
+  const int w = threadIdx.x / kWave, lane = lane_id();
+  const int sr = t * kPW + w;  // segment index in the row
+  const bool on = sr < P.segsPerRow;
+  int rowOf;
+  const Seg c = pipe_seg(P, row * P.segsPerRow + min(sr, P.segsPerRow - 1), rowOf);
+  unsigned* ep = W.epoch + row * 4 + gi;
+  unsigned eRaw = 0;
+  if (w == 0 && lane == 0) eRaw = __hip_atomic_load(ep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const int a = c.s0 + lane * kPK;
+  PBars B;
+  PSeeds S;
+  PQLane Lq;
+  PC F;
+  bool steady = false;
+  if (on) {
+    PLds& Ls = L.seg[w];
+    const float c0 = io.c[c.rowOff];  // in flight with the staging loads
+    stage<T>(Ls.c, io.c, c);
+    if (GM == kGA) {
+      stage<T>(Ls.h, io.h, c);
+      stage<T>(Ls.l, io.l, c);
+    }
+    wave_sync();
+    const int f = __builtin_isnan(c0) ? first_valid(c, io.c) : 0;
+    pipe_bars<T>(Ls.c, Ls.h, Ls.l, io, c, P, a, B);
+    S = pipe_seeds<T, GM>(Ls.c, Ls.h, Ls.l, io, c, P, f);
+    steady = pipe_steady<GM>(c, P, S);
+    if (steady) pq_lane<GM>(P, Lq);
+    F = steady ? pipe_fold<true, GM>(P, S, B, a, c.s1) : pipe_fold<false, GM>(P, S, B, a, c.s1);
+    PC Fs = F;
+    if (steady) {
+      pc_scan_steady<GM>(Fs, P, Lq);
+      pc_set_a(Fs, P.q[6]);
+    } else {
+      pc_scan<GM>(Fs);
+    }
+    const PC C = pc_lane_g<GM>(Fs, kWave - 1);
+    if (lane == 0) pc_store(L.comp[w], C);
+  } else if (lane == 0) {
+    PC I;
+    pc_id(I);
+    pc_store(L.comp[w], I);
+  }
+  __syncthreads();
+  if (w == 0) {
+    PC Ct = pc_load(L.comp[0]);
+#pragma unroll
+    for (int k = 1; k < kPW; ++k) Ct = pc_after(Ct, pc_load(L.comp[k]));
+    const unsigned e = (unsigned)__builtin_amdgcn_readfirstlane((int)eRaw);
+    const unsigned long long tag = solo_tag(row, gi, e);
+    constexpr size_t kStride = 4 * kSR;
+    const double* base = W.rec + ((size_t)row * tilesPerRow * 4 + gi) * kSR;
+    if (t + 1 < tilesPerRow) solo_put<GM>(const_cast<double*>(base) + (size_t)t * kStride, Ct, tag);
+    bool ok = true;
+    PC Cp;
+    pc_id(Cp);
+    if (t > 0) Cp = solo_gather<GM>(base, kStride, t, tag, W.stall + row * 4 + gi, ok);
+    // the row's last tile has seen every tile of the row publish, so all of
+    // them have read the epoch: the next launch may use the next one
+    if (t + 1 == tilesPerRow && lane == 0)
+      __hip_atomic_store(ep, e + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    PState y = pc_apply(Cp, PState{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0});
+    if (!ok) {
+      const double qn = qnan_d();
+      y = PState{qn, qn, qn, qn, qn, qn, qn};
+    }
+    if (lane == 0) {
+#pragma unroll
+      for (int k = 0; k < kPW; ++k) {
+        double* o = L.ent[k];
+        o[0] = y.e; o[1] = y.G; o[2] = y.L; o[3] = y.f; o[4] = y.s; o[5] = y.g; o[6] = y.a;
+        y = pc_apply(pc_load(L.comp[k]), y);
+      }
+    }
+  }
+  __syncthreads();
+  if (on) {
+    const double* g = L.ent[w];
+    const PState y0{g[0], g[1], g[2], g[3], g[4], g[5], g[6]};
+    pipe_emit_scan<GM>(P, S, B, c, a, steady, Lq, F, y0, io);
+  }
+}
+
+// The one-launch pass of group GM: workgroup u holds tile u (rows in order).
+template <unsigned GM>
+__global__ __launch_bounds__(kPW* kWave) void k_pipe_solo(PipeP P, PackIO io, SoloWs W,
+                                                          int tilesPerRow, int tiles) {
+  __shared__ __align__(16) SoloTileLds sm;
+  const int u = (int)blockIdx.x;
+  if (u >= tiles) return;
+  const int row = u / tilesPerRow, t = u - row * tilesPerRow;
+  solo_tile<GM>(P, sm, io, W, row, t, tilesPerRow);
 }
 
 // Whether the two passes serve this pass: dense rows of at least kPMinLen
@@ -2276,6 +2555,75 @@ static ctStatus_t pipe_qt_cached(const PipeP& P, hipStream_t s, const double** o
   return CT_STATUS_SUCCESS;
 }
 
+// The recurrence group of a pass of one group and no window part, else 0.
+static unsigned solo_group(const PipeP& P, const PackP& prm) {
+  if (prm.mask & kWinMask) return 0u;
+  const unsigned g = (P.wE ? kGE : 0u) | (P.wR ? kGR : 0u) | (P.wM ? kGM : 0u) | (P.wA ? kGA : 0u);
+  return (g == kGE || g == kGR || g == kGM || g == kGA) ? g : 0u;
+}
+
+// Whether the one-launch form serves a pass: one recurrence group, no window
+// part, at most kSoloMaxRows rows, kSoloMaxSegs segments and kSoloMaxTiles
+// tiles per row.
+static bool solo_serves(const PipeP& P, const PackP& prm) {
+  return solo_group(P, prm) != 0u && P.rows <= kSoloMaxRows &&
+         (long long)P.rows * P.segsPerRow <= kSoloMaxSegs &&
+         div_up(P.segsPerRow, kPW) <= kSoloMaxTiles;
+}
+
+// The one-launch form.  Its workspace part: the epoch and stall words of
+// every row and group and the steady multiplier table (when not cached) at
+// fixed offsets, then the tile records, so records of an earlier launch of
+// another shape never overlay a word read as a tag.
+static ctStatus_t launch_pipe_solo(PipeP& P, const PackP& prm, const PackIO& io,
+                                   hipStream_t s) {
+  const int tilesPerRow = div_up(P.segsPerRow, kPW);
+  const int tiles = tilesPerRow * P.rows;
+  constexpr size_t kHdr = 2 * (size_t)kSoloMaxRows * 4 * sizeof(unsigned);
+  constexpr size_t kQt = 2 * kQF * kWave * sizeof(double);
+  const size_t bytes = kHdr + kQt + (size_t)tiles * 4 * kSR * sizeof(double);
+  Ws ws;
+  ctStatus_t st = workspace_for(s, bytes, &ws, kSoloPart);
+  if (st != CT_STATUS_SUCCESS) return st;
+  char* base = reinterpret_cast<char*>(ws.area);
+  SoloWs W;
+  W.epoch = reinterpret_cast<unsigned*>(base);
+  W.stall = W.epoch + kSoloMaxRows * 4;
+  W.rec = reinterpret_cast<double*>(base + kHdr + kQt);
+  const double* qt = nullptr;
+  st = pipe_qt_cached(P, s, &qt);
+  if (st != CT_STATUS_SUCCESS) return st;
+  if (qt) {
+    P.qt = const_cast<double*>(qt);
+  } else {
+    P.qt = reinterpret_cast<double*>(base + kHdr);
+    KTime kt(s, "k_pipe_qt");
+    hipLaunchKernelGGL(k_pipe_qt, dim3(1), dim3(kWave), 0, s, P);
+    TCA_HIP(hipGetLastError());
+  }
+  const unsigned g = solo_group(P, prm);
+  const dim3 grid((unsigned)tiles), blk(kPW * kWave);
+  KTime kt(s, "k_pipe_solo");
+  if (g == kGE) hipLaunchKernelGGL(k_pipe_solo<kGE>, grid, blk, 0, s, P, io, W, tilesPerRow, tiles);
+  else if (g == kGR) hipLaunchKernelGGL(k_pipe_solo<kGR>, grid, blk, 0, s, P, io, W, tilesPerRow, tiles);
+  else if (g == kGM) hipLaunchKernelGGL(k_pipe_solo<kGM>, grid, blk, 0, s, P, io, W, tilesPerRow, tiles);
+  else hipLaunchKernelGGL(k_pipe_solo<kGA>, grid, blk, 0, s, P, io, W, tilesPerRow, tiles);
+  TCA_HIP(hipGetLastError());
+  return CT_STATUS_SUCCESS;
+}
+
+// Whether launch_pipe takes the one-launch form for this pass and shape.
+static bool pipe_one_launch(int len, int rows, const PackP& prm) {
+  PipeP P{};
+  P.rows = rows;
+  P.segsPerRow = div_up(len, kPSeg);
+  P.wE = (prm.mask & (1u << kOEma)) ? 1 : 0;
+  P.wR = (prm.mask & (1u << kORsi)) ? 1 : 0;
+  P.wM = (prm.mask & ((1u << kOMacd) | (1u << kOMacdSig) | (1u << kOMacdHist))) ? 1 : 0;
+  P.wA = (prm.mask & (1u << kOAtr)) ? 1 : 0;
+  return solo_serves(P, prm);
+}
+
 ctStatus_t launch_pipe(int len, int rows, hipStream_t s, const PackP& prm, const PackIO& io) {
   PipeP P{};
   P.n = len;
@@ -2310,6 +2658,7 @@ ctStatus_t launch_pipe(int len, int rows, hipStream_t s, const PackP& prm, const
   const long long segs = (long long)P.segsPerRow * rows;
   if (segs > (1ll << 30)) return fail(CT_STATUS_UNSUPPORTED, "pack: batch too large");
   pipe_steady_q(P);
+  if (solo_serves(P, prm)) return launch_pipe_solo(P, prm, io, s);
   const long long groups = (long long)div_up(P.segsPerRow, kWave) * rows;
   const size_t bytes =
       ((size_t)segs * 16 + (size_t)groups * (16 + 7) + 2 * kQF * kWave) * sizeof(double);
@@ -2515,9 +2864,7 @@ ctStatus_t pipe_family(const ctxPackParams& params, const ctxPackOutputs& output
                        const float* h, const float* l, const float* c, int len, int rows,
                        hipStream_t stream, long long fewRowsMinSegs, bool* served) {
   *served = false;
-  const long long segs = (long long)rows * div_up(len, kPSeg);
-  const bool dense = rows >= kPipeMinRows && segs >= kPipeMinSegments;
-  if (t_ragged || len < kPMinLen || !(dense || segs >= fewRowsMinSegs)) return CT_STATUS_SUCCESS;
+  if (t_ragged || len < kPMinLen) return CT_STATUS_SUCCESS;
   PackP P;
   PackIO io;
   if (pack_validate(&params, &outputs, P, io, h, l, c) != CT_STATUS_SUCCESS) {
@@ -2525,6 +2872,14 @@ ctStatus_t pipe_family(const ctxPackParams& params, const ctxPackOutputs& output
     return CT_STATUS_SUCCESS;
   }
   if (!pipe_serves(len, P)) return CT_STATUS_SUCCESS;
+  // the one-launch form, where it serves, is at least as fast as the
+  // look-back kernels at every measured shape (one row of 8192 to 10^6
+  // bars, up to 256 rows of 16384); beyond it, the two passes take dense
+  // batches and rows long enough that the look-back chains cost more
+  const long long segs = (long long)rows * div_up(len, kPSeg);
+  const bool dense = rows >= kPipeMinRows && segs >= kPipeMinSegments;
+  if (!pipe_one_launch(len, rows, P) && !(dense || segs >= fewRowsMinSegs))
+    return CT_STATUS_SUCCESS;
   *served = true;
   return launch_pipe(len, rows, stream, P, io);
 }

@@ -182,6 +182,12 @@ PIPE_OUT = ("sma", "ema", "rsi", "macd", "macd_signal", "macd_hist", "bb_upper",
 PIPE_REC = ("ema", "rsi", "macd", "macd_signal", "macd_hist", "atr")
 
 
+def _pipe_launches(rec):
+    """Fused-pass calls in a kernel-timer record: the first kernel of each,
+    k_pipe_solo (one launch, few rows) or k_pipe_fold (the two passes)."""
+    return sum(1 for k, _ in rec if k in ("k_pipe_solo", "k_pipe_fold"))
+
+
 @pytest.mark.parametrize("shape", [(1, 8192), (1, 8193), (1, 70001), (3, 20037), (40, 9000)])
 def test_pack_two_pass_rows(dev, shape):
     rows, n = shape
@@ -205,6 +211,54 @@ def test_pack_two_pass_row_carry_launch(dev):
         ta.kernel_timing(False)
     names = [k for k, _ in ta.kernel_times()]
     assert names == ["k_pipe_fold", "k_pipe_group", "k_pipe_rows", "k_pipe_emit"]
+
+
+def _bits_equal(got, want, what):
+    for k in want:
+        a, b = got[k].cpu().numpy(), want[k].cpu().numpy()
+        assert np.array_equal(np.isnan(a), np.isnan(b)), (what, k)
+        ok = ~np.isnan(a)
+        assert np.array_equal(a[ok].view(np.uint32), b[ok].view(np.uint32)), (what, k)
+
+
+@pytest.mark.parametrize("outs", [("rsi",), ("macd", "macd_signal", "macd_hist"), ("atr",),
+                                  ("ema",)])
+def test_pack_one_launch_epochs_across_shapes_and_graphs(dev, outs):
+    """Passes of one recurrence group run as one launch whose look-back
+    records carry per-row epochs that each launch's last tiles advance.
+    Calls of four shapes on one stream, a graph of one of them replayed
+    between eager calls of the others, and repeats of every shape give the
+    bits of each shape's first call, which match the oracle."""
+    import tacuda_amd as ta
+
+    print(f"pack one-launch epochs across shapes {outs}", flush=True)
+    PIPE_OUT = outs
+    shapes = [(1, 70001), (3, 20037), (5, 9000), (1, 300000)]
+    data = {sh: _data(sh[1], rows=sh[0], seed=71 + i) for i, sh in enumerate(shapes)}
+    ins = {sh: tuple(torch.from_numpy(np.ascontiguousarray(data[sh][k])).to(dev)
+                     for k in ("high", "low", "close")) for sh in shapes}
+    s = torch.cuda.Stream()
+    first = {}
+    with torch.cuda.stream(s):
+        for sh in shapes:
+            out = ta.ops.pack(*ins[sh], outputs=PIPE_OUT, **DEFAULT)
+            first[sh] = {k: v.clone() for k, v in out.items()}
+    s.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        gout = ta.ops.pack(*ins[shapes[1]], outputs=PIPE_OUT, **DEFAULT)
+    for rep in range(3):
+        with torch.cuda.stream(s):
+            g.replay()
+            got = {k: v.clone() for k, v in gout.items()}
+            others = [(sh, ta.ops.pack(*ins[sh], outputs=PIPE_OUT, **DEFAULT))
+                      for sh in (shapes[0], shapes[3], shapes[2])]
+        s.synchronize()
+        _bits_equal(got, first[shapes[1]], f"replay {rep}")
+        for sh, out in others:
+            _bits_equal(out, first[sh], f"eager {sh} after replay {rep}")
+    for sh in shapes:
+        _run(dev, data[sh], DEFAULT, outputs=PIPE_OUT)
 
 
 @pytest.mark.parametrize("pname", ["long", "ema2"])
@@ -263,7 +317,7 @@ def test_family_batches_routed_through_two_passes(dev, case, shape):
     finally:
         ta.kernel_timing(False)
     torch.cuda.synchronize()
-    folds = sum(1 for k, _ in ta.kernel_times() if k == "k_pipe_fold")
+    folds = _pipe_launches(ta.kernel_times())
     assert folds == (1 if case == "period1" else 3), folds  # MACD's periods are >= 2
     H, L, C = d["high"], d["low"], d["close"]
     for r in range(rows):
@@ -277,9 +331,8 @@ def test_family_batches_routed_through_two_passes(dev, case, shape):
 
 
 def test_single_long_row_macd_routed(dev):
-    """One row of 600,000 bars (1172 segments): MACD takes the two passes
-    (1024 segments suffice for its three-level look-back), RSI and ATR keep
-    the look-back kernels; the oracle's results either way."""
+    """One row of 600,000 bars (1172 segments): MACD, RSI and ATR each take
+    the fused pass's one-launch form; the oracle's results."""
     import tacuda_amd as ta
     from oracle import ORC
 
@@ -296,7 +349,7 @@ def test_single_long_row_macd_routed(dev):
     finally:
         ta.kernel_timing(False)
     torch.cuda.synchronize()
-    assert sum(1 for k, _ in ta.kernel_times() if k == "k_pipe_fold") == 1
+    assert [k for k, _ in ta.kernel_times()] == ["k_pipe_solo"] * 3
     w = ORC.macd(d["close"], 12, 26, 9)
     for i in range(3):
         compare(m[i].cpu().numpy(), w[i], tol=1e-5, min_exact=0.999, name=f"macd{i}")
@@ -306,8 +359,9 @@ def test_single_long_row_macd_routed(dev):
 
 
 def test_single_long_row_ema_rsi_routed(dev):
-    """One row of 2^20 bars (2048 segments): EMA and RSI take the two passes,
-    ATR keeps the look-back kernels; the oracle's results either way."""
+    """One row of 2^20 bars (2048 segments, 512 tiles: the one-launch form's
+    longest row but one tile): EMA, RSI and ATR each take the one-launch
+    form; the oracle's results."""
     import tacuda_amd as ta
     from oracle import ORC
 
@@ -324,7 +378,7 @@ def test_single_long_row_ema_rsi_routed(dev):
     finally:
         ta.kernel_timing(False)
     torch.cuda.synchronize()
-    assert sum(1 for k, _ in ta.kernel_times() if k == "k_pipe_fold") == 2
+    assert [k for k, _ in ta.kernel_times()] == ["k_pipe_solo"] * 3
     compare(ema.cpu().numpy(), ORC.ema(d["close"], 20), tol=1e-5, min_exact=0.999, name="ema")
     compare(rsi.cpu().numpy(), ORC.rsi(d["close"], 14), tol=1e-5, min_exact=0.999, name="rsi")
     compare(atr.cpu().numpy(), ORC.atr(d["high"], d["low"], d["close"], 14), tol=1e-5,

@@ -44,6 +44,26 @@ def test_two_pass_kernels_are_timed(dev):
     assert ta.kernel_times() == []
 
 
+def test_one_launch_pass_is_timed(dev):
+    import tacuda_amd as ta
+
+    print("timing one-launch pass", flush=True)
+    h, l, c = _pipeline(dev)
+    ta.ops.rsi(c, 14)
+    torch.cuda.synchronize()
+    ta.kernel_times()
+    ta.kernel_timing(True)
+    try:
+        ta.ops.pack(h, l, c, outputs=("rsi",))
+        ta.ops.pack(h, l, c, outputs=("macd", "macd_hist"))
+    finally:
+        ta.kernel_timing(False)
+    rec = ta.kernel_times()
+    # one recurrence group and no window part: one launch each
+    assert [k for k, _ in rec] == ["k_pipe_solo", "k_pipe_solo"]
+    assert all(ms > 0.0 for _, ms in rec)
+
+
 def test_scan_and_candles_are_timed(dev):
     import tacuda_amd as ta
 
