@@ -229,14 +229,16 @@ CTAPI_EXPORT ctStatus_t ctx_plan_destroy(ctxPlan plan);
  * parameter and keep the TA-Lib seed of `period` samples (EMA: mean of bars
  * 0..p-1 at bar p-1; RSI: (sum of the first p gains/losses)/p at bar p);
  * alpha = 2/(p+1) (EMA) or 1/p (RSI) gives the TA-Lib indicator.  The
- * forward writes a checkpoint of the recurrence state at 64 points per row
- * (EMA: 64 doubles per row, RSI: 128); the adjoint recomputes the rest.
+ * forward writes a checkpoint of the recurrence state at
+ * ctx_alpha_ckpt_per_row() points per row (EMA: that many doubles per row,
+ * RSI: twice as many); the adjoint recomputes the rest.
  * Adjoints write the gradient w.r.t. the prices (gx) and, per row, the
  * gradient w.r.t. alpha (galpha, doubles; sum over rows for a shared alpha).
  * MACD (ta_MACD.c, EMA type) and BBANDS (SMA middle) adjoints take the
  * output gradients; their forwards are ct_macd_batch_d / ct_bbands_batch_d.
  * Inputs must be finite.
  * ------------------------------------------------------------------------- */
+CTAPI_EXPORT int ctx_alpha_ckpt_per_row(void);
 CTAPI_EXPORT ctStatus_t ctx_ema_alpha_fwd_d(const float* x, float* y, double* ckpt,
                                             int seriesLen, int numSeries, int period,
                                             double alpha, hipStream_t stream);

@@ -264,6 +264,7 @@ for _n in ("ema", "rsi"):
     _sig(f"ctx_{_n}_alpha_bwd_d", P, P, P, P, P, c_int, c_int, c_int, _D)
     _sig(f"ctx_{_n}_alpha_fwd_dp", P, P, P, c_int, c_int, c_int, P)
     _sig(f"ctx_{_n}_alpha_bwd_dp", P, P, P, P, P, c_int, c_int, c_int, P)
+_sig("ctx_alpha_ckpt_per_row", stream=False)
 _sig("ctx_macd_bwd_d", P, P, P, P, c_int, c_int, c_int, c_int, c_int)
 _sig("ctx_bbands_bwd_d", P, P, P, P, P, c_int, c_int, c_int, c_float, c_float)
 

@@ -8,8 +8,8 @@ ctx_bbands_bwd_d).  Inputs: float32 CUDA tensors of shape [series, bars]
   macd(x, fast, slow, signal)  TA-Lib MACD (line, signal, histogram)
   bbands(x, period, up, dn)    Bollinger bands (upper, middle, lower)
 
-Backward passes save only block-boundary carries of the recurrences
-(64 per series) and recompute the rest on the GPU.  The adjoint kernels
+Backward passes save only chunk-boundary carries of the recurrences
+(ctx_alpha_ckpt_per_row() per series) and recompute the rest on the GPU.  The adjoint kernels
 never read the incoming gradient at warm-up bars (where the outputs are
 NaN), so a loss that leaves NaN gradients there needs no masking.
 """
@@ -22,6 +22,18 @@ from . import call
 
 def _stream():
     return torch.cuda.current_stream().cuda_stream
+
+
+_CKPT = []
+
+
+def _ckpt_per_row():
+    """Checkpointed states per row and recurrence (one per thread chunk)."""
+    if not _CKPT:
+        from . import lib
+
+        _CKPT.append(int(lib().ctx_alpha_ckpt_per_row()))
+    return _CKPT[0]
 
 
 def _as2d(x):
@@ -55,7 +67,8 @@ class _AlphaRecurrence(torch.autograd.Function):
         n, rows = _shape(x2)
         a = _alpha_arg(alpha, x.device)
         y = torch.empty_like(x2)
-        ck = torch.empty(rows * 64 * cls.ck_per_lane, dtype=torch.float64, device=x.device)
+        ck = torch.empty(rows * _ckpt_per_row() * cls.ck_per_lane, dtype=torch.float64,
+                         device=x.device)
         if torch.is_tensor(a):
             call(f"ctx_{cls.fam}_alpha_fwd_dp", x2.data_ptr(), y.data_ptr(), ck.data_ptr(), n,
                  rows, int(period), a.data_ptr(), _stream())

@@ -1,15 +1,18 @@
 // Differentiable indicators (no counterpart in the reference): forward and
 // adjoint (reverse-mode) kernels behind torch.autograd.Function.
 //
-// One wave per row.  Lane l owns the contiguous chunk [l*C, l*C + C) of the
-// row (C = ceil(n/64)).  A recurrence y_t = beta*y_{t-1} + u_t is folded per
-// chunk, composed across lanes by an offset scan with the known multiplier
-// beta^C, and replayed.  The adjoint of such a recurrence is the same
-// recurrence run backwards, lambda_t = g_t + beta*lambda_{t+1}: the chunk
-// folds from its end and the lane scan runs from lane 63 down.  Forward
-// passes save only the state entering each chunk (64 doubles per row and
-// recurrence); adjoint passes recompute each chunk's forward values from it
-// into LDS (as float32) before walking the chunk backwards.
+// One workgroup of kAW waves per row.  Thread i owns the contiguous chunk
+// [i*C, i*C + C) of the row (C = ceil(n / (64 kAW))).  A recurrence
+// y_t = beta*y_{t-1} + u_t is folded per chunk, composed across the lanes of
+// a wave by an offset scan with the known multiplier beta^C, across the
+// waves through LDS with beta^(64 C), and replayed.  The adjoint of such a
+// recurrence is the same recurrence run backwards,
+// lambda_t = g_t + beta*lambda_{t+1}: the chunk folds from its end and the
+// scans run from the row's last thread down.  Forward passes save only the
+// state entering each chunk (64 kAW doubles per row and recurrence);
+// adjoint passes recompute each chunk's forward values from it (into LDS
+// for EMA and MACD, block by block for RSI) before walking the chunk
+// backwards.
 //
 // Operators and their seeds follow the TA-Lib definitions of the families
 // (SMA-mean seed of the first p samples), with the smoothing weight alpha as
@@ -23,6 +26,7 @@
 // Inputs must be finite (a NaN gives NaN gradients).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <utility>
 
 #include "../../include/tacuda_amd.h"
@@ -32,62 +36,145 @@
 namespace tca {
 namespace {
 
-constexpr int kMaxChunk = 256;  // C <= 256: rows up to 16384 bars
+constexpr int kAW = 8;            // waves per row
+constexpr int kAT = kAW * kWave;  // threads per row (one workgroup)
+constexpr int kMaxChunk = 32;     // C <= 32: rows up to 16384 bars
 
 struct Chunk {
-  int lane, C, a, e;  // chunk [a, e) of a row of n bars
+  int tid, lane, wave, C, a, e;  // chunk [a, e) of a row of n bars
 };
 __device__ __forceinline__ Chunk chunk_of(int n) {
   Chunk k;
+  k.tid = (int)threadIdx.x;
   k.lane = lane_id();
-  k.C = (n + kWave - 1) / kWave;
-  k.a = min(n, k.lane * k.C);
+  k.wave = k.tid / kWave;
+  k.C = (n + kAT - 1) / kAT;
+  k.a = min(n, k.tid * k.C);
   k.e = min(n, k.a + k.C);
   return k;
 }
 
-// Per-lane float array in LDS, padded so lanes hit distinct banks.
+// Per-thread float array in LDS, padded so neighbouring threads hit
+// distinct banks.
 struct LdsCol {
   float* base;
   int stride;
-  __device__ float& at(int lane, int i) { return base[lane * stride + i]; }
+  __device__ float& at(int tid, int i) { return base[tid * stride + i]; }
 };
 
-// Forward offset scan: returns the state entering this lane's chunk.
-__device__ __forceinline__ double carry_fwd(double b, double beta, int C) {
-  double bb[1] = {b}, A[1] = {pow_n(beta, C)};
-  const double be[1] = {beta};
-  bscan_incl<1, true>(bb, A, be);
-  return dpp_d<kWaveShr1, 0xf, 0xf>(0.0, bb[0]);
+// The row's cross-wave exchange: one slot per wave.
+struct Xw {
+  double v[kAW][2];
+};
+
+// Inclusive sum scan over the wave (lane 0 earliest).
+template <int R>
+__device__ __forceinline__ void wave_iscan_add(double (&v)[R]) {
+#define TCA_ADD_STEP(CTRL, RM)                                         \
+  _Pragma("unroll") for (int r = 0; r < R; ++r) v[r] += dpp_d<CTRL, RM, 0xf>(0.0, v[r]);
+  TCA_ADD_STEP(kRowShr1, 0xf)
+  TCA_ADD_STEP(kRowShr2, 0xf)
+  TCA_ADD_STEP(kRowShr4, 0xf)
+  TCA_ADD_STEP(kRowShr8, 0xf)
+  TCA_ADD_STEP(kRowBcast15, 0xa)
+  TCA_ADD_STEP(kRowBcast31, 0xc)
+#undef TCA_ADD_STEP
 }
 
-// Reverse offset scan: `b` is the chunk's adjoint folded from its end (the
-// adjoint at its first bar given zero beyond it); returns the adjoint just
-// past the chunk's end.
-__device__ __forceinline__ double carry_bwd(double b, double beta, int C) {
-  double A = pow_clamp(pow_n(beta, C), beta);
-  const int lane = lane_id();
+// Sum over the row's workgroup; every thread receives it.  Every thread of
+// the workgroup calls it (it holds barriers).
+template <int R>
+__device__ __forceinline__ void rsum(double (&v)[R], Xw& x) {
+  wave_total<R>(v);
+  const int w = (int)threadIdx.x / kWave;
+  if (lane_id() == 0) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) x.v[w][r] = v[r];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    double s = 0.0;
+    for (int k = 0; k < kAW; ++k) s += x.v[k][r];
+    v[r] = s;
+  }
+  __syncthreads();
+}
+
+// Forward offset scan: `b` holds each chunk's fold from a zero state;
+// returns in `b` the state entering the thread's chunk.  Bars before a
+// recurrence's seed contribute exactly 0, so every chunk's multiplier is
+// beta^C (powers that underflow are clamped as pow_n does).
+template <int R>
+__device__ __forceinline__ void carry_fwd(double (&b)[R], const double (&beta)[R], int C,
+                                          Xw& x) {
+  const int lane = lane_id(), w = (int)threadIdx.x / kWave;
+  double A[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) A[r] = pow_n(beta[r], C);
+  bscan_incl<R, true>(b, A, beta);
+  double ex[R], Al[R], Pw[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (lane == kWave - 1) x.v[w][r] = b[r];
+    ex[r] = dpp_d<kWaveShr1, 0xf, 0xf>(0.0, b[r]);  // lanes before this one
+    Al[r] = dpp_d<kWaveShr1, 0xf, 0xf>(1.0, A[r]);  // beta^(C lane)
+    Pw[r] = bcast_d(A[r], kWave - 1);               // beta^(64 C)
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    double e = 0.0;  // state entering this wave
+    for (int k = 0; k < w; ++k) e = fma(Pw[r], e, x.v[k][r]);
+    b[r] = fma(Al[r], e, ex[r]);
+  }
+  __syncthreads();
+}
+
+// Reverse offset scan: `b` is each chunk's adjoint folded from its end (the
+// adjoint at its first bar given zero beyond it); returns in `b` the
+// adjoint just past the thread's chunk.
+template <int R>
+__device__ __forceinline__ void carry_bwd(double (&b)[R], const double (&beta)[R], int C,
+                                          Xw& x) {
+  const int lane = lane_id(), w = (int)threadIdx.x / kWave;
+  double A[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) A[r] = pow_n(beta[r], C);
 #pragma unroll
   for (int d = 1; d < kWave; d <<= 1) {
-    const double ob = __shfl_down(b, d);
-    const double oA = __shfl_down(A, d);
-    if (lane + d < kWave) {
-      b = fma(A, ob, b);
-      A = pow_clamp(A * oA, beta);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const double ob = __shfl_down(b[r], d);
+      const double oA = __shfl_down(A[r], d);
+      if (lane + d < kWave) {
+        b[r] = fma(A[r], ob, b[r]);
+        A[r] = pow_clamp(A[r] * oA, beta[r]);
+      }
     }
   }
-  const double nb = __shfl_down(b, 1);
-  return lane + 1 < kWave ? nb : 0.0;
+  // b: adjoint at the chunk's first bar given zero past the wave; A:
+  // beta^(C (64 - lane))
+  double ex[R], An[R], Pw[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (lane == 0) x.v[w][r] = b[r];
+    const double nb = __shfl_down(b[r], 1), nA = __shfl_down(A[r], 1);
+    ex[r] = lane + 1 < kWave ? nb : 0.0;
+    An[r] = lane + 1 < kWave ? nA : 1.0;
+    Pw[r] = bcast_d(A[r], 0);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    double e = 0.0;  // adjoint just past this wave
+    for (int k = kAW - 1; k > w; --k) e = fma(Pw[r], e, x.v[k][r]);
+    b[r] = fma(An[r], e, ex[r]);
+  }
+  __syncthreads();
 }
 
-// Sum over the wave (every lane receives it).
-__device__ __forceinline__ double wsum(double v) {
-  double t[1] = {v};
-  wave_total<1>(t);
-  return t[0];
-}
-
-// Lanes walk their chunks in blocks of kB bars: every load of a block is
+// Threads walk their chunks in blocks of kB bars: every load of a block is
 // issued before the block's arithmetic (one memory round trip per kB bars,
 // not per bar), as two float4 accesses when the block lies inside [lo, hi)
 // and is 16-byte aligned.  Bars outside [lo, hi) read as 0 and are not
@@ -132,7 +219,7 @@ __device__ __forceinline__ void st8(float* r, int t, int lo, int hi, const float
   }
 }
 
-// Sum of bars [lo, hi) of a lane's chunk, in bar order.
+// Sum of bars [lo, hi) of a thread's chunk, in bar order.
 __device__ __forceinline__ double chunk_sum(const float* r, int lo, int hi) {
   double part = 0.0;
   for (int t0 = lo; t0 < hi; t0 += kB) {
@@ -146,9 +233,10 @@ __device__ __forceinline__ double chunk_sum(const float* r, int lo, int hi) {
 }
 
 // =================================================================== EMA ==
-__global__ __launch_bounds__(64) void k_ema_fwd(const float* x, float* y, double* ck, int rows,
-                                                int n, int p, double alphaArg,
-                                                const double* alphaPtr) {
+__global__ __launch_bounds__(kAT) void k_ema_fwd(const float* x, float* y, double* ck, int rows,
+                                                 int n, int p, double alphaArg,
+                                                 const double* alphaPtr) {
+  __shared__ Xw xw;
   const double alpha = alphaPtr ? *alphaPtr : alphaArg;
   const int row = blockIdx.x;
   if (row >= rows) return;
@@ -157,8 +245,10 @@ __global__ __launch_bounds__(64) void k_ema_fwd(const float* x, float* y, double
   float* yr = y + (size_t)row * n;
   const double beta = 1.0 - alpha;
   const int s = p - 1;
-  const double seed = wsum(chunk_sum(xr, k.a, min(k.e, s + 1))) / p;
-  double b = 0.0;
+  double sv[1] = {chunk_sum(xr, k.a, min(k.e, s + 1))};
+  rsum<1>(sv, xw);
+  const double seed = sv[0] / p;
+  double b[1] = {0.0};
   for (int t0 = max(k.a, s); t0 < k.e; t0 += kB) {
     float v[kB];
     ld8(xr, t0, k.a, k.e, v);
@@ -166,11 +256,13 @@ __global__ __launch_bounds__(64) void k_ema_fwd(const float* x, float* y, double
     for (int i = 0; i < kB; ++i) {
       const int t = t0 + i;
       if (t >= k.e) break;
-      b = (t == s) ? seed : fma(beta, b, alpha * (double)v[i]);
+      b[0] = (t == s) ? seed : fma(beta, b[0], alpha * (double)v[i]);
     }
   }
-  double w = carry_fwd(b, beta, k.C);
-  ck[(size_t)row * kWave + k.lane] = w;
+  const double be[1] = {beta};
+  carry_fwd<1>(b, be, k.C, xw);
+  double w = b[0];
+  ck[(size_t)row * kAT + k.tid] = w;
   for (int t0 = k.a; t0 < k.e; t0 += kB) {
     float v[kB], o[kB];
     ld8(xr, t0, k.a, k.e, v);
@@ -187,11 +279,12 @@ __global__ __launch_bounds__(64) void k_ema_fwd(const float* x, float* y, double
   }
 }
 
-__global__ __launch_bounds__(64) void k_ema_bwd(const float* x, const float* g, const double* ck,
-                                                float* dx, double* dal, int rows, int n, int p,
-                                                double alphaArg, const double* alphaPtr) {
-  const double alpha = alphaPtr ? *alphaPtr : alphaArg;
+__global__ __launch_bounds__(kAT) void k_ema_bwd(const float* x, const float* g, const double* ck,
+                                                 float* dx, double* dal, int rows, int n, int p,
+                                                 double alphaArg, const double* alphaPtr) {
+  __shared__ Xw xw;
   extern __shared__ float s_col[];
+  const double alpha = alphaPtr ? *alphaPtr : alphaArg;
   const int row = blockIdx.x;
   if (row >= rows) return;
   const Chunk k = chunk_of(n);
@@ -202,8 +295,10 @@ __global__ __launch_bounds__(64) void k_ema_bwd(const float* x, const float* g, 
   const double beta = 1.0 - alpha;
   const int s = p - 1;
   // forward recompute: residual x_t - y_{t-1} of every bar after the seed
-  double w = ck[(size_t)row * kWave + k.lane];
-  const double seed = wsum(chunk_sum(xr, k.a, min(k.e, s + 1))) / p;
+  double w = ck[(size_t)row * kAT + k.tid];
+  double sv[1] = {chunk_sum(xr, k.a, min(k.e, s + 1))};
+  rsum<1>(sv, xw);
+  const double seed = sv[0] / p;
   for (int t0 = max(k.a, s); t0 < k.e; t0 += kB) {
     float v[kB];
     ld8(xr, t0, k.a, k.e, v);
@@ -216,24 +311,26 @@ __global__ __launch_bounds__(64) void k_ema_bwd(const float* x, const float* g, 
         continue;
       }
       const double xv = (double)v[i];
-      r.at(k.lane, t - k.a) = (float)(xv - w);
+      r.at(k.tid, t - k.a) = (float)(xv - w);
       w = fma(beta, w, alpha * xv);
     }
   }
   // adjoint: lambda_t = g_t + beta*lambda_{t+1} for t >= s
   const int lo = max(k.a, s);
-  double b = 0.0;
+  double b[1] = {0.0};
   for (int t1 = k.e - 1; t1 >= lo; t1 -= kB) {
     float v[kB];
     ld8(gr, t1 - (kB - 1), lo, k.e, v);
 #pragma unroll
     for (int i = 0; i < kB; ++i) {
       if (t1 - i < lo) break;
-      b = fma(beta, b, (double)v[kB - 1 - i]);
+      b[0] = fma(beta, b[0], (double)v[kB - 1 - i]);
     }
   }
-  double lam = carry_bwd(b, beta, k.C);
-  double da = 0.0, lamS = 0.0;
+  const double be[1] = {beta};
+  carry_bwd<1>(b, be, k.C, xw);
+  double lam = b[0];
+  double red[2] = {0.0, 0.0};  // d alpha, lambda at the seed bar
   for (int t1 = k.e - 1; t1 >= k.a; t1 -= kB) {
     float v[kB], o[kB];
     ld8(gr, t1 - (kB - 1), k.a, k.e, v);
@@ -244,24 +341,23 @@ __global__ __launch_bounds__(64) void k_ema_bwd(const float* x, const float* g, 
       if (t < s || t < k.a) continue;
       lam = fma(beta, lam, (double)v[kB - 1 - i]);
       if (t == s) {
-        lamS = lam;
+        red[1] = lam;
       } else {
         o[kB - 1 - i] = (float)(alpha * lam);
-        da += lam * (double)r.at(k.lane, t - k.a);
+        red[0] += lam * (double)r.at(k.tid, t - k.a);
       }
     }
     st8(dr, t1 - (kB - 1), k.a, k.e, o);
   }
+  rsum<2>(red, xw);
   // the seed is the mean of bars 0..s
-  lamS = wsum(lamS);
-  for (int t = k.a; t < k.e && t <= s; ++t) dr[t] = (float)(lamS / p);
-  da = wsum(da);
-  if (k.lane == 0) dal[row] = da;
+  for (int t = k.a; t < k.e && t <= s; ++t) dr[t] = (float)(red[1] / p);
+  if (k.tid == 0) dal[row] = red[0];
 }
 
 // =================================================================== RSI ==
-// Gains and losses of the bars [lo, hi) of a lane's chunk (bar t uses x_{t-1}),
-// in bar order: f(t, gain, loss, d) for every t >= 1.
+// Gains and losses of the bars [lo, hi) of a thread's chunk (bar t uses
+// x_{t-1}), in bar order: f(t, gain, loss, d) for every t >= 1.
 template <class F>
 __device__ __forceinline__ void walk_moves(const float* xr, int lo, int hi, F&& f) {
   float prev = lo >= 1 ? xr[lo - 1] : 0.0f;
@@ -282,9 +378,23 @@ __device__ __forceinline__ void walk_moves(const float* xr, int lo, int hi, F&& 
   }
 }
 
-__global__ __launch_bounds__(64) void k_rsi_fwd(const float* x, float* y, double* ck, int rows,
-                                                int n, int p, double alphaArg,
-                                                const double* alphaPtr) {
+// The seed averages (sum of the first p gains / losses) * (1/p).
+__device__ __forceinline__ void rsi_seeds(const float* xr, const Chunk& k, int s, double inv,
+                                          Xw& xw, double& sG, double& sL) {
+  double ps[2] = {0.0, 0.0};
+  walk_moves(xr, k.a, min(k.e, s + 1), [&](int, double gd, double ld, double) {
+    ps[0] += gd;
+    ps[1] += ld;
+  });
+  rsum<2>(ps, xw);
+  sG = ps[0] * inv;
+  sL = ps[1] * inv;
+}
+
+__global__ __launch_bounds__(kAT) void k_rsi_fwd(const float* x, float* y, double* ck, int rows,
+                                                 int n, int p, double alphaArg,
+                                                 const double* alphaPtr) {
+  __shared__ Xw xw;
   const double alpha = alphaPtr ? *alphaPtr : alphaArg;
   const int row = blockIdx.x;
   if (row >= rows) return;
@@ -293,25 +403,23 @@ __global__ __launch_bounds__(64) void k_rsi_fwd(const float* x, float* y, double
   float* yr = y + (size_t)row * n;
   const double beta = 1.0 - alpha, inv = 1.0 / (double)p;
   const int s = p;
-  double pg = 0.0, pl = 0.0;
-  walk_moves(xr, k.a, min(k.e, s + 1), [&](int, double gd, double ld, double) {
-    pg += gd;
-    pl += ld;
-  });
-  const double sG = wsum(pg) * inv, sL = wsum(pl) * inv;
-  double bg = 0.0, bl = 0.0;
+  double sG, sL;
+  rsi_seeds(xr, k, s, inv, xw, sG, sL);
+  double b[2] = {0.0, 0.0};
   walk_moves(xr, k.a, k.e, [&](int t, double gd, double ld, double) {
     if (t == s) {
-      bg = sG;
-      bl = sL;
+      b[0] = sG;
+      b[1] = sL;
     } else if (t > s) {
-      bg = fma(beta, bg, alpha * gd);
-      bl = fma(beta, bl, alpha * ld);
+      b[0] = fma(beta, b[0], alpha * gd);
+      b[1] = fma(beta, b[1], alpha * ld);
     }
   });
-  double G = carry_fwd(bg, beta, k.C), L = carry_fwd(bl, beta, k.C);
-  ck[((size_t)row * kWave + k.lane) * 2] = G;
-  ck[((size_t)row * kWave + k.lane) * 2 + 1] = L;
+  const double be[2] = {beta, beta};
+  carry_fwd<2>(b, be, k.C, xw);
+  double G = b[0], L = b[1];
+  ck[((size_t)row * kAT + k.tid) * 2] = G;
+  ck[((size_t)row * kAT + k.tid) * 2 + 1] = L;
   float prev = k.a >= 1 ? xr[k.a - 1] : 0.0f;
   for (int t0 = k.a; t0 < k.e; t0 += kB) {
     float v[kB], o[kB];
@@ -353,11 +461,19 @@ __device__ __forceinline__ void rsi_step(int t, int s, double sG, double sL, dou
   }
 }
 
+// 1/v: the hardware estimate and two Newton steps.
+__device__ __forceinline__ double rcp_d(double v) {
+  double y = __builtin_amdgcn_rcp(v);
+#pragma unroll
+  for (int k = 0; k < 2; ++k) y = fma(y, fma(-v, y, 1.0), y);
+  return y;
+}
+
 // d(100 G/(G+L)) / dG and / dL at the state (G, L).
 __device__ __forceinline__ void rsi_coef(double G, double L, double& cg, double& cl) {
   const double sum = G + L;
   if (sum > 0.0) {
-    const double q = 100.0 / (sum * sum);
+    const double r = rcp_d(sum), q = 100.0 * (r * r);
     cg = q * L;
     cl = -q * G;
   } else {
@@ -365,40 +481,38 @@ __device__ __forceinline__ void rsi_coef(double G, double L, double& cg, double&
   }
 }
 
-// The adjoint walks each lane's chunk backwards in kB-bar blocks.  The
+// The adjoint walks each thread's chunk backwards in kB-bar blocks.  The
 // forward states inside a block are recomputed from the state entering the
-// block, kept in LDS for every block (2 doubles per lane and block), so a
-// row needs 16 KB of LDS at 8192 bars instead of its whole state history.
-__global__ __launch_bounds__(64) void k_rsi_bwd(const float* x, const float* g, const double* ck,
-                                                float* dx, double* dal, int rows, int n, int p,
-                                                double alphaArg, const double* alphaPtr) {
-  const double alpha = alphaPtr ? *alphaPtr : alphaArg;
+// block, kept in LDS for every block (2 doubles per thread and block), so a
+// row needs 32 KB of LDS at 8192 bars instead of its whole state history.
+__global__ __launch_bounds__(kAT) void k_rsi_bwd(const float* x, const float* g, const double* ck,
+                                                 float* dx, double* dal, int rows, int n, int p,
+                                                 double alphaArg, const double* alphaPtr) {
+  __shared__ Xw xw;
+  __shared__ float s_first[kAT];
   extern __shared__ double s_ck[];
+  const double alpha = alphaPtr ? *alphaPtr : alphaArg;
   const int row = blockIdx.x;
   if (row >= rows) return;
   const Chunk k = chunk_of(n);
   const int nb = (k.C + kB - 1) / kB;
-  double* ckG = s_ck;               // [block][lane]
-  double* ckL = s_ck + nb * kWave;  // [block][lane]
+  double* ckG = s_ck;             // [block][thread]
+  double* ckL = s_ck + nb * kAT;  // [block][thread]
   const float* xr = x + (size_t)row * n;
   const float* gr = g + (size_t)row * n;
   float* dr = dx + (size_t)row * n;
   const double beta = 1.0 - alpha, inv = 1.0 / (double)p;
   const int s = p;
-  double pg = 0.0, pl = 0.0;
-  walk_moves(xr, k.a, min(k.e, s + 1), [&](int, double gd, double ld, double) {
-    pg += gd;
-    pl += ld;
-  });
-  const double sG = wsum(pg) * inv, sL = wsum(pl) * inv;
+  double sG, sL;
+  rsi_seeds(xr, k, s, inv, xw, sG, sL);
   // forward: the state entering every block of the chunk
   {
-    double G = ck[((size_t)row * kWave + k.lane) * 2], L = ck[((size_t)row * kWave + k.lane) * 2 + 1];
+    double G = ck[((size_t)row * kAT + k.tid) * 2], L = ck[((size_t)row * kAT + k.tid) * 2 + 1];
     float prev = k.a >= 1 ? xr[k.a - 1] : 0.0f;
     for (int j = 0; j < nb; ++j) {
       const int t0 = k.a + j * kB;
-      ckG[j * kWave + k.lane] = G;
-      ckL[j * kWave + k.lane] = L;
+      ckG[j * kAT + k.tid] = G;
+      ckL[j * kAT + k.tid] = L;
       if (t0 >= k.e) continue;
       float v[kB];
       ld8(xr, t0, k.a, k.e, v);
@@ -422,7 +536,7 @@ __global__ __launch_bounds__(64) void k_rsi_bwd(const float* x, const float* g, 
     const int t0 = k.a + j * kB;
     ld8(xr, t0, 0, n, xv);
     float prev = t0 >= 1 ? xr[t0 - 1] : 0.0f;
-    double G = ckG[j * kWave + k.lane], L = ckL[j * kWave + k.lane];
+    double G = ckG[j * kAT + k.tid], L = ckL[j * kAT + k.tid];
 #pragma unroll
     for (int i = 0; i < kB; ++i) {
       const int t = t0 + i;
@@ -439,7 +553,7 @@ __global__ __launch_bounds__(64) void k_rsi_bwd(const float* x, const float* g, 
   };
   // lambdaG_t = cG_t g_t + beta*lambdaG_{t+1} (t >= s), same for L
   const int lo = max(k.a, s);
-  double bg = 0.0, bl = 0.0;
+  double bq[2] = {0.0, 0.0};
   for (int j = nb - 1; j >= 0; --j) {
     const int t0 = k.a + j * kB;
     if (t0 >= k.e || t0 + kB <= lo) continue;
@@ -453,14 +567,16 @@ __global__ __launch_bounds__(64) void k_rsi_bwd(const float* x, const float* g, 
       if (t < lo || t >= k.e) continue;
       double cg, cl;
       rsi_coef(Ga[i], La[i], cg, cl);
-      bg = fma(beta, bg, cg * (double)gv[i]);
-      bl = fma(beta, bl, cl * (double)gv[i]);
+      bq[0] = fma(beta, bq[0], cg * (double)gv[i]);
+      bq[1] = fma(beta, bq[1], cl * (double)gv[i]);
     }
   }
-  double lg = carry_bwd(bg, beta, k.C), lq = carry_bwd(bl, beta, k.C);
+  const double be[2] = {beta, beta};
+  carry_bwd<2>(bq, be, k.C, xw);
+  double lg = bq[0], lq = bq[1];
   // d(gain)_t and d(loss)_t adjoints, then through d_t = x_t - x_{t-1}.
-  // dd_t (the adjoint of d_t) goes to this lane's own part of dx first.
-  double da = 0.0, lgS = 0.0, lqS = 0.0;
+  // dd_t (the adjoint of d_t) goes to this thread's own part of dx first.
+  double da[1] = {0.0}, lS[2] = {0.0, 0.0};
   for (int j = nb - 1; j >= 0; --j) {
     const int t0 = k.a + j * kB;
     if (t0 >= k.e) continue;
@@ -468,7 +584,7 @@ __global__ __launch_bounds__(64) void k_rsi_bwd(const float* x, const float* g, 
     double Ga[kB], La[kB], gdv[kB], dv[kB];
     block(j, xv, Ga, La, gdv, dv);
     ld8(gr, t0, lo, k.e, gv);
-    const double G0 = ckG[j * kWave + k.lane], L0 = ckL[j * kWave + k.lane];
+    const double G0 = ckG[j * kAT + k.tid], L0 = ckL[j * kAT + k.tid];
 #pragma unroll
     for (int i = kB - 1; i >= 0; --i) {
       const int t = t0 + i;
@@ -480,30 +596,30 @@ __global__ __launch_bounds__(64) void k_rsi_bwd(const float* x, const float* g, 
         lg = fma(beta, lg, cg * gvv);
         lq = fma(beta, lq, cl * gvv);
         if (t == s) {
-          lgS = lg;
-          lqS = lq;
+          lS[0] = lg;
+          lS[1] = lq;
         } else {
           const double d = dv[i], gd = gdv[i], ld = gd - d;
           const double Gp = i > 0 ? Ga[i - 1] : G0, Lp = i > 0 ? La[i - 1] : L0;
           ddt = (d > 0.0 ? alpha * lg : 0.0) - (d < 0.0 ? alpha * lq : 0.0);
-          da += lg * (gd - Gp) + lq * (ld - Lp);
+          da[0] += lg * (gd - Gp) + lq * (ld - Lp);
         }
       }
       o[i] = (float)ddt;
     }
     st8(dr, t0, k.a, k.e, o);
   }
-  lgS = wsum(lgS);
-  lqS = wsum(lqS);
+  rsum<2>(lS, xw);
   // the seeds are (sum of the first p gains / losses) * (1/p)
   for (int t = max(k.a, 1); t < k.e && t <= s; ++t) {
     const double d = (double)xr[t] - (double)xr[t - 1];
-    dr[t] = (float)((d > 0.0 ? lgS * inv : 0.0) - (d < 0.0 ? lqS * inv : 0.0));
+    dr[t] = (float)((d > 0.0 ? lS[0] * inv : 0.0) - (d < 0.0 ? lS[1] * inv : 0.0));
   }
-  // dx_t = dd_t - dd_{t+1}; the next chunk's first dd comes from lane+1
+  // dx_t = dd_t - dd_{t+1}; the next chunk's first dd comes from thread+1
   const float first = k.a < k.e ? dr[k.a] : 0.0f;
-  const float nextFirst = __shfl_down(first, 1);
-  const float tail = (k.lane + 1 < kWave && k.e < n) ? nextFirst : 0.0f;
+  s_first[k.tid] = first;
+  __syncthreads();
+  const float tail = (k.tid + 1 < kAT && k.e < n) ? s_first[k.tid + 1] : 0.0f;
   float cur = first;
   for (int t0 = k.a; t0 < k.e; t0 += kB) {
     float nx[kB], o[kB];
@@ -517,16 +633,17 @@ __global__ __launch_bounds__(64) void k_rsi_bwd(const float* x, const float* g, 
     }
     st8(dr, t0, k.a, k.e, o);
   }
-  da = wsum(da);
-  if (k.lane == 0) dal[row] = da;
+  rsum<1>(da, xw);
+  if (k.tid == 0) dal[row] = da[0];
 }
 
 // ================================================================== MACD ==
 // Adjoint of ta_MACD.c (linear in the prices).  gm, gs, gh: gradients of the
 // MACD line, signal and histogram outputs.
-__global__ __launch_bounds__(64) void k_macd_bwd(const float* gm, const float* gs,
-                                                 const float* gh, float* dx, int rows, int n,
-                                                 int fast, int slow, int sig) {
+__global__ __launch_bounds__(kAT) void k_macd_bwd(const float* gm, const float* gs,
+                                                  const float* gh, float* dx, int rows, int n,
+                                                  int fast, int slow, int sig) {
+  __shared__ Xw xw;
   extern __shared__ float s_col[];
   const int row = blockIdx.x;
   if (row >= rows) return;
@@ -554,7 +671,7 @@ __global__ __launch_bounds__(64) void k_macd_bwd(const float* gm, const float* g
   // signal adjoint mu_t = (gs - gh)_t + gb*mu_{t+1} for t >= L2
   const float *gmr = gm + ro, *gsr = gs + ro, *ghr = gh + ro;
   const int lo2 = max(k.a, L2);
-  double b = 0.0;
+  double b[1] = {0.0};
   for (int t1 = k.e - 1; t1 >= lo2; t1 -= kB) {
     float vs[kB], vh[kB];
     ld8(gsr, t1 - (kB - 1), lo2, k.e, vs);
@@ -562,10 +679,14 @@ __global__ __launch_bounds__(64) void k_macd_bwd(const float* gm, const float* g
 #pragma unroll
     for (int i = 0; i < kB; ++i) {
       if (t1 - i < lo2) break;
-      b = fma(gb, b, (double)vs[kB - 1 - i] - (double)vh[kB - 1 - i]);
+      b[0] = fma(gb, b[0], (double)vs[kB - 1 - i] - (double)vh[kB - 1 - i]);
     }
   }
-  double mu = carry_bwd(b, gb, k.C), muS = 0.0;
+  {
+    const double be[1] = {gb};
+    carry_bwd<1>(b, be, k.C, xw);
+  }
+  double mu = b[0], muS[1] = {0.0};
   for (int t1 = k.e - 1; t1 >= k.a; t1 -= kB) {
     float vs[kB], vh[kB], vm[kB];
     ld8(gsr, t1 - (kB - 1), lo2, k.e, vs);
@@ -580,23 +701,27 @@ __global__ __launch_bounds__(64) void k_macd_bwd(const float* gm, const float* g
       if (t >= L2) {
         mu = fma(gb, mu, (double)vs[j] - (double)vh[j]);
         d = (double)vm[j] + (double)vh[j];
-        if (t == L2) muS = mu;
+        if (t == L2) muS[0] = mu;
         else d += gk * mu;
       }
-      dm.at(k.lane, t - k.a) = (float)d;
+      dm.at(k.tid, t - k.a) = (float)d;
     }
   }
-  muS = wsum(muS);
+  rsum<1>(muS, xw);
   for (int t = max(k.a, L1); t < k.e && t <= L2; ++t)
-    dm.at(k.lane, t - k.a) += (float)(muS / sig);
+    dm.at(k.tid, t - k.a) += (float)(muS[0] / sig);
   // fast / slow adjoints from dm (macd = fast - slow) for t >= L1
-  double bf = 0.0, bs = 0.0;
+  double bfs[2] = {0.0, 0.0};
   for (int t = k.e - 1; t >= k.a && t >= L1; --t) {
-    const double d = dm.at(k.lane, t - k.a);
-    bf = fma(fb, bf, d);
-    bs = fma(sb, bs, -d);
+    const double d = dm.at(k.tid, t - k.a);
+    bfs[0] = fma(fb, bfs[0], d);
+    bfs[1] = fma(sb, bfs[1], -d);
   }
-  double nf = carry_bwd(bf, fb, k.C), ns = carry_bwd(bs, sb, k.C), nfS = 0.0, nsS = 0.0;
+  {
+    const double be[2] = {fb, sb};
+    carry_bwd<2>(bfs, be, k.C, xw);
+  }
+  double nf = bfs[0], ns = bfs[1], nS[2] = {0.0, 0.0};
   for (int t1 = k.e - 1; t1 >= k.a; t1 -= kB) {
     float o[kB];
 #pragma unroll
@@ -604,12 +729,12 @@ __global__ __launch_bounds__(64) void k_macd_bwd(const float* gm, const float* g
       const int t = t1 - i;
       double out = 0.0;
       if (t >= L1 && t >= k.a) {
-        const double d = dm.at(k.lane, t - k.a);
+        const double d = dm.at(k.tid, t - k.a);
         nf = fma(fb, nf, d);
         ns = fma(sb, ns, -d);
         if (t == L1) {
-          nfS = nf;
-          nsS = ns;
+          nS[0] = nf;
+          nS[1] = ns;
         } else {
           out = fk * nf + sk * ns;
         }
@@ -618,11 +743,10 @@ __global__ __launch_bounds__(64) void k_macd_bwd(const float* gm, const float* g
     }
     st8(dx + ro, t1 - (kB - 1), k.a, k.e, o);
   }
-  nfS = wsum(nfS);
-  nsS = wsum(nsS);
+  rsum<2>(nS, xw);
   for (int t = k.a; t < k.e && t <= L1; ++t) {
-    double add = nsS / slow;
-    if (t > L1 - fast) add += nfS / fast;
+    double add = nS[1] / slow;
+    if (t > L1 - fast) add += nS[0] / fast;
     dx[ro + t] = (float)((double)dx[ro + t] + add);
   }
 }
@@ -630,9 +754,179 @@ __global__ __launch_bounds__(64) void k_macd_bwd(const float* gm, const float* g
 // ================================================================ BBANDS ==
 // Window adjoint: out_t = mid_t + ku*sd_t (upper), mid_t, mid_t - kd*sd_t,
 // mid = window mean, sd = population deviation.  dL/dx_j sums, over the
-// windows t = j..j+p-1 holding bar j, (gu+gm+gl)_t/p + (ku*gu - kd*gl)_t *
-// (x_j - mid_t)/(p*sd_t).  One thread per bar of a 256-bar tile; the tile's
-// per-window coefficients are built once in LDS.
+// windows t = j..j+p-1 holding bar j, A_t + B_t (x_j - mid_t) with
+// A_t = (gu+gm+gl)_t/p and B_t = (ku*gu - kd*gl)_t/(p*sd_t).
+//
+// One workgroup of kBT threads per kBB-bar tile.  The tile's windows come
+// from prefix sums of (x - c) and (x - c)^2 over its bars (c: the tile's
+// first bar, so the sums stay small next to the window moments), and every
+// bar's sum over its p windows from prefix sums of A, B and B*(mid - c):
+// O(1) a bar after two tile scans.  Threads own contiguous runs of bars and
+// of windows for the scans; 1/sd is a refined hardware reciprocal square
+// root.
+constexpr int kBT = 256;           // threads per tile
+constexpr int kBB = 512;           // bars per tile
+constexpr int kBBScanMaxP = 1024;  // longer periods take the direct kernel
+constexpr int kBBRun = 8;          // windows per thread at most: (kBB + kBBScanMaxP - 1) / kBT
+
+// Exclusive sum scan over the workgroup's kBT threads (every thread calls
+// it; it holds barriers).
+template <int R>
+__device__ __forceinline__ void blk_escan(double (&v)[R], double (*sh)[3]) {
+  const int lane = lane_id(), w = (int)threadIdx.x / kWave;
+  wave_iscan_add<R>(v);
+  double ex[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (lane == kWave - 1) sh[w][r] = v[r];
+    ex[r] = dpp_d<kWaveShr1, 0xf, 0xf>(0.0, v[r]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    double e = 0.0;
+    for (int k = 0; k < w; ++k) e += sh[k][r];
+    v[r] = ex[r] + e;
+  }
+  __syncthreads();
+}
+
+// The tile's LDS: the bar prefix sums (2 x (L + 1) doubles), later the
+// window prefix sums in the same place (3 x (W + 1)), then the staged bars.
+__host__ __device__ size_t bb_scan_doubles(int p) {
+  const size_t L = kBB + 2 * (size_t)p - 2, W = kBB + (size_t)p - 1;
+  return 2 * (L + 1) > 3 * (W + 1) ? 2 * (L + 1) : 3 * (W + 1);
+}
+size_t bb_scan_lds(int p) {
+  return bb_scan_doubles(p) * sizeof(double) + (kBB + 2 * (size_t)p - 2) * sizeof(float);
+}
+
+// 1/sqrt(v) for v > 0: the hardware estimate and two Newton steps.
+__device__ __forceinline__ double rsqrt_d(double v) {
+  double y = __builtin_amdgcn_rsq(v);
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const double e = fma(-v * y, y, 1.0);
+    y = fma(0.5 * y, e, y);
+  }
+  return y;
+}
+
+__global__ __launch_bounds__(kBT) void k_bbands_bwd_scan(const float* x, const float* gu,
+                                                         const float* gm, const float* gl,
+                                                         float* dx, int rows, int n, int p,
+                                                         double ku, double kd) {
+  extern __shared__ double s_bb[];
+  __shared__ double sh[kBT / kWave][3];
+  const int tiles = (n + kBB - 1) / kBB;
+  const int row = blockIdx.x / tiles, tile = blockIdx.x % tiles;
+  if (row >= rows) return;
+  const int tid = (int)threadIdx.x;
+  const int Lmax = kBB + 2 * p - 2, Wmax = kBB + p - 1;
+  const int j0 = tile * kBB;
+  const int T0 = j0, T1 = min(n, j0 + kBB + p - 1);  // windows used by this tile
+  const int X0 = max(0, T0 - p + 1);                  // bars those windows read
+  const int L = T1 - X0, W = T1 - T0;
+  double* P1 = s_bb;             // [L + 1] prefix sums of x - c
+  double* P2 = P1 + (Lmax + 1);  // [L + 1] prefix sums of (x - c)^2
+  double* QA = s_bb;             // [W + 1] prefix sums of A (after P1, P2 are read)
+  double* QB = QA + (Wmax + 1);  // [W + 1] prefix sums of B
+  double* QM = QB + (Wmax + 1);  // [W + 1] prefix sums of B (mid - c)
+  float* xs = reinterpret_cast<float*>(s_bb + bb_scan_doubles(p));  // [L]
+  const size_t ro = (size_t)row * n;
+  // this thread's run of windows: their output gradients are loaded first,
+  // so the loads overlap the staging and the bar scan
+  const int wm = (W + kBT - 1) / kBT;
+  const int w0 = min(W, tid * wm), wcnt = min(W, w0 + wm) - w0;
+  float U[kBBRun], Mg[kBBRun], Lg[kBBRun];
+#pragma unroll
+  for (int k = 0; k < kBBRun; ++k) {
+    const int t = T0 + w0 + k;
+    const bool on = k < wcnt && t >= p - 1;
+    U[k] = on ? gu[ro + t] : 0.0f;
+    Mg[k] = on ? gm[ro + t] : 0.0f;
+    Lg[k] = on ? gl[ro + t] : 0.0f;
+  }
+  for (int i = tid; i < L; i += kBT) xs[i] = x[ro + X0 + i];
+  __syncthreads();
+  const double c = (double)xs[T0 - X0];
+  // prefix sums over the staged bars, thread runs of m bars
+  {
+    const int m = (L + kBT - 1) / kBT;
+    const int r0 = min(L, tid * m), r1 = min(L, r0 + m);
+    double v[2] = {0.0, 0.0};
+    for (int i = r0; i < r1; ++i) {
+      const double d = (double)xs[i] - c;
+      v[0] += d;
+      v[1] += d * d;
+    }
+    blk_escan<2>(v, sh);
+    if (tid == 0) P1[0] = P2[0] = 0.0;
+    for (int i = r0; i < r1; ++i) {
+      const double d = (double)xs[i] - c;
+      v[0] += d;
+      v[1] += d * d;
+      P1[i + 1] = v[0];
+      P2[i + 1] = v[1];
+    }
+  }
+  __syncthreads();
+  // every window's A, B, B (mid - c) for a run of windows per thread, in
+  // registers; then their prefix sums over the tile
+  {
+    const double inv = 1.0 / (double)p;
+    const int r0 = w0, cnt = wcnt;
+    double A[kBBRun], B[kBBRun], BM[kBBRun];
+    double v[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < kBBRun; ++k) {
+      A[k] = B[k] = BM[k] = 0.0;
+      const int t = T0 + r0 + k;
+      if (k < cnt && t >= p - 1) {
+        const int lo = t - p + 1 - X0, hi = t + 1 - X0;
+        const double mean = (P1[hi] - P1[lo]) * inv, e2 = (P2[hi] - P2[lo]) * inv;
+        const double var = e2 - mean * mean;
+        const double u = U[k], mm = Mg[k], l = Lg[k];
+        A[k] = (u + mm + l) * inv;
+        if (var > 0.0 && !(var < 1e-12 * e2)) B[k] = (ku * u - kd * l) * inv * rsqrt_d(var);
+        BM[k] = B[k] * mean;
+      }
+      v[0] += A[k];
+      v[1] += B[k];
+      v[2] += BM[k];
+    }
+    blk_escan<3>(v, sh);  // its barriers also end every read of P1 and P2
+    if (tid == 0) QA[0] = QB[0] = QM[0] = 0.0;
+#pragma unroll
+    for (int k = 0; k < kBBRun; ++k) {
+      if (k >= cnt) break;
+      v[0] += A[k];
+      v[1] += B[k];
+      v[2] += BM[k];
+      QA[r0 + k + 1] = v[0];
+      QB[r0 + k + 1] = v[1];
+      QM[r0 + k + 1] = v[2];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < kBB / kBT; ++q) {
+    const int j = j0 + q * kBT + tid;
+    if (j >= n) break;
+    const int tlo = max(j, p - 1), thi = min(n - 1, j + p - 1);
+    double acc = 0.0;
+    if (tlo <= thi) {
+      const int i0 = tlo - T0, i1 = thi - T0 + 1;
+      const double sa = QA[i1] - QA[i0], sb = QB[i1] - QB[i0], sm = QM[i1] - QM[i0];
+      acc = sa + ((double)xs[j - X0] - c) * sb - sm;
+    }
+    dx[ro + j] = (float)acc;
+  }
+}
+
+// Periods above kBBScanMaxP: one thread per bar of a 256-bar tile, every
+// window's moments and every bar's window sum evaluated directly; the
+// tile's per-window coefficients are built once in LDS.
 __global__ __launch_bounds__(256) void k_bbands_bwd(const float* x, const float* gu,
                                                     const float* gm, const float* gl, float* dx,
                                                     int rows, int n, int p, double ku,
@@ -692,15 +986,15 @@ __global__ __launch_bounds__(256) void k_bbands_bwd(const float* x, const float*
 ctStatus_t check(const void* a, const void* b, int rows, int n, int p, const char* fn) {
   if (!a || !b) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: null pointer", fn);
   if (rows <= 0 || n <= 0 || p <= 0) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: invalid shape or period", fn);
-  if ((n + kWave - 1) / kWave > kMaxChunk)
+  if ((n + kAT - 1) / kAT > kMaxChunk)
     return fail(CT_STATUS_UNSUPPORTED, "%s: rows longer than %d bars are not provided", fn,
-                kWave * kMaxChunk);
+                kAT * kMaxChunk);
   return CT_STATUS_SUCCESS;
 }
 
 size_t col_bytes(int n, int arrays) {
-  const int C = (n + kWave - 1) / kWave;
-  return (size_t)arrays * kWave * (C + 1) * sizeof(float);
+  const int C = (n + kAT - 1) / kAT;
+  return (size_t)arrays * kAT * (C + 1) * sizeof(float);
 }
 
 template <class K>
@@ -716,10 +1010,6 @@ ctStatus_t allow_lds(K kernel, size_t bytes) {
 
 using namespace tca;
 
-extern "C" {
-
-}  // extern "C"
-
 namespace {
 
 ctStatus_t ema_alpha_fwd(const float* x, float* y, double* ckpt, int n, int rows, int period,
@@ -727,7 +1017,7 @@ ctStatus_t ema_alpha_fwd(const float* x, float* y, double* ckpt, int n, int rows
   ctStatus_t r = check(x, y, rows, n, period, "ema_alpha");
   if (r != CT_STATUS_SUCCESS) return r;
   if (!ckpt) return fail(CT_STATUS_INVALID_ARGUMENT, "ema_alpha: null checkpoint");
-  hipLaunchKernelGGL(k_ema_fwd, dim3(rows), dim3(kWave), 0, s, x, y, ckpt, rows, n, period, alpha,
+  hipLaunchKernelGGL(k_ema_fwd, dim3(rows), dim3(kAT), 0, s, x, y, ckpt, rows, n, period, alpha,
                      alphaPtr);
   TCA_HIP(hipGetLastError());
   return CT_STATUS_SUCCESS;
@@ -743,7 +1033,7 @@ ctStatus_t ema_alpha_bwd(const float* x, const float* gy, const double* ckpt, fl
   const size_t lds = col_bytes(n, 1);
   r = allow_lds(k_ema_bwd, lds);
   if (r != CT_STATUS_SUCCESS) return r;
-  hipLaunchKernelGGL(k_ema_bwd, dim3(rows), dim3(kWave), lds, s, x, gy, ckpt, gx, galpha, rows, n,
+  hipLaunchKernelGGL(k_ema_bwd, dim3(rows), dim3(kAT), lds, s, x, gy, ckpt, gx, galpha, rows, n,
                      period, alpha, alphaPtr);
   TCA_HIP(hipGetLastError());
   return CT_STATUS_SUCCESS;
@@ -754,7 +1044,7 @@ ctStatus_t rsi_alpha_fwd(const float* x, float* y, double* ckpt, int n, int rows
   ctStatus_t r = check(x, y, rows, n, period, "rsi_alpha");
   if (r != CT_STATUS_SUCCESS) return r;
   if (!ckpt) return fail(CT_STATUS_INVALID_ARGUMENT, "rsi_alpha: null checkpoint");
-  hipLaunchKernelGGL(k_rsi_fwd, dim3(rows), dim3(kWave), 0, s, x, y, ckpt, rows, n, period, alpha,
+  hipLaunchKernelGGL(k_rsi_fwd, dim3(rows), dim3(kAT), 0, s, x, y, ckpt, rows, n, period, alpha,
                      alphaPtr);
   TCA_HIP(hipGetLastError());
   return CT_STATUS_SUCCESS;
@@ -767,11 +1057,11 @@ ctStatus_t rsi_alpha_bwd(const float* x, const float* gy, const double* ckpt, fl
   if (r != CT_STATUS_SUCCESS) return r;
   if (!ckpt || !gx || !galpha)
     return fail(CT_STATUS_INVALID_ARGUMENT, "rsi_alpha backward: null pointer");
-  // per lane and kB-bar block: the entering G and L (doubles)
-  const size_t lds = (size_t)2 * kWave * (((n + kWave - 1) / kWave + kB - 1) / kB) * sizeof(double);
+  // per thread and kB-bar block: the entering G and L (doubles)
+  const size_t lds = (size_t)2 * kAT * (((n + kAT - 1) / kAT + kB - 1) / kB) * sizeof(double);
   r = allow_lds(k_rsi_bwd, lds);
   if (r != CT_STATUS_SUCCESS) return r;
-  hipLaunchKernelGGL(k_rsi_bwd, dim3(rows), dim3(kWave), lds, s, x, gy, ckpt, gx, galpha, rows, n,
+  hipLaunchKernelGGL(k_rsi_bwd, dim3(rows), dim3(kAT), lds, s, x, gy, ckpt, gx, galpha, rows, n,
                      period, alpha, alphaPtr);
   TCA_HIP(hipGetLastError());
   return CT_STATUS_SUCCESS;
@@ -785,6 +1075,8 @@ ctStatus_t alpha_ptr_ok(const double* d_alpha, const char* what) {
 }  // namespace
 
 extern "C" {
+
+CTAPI_EXPORT int ctx_alpha_ckpt_per_row(void) { return kAT; }
 
 CTAPI_EXPORT ctStatus_t ctx_ema_alpha_fwd_d(const float* x, float* y, double* ckpt, int n,
                                             int rows, int period, double alpha, hipStream_t s) {
@@ -867,7 +1159,7 @@ CTAPI_EXPORT ctStatus_t ctx_macd_bwd_d(const float* gMacd, const float* gSignal,
     const size_t lds = col_bytes(n, 1);
     r = allow_lds(k_macd_bwd, lds);
     if (r != CT_STATUS_SUCCESS) return r;
-    hipLaunchKernelGGL(k_macd_bwd, dim3(rows), dim3(kWave), lds, s, gMacd, gSignal, gHist, gx,
+    hipLaunchKernelGGL(k_macd_bwd, dim3(rows), dim3(kAT), lds, s, gMacd, gSignal, gHist, gx,
                        rows, n, fast, slow, signal);
     TCA_HIP(hipGetLastError());
     return CT_STATUS_SUCCESS;
@@ -884,6 +1176,17 @@ CTAPI_EXPORT ctStatus_t ctx_bbands_bwd_d(const float* x, const float* gUpper,
     if (rows <= 0 || n <= 0 || period <= 0)
       return fail(CT_STATUS_INVALID_ARGUMENT, "bbands backward: invalid shape or period");
     if (period > 4096) return fail(CT_STATUS_UNSUPPORTED, "bbands backward: period above 4096");
+    if (period <= kBBScanMaxP) {
+      const size_t lds = bb_scan_lds(period);
+      ctStatus_t r = allow_lds(k_bbands_bwd_scan, lds);
+      if (r != CT_STATUS_SUCCESS) return r;
+      const int tiles = (n + kBB - 1) / kBB;
+      hipLaunchKernelGGL(k_bbands_bwd_scan, dim3((unsigned)rows * tiles), dim3(kBT), lds, s, x,
+                         gUpper, gMiddle, gLower, gx, rows, n, period, (double)nbDevUp,
+                         (double)nbDevDn);
+      TCA_HIP(hipGetLastError());
+      return CT_STATUS_SUCCESS;
+    }
     const size_t lds = (size_t)3 * (256 + period) * sizeof(double) +
                        (size_t)(256 + 2 * period) * sizeof(float);
     ctStatus_t r = allow_lds(k_bbands_bwd, lds);

@@ -34,14 +34,35 @@ ALPHA0 = {"ema": 2.0 / 21.0, "rsi": 1.0 / 14.0, "macd": 0.0, "bbands": 0.0}
 
 @pytest.mark.parametrize("name", sorted(CASES))
 def test_gradients_against_fp64_reference(dev, name):
-    print(f"autograd {name}", flush=True)
+    _check_against_reference(dev, name, ROWS, N)
+
+
+# Rows of 40 and 500 bars leave most of a row's 512 threads without bars; 777
+# and 4099 bars end inside a thread's chunk and inside a wave; 16384 is the
+# longest row (32 bars a thread).
+@pytest.mark.parametrize("n", [40, 500, 777, 4099, 16384])
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_gradients_row_lengths(dev, name, n):
+    if n == 16384 and name in ("macd", "bbands"):
+        pytest.skip("the 16384-bar row is covered by the recurrences with alpha")
+    _check_against_reference(dev, name, 3, n, seed=n)
+
+
+def _check_against_reference(dev, name, rows, n, seed=0):
+    print(f"autograd {name} rows {rows} bars {n}", flush=True)
     spec = CASES[name]
-    c = _close(ROWS, N)
+    c = _close(rows, n, seed=seed)
+    ROWS, N = rows, n
     nout = spec["nout"]
-    ws = [_weights((ROWS, N)) for _ in range(nout)]
+    # the weights and alpha the GPU receives are float32: the FP64 reference
+    # takes the same rounded values, so the bound measures the kernels'
+    # arithmetic (at 500 bars the alpha gradient's cancellation turns the
+    # weights' float32 rounding alone into a 1.2e-5 relative change)
+    ws = [_weights((ROWS, N)).astype(np.float32).astype(np.float64) for _ in range(nout)]
+    a32 = float(np.float32(ALPHA0[name]))
     # FP64 reference gradient
     xr = torch.tensor(c, dtype=torch.float64, requires_grad=True)
-    ar = torch.tensor(ALPHA0[name], dtype=torch.float64, requires_grad=True)
+    ar = torch.tensor(a32, dtype=torch.float64, requires_grad=True)
     ref_out = _ref_fn(name)(xr, ar)
     Lr = _loss(ref_out, [torch.tensor(w) for w in ws])
     Lr.backward()
