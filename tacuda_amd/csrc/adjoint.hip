@@ -834,19 +834,6 @@ __global__ __launch_bounds__(kBT) void k_bbands_bwd_scan(const float* x, const f
   double* QM = QB + (Wmax + 1);  // [W + 1] prefix sums of B (mid - c)
   float* xs = reinterpret_cast<float*>(s_bb + bb_scan_doubles(p));  // [L]
   const size_t ro = (size_t)row * n;
-  // this thread's run of windows: their output gradients are loaded first,
-  // so the loads overlap the staging and the bar scan
-  const int wm = (W + kBT - 1) / kBT;
-  const int w0 = min(W, tid * wm), wcnt = min(W, w0 + wm) - w0;
-  float U[kBBRun], Mg[kBBRun], Lg[kBBRun];
-#pragma unroll
-  for (int k = 0; k < kBBRun; ++k) {
-    const int t = T0 + w0 + k;
-    const bool on = k < wcnt && t >= p - 1;
-    U[k] = on ? gu[ro + t] : 0.0f;
-    Mg[k] = on ? gm[ro + t] : 0.0f;
-    Lg[k] = on ? gl[ro + t] : 0.0f;
-  }
   for (int i = tid; i < L; i += kBT) xs[i] = x[ro + X0 + i];
   __syncthreads();
   const double c = (double)xs[T0 - X0];
@@ -875,7 +862,8 @@ __global__ __launch_bounds__(kBT) void k_bbands_bwd_scan(const float* x, const f
   // registers; then their prefix sums over the tile
   {
     const double inv = 1.0 / (double)p;
-    const int r0 = w0, cnt = wcnt;
+    const int m = (W + kBT - 1) / kBT;
+    const int r0 = min(W, tid * m), cnt = min(W, r0 + m) - r0;
     double A[kBBRun], B[kBBRun], BM[kBBRun];
     double v[3] = {0.0, 0.0, 0.0};
 #pragma unroll
@@ -886,7 +874,7 @@ __global__ __launch_bounds__(kBT) void k_bbands_bwd_scan(const float* x, const f
         const int lo = t - p + 1 - X0, hi = t + 1 - X0;
         const double mean = (P1[hi] - P1[lo]) * inv, e2 = (P2[hi] - P2[lo]) * inv;
         const double var = e2 - mean * mean;
-        const double u = U[k], mm = Mg[k], l = Lg[k];
+        const double u = gu[ro + t], mm = gm[ro + t], l = gl[ro + t];
         A[k] = (u + mm + l) * inv;
         if (var > 0.0 && !(var < 1e-12 * e2)) B[k] = (ku * u - kd * l) * inv * rsqrt_d(var);
         BM[k] = B[k] * mean;
