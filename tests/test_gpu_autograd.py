@@ -40,11 +40,10 @@ def test_gradients_against_fp64_reference(dev, name):
 # Rows of 40 and 500 bars leave most of a row's 512 threads without bars; 777
 # and 4099 bars end inside a thread's chunk and inside a wave; 16384 is the
 # longest row (32 bars a thread).
-@pytest.mark.parametrize("n", [40, 500, 777, 4099, 16384])
-@pytest.mark.parametrize("name", sorted(CASES))
+@pytest.mark.parametrize("name,n", [(name, n) for name in sorted(CASES)
+                                    for n in (40, 500, 777, 4099)] +
+                         [("ema", 16384), ("rsi", 16384)])
 def test_gradients_row_lengths(dev, name, n):
-    if n == 16384 and name in ("macd", "bbands"):
-        pytest.skip("the 16384-bar row is covered by the recurrences with alpha")
     _check_against_reference(dev, name, 3, n, seed=n)
 
 
