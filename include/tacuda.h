@@ -253,6 +253,84 @@ CTAPI_EXPORT ctStatus_t ct_plus_dm_batch_d(const float* d_highs, const float* d_
 CTAPI_EXPORT ctStatus_t ct_minus_dm_batch(const float* host_highs, const float* host_lows, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
 CTAPI_EXPORT ctStatus_t ct_minus_dm_batch_d(const float* d_highs, const float* d_lows, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
 
+
+/* ---------------------------------------------------------------------------
+ * Price transforms, the momentum family, window midpoints and the
+ * linear-regression family (csrc/transforms.hip): host, device-alias and
+ * batch forms.  Arithmetic in double with TA-Lib 0.8.3's formulas; a
+ * period of 0 or less is CT_STATUS_INVALID_ARGUMENT, a period beyond the
+ * series (MOMENTUM/ROC*: at least the series) gives an all-NaN line.
+ * ------------------------------------------------------------------------- */
+/* :111 */ CTAPI_EXPORT ctStatus_t ct_momentum(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :115 */ CTAPI_EXPORT ctStatus_t ct_roc(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :117 */ CTAPI_EXPORT ctStatus_t ct_rocp(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :119 */ CTAPI_EXPORT ctStatus_t ct_rocr(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :121 */ CTAPI_EXPORT ctStatus_t ct_rocr100(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :422 */ CTAPI_EXPORT ctStatus_t ct_midpoint(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :742 */ CTAPI_EXPORT ctStatus_t ct_linearreg(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :744 */ CTAPI_EXPORT ctStatus_t ct_linearreg_slope(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :747 */ CTAPI_EXPORT ctStatus_t ct_linearreg_intercept(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :750 */ CTAPI_EXPORT ctStatus_t ct_linearreg_angle(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :753 */ CTAPI_EXPORT ctStatus_t ct_tsf(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :403 */ CTAPI_EXPORT ctStatus_t ct_avgprice(const float* host_open, const float* host_high, const float* host_low, const float* host_close, float* host_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :408 */ CTAPI_EXPORT ctStatus_t ct_medprice(const float* host_high, const float* host_low, float* host_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :411 */ CTAPI_EXPORT ctStatus_t ct_typprice(const float* host_high, const float* host_low, const float* host_close, float* host_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :415 */ CTAPI_EXPORT ctStatus_t ct_wclprice(const float* host_high, const float* host_low, const float* host_close, float* host_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :424 */ CTAPI_EXPORT ctStatus_t ct_midprice(const float* host_high, const float* host_low, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :212 */ CTAPI_EXPORT ctStatus_t ct_momentum_device(const float* device_input, float* device_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :218 */ CTAPI_EXPORT ctStatus_t ct_roc_device(const float* device_input, float* device_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :221 */ CTAPI_EXPORT ctStatus_t ct_rocp_device(const float* device_input, float* device_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :224 */ CTAPI_EXPORT ctStatus_t ct_rocr_device(const float* device_input, float* device_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :227 */ CTAPI_EXPORT ctStatus_t ct_rocr100_device(const float* device_input, float* device_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :992 / :997 */
+CTAPI_EXPORT ctStatus_t ct_momentum_batch(const float* host_inputs, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_momentum_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1002 / :1007 */
+CTAPI_EXPORT ctStatus_t ct_roc_batch(const float* host_inputs, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_roc_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1011 / :1016 */
+CTAPI_EXPORT ctStatus_t ct_rocp_batch(const float* host_inputs, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_rocp_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1021 / :1026 */
+CTAPI_EXPORT ctStatus_t ct_rocr_batch(const float* host_inputs, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_rocr_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1031 / :1036 */
+CTAPI_EXPORT ctStatus_t ct_rocr100_batch(const float* host_inputs, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_rocr100_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1511 / :1516 */
+CTAPI_EXPORT ctStatus_t ct_midpoint_batch(const float* host_inputs, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_midpoint_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1096 / :1101 */
+CTAPI_EXPORT ctStatus_t ct_linearreg_batch(const float* host_inputs, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_linearreg_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1106 / :1112 */
+CTAPI_EXPORT ctStatus_t ct_linearreg_slope_batch(const float* host_inputs, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_linearreg_slope_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1118 / :1124 */
+CTAPI_EXPORT ctStatus_t ct_linearreg_intercept_batch(const float* host_inputs, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_linearreg_intercept_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1131 / :1137 */
+CTAPI_EXPORT ctStatus_t ct_linearreg_angle_batch(const float* host_inputs, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_linearreg_angle_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1143 / :1148 */
+CTAPI_EXPORT ctStatus_t ct_tsf_batch(const float* host_inputs, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_tsf_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1421 / :1428 */
+CTAPI_EXPORT ctStatus_t ct_avgprice_batch(const float* host_opens, const float* host_highs, const float* host_lows, const float* host_closes, float* host_outputs, int seriesLen, int numSeries, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_avgprice_batch_d(const float* d_opens, const float* d_highs, const float* d_lows, const float* d_closes, float* d_outputs, int seriesLen, int numSeries, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1334 / :1339 */
+CTAPI_EXPORT ctStatus_t ct_medprice_batch(const float* host_highs, const float* host_lows, float* host_outputs, int seriesLen, int numSeries, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_medprice_batch_d(const float* d_highs, const float* d_lows, float* d_outputs, int seriesLen, int numSeries, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1294 / :1300 */
+CTAPI_EXPORT ctStatus_t ct_typprice_batch(const float* host_highs, const float* host_lows, const float* host_closes, float* host_outputs, int seriesLen, int numSeries, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_typprice_batch_d(const float* d_highs, const float* d_lows, const float* d_closes, float* d_outputs, int seriesLen, int numSeries, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1306 / :1312 */
+CTAPI_EXPORT ctStatus_t ct_wclprice_batch(const float* host_highs, const float* host_lows, const float* host_closes, float* host_outputs, int seriesLen, int numSeries, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_wclprice_batch_d(const float* d_highs, const float* d_lows, const float* d_closes, float* d_outputs, int seriesLen, int numSeries, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1721 / :1727 */
+CTAPI_EXPORT ctStatus_t ct_midprice_batch(const float* host_highs, const float* host_lows, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+CTAPI_EXPORT ctStatus_t ct_midprice_batch_d(const float* d_highs, const float* d_lows, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+
 #ifdef __cplusplus
 }
 #endif

@@ -195,6 +195,26 @@ CT_CDL_DECL(bullish_engulfing)
 CT_CDL_DECL(bearish_engulfing)
 #undef CT_CDL_DECL
 
+
+/* Price transforms, momentum, window midpoints and linear regression on
+ * device pointers (csrc/transforms.hip; enqueued on the stream). */
+/* :171 */ CTAPI_EXPORT ctStatus_t ct_momentum_d(const float* d_input, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :174 */ CTAPI_EXPORT ctStatus_t ct_roc_d(const float* d_input, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :177 */ CTAPI_EXPORT ctStatus_t ct_rocp_d(const float* d_input, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :180 */ CTAPI_EXPORT ctStatus_t ct_rocr_d(const float* d_input, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :183 */ CTAPI_EXPORT ctStatus_t ct_rocr100_d(const float* d_input, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :447 */ CTAPI_EXPORT ctStatus_t ct_midpoint_d(const float* d_input, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :357 */ CTAPI_EXPORT ctStatus_t ct_linearreg_d(const float* d_input, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :360 */ CTAPI_EXPORT ctStatus_t ct_linearreg_slope_d(const float* d_input, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :363 */ CTAPI_EXPORT ctStatus_t ct_linearreg_intercept_d(const float* d_input, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :366 */ CTAPI_EXPORT ctStatus_t ct_linearreg_angle_d(const float* d_input, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :369 */ CTAPI_EXPORT ctStatus_t ct_tsf_d(const float* d_input, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :338 */ CTAPI_EXPORT ctStatus_t ct_avgprice_d(const float* d_open, const float* d_high, const float* d_low, const float* d_close, float* d_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :342 */ CTAPI_EXPORT ctStatus_t ct_medprice_d(const float* d_high, const float* d_low, float* d_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :345 */ CTAPI_EXPORT ctStatus_t ct_typprice_d(const float* d_high, const float* d_low, const float* d_close, float* d_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :349 */ CTAPI_EXPORT ctStatus_t ct_wclprice_d(const float* d_high, const float* d_low, const float* d_close, float* d_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :511 */ CTAPI_EXPORT ctStatus_t ct_midprice_d(const float* d_high, const float* d_low, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+
 #ifdef __cplusplus
 }
 #endif

@@ -188,6 +188,55 @@ class _Family:
     def ad(self, h, l, c, v):
         return _call(self._fn("ad", 4, 1, []), [h, l, c, v], 1, len(c))
 
+    # price transforms, momentum, window midpoints, linear regression
+    def avgprice(self, o, h, l, c):
+        return _call(self._fn("avgprice", 4, 1, []), [o, h, l, c], 1, len(c))
+
+    def medprice(self, h, l):
+        return _call(self._fn("medprice", 2, 1, []), [h, l], 1, len(h))
+
+    def typprice(self, h, l, c):
+        return _call(self._fn("typprice", 3, 1, []), [h, l, c], 1, len(c))
+
+    def wclprice(self, h, l, c):
+        return _call(self._fn("wclprice", 3, 1, []), [h, l, c], 1, len(c))
+
+    def momentum(self, x, p):
+        return _call(self._fn("momentum", 1, 1, [_I]), [x], 1, len(x), p)
+
+    def roc(self, x, p):
+        return _call(self._fn("roc", 1, 1, [_I]), [x], 1, len(x), p)
+
+    def rocp(self, x, p):
+        return _call(self._fn("rocp", 1, 1, [_I]), [x], 1, len(x), p)
+
+    def rocr(self, x, p):
+        return _call(self._fn("rocr", 1, 1, [_I]), [x], 1, len(x), p)
+
+    def rocr100(self, x, p):
+        return _call(self._fn("rocr100", 1, 1, [_I]), [x], 1, len(x), p)
+
+    def midprice(self, h, l, p):
+        return _call(self._fn("midprice", 2, 1, [_I]), [h, l], 1, len(h), p)
+
+    def midpoint(self, x, p):
+        return _call(self._fn("midpoint", 1, 1, [_I]), [x], 1, len(x), p)
+
+    def linearreg(self, x, p):
+        return _call(self._fn("linearreg", 1, 1, [_I]), [x], 1, len(x), p)
+
+    def linearreg_slope(self, x, p):
+        return _call(self._fn("linearreg_slope", 1, 1, [_I]), [x], 1, len(x), p)
+
+    def linearreg_intercept(self, x, p):
+        return _call(self._fn("linearreg_intercept", 1, 1, [_I]), [x], 1, len(x), p)
+
+    def linearreg_angle(self, x, p):
+        return _call(self._fn("linearreg_angle", 1, 1, [_I]), [x], 1, len(x), p)
+
+    def tsf(self, x, p):
+        return _call(self._fn("tsf", 1, 1, [_I]), [x], 1, len(x), p)
+
     def candle(self, name, o, h, l, c):
         """int32 signal of one of the 61 TA-Lib patterns (see candles.c)."""
         L = self._get()

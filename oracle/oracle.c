@@ -1220,3 +1220,146 @@ API int orc_macdext(const float* x, float* m, float* s, float* h, int n, int fas
   free(b);
   return ORC_OK;
 }
+
+/* ---------------------------------------------------- price transforms ---
+ * MEDPRICE (h+l)/2, TYPPRICE (h+l+c)/3 in double (ta_MEDPRICE.c:70,
+ * ta_TYPPRICE.c:68; pinned to _ref TA_MEDPRICE / TA_TYPPRICE); AVGPRICE
+ * (o+h+l+c)/4 and WCLPRICE (h+l+2c)/4, the reference's AvgPrice.cu:18 and
+ * WclPrice.cu:16 formulas in double (not vendored; pinned by closed forms on
+ * dyadic prices, where float arithmetic is exact). */
+API int orc_avgprice(const float* o, const float* h, const float* l, const float* c, float* out,
+                     int n) {
+  if (n <= 0) return ORC_BAD;
+  for (int i = 0; i < n; ++i)
+    out[i] = (float)(((double)o[i] + (double)h[i] + (double)l[i] + (double)c[i]) / 4.0);
+  return ORC_OK;
+}
+API int orc_medprice(const float* h, const float* l, float* out, int n) {
+  if (n <= 0) return ORC_BAD;
+  for (int i = 0; i < n; ++i) out[i] = (float)(((double)h[i] + (double)l[i]) / 2.0);
+  return ORC_OK;
+}
+API int orc_typprice(const float* h, const float* l, const float* c, float* out, int n) {
+  if (n <= 0) return ORC_BAD;
+  for (int i = 0; i < n; ++i)
+    out[i] = (float)(((double)h[i] + (double)l[i] + (double)c[i]) / 3.0);
+  return ORC_OK;
+}
+API int orc_wclprice(const float* h, const float* l, const float* c, float* out, int n) {
+  if (n <= 0) return ORC_BAD;
+  for (int i = 0; i < n; ++i)
+    out[i] = (float)(((double)h[i] + (double)l[i] + (double)c[i] * 2.0) / 4.0);
+  return ORC_OK;
+}
+
+/* ------------------------------------------------------------ momentum ---
+ * y[t] = f(x[t], x[t-p]) from bar p on.  ROC ((x/prev - 1) * 100) and ROCP
+ * ((x - prev)/prev), 0 for a zero earlier price: ta_ROC.c:111-124,
+ * ta_ROCP.c:113-123 (pinned to _ref TA_ROC / TA_ROCP).  MOMENTUM (x - prev),
+ * ROCR (x/prev) and ROCR100 (100 x/prev): the reference's Momentum.cu:16,
+ * ROCR.cu:18, ROCR100.cu:19 definitions in double (not vendored; pinned by
+ * brute force against numpy and by ROCR100 = 100 ROCR on exact ratios). */
+static int mom_family(const float* x, float* out, int n, int p, int op) {
+  if (n <= 0 || p <= 0) return ORC_BAD;
+  fill_nan(out, n);
+  for (int t = p; t < n; ++t) {
+    const double cur = x[t], prev = x[t - p];
+    double v;
+    switch (op) {
+      case 0: v = cur - prev; break;
+      case 1: v = prev != 0.0 ? (cur / prev - 1.0) * 100.0 : 0.0; break;
+      case 2: v = prev != 0.0 ? (cur - prev) / prev : 0.0; break;
+      case 3: v = prev != 0.0 ? cur / prev : 0.0; break;
+      default: v = prev != 0.0 ? (cur / prev) * 100.0 : 0.0; break;
+    }
+    out[t] = (float)v;
+  }
+  return ORC_OK;
+}
+API int orc_momentum(const float* x, float* out, int n, int p) { return mom_family(x, out, n, p, 0); }
+API int orc_roc(const float* x, float* out, int n, int p) { return mom_family(x, out, n, p, 1); }
+API int orc_rocp(const float* x, float* out, int n, int p) { return mom_family(x, out, n, p, 2); }
+API int orc_rocr(const float* x, float* out, int n, int p) { return mom_family(x, out, n, p, 3); }
+API int orc_rocr100(const float* x, float* out, int n, int p) { return mom_family(x, out, n, p, 4); }
+
+/* ---------------------------------------------------- window midpoints ---
+ * (max + min)/2 over the p bars ending at t, from bar p-1 on; NaN samples
+ * are skipped (fmaxf/fminf, reference MIDPRICE.cu:14-17, MIDPOINT.cu:19-24);
+ * a MIDPOINT window that starts inside the row's leading NaN run is NaN
+ * (MIDPOINT.cu:25-29, LeadingNaN.cuh:22-28).  MIDPRICE pinned to _ref
+ * TA_MIDPRICE on finite data; MIDPOINT by brute force (numpy) and by
+ * MIDPOINT(x) = MIDPRICE(x, x) on finite data. */
+static int mid_family(const float* hi, const float* lo, float* out, int n, int p, int one) {
+  if (n <= 0 || p <= 0) return ORC_BAD;
+  fill_nan(out, n);
+  for (int t = p - 1; t < n; ++t) {
+    const int s = t - p + 1;
+    float mx = hi[s], mn = lo[s];
+    for (int k = s + 1; k <= t; ++k) {
+      mx = fmaxf(mx, hi[k]);
+      mn = fminf(mn, lo[k]);
+    }
+    if (one && !isnan(mx) && isnan(hi[s])) {
+      int lead = 1;
+      for (int q = s - 1; q >= 0 && lead; --q) lead = isnan(hi[q]);
+      if (lead) continue;
+    }
+    out[t] = (float)(((double)mx + (double)mn) / 2.0);
+  }
+  return ORC_OK;
+}
+API int orc_midprice(const float* h, const float* l, float* out, int n, int p) {
+  return mid_family(h, l, out, n, p, 0);
+}
+API int orc_midpoint(const float* x, float* out, int n, int p) {
+  return mid_family(x, x, out, n, p, 1);
+}
+
+/* --------------------------------------------------- linear regression ---
+ * ta_LINEARREG.c:105-131: x is the distance back from t (0..p-1), SumX,
+ * SumXSqr and Divisor = SumX^2 - p SumXSqr of those, the window primed
+ * oldest first; m = (p SumXY - SumX SumY)/Divisor, b = (SumY - m SumX)/p.
+ * LINEARREG = fma(m, p-1, b) (pinned to _ref TA_LINEARREG); SLOPE m,
+ * INTERCEPT b, ANGLE atan(m) 180/pi and TSF b + m p (TA-Lib's definitions,
+ * ta_LINEARREG.c:90-96; not vendored: pinned by exact lines, where the
+ * slope, intercept and forecast are known, and by LINEARREG = INTERCEPT +
+ * (p-1) SLOPE).  A one-bar window has slope 0 and intercept x (the
+ * reference's zero-denominator rule, LinearRegression.cuh:36-40). */
+static int linreg_family(const float* x, float* out, int n, int p, int kind) {
+  if (n <= 0 || p <= 0) return ORC_BAD;
+  fill_nan(out, n);
+  const double sumX = (double)p * (p - 1) * 0.5;
+  const double sumXSqr = (double)p * (p - 1) * (2 * p - 1) / 6.0;
+  const double divisor = sumX * sumX - p * sumXSqr;
+  for (int t = p - 1; t < n; ++t) {
+    double sumY = 0.0, sumXY = 0.0;
+    for (int i = p - 1; i >= 0; --i) {
+      const double v = x[t - i];
+      sumY += v;
+      sumXY += (double)i * v;
+    }
+    double m, b;
+    if (divisor == 0.0) {
+      m = 0.0;
+      b = sumY / (double)p;
+    } else {
+      m = ((double)p * sumXY - sumX * sumY) / divisor;
+      b = (sumY - m * sumX) / (double)p;
+    }
+    double v;
+    switch (kind) {
+      case 0: v = fma(m, (double)(p - 1), b); break;
+      case 1: v = m; break;
+      case 2: v = b; break;
+      case 3: v = atan(m) * (180.0 / 3.14159265358979323846); break;
+      default: v = b + m * (double)p; break;
+    }
+    out[t] = (float)v;
+  }
+  return ORC_OK;
+}
+API int orc_linearreg(const float* x, float* out, int n, int p) { return linreg_family(x, out, n, p, 0); }
+API int orc_linearreg_slope(const float* x, float* out, int n, int p) { return linreg_family(x, out, n, p, 1); }
+API int orc_linearreg_intercept(const float* x, float* out, int n, int p) { return linreg_family(x, out, n, p, 2); }
+API int orc_linearreg_angle(const float* x, float* out, int n, int p) { return linreg_family(x, out, n, p, 3); }
+API int orc_tsf(const float* x, float* out, int n, int p) { return linreg_family(x, out, n, p, 4); }

@@ -183,3 +183,39 @@ API int ref_ma(const float* x, float* out, int n, int p, int type) {
   place(out, n, beg, nb, o.data());
   return 0;
 }
+
+// Price transforms, ROC / ROCP, MIDPRICE and LINEARREG of the vendored
+// TA-Lib 0.8.3 (checks of the oracle's restatements).
+REF_1IN_1OUT(roc, TA_ROC(a, 0, n - 1, in.data(), p, &beg, &nb, o.data()))
+REF_1IN_1OUT(rocp, TA_ROCP(a, 0, n - 1, in.data(), p, &beg, &nb, o.data()))
+REF_1IN_1OUT(linearreg, TA_LINEARREG(a, 0, n - 1, in.data(), p, &beg, &nb, o.data()))
+
+API int ref_medprice(const float* hi, const float* lo, float* out, int n) {
+  std::vector<double> h = widen(hi, n), l = widen(lo, n), o(n);
+  Arena a = make_arena(n);
+  int beg = 0, nb = 0;
+  int rc = TA_MEDPRICE(a, 0, n - 1, h.data(), l.data(), &beg, &nb, o.data());
+  if (rc) return rc;
+  place(out, n, beg, nb, o.data());
+  return 0;
+}
+
+API int ref_typprice(const float* hi, const float* lo, const float* cl, float* out, int n) {
+  std::vector<double> h = widen(hi, n), l = widen(lo, n), c = widen(cl, n), o(n);
+  Arena a = make_arena(n);
+  int beg = 0, nb = 0;
+  int rc = TA_TYPPRICE(a, 0, n - 1, h.data(), l.data(), c.data(), &beg, &nb, o.data());
+  if (rc) return rc;
+  place(out, n, beg, nb, o.data());
+  return 0;
+}
+
+API int ref_midprice(const float* hi, const float* lo, float* out, int n, int p) {
+  std::vector<double> h = widen(hi, n), l = widen(lo, n), o(n);
+  Arena a = make_arena(n);
+  int beg = 0, nb = 0;
+  int rc = TA_MIDPRICE(a, 0, n - 1, h.data(), l.data(), p, &beg, &nb, o.data());
+  if (rc) return rc;
+  place(out, n, beg, nb, o.data());
+  return 0;
+}

@@ -265,6 +265,24 @@ for _n in ("ema", "rsi"):
     _sig(f"ctx_{_n}_alpha_fwd_dp", P, P, P, c_int, c_int, c_int, P)
     _sig(f"ctx_{_n}_alpha_bwd_dp", P, P, P, P, P, c_int, c_int, c_int, P)
 _sig("ctx_alpha_ckpt_per_row", stream=False)
+# price transforms, momentum, window midpoints, linear regression
+for _n in ("momentum", "roc", "rocp", "rocr", "rocr100", "midpoint", "linearreg",
+           "linearreg_slope", "linearreg_intercept", "linearreg_angle", "tsf"):
+    _sig(f"ct_{_n}_d", P, P, c_int, c_int)
+    _sig(f"ct_{_n}", P, P, c_int, c_int)
+    _sig(f"ct_{_n}_batch_d", P, P, c_int, c_int, c_int, c_int)
+    _sig(f"ct_{_n}_batch", P, P, c_int, c_int, c_int, c_int)
+for _n in ("momentum", "roc", "rocp", "rocr", "rocr100"):
+    _sig(f"ct_{_n}_device", P, P, c_int, c_int)
+for _n, _k in (("avgprice", 4), ("medprice", 2), ("typprice", 3), ("wclprice", 3)):
+    for _f in (f"ct_{_n}_d", f"ct_{_n}"):
+        _sig(_f, *([P] * (_k + 1)), c_int)
+    for _f in (f"ct_{_n}_batch_d", f"ct_{_n}_batch"):
+        _sig(_f, *([P] * (_k + 1)), c_int, c_int, c_int)
+for _f in ("ct_midprice_d", "ct_midprice"):
+    _sig(_f, P, P, P, c_int, c_int)
+for _f in ("ct_midprice_batch_d", "ct_midprice_batch"):
+    _sig(_f, P, P, P, c_int, c_int, c_int, c_int)
 _sig("ctx_macd_bwd_d", P, P, P, P, c_int, c_int, c_int, c_int, c_int)
 _sig("ctx_bbands_bwd_d", P, P, P, P, P, c_int, c_int, c_int, c_float, c_float)
 

@@ -1,0 +1,543 @@
+// Batch-surface families beyond the fused pass, on one elementwise / window
+// kernel each: the price transforms (AVGPRICE, MEDPRICE, TYPPRICE,
+// WCLPRICE), the momentum family (MOMENTUM, ROC, ROCP, ROCR, ROCR100), the
+// window midpoints (MIDPRICE, MIDPOINT) and the linear-regression family
+// (LINEARREG, LINEARREG_SLOPE, LINEARREG_INTERCEPT, LINEARREG_ANGLE, TSF).
+// Reference entry points: include/tacuda.h:111-121, 212-227, 403-424,
+// 742-753, 992-1148, 1294-1312, 1334-1339, 1421-1428, 1511-1516, 1721-1727;
+// include/tacuda_device.h:171-183, 338-369, 447, 511.
+//
+// One thread per output over the series-major matrix (rows of seriesLen
+// bars), grid-stride.  Arithmetic is TA-Lib 0.8.3's in double where the
+// reference vendors the function (ta_ROC.c, ta_ROCP.c, ta_MEDPRICE.c,
+// ta_TYPPRICE.c, ta_MIDPRICE.c, ta_LINEARREG.c) and TA-Lib's definitions of
+// the others in the same style; window extrema follow the reference's
+// MIDPRICE.cu / MIDPOINT.cu (NaN samples skipped; a MIDPOINT window that
+// starts inside the row's leading NaN run is NaN).  Window sums are
+// evaluated directly per output in TA-Lib's priming order: for float
+// inputs of ordinary range every partial sum is exact, so they equal
+// TA-Lib's running sums bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "../../include/tacuda.h"
+#include "../../include/tacuda_device.h"
+#include "host.hpp"
+#include "hostio.hpp"
+
+namespace tca {
+namespace {
+
+constexpr int kTB = 256;  // threads per block
+
+__device__ __forceinline__ float qnan_f() { return __int_as_float(0x7fc00000); }
+
+dim3 grid_for(long long total) {
+  const long long b = (total + kTB - 1) / kTB;
+  return dim3((unsigned)(b < 65536 ? b : 65536));
+}
+
+// ------------------------------------------------------ price transforms --
+enum PriceOp { kAvgPrice, kMedPrice, kTypPrice, kWclPrice };
+
+template <int OP>
+__global__ __launch_bounds__(kTB) void k_price(const float* __restrict__ o,
+                                               const float* __restrict__ h,
+                                               const float* __restrict__ l,
+                                               const float* __restrict__ c,
+                                               float* __restrict__ y, long long total) {
+  for (long long i = (long long)blockIdx.x * kTB + threadIdx.x; i < total;
+       i += (long long)gridDim.x * kTB) {
+    double v;
+    if constexpr (OP == kAvgPrice) {
+      v = ((double)o[i] + (double)h[i] + (double)l[i] + (double)c[i]) / 4.0;
+    } else if constexpr (OP == kMedPrice) {
+      v = ((double)h[i] + (double)l[i]) / 2.0;  // ta_MEDPRICE.c:70
+    } else if constexpr (OP == kTypPrice) {
+      v = ((double)h[i] + (double)l[i] + (double)c[i]) / 3.0;  // ta_TYPPRICE.c:68
+    } else {
+      v = ((double)h[i] + (double)l[i] + (double)c[i] * 2.0) / 4.0;
+    }
+    y[i] = (float)v;
+  }
+}
+
+// -------------------------------------------------------------- momentum --
+enum MomOp { kMom, kRoc, kRocp, kRocr, kRocr100 };
+
+// y[t] = f(x[t], x[t-p]) from bar p of every row (NaN before); a zero
+// earlier price gives 0 for the ratios (ta_ROC.c:117-123, ta_ROCP.c:116-122).
+template <int OP>
+__global__ __launch_bounds__(kTB) void k_mom(const float* __restrict__ x, float* __restrict__ y,
+                                             int len, long long total, int p) {
+  for (long long i = (long long)blockIdx.x * kTB + threadIdx.x; i < total;
+       i += (long long)gridDim.x * kTB) {
+    const int t = (int)(i % len);
+    if (t < p) {
+      y[i] = qnan_f();
+      continue;
+    }
+    const double cur = (double)x[i], prev = (double)x[i - p];
+    double v;
+    if constexpr (OP == kMom) {
+      v = cur - prev;
+    } else if constexpr (OP == kRoc) {
+      v = prev != 0.0 ? (cur / prev - 1.0) * 100.0 : 0.0;
+    } else if constexpr (OP == kRocp) {
+      v = prev != 0.0 ? (cur - prev) / prev : 0.0;
+    } else if constexpr (OP == kRocr) {
+      v = prev != 0.0 ? cur / prev : 0.0;
+    } else {
+      v = prev != 0.0 ? (cur / prev) * 100.0 : 0.0;
+    }
+    y[i] = (float)v;
+  }
+}
+
+// ------------------------------------------------------ window midpoints --
+// (max + min) / 2 over the p bars ending at t: MIDPRICE of high and low,
+// MIDPOINT of one input (hi == lo).
+template <bool ONE>
+__global__ __launch_bounds__(kTB) void k_mid(const float* __restrict__ hi,
+                                             const float* __restrict__ lo, float* __restrict__ y,
+                                             int len, long long total, int p) {
+  for (long long i = (long long)blockIdx.x * kTB + threadIdx.x; i < total;
+       i += (long long)gridDim.x * kTB) {
+    const int t = (int)(i % len);
+    if (t < p - 1) {
+      y[i] = qnan_f();
+      continue;
+    }
+    const long long s = i - (p - 1);
+    float mx = hi[s], mn = lo[s];
+    for (int k = 1; k < p; ++k) {
+      mx = fmaxf(mx, hi[s + k]);
+      mn = fminf(mn, lo[s + k]);
+    }
+    if constexpr (ONE) {
+      // a window starting inside the row's leading NaN run is NaN
+      // (reference MIDPOINT.cu:28-33)
+      if (!__builtin_isnan(mx) && __builtin_isnan(hi[s])) {
+        const long long r0 = i - t;
+        bool lead = true;
+        for (long long q = s - 1; q >= r0 && lead; --q) lead = __builtin_isnan(hi[q]);
+        if (lead) {
+          y[i] = qnan_f();
+          continue;
+        }
+      }
+    }
+    y[i] = (float)(((double)mx + (double)mn) / 2.0);
+  }
+}
+
+// ------------------------------------------------------- linear regression --
+enum RegOut { kLinReg, kSlope, kIntercept, kAngle, kTsf };
+
+// The least-squares line over the p bars ending at t with TA-Lib's sums
+// (ta_LINEARREG.c: x the distance back from t, SumX and SumXSqr of 0..p-1,
+// the window primed oldest first), m = (p SumXY - SumX SumY) / Divisor,
+// b = (SumY - m SumX) / p; a one-bar window has slope 0 (the reference's
+// denominator-zero rule, include/indicators/detail/LinearRegCommon.cuh:35-40).
+template <int OUT>
+__global__ __launch_bounds__(kTB) void k_linreg(const float* __restrict__ x,
+                                                float* __restrict__ y, int len, long long total,
+                                                int p, double sumX, double divisor) {
+  for (long long i = (long long)blockIdx.x * kTB + threadIdx.x; i < total;
+       i += (long long)gridDim.x * kTB) {
+    const int t = (int)(i % len);
+    if (t < p - 1) {
+      y[i] = qnan_f();
+      continue;
+    }
+    double sumY = 0.0, sumXY = 0.0;
+    for (int k = p - 1; k >= 0; --k) {
+      const double v = (double)x[i - k];
+      sumY += v;
+      sumXY += (double)k * v;
+    }
+    double m, b;
+    if (divisor == 0.0) {
+      m = 0.0;
+      b = sumY / (double)p;
+    } else {
+      m = ((double)p * sumXY - sumX * sumY) / divisor;
+      b = (sumY - m * sumX) / (double)p;
+    }
+    double v;
+    if constexpr (OUT == kLinReg) v = fma(m, (double)(p - 1), b);
+    else if constexpr (OUT == kSlope) v = m;
+    else if constexpr (OUT == kIntercept) v = b;
+    else if constexpr (OUT == kAngle) v = atan(m) * (180.0 / 3.14159265358979323846);
+    else v = b + m * (double)p;
+    y[i] = (float)v;
+  }
+}
+
+// -------------------------------------------------------------- dispatch --
+ctStatus_t shape_ok(const char* fn, int len, int rows) {
+  if (len <= 0) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: size/seriesLen must be positive", fn);
+  if (rows <= 0) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: numSeries must be positive", fn);
+  return CT_STATUS_SUCCESS;
+}
+
+// period > (or >= for the momentum family) the row length: success with an
+// all-NaN line (reference ROC.cu / MIDPRICE.cu / LINEARREG.cu, TA-Lib's
+// zero-count convention)
+__global__ void k_fill_nan(float* y, long long total) {
+  for (long long i = (long long)blockIdx.x * kTB + threadIdx.x; i < total;
+       i += (long long)gridDim.x * kTB)
+    y[i] = __int_as_float(0x7fc00000);
+}
+
+ctStatus_t all_nan(float* y, long long total, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_nan, grid_for(total), dim3(kTB), 0, s, y, total);
+  TCA_HIP(hipGetLastError());
+  return CT_STATUS_SUCCESS;
+}
+
+template <int OP>
+ctStatus_t run_price(const char* fn, const float* o, const float* h, const float* l,
+                     const float* c, float* y, int len, int rows, hipStream_t s) {
+  ctStatus_t r = shape_ok(fn, len, rows);
+  if (r != CT_STATUS_SUCCESS) return r;
+  const bool needO = OP == kAvgPrice, needC = OP != kMedPrice;
+  if (!h || !l || !y || (needO && !o) || (needC && !c))
+    return fail(CT_STATUS_INVALID_ARGUMENT, "%s: null pointer", fn);
+  const long long total = (long long)len * rows;
+  hipLaunchKernelGGL(k_price<OP>, grid_for(total), dim3(kTB), 0, s, o, h, l, c, y, total);
+  TCA_HIP(hipGetLastError());
+  return CT_STATUS_SUCCESS;
+}
+
+template <int OP>
+ctStatus_t run_mom(const char* fn, const float* x, float* y, int len, int rows, int p,
+                   hipStream_t s) {
+  ctStatus_t r = shape_ok(fn, len, rows);
+  if (r != CT_STATUS_SUCCESS) return r;
+  if (!x || !y) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: null pointer", fn);
+  if (p <= 0) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: invalid period", fn);
+  const long long total = (long long)len * rows;
+  if (p >= len) return all_nan(y, total, s);
+  hipLaunchKernelGGL(k_mom<OP>, grid_for(total), dim3(kTB), 0, s, x, y, len, total, p);
+  TCA_HIP(hipGetLastError());
+  return CT_STATUS_SUCCESS;
+}
+
+template <bool ONE>
+ctStatus_t run_mid(const char* fn, const float* hi, const float* lo, float* y, int len,
+                   int rows, int p, hipStream_t s) {
+  ctStatus_t r = shape_ok(fn, len, rows);
+  if (r != CT_STATUS_SUCCESS) return r;
+  if (!hi || !lo || !y) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: null pointer", fn);
+  if (p <= 0) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: invalid period", fn);
+  const long long total = (long long)len * rows;
+  if (p > len) return all_nan(y, total, s);
+  hipLaunchKernelGGL(k_mid<ONE>, grid_for(total), dim3(kTB), 0, s, hi, lo, y, len, total, p);
+  TCA_HIP(hipGetLastError());
+  return CT_STATUS_SUCCESS;
+}
+
+template <int OUT>
+ctStatus_t run_linreg(const char* fn, const float* x, float* y, int len, int rows, int p,
+                      hipStream_t s) {
+  ctStatus_t r = shape_ok(fn, len, rows);
+  if (r != CT_STATUS_SUCCESS) return r;
+  if (!x || !y) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: null pointer", fn);
+  if (p <= 0) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: invalid period", fn);
+  const long long total = (long long)len * rows;
+  if (p > len) return all_nan(y, total, s);
+  // ta_LINEARREG.c: SumX, SumXSqr and Divisor of x = 0..p-1
+  const double sumX = (double)p * (p - 1) * 0.5;
+  const double sumXSqr = (double)p * (p - 1) * (2 * p - 1) / 6.0;
+  const double divisor = sumX * sumX - p * sumXSqr;
+  hipLaunchKernelGGL(k_linreg<OUT>, grid_for(total), dim3(kTB), 0, s, x, y, len, total, p, sumX,
+                     divisor);
+  TCA_HIP(hipGetLastError());
+  return CT_STATUS_SUCCESS;
+}
+
+size_t elems(int len, int rows) {
+  return (len > 0 && rows > 0) ? (size_t)len * (size_t)rows : 0;
+}
+
+}  // namespace
+}  // namespace tca
+
+using namespace tca;
+
+#define T_RET(x)                                \
+  do {                                          \
+    const ctStatus_t r_ = (x);                  \
+    if (r_ != CT_STATUS_SUCCESS) return r_;     \
+  } while (0)
+
+// One-input, one-period families: device, device alias, batch device, host
+// and host batch forms.
+#define TCA_X1P(NAME, RUN)                                                                        \
+  extern "C" {                                                                                    \
+  CTAPI_EXPORT ctStatus_t ct_##NAME##_d(const float* x, float* y, int n, int p, hipStream_t s) {  \
+    return guarded([&] { return RUN("ct_" #NAME, x, y, n, 1, p, s); });                           \
+  }                                                                                               \
+  CTAPI_EXPORT ctStatus_t ct_##NAME##_batch_d(const float* x, float* y, int len, int rows, int p, \
+                                              int, hipStream_t s) {                               \
+    return guarded([&]() -> ctStatus_t {                                                          \
+      T_RET(RUN("ct_" #NAME "_batch_d", x, y, len, rows, p, s));                                  \
+      return finish_batch(s);                                                                     \
+    });                                                                                           \
+  }                                                                                               \
+  CTAPI_EXPORT ctStatus_t ct_##NAME(const float* hx, float* hy, int n, int p, hipStream_t s) {    \
+    return guarded([&]() -> ctStatus_t {                                                          \
+      T_RET(shape_ok("ct_" #NAME, n, 1));                                                         \
+      if (!hx || !hy) return fail(CT_STATUS_INVALID_ARGUMENT, "ct_" #NAME ": null pointer");      \
+      HostRun hr(s);                                                                              \
+      const float* x;                                                                             \
+      float* y;                                                                                   \
+      T_RET(hr.in(hx, elems(n, 1), &x));                                                          \
+      T_RET(hr.out(hy, elems(n, 1), &y));                                                         \
+      T_RET(RUN("ct_" #NAME, x, y, n, 1, p, s));                                                  \
+      return hr.finish();                                                                         \
+    });                                                                                           \
+  }                                                                                               \
+  CTAPI_EXPORT ctStatus_t ct_##NAME##_batch(const float* hx, float* hy, int len, int rows, int p, \
+                                            int, hipStream_t s) {                                 \
+    return guarded([&]() -> ctStatus_t {                                                          \
+      T_RET(shape_ok("ct_" #NAME "_batch", len, rows));                                           \
+      if (!hx || !hy) return fail(CT_STATUS_INVALID_ARGUMENT, "ct_" #NAME "_batch: null pointer"); \
+      HostRun hr(s);                                                                              \
+      const float* x;                                                                             \
+      float* y;                                                                                   \
+      T_RET(hr.in(hx, elems(len, rows), &x));                                                     \
+      T_RET(hr.out(hy, elems(len, rows), &y));                                                    \
+      T_RET(RUN("ct_" #NAME "_batch", x, y, len, rows, p, s));                                    \
+      return hr.finish();                                                                         \
+    });                                                                                           \
+  }                                                                                               \
+  }
+
+#define TCA_DEVICE_ALIAS(NAME)                                                                  \
+  extern "C" CTAPI_EXPORT ctStatus_t ct_##NAME##_device(const float* x, float* y, int n, int p, \
+                                                        hipStream_t s) {                        \
+    return ct_##NAME##_d(x, y, n, p, s);                                                        \
+  }
+
+namespace {
+template <int OP>
+ctStatus_t mom(const char* fn, const float* x, float* y, int len, int rows, int p, hipStream_t s) {
+  return run_mom<OP>(fn, x, y, len, rows, p, s);
+}
+template <int OUT>
+ctStatus_t reg(const char* fn, const float* x, float* y, int len, int rows, int p, hipStream_t s) {
+  return run_linreg<OUT>(fn, x, y, len, rows, p, s);
+}
+ctStatus_t midpoint(const char* fn, const float* x, float* y, int len, int rows, int p,
+                    hipStream_t s) {
+  return run_mid<true>(fn, x, x, y, len, rows, p, s);
+}
+}  // namespace
+
+TCA_X1P(momentum, mom<kMom>)
+TCA_X1P(roc, mom<kRoc>)
+TCA_X1P(rocp, mom<kRocp>)
+TCA_X1P(rocr, mom<kRocr>)
+TCA_X1P(rocr100, mom<kRocr100>)
+TCA_DEVICE_ALIAS(momentum)
+TCA_DEVICE_ALIAS(roc)
+TCA_DEVICE_ALIAS(rocp)
+TCA_DEVICE_ALIAS(rocr)
+TCA_DEVICE_ALIAS(rocr100)
+TCA_X1P(midpoint, midpoint)
+TCA_X1P(linearreg, reg<kLinReg>)
+TCA_X1P(linearreg_slope, reg<kSlope>)
+TCA_X1P(linearreg_intercept, reg<kIntercept>)
+TCA_X1P(linearreg_angle, reg<kAngle>)
+TCA_X1P(tsf, reg<kTsf>)
+
+// ------------------------------------------------------ price transforms --
+extern "C" {
+
+CTAPI_EXPORT ctStatus_t ct_avgprice_d(const float* o, const float* h, const float* l,
+                                      const float* c, float* y, int n, hipStream_t s) {
+  return guarded([&] { return run_price<kAvgPrice>("ct_avgprice", o, h, l, c, y, n, 1, s); });
+}
+CTAPI_EXPORT ctStatus_t ct_avgprice_batch_d(const float* o, const float* h, const float* l,
+                                            const float* c, float* y, int len, int rows, int,
+                                            hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    T_RET(run_price<kAvgPrice>("ct_avgprice_batch_d", o, h, l, c, y, len, rows, s));
+    return finish_batch(s);
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_medprice_d(const float* h, const float* l, float* y, int n,
+                                      hipStream_t s) {
+  return guarded(
+      [&] { return run_price<kMedPrice>("ct_medprice", nullptr, h, l, nullptr, y, n, 1, s); });
+}
+CTAPI_EXPORT ctStatus_t ct_medprice_batch_d(const float* h, const float* l, float* y, int len,
+                                            int rows, int, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    T_RET(run_price<kMedPrice>("ct_medprice_batch_d", nullptr, h, l, nullptr, y, len, rows, s));
+    return finish_batch(s);
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_typprice_d(const float* h, const float* l, const float* c, float* y,
+                                      int n, hipStream_t s) {
+  return guarded([&] { return run_price<kTypPrice>("ct_typprice", nullptr, h, l, c, y, n, 1, s); });
+}
+CTAPI_EXPORT ctStatus_t ct_typprice_batch_d(const float* h, const float* l, const float* c,
+                                            float* y, int len, int rows, int, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    T_RET(run_price<kTypPrice>("ct_typprice_batch_d", nullptr, h, l, c, y, len, rows, s));
+    return finish_batch(s);
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_wclprice_d(const float* h, const float* l, const float* c, float* y,
+                                      int n, hipStream_t s) {
+  return guarded([&] { return run_price<kWclPrice>("ct_wclprice", nullptr, h, l, c, y, n, 1, s); });
+}
+CTAPI_EXPORT ctStatus_t ct_wclprice_batch_d(const float* h, const float* l, const float* c,
+                                            float* y, int len, int rows, int, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    T_RET(run_price<kWclPrice>("ct_wclprice_batch_d", nullptr, h, l, c, y, len, rows, s));
+    return finish_batch(s);
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_midprice_d(const float* h, const float* l, float* y, int n, int p,
+                                      hipStream_t s) {
+  return guarded([&] { return run_mid<false>("ct_midprice", h, l, y, n, 1, p, s); });
+}
+CTAPI_EXPORT ctStatus_t ct_midprice_batch_d(const float* h, const float* l, float* y, int len,
+                                            int rows, int p, int, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    T_RET(run_mid<false>("ct_midprice_batch_d", h, l, y, len, rows, p, s));
+    return finish_batch(s);
+  });
+}
+
+}  // extern "C"
+
+// Host forms of the multi-input families: the inputs through the pooled
+// staging of hostio, then the device form.
+namespace {
+template <class F>
+ctStatus_t host_multi(const char* fn, int len, int rows, int nin, const float* const* hin,
+                      float* hy, hipStream_t s, F&& body) {
+  T_RET(shape_ok(fn, len, rows));
+  for (int k = 0; k < nin; ++k)
+    if (!hin[k]) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: null pointer", fn);
+  if (!hy) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: null pointer", fn);
+  HostRun hr(s);
+  const float* d[4] = {nullptr, nullptr, nullptr, nullptr};
+  float* y;
+  for (int k = 0; k < nin; ++k) T_RET(hr.in(hin[k], elems(len, rows), &d[k]));
+  T_RET(hr.out(hy, elems(len, rows), &y));
+  T_RET(body(d, y));
+  return hr.finish();
+}
+}  // namespace
+
+extern "C" {
+
+CTAPI_EXPORT ctStatus_t ct_avgprice(const float* ho, const float* hh, const float* hl,
+                                    const float* hc, float* hy, int n, hipStream_t s) {
+  return guarded([&] {
+    const float* in[4] = {ho, hh, hl, hc};
+    return host_multi("ct_avgprice", n, 1, 4, in, hy, s, [&](const float* const* d, float* y) {
+      return run_price<kAvgPrice>("ct_avgprice", d[0], d[1], d[2], d[3], y, n, 1, s);
+    });
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_avgprice_batch(const float* ho, const float* hh, const float* hl,
+                                          const float* hc, float* hy, int len, int rows, int,
+                                          hipStream_t s) {
+  return guarded([&] {
+    const float* in[4] = {ho, hh, hl, hc};
+    return host_multi("ct_avgprice_batch", len, rows, 4, in, hy, s,
+                      [&](const float* const* d, float* y) {
+                        return run_price<kAvgPrice>("ct_avgprice_batch", d[0], d[1], d[2], d[3],
+                                                    y, len, rows, s);
+                      });
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_medprice(const float* hh, const float* hl, float* hy, int n,
+                                    hipStream_t s) {
+  return guarded([&] {
+    const float* in[2] = {hh, hl};
+    return host_multi("ct_medprice", n, 1, 2, in, hy, s, [&](const float* const* d, float* y) {
+      return run_price<kMedPrice>("ct_medprice", nullptr, d[0], d[1], nullptr, y, n, 1, s);
+    });
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_medprice_batch(const float* hh, const float* hl, float* hy, int len,
+                                          int rows, int, hipStream_t s) {
+  return guarded([&] {
+    const float* in[2] = {hh, hl};
+    return host_multi("ct_medprice_batch", len, rows, 2, in, hy, s,
+                      [&](const float* const* d, float* y) {
+                        return run_price<kMedPrice>("ct_medprice_batch", nullptr, d[0], d[1],
+                                                    nullptr, y, len, rows, s);
+                      });
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_typprice(const float* hh, const float* hl, const float* hc, float* hy,
+                                    int n, hipStream_t s) {
+  return guarded([&] {
+    const float* in[3] = {hh, hl, hc};
+    return host_multi("ct_typprice", n, 1, 3, in, hy, s, [&](const float* const* d, float* y) {
+      return run_price<kTypPrice>("ct_typprice", nullptr, d[0], d[1], d[2], y, n, 1, s);
+    });
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_typprice_batch(const float* hh, const float* hl, const float* hc,
+                                          float* hy, int len, int rows, int, hipStream_t s) {
+  return guarded([&] {
+    const float* in[3] = {hh, hl, hc};
+    return host_multi("ct_typprice_batch", len, rows, 3, in, hy, s,
+                      [&](const float* const* d, float* y) {
+                        return run_price<kTypPrice>("ct_typprice_batch", nullptr, d[0], d[1],
+                                                    d[2], y, len, rows, s);
+                      });
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_wclprice(const float* hh, const float* hl, const float* hc, float* hy,
+                                    int n, hipStream_t s) {
+  return guarded([&] {
+    const float* in[3] = {hh, hl, hc};
+    return host_multi("ct_wclprice", n, 1, 3, in, hy, s, [&](const float* const* d, float* y) {
+      return run_price<kWclPrice>("ct_wclprice", nullptr, d[0], d[1], d[2], y, n, 1, s);
+    });
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_wclprice_batch(const float* hh, const float* hl, const float* hc,
+                                          float* hy, int len, int rows, int, hipStream_t s) {
+  return guarded([&] {
+    const float* in[3] = {hh, hl, hc};
+    return host_multi("ct_wclprice_batch", len, rows, 3, in, hy, s,
+                      [&](const float* const* d, float* y) {
+                        return run_price<kWclPrice>("ct_wclprice_batch", nullptr, d[0], d[1],
+                                                    d[2], y, len, rows, s);
+                      });
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_midprice(const float* hh, const float* hl, float* hy, int n, int p,
+                                    hipStream_t s) {
+  return guarded([&] {
+    const float* in[2] = {hh, hl};
+    return host_multi("ct_midprice", n, 1, 2, in, hy, s, [&](const float* const* d, float* y) {
+      return run_mid<false>("ct_midprice", d[0], d[1], y, n, 1, p, s);
+    });
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_midprice_batch(const float* hh, const float* hl, float* hy, int len,
+                                          int rows, int p, int, hipStream_t s) {
+  return guarded([&] {
+    const float* in[2] = {hh, hl};
+    return host_multi("ct_midprice_batch", len, rows, 2, in, hy, s,
+                      [&](const float* const* d, float* y) {
+                        return run_mid<false>("ct_midprice_batch", d[0], d[1], y, len, rows, p, s);
+                      });
+  });
+}
+
+}  // extern "C"

@@ -12,12 +12,16 @@ import sys
 REF = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_abi.json")
 
-# indicator families of the hot path (every host, device, batch and typed
-# form the reference declares for them is taken)
+# indicator families the build provides (every host, device, batch and
+# typed form the reference declares for them is taken)
 FAMILIES = ["sma", "ema", "wma", "dema", "tema", "t3", "kama", "trima", "rsi", "macd",
             "macdext", "bbands", "atr", "natr", "trange", "adx", "dx", "plus_di", "minus_di",
             "plus_dm", "minus_dm", "stochastic", "willr", "cci", "aroon", "aroonosc", "stddev",
-            "var", "sum", "min", "max", "ma"]
+            "var", "sum", "min", "max", "ma",
+            # price transforms, momentum, window midpoints, linear regression
+            "avgprice", "medprice", "typprice", "wclprice", "momentum", "roc", "rocp", "rocr",
+            "rocr100", "midprice", "midpoint", "linearreg", "linearreg_slope",
+            "linearreg_intercept", "linearreg_angle", "tsf"]
 SUFFIXES = ["", "_d", "_device", "_batch", "_batch_d", "_ex", "_ex_d"]
 EXTRA = ["ct_status_string", "ct_get_last_error", "ct_version", "ct_device_alloc",
          "ct_device_alloc_ex", "ct_device_free", "ct_device_free_ex", "ct_device_upload",
