@@ -331,6 +331,51 @@ CTAPI_EXPORT ctStatus_t ct_wclprice_batch_d(const float* d_highs, const float* d
 CTAPI_EXPORT ctStatus_t ct_midprice_batch(const float* host_highs, const float* host_lows, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
 CTAPI_EXPORT ctStatus_t ct_midprice_batch_d(const float* d_highs, const float* d_lows, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
 
+/* ---------------------------------------------------------------------------
+ * Element-wise math transforms and operators (csrc/vecmath.hip): the
+ * float function of each element (device-library single precision for the
+ * transcendental ones; CEIL, FLOOR, SQRT and the operators exactly rounded).
+ * BOP (float, 0 on a zero range), IBS (double, 0.5 on a non-positive
+ * range), CHANGE (float x[t] - x[t-period]), AVGDEV (float mean absolute
+ * deviation, the reference's order) and MINMAX (NaN-skipping window
+ * extrema) in csrc/transforms.hip.  Reference lines cited per declaration.
+ * ------------------------------------------------------------------------- */
+/* :781 */ CTAPI_EXPORT ctStatus_t ct_acos(const float* host_input, float* host_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :783 */ CTAPI_EXPORT ctStatus_t ct_asin(const float* host_input, float* host_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :785 */ CTAPI_EXPORT ctStatus_t ct_atan(const float* host_input, float* host_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :787 */ CTAPI_EXPORT ctStatus_t ct_ceil(const float* host_input, float* host_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :789 */ CTAPI_EXPORT ctStatus_t ct_cos(const float* host_input, float* host_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :791 */ CTAPI_EXPORT ctStatus_t ct_cosh(const float* host_input, float* host_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :793 */ CTAPI_EXPORT ctStatus_t ct_exp(const float* host_input, float* host_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :795 */ CTAPI_EXPORT ctStatus_t ct_floor(const float* host_input, float* host_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :797 */ CTAPI_EXPORT ctStatus_t ct_ln(const float* host_input, float* host_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :799 */ CTAPI_EXPORT ctStatus_t ct_log10(const float* host_input, float* host_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :801 */ CTAPI_EXPORT ctStatus_t ct_sin(const float* host_input, float* host_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :803 */ CTAPI_EXPORT ctStatus_t ct_sinh(const float* host_input, float* host_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :805 */ CTAPI_EXPORT ctStatus_t ct_sqrt(const float* host_input, float* host_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :807 */ CTAPI_EXPORT ctStatus_t ct_tan(const float* host_input, float* host_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :809 */ CTAPI_EXPORT ctStatus_t ct_tanh(const float* host_input, float* host_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :811 */ CTAPI_EXPORT ctStatus_t ct_add(const float* host_input1, const float* host_input2, float* host_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :814 */ CTAPI_EXPORT ctStatus_t ct_sub(const float* host_input1, const float* host_input2, float* host_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :817 */ CTAPI_EXPORT ctStatus_t ct_mult(const float* host_input1, const float* host_input2, float* host_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :820 */ CTAPI_EXPORT ctStatus_t ct_div(const float* host_input1, const float* host_input2, float* host_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :429 */ CTAPI_EXPORT ctStatus_t ct_bop(const float* host_open, const float* host_high, const float* host_low, const float* host_close, float* host_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3732 */ CTAPI_EXPORT ctStatus_t ct_ibs(const float* host_high, const float* host_low, const float* host_close, float* host_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :113 */ CTAPI_EXPORT ctStatus_t ct_change(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :153 */ CTAPI_EXPORT ctStatus_t ct_avgdev(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :146 */ CTAPI_EXPORT ctStatus_t ct_minmax(const float* host_input, float* host_min, float* host_max, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :215 */ CTAPI_EXPORT ctStatus_t ct_change_device(const float* device_input, float* device_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1856 */ CTAPI_EXPORT ctStatus_t ct_bop_batch(const float* host_opens, const float* host_highs, const float* host_lows, const float* host_closes, float* host_outputs, int seriesLen, int numSeries, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1863 */ CTAPI_EXPORT ctStatus_t ct_bop_batch_d(const float* d_opens, const float* d_highs, const float* d_lows, const float* d_closes, float* d_outputs, int seriesLen, int numSeries, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3736 */ CTAPI_EXPORT ctStatus_t ct_ibs_batch(const float* host_highs, const float* host_lows, const float* host_closes, float* host_outputs, int seriesLen, int numSeries, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3742 */ CTAPI_EXPORT ctStatus_t ct_ibs_batch_d(const float* d_highs, const float* d_lows, const float* d_closes, float* d_outputs, int seriesLen, int numSeries, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1501 */ CTAPI_EXPORT ctStatus_t ct_change_batch(const float* host_inputs, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1506 */ CTAPI_EXPORT ctStatus_t ct_change_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1521 */ CTAPI_EXPORT ctStatus_t ct_avgdev_batch(const float* host_inputs, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1526 */ CTAPI_EXPORT ctStatus_t ct_avgdev_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1666 */ CTAPI_EXPORT ctStatus_t ct_minmax_batch(const float* host_inputs, float* host_mins, float* host_maxs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1671 */ CTAPI_EXPORT ctStatus_t ct_minmax_batch_d(const float* d_inputs, float* d_mins, float* d_maxs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+
 #ifdef __cplusplus
 }
 #endif

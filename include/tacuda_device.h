@@ -214,6 +214,30 @@ CT_CDL_DECL(bearish_engulfing)
 /* :345 */ CTAPI_EXPORT ctStatus_t ct_typprice_d(const float* d_high, const float* d_low, const float* d_close, float* d_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
 /* :349 */ CTAPI_EXPORT ctStatus_t ct_wclprice_d(const float* d_high, const float* d_low, const float* d_close, float* d_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
 /* :511 */ CTAPI_EXPORT ctStatus_t ct_midprice_d(const float* d_high, const float* d_low, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :399 */ CTAPI_EXPORT ctStatus_t ct_acos_d(const float* d_input, float* d_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :401 */ CTAPI_EXPORT ctStatus_t ct_asin_d(const float* d_input, float* d_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :403 */ CTAPI_EXPORT ctStatus_t ct_atan_d(const float* d_input, float* d_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :405 */ CTAPI_EXPORT ctStatus_t ct_ceil_d(const float* d_input, float* d_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :407 */ CTAPI_EXPORT ctStatus_t ct_cos_d(const float* d_input, float* d_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :409 */ CTAPI_EXPORT ctStatus_t ct_cosh_d(const float* d_input, float* d_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :411 */ CTAPI_EXPORT ctStatus_t ct_exp_d(const float* d_input, float* d_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :413 */ CTAPI_EXPORT ctStatus_t ct_floor_d(const float* d_input, float* d_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :415 */ CTAPI_EXPORT ctStatus_t ct_ln_d(const float* d_input, float* d_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :417 */ CTAPI_EXPORT ctStatus_t ct_log10_d(const float* d_input, float* d_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :419 */ CTAPI_EXPORT ctStatus_t ct_sin_d(const float* d_input, float* d_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :421 */ CTAPI_EXPORT ctStatus_t ct_sinh_d(const float* d_input, float* d_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :423 */ CTAPI_EXPORT ctStatus_t ct_sqrt_d(const float* d_input, float* d_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :425 */ CTAPI_EXPORT ctStatus_t ct_tan_d(const float* d_input, float* d_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :427 */ CTAPI_EXPORT ctStatus_t ct_tanh_d(const float* d_input, float* d_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :429 */ CTAPI_EXPORT ctStatus_t ct_add_d(const float* d_input1, const float* d_input2, float* d_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :431 */ CTAPI_EXPORT ctStatus_t ct_sub_d(const float* d_input1, const float* d_input2, float* d_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :433 */ CTAPI_EXPORT ctStatus_t ct_mult_d(const float* d_input1, const float* d_input2, float* d_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :435 */ CTAPI_EXPORT ctStatus_t ct_div_d(const float* d_input1, const float* d_input2, float* d_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :551 */ CTAPI_EXPORT ctStatus_t ct_bop_d(const float* d_open, const float* d_high, const float* d_low, const float* d_close, float* d_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1120 */ CTAPI_EXPORT ctStatus_t ct_ibs_d(const float* d_high, const float* d_low, const float* d_close, float* d_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :444 */ CTAPI_EXPORT ctStatus_t ct_change_d(const float* d_input, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :450 */ CTAPI_EXPORT ctStatus_t ct_avgdev_d(const float* d_input, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :495 */ CTAPI_EXPORT ctStatus_t ct_minmax_d(const float* d_input, float* d_min, float* d_max, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
 
 #ifdef __cplusplus
 }

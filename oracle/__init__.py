@@ -237,6 +237,49 @@ class _Family:
     def tsf(self, x, p):
         return _call(self._fn("tsf", 1, 1, [_I]), [x], 1, len(x), p)
 
+    # element-wise math: op index into MATH1 / MATH2
+    MATH1 = ("acos", "asin", "atan", "ceil", "cos", "cosh", "exp", "floor", "ln", "log10", "sin",
+             "sinh", "sqrt", "tan", "tanh")
+    MATH2 = ("add", "sub", "mult", "div")
+
+    def math1(self, name, x):
+        fn = getattr(self._get(), self._prefix + "math1")
+        fn.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+        fn.restype = ctypes.c_int
+        x = _f32(x)
+        out = np.empty(len(x), np.float32)
+        rc = fn(self.MATH1.index(name), _fp(x), _fp(out), len(x))
+        if rc != 0:
+            raise ValueError(f"math1 {name} returned status {rc}")
+        return out
+
+    def math2(self, name, a, b):
+        fn = getattr(self._get(), self._prefix + "math2")
+        fn.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                       ctypes.c_int]
+        fn.restype = ctypes.c_int
+        a, b = _f32(a), _f32(b)
+        out = np.empty(len(a), np.float32)
+        rc = fn(self.MATH2.index(name), _fp(a), _fp(b), _fp(out), len(a))
+        if rc != 0:
+            raise ValueError(f"math2 {name} returned status {rc}")
+        return out
+
+    def bop(self, o, h, l, c):
+        return _call(self._fn("bop", 4, 1, []), [o, h, l, c], 1, len(c))
+
+    def ibs(self, h, l, c):
+        return _call(self._fn("ibs", 3, 1, []), [h, l, c], 1, len(c))
+
+    def change(self, x, p):
+        return _call(self._fn("change", 1, 1, [_I]), [x], 1, len(x), p)
+
+    def avgdev(self, x, p):
+        return _call(self._fn("avgdev", 1, 1, [_I]), [x], 1, len(x), p)
+
+    def minmax(self, x, p):
+        return _call(self._fn("minmax", 1, 2, [_I]), [x], 2, len(x), p)
+
     def candle(self, name, o, h, l, c):
         """int32 signal of one of the 61 TA-Lib patterns (see candles.c)."""
         L = self._get()

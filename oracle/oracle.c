@@ -1363,3 +1363,123 @@ API int orc_linearreg_slope(const float* x, float* out, int n, int p) { return l
 API int orc_linearreg_intercept(const float* x, float* out, int n, int p) { return linreg_family(x, out, n, p, 2); }
 API int orc_linearreg_angle(const float* x, float* out, int n, int p) { return linreg_family(x, out, n, p, 3); }
 API int orc_tsf(const float* x, float* out, int n, int p) { return linreg_family(x, out, n, p, 4); }
+
+/* ------------------------------------------------ element-wise math -----
+ * Reference src/api/api_vector_math.cu:28-51: the float function of each
+ * element (acosf ... tanhf) and the four float operators.  Here each is the
+ * double function of the widened element rounded once to float: the
+ * correctly rounded value up to the rare case double's rounding lands on a
+ * float halfway point.  For CEIL, FLOOR, SQRT, +, -, * and / double's 53
+ * bits are at least 2 x 24 + 2, so the double result rounded to float IS
+ * the correctly rounded float result, which IEEE fixes bit for bit
+ * (pinned against numpy's float32 operators).  The transcendental ones are
+ * pinned against numpy's float64 functions and special values; the device
+ * library is a few ulp from them, the tolerance the tests state. */
+API int orc_math1(int op, const float* x, float* out, int n) {
+  if (!x || !out || n <= 0 || op < 0 || op > 14) return ORC_BAD;
+  for (int i = 0; i < n; ++i) {
+    const double v = x[i];
+    double r;
+    switch (op) {
+      case 0: r = acos(v); break;
+      case 1: r = asin(v); break;
+      case 2: r = atan(v); break;
+      case 3: r = ceil(v); break;
+      case 4: r = cos(v); break;
+      case 5: r = cosh(v); break;
+      case 6: r = exp(v); break;
+      case 7: r = floor(v); break;
+      case 8: r = log(v); break;
+      case 9: r = log10(v); break;
+      case 10: r = sin(v); break;
+      case 11: r = sinh(v); break;
+      case 12: r = sqrt(v); break;
+      case 13: r = tan(v); break;
+      default: r = tanh(v); break;
+    }
+    out[i] = (float)r;
+  }
+  return ORC_OK;
+}
+API int orc_math2(int op, const float* a, const float* b, float* out, int n) {
+  if (!a || !b || !out || n <= 0 || op < 0 || op > 3) return ORC_BAD;
+  for (int i = 0; i < n; ++i) {
+    const double u = a[i], v = b[i];
+    const double r = op == 0 ? u + v : op == 1 ? u - v : op == 2 ? u * v : u / v;
+    out[i] = (float)r;
+  }
+  return ORC_OK;
+}
+
+/* ------------------------------------------------------------ BOP, IBS ---
+ * BOP: reference BOP.cu:14-24, in float: d = high - low, 0 when d == 0,
+ * else (close - open) / d (pinned by integer bars whose quotients are exact
+ * binary fractions and the zero-range rule).  IBS: TA-Lib 0.8.3 ta_IBS
+ * (vendored; pinned bit-identical to _ref TA_IBS): in double, range =
+ * high - low, (close - low) / range, 0.5 when range <= 0. */
+API int orc_bop(const float* o, const float* h, const float* l, const float* c, float* out,
+                int n) {
+  if (!o || !h || !l || !c || !out || n <= 0) return ORC_BAD;
+  for (int i = 0; i < n; ++i) {
+    const float d = h[i] - l[i];
+    out[i] = d == 0.0f ? 0.0f : (c[i] - o[i]) / d;
+  }
+  return ORC_OK;
+}
+API int orc_ibs(const float* h, const float* l, const float* c, float* out, int n) {
+  if (!h || !l || !c || !out || n <= 0) return ORC_BAD;
+  for (int i = 0; i < n; ++i) {
+    const double range = (double)h[i] - (double)l[i];
+    double v = ((double)c[i] - (double)l[i]) / range;
+    if (range <= 0.0) v = 0.5;
+    out[i] = (float)v;
+  }
+  return ORC_OK;
+}
+
+/* ------------------------------------------------ CHANGE, AVGDEV, MINMAX ---
+ * CHANGE: reference Change.cu:24-29 and :37-42: x[t] - x[t-p] in float
+ * from bar p, period >= size all NaN (pinned against numpy float32).
+ * AVGDEV: reference AVGDEV.cu:18-33, in float: the window summed oldest
+ * first, mean = sum / p, |x - mean| summed oldest first, / p (pinned by
+ * windows whose float sums are exact, where the value is known in closed
+ * form, and a float32 brute force over short windows).  MINMAX: reference
+ * MINMAX.cu:17-31: fminf / fmaxf folded from the oldest bar (NaN skipped,
+ * an all-NaN window NaN; pinned against numpy nanmin / nanmax). */
+API int orc_change(const float* x, float* out, int n, int p) {
+  if (!x || !out || n <= 0 || p <= 0) return ORC_BAD;
+  fill_nan(out, n);
+  for (int t = p; t < n; ++t) out[t] = x[t] - x[t - p];
+  return ORC_OK;
+}
+API int orc_avgdev(const float* x, float* out, int n, int p) {
+  if (!x || !out || n <= 0 || p <= 0) return ORC_BAD;
+  fill_nan(out, n);
+  const float fp = (float)p;
+  for (int t = p - 1; t < n; ++t) {
+    const float* w = x + t - (p - 1);
+    float sum = 0.0f;
+    for (int k = 0; k < p; ++k) sum += w[k];
+    const float mean = sum / fp;
+    float dev = 0.0f;
+    for (int k = 0; k < p; ++k) dev += fabsf(w[k] - mean);
+    out[t] = dev / fp;
+  }
+  return ORC_OK;
+}
+API int orc_minmax(const float* x, float* mn, float* mx, int n, int p) {
+  if (!x || !mn || !mx || n <= 0 || p <= 0) return ORC_BAD;
+  fill_nan(mn, n);
+  fill_nan(mx, n);
+  for (int t = p - 1; t < n; ++t) {
+    const float* w = x + t - (p - 1);
+    float a = w[0], b = w[0];
+    for (int k = 1; k < p; ++k) {
+      a = fminf(a, w[k]);
+      b = fmaxf(b, w[k]);
+    }
+    mn[t] = a;
+    mx[t] = b;
+  }
+  return ORC_OK;
+}

@@ -219,3 +219,13 @@ API int ref_midprice(const float* hi, const float* lo, float* out, int n, int p)
   place(out, n, beg, nb, o.data());
   return 0;
 }
+
+API int ref_ibs(const float* hi, const float* lo, const float* cl, float* out, int n) {
+  std::vector<double> h = widen(hi, n), l = widen(lo, n), c = widen(cl, n), o(n);
+  Arena a = make_arena(n);
+  int beg = 0, nb = 0;
+  int rc = TA_IBS(a, 0, n - 1, h.data(), l.data(), c.data(), &beg, &nb, o.data());
+  if (rc) return rc;
+  place(out, n, beg, nb, o.data());
+  return 0;
+}

@@ -283,6 +283,31 @@ for _f in ("ct_midprice_d", "ct_midprice"):
     _sig(_f, P, P, P, c_int, c_int)
 for _f in ("ct_midprice_batch_d", "ct_midprice_batch"):
     _sig(_f, P, P, P, c_int, c_int, c_int, c_int)
+# element-wise math transforms and operators; BOP, IBS, CHANGE, AVGDEV, MINMAX
+MATH_UNARY = ("acos", "asin", "atan", "ceil", "cos", "cosh", "exp", "floor", "ln", "log10",
+              "sin", "sinh", "sqrt", "tan", "tanh")
+MATH_BINARY = ("add", "sub", "mult", "div")
+for _n in MATH_UNARY:
+    for _f in (f"ct_{_n}_d", f"ct_{_n}"):
+        _sig(_f, P, P, c_int)
+for _n in MATH_BINARY:
+    for _f in (f"ct_{_n}_d", f"ct_{_n}"):
+        _sig(_f, P, P, P, c_int)
+for _n in ("change", "avgdev"):
+    for _f in (f"ct_{_n}_d", f"ct_{_n}"):
+        _sig(_f, P, P, c_int, c_int)
+    for _f in (f"ct_{_n}_batch_d", f"ct_{_n}_batch"):
+        _sig(_f, P, P, c_int, c_int, c_int, c_int)
+_sig("ct_change_device", P, P, c_int, c_int)
+for _n, _k in (("bop", 4), ("ibs", 3)):
+    for _f in (f"ct_{_n}_d", f"ct_{_n}"):
+        _sig(_f, *([P] * (_k + 1)), c_int)
+    for _f in (f"ct_{_n}_batch_d", f"ct_{_n}_batch"):
+        _sig(_f, *([P] * (_k + 1)), c_int, c_int, c_int)
+for _f in ("ct_minmax_d", "ct_minmax"):
+    _sig(_f, P, P, P, c_int, c_int)
+for _f in ("ct_minmax_batch_d", "ct_minmax_batch"):
+    _sig(_f, P, P, P, c_int, c_int, c_int, c_int)
 _sig("ctx_macd_bwd_d", P, P, P, P, c_int, c_int, c_int, c_int, c_int)
 _sig("ctx_bbands_bwd_d", P, P, P, P, P, c_int, c_int, c_int, c_float, c_float)
 

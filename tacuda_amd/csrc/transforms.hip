@@ -64,7 +64,7 @@ __global__ __launch_bounds__(kTB) void k_price(const float* __restrict__ o,
 }
 
 // -------------------------------------------------------------- momentum --
-enum MomOp { kMom, kRoc, kRocp, kRocr, kRocr100 };
+enum MomOp { kMom, kRoc, kRocp, kRocr, kRocr100, kChange };
 
 // y[t] = f(x[t], x[t-p]) from bar p of every row (NaN before); a zero
 // earlier price gives 0 for the ratios (ta_ROC.c:117-123, ta_ROCP.c:116-122).
@@ -76,6 +76,10 @@ __global__ __launch_bounds__(kTB) void k_mom(const float* __restrict__ x, float*
     const int t = (int)(i % len);
     if (t < p) {
       y[i] = qnan_f();
+      continue;
+    }
+    if constexpr (OP == kChange) {
+      y[i] = x[i] - x[i - p];  // float difference (reference Change.cu:24-29)
       continue;
     }
     const double cur = (double)x[i], prev = (double)x[i - p];
@@ -129,6 +133,80 @@ __global__ __launch_bounds__(kTB) void k_mid(const float* __restrict__ hi,
       }
     }
     y[i] = (float)(((double)mx + (double)mn) / 2.0);
+  }
+}
+
+// ------------------------------------------------------- bar transforms --
+// BOP (close - open) / (high - low) in float, 0 on a zero range (reference
+// BOP.cu:14-24); IBS (close - low) / (high - low) in double, 0.5 when the
+// range is not positive (TA-Lib 0.8.3 ta_IBS).  No look-back.
+template <bool IBS>
+__global__ __launch_bounds__(kTB) void k_bar(const float* __restrict__ o,
+                                             const float* __restrict__ h,
+                                             const float* __restrict__ l,
+                                             const float* __restrict__ c, float* __restrict__ y,
+                                             long long total) {
+  for (long long i = (long long)blockIdx.x * kTB + threadIdx.x; i < total;
+       i += (long long)gridDim.x * kTB) {
+    if constexpr (IBS) {
+      const double range = (double)h[i] - (double)l[i];
+      double v = ((double)c[i] - (double)l[i]) / range;
+      if (range <= 0.0) v = 0.5;
+      y[i] = (float)v;
+    } else {
+      const float d = h[i] - l[i];
+      y[i] = d == 0.0f ? 0.0f : (c[i] - o[i]) / d;
+    }
+  }
+}
+
+// ---------------------------------------------------- window statistics --
+// AVGDEV: the mean absolute deviation from the window mean over the p bars
+// ending at t, in float with the reference's order (AVGDEV.cu:18-33: the
+// sum oldest first, mean = sum / p, deviations summed oldest first, / p).
+__global__ __launch_bounds__(kTB) void k_avgdev(const float* __restrict__ x,
+                                                float* __restrict__ y, int len, long long total,
+                                                int p) {
+  const float fp = (float)p;
+  for (long long i = (long long)blockIdx.x * kTB + threadIdx.x; i < total;
+       i += (long long)gridDim.x * kTB) {
+    const int t = (int)(i % len);
+    if (t < p - 1) {
+      y[i] = qnan_f();
+      continue;
+    }
+    const float* w = x + (i - (p - 1));
+    float sum = 0.0f;
+    for (int k = 0; k < p; ++k) sum += w[k];
+    const float mean = sum / fp;
+    float dev = 0.0f;
+    for (int k = 0; k < p; ++k) dev += fabsf(w[k] - mean);
+    y[i] = dev / fp;
+  }
+}
+
+// MINMAX: fminf / fmaxf folded from the oldest bar of the p-bar window (a
+// NaN sample is skipped; an all-NaN window gives NaN; reference
+// MINMAX.cu:17-31).
+__global__ __launch_bounds__(kTB) void k_minmax(const float* __restrict__ x,
+                                                float* __restrict__ mn, float* __restrict__ mx,
+                                                int len, long long total, int p) {
+  for (long long i = (long long)blockIdx.x * kTB + threadIdx.x; i < total;
+       i += (long long)gridDim.x * kTB) {
+    const int t = (int)(i % len);
+    if (t < p - 1) {
+      mn[i] = qnan_f();
+      mx[i] = qnan_f();
+      continue;
+    }
+    const float* w = x + (i - (p - 1));
+    float a = w[0], b = w[0];
+    for (int k = 1; k < p; ++k) {
+      a = fminf(a, w[k]);
+      b = fmaxf(b, w[k]);
+    }
+    mn[i] = a;
+    mx[i] = b;
   }
 }
 
@@ -258,6 +336,48 @@ ctStatus_t run_linreg(const char* fn, const float* x, float* y, int len, int row
   return CT_STATUS_SUCCESS;
 }
 
+template <bool IBS>
+ctStatus_t run_bar(const char* fn, const float* o, const float* h, const float* l, const float* c,
+                   float* y, int len, int rows, hipStream_t s) {
+  ctStatus_t r = shape_ok(fn, len, rows);
+  if (r != CT_STATUS_SUCCESS) return r;
+  if (!h || !l || !c || !y || (!IBS && !o))
+    return fail(CT_STATUS_INVALID_ARGUMENT, "%s: null pointer", fn);
+  const long long total = (long long)len * rows;
+  hipLaunchKernelGGL(k_bar<IBS>, grid_for(total), dim3(kTB), 0, s, o, h, l, c, y, total);
+  TCA_HIP(hipGetLastError());
+  return CT_STATUS_SUCCESS;
+}
+
+ctStatus_t run_avgdev(const char* fn, const float* x, float* y, int len, int rows, int p,
+                      hipStream_t s) {
+  ctStatus_t r = shape_ok(fn, len, rows);
+  if (r != CT_STATUS_SUCCESS) return r;
+  if (!x || !y) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: null pointer", fn);
+  if (p <= 0) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: invalid period", fn);
+  const long long total = (long long)len * rows;
+  if (p > len) return all_nan(y, total, s);
+  hipLaunchKernelGGL(k_avgdev, grid_for(total), dim3(kTB), 0, s, x, y, len, total, p);
+  TCA_HIP(hipGetLastError());
+  return CT_STATUS_SUCCESS;
+}
+
+ctStatus_t run_minmax(const char* fn, const float* x, float* mn, float* mx, int len, int rows,
+                      int p, hipStream_t s) {
+  ctStatus_t r = shape_ok(fn, len, rows);
+  if (r != CT_STATUS_SUCCESS) return r;
+  if (!x || !mn || !mx) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: null pointer", fn);
+  if (p <= 0) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: invalid period", fn);
+  const long long total = (long long)len * rows;
+  if (p > len) {
+    r = all_nan(mn, total, s);
+    return r != CT_STATUS_SUCCESS ? r : all_nan(mx, total, s);
+  }
+  hipLaunchKernelGGL(k_minmax, grid_for(total), dim3(kTB), 0, s, x, mn, mx, len, total, p);
+  TCA_HIP(hipGetLastError());
+  return CT_STATUS_SUCCESS;
+}
+
 size_t elems(int len, int rows) {
   return (len > 0 && rows > 0) ? (size_t)len * (size_t)rows : 0;
 }
@@ -336,6 +456,10 @@ ctStatus_t midpoint(const char* fn, const float* x, float* y, int len, int rows,
   return run_mid<true>(fn, x, x, y, len, rows, p, s);
 }
 }  // namespace
+
+TCA_X1P(change, mom<kChange>)
+TCA_DEVICE_ALIAS(change)
+TCA_X1P(avgdev, run_avgdev)
 
 TCA_X1P(momentum, mom<kMom>)
 TCA_X1P(roc, mom<kRoc>)
@@ -540,4 +664,113 @@ CTAPI_EXPORT ctStatus_t ct_midprice_batch(const float* hh, const float* hl, floa
   });
 }
 
+}  // extern "C"
+
+// ------------------------------------------- BOP, IBS and MINMAX forms --
+extern "C" {
+
+CTAPI_EXPORT ctStatus_t ct_bop_d(const float* o, const float* h, const float* l, const float* c,
+                                 float* y, int n, hipStream_t s) {
+  return guarded([&] { return run_bar<false>("ct_bop_d", o, h, l, c, y, n, 1, s); });
+}
+CTAPI_EXPORT ctStatus_t ct_bop_batch_d(const float* o, const float* h, const float* l,
+                                       const float* c, float* y, int len, int rows, int,
+                                       hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    T_RET(run_bar<false>("ct_bop_batch_d", o, h, l, c, y, len, rows, s));
+    return finish_batch(s);
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_bop(const float* ho, const float* hh, const float* hl, const float* hc,
+                               float* hy, int n, hipStream_t s) {
+  return guarded([&] {
+    const float* in[4] = {ho, hh, hl, hc};
+    return host_multi("ct_bop", n, 1, 4, in, hy, s, [&](const float* const* d, float* y) {
+      return run_bar<false>("ct_bop", d[0], d[1], d[2], d[3], y, n, 1, s);
+    });
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_bop_batch(const float* ho, const float* hh, const float* hl,
+                                     const float* hc, float* hy, int len, int rows, int,
+                                     hipStream_t s) {
+  return guarded([&] {
+    const float* in[4] = {ho, hh, hl, hc};
+    return host_multi("ct_bop_batch", len, rows, 4, in, hy, s,
+                      [&](const float* const* d, float* y) {
+                        return run_bar<false>("ct_bop_batch", d[0], d[1], d[2], d[3], y, len,
+                                              rows, s);
+                      });
+  });
+}
+
+CTAPI_EXPORT ctStatus_t ct_ibs_d(const float* h, const float* l, const float* c, float* y, int n,
+                                 hipStream_t s) {
+  return guarded([&] { return run_bar<true>("ct_ibs_d", nullptr, h, l, c, y, n, 1, s); });
+}
+CTAPI_EXPORT ctStatus_t ct_ibs_batch_d(const float* h, const float* l, const float* c, float* y,
+                                       int len, int rows, int, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    T_RET(run_bar<true>("ct_ibs_batch_d", nullptr, h, l, c, y, len, rows, s));
+    return finish_batch(s);
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_ibs(const float* hh, const float* hl, const float* hc, float* hy,
+                               int n, hipStream_t s) {
+  return guarded([&] {
+    const float* in[3] = {hh, hl, hc};
+    return host_multi("ct_ibs", n, 1, 3, in, hy, s, [&](const float* const* d, float* y) {
+      return run_bar<true>("ct_ibs", nullptr, d[0], d[1], d[2], y, n, 1, s);
+    });
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_ibs_batch(const float* hh, const float* hl, const float* hc,
+                                     float* hy, int len, int rows, int, hipStream_t s) {
+  return guarded([&] {
+    const float* in[3] = {hh, hl, hc};
+    return host_multi("ct_ibs_batch", len, rows, 3, in, hy, s,
+                      [&](const float* const* d, float* y) {
+                        return run_bar<true>("ct_ibs_batch", nullptr, d[0], d[1], d[2], y, len,
+                                             rows, s);
+                      });
+  });
+}
+
+CTAPI_EXPORT ctStatus_t ct_minmax_d(const float* x, float* mn, float* mx, int n, int p,
+                                    hipStream_t s) {
+  return guarded([&] { return run_minmax("ct_minmax_d", x, mn, mx, n, 1, p, s); });
+}
+CTAPI_EXPORT ctStatus_t ct_minmax_batch_d(const float* x, float* mn, float* mx, int len, int rows,
+                                          int p, int, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    T_RET(run_minmax("ct_minmax_batch_d", x, mn, mx, len, rows, p, s));
+    return finish_batch(s);
+  });
+}
+}  // extern "C"
+
+namespace {
+ctStatus_t host_minmax(const char* fn, const float* hx, float* hmn, float* hmx, int len, int rows,
+                       int p, hipStream_t s) {
+  T_RET(shape_ok(fn, len, rows));
+  if (!hx || !hmn || !hmx) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: null pointer", fn);
+  HostRun hr(s);
+  const float* x;
+  float *mn, *mx;
+  T_RET(hr.in(hx, elems(len, rows), &x));
+  T_RET(hr.out(hmn, elems(len, rows), &mn));
+  T_RET(hr.out(hmx, elems(len, rows), &mx));
+  T_RET(run_minmax(fn, x, mn, mx, len, rows, p, s));
+  return hr.finish();
+}
+}  // namespace
+
+extern "C" {
+CTAPI_EXPORT ctStatus_t ct_minmax(const float* hx, float* hmn, float* hmx, int n, int p,
+                                  hipStream_t s) {
+  return guarded([&] { return host_minmax("ct_minmax", hx, hmn, hmx, n, 1, p, s); });
+}
+CTAPI_EXPORT ctStatus_t ct_minmax_batch(const float* hx, float* hmn, float* hmx, int len, int rows,
+                                        int p, int, hipStream_t s) {
+  return guarded([&] { return host_minmax("ct_minmax_batch", hx, hmn, hmx, len, rows, p, s); });
+}
 }  // extern "C"

@@ -21,7 +21,11 @@ FAMILIES = ["sma", "ema", "wma", "dema", "tema", "t3", "kama", "trima", "rsi", "
             # price transforms, momentum, window midpoints, linear regression
             "avgprice", "medprice", "typprice", "wclprice", "momentum", "roc", "rocp", "rocr",
             "rocr100", "midprice", "midpoint", "linearreg", "linearreg_slope",
-            "linearreg_intercept", "linearreg_angle", "tsf"]
+            "linearreg_intercept", "linearreg_angle", "tsf",
+            # element-wise math, bar and window statistics
+            "acos", "asin", "atan", "ceil", "cos", "cosh", "exp", "floor", "ln", "log10", "sin",
+            "sinh", "sqrt", "tan", "tanh", "add", "sub", "mult", "div", "bop", "ibs", "change",
+            "avgdev", "minmax"]
 SUFFIXES = ["", "_d", "_device", "_batch", "_batch_d", "_ex", "_ex_d"]
 EXTRA = ["ct_status_string", "ct_get_last_error", "ct_version", "ct_device_alloc",
          "ct_device_alloc_ex", "ct_device_free", "ct_device_free_ex", "ct_device_upload",

@@ -1,0 +1,215 @@
+"""Element-wise math transforms and operators, BOP, IBS, CHANGE, AVGDEV and
+MINMAX on the GPU against the oracle: device calls on aligned and unaligned
+arrays of odd sizes, host calls, dense batches, period edges and argument
+errors.  Exactly rounded operations (CEIL, FLOOR, SQRT, ADD, SUB, MULT, DIV)
+and the float/double formulas of the bar and window families must match bit
+for bit; the transcendental functions are the device library's single
+precision ones and must lie within 4 float32 ulp of the oracle's (the
+double function rounded once, itself within 1 ulp of numpy's float64)."""
+import numpy as np
+import pytest
+
+from tests.parity import compare
+from tests.test_oracle_vecmath import EXACT1, NP1, _domain, ulps
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+MAX_ULP = 4
+
+
+def _dev(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def _check1(name, got, want):
+    if name in EXACT1:
+        compare(got, want, tol=0.0, min_exact=1.0, name=name)
+    else:
+        u = ulps(got, want)
+        assert u.max() <= MAX_ULP, (name, int(u.max()))
+
+
+@pytest.mark.parametrize("name", list(NP1))
+@pytest.mark.parametrize("n,off", [(4096 * 9 + 3, 0), (4099, 1), (1, 0), (7, 3)])
+def test_unary_device(dev, name, n, off):
+    import tacuda_amd as ta
+    from oracle import ORC
+
+    print(f"vecmath {name} n{n} off{off}", flush=True)
+    L = ta.lib()
+    x = _domain(name, n, seed=n)
+    base = _dev(np.concatenate([np.zeros(off, np.float32), x]), dev)
+    ybase = torch.full((n + off,), 7.0, device=dev)
+    xi, yi = base[off:], ybase[off:]
+    assert getattr(L, f"ct_{name}_d")(xi.data_ptr(), yi.data_ptr(), n, None) == 0
+    torch.cuda.synchronize()
+    _check1(name, yi.cpu().numpy(), ORC.math1(name, x))
+    assert (ybase[:off].cpu() == 7.0).all()  # nothing written before the array
+
+
+@pytest.mark.parametrize("name", ["add", "sub", "mult", "div"])
+@pytest.mark.parametrize("n,off", [(100003, 0), (4097, 2), (3, 1)])
+def test_binary_device(dev, name, n, off):
+    import tacuda_amd as ta
+    from oracle import ORC
+
+    print(f"vecmath {name} n{n} off{off}", flush=True)
+    L = ta.lib()
+    rng = np.random.default_rng(n)
+    a = (rng.standard_normal(n) * 10.0 ** rng.integers(-5, 5, n)).astype(np.float32)
+    b = (rng.standard_normal(n) * 10.0 ** rng.integers(-5, 5, n)).astype(np.float32)
+    b[::53] = 0.0
+    pad = np.zeros(off, np.float32)
+    A, B = _dev(np.concatenate([pad, a]), dev), _dev(np.concatenate([pad, b]), dev)
+    Y = torch.empty(n + off, device=dev)
+    assert getattr(L, f"ct_{name}_d")(A[off:].data_ptr(), B[off:].data_ptr(),
+                                      Y[off:].data_ptr(), n, None) == 0
+    torch.cuda.synchronize()
+    compare(Y[off:].cpu().numpy(), ORC.math2(name, a, b), tol=0.0, min_exact=1.0, name=name)
+
+
+@pytest.mark.parametrize("name", ["sqrt", "exp", "sin", "add", "div"])
+def test_host_forms(dev, name):
+    import tacuda_amd as ta
+    from oracle import ORC
+
+    print(f"vecmath host {name}", flush=True)
+    L = ta.lib()
+    n = 70001
+    if name in NP1:
+        x = _domain(name, n, seed=3)
+        y = np.empty(n, np.float32)
+        assert getattr(L, f"ct_{name}")(x.ctypes.data, y.ctypes.data, n, None) == 0
+        _check1(name, y, ORC.math1(name, x))
+    else:
+        rng = np.random.default_rng(4)
+        a = rng.standard_normal(n).astype(np.float32)
+        b = rng.standard_normal(n).astype(np.float32) + 3.0
+        y = np.empty(n, np.float32)
+        assert getattr(L, f"ct_{name}")(a.ctypes.data, b.ctypes.data, y.ctypes.data, n, None) == 0
+        compare(y, ORC.math2(name, a, b), tol=0.0, min_exact=1.0, name=name)
+
+
+def test_vecmath_argument_errors(dev):
+    import tacuda_amd as ta
+
+    print("vecmath errors", flush=True)
+    L = ta.lib()
+    x = torch.ones(8, device=dev)
+    assert L.ct_sin_d(x.data_ptr(), x.data_ptr(), 0, None) == 4
+    assert L.ct_sin_d(None, x.data_ptr(), 8, None) == 4
+    assert L.ct_add_d(x.data_ptr(), None, x.data_ptr(), 8, None) == 4
+    assert L.ct_exp(None, None, 8, None) == 4
+    # in place is allowed (element-wise)
+    assert L.ct_sqrt_d(x.data_ptr(), x.data_ptr(), 8, None) == 0
+    torch.cuda.synchronize()
+    assert (x.cpu() == 1.0).all()
+
+
+# ------------------------------------------- BOP, IBS, CHANGE, AVGDEV, MINMAX
+def _ohlc(rows, n, seed):
+    import tacuda_amd as ta
+
+    d = ta.synth_ohlcv(n, rows=rows, seed=seed)
+    d = {k: np.asarray(v, np.float32).reshape(rows, n) for k, v in d.items()}
+    d["high"][:, 5] = d["low"][:, 5]  # zero ranges
+    d["close"][:, 9] = np.nan
+    return d
+
+
+@pytest.mark.parametrize("name", ["bop", "ibs"])
+@pytest.mark.parametrize("rows,n", [(1, 5001), (9, 777)])
+def test_bar_families(dev, name, rows, n):
+    import tacuda_amd as ta
+    from oracle import ORC
+
+    print(f"bar {name} {rows}x{n}", flush=True)
+    L = ta.lib()
+    d = _ohlc(rows, n, seed=rows)
+    keys = ("open", "high", "low", "close") if name == "bop" else ("high", "low", "close")
+    ins = [_dev(d[k], dev) for k in keys]
+    y = torch.empty((rows, n), device=dev)
+    args = [t.data_ptr() for t in ins] + [y.data_ptr()]
+    if rows == 1:
+        assert getattr(L, f"ct_{name}_d")(*args, n, None) == 0
+    else:
+        assert getattr(L, f"ct_{name}_batch_d")(*args, n, rows, 4, None) == 0
+    torch.cuda.synchronize()
+    got = y.cpu().numpy()
+    for r in range(rows):
+        want = getattr(ORC, name)(*[d[k][r] for k in keys])
+        compare(got[r], want, tol=0.0, min_exact=1.0, name=f"{name} row {r}")
+    # host batch form equals the device batch
+    out = np.empty((rows, n), np.float32)
+    host = [np.ascontiguousarray(d[k]) for k in keys]
+    assert getattr(L, f"ct_{name}_batch")(*[a.ctypes.data for a in host], out.ctypes.data, n,
+                                          rows, 4, None) == 0
+    assert np.array_equal(np.nan_to_num(out), np.nan_to_num(got))
+
+
+@pytest.mark.parametrize("name", ["change", "avgdev", "minmax"])
+@pytest.mark.parametrize("rows,n,p", [(1, 20011, 14), (1, 300, 1), (12, 640, 30), (5, 40, 40)])
+def test_window_families(dev, name, rows, n, p):
+    import tacuda_amd as ta
+    from oracle import ORC
+
+    print(f"window {name} {rows}x{n} p{p}", flush=True)
+    L = ta.lib()
+    x = np.asarray(ta.synth_ohlcv(n, rows=rows, seed=p)["close"], np.float32).reshape(rows, n)
+    x = x.copy()
+    x[:, n // 3] = np.nan
+    xi = _dev(x, dev)
+    nout = 2 if name == "minmax" else 1
+    ys = [torch.empty((rows, n), device=dev) for _ in range(nout)]
+    outs = [y.data_ptr() for y in ys]
+    if rows == 1:
+        rc = getattr(L, f"ct_{name}_d")(xi.data_ptr(), *outs, n, p, None)
+    else:
+        rc = getattr(L, f"ct_{name}_batch_d")(xi.data_ptr(), *outs, n, rows, p, 4, None)
+    assert rc == 0, L.ct_get_last_error()
+    torch.cuda.synchronize()
+    got = [y.cpu().numpy() for y in ys]
+    for r in range(rows):
+        want = getattr(ORC, name)(x[r], p)
+        want = want if nout == 2 else (want,)
+        for k in range(nout):
+            compare(got[k][r], want[k], tol=0.0, min_exact=1.0, name=f"{name} row {r} out {k}")
+    # host forms: single (row 0) and batch
+    one = [np.empty(n, np.float32) for _ in range(nout)]
+    assert getattr(L, f"ct_{name}")(np.ascontiguousarray(x[0]).ctypes.data,
+                                    *[o.ctypes.data for o in one], n, p, None) == 0
+    for k in range(nout):
+        assert np.array_equal(np.nan_to_num(one[k]), np.nan_to_num(got[k][0]))
+    hb = [np.empty((rows, n), np.float32) for _ in range(nout)]
+    assert getattr(L, f"ct_{name}_batch")(x.ctypes.data, *[o.ctypes.data for o in hb], n, rows, p,
+                                          4, None) == 0
+    for k in range(nout):
+        assert np.array_equal(np.nan_to_num(hb[k]), np.nan_to_num(got[k]))
+
+
+def test_window_period_edges(dev):
+    import tacuda_amd as ta
+
+    print("window edges", flush=True)
+    L = ta.lib()
+    x = _dev(np.arange(1, 51, dtype=np.float32), dev)
+    y = torch.zeros_like(x)
+    z = torch.zeros_like(x)
+    # CHANGE: period >= size all NaN; AVGDEV / MINMAX: period > size
+    assert L.ct_change_d(x.data_ptr(), y.data_ptr(), 50, 50, None) == 0
+    torch.cuda.synchronize()
+    assert torch.isnan(y).all()
+    assert L.ct_change_device(x.data_ptr(), y.data_ptr(), 50, 3, None) == 0
+    torch.cuda.synchronize()
+    assert (y[3:].cpu() == 3.0).all() and torch.isnan(y[:3]).all()
+    assert L.ct_avgdev_d(x.data_ptr(), y.data_ptr(), 50, 51, None) == 0
+    assert L.ct_minmax_d(x.data_ptr(), y.data_ptr(), z.data_ptr(), 50, 51, None) == 0
+    torch.cuda.synchronize()
+    assert torch.isnan(y).all() and torch.isnan(z).all()
+    assert L.ct_avgdev_d(x.data_ptr(), y.data_ptr(), 50, 0, None) == 4
+    assert L.ct_minmax_d(x.data_ptr(), y.data_ptr(), None, 50, 5, None) == 4
+    assert L.ct_bop_d(x.data_ptr(), x.data_ptr(), x.data_ptr(), None, y.data_ptr(), 50, None) == 4
+    assert L.ct_ibs_batch_d(x.data_ptr(), x.data_ptr(), x.data_ptr(), y.data_ptr(), 50, 0, 1,
+                            None) == 4

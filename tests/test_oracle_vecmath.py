@@ -1,0 +1,184 @@
+"""Pins of the oracle's element-wise math, BOP, IBS, CHANGE, AVGDEV and MINMAX
+(oracle/oracle.c): numpy's float32 operators for the exactly rounded
+operations, numpy's float64 functions and special values for the
+transcendental ones, the vendored TA-Lib 0.8.3 TA_IBS (oracle/_ref) bit for
+bit, closed forms where float arithmetic is exact, and brute force over
+short windows."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import ORC, REF, ref
+
+needs_ref = pytest.mark.skipif(ref() is None, reason="reference TA-Lib host build not present")
+
+NP1 = {"acos": np.arccos, "asin": np.arcsin, "atan": np.arctan, "ceil": np.ceil, "cos": np.cos,
+       "cosh": np.cosh, "exp": np.exp, "floor": np.floor, "ln": np.log, "log10": np.log10,
+       "sin": np.sin, "sinh": np.sinh, "sqrt": np.sqrt, "tan": np.tan, "tanh": np.tanh}
+EXACT1 = ("ceil", "floor", "sqrt")
+
+
+def _domain(name, n=4000, seed=0):
+    rng = np.random.default_rng(seed)
+    if name in ("acos", "asin"):
+        x = rng.uniform(-1, 1, n)
+    elif name in ("ln", "log10", "sqrt"):
+        x = np.exp(rng.uniform(-20, 20, n))
+    elif name in ("cosh", "sinh", "exp"):
+        x = rng.uniform(-80, 80, n)
+    else:
+        x = rng.uniform(-200, 200, n)
+    return x.astype(np.float32)
+
+
+def ulps(a, b):
+    """Distance in float32 units in the last place (NaN/Inf positions must agree)."""
+    a = np.asarray(a, np.float32)
+    b = np.asarray(b, np.float32)
+    assert np.array_equal(np.isnan(a), np.isnan(b))
+    ok = np.isfinite(a) & np.isfinite(b)
+    assert np.array_equal(a[~ok & ~np.isnan(a)], b[~ok & ~np.isnan(b)])
+
+    def key(v):
+        i = v.view(np.int32).astype(np.int64)
+        return np.where(i < 0, -(i & 0x7FFFFFFF), i)
+
+    return np.abs(key(a[ok]) - key(b[ok]))
+
+
+@pytest.mark.parametrize("name", list(NP1))
+def test_math1_against_numpy(name):
+    x = _domain(name)
+    got = ORC.math1(name, x)
+    want = NP1[name](x.astype(np.float64)).astype(np.float32)
+    if name in EXACT1:
+        assert np.array_equal(got, want), name
+        # and numpy's own float32 operation gives the same bits
+        assert np.array_equal(got, NP1[name](x)), name
+    else:
+        assert ulps(got, want).max() <= 1, name
+
+
+def test_math1_special_values():
+    f = np.float32
+    assert ORC.math1("exp", np.array([0.0], f))[0] == 1.0
+    assert ORC.math1("ln", np.array([1.0], f))[0] == 0.0
+    assert ORC.math1("log10", np.array([1000.0], f))[0] == 3.0
+    assert ORC.math1("sqrt", np.array([4.0, 2.25], f)).tolist() == [2.0, 1.5]
+    assert ORC.math1("ceil", np.array([-1.5, 1.25], f)).tolist() == [-1.0, 2.0]
+    assert ORC.math1("floor", np.array([-1.5, 1.25], f)).tolist() == [-2.0, 1.0]
+    assert ORC.math1("acos", np.array([1.0], f))[0] == 0.0
+    assert ORC.math1("atan", np.array([np.inf], f))[0] == np.float32(math.pi / 2)
+    assert np.isnan(ORC.math1("ln", np.array([-1.0], f))[0])
+    assert np.isnan(ORC.math1("asin", np.array([2.0], f))[0])
+    assert ORC.math1("ln", np.array([0.0], f))[0] == -np.inf
+    # odd / even symmetry
+    x = _domain("sin", 500)
+    for name, sgn in (("sin", -1), ("tan", -1), ("sinh", -1), ("tanh", -1), ("atan", -1),
+                      ("cos", 1), ("cosh", 1)):
+        assert np.array_equal(ORC.math1(name, -x), sgn * ORC.math1(name, x)), name
+
+
+@pytest.mark.parametrize("name", ["add", "sub", "mult", "div"])
+def test_math2_bit_exact_against_numpy_float32(name):
+    rng = np.random.default_rng(1)
+    a = (rng.standard_normal(5000) * 10.0 ** rng.integers(-6, 6, 5000)).astype(np.float32)
+    b = (rng.standard_normal(5000) * 10.0 ** rng.integers(-6, 6, 5000)).astype(np.float32)
+    b[::97] = 0.0  # division by zero: signed infinities and NaN
+    a[::389] = 0.0
+    op = {"add": np.add, "sub": np.subtract, "mult": np.multiply, "div": np.divide}[name]
+    with np.errstate(all="ignore"):
+        want = op(a, b)
+    got = ORC.math2(name, a, b)
+    assert np.array_equal(np.isnan(got), np.isnan(want))
+    ok = ~np.isnan(want)
+    assert np.array_equal(got[ok].view(np.uint32), want[ok].view(np.uint32)), name
+
+
+def _bits(a, b, name):
+    assert np.array_equal(np.isnan(a), np.isnan(b)), f"{name}: NaN masks differ"
+    ok = ~np.isnan(a)
+    assert np.array_equal(a[ok].view(np.uint32), b[ok].view(np.uint32)), f"{name}: bits differ"
+
+
+@needs_ref
+@pytest.mark.parametrize("seed", [0, 3, 11])
+def test_ibs_matches_vendored_talib(seed):
+    import tacuda_amd as ta
+
+    d = ta.synth_ohlcv(3000, seed=seed)
+    h, l, c = d["high"].copy(), d["low"].copy(), d["close"].copy()
+    h[10] = l[10]  # a zero range: 0.5
+    h[20] = l[20] - 1.0  # a negative range: 0.5
+    _bits(ORC.ibs(h, l, c), REF.ibs(h, l, c), f"ibs seed {seed}")
+
+
+def test_ibs_and_bop_closed_forms():
+    rng = np.random.default_rng(2)
+    n = 600
+    lo = rng.integers(1, 500, n).astype(np.float32)
+    span = (2.0 ** rng.integers(0, 6, n)).astype(np.float32)  # power-of-two ranges
+    hi = lo + span
+    cl = lo + np.floor(rng.random(n) * span)
+    op = lo + np.floor(rng.random(n) * span)
+    assert np.array_equal(ORC.ibs(hi, lo, cl), ((cl - lo) / span).astype(np.float32))
+    assert np.array_equal(ORC.bop(op, hi, lo, cl), ((cl - op) / span).astype(np.float32))
+    # zero ranges
+    z = lo.copy()
+    assert np.array_equal(ORC.bop(op, z, lo, cl), np.zeros(n, np.float32))
+    assert np.array_equal(ORC.ibs(z, lo, cl), np.full(n, 0.5, np.float32))
+
+
+def test_change_against_numpy_float32():
+    import tacuda_amd as ta
+
+    x = ta.synth_ohlcv(2000, seed=4)["close"]
+    for p in (1, 5, 60):
+        got = ORC.change(x, p)
+        assert np.isnan(got[:p]).all()
+        assert np.array_equal(got[p:], x[p:] - x[:-p])
+    assert np.isnan(ORC.change(x, 2000)).all()
+
+
+def test_avgdev_closed_form_and_brute_force():
+    # integer data and a power-of-two period: every float sum, the mean and
+    # the deviations are exact, so the value is the exact mean deviation
+    rng = np.random.default_rng(5)
+    x = rng.integers(0, 1000, 700).astype(np.float32)
+    for p in (1, 4, 16):
+        w = np.lib.stride_tricks.sliding_window_view(x.astype(np.float64), p)
+        want = np.abs(w - w.mean(1, keepdims=True)).mean(1).astype(np.float32)
+        got = ORC.avgdev(x, p)
+        assert np.isnan(got[:p - 1]).all()
+        assert np.array_equal(got[p - 1:], want), p
+    # float32 brute force (oldest first) on prices
+    import tacuda_amd as ta
+
+    y = ta.synth_ohlcv(300, seed=6)["close"]
+    p = 7
+    got = ORC.avgdev(y, p)
+    for t in range(p - 1, 300):
+        s = np.float32(0)
+        for v in y[t - p + 1:t + 1]:
+            s = np.float32(s + v)
+        m = np.float32(s / np.float32(p))
+        dv = np.float32(0)
+        for v in y[t - p + 1:t + 1]:
+            dv = np.float32(dv + np.float32(abs(np.float32(v - m))))
+        assert got[t] == np.float32(dv / np.float32(p)), t
+
+
+@pytest.mark.filterwarnings("ignore:All-NaN slice")
+def test_minmax_against_numpy():
+    import tacuda_amd as ta
+
+    x = ta.synth_ohlcv(900, seed=7)["close"].copy()
+    x[100] = np.nan
+    x[400:410] = np.nan  # a run longer than p = 5: all-NaN windows
+    for p in (1, 5, 30):
+        mn, mx = ORC.minmax(x, p)
+        w = np.lib.stride_tricks.sliding_window_view(x, p)
+        assert np.isnan(mn[:p - 1]).all() and np.isnan(mx[:p - 1]).all()
+        _bits(mn[p - 1:], np.nanmin(w, axis=1), f"min {p}")
+        _bits(mx[p - 1:], np.nanmax(w, axis=1), f"max {p}")
