@@ -376,6 +376,69 @@ CTAPI_EXPORT ctStatus_t ct_midprice_batch_d(const float* d_highs, const float* d
 /* :1666 */ CTAPI_EXPORT ctStatus_t ct_minmax_batch(const float* host_inputs, float* host_mins, float* host_maxs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
 /* :1671 */ CTAPI_EXPORT ctStatus_t ct_minmax_batch_d(const float* d_inputs, float* d_mins, float* d_maxs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
 
+/* ---------------------------------------------------------------------------
+ * TA-Lib 0.8.3 row functions (csrc/rowfn.hip): CUMSUM, PVT, RMA, ZLEMA, EFI,
+ * VWMA, ER, VHF, DPO, CHOP, ADR, AO, QSTICK, DONCHIAN, MARKETFI, WAD, MCGD
+ * and RVOL.  Per row: the leading bars where any input is NaN are skipped,
+ * TA-Lib's routine runs in double over the rest, outputs are float with NaN
+ * before the first one.  Periods outside TA-Lib's ranges are
+ * CT_STATUS_INVALID_ARGUMENT; device calls on stream 0 synchronize.
+ * ------------------------------------------------------------------------- */
+/* :3491 */ CTAPI_EXPORT ctStatus_t ct_cumsum(const float* host_input, float* host_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :4072 */ CTAPI_EXPORT ctStatus_t ct_pvt(const float* host_close, const float* host_volume, float* host_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :4102 */ CTAPI_EXPORT ctStatus_t ct_rma(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :4595 */ CTAPI_EXPORT ctStatus_t ct_zlema(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3569 */ CTAPI_EXPORT ctStatus_t ct_efi(const float* host_close, const float* host_volume, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :4516 */ CTAPI_EXPORT ctStatus_t ct_vwma(const float* host_input, const float* host_volume, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3604 */ CTAPI_EXPORT ctStatus_t ct_er(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :4452 */ CTAPI_EXPORT ctStatus_t ct_vhf(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3557 */ CTAPI_EXPORT ctStatus_t ct_dpo(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3354 */ CTAPI_EXPORT ctStatus_t ct_chop(const float* host_high, const float* host_low, const float* host_close, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3221 */ CTAPI_EXPORT ctStatus_t ct_adr(const float* host_high, const float* host_low, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3250 */ CTAPI_EXPORT ctStatus_t ct_ao(const float* host_high, const float* host_low, float* host_output, int size, int fastPeriod, int slowPeriod, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :4086 */ CTAPI_EXPORT ctStatus_t ct_qstick(const float* host_open, const float* host_close, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3518 */ CTAPI_EXPORT ctStatus_t ct_donchian(const float* host_high, const float* host_low, float* host_realUpperBand, float* host_realMiddleBand, float* host_realLowerBand, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3921 */ CTAPI_EXPORT ctStatus_t ct_marketfi(const float* host_high, const float* host_low, const float* host_volume, float* host_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :4532 */ CTAPI_EXPORT ctStatus_t ct_wad(const float* host_high, const float* host_low, const float* host_close, float* host_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3954 */ CTAPI_EXPORT ctStatus_t ct_mcgd(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :4170 */ CTAPI_EXPORT ctStatus_t ct_rvol(const float* host_volume, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3494 */ CTAPI_EXPORT ctStatus_t ct_cumsum_batch(const float* host_inputs, float* host_outputs, int seriesLen, int numSeries, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3498 */ CTAPI_EXPORT ctStatus_t ct_cumsum_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :4076 */ CTAPI_EXPORT ctStatus_t ct_pvt_batch(const float* host_closes, const float* host_volumes, float* host_outputs, int seriesLen, int numSeries, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :4081 */ CTAPI_EXPORT ctStatus_t ct_pvt_batch_d(const float* d_closes, const float* d_volumes, float* d_outputs, int seriesLen, int numSeries, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :4105 */ CTAPI_EXPORT ctStatus_t ct_rma_batch(const float* host_inputs, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :4110 */ CTAPI_EXPORT ctStatus_t ct_rma_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :4598 */ CTAPI_EXPORT ctStatus_t ct_zlema_batch(const float* host_inputs, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :4603 */ CTAPI_EXPORT ctStatus_t ct_zlema_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3573 */ CTAPI_EXPORT ctStatus_t ct_efi_batch(const float* host_closes, const float* host_volumes, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3579 */ CTAPI_EXPORT ctStatus_t ct_efi_batch_d(const float* d_closes, const float* d_volumes, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :4520 */ CTAPI_EXPORT ctStatus_t ct_vwma_batch(const float* host_inputs, const float* host_volumes, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :4526 */ CTAPI_EXPORT ctStatus_t ct_vwma_batch_d(const float* d_inputs, const float* d_volumes, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3607 */ CTAPI_EXPORT ctStatus_t ct_er_batch(const float* host_inputs, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3612 */ CTAPI_EXPORT ctStatus_t ct_er_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :4455 */ CTAPI_EXPORT ctStatus_t ct_vhf_batch(const float* host_inputs, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :4460 */ CTAPI_EXPORT ctStatus_t ct_vhf_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3560 */ CTAPI_EXPORT ctStatus_t ct_dpo_batch(const float* host_inputs, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3565 */ CTAPI_EXPORT ctStatus_t ct_dpo_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3358 */ CTAPI_EXPORT ctStatus_t ct_chop_batch(const float* host_highs, const float* host_lows, const float* host_closes, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3365 */ CTAPI_EXPORT ctStatus_t ct_chop_batch_d(const float* d_highs, const float* d_lows, const float* d_closes, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3224 */ CTAPI_EXPORT ctStatus_t ct_adr_batch(const float* host_highs, const float* host_lows, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3230 */ CTAPI_EXPORT ctStatus_t ct_adr_batch_d(const float* d_highs, const float* d_lows, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3254 */ CTAPI_EXPORT ctStatus_t ct_ao_batch(const float* host_highs, const float* host_lows, float* host_outputs, int seriesLen, int numSeries, int fastPeriod, int slowPeriod, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3260 */ CTAPI_EXPORT ctStatus_t ct_ao_batch_d(const float* d_highs, const float* d_lows, float* d_outputs, int seriesLen, int numSeries, int fastPeriod, int slowPeriod, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :4090 */ CTAPI_EXPORT ctStatus_t ct_qstick_batch(const float* host_opens, const float* host_closes, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :4096 */ CTAPI_EXPORT ctStatus_t ct_qstick_batch_d(const float* d_opens, const float* d_closes, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3525 */ CTAPI_EXPORT ctStatus_t ct_donchian_batch(const float* host_highs, const float* host_lows, float* host_realUpperBand, float* host_realMiddleBand, float* host_realLowerBand, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3533 */ CTAPI_EXPORT ctStatus_t ct_donchian_batch_d(const float* d_highs, const float* d_lows, float* d_realUpperBand, float* d_realMiddleBand, float* d_realLowerBand, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3926 */ CTAPI_EXPORT ctStatus_t ct_marketfi_batch(const float* host_highs, const float* host_lows, const float* host_volumes, float* host_outputs, int seriesLen, int numSeries, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3932 */ CTAPI_EXPORT ctStatus_t ct_marketfi_batch_d(const float* d_highs, const float* d_lows, const float* d_volumes, float* d_outputs, int seriesLen, int numSeries, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :4536 */ CTAPI_EXPORT ctStatus_t ct_wad_batch(const float* host_highs, const float* host_lows, const float* host_closes, float* host_outputs, int seriesLen, int numSeries, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :4542 */ CTAPI_EXPORT ctStatus_t ct_wad_batch_d(const float* d_highs, const float* d_lows, const float* d_closes, float* d_outputs, int seriesLen, int numSeries, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3957 */ CTAPI_EXPORT ctStatus_t ct_mcgd_batch(const float* host_inputs, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3962 */ CTAPI_EXPORT ctStatus_t ct_mcgd_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :4173 */ CTAPI_EXPORT ctStatus_t ct_rvol_batch(const float* host_volumes, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :4178 */ CTAPI_EXPORT ctStatus_t ct_rvol_batch_d(const float* d_volumes, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+
 #ifdef __cplusplus
 }
 #endif

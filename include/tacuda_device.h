@@ -238,6 +238,24 @@ CT_CDL_DECL(bearish_engulfing)
 /* :444 */ CTAPI_EXPORT ctStatus_t ct_change_d(const float* d_input, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
 /* :450 */ CTAPI_EXPORT ctStatus_t ct_avgdev_d(const float* d_input, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
 /* :495 */ CTAPI_EXPORT ctStatus_t ct_minmax_d(const float* d_input, float* d_min, float* d_max, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1064 */ CTAPI_EXPORT ctStatus_t ct_cumsum_d(const float* d_input, float* d_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1206 */ CTAPI_EXPORT ctStatus_t ct_pvt_d(const float* d_close, const float* d_volume, float* d_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1212 */ CTAPI_EXPORT ctStatus_t ct_rma_d(const float* d_input, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1329 */ CTAPI_EXPORT ctStatus_t ct_zlema_d(const float* d_input, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1084 */ CTAPI_EXPORT ctStatus_t ct_efi_d(const float* d_close, const float* d_volume, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1311 */ CTAPI_EXPORT ctStatus_t ct_vwma_d(const float* d_input, const float* d_volume, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1091 */ CTAPI_EXPORT ctStatus_t ct_er_d(const float* d_input, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1297 */ CTAPI_EXPORT ctStatus_t ct_vhf_d(const float* d_input, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1081 */ CTAPI_EXPORT ctStatus_t ct_dpo_d(const float* d_input, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1033 */ CTAPI_EXPORT ctStatus_t ct_chop_d(const float* d_high, const float* d_low, const float* d_close, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :998 */ CTAPI_EXPORT ctStatus_t ct_adr_d(const float* d_high, const float* d_low, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1005 */ CTAPI_EXPORT ctStatus_t ct_ao_d(const float* d_high, const float* d_low, float* d_output, int size, int fastPeriod, int slowPeriod, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1209 */ CTAPI_EXPORT ctStatus_t ct_qstick_d(const float* d_open, const float* d_close, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1071 */ CTAPI_EXPORT ctStatus_t ct_donchian_d(const float* d_high, const float* d_low, float* d_realUpperBand, float* d_realMiddleBand, float* d_realLowerBand, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1169 */ CTAPI_EXPORT ctStatus_t ct_marketfi_d(const float* d_high, const float* d_low, const float* d_volume, float* d_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1314 */ CTAPI_EXPORT ctStatus_t ct_wad_d(const float* d_high, const float* d_low, const float* d_close, float* d_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1177 */ CTAPI_EXPORT ctStatus_t ct_mcgd_d(const float* d_input, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1229 */ CTAPI_EXPORT ctStatus_t ct_rvol_d(const float* d_volume, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
 
 #ifdef __cplusplus
 }

@@ -280,6 +280,32 @@ class _Family:
     def minmax(self, x, p):
         return _call(self._fn("minmax", 1, 2, [_I]), [x], 2, len(x), p)
 
+    # TA-Lib 0.8.3 row functions (orc_rows / ref_rows): name, inputs, outputs
+    ROWS = {"cumsum": (1, 1), "pvt": (2, 1), "rma": (1, 1), "zlema": (1, 1), "efi": (2, 1),
+            "vwma": (2, 1), "er": (1, 1), "vhf": (1, 1), "dpo": (1, 1), "chop": (3, 1),
+            "adr": (2, 1), "ao": (2, 1), "qstick": (2, 1), "donchian": (2, 3),
+            "marketfi": (3, 1), "wad": (3, 1), "mcgd": (1, 1), "rvol": (1, 1)}
+
+    def rows(self, name, ins, p0=0, p1=0):
+        """One row through the backend's leading-NaN rule; a tuple for
+        several outputs."""
+        nin, nout = self.ROWS[name]
+        assert len(ins) == nin
+        fn = getattr(self._get(), self._prefix + "rows")
+        fn.argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                       ctypes.c_int, ctypes.c_int]
+        fn.restype = ctypes.c_int
+        ins = [_f32(a) for a in ins]
+        n = len(ins[0])
+        outs = [np.empty(n, np.float32) for _ in range(nout)]
+        pin = (ctypes.c_void_p * 4)(*([a.ctypes.data for a in ins] + [None] * (4 - nin)))
+        pout = (ctypes.c_void_p * 4)(*([o.ctypes.data for o in outs] + [None] * (4 - nout)))
+        rc = fn(name.encode(), ctypes.cast(pin, ctypes.c_void_p), ctypes.cast(pout, ctypes.c_void_p),
+                n, int(p0), int(p1))
+        if rc != 0:
+            raise ValueError(f"rows {name} returned status {rc}")
+        return outs[0] if nout == 1 else tuple(outs)
+
     def candle(self, name, o, h, l, c):
         """int32 signal of one of the 61 TA-Lib patterns (see candles.c)."""
         L = self._get()

@@ -1483,3 +1483,437 @@ API int orc_minmax(const float* x, float* mn, float* mx, int n, int p) {
   }
   return ORC_OK;
 }
+
+/* ------------------------------------ TA-Lib 0.8.3 row functions ---------
+ * The reference runs these through its 0.8.3 formula backend
+ * (src/compat/Talib08.cu:47-103): per row, the leading bars where any input
+ * is NaN are skipped (keeping at least the last bar), the TA_* routine runs
+ * over the rest in double from index 0, and its outputs land as float at
+ * prefix + outBegIdx; everything else is NaN.  Each routine below restates
+ * the vendored ta_*.c loop of the same name (src/compat/talib08_formulas.inc)
+ * in its order of operations and is pinned bit-identical to _ref's TA_*
+ * (tests/test_oracle_rows.py).  Periods outside TA-Lib's ranges are invalid
+ * arguments; a series no longer than the look-back is all NaN. */
+typedef struct {
+  const double* x[4];
+  double* y[4];
+  int n;
+} RowsIO;
+
+static int rows_prefix(const float* const* in, int nin, int n) {
+  int f = 0;
+  for (; f < n - 1; ++f) {
+    int any = 0;
+    for (int j = 0; j < nin; ++j) any |= isnan(in[j][f]);
+    if (!any) break;
+  }
+  return f;
+}
+
+static int tafinite(double v) { return v <= 1.7976931348623157e308 && v >= -1.7976931348623157e308; }
+
+/* one routine: writes y[k][t] for t >= its look-back (NaN before) */
+typedef void (*RowRoutine)(const RowsIO* io, const int* par);
+
+static void r_cumsum(const RowsIO* io, const int* par) {
+  double total = 0.0;
+  for (int t = 0; t < io->n; ++t) {
+    total += io->x[0][t];
+    io->y[0][t] = total;
+  }
+}
+static void r_pvt(const RowsIO* io, const int* par) {
+  const double *c = io->x[0], *v = io->x[1];
+  double acc = 0.0, pc = c[0];
+  for (int t = 0; t < io->n; ++t) {
+    const double cc = c[t];
+    if (pc != 0.0) acc += (cc - pc) / pc * v[t];
+    io->y[0][t] = acc;
+    pc = cc;
+  }
+}
+static void r_rma(const RowsIO* io, const int* par) {
+  const int p = par[0], lb = p - 1, n = io->n;
+  if (n <= lb) return;
+  const double* x = io->x[0];
+  const double wb = (double)(p - 1) / (double)p, wa = 1.0 - wb;
+  double tot = 0.0;
+  for (int t = 0; t < p; ++t) tot += x[t];
+  double prev = tot / p;
+  io->y[0][lb] = prev;
+  for (int t = lb + 1; t < n; ++t) {
+    prev = fma(wb, prev, wa * x[t]);
+    io->y[0][t] = prev;
+  }
+}
+static void r_zlema(const RowsIO* io, const int* par) {
+  const int p = par[0], lag = (p - 1) / 2, lb = lag + p - 1, n = io->n;
+  const double* x = io->x[0];
+  if (n <= lb) return;
+  if (p == 1) {
+    for (int t = 0; t < n; ++t) io->y[0][t] = x[t];
+    return;
+  }
+  double beta, k;
+  ema_coef(p, &beta, &k);
+  int tr = 0, today = lag;
+  double tmp = 0.0;
+  for (int i = 0; i < p; ++i, ++today, ++tr) tmp += 2.0 * x[today] - x[tr];
+  double prev = tmp / p;
+  io->y[0][lb] = prev;
+  for (; today < n; ++today, ++tr) {
+    prev = fma(beta, prev, k * (2.0 * x[today] - x[tr]));
+    io->y[0][today] = prev;
+  }
+}
+static void r_efi(const RowsIO* io, const int* par) {
+  const int p = par[0], lb = p, n = io->n;
+  const double *c = io->x[0], *v = io->x[1];
+  if (n <= lb) return;
+  if (p == 1) {
+    for (int t = 1; t < n; ++t) io->y[0][t] = (c[t] - c[t - 1]) * v[t];
+    return;
+  }
+  double beta, k;
+  ema_coef(p, &beta, &k);
+  double pc = c[0], tmp = 0.0;
+  int today = 1;
+  for (int i = 0; i < p; ++i, ++today) {
+    const double f = (c[today] - pc) * v[today];
+    pc = c[today];
+    tmp += f;
+  }
+  double prev = tmp / p;
+  io->y[0][lb] = prev;
+  for (; today < n; ++today) {
+    const double f = (c[today] - pc) * v[today];
+    pc = c[today];
+    prev = fma(beta, prev, k * f);
+    io->y[0][today] = prev;
+  }
+}
+static void r_vwma(const RowsIO* io, const int* par) {
+  const int p = par[0], lb = p - 1, n = io->n;
+  const double *x = io->x[0], *v = io->x[1];
+  if (n <= lb) return;
+  if (p == 1) {
+    for (int t = 0; t < n; ++t) io->y[0][t] = x[t];
+    return;
+  }
+  double spv = 0.0, sv = 0.0;
+  int zc = 0, tr = 0, i = 0;
+  for (; i < lb; ++i) {
+    spv += x[i] * v[i];
+    sv += v[i];
+    zc += fabs(v[i]) <= 0.0;
+  }
+  for (; i < n; ++i, ++tr) {
+    spv += x[i] * v[i];
+    sv += v[i];
+    zc += fabs(v[i]) <= 0.0;
+    const double tv = v[tr], tpv = x[tr] * tv;
+    double a, b;
+    if (zc >= p) {
+      a = 0.0;
+      b = 0.0;
+      spv = 0.0 - tpv;
+      sv = 0.0 - tv;
+    } else {
+      a = spv;
+      b = sv;
+      spv -= tpv;
+      sv -= tv;
+    }
+    io->y[0][i] = a / (double)p / (b / (double)p);
+    zc -= fabs(tv) <= 0.0;
+  }
+}
+static void r_er(const RowsIO* io, const int* par) {
+  const int p = par[0], lb = p, n = io->n;
+  const double* x = io->x[0];
+  if (n <= lb) return;
+  double sum = 0.0;
+  int run = 0, today = 0, tr = 0;
+  for (int i = 0; i < p; ++i) {
+    double d = x[today];
+    ++today;
+    d -= x[today];
+    sum += fabs(d);
+    run = (run + 1) & ((fabs(d) <= 0.0 ? 0 : 1) - 1);
+  }
+  double cur = x[today], old = x[tr++];
+  double roc = cur - old, trailing = old;
+  if (sum <= 0.0 || sum <= roc) {
+    io->y[0][today] = 1.0;
+  } else {
+    double r = fabs(roc / sum);
+    io->y[0][today] = r > 1.0 ? 1.0 : r;
+  }
+  for (++today; today < n; ++today) {
+    cur = x[today];
+    old = x[tr++];
+    roc = cur - old;
+    sum -= fabs(trailing - old);
+    sum += fabs(cur - x[today - 1]);
+    run = (run + 1) & ((fabs(cur - x[today - 1]) <= 0.0 ? 0 : 1) - 1);
+    double out;
+    if (run >= p) {
+      run = p;
+      sum = 0.0;
+      out = 1.0;
+    } else if (sum <= 0.0 || sum <= roc) {
+      out = 1.0;
+    } else {
+      out = fabs(roc / sum);
+      if (out > 1.0) out = 1.0;
+    }
+    io->y[0][today] = out;
+    trailing = old;
+  }
+}
+static void r_vhf(const RowsIO* io, const int* par) {
+  const int p = par[0], n = io->n;
+  const double* x = io->x[0];
+  for (int t = p; t < n; ++t) {
+    double hi = x[t], lo = hi, prev = hi, chg = 0.0;
+    for (int i = p; i >= 0; --i) {
+      const double v = x[t - i];
+      if (i < p) {
+        chg += fabs(v - prev);
+        if (v > hi) hi = v;
+        if (v < lo) lo = v;
+      }
+      prev = v;
+    }
+    io->y[0][t] = chg > 0.0 ? (hi - lo) / chg : 0.0;
+  }
+}
+static void r_dpo(const RowsIO* io, const int* par) {
+  const int p = par[0], n = io->n;
+  const int lb = (p - 1) > (p / 2 + 1) ? (p - 1) : (p / 2 + 1);
+  const double* x = io->x[0];
+  if (n <= lb) return;
+  double tot = 0.0;
+  int tr = lb - (p - 1), disp = lb - (p / 2 + 1), i = tr;
+  for (; i < lb; ++i) tot += x[i];
+  for (; i < n; ++i) {
+    tot += x[i];
+    const double tmp = tot;
+    tot -= x[tr++];
+    io->y[0][i] = x[disp++] - tmp / (double)p;
+  }
+}
+static void r_chop(const RowsIO* io, const int* par) {
+  const int p = par[0], n = io->n;
+  const double *h = io->x[0], *l = io->x[1], *c = io->x[2];
+  const double lp = log10((double)p);
+  for (int t = p; t < n; ++t) {
+    double hi = h[t], lo = l[t], str = 0.0, pc = 0.0;
+    for (int i = p; i >= 0; --i) {
+      if (i < p) {
+        const double th = h[t - i], tl = l[t - i];
+        const double thi = pc > th ? pc : th, tlo = pc < tl ? pc : tl;
+        str += thi - tlo;
+        if (th > hi) hi = th;
+        if (tl < lo) lo = tl;
+      }
+      pc = c[t - i];
+    }
+    io->y[0][t] = (hi - lo <= 0.0 || str <= 0.0) ? 100.0 : 100.0 * (log10(str / (hi - lo)) / lp);
+  }
+}
+/* ADR (high - low) and QSTICK (close - open): the running SMA of a difference */
+static void r_sma_diff(const RowsIO* io, const int* par) {
+  const int p = par[0], lb = p - 1, n = io->n;
+  const double *a = io->x[0], *b = io->x[1]; /* value a - b */
+  if (n <= lb) return;
+  double tot = 0.0;
+  int tr = 0, i = 0;
+  if (p > 1)
+    for (; i < lb; ++i) tot += a[i] - b[i];
+  for (; i < n; ++i, ++tr) {
+    tot += a[i] - b[i];
+    const double tmp = tot;
+    tot -= a[tr] - b[tr];
+    io->y[0][i] = tmp / (double)p;
+  }
+}
+static void r_adr(const RowsIO* io, const int* par) { r_sma_diff(io, par); }
+static void r_qstick(const RowsIO* io, const int* par) {
+  RowsIO s = *io; /* inputs (open, close): close - open */
+  s.x[0] = io->x[1];
+  s.x[1] = io->x[0];
+  r_sma_diff(&s, par);
+}
+static void r_ao(const RowsIO* io, const int* par) {
+  const int f = par[0], s = par[1], n = io->n;
+  const int lb = (f > s ? f : s) - 1;
+  const double *h = io->x[0], *l = io->x[1];
+  if (n <= lb) return;
+  double sf = 0.0, ss = 0.0;
+  int tf = lb - (f - 1), ts = lb - (s - 1), i = 0;
+  for (; i < lb; ++i) {
+    const double m = (h[i] + l[i]) / 2.0;
+    if (i >= tf) sf += m;
+    if (i >= ts) ss += m;
+  }
+  for (; i < n; ++i, ++tf, ++ts) {
+    const double m = (h[i] + l[i]) / 2.0;
+    sf += m;
+    ss += m;
+    const double out = sf / (double)f - ss / (double)s;
+    sf -= (h[tf] + l[tf]) / 2.0;
+    ss -= (h[ts] + l[ts]) / 2.0;
+    io->y[0][i] = out;
+  }
+}
+static void r_donchian(const RowsIO* io, const int* par) {
+  const int p = par[0], lb = p - 1, n = io->n;
+  const double *h = io->x[0], *l = io->x[1];
+  int hiIdx = -1, loIdx = -1, tr = 0;
+  double hi = 0.0, lo = 0.0;
+  for (int t = lb; t < n; ++t, ++tr) {
+    if (hiIdx < tr) {
+      hiIdx = tr;
+      hi = h[tr];
+      for (int i = tr + 1; i <= t; ++i)
+        if (h[i] > hi) {
+          hiIdx = i;
+          hi = h[i];
+        }
+    } else if (h[t] >= hi) {
+      hiIdx = t;
+      hi = h[t];
+    }
+    if (loIdx < tr) {
+      loIdx = tr;
+      lo = l[tr];
+      for (int i = tr + 1; i <= t; ++i)
+        if (l[i] < lo) {
+          loIdx = i;
+          lo = l[i];
+        }
+    } else if (l[t] <= lo) {
+      loIdx = t;
+      lo = l[t];
+    }
+    io->y[0][t] = hi;
+    io->y[1][t] = (hi + lo) / 2.0;
+    io->y[2][t] = lo;
+  }
+}
+static void r_marketfi(const RowsIO* io, const int* par) {
+  const double *h = io->x[0], *l = io->x[1], *v = io->x[2];
+  for (int t = 0; t < io->n; ++t) io->y[0][t] = v[t] != 0.0 ? (h[t] - l[t]) / v[t] : 0.0;
+}
+static void r_wad(const RowsIO* io, const int* par) {
+  const double *h = io->x[0], *l = io->x[1], *c = io->x[2];
+  double sum = 0.0, pc = c[0];
+  for (int t = 0; t < io->n; ++t) {
+    const double cc = c[t];
+    if (cc > pc) sum += cc - (pc < l[t] ? pc : l[t]);
+    else if (cc < pc) sum += cc - (pc > h[t] ? pc : h[t]);
+    io->y[0][t] = sum;
+    pc = cc;
+  }
+}
+static void r_mcgd(const RowsIO* io, const int* par) {
+  const int p = par[0], lb = p - 1, n = io->n;
+  const double* x = io->x[0];
+  if (n <= lb) return;
+  const double per = (double)p;
+  double md = x[0];
+  for (int t = 1; t < n; ++t) {
+    const double r = x[t] / md;
+    const double nm = md + (x[t] - md) / (per * (r * r * (r * r)));
+    if (tafinite(nm)) md = nm;
+    if (t >= lb) io->y[0][t] = md;
+  }
+}
+static void r_rvol(const RowsIO* io, const int* par) {
+  const int p = par[0], n = io->n;
+  const double* v = io->x[0];
+  if (n <= p) return;
+  double tot = 0.0;
+  int run = 0, tr = 0, i = 0;
+  for (; i < p; ++i) {
+    tot += v[i];
+    run = (run + 1) & ((fabs(v[i]) <= 0.0 ? 0 : 1) - 1);
+  }
+  for (; i < n; ++i) {
+    const double base = tot / (double)p, tv = v[tr++], cv = v[i];
+    io->y[0][i] = cv / base;
+    tot -= tv;
+    tot += cv;
+    run = (run + 1) & ((fabs(cv) <= 0.0 ? 0 : 1) - 1);
+    if (run >= p) tot = 0.0;
+  }
+}
+
+typedef struct {
+  const char* name;
+  RowRoutine fn;
+  int nin, nout, npar;
+  int lo[2], hi[2]; /* valid period ranges */
+} RowSpec;
+
+static const RowSpec kRows[] = {
+    {"cumsum", r_cumsum, 1, 1, 0, {0, 0}, {0, 0}},
+    {"pvt", r_pvt, 2, 1, 0, {0, 0}, {0, 0}},
+    {"rma", r_rma, 1, 1, 1, {1, 0}, {100000, 0}},
+    {"zlema", r_zlema, 1, 1, 1, {1, 0}, {100000, 0}},
+    {"efi", r_efi, 2, 1, 1, {1, 0}, {100000, 0}},
+    {"vwma", r_vwma, 2, 1, 1, {1, 0}, {100000, 0}},
+    {"er", r_er, 1, 1, 1, {2, 0}, {100000, 0}},
+    {"vhf", r_vhf, 1, 1, 1, {2, 0}, {100000, 0}},
+    {"dpo", r_dpo, 1, 1, 1, {2, 0}, {100000, 0}},
+    {"chop", r_chop, 3, 1, 1, {2, 0}, {100000, 0}},
+    {"adr", r_adr, 2, 1, 1, {1, 0}, {100000, 0}},
+    {"ao", r_ao, 2, 1, 2, {2, 2}, {100000, 100000}},
+    {"qstick", r_qstick, 2, 1, 1, {1, 0}, {100000, 0}},
+    {"donchian", r_donchian, 2, 3, 1, {2, 0}, {100000, 0}},
+    {"marketfi", r_marketfi, 3, 1, 0, {0, 0}, {0, 0}},
+    {"wad", r_wad, 3, 1, 0, {0, 0}, {0, 0}},
+    {"mcgd", r_mcgd, 1, 1, 1, {2, 0}, {100000, 0}},
+    {"rvol", r_rvol, 1, 1, 1, {1, 0}, {100000, 0}},
+};
+
+/* orc_rows(name, inputs, outputs, n, p0, p1): one row, the backend's
+ * leading-NaN rule, float outputs. */
+API int orc_rows(const char* name, const float* const* in, float* const* out, int n, int p0,
+                 int p1) {
+  const RowSpec* s = NULL;
+  for (size_t i = 0; i < sizeof(kRows) / sizeof(kRows[0]); ++i)
+    if (!strcmp(kRows[i].name, name)) s = &kRows[i];
+  if (!s || n <= 0) return ORC_BAD;
+  const int par[2] = {p0, p1};
+  for (int k = 0; k < s->npar; ++k)
+    if (par[k] < s->lo[k] || par[k] > s->hi[k]) return ORC_BAD;
+  for (int j = 0; j < s->nin; ++j)
+    if (!in[j]) return ORC_BAD;
+  for (int k = 0; k < s->nout; ++k) {
+    if (!out[k]) return ORC_BAD;
+    fill_nan(out[k], n);
+  }
+  const int f = rows_prefix(in, s->nin, n);
+  const int m = n - f;
+  RowsIO io;
+  io.n = m;
+  double* buf = (double*)malloc(sizeof(double) * (size_t)m * (size_t)(s->nin + s->nout));
+  if (!buf) return ORC_BAD;
+  for (int j = 0; j < s->nin; ++j) {
+    double* d = buf + (size_t)j * m;
+    for (int t = 0; t < m; ++t) d[t] = in[j][f + t];
+    io.x[j] = d;
+  }
+  for (int k = 0; k < s->nout; ++k) {
+    double* d = buf + (size_t)(s->nin + k) * m;
+    for (int t = 0; t < m; ++t) d[t] = NAN;
+    io.y[k] = d;
+  }
+  s->fn(&io, par);
+  for (int k = 0; k < s->nout; ++k)
+    for (int t = 0; t < m; ++t) out[k][f + t] = (float)io.y[k][t];
+  free(buf);
+  return ORC_OK;
+}

@@ -14,6 +14,7 @@
 
 #include <cmath>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 using namespace tacuda::talib08::ref;
@@ -227,5 +228,71 @@ API int ref_ibs(const float* hi, const float* lo, const float* cl, float* out, i
   int rc = TA_IBS(a, 0, n - 1, h.data(), l.data(), c.data(), &beg, &nb, o.data());
   if (rc) return rc;
   place(out, n, beg, nb, o.data());
+  return 0;
+}
+
+// The reference's 0.8.3 backend around one TA_* call (src/compat/Talib08.cu
+// :72-99): skip the leading bars where any input is NaN (keeping the last
+// bar), run the routine from index 0 over the rest, place its outputs at
+// prefix + outBegIdx.  name selects the routine; p0/p1 are its periods.
+API int ref_rows(const char* name, const float* const* in, float* const* out, int n, int p0,
+                 int p1) {
+  struct Spec {
+    const char* name;
+    int nin, nout;
+  };
+  static const Spec specs[] = {{"cumsum", 1, 1}, {"pvt", 2, 1},      {"rma", 1, 1},
+                               {"zlema", 1, 1},  {"efi", 2, 1},      {"vwma", 2, 1},
+                               {"er", 1, 1},     {"vhf", 1, 1},      {"dpo", 1, 1},
+                               {"chop", 3, 1},   {"adr", 2, 1},      {"ao", 2, 1},
+                               {"qstick", 2, 1}, {"donchian", 2, 3}, {"marketfi", 3, 1},
+                               {"wad", 3, 1},    {"mcgd", 1, 1},     {"rvol", 1, 1}};
+  const Spec* s = nullptr;
+  for (const Spec& q : specs)
+    if (std::string(q.name) == name) s = &q;
+  if (!s || n <= 0) return 4;
+  for (int k = 0; k < s->nout; ++k)
+    for (int t = 0; t < n; ++t) out[k][t] = NAN;
+  int f = 0;
+  for (; f < n - 1; ++f) {
+    bool any = false;
+    for (int j = 0; j < s->nin; ++j) any |= std::isnan(in[j][f]);
+    if (!any) break;
+  }
+  const int m = n - f;
+  std::vector<std::vector<double>> x(s->nin), y(s->nout, std::vector<double>(m));
+  for (int j = 0; j < s->nin; ++j) x[j].assign(in[j] + f, in[j] + n);
+  Arena a = make_arena(m);
+  int beg = 0, nb = 0, rc = 0;
+  const std::string nm(name);
+  const int e = m - 1;
+  if (nm == "cumsum") rc = TA_CUMSUM(a, 0, e, x[0].data(), &beg, &nb, y[0].data());
+  else if (nm == "pvt") rc = TA_PVT(a, 0, e, x[0].data(), x[1].data(), &beg, &nb, y[0].data());
+  else if (nm == "rma") rc = TA_RMA(a, 0, e, x[0].data(), p0, &beg, &nb, y[0].data());
+  else if (nm == "zlema") rc = TA_ZLEMA(a, 0, e, x[0].data(), p0, &beg, &nb, y[0].data());
+  else if (nm == "efi") rc = TA_EFI(a, 0, e, x[0].data(), x[1].data(), p0, &beg, &nb, y[0].data());
+  else if (nm == "vwma") rc = TA_VWMA(a, 0, e, x[0].data(), x[1].data(), p0, &beg, &nb, y[0].data());
+  else if (nm == "er") rc = TA_ER(a, 0, e, x[0].data(), p0, &beg, &nb, y[0].data());
+  else if (nm == "vhf") rc = TA_VHF(a, 0, e, x[0].data(), p0, &beg, &nb, y[0].data());
+  else if (nm == "dpo") rc = TA_DPO(a, 0, e, x[0].data(), p0, &beg, &nb, y[0].data());
+  else if (nm == "chop")
+    rc = TA_CHOP(a, 0, e, x[0].data(), x[1].data(), x[2].data(), p0, &beg, &nb, y[0].data());
+  else if (nm == "adr") rc = TA_ADR(a, 0, e, x[0].data(), x[1].data(), p0, &beg, &nb, y[0].data());
+  else if (nm == "ao")
+    rc = TA_AO(a, 0, e, x[0].data(), x[1].data(), p0, p1, &beg, &nb, y[0].data());
+  else if (nm == "qstick")
+    rc = TA_QSTICK(a, 0, e, x[0].data(), x[1].data(), p0, &beg, &nb, y[0].data());
+  else if (nm == "donchian")
+    rc = TA_DONCHIAN(a, 0, e, x[0].data(), x[1].data(), p0, &beg, &nb, y[0].data(), y[1].data(),
+                     y[2].data());
+  else if (nm == "marketfi")
+    rc = TA_MARKETFI(a, 0, e, x[0].data(), x[1].data(), x[2].data(), &beg, &nb, y[0].data());
+  else if (nm == "wad")
+    rc = TA_WAD(a, 0, e, x[0].data(), x[1].data(), x[2].data(), &beg, &nb, y[0].data());
+  else if (nm == "mcgd") rc = TA_MCGD(a, 0, e, x[0].data(), p0, &beg, &nb, y[0].data());
+  else if (nm == "rvol") rc = TA_RVOL(a, 0, e, x[0].data(), p0, &beg, &nb, y[0].data());
+  if (rc) return 4;
+  for (int k = 0; k < s->nout; ++k)
+    for (int i = 0; i < nb; ++i) out[k][f + beg + i] = (float)y[k][i];
   return 0;
 }

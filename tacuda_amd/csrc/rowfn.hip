@@ -1,0 +1,628 @@
+// TA-Lib 0.8.3 functions the reference serves from its formula backend
+// (src/compat/Talib08.cu:47-103, one work-item per row running the TA_*
+// routine in double): CUMSUM, PVT, RMA, ZLEMA, EFI, VWMA, ER, VHF, DPO,
+// CHOP, ADR, AO, QSTICK, DONCHIAN, MARKETFI, WAD, MCGD and RVOL, each in
+// host, device, batch and host-batch forms.  Reference entry points:
+// include/tacuda.h (ct_<name> and ct_<name>_batch[_d], cited per
+// declaration), include/tacuda_device.h (ct_<name>_d).
+//
+// Semantics are the backend's: per row, the leading bars where any input is
+// NaN are skipped (the last bar is always kept), the routine runs in double
+// from the first remaining bar, outputs are rounded to float, and every bar
+// before the routine's first output is NaN.  A period outside TA-Lib's range
+// is CT_STATUS_INVALID_ARGUMENT; a row no longer than the look-back is all
+// NaN.
+//
+// These are sequential recurrences and running sums whose TA-Lib rounding
+// a parallel scan would not reproduce, so each row is one work-item, as in
+// the reference.  Few long rows get a wave each (lane 0 works, so the rows'
+// loads do not queue behind one another in one CU); otherwise 64 rows share
+// a wave, one per lane.  The state machines are
+// restated from the vendored ta_*.c routines (src/compat/talib08_formulas.inc)
+// in the same order of floating-point operations, with -ffp-contract=off
+// and fma() where TA-Lib writes it, so the outputs equal TA-Lib's bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "../../include/tacuda.h"
+#include "../../include/tacuda_device.h"
+#include "host.hpp"
+#include "hostio.hpp"
+
+namespace tca {
+namespace {
+
+enum RowFn {
+  kCumsum, kPvt, kRma, kZlema, kEfi, kVwma, kEr, kVhf, kDpo, kChop, kAdr, kAo, kQstick,
+  kDonchian, kMarketfi, kWad, kMcgd, kRvol
+};
+
+struct RowSpec {
+  int nin, nout, npar;
+  int lo[2], hi[2];
+};
+
+// inputs, outputs, periods and their valid ranges (TA-Lib's checks)
+constexpr RowSpec kSpec[] = {
+    {1, 1, 0, {0, 0}, {0, 0}},            // CUMSUM
+    {2, 1, 0, {0, 0}, {0, 0}},            // PVT (close, volume)
+    {1, 1, 1, {1, 0}, {100000, 0}},       // RMA
+    {1, 1, 1, {1, 0}, {100000, 0}},       // ZLEMA
+    {2, 1, 1, {1, 0}, {100000, 0}},       // EFI (close, volume)
+    {2, 1, 1, {1, 0}, {100000, 0}},       // VWMA (input, volume)
+    {1, 1, 1, {2, 0}, {100000, 0}},       // ER
+    {1, 1, 1, {2, 0}, {100000, 0}},       // VHF
+    {1, 1, 1, {2, 0}, {100000, 0}},       // DPO
+    {3, 1, 1, {2, 0}, {100000, 0}},       // CHOP (high, low, close)
+    {2, 1, 1, {1, 0}, {100000, 0}},       // ADR (high, low)
+    {2, 1, 2, {2, 2}, {100000, 100000}},  // AO (high, low; fast, slow)
+    {2, 1, 1, {1, 0}, {100000, 0}},       // QSTICK (open, close)
+    {2, 3, 1, {2, 0}, {100000, 0}},       // DONCHIAN (high, low -> upper, middle, lower)
+    {3, 1, 0, {0, 0}, {0, 0}},            // MARKETFI (high, low, volume)
+    {3, 1, 0, {0, 0}, {0, 0}},            // WAD (high, low, close)
+    {1, 1, 1, {2, 0}, {100000, 0}},       // MCGD
+    {1, 1, 1, {1, 0}, {100000, 0}},       // RVOL (volume)
+};
+
+struct RowArgs {
+  const float* in[3];
+  float* out[3];
+  int len, rows, p0, p1;
+};
+
+// One row's view: inputs and outputs offset to the first bar after the
+// leading-NaN prefix; n bars remain.
+struct Row {
+  const float* x[3];
+  float* y[3];
+  int n;
+  __device__ double in(int j, int t) const { return (double)x[j][t]; }
+  __device__ void put(int k, int t, double v) const { y[k][t] = (float)v; }
+};
+
+__device__ __forceinline__ bool finite_d(double v) {
+  return v <= 1.7976931348623157e308 && v >= -1.7976931348623157e308;
+}
+
+// EMA weights as TA-Lib forms them (ta_EMA.c): beta = (p-1)/(p+1) and k =
+// 1 - beta, with beta recomputed as 1 - k when below one half.
+__device__ __forceinline__ void ema_bk(int p, double& b, double& k) {
+  b = (double)(p - 1) / (double)(p + 1);
+  k = 1.0 - b;
+  if (b < 0.5) b = 1.0 - k;
+}
+
+// ------------------------------------------------------------ routines --
+// Each writes y[k][t] for t at or past its first output; the kernel has
+// already filled the row with NaN.
+
+__device__ void f_cumsum(const Row& r) {
+  double total = 0.0;
+  for (int t = 0; t < r.n; ++t) {
+    total += r.in(0, t);
+    r.put(0, t, total);
+  }
+}
+
+// ta_PVT: acc += (c - prev) / prev * volume while prev != 0
+__device__ void f_pvt(const Row& r) {
+  double acc = 0.0, prev = r.in(0, 0);
+  for (int t = 0; t < r.n; ++t) {
+    const double c = r.in(0, t);
+    if (prev != 0.0) acc += (c - prev) / prev * r.in(1, t);
+    r.put(0, t, acc);
+    prev = c;
+  }
+}
+
+// ta_RMA: SMA seed of the first p bars, then fma(beta, prev, alpha x)
+__device__ void f_rma(const Row& r, int p) {
+  if (r.n < p) return;
+  const double wb = (double)(p - 1) / (double)p, wa = 1.0 - wb;
+  double s = 0.0;
+  for (int t = 0; t < p; ++t) s += r.in(0, t);
+  double v = s / p;
+  r.put(0, p - 1, v);
+  for (int t = p; t < r.n; ++t) {
+    v = fma(wb, v, wa * r.in(0, t));
+    r.put(0, t, v);
+  }
+}
+
+// ta_ZLEMA: EMA of 2 x[t] - x[t - lag], lag = (p-1)/2, seeded by the mean of
+// its first p terms
+__device__ void f_zlema(const Row& r, int p) {
+  const int lag = (p - 1) / 2, first = lag + p - 1;
+  if (r.n <= first) return;
+  if (p == 1) {
+    for (int t = 0; t < r.n; ++t) r.put(0, t, r.in(0, t));
+    return;
+  }
+  double b, k;
+  ema_bk(p, b, k);
+  double s = 0.0;
+  for (int i = 0; i < p; ++i) s += 2.0 * r.in(0, lag + i) - r.in(0, i);
+  double v = s / p;
+  r.put(0, first, v);
+  for (int t = first + 1; t < r.n; ++t) {
+    v = fma(b, v, k * (2.0 * r.in(0, t) - r.in(0, t - lag)));
+    r.put(0, t, v);
+  }
+}
+
+// ta_EFI: EMA of the force (c[t] - c[t-1]) v[t], seeded by the mean of the
+// first p forces; p = 1 is the raw force
+__device__ void f_efi(const Row& r, int p) {
+  if (r.n <= p) return;
+  if (p == 1) {
+    for (int t = 1; t < r.n; ++t) r.put(0, t, (r.in(0, t) - r.in(0, t - 1)) * r.in(1, t));
+    return;
+  }
+  double b, k;
+  ema_bk(p, b, k);
+  double s = 0.0;
+  for (int t = 1; t <= p; ++t) s += (r.in(0, t) - r.in(0, t - 1)) * r.in(1, t);
+  double v = s / p;
+  r.put(0, p, v);
+  for (int t = p + 1; t < r.n; ++t) {
+    v = fma(b, v, k * ((r.in(0, t) - r.in(0, t - 1)) * r.in(1, t)));
+    r.put(0, t, v);
+  }
+}
+
+// ta_VWMA: running sums of price x volume and volume; a window of zero
+// volumes restarts the sums and gives 0/0
+__device__ void f_vwma(const Row& r, int p) {
+  if (r.n < p) return;
+  if (p == 1) {
+    for (int t = 0; t < r.n; ++t) r.put(0, t, r.in(0, t));
+    return;
+  }
+  double spv = 0.0, sv = 0.0;
+  int zeros = 0;
+  for (int t = 0; t < p - 1; ++t) {
+    const double v = r.in(1, t);
+    spv += r.in(0, t) * v;
+    sv += v;
+    zeros += fabs(v) <= 0.0;
+  }
+  for (int t = p - 1; t < r.n; ++t) {
+    const double v = r.in(1, t);
+    spv += r.in(0, t) * v;
+    sv += v;
+    zeros += fabs(v) <= 0.0;
+    const int o = t - (p - 1);
+    const double ov = r.in(1, o), opv = r.in(0, o) * ov;
+    double a = spv, c = sv;
+    if (zeros >= p) {
+      a = 0.0;
+      c = 0.0;
+      spv = 0.0 - opv;
+      sv = 0.0 - ov;
+    } else {
+      spv -= opv;
+      sv -= ov;
+    }
+    r.put(0, t, a / (double)p / (c / (double)p));
+    zeros -= fabs(ov) <= 0.0;
+  }
+}
+
+// ta_ER: |x[t] - x[t-p]| over the running sum of |one-bar changes|, clamped
+// to 1; a full window of unchanged bars restarts the sum and gives 1
+__device__ void f_er(const Row& r, int p) {
+  if (r.n <= p) return;
+  double sum = 0.0;
+  int run = 0;
+  for (int t = 1; t <= p; ++t) {
+    const double d = r.in(0, t - 1) - r.in(0, t);
+    sum += fabs(d);
+    run = fabs(d) <= 0.0 ? run + 1 : 0;
+  }
+  double trailing = r.in(0, 0);
+  {
+    const double roc = r.in(0, p) - trailing;
+    double o = 1.0;
+    if (!(sum <= 0.0 || sum <= roc)) {
+      o = fabs(roc / sum);
+      if (o > 1.0) o = 1.0;
+    }
+    r.put(0, p, o);
+  }
+  for (int t = p + 1; t < r.n; ++t) {
+    const double cur = r.in(0, t), old = r.in(0, t - p), prev = r.in(0, t - 1);
+    const double roc = cur - old;
+    sum -= fabs(trailing - old);
+    sum += fabs(cur - prev);
+    run = fabs(cur - prev) <= 0.0 ? run + 1 : 0;
+    double o;
+    if (run >= p) {
+      run = p;
+      sum = 0.0;
+      o = 1.0;
+    } else if (sum <= 0.0 || sum <= roc) {
+      o = 1.0;
+    } else {
+      o = fabs(roc / sum);
+      if (o > 1.0) o = 1.0;
+    }
+    r.put(0, t, o);
+    trailing = old;
+  }
+}
+
+// ta_VHF: (max - min) / sum |changes| over the p + 1 bars ending at t, every
+// window evaluated afresh oldest first (the extremes seeded with x[t])
+__device__ void f_vhf(const Row& r, int p) {
+  for (int t = p; t < r.n; ++t) {
+    double hi = r.in(0, t), lo = hi, prev = r.in(0, t - p), chg = 0.0;
+    for (int s = t - p + 1; s <= t; ++s) {
+      const double v = r.in(0, s);
+      chg += fabs(v - prev);
+      if (v > hi) hi = v;
+      if (v < lo) lo = v;
+      prev = v;
+    }
+    r.put(0, t, chg > 0.0 ? (hi - lo) / chg : 0.0);
+  }
+}
+
+// ta_DPO: x[t - (p/2 + 1)] minus the running p-bar mean ending at t
+__device__ void f_dpo(const Row& r, int p) {
+  const int shift = p / 2 + 1, first = (p - 1) > shift ? (p - 1) : shift;
+  if (r.n <= first) return;
+  double s = 0.0;
+  for (int t = first - (p - 1); t < first; ++t) s += r.in(0, t);
+  for (int t = first; t < r.n; ++t) {
+    s += r.in(0, t);
+    const double tot = s;
+    s -= r.in(0, t - (p - 1));
+    r.put(0, t, r.in(0, t - shift) - tot / (double)p);
+  }
+}
+
+// ta_CHOP: 100 log10(sum TR / (max high - min low)) / log10(p) over the p
+// bars ending at t (true ranges against the close before each bar), 100
+// when the range or the sum is not positive
+__device__ void f_chop(const Row& r, int p) {
+  const double lp = log10((double)p);
+  for (int t = p; t < r.n; ++t) {
+    double hi = r.in(0, t), lo = r.in(1, t), str = 0.0, pc = r.in(2, t - p);
+    for (int s = t - p + 1; s <= t; ++s) {
+      const double h = r.in(0, s), l = r.in(1, s);
+      const double th = pc > h ? pc : h, tl = pc < l ? pc : l;
+      str += th - tl;
+      if (h > hi) hi = h;
+      if (l < lo) lo = l;
+      pc = r.in(2, s);
+    }
+    r.put(0, t,
+          (hi - lo <= 0.0 || str <= 0.0) ? 100.0 : 100.0 * (log10(str / (hi - lo)) / lp));
+  }
+}
+
+// ta_ADR (high - low) and ta_QSTICK (close - open): the running p-bar mean
+// of a - b
+__device__ void f_mean_diff(const Row& r, int ia, int ib, int p) {
+  if (r.n < p) return;
+  double s = 0.0;
+  for (int t = 0; t < p - 1; ++t) s += r.in(ia, t) - r.in(ib, t);
+  for (int t = p - 1; t < r.n; ++t) {
+    s += r.in(ia, t) - r.in(ib, t);
+    const double tot = s;
+    const int o = t - (p - 1);
+    s -= r.in(ia, o) - r.in(ib, o);
+    r.put(0, t, tot / (double)p);
+  }
+}
+
+// ta_AO: the fast minus the slow running mean of the median price
+__device__ void f_ao(const Row& r, int f, int sl) {
+  const int first = (f > sl ? f : sl) - 1;
+  if (r.n <= first) return;
+  double sf = 0.0, ss = 0.0;
+  for (int t = 0; t < first; ++t) {
+    const double m = (r.in(0, t) + r.in(1, t)) / 2.0;
+    if (t >= first - (f - 1)) sf += m;
+    if (t >= first - (sl - 1)) ss += m;
+  }
+  for (int t = first; t < r.n; ++t) {
+    const double m = (r.in(0, t) + r.in(1, t)) / 2.0;
+    sf += m;
+    ss += m;
+    const double o = sf / (double)f - ss / (double)sl;
+    const int of = t - (f - 1), os = t - (sl - 1);
+    sf -= (r.in(0, of) + r.in(1, of)) / 2.0;
+    ss -= (r.in(0, os) + r.in(1, os)) / 2.0;
+    r.put(0, t, o);
+  }
+}
+
+// ta_DONCHIAN: window extremes kept with their bar index; when the extreme
+// leaves the window it is searched again from the oldest bar (first strict
+// improvement wins), otherwise a new bar at or beyond it replaces it.  Mid =
+// (upper + lower) / 2.
+__device__ void f_donchian(const Row& r, int p) {
+  int hiAt = -1, loAt = -1;
+  double hi = 0.0, lo = 0.0;
+  for (int t = p - 1; t < r.n; ++t) {
+    const int o = t - (p - 1);
+    if (hiAt < o) {
+      hiAt = o;
+      hi = r.in(0, o);
+      for (int s = o + 1; s <= t; ++s) {
+        const double v = r.in(0, s);
+        if (v > hi) {
+          hiAt = s;
+          hi = v;
+        }
+      }
+    } else if (r.in(0, t) >= hi) {
+      hiAt = t;
+      hi = r.in(0, t);
+    }
+    if (loAt < o) {
+      loAt = o;
+      lo = r.in(1, o);
+      for (int s = o + 1; s <= t; ++s) {
+        const double v = r.in(1, s);
+        if (v < lo) {
+          loAt = s;
+          lo = v;
+        }
+      }
+    } else if (r.in(1, t) <= lo) {
+      loAt = t;
+      lo = r.in(1, t);
+    }
+    r.put(0, t, hi);
+    r.put(1, t, (hi + lo) / 2.0);
+    r.put(2, t, lo);
+  }
+}
+
+__device__ void f_marketfi(const Row& r) {
+  for (int t = 0; t < r.n; ++t) {
+    const double v = r.in(2, t);
+    r.put(0, t, v != 0.0 ? (r.in(0, t) - r.in(1, t)) / v : 0.0);
+  }
+}
+
+// ta_WAD: accumulation against the true low on up bars, the true high on
+// down bars, nothing on unchanged closes
+__device__ void f_wad(const Row& r) {
+  double sum = 0.0, pc = r.in(2, 0);
+  for (int t = 0; t < r.n; ++t) {
+    const double c = r.in(2, t);
+    if (c > pc) {
+      const double l = r.in(1, t);
+      sum += c - (pc < l ? pc : l);
+    } else if (c < pc) {
+      const double h = r.in(0, t);
+      sum += c - (pc > h ? pc : h);
+    }
+    r.put(0, t, sum);
+    pc = c;
+  }
+}
+
+// ta_MCGD: md += (x - md) / (p (x/md)^4), a non-finite step keeps md
+__device__ void f_mcgd(const Row& r, int p) {
+  if (r.n < p) return;
+  const double per = (double)p;
+  double md = r.in(0, 0);
+  for (int t = 1; t < r.n; ++t) {
+    const double x = r.in(0, t), q = x / md;
+    const double nm = md + (x - md) / (per * (q * q * (q * q)));
+    if (finite_d(nm)) md = nm;
+    if (t >= p - 1) r.put(0, t, md);
+  }
+}
+
+// ta_RVOL: volume over the running mean of the p bars before it; a full
+// window of zero volumes resets the sum
+__device__ void f_rvol(const Row& r, int p) {
+  if (r.n <= p) return;
+  double s = 0.0;
+  int run = 0;
+  for (int t = 0; t < p; ++t) {
+    const double v = r.in(0, t);
+    s += v;
+    run = fabs(v) <= 0.0 ? run + 1 : 0;
+  }
+  for (int t = p; t < r.n; ++t) {
+    const double base = s / (double)p, old = r.in(0, t - p), v = r.in(0, t);
+    r.put(0, t, v / base);
+    s -= old;
+    s += v;
+    run = fabs(v) <= 0.0 ? run + 1 : 0;
+    if (run >= p) s = 0.0;
+  }
+}
+
+template <int F>
+__device__ void run_row(const Row& r, int p0, int p1) {
+  if constexpr (F == kCumsum) f_cumsum(r);
+  else if constexpr (F == kPvt) f_pvt(r);
+  else if constexpr (F == kRma) f_rma(r, p0);
+  else if constexpr (F == kZlema) f_zlema(r, p0);
+  else if constexpr (F == kEfi) f_efi(r, p0);
+  else if constexpr (F == kVwma) f_vwma(r, p0);
+  else if constexpr (F == kEr) f_er(r, p0);
+  else if constexpr (F == kVhf) f_vhf(r, p0);
+  else if constexpr (F == kDpo) f_dpo(r, p0);
+  else if constexpr (F == kChop) f_chop(r, p0);
+  else if constexpr (F == kAdr) f_mean_diff(r, 0, 1, p0);
+  else if constexpr (F == kAo) f_ao(r, p0, p1);
+  else if constexpr (F == kQstick) f_mean_diff(r, 1, 0, p0);
+  else if constexpr (F == kDonchian) f_donchian(r, p0);
+  else if constexpr (F == kMarketfi) f_marketfi(r);
+  else if constexpr (F == kWad) f_wad(r);
+  else if constexpr (F == kMcgd) f_mcgd(r, p0);
+  else f_rvol(r, p0);
+}
+
+// SPREAD: one row per wave (lane 0 works, the block is one wave); else one
+// row per lane.  Each row: NaN fill, the leading-NaN prefix, the routine.
+template <int F, bool SPREAD>
+__global__ __launch_bounds__(64) void k_rows(RowArgs a) {
+  constexpr RowSpec S = kSpec[F];
+  const long long row = SPREAD ? (long long)blockIdx.x : (long long)blockIdx.x * 64 + threadIdx.x;
+  if (SPREAD && threadIdx.x != 0) return;
+  if (row >= a.rows) return;
+  const size_t base = (size_t)row * (size_t)a.len;
+  const float nanf_ = __int_as_float(0x7fc00000);
+  for (int k = 0; k < S.nout; ++k)
+    for (int t = 0; t < a.len; ++t) a.out[k][base + t] = nanf_;
+  int f = 0;
+  for (; f < a.len - 1; ++f) {
+    bool any = false;
+    for (int j = 0; j < S.nin; ++j) any |= __builtin_isnan(a.in[j][base + f]);
+    if (!any) break;
+  }
+  Row r;
+  for (int j = 0; j < 3; ++j) r.x[j] = j < S.nin ? a.in[j] + base + f : nullptr;
+  for (int k = 0; k < 3; ++k) r.y[k] = k < S.nout ? a.out[k] + base + f : nullptr;
+  r.n = a.len - f;
+  run_row<F>(r, a.p0, a.p1);
+}
+
+template <int F>
+ctStatus_t run_rows(const char* fn, const float* const* in, float* const* out, int len,
+                    int rows, int p0, int p1, hipStream_t s) {
+  constexpr RowSpec S = kSpec[F];
+  if (len <= 0 || rows <= 0)
+    return fail(CT_STATUS_INVALID_ARGUMENT, "%s: size/seriesLen and numSeries must be positive",
+                fn);
+  RowArgs a{};
+  for (int j = 0; j < S.nin; ++j) {
+    if (!in[j]) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: null pointer", fn);
+    a.in[j] = in[j];
+  }
+  for (int k = 0; k < S.nout; ++k) {
+    if (!out[k]) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: null pointer", fn);
+    a.out[k] = out[k];
+  }
+  const int par[2] = {p0, p1};
+  for (int k = 0; k < S.npar; ++k)
+    if (par[k] < S.lo[k] || par[k] > S.hi[k])
+      return fail(CT_STATUS_INVALID_ARGUMENT, "%s: period out of range", fn);
+  a.len = len;
+  a.rows = rows;
+  a.p0 = p0;
+  a.p1 = p1;
+  // few long rows: a wave each, so the rows' loads do not share one CU
+  if (rows <= 2048 && len >= 4096)
+    hipLaunchKernelGGL((k_rows<F, true>), dim3(rows), dim3(64), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_rows<F, false>), dim3((rows + 63) / 64), dim3(64), 0, s, a);
+  TCA_HIP(hipGetLastError());
+  return CT_STATUS_SUCCESS;
+}
+
+#define RF_RET(x)                               \
+  do {                                          \
+    const ctStatus_t r_ = (x);                  \
+    if (r_ != CT_STATUS_SUCCESS) return r_;     \
+  } while (0)
+
+// Device calls synchronize on the legacy default stream, as the reference's
+// backend does (Talib08.cu:143-144).
+template <int F>
+ctStatus_t dev_rows(const char* fn, const float* const* in, float* const* out, int len, int rows,
+                    int p0, int p1, hipStream_t s) {
+  RF_RET(run_rows<F>(fn, in, out, len, rows, p0, p1, s));
+  return finish_batch(s);
+}
+
+template <int F>
+ctStatus_t host_rows(const char* fn, const float* const* hin, float* const* hout, int len,
+                     int rows, int p0, int p1, hipStream_t s) {
+  constexpr RowSpec S = kSpec[F];
+  if (len <= 0 || rows <= 0)
+    return fail(CT_STATUS_INVALID_ARGUMENT, "%s: size/seriesLen and numSeries must be positive",
+                fn);
+  for (int j = 0; j < S.nin; ++j)
+    if (!hin[j]) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: null pointer", fn);
+  for (int k = 0; k < S.nout; ++k)
+    if (!hout[k]) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: null pointer", fn);
+  const size_t n = (size_t)len * (size_t)rows;
+  HostRun hr(s);
+  const float* d_in[3] = {nullptr, nullptr, nullptr};
+  float* d_out[3] = {nullptr, nullptr, nullptr};
+  for (int j = 0; j < S.nin; ++j) RF_RET(hr.in(hin[j], n, &d_in[j]));
+  for (int k = 0; k < S.nout; ++k) RF_RET(hr.out(hout[k], n, &d_out[k]));
+  RF_RET(run_rows<F>(fn, d_in, d_out, len, rows, p0, p1, s));
+  return hr.finish();
+}
+
+}  // namespace
+}  // namespace tca
+
+using namespace tca;
+
+// Parameter lists by shape: inputs, outputs and periods.
+#define RF_IN1 const float* i0
+#define RF_IN2 const float* i0, const float* i1
+#define RF_IN3 const float* i0, const float* i1, const float* i2
+#define RF_INA1 {i0, nullptr, nullptr}
+#define RF_INA2 {i0, i1, nullptr}
+#define RF_INA3 {i0, i1, i2}
+#define RF_OUT1 float* o0
+#define RF_OUT3 float* o0, float* o1, float* o2
+#define RF_OUTA1 {o0, nullptr, nullptr}
+#define RF_OUTA3 {o0, o1, o2}
+#define RF_P0
+#define RF_P1 , int p0
+#define RF_P2 , int p0, int p1
+#define RF_PV0 0, 0
+#define RF_PV1 p0, 0
+#define RF_PV2 p0, p1
+
+#define TCA_ROWFN(NAME, F, NI, NO, NP)                                                           \
+  extern "C" {                                                                                   \
+  CTAPI_EXPORT ctStatus_t ct_##NAME##_d(RF_IN##NI, RF_OUT##NO, int n RF_P##NP, hipStream_t s) {  \
+    const float* in[3] = RF_INA##NI;                                                             \
+    float* out[3] = RF_OUTA##NO;                                                                 \
+    return guarded([&] { return dev_rows<F>("ct_" #NAME "_d", in, out, n, 1, RF_PV##NP, s); });  \
+  }                                                                                              \
+  CTAPI_EXPORT ctStatus_t ct_##NAME##_batch_d(RF_IN##NI, RF_OUT##NO, int len,                    \
+                                              int rows RF_P##NP, int, hipStream_t s) {           \
+    const float* in[3] = RF_INA##NI;                                                             \
+    float* out[3] = RF_OUTA##NO;                                                                 \
+    return guarded(                                                                              \
+        [&] { return dev_rows<F>("ct_" #NAME "_batch_d", in, out, len, rows, RF_PV##NP, s); });  \
+  }                                                                                              \
+  CTAPI_EXPORT ctStatus_t ct_##NAME(RF_IN##NI, RF_OUT##NO, int n RF_P##NP, hipStream_t s) {      \
+    const float* in[3] = RF_INA##NI;                                                             \
+    float* out[3] = RF_OUTA##NO;                                                                 \
+    return guarded([&] { return host_rows<F>("ct_" #NAME, in, out, n, 1, RF_PV##NP, s); });      \
+  }                                                                                              \
+  CTAPI_EXPORT ctStatus_t ct_##NAME##_batch(RF_IN##NI, RF_OUT##NO, int len, int rows RF_P##NP,   \
+                                            int, hipStream_t s) {                                \
+    const float* in[3] = RF_INA##NI;                                                             \
+    float* out[3] = RF_OUTA##NO;                                                                 \
+    return guarded(                                                                              \
+        [&] { return host_rows<F>("ct_" #NAME "_batch", in, out, len, rows, RF_PV##NP, s); });   \
+  }                                                                                              \
+  }
+
+TCA_ROWFN(cumsum, kCumsum, 1, 1, 0)
+TCA_ROWFN(pvt, kPvt, 2, 1, 0)
+TCA_ROWFN(rma, kRma, 1, 1, 1)
+TCA_ROWFN(zlema, kZlema, 1, 1, 1)
+TCA_ROWFN(efi, kEfi, 2, 1, 1)
+TCA_ROWFN(vwma, kVwma, 2, 1, 1)
+TCA_ROWFN(er, kEr, 1, 1, 1)
+TCA_ROWFN(vhf, kVhf, 1, 1, 1)
+TCA_ROWFN(dpo, kDpo, 1, 1, 1)
+TCA_ROWFN(chop, kChop, 3, 1, 1)
+TCA_ROWFN(adr, kAdr, 2, 1, 1)
+TCA_ROWFN(ao, kAo, 2, 1, 2)
+TCA_ROWFN(qstick, kQstick, 2, 1, 1)
+TCA_ROWFN(donchian, kDonchian, 2, 3, 1)
+TCA_ROWFN(marketfi, kMarketfi, 3, 1, 0)
+TCA_ROWFN(wad, kWad, 3, 1, 0)
+TCA_ROWFN(mcgd, kMcgd, 1, 1, 1)
+TCA_ROWFN(rvol, kRvol, 1, 1, 1)
