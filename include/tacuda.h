@@ -378,8 +378,8 @@ CTAPI_EXPORT ctStatus_t ct_midprice_batch_d(const float* d_highs, const float* d
 
 /* ---------------------------------------------------------------------------
  * TA-Lib 0.8.3 row functions (csrc/rowfn.hip): CUMSUM, PVT, RMA, ZLEMA, EFI,
- * VWMA, ER, VHF, DPO, CHOP, ADR, AO, QSTICK, DONCHIAN, MARKETFI, WAD, MCGD
- * and RVOL.  Per row: the leading bars where any input is NaN are skipped,
+ * VWMA, ER, VHF, DPO, CHOP, ADR, AO, QSTICK, DONCHIAN, MARKETFI, WAD, MCGD,
+ * RVOL, CMF, EMV, FOSC, ERI, PERCENTRANK, SI and ASI_WILDER (TA_ASI).  Per row: the leading bars where any input is NaN are skipped,
  * TA-Lib's routine runs in double over the rest, outputs are float with NaN
  * before the first one.  Periods outside TA-Lib's ranges are
  * CT_STATUS_INVALID_ARGUMENT; device calls on stream 0 synchronize.
@@ -438,6 +438,27 @@ CTAPI_EXPORT ctStatus_t ct_midprice_batch_d(const float* d_highs, const float* d
 /* :3962 */ CTAPI_EXPORT ctStatus_t ct_mcgd_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
 /* :4173 */ CTAPI_EXPORT ctStatus_t ct_rvol_batch(const float* host_volumes, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
 /* :4178 */ CTAPI_EXPORT ctStatus_t ct_rvol_batch_d(const float* d_volumes, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3413 */ CTAPI_EXPORT ctStatus_t ct_cmf(const float* host_high, const float* host_low, const float* host_close, const float* host_volume, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3585 */ CTAPI_EXPORT ctStatus_t ct_emv(const float* host_high, const float* host_low, const float* host_volume, float* host_output, int size, int period, double volumeDivisor, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3652 */ CTAPI_EXPORT ctStatus_t ct_fosc(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3616 */ CTAPI_EXPORT ctStatus_t ct_eri(const float* host_high, const float* host_low, const float* host_close, float* host_bullPower, float* host_bearPower, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :4012 */ CTAPI_EXPORT ctStatus_t ct_percentrank(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :4183 */ CTAPI_EXPORT ctStatus_t ct_si(const float* host_open, const float* host_high, const float* host_low, const float* host_close, float* host_output, int size, double limitMove, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3278 */ CTAPI_EXPORT ctStatus_t ct_asi_wilder(const float* host_open, const float* host_high, const float* host_low, const float* host_close, float* host_output, int size, double limitMove, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3418 */ CTAPI_EXPORT ctStatus_t ct_cmf_batch(const float* host_highs, const float* host_lows, const float* host_closes, const float* host_volumes, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3426 */ CTAPI_EXPORT ctStatus_t ct_cmf_batch_d(const float* d_highs, const float* d_lows, const float* d_closes, const float* d_volumes, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3589 */ CTAPI_EXPORT ctStatus_t ct_emv_batch(const float* host_highs, const float* host_lows, const float* host_volumes, float* host_outputs, int seriesLen, int numSeries, int period, double volumeDivisor, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3596 */ CTAPI_EXPORT ctStatus_t ct_emv_batch_d(const float* d_highs, const float* d_lows, const float* d_volumes, float* d_outputs, int seriesLen, int numSeries, int period, double volumeDivisor, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3655 */ CTAPI_EXPORT ctStatus_t ct_fosc_batch(const float* host_inputs, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3660 */ CTAPI_EXPORT ctStatus_t ct_fosc_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3620 */ CTAPI_EXPORT ctStatus_t ct_eri_batch(const float* host_highs, const float* host_lows, const float* host_closes, float* host_bullPower, float* host_bearPower, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3628 */ CTAPI_EXPORT ctStatus_t ct_eri_batch_d(const float* d_highs, const float* d_lows, const float* d_closes, float* d_bullPower, float* d_bearPower, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :4016 */ CTAPI_EXPORT ctStatus_t ct_percentrank_batch(const float* host_inputs, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :4021 */ CTAPI_EXPORT ctStatus_t ct_percentrank_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :4187 */ CTAPI_EXPORT ctStatus_t ct_si_batch(const float* host_opens, const float* host_highs, const float* host_lows, const float* host_closes, float* host_outputs, int seriesLen, int numSeries, double limitMove, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :4195 */ CTAPI_EXPORT ctStatus_t ct_si_batch_d(const float* d_opens, const float* d_highs, const float* d_lows, const float* d_closes, float* d_outputs, int seriesLen, int numSeries, double limitMove, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3285 */ CTAPI_EXPORT ctStatus_t ct_asi_wilder_batch(const float* host_opens, const float* host_highs, const float* host_lows, const float* host_closes, float* host_outputs, int seriesLen, int numSeries, double limitMove, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3293 */ CTAPI_EXPORT ctStatus_t ct_asi_wilder_batch_d(const float* d_opens, const float* d_highs, const float* d_lows, const float* d_closes, float* d_outputs, int seriesLen, int numSeries, double limitMove, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
 
 #ifdef __cplusplus
 }

@@ -256,6 +256,13 @@ CT_CDL_DECL(bearish_engulfing)
 /* :1314 */ CTAPI_EXPORT ctStatus_t ct_wad_d(const float* d_high, const float* d_low, const float* d_close, float* d_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
 /* :1177 */ CTAPI_EXPORT ctStatus_t ct_mcgd_d(const float* d_input, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
 /* :1229 */ CTAPI_EXPORT ctStatus_t ct_rvol_d(const float* d_volume, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1046 */ CTAPI_EXPORT ctStatus_t ct_cmf_d(const float* d_high, const float* d_low, const float* d_close, const float* d_volume, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1087 */ CTAPI_EXPORT ctStatus_t ct_emv_d(const float* d_high, const float* d_low, const float* d_volume, float* d_output, int size, int period, double volumeDivisor, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1102 */ CTAPI_EXPORT ctStatus_t ct_fosc_d(const float* d_input, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1094 */ CTAPI_EXPORT ctStatus_t ct_eri_d(const float* d_high, const float* d_low, const float* d_close, float* d_bullPower, float* d_bearPower, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1191 */ CTAPI_EXPORT ctStatus_t ct_percentrank_d(const float* d_input, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1232 */ CTAPI_EXPORT ctStatus_t ct_si_d(const float* d_open, const float* d_high, const float* d_low, const float* d_close, float* d_output, int size, double limitMove, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1013 */ CTAPI_EXPORT ctStatus_t ct_asi_wilder_d(const float* d_open, const float* d_high, const float* d_low, const float* d_close, float* d_output, int size, double limitMove, hipStream_t stream CTAPI_STREAM_DEFAULT);
 
 #ifdef __cplusplus
 }

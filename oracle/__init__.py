@@ -284,16 +284,18 @@ class _Family:
     ROWS = {"cumsum": (1, 1), "pvt": (2, 1), "rma": (1, 1), "zlema": (1, 1), "efi": (2, 1),
             "vwma": (2, 1), "er": (1, 1), "vhf": (1, 1), "dpo": (1, 1), "chop": (3, 1),
             "adr": (2, 1), "ao": (2, 1), "qstick": (2, 1), "donchian": (2, 3),
-            "marketfi": (3, 1), "wad": (3, 1), "mcgd": (1, 1), "rvol": (1, 1)}
+            "marketfi": (3, 1), "wad": (3, 1), "mcgd": (1, 1), "rvol": (1, 1), "cmf": (4, 1),
+            "emv": (3, 1), "fosc": (1, 1), "eri": (3, 2), "percentrank": (1, 1), "si": (4, 1),
+            "asi_wilder": (4, 1)}
 
-    def rows(self, name, ins, p0=0, p1=0):
+    def rows(self, name, ins, p0=0, p1=0, d0=0.0):
         """One row through the backend's leading-NaN rule; a tuple for
         several outputs."""
         nin, nout = self.ROWS[name]
         assert len(ins) == nin
         fn = getattr(self._get(), self._prefix + "rows")
         fn.argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
-                       ctypes.c_int, ctypes.c_int]
+                       ctypes.c_int, ctypes.c_int, ctypes.c_double]
         fn.restype = ctypes.c_int
         ins = [_f32(a) for a in ins]
         n = len(ins[0])
@@ -301,7 +303,7 @@ class _Family:
         pin = (ctypes.c_void_p * 4)(*([a.ctypes.data for a in ins] + [None] * (4 - nin)))
         pout = (ctypes.c_void_p * 4)(*([o.ctypes.data for o in outs] + [None] * (4 - nout)))
         rc = fn(name.encode(), ctypes.cast(pin, ctypes.c_void_p), ctypes.cast(pout, ctypes.c_void_p),
-                n, int(p0), int(p1))
+                n, int(p0), int(p1), float(d0))
         if rc != 0:
             raise ValueError(f"rows {name} returned status {rc}")
         return outs[0] if nout == 1 else tuple(outs)

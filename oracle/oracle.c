@@ -1512,8 +1512,10 @@ static int rows_prefix(const float* const* in, int nin, int n) {
 
 static int tafinite(double v) { return v <= 1.7976931348623157e308 && v >= -1.7976931348623157e308; }
 
-/* one routine: writes y[k][t] for t >= its look-back (NaN before) */
+/* one routine: writes y[k][t] for t >= its look-back (NaN before); par
+ * holds the integer periods, g_row_d the real parameter */
 typedef void (*RowRoutine)(const RowsIO* io, const int* par);
+static double g_row_d; /* the real parameter of the routine being run */
 
 static void r_cumsum(const RowsIO* io, const int* par) {
   double total = 0.0;
@@ -1850,38 +1852,219 @@ static void r_rvol(const RowsIO* io, const int* par) {
   }
 }
 
+
+/* ta_CMF: money-flow volume (close - low - (high - close)) / (high - low) *
+ * volume (0 on a non-positive range) and volume in running p-bar sums; the
+ * ratio (0 when the volume sum is not positive), a full window of zero
+ * volumes restarts the sums and gives 0. */
+static double cmf_mfv(const RowsIO* io, int t) {
+  const double h = io->x[0][t], l = io->x[1][t], c = io->x[2][t];
+  const double r = h - l;
+  return r > 0.0 ? (c - l - (h - c)) / r * io->x[3][t] : 0.0;
+}
+static void r_cmf(const RowsIO* io, const int* par) {
+  const int p = par[0], n = io->n;
+  if (n < p) return;
+  double sm = 0.0, sv = 0.0;
+  int run = 0, t = 0;
+  for (; t < p; ++t) {
+    sm += cmf_mfv(io, t);
+    sv += io->x[3][t];
+    run = (run + 1) & ((fabs(io->x[3][t]) <= 0.0 ? 0 : 1) - 1);
+  }
+  io->y[0][p - 1] = sv > 0.0 ? sm / sv : 0.0;
+  for (; t < n; ++t) {
+    sm -= cmf_mfv(io, t - p);
+    sv -= io->x[3][t - p];
+    sm += cmf_mfv(io, t);
+    sv += io->x[3][t];
+    run = (run + 1) & ((fabs(io->x[3][t]) <= 0.0 ? 0 : 1) - 1);
+    if (run >= p) {
+      run = p;
+      sm = 0.0;
+      sv = 0.0;
+      io->y[0][t] = 0.0;
+    } else {
+      io->y[0][t] = sv > 0.0 ? sm / sv : 0.0;
+    }
+  }
+}
+/* ta_EMV: raw = (mid - previous mid) / (volume / divisor / range) (0 on a
+ * zero range or ratio), its running p-bar mean from bar p */
+static double emv_raw(const RowsIO* io, int t, double div) {
+  const double h = io->x[0][t], l = io->x[1][t];
+  const double mid = (h + l) / 2.0, pm = (io->x[0][t - 1] + io->x[1][t - 1]) / 2.0;
+  const double range = h - l;
+  double raw = 0.0;
+  if (range != 0.0) {
+    const double box = io->x[2][t] / div / range;
+    if (box != 0.0) raw = (mid - pm) / box;
+  }
+  return raw;
+}
+static void r_emv(const RowsIO* io, const int* par) {
+  const int p = par[0], n = io->n;
+  const double div = g_row_d;
+  if (n <= p) return;
+  double tot = 0.0;
+  for (int t = 1; t < p; ++t) tot += emv_raw(io, t, div);
+  for (int t = p; t < n; ++t) {
+    tot += emv_raw(io, t, div);
+    const double tmp = tot;
+    tot -= emv_raw(io, t - p + 1, div);
+    io->y[0][t] = tmp / (double)p;
+  }
+}
+/* ta_FOSC: 100 (x - forecast) / x with the least-squares line of the p
+ * bars before t (x the distance back, as ta_LINEARREG) projected to
+ * fma(m, p, b); sums updated incrementally and re-formed every 32 p bars
+ * or when the leaving term dwarfs the window's magnitude; 0 when x is 0 */
+static void r_fosc(const RowsIO* io, const int* par) {
+  const int p = par[0], n = io->n;
+  const double* x = io->x[0];
+  if (n <= p) return;
+  const double sumX = (double)p * (p - 1) * 0.5;
+  const double sumXSqr = (double)p * (p - 1) * (2 * p - 1) / 6.0;
+  const double div = sumX * sumX - p * sumXSqr;
+  double sxy = 0, sy = 0, sa = 0;
+  for (int i = p - 1; i >= 0; --i) {
+    const double v = x[p - 1 - i];
+    sy += v;
+    sxy += (double)i * v;
+    sa += fabs(v);
+  }
+  int reseed = 32 * p, tr = 0;
+  double trailing = x[tr++];
+  for (int t = p; t < n; ++t) {
+    if (t > p) {
+      const double wt = (double)p * trailing;
+      sxy = sxy + sy - wt;
+      sy = sy - trailing + x[t - 1];
+      sa = sa - fabs(trailing) + fabs(x[t - 1]);
+      if (--reseed <= 0 || fabs(wt) > 100.0 * sa) {
+        reseed = 32 * p;
+        sy = 0;
+        sxy = 0;
+        sa = 0;
+        double w = (double)(p - 1);
+        for (int j = t - p; j < t; ++j, w -= 1.0) {
+          sy += x[j];
+          sxy += w * x[j];
+          sa += fabs(x[j]);
+        }
+      }
+      trailing = x[tr++];
+    }
+    const double m = (p * sxy - sumX * sy) / div;
+    const double b = (sy - m * sumX) / (double)p;
+    const double c = x[t];
+    io->y[0][t] = c != 0.0 ? 100.0 * (c - fma(m, (double)p, b)) / c : 0.0;
+  }
+}
+/* ta_ERI: bull = high - EMA(close), bear = low - EMA(close), the EMA
+ * seeded with the mean of the first p closes; p = 1 uses the close */
+static void r_eri(const RowsIO* io, const int* par) {
+  const int p = par[0], n = io->n;
+  const double *h = io->x[0], *l = io->x[1], *c = io->x[2];
+  if (n < p) return;
+  if (p == 1) {
+    for (int t = 0; t < n; ++t) {
+      io->y[0][t] = h[t] - c[t];
+      io->y[1][t] = l[t] - c[t];
+    }
+    return;
+  }
+  double beta, k;
+  ema_coef(p, &beta, &k);
+  double s = 0.0;
+  for (int t = 0; t < p; ++t) s += c[t];
+  double ma = s / p;
+  io->y[0][p - 1] = h[p - 1] - ma;
+  io->y[1][p - 1] = l[p - 1] - ma;
+  for (int t = p; t < n; ++t) {
+    ma = fma(beta, ma, k * c[t]);
+    io->y[0][t] = h[t] - ma;
+    io->y[1][t] = l[t] - ma;
+  }
+}
+/* ta_PERCENTRANK: the share (x100) of the p bars before t below x[t] */
+static void r_percentrank(const RowsIO* io, const int* par) {
+  const int p = par[0], n = io->n;
+  const double* x = io->x[0];
+  for (int t = p; t < n; ++t) {
+    int cnt = 0;
+    for (int i = p; i >= 1; --i) cnt += x[t - i] < x[t];
+    io->y[0][t] = (double)cnt / (double)p * 100.0;
+  }
+}
+/* Wilder's swing index of bar t against bar t-1 (ta_SI / ta_ASI):
+ * inputs (open, high, low, close), limit move L */
+static double swing(const RowsIO* io, int t, double lim) {
+  const double *o = io->x[0], *h = io->x[1], *l = io->x[2], *c = io->x[3];
+  const double pc = c[t - 1], pb = pc - o[t - 1], body = c[t] - o[t];
+  double nn = c[t] - pc;
+  nn += 0.5 * body;
+  nn += 0.25 * pb;
+  const double up = fabs(h[t] - pc), dn = fabs(l[t] - pc), rg = fabs(h[t] - l[t]);
+  const double k = up > dn ? up : dn;
+  double r = (up - 0.5 * dn) > (dn - 0.5 * up) ? (up - 0.5 * dn) : (dn - 0.5 * up);
+  r = r > rg ? r : rg;
+  r += 0.25 * fabs(pb);
+  return r == 0.0 ? 0.0 : 50.0 * (nn / r) * (k / lim);
+}
+static void r_si(const RowsIO* io, const int* par) {
+  for (int t = 1; t < io->n; ++t) io->y[0][t] = swing(io, t, g_row_d);
+}
+static void r_asi_wilder(const RowsIO* io, const int* par) {
+  double sum = 0.0;
+  io->y[0][0] = sum;
+  for (int t = 1; t < io->n; ++t) {
+    sum += swing(io, t, g_row_d);
+    io->y[0][t] = sum;
+  }
+}
+
 typedef struct {
   const char* name;
   RowRoutine fn;
   int nin, nout, npar;
   int lo[2], hi[2]; /* valid period ranges */
+  double dlo, dhi;  /* the real parameter's range (dlo > dhi: none) */
 } RowSpec;
 
 static const RowSpec kRows[] = {
-    {"cumsum", r_cumsum, 1, 1, 0, {0, 0}, {0, 0}},
-    {"pvt", r_pvt, 2, 1, 0, {0, 0}, {0, 0}},
-    {"rma", r_rma, 1, 1, 1, {1, 0}, {100000, 0}},
-    {"zlema", r_zlema, 1, 1, 1, {1, 0}, {100000, 0}},
-    {"efi", r_efi, 2, 1, 1, {1, 0}, {100000, 0}},
-    {"vwma", r_vwma, 2, 1, 1, {1, 0}, {100000, 0}},
-    {"er", r_er, 1, 1, 1, {2, 0}, {100000, 0}},
-    {"vhf", r_vhf, 1, 1, 1, {2, 0}, {100000, 0}},
-    {"dpo", r_dpo, 1, 1, 1, {2, 0}, {100000, 0}},
-    {"chop", r_chop, 3, 1, 1, {2, 0}, {100000, 0}},
-    {"adr", r_adr, 2, 1, 1, {1, 0}, {100000, 0}},
-    {"ao", r_ao, 2, 1, 2, {2, 2}, {100000, 100000}},
-    {"qstick", r_qstick, 2, 1, 1, {1, 0}, {100000, 0}},
-    {"donchian", r_donchian, 2, 3, 1, {2, 0}, {100000, 0}},
-    {"marketfi", r_marketfi, 3, 1, 0, {0, 0}, {0, 0}},
-    {"wad", r_wad, 3, 1, 0, {0, 0}, {0, 0}},
-    {"mcgd", r_mcgd, 1, 1, 1, {2, 0}, {100000, 0}},
-    {"rvol", r_rvol, 1, 1, 1, {1, 0}, {100000, 0}},
+    {"cumsum", r_cumsum, 1, 1, 0, {0, 0}, {0, 0}, 1, 0},
+    {"pvt", r_pvt, 2, 1, 0, {0, 0}, {0, 0}, 1, 0},
+    {"rma", r_rma, 1, 1, 1, {1, 0}, {100000, 0}, 1, 0},
+    {"zlema", r_zlema, 1, 1, 1, {1, 0}, {100000, 0}, 1, 0},
+    {"efi", r_efi, 2, 1, 1, {1, 0}, {100000, 0}, 1, 0},
+    {"vwma", r_vwma, 2, 1, 1, {1, 0}, {100000, 0}, 1, 0},
+    {"er", r_er, 1, 1, 1, {2, 0}, {100000, 0}, 1, 0},
+    {"vhf", r_vhf, 1, 1, 1, {2, 0}, {100000, 0}, 1, 0},
+    {"dpo", r_dpo, 1, 1, 1, {2, 0}, {100000, 0}, 1, 0},
+    {"chop", r_chop, 3, 1, 1, {2, 0}, {100000, 0}, 1, 0},
+    {"adr", r_adr, 2, 1, 1, {1, 0}, {100000, 0}, 1, 0},
+    {"ao", r_ao, 2, 1, 2, {2, 2}, {100000, 100000}, 1, 0},
+    {"qstick", r_qstick, 2, 1, 1, {1, 0}, {100000, 0}, 1, 0},
+    {"donchian", r_donchian, 2, 3, 1, {2, 0}, {100000, 0}, 1, 0},
+    {"marketfi", r_marketfi, 3, 1, 0, {0, 0}, {0, 0}, 1, 0},
+    {"wad", r_wad, 3, 1, 0, {0, 0}, {0, 0}, 1, 0},
+    {"mcgd", r_mcgd, 1, 1, 1, {2, 0}, {100000, 0}, 1, 0},
+    {"rvol", r_rvol, 1, 1, 1, {1, 0}, {100000, 0}, 1, 0},
+    {"cmf", r_cmf, 4, 1, 1, {2, 0}, {100000, 0}, 1, 0},
+    {"emv", r_emv, 3, 1, 1, {1, 0}, {100000, 0}, 1.0, 3e37},
+    {"fosc", r_fosc, 1, 1, 1, {2, 0}, {100000, 0}, 1, 0},
+    {"eri", r_eri, 3, 2, 1, {1, 0}, {100000, 0}, 1, 0},
+    {"percentrank", r_percentrank, 1, 1, 1, {2, 0}, {10000, 0}, 1, 0},
+    {"si", r_si, 4, 1, 0, {0, 0}, {0, 0}, 1e-8, 3e37},
+    {"asi_wilder", r_asi_wilder, 4, 1, 0, {0, 0}, {0, 0}, 1e-8, 3e37},
 };
 
-/* orc_rows(name, inputs, outputs, n, p0, p1): one row, the backend's
- * leading-NaN rule, float outputs. */
+/* orc_rows(name, inputs, outputs, n, p0, p1, d0): one row, the backend's
+ * leading-NaN rule, float outputs; d0 is the real parameter (EMV's volume
+ * divisor, SI's and ASI's limit move). */
 API int orc_rows(const char* name, const float* const* in, float* const* out, int n, int p0,
-                 int p1) {
+                 int p1, double d0) {
   const RowSpec* s = NULL;
   for (size_t i = 0; i < sizeof(kRows) / sizeof(kRows[0]); ++i)
     if (!strcmp(kRows[i].name, name)) s = &kRows[i];
@@ -1889,6 +2072,8 @@ API int orc_rows(const char* name, const float* const* in, float* const* out, in
   const int par[2] = {p0, p1};
   for (int k = 0; k < s->npar; ++k)
     if (par[k] < s->lo[k] || par[k] > s->hi[k]) return ORC_BAD;
+  if (s->dlo <= s->dhi && !(d0 >= s->dlo && d0 <= s->dhi)) return ORC_BAD;
+  g_row_d = d0;
   for (int j = 0; j < s->nin; ++j)
     if (!in[j]) return ORC_BAD;
   for (int k = 0; k < s->nout; ++k) {

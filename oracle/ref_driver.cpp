@@ -236,7 +236,7 @@ API int ref_ibs(const float* hi, const float* lo, const float* cl, float* out, i
 // bar), run the routine from index 0 over the rest, place its outputs at
 // prefix + outBegIdx.  name selects the routine; p0/p1 are its periods.
 API int ref_rows(const char* name, const float* const* in, float* const* out, int n, int p0,
-                 int p1) {
+                 int p1, double d0) {
   struct Spec {
     const char* name;
     int nin, nout;
@@ -246,7 +246,10 @@ API int ref_rows(const char* name, const float* const* in, float* const* out, in
                                {"er", 1, 1},     {"vhf", 1, 1},      {"dpo", 1, 1},
                                {"chop", 3, 1},   {"adr", 2, 1},      {"ao", 2, 1},
                                {"qstick", 2, 1}, {"donchian", 2, 3}, {"marketfi", 3, 1},
-                               {"wad", 3, 1},    {"mcgd", 1, 1},     {"rvol", 1, 1}};
+                               {"wad", 3, 1},    {"mcgd", 1, 1},     {"rvol", 1, 1},
+                               {"cmf", 4, 1},    {"emv", 3, 1},      {"fosc", 1, 1},
+                               {"eri", 3, 2},    {"percentrank", 1, 1}, {"si", 4, 1},
+                               {"asi_wilder", 4, 1}};
   const Spec* s = nullptr;
   for (const Spec& q : specs)
     if (std::string(q.name) == name) s = &q;
@@ -291,6 +294,23 @@ API int ref_rows(const char* name, const float* const* in, float* const* out, in
     rc = TA_WAD(a, 0, e, x[0].data(), x[1].data(), x[2].data(), &beg, &nb, y[0].data());
   else if (nm == "mcgd") rc = TA_MCGD(a, 0, e, x[0].data(), p0, &beg, &nb, y[0].data());
   else if (nm == "rvol") rc = TA_RVOL(a, 0, e, x[0].data(), p0, &beg, &nb, y[0].data());
+  else if (nm == "cmf")
+    rc = TA_CMF(a, 0, e, x[0].data(), x[1].data(), x[2].data(), x[3].data(), p0, &beg, &nb,
+                y[0].data());
+  else if (nm == "emv")
+    rc = TA_EMV(a, 0, e, x[0].data(), x[1].data(), x[2].data(), p0, d0, &beg, &nb, y[0].data());
+  else if (nm == "fosc") rc = TA_FOSC(a, 0, e, x[0].data(), p0, &beg, &nb, y[0].data());
+  else if (nm == "eri")
+    rc = TA_ERI(a, 0, e, x[0].data(), x[1].data(), x[2].data(), p0, &beg, &nb, y[0].data(),
+                y[1].data());
+  else if (nm == "percentrank")
+    rc = TA_PERCENTRANK(a, 0, e, x[0].data(), p0, &beg, &nb, y[0].data());
+  else if (nm == "si")
+    rc = TA_SI(a, 0, e, x[0].data(), x[1].data(), x[2].data(), x[3].data(), d0, &beg, &nb,
+               y[0].data());
+  else if (nm == "asi_wilder")
+    rc = TA_ASI(a, 0, e, x[0].data(), x[1].data(), x[2].data(), x[3].data(), d0, &beg, &nb,
+                y[0].data());
   if (rc) return 4;
   for (int k = 0; k < s->nout; ++k)
     for (int i = 0; i < nb; ++i) out[k][f + beg + i] = (float)y[k][i];

@@ -313,12 +313,16 @@ ROW_FUNCTIONS = {"cumsum": (1, 1, 0), "pvt": (2, 1, 0), "rma": (1, 1, 1), "zlema
                  "efi": (2, 1, 1), "vwma": (2, 1, 1), "er": (1, 1, 1), "vhf": (1, 1, 1),
                  "dpo": (1, 1, 1), "chop": (3, 1, 1), "adr": (2, 1, 1), "ao": (2, 1, 2),
                  "qstick": (2, 1, 1), "donchian": (2, 3, 1), "marketfi": (3, 1, 0),
-                 "wad": (3, 1, 0), "mcgd": (1, 1, 1), "rvol": (1, 1, 1)}
+                 "wad": (3, 1, 0), "mcgd": (1, 1, 1), "rvol": (1, 1, 1), "cmf": (4, 1, 1),
+                 "emv": (3, 1, "1d"), "fosc": (1, 1, 1), "eri": (3, 2, 1),
+                 "percentrank": (1, 1, 1), "si": (4, 1, "d"), "asi_wilder": (4, 1, "d")}
+_PARAMS = {0: [], 1: [c_int], 2: [c_int, c_int], "d": [ctypes.c_double],
+           "1d": [c_int, ctypes.c_double]}
 for _n, (_i, _o, _p) in ROW_FUNCTIONS.items():
     for _f in (f"ct_{_n}_d", f"ct_{_n}"):
-        _sig(_f, *([P] * (_i + _o)), c_int, *([c_int] * _p))
+        _sig(_f, *([P] * (_i + _o)), c_int, *_PARAMS[_p])
     for _f in (f"ct_{_n}_batch_d", f"ct_{_n}_batch"):
-        _sig(_f, *([P] * (_i + _o)), c_int, c_int, *([c_int] * _p), c_int)
+        _sig(_f, *([P] * (_i + _o)), c_int, c_int, *_PARAMS[_p], c_int)
 _sig("ctx_macd_bwd_d", P, P, P, P, c_int, c_int, c_int, c_int, c_int)
 _sig("ctx_bbands_bwd_d", P, P, P, P, P, c_int, c_int, c_int, c_float, c_float)
 

@@ -1,8 +1,9 @@
 // TA-Lib 0.8.3 functions the reference serves from its formula backend
 // (src/compat/Talib08.cu:47-103, one work-item per row running the TA_*
 // routine in double): CUMSUM, PVT, RMA, ZLEMA, EFI, VWMA, ER, VHF, DPO,
-// CHOP, ADR, AO, QSTICK, DONCHIAN, MARKETFI, WAD, MCGD and RVOL, each in
-// host, device, batch and host-batch forms.  Reference entry points:
+// CHOP, ADR, AO, QSTICK, DONCHIAN, MARKETFI, WAD, MCGD, RVOL, CMF, EMV,
+// FOSC, ERI, PERCENTRANK, SI and ASI_WILDER (TA_ASI), each in host, device,
+// batch and host-batch forms.  Reference entry points:
 // include/tacuda.h (ct_<name> and ct_<name>_batch[_d], cited per
 // declaration), include/tacuda_device.h (ct_<name>_d).
 //
@@ -35,46 +36,58 @@ namespace {
 
 enum RowFn {
   kCumsum, kPvt, kRma, kZlema, kEfi, kVwma, kEr, kVhf, kDpo, kChop, kAdr, kAo, kQstick,
-  kDonchian, kMarketfi, kWad, kMcgd, kRvol
+  kDonchian, kMarketfi, kWad, kMcgd, kRvol, kCmf, kEmv, kFosc, kEri, kPercentrank, kSi,
+  kAsiWilder
 };
 
 struct RowSpec {
   int nin, nout, npar;
   int lo[2], hi[2];
+  double dlo, dhi;  // the real parameter's range (dlo > dhi: none)
 };
+
+constexpr double kRealMax = 3e37;  // TA_REAL_MAX
 
 // inputs, outputs, periods and their valid ranges (TA-Lib's checks)
 constexpr RowSpec kSpec[] = {
-    {1, 1, 0, {0, 0}, {0, 0}},            // CUMSUM
-    {2, 1, 0, {0, 0}, {0, 0}},            // PVT (close, volume)
-    {1, 1, 1, {1, 0}, {100000, 0}},       // RMA
-    {1, 1, 1, {1, 0}, {100000, 0}},       // ZLEMA
-    {2, 1, 1, {1, 0}, {100000, 0}},       // EFI (close, volume)
-    {2, 1, 1, {1, 0}, {100000, 0}},       // VWMA (input, volume)
-    {1, 1, 1, {2, 0}, {100000, 0}},       // ER
-    {1, 1, 1, {2, 0}, {100000, 0}},       // VHF
-    {1, 1, 1, {2, 0}, {100000, 0}},       // DPO
-    {3, 1, 1, {2, 0}, {100000, 0}},       // CHOP (high, low, close)
-    {2, 1, 1, {1, 0}, {100000, 0}},       // ADR (high, low)
-    {2, 1, 2, {2, 2}, {100000, 100000}},  // AO (high, low; fast, slow)
-    {2, 1, 1, {1, 0}, {100000, 0}},       // QSTICK (open, close)
-    {2, 3, 1, {2, 0}, {100000, 0}},       // DONCHIAN (high, low -> upper, middle, lower)
-    {3, 1, 0, {0, 0}, {0, 0}},            // MARKETFI (high, low, volume)
-    {3, 1, 0, {0, 0}, {0, 0}},            // WAD (high, low, close)
-    {1, 1, 1, {2, 0}, {100000, 0}},       // MCGD
-    {1, 1, 1, {1, 0}, {100000, 0}},       // RVOL (volume)
+    {1, 1, 0, {0, 0}, {0, 0}, 1, 0},            // CUMSUM
+    {2, 1, 0, {0, 0}, {0, 0}, 1, 0},            // PVT (close, volume)
+    {1, 1, 1, {1, 0}, {100000, 0}, 1, 0},       // RMA
+    {1, 1, 1, {1, 0}, {100000, 0}, 1, 0},       // ZLEMA
+    {2, 1, 1, {1, 0}, {100000, 0}, 1, 0},       // EFI (close, volume)
+    {2, 1, 1, {1, 0}, {100000, 0}, 1, 0},       // VWMA (input, volume)
+    {1, 1, 1, {2, 0}, {100000, 0}, 1, 0},       // ER
+    {1, 1, 1, {2, 0}, {100000, 0}, 1, 0},       // VHF
+    {1, 1, 1, {2, 0}, {100000, 0}, 1, 0},       // DPO
+    {3, 1, 1, {2, 0}, {100000, 0}, 1, 0},       // CHOP (high, low, close)
+    {2, 1, 1, {1, 0}, {100000, 0}, 1, 0},       // ADR (high, low)
+    {2, 1, 2, {2, 2}, {100000, 100000}, 1, 0},  // AO (high, low; fast, slow)
+    {2, 1, 1, {1, 0}, {100000, 0}, 1, 0},       // QSTICK (open, close)
+    {2, 3, 1, {2, 0}, {100000, 0}, 1, 0},       // DONCHIAN (high, low -> upper, middle, lower)
+    {3, 1, 0, {0, 0}, {0, 0}, 1, 0},            // MARKETFI (high, low, volume)
+    {3, 1, 0, {0, 0}, {0, 0}, 1, 0},            // WAD (high, low, close)
+    {1, 1, 1, {2, 0}, {100000, 0}, 1, 0},       // MCGD
+    {1, 1, 1, {1, 0}, {100000, 0}, 1, 0},       // RVOL (volume)
+    {4, 1, 1, {2, 0}, {100000, 0}, 1, 0},       // CMF (high, low, close, volume)
+    {3, 1, 1, {1, 0}, {100000, 0}, 1.0, kRealMax},  // EMV (high, low, volume; divisor)
+    {1, 1, 1, {2, 0}, {100000, 0}, 1, 0},       // FOSC
+    {3, 2, 1, {1, 0}, {100000, 0}, 1, 0},       // ERI (high, low, close -> bull, bear)
+    {1, 1, 1, {2, 0}, {10000, 0}, 1, 0},        // PERCENTRANK
+    {4, 1, 0, {0, 0}, {0, 0}, 1e-8, kRealMax},  // SI (open, high, low, close; limit move)
+    {4, 1, 0, {0, 0}, {0, 0}, 1e-8, kRealMax},  // ASI_WILDER (the same)
 };
 
 struct RowArgs {
-  const float* in[3];
+  const float* in[4];
   float* out[3];
   int len, rows, p0, p1;
+  double d0;
 };
 
 // One row's view: inputs and outputs offset to the first bar after the
 // leading-NaN prefix; n bars remain.
 struct Row {
-  const float* x[3];
+  const float* x[4];
   float* y[3];
   int n;
   __device__ double in(int j, int t) const { return (double)x[j][t]; }
@@ -441,8 +454,175 @@ __device__ void f_rvol(const Row& r, int p) {
   }
 }
 
+// ta_CMF: running sums of money-flow volume and volume (a bar's flow is a
+// function of that bar alone, so the leaving term is formed again rather
+// than kept in a ring); 0 when the volume sum is not positive, and a full
+// window of zero volumes restarts both sums and gives 0
+__device__ __forceinline__ double cmf_flow(const Row& r, int t) {
+  const double h = r.in(0, t), l = r.in(1, t), c = r.in(2, t), rg = h - l;
+  return rg > 0.0 ? (c - l - (h - c)) / rg * r.in(3, t) : 0.0;
+}
+__device__ void f_cmf(const Row& r, int p) {
+  if (r.n < p) return;
+  double sm = 0.0, sv = 0.0;
+  int run = 0;
+  for (int t = 0; t < p; ++t) {
+    const double v = r.in(3, t);
+    sm += cmf_flow(r, t);
+    sv += v;
+    run = fabs(v) <= 0.0 ? run + 1 : 0;
+  }
+  r.put(0, p - 1, sv > 0.0 ? sm / sv : 0.0);
+  for (int t = p; t < r.n; ++t) {
+    const double v = r.in(3, t);
+    sm -= cmf_flow(r, t - p);
+    sv -= r.in(3, t - p);
+    sm += cmf_flow(r, t);
+    sv += v;
+    run = fabs(v) <= 0.0 ? run + 1 : 0;
+    double o;
+    if (run >= p) {
+      run = p;
+      sm = 0.0;
+      sv = 0.0;
+      o = 0.0;
+    } else {
+      o = sv > 0.0 ? sm / sv : 0.0;
+    }
+    r.put(0, t, o);
+  }
+}
+
+// ta_EMV: the running p-bar mean of (mid - previous mid) / box ratio, box =
+// volume / divisor / range (a zero range or box contributes 0)
+__device__ __forceinline__ double emv_term(const Row& r, int t, double div) {
+  const double h = r.in(0, t), l = r.in(1, t), rg = h - l;
+  if (rg == 0.0) return 0.0;
+  const double box = r.in(2, t) / div / rg;
+  if (box == 0.0) return 0.0;
+  return ((h + l) / 2.0 - (r.in(0, t - 1) + r.in(1, t - 1)) / 2.0) / box;
+}
+__device__ void f_emv(const Row& r, int p, double div) {
+  if (r.n <= p) return;
+  double s = 0.0;
+  for (int t = 1; t < p; ++t) s += emv_term(r, t, div);
+  for (int t = p; t < r.n; ++t) {
+    s += emv_term(r, t, div);
+    const double tot = s;
+    s -= emv_term(r, t - p + 1, div);
+    r.put(0, t, tot / (double)p);
+  }
+}
+
+// ta_FOSC: 100 (x - forecast) / x, the forecast fma(m, p, b) of the
+// least-squares line over the p bars before t (abscissa: bars back); the
+// sums move one bar at a time and are formed afresh every 32 p bars or when
+// the leaving term exceeds 100 times the window's absolute sum
+__device__ void f_fosc(const Row& r, int p) {
+  if (r.n <= p) return;
+  const double sumX = (double)p * (p - 1) * 0.5;
+  const double sumXSqr = (double)p * (p - 1) * (2 * p - 1) / 6.0;
+  const double div = sumX * sumX - p * sumXSqr;
+  double sxy = 0, sy = 0, sa = 0;
+  for (int back = p - 1; back >= 0; --back) {
+    const double v = r.in(0, p - 1 - back);
+    sy += v;
+    sxy += (double)back * v;
+    sa += fabs(v);
+  }
+  int left = 32 * p;
+  for (int t = p; t < r.n; ++t) {
+    if (t > p) {
+      const double old = r.in(0, t - p - 1), wt = (double)p * old, nw = r.in(0, t - 1);
+      sxy = sxy + sy - wt;
+      sy = sy - old + nw;
+      sa = sa - fabs(old) + fabs(nw);
+      if (--left <= 0 || fabs(wt) > 100.0 * sa) {
+        left = 32 * p;
+        sy = 0;
+        sxy = 0;
+        sa = 0;
+        double w = (double)(p - 1);
+        for (int j = t - p; j < t; ++j) {
+          const double v = r.in(0, j);
+          sy += v;
+          sxy += w * v;
+          sa += fabs(v);
+          w -= 1.0;
+        }
+      }
+    }
+    const double m = (p * sxy - sumX * sy) / div;
+    const double b = (sy - m * sumX) / (double)p;
+    const double c = r.in(0, t);
+    r.put(0, t, c != 0.0 ? 100.0 * (c - fma(m, (double)p, b)) / c : 0.0);
+  }
+}
+
+// ta_ERI: bull = high - EMA(close), bear = low - EMA(close)
+__device__ void f_eri(const Row& r, int p) {
+  if (r.n < p) return;
+  if (p == 1) {
+    for (int t = 0; t < r.n; ++t) {
+      r.put(0, t, r.in(0, t) - r.in(2, t));
+      r.put(1, t, r.in(1, t) - r.in(2, t));
+    }
+    return;
+  }
+  double b, k;
+  ema_bk(p, b, k);
+  double s = 0.0;
+  for (int t = 0; t < p; ++t) s += r.in(2, t);
+  double ma = s / p;
+  r.put(0, p - 1, r.in(0, p - 1) - ma);
+  r.put(1, p - 1, r.in(1, p - 1) - ma);
+  for (int t = p; t < r.n; ++t) {
+    ma = fma(b, ma, k * r.in(2, t));
+    r.put(0, t, r.in(0, t) - ma);
+    r.put(1, t, r.in(1, t) - ma);
+  }
+}
+
+// ta_PERCENTRANK: 100 x the share of the p bars before t below x[t]
+__device__ void f_percentrank(const Row& r, int p) {
+  for (int t = p; t < r.n; ++t) {
+    const double c = r.in(0, t);
+    int below = 0;
+    for (int s = t - p; s < t; ++s) below += r.in(0, s) < c;
+    r.put(0, t, (double)below / (double)p * 100.0);
+  }
+}
+
+// Wilder's swing index of bar t (inputs open, high, low, close) against bar
+// t - 1, limit move lim (ta_SI, ta_ASI)
+__device__ __forceinline__ double swing_index(const Row& r, int t, double lim) {
+  const double pc = r.in(3, t - 1), pb = pc - r.in(0, t - 1), body = r.in(3, t) - r.in(0, t);
+  double num = r.in(3, t) - pc;
+  num += 0.5 * body;
+  num += 0.25 * pb;
+  const double up = fabs(r.in(1, t) - pc), dn = fabs(r.in(2, t) - pc);
+  const double rg = fabs(r.in(1, t) - r.in(2, t));
+  const double kk = up > dn ? up : dn;
+  const double a = up - 0.5 * dn, c = dn - 0.5 * up;
+  double den = a > c ? a : c;
+  den = den > rg ? den : rg;
+  den += 0.25 * fabs(pb);
+  return den == 0.0 ? 0.0 : 50.0 * (num / den) * (kk / lim);
+}
+__device__ void f_si(const Row& r, double lim) {
+  for (int t = 1; t < r.n; ++t) r.put(0, t, swing_index(r, t, lim));
+}
+__device__ void f_asi(const Row& r, double lim) {
+  double acc = 0.0;
+  r.put(0, 0, acc);
+  for (int t = 1; t < r.n; ++t) {
+    acc += swing_index(r, t, lim);
+    r.put(0, t, acc);
+  }
+}
+
 template <int F>
-__device__ void run_row(const Row& r, int p0, int p1) {
+__device__ void run_row(const Row& r, int p0, int p1, double d0) {
   if constexpr (F == kCumsum) f_cumsum(r);
   else if constexpr (F == kPvt) f_pvt(r);
   else if constexpr (F == kRma) f_rma(r, p0);
@@ -460,7 +640,14 @@ __device__ void run_row(const Row& r, int p0, int p1) {
   else if constexpr (F == kMarketfi) f_marketfi(r);
   else if constexpr (F == kWad) f_wad(r);
   else if constexpr (F == kMcgd) f_mcgd(r, p0);
-  else f_rvol(r, p0);
+  else if constexpr (F == kRvol) f_rvol(r, p0);
+  else if constexpr (F == kCmf) f_cmf(r, p0);
+  else if constexpr (F == kEmv) f_emv(r, p0, d0);
+  else if constexpr (F == kFosc) f_fosc(r, p0);
+  else if constexpr (F == kEri) f_eri(r, p0);
+  else if constexpr (F == kPercentrank) f_percentrank(r, p0);
+  else if constexpr (F == kSi) f_si(r, d0);
+  else f_asi(r, d0);
 }
 
 // SPREAD: one row per wave (lane 0 works, the block is one wave); else one
@@ -482,15 +669,15 @@ __global__ __launch_bounds__(64) void k_rows(RowArgs a) {
     if (!any) break;
   }
   Row r;
-  for (int j = 0; j < 3; ++j) r.x[j] = j < S.nin ? a.in[j] + base + f : nullptr;
+  for (int j = 0; j < 4; ++j) r.x[j] = j < S.nin ? a.in[j] + base + f : nullptr;
   for (int k = 0; k < 3; ++k) r.y[k] = k < S.nout ? a.out[k] + base + f : nullptr;
   r.n = a.len - f;
-  run_row<F>(r, a.p0, a.p1);
+  run_row<F>(r, a.p0, a.p1, a.d0);
 }
 
 template <int F>
 ctStatus_t run_rows(const char* fn, const float* const* in, float* const* out, int len,
-                    int rows, int p0, int p1, hipStream_t s) {
+                    int rows, int p0, int p1, double d0, hipStream_t s) {
   constexpr RowSpec S = kSpec[F];
   if (len <= 0 || rows <= 0)
     return fail(CT_STATUS_INVALID_ARGUMENT, "%s: size/seriesLen and numSeries must be positive",
@@ -508,6 +695,9 @@ ctStatus_t run_rows(const char* fn, const float* const* in, float* const* out, i
   for (int k = 0; k < S.npar; ++k)
     if (par[k] < S.lo[k] || par[k] > S.hi[k])
       return fail(CT_STATUS_INVALID_ARGUMENT, "%s: period out of range", fn);
+  if (S.dlo <= S.dhi && !(d0 >= S.dlo && d0 <= S.dhi))
+    return fail(CT_STATUS_INVALID_ARGUMENT, "%s: parameter out of range", fn);
+  a.d0 = d0;
   a.len = len;
   a.rows = rows;
   a.p0 = p0;
@@ -531,14 +721,14 @@ ctStatus_t run_rows(const char* fn, const float* const* in, float* const* out, i
 // backend does (Talib08.cu:143-144).
 template <int F>
 ctStatus_t dev_rows(const char* fn, const float* const* in, float* const* out, int len, int rows,
-                    int p0, int p1, hipStream_t s) {
-  RF_RET(run_rows<F>(fn, in, out, len, rows, p0, p1, s));
+                    int p0, int p1, double d0, hipStream_t s) {
+  RF_RET(run_rows<F>(fn, in, out, len, rows, p0, p1, d0, s));
   return finish_batch(s);
 }
 
 template <int F>
 ctStatus_t host_rows(const char* fn, const float* const* hin, float* const* hout, int len,
-                     int rows, int p0, int p1, hipStream_t s) {
+                     int rows, int p0, int p1, double d0, hipStream_t s) {
   constexpr RowSpec S = kSpec[F];
   if (len <= 0 || rows <= 0)
     return fail(CT_STATUS_INVALID_ARGUMENT, "%s: size/seriesLen and numSeries must be positive",
@@ -549,11 +739,11 @@ ctStatus_t host_rows(const char* fn, const float* const* hin, float* const* hout
     if (!hout[k]) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: null pointer", fn);
   const size_t n = (size_t)len * (size_t)rows;
   HostRun hr(s);
-  const float* d_in[3] = {nullptr, nullptr, nullptr};
+  const float* d_in[4] = {nullptr, nullptr, nullptr, nullptr};
   float* d_out[3] = {nullptr, nullptr, nullptr};
   for (int j = 0; j < S.nin; ++j) RF_RET(hr.in(hin[j], n, &d_in[j]));
   for (int k = 0; k < S.nout; ++k) RF_RET(hr.out(hout[k], n, &d_out[k]));
-  RF_RET(run_rows<F>(fn, d_in, d_out, len, rows, p0, p1, s));
+  RF_RET(run_rows<F>(fn, d_in, d_out, len, rows, p0, p1, d0, s));
   return hr.finish();
 }
 
@@ -566,42 +756,50 @@ using namespace tca;
 #define RF_IN1 const float* i0
 #define RF_IN2 const float* i0, const float* i1
 #define RF_IN3 const float* i0, const float* i1, const float* i2
-#define RF_INA1 {i0, nullptr, nullptr}
-#define RF_INA2 {i0, i1, nullptr}
-#define RF_INA3 {i0, i1, i2}
+#define RF_IN4 const float* i0, const float* i1, const float* i2, const float* i3
+#define RF_INA1 {i0, nullptr, nullptr, nullptr}
+#define RF_INA2 {i0, i1, nullptr, nullptr}
+#define RF_INA3 {i0, i1, i2, nullptr}
+#define RF_INA4 {i0, i1, i2, i3}
 #define RF_OUT1 float* o0
+#define RF_OUT2 float* o0, float* o1
 #define RF_OUT3 float* o0, float* o1, float* o2
 #define RF_OUTA1 {o0, nullptr, nullptr}
+#define RF_OUTA2 {o0, o1, nullptr}
 #define RF_OUTA3 {o0, o1, o2}
 #define RF_P0
 #define RF_P1 , int p0
 #define RF_P2 , int p0, int p1
-#define RF_PV0 0, 0
-#define RF_PV1 p0, 0
-#define RF_PV2 p0, p1
+#define RF_PD , double d0
+#define RF_P1D , int p0, double d0
+#define RF_PV0 0, 0, 0.0
+#define RF_PV1 p0, 0, 0.0
+#define RF_PV2 p0, p1, 0.0
+#define RF_PVD 0, 0, d0
+#define RF_PV1D p0, 0, d0
 
 #define TCA_ROWFN(NAME, F, NI, NO, NP)                                                           \
   extern "C" {                                                                                   \
   CTAPI_EXPORT ctStatus_t ct_##NAME##_d(RF_IN##NI, RF_OUT##NO, int n RF_P##NP, hipStream_t s) {  \
-    const float* in[3] = RF_INA##NI;                                                             \
+    const float* in[4] = RF_INA##NI;                                                             \
     float* out[3] = RF_OUTA##NO;                                                                 \
     return guarded([&] { return dev_rows<F>("ct_" #NAME "_d", in, out, n, 1, RF_PV##NP, s); });  \
   }                                                                                              \
   CTAPI_EXPORT ctStatus_t ct_##NAME##_batch_d(RF_IN##NI, RF_OUT##NO, int len,                    \
                                               int rows RF_P##NP, int, hipStream_t s) {           \
-    const float* in[3] = RF_INA##NI;                                                             \
+    const float* in[4] = RF_INA##NI;                                                             \
     float* out[3] = RF_OUTA##NO;                                                                 \
     return guarded(                                                                              \
         [&] { return dev_rows<F>("ct_" #NAME "_batch_d", in, out, len, rows, RF_PV##NP, s); });  \
   }                                                                                              \
   CTAPI_EXPORT ctStatus_t ct_##NAME(RF_IN##NI, RF_OUT##NO, int n RF_P##NP, hipStream_t s) {      \
-    const float* in[3] = RF_INA##NI;                                                             \
+    const float* in[4] = RF_INA##NI;                                                             \
     float* out[3] = RF_OUTA##NO;                                                                 \
     return guarded([&] { return host_rows<F>("ct_" #NAME, in, out, n, 1, RF_PV##NP, s); });      \
   }                                                                                              \
   CTAPI_EXPORT ctStatus_t ct_##NAME##_batch(RF_IN##NI, RF_OUT##NO, int len, int rows RF_P##NP,   \
                                             int, hipStream_t s) {                                \
-    const float* in[3] = RF_INA##NI;                                                             \
+    const float* in[4] = RF_INA##NI;                                                             \
     float* out[3] = RF_OUTA##NO;                                                                 \
     return guarded(                                                                              \
         [&] { return host_rows<F>("ct_" #NAME "_batch", in, out, len, rows, RF_PV##NP, s); });   \
@@ -626,3 +824,10 @@ TCA_ROWFN(marketfi, kMarketfi, 3, 1, 0)
 TCA_ROWFN(wad, kWad, 3, 1, 0)
 TCA_ROWFN(mcgd, kMcgd, 1, 1, 1)
 TCA_ROWFN(rvol, kRvol, 1, 1, 1)
+TCA_ROWFN(cmf, kCmf, 4, 1, 1)
+TCA_ROWFN(emv, kEmv, 3, 1, 1D)
+TCA_ROWFN(fosc, kFosc, 1, 1, 1)
+TCA_ROWFN(eri, kEri, 3, 2, 1)
+TCA_ROWFN(percentrank, kPercentrank, 1, 1, 1)
+TCA_ROWFN(si, kSi, 4, 1, D)
+TCA_ROWFN(asi_wilder, kAsiWilder, 4, 1, D)

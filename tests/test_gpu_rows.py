@@ -1,5 +1,6 @@
 """TA-Lib 0.8.3 row functions (CUMSUM, PVT, RMA, ZLEMA, EFI, VWMA, ER, VHF,
-DPO, CHOP, ADR, AO, QSTICK, DONCHIAN, MARKETFI, WAD, MCGD, RVOL) on the GPU
+DPO, CHOP, ADR, AO, QSTICK, DONCHIAN, MARKETFI, WAD, MCGD, RVOL, CMF, EMV,
+FOSC, ERI, PERCENTRANK, SI, ASI_WILDER) on the GPU
 against the oracle, bit for bit: device single calls on long rows (one wave
 per row), dense batches of many short rows (one lane per row) with leading
 and interior NaNs, zero-volume and unchanged-price runs, host single and
@@ -7,6 +8,8 @@ batch calls, period edges and argument errors.  CHOP takes log10 in double,
 where the device library and the host's libm may round the last double bit
 apart: at least 99.9% of its outputs must be bit-identical and the rest
 within 1e-6 relative."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -17,7 +20,7 @@ pytestmark = pytest.mark.gpu
 
 torch = pytest.importorskip("torch")
 
-NOUT = {"donchian": 3}
+NOUT = {"donchian": 3, "eri": 2}
 
 
 def _check(name, got, want, what):
@@ -33,7 +36,12 @@ def _want(name, ins, p):
 
 
 def _call_args(name, p):
-    return list(p[:NPAR.get(name, 1)])
+    k = NPAR.get(name, 1)
+    if k == "d":
+        return [ctypes.c_double(p[2])]
+    if k == "1d":
+        return [p[0], ctypes.c_double(p[2])]
+    return list(p[:k])
 
 
 @pytest.mark.parametrize("name", list(SPEC))
@@ -125,3 +133,8 @@ def test_row_edges_and_errors(dev):
                            None) == 4
     assert L.ct_cumsum_batch_d(x.data_ptr(), y.data_ptr(), 40, 0, 1, None) == 4
     assert L.ct_wad(None, None, None, None, 40, None) == 4
+    assert L.ct_percentrank_d(x.data_ptr(), y.data_ptr(), 40, 10001, None) == 4
+    assert L.ct_emv_d(x.data_ptr(), x.data_ptr(), x.data_ptr(), y.data_ptr(), 40, 5,
+                      ctypes.c_double(0.5), None) == 4
+    assert L.ct_si_d(x.data_ptr(), x.data_ptr(), x.data_ptr(), x.data_ptr(), y.data_ptr(), 40,
+                     ctypes.c_double(0.0), None) == 4

@@ -15,11 +15,16 @@ needs_ref = pytest.mark.skipif(ref() is None, reason="reference TA-Lib host buil
 SPEC = {"cumsum": "x", "pvt": "cv", "rma": "x", "zlema": "x", "efi": "cv", "vwma": "xv",
         "er": "x", "vhf": "x", "dpo": "x", "chop": "hlc", "adr": "hl", "ao": "hl",
         "qstick": "oc", "donchian": "hl", "marketfi": "hlv", "wad": "hlc", "mcgd": "x",
-        "rvol": "v"}
-NPAR = {"cumsum": 0, "pvt": 0, "marketfi": 0, "wad": 0, "ao": 2}
+        "rvol": "v", "cmf": "hlcv", "emv": "hlv", "fosc": "x", "eri": "hlc", "percentrank": "x",
+        "si": "ohlc", "asi_wilder": "ohlc"}
+# integer periods: 0, 1 or 2; "d": one real parameter; "1d": a period and a real
+NPAR = {"cumsum": 0, "pvt": 0, "marketfi": 0, "wad": 0, "ao": 2, "emv": "1d", "si": "d",
+        "asi_wilder": "d"}
 PERIODS = {1: [(1, 0), (2, 0), (14, 0), (57, 0)], 2: [(2, 3), (5, 34), (34, 5), (9, 9)],
-           0: [(0, 0)]}
-MIN_P = {"er": 2, "vhf": 2, "dpo": 2, "chop": 2, "donchian": 2, "mcgd": 2}
+           0: [(0, 0)], "d": [(0, 0, 3.0), (0, 0, 0.5), (0, 0, 1e4)],
+           "1d": [(1, 0, 1.0), (2, 0, 1e4), (14, 0, 1e8), (57, 0, 10.0)]}
+MIN_P = {"er": 2, "vhf": 2, "dpo": 2, "chop": 2, "donchian": 2, "mcgd": 2, "cmf": 2, "fosc": 2,
+         "percentrank": 2}
 
 
 def bars(n, seed, holes=False):
@@ -45,7 +50,7 @@ def inputs(name, d):
 
 def params(name):
     k = NPAR.get(name, 1)
-    return [p for p in PERIODS[k] if k != 1 or p[0] >= MIN_P.get(name, 1)]
+    return [p for p in PERIODS[k] if k not in (1, "1d") or p[0] >= MIN_P.get(name, 1)]
 
 
 def bits_equal(a, b, what):
@@ -71,12 +76,13 @@ def test_rows_match_vendored_talib(name, seed, holes):
 @needs_ref
 @pytest.mark.parametrize("name", [n for n in SPEC if NPAR.get(n, 1) == 1])
 def test_rows_short_series_and_invalid_periods(name):
+    hi = 10000 if name == "percentrank" else 100000
     d = bars(40, 1)
     lo = MIN_P.get(name, 1)
     for p in (lo, 39, 40, 41, 1000):  # around the series length
         bits_equal(ORC.rows(name, inputs(name, d), p), REF.rows(name, inputs(name, d), p),
                    f"{name} p{p}")
-    for bad in (lo - 1, 100001):
+    for bad in (lo - 1, hi + 1):
         with pytest.raises(ValueError):
             ORC.rows(name, inputs(name, d), bad)
         with pytest.raises(ValueError):
@@ -91,6 +97,20 @@ def test_rows_all_nan_row_keeps_the_last_bar():
     for name in SPEC:
         p = params(name)[0]
         bits_equal(ORC.rows(name, inputs(name, d), *p), REF.rows(name, inputs(name, d), *p), name)
+
+
+@needs_ref
+def test_rows_real_parameter_ranges():
+    d = bars(60, 4)
+    for name, good, bad in (("emv", (5, 0, 1.0), (5, 0, 0.999)), ("emv", (5, 0, 3e37), (5, 0, 4e37)),
+                            ("si", (0, 0, 1e-8), (0, 0, 5e-9)),
+                            ("asi_wilder", (0, 0, 2.0), (0, 0, float("nan")))):
+        bits_equal(ORC.rows(name, inputs(name, d), *good), REF.rows(name, inputs(name, d), *good),
+                   f"{name}{good}")
+        with pytest.raises(ValueError):
+            ORC.rows(name, inputs(name, d), *bad)
+        with pytest.raises(ValueError):
+            REF.rows(name, inputs(name, d), *bad)
 
 
 def test_rows_closed_forms():
@@ -118,3 +138,13 @@ def test_rows_closed_forms():
     v = np.array([2, 0, 4], np.float32)
     assert ORC.rows("marketfi", [np.array([5, 5, 9], np.float32),
                                  np.array([1, 1, 1], np.float32), v]).tolist() == [2.0, 0.0, 2.0]
+    # PERCENTRANK of a rising series is 100, of a falling one 0
+    assert np.all(ORC.rows("percentrank", [x], 5)[5:] == 100.0)
+    assert np.all(ORC.rows("percentrank", [x[::-1].copy()], 5)[5:] == 0.0)
+    # ERI of constant bars: bull = high - close, bear = low - close
+    hh, ll, cc = np.full(n, 5.0, np.float32), np.full(n, 1.0, np.float32), c
+    bull, bear = ORC.rows("eri", [hh, ll, cc], 4)
+    assert np.all(bull[3:] == 1.5) and np.all(bear[3:] == -2.5)
+    # CMF with the close at the high on every bar: money-flow multiplier 1
+    vv = np.full(n, 7.0, np.float32)
+    assert np.all(ORC.rows("cmf", [hh, ll, hh, vv], 5)[4:] == 1.0)
