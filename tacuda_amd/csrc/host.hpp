@@ -154,22 +154,17 @@ bool scan_serves(int len, int rows, int windowLookback);
 // The family calls of EMA, RSI, MACD and ATR on rows of at least 8192 bars run
 // the fused pass (the pack's outputs for that family, the same
 // oracle-following arithmetic): its one-launch form wherever that serves (at
-// most 1024 rows, 16384 512-bar segments and 1,050,624 bars per row; as fast
-// as or faster than the look-back kernels at every measured shape), else its
-// two passes for dense batches of at least kPipeMinRows rows and
-// kPipeMinSegments segments in all, or for fewer rows holding at least
-// fewRowsMinSegs segments (the look-back kernels' chains grow with the row
-// length, the passes' launches do not): *served says whether it did; when
-// not, nothing was launched and the caller takes its own path.
-constexpr int kPipeMinRows = 16;
-constexpr long long kPipeMinSegments = 512;
-constexpr long long kPipeFewRowsSegs = 4096;      // ATR
-constexpr long long kPipeFewRowsSegsRsi = 1900;  // one row of 10^6 bars
-constexpr long long kPipeFewRowsSegsEma = 2048;   // below the long scan's batches
-constexpr long long kPipeFewRowsSegsMacd = 1024;  // MACD's three-level look-back
+// most 1024 rows, 16384 512-bar segments and 1,050,624 bars per row), its
+// two passes for everything larger.  Both were measured at least as fast as
+// the per-family look-back kernels on every shape they take (one row of 8192
+// to 2.2 million bars, up to 256 rows of 16384 and 8 of 2^20; within 2% on
+// one ATR row of 1.1-1.5 million bars, where the look-back chains are
+// shortest relative to the passes' fixed launches).  *served says whether
+// the pass ran; when not, nothing was launched and the caller takes its own
+// path.
 ctStatus_t pipe_family(const ctxPackParams& params, const ctxPackOutputs& outputs,
                        const float* h, const float* l, const float* c, int len, int rows,
-                       hipStream_t stream, long long fewRowsMinSegs, bool* served);
+                       hipStream_t stream, bool* served);
 // The family calls of SMA, WMA, BBANDS, STOCH, WILLR and CCI (dense rows of
 // any length, window look-backs within the pack's 1024 bars) run the fused
 // pass's window kernel: one wave per 256-bar column tile, every window

@@ -2612,17 +2612,6 @@ static ctStatus_t launch_pipe_solo(PipeP& P, const PackP& prm, const PackIO& io,
   return CT_STATUS_SUCCESS;
 }
 
-// Whether launch_pipe takes the one-launch form for this pass and shape.
-static bool pipe_one_launch(int len, int rows, const PackP& prm) {
-  PipeP P{};
-  P.rows = rows;
-  P.segsPerRow = div_up(len, kPSeg);
-  P.wE = (prm.mask & (1u << kOEma)) ? 1 : 0;
-  P.wR = (prm.mask & (1u << kORsi)) ? 1 : 0;
-  P.wM = (prm.mask & ((1u << kOMacd) | (1u << kOMacdSig) | (1u << kOMacdHist))) ? 1 : 0;
-  P.wA = (prm.mask & (1u << kOAtr)) ? 1 : 0;
-  return solo_serves(P, prm);
-}
 
 ctStatus_t launch_pipe(int len, int rows, hipStream_t s, const PackP& prm, const PackIO& io) {
   PipeP P{};
@@ -2862,7 +2851,7 @@ namespace tca {
 
 ctStatus_t pipe_family(const ctxPackParams& params, const ctxPackOutputs& outputs,
                        const float* h, const float* l, const float* c, int len, int rows,
-                       hipStream_t stream, long long fewRowsMinSegs, bool* served) {
+                       hipStream_t stream, bool* served) {
   *served = false;
   if (t_ragged || len < kPMinLen) return CT_STATUS_SUCCESS;
   PackP P;
@@ -2872,14 +2861,8 @@ ctStatus_t pipe_family(const ctxPackParams& params, const ctxPackOutputs& output
     return CT_STATUS_SUCCESS;
   }
   if (!pipe_serves(len, P)) return CT_STATUS_SUCCESS;
-  // the one-launch form, where it serves, is at least as fast as the
-  // look-back kernels at every measured shape (one row of 8192 to 10^6
-  // bars, up to 256 rows of 16384); beyond it, the two passes take dense
-  // batches and rows long enough that the look-back chains cost more
-  const long long segs = (long long)rows * div_up(len, kPSeg);
-  const bool dense = rows >= kPipeMinRows && segs >= kPipeMinSegments;
-  if (!pipe_one_launch(len, rows, P) && !(dense || segs >= fewRowsMinSegs))
-    return CT_STATUS_SUCCESS;
+  // launch_pipe takes the one-launch form where it serves, the two passes
+  // otherwise
   *served = true;
   return launch_pipe(len, rows, stream, P, io);
 }

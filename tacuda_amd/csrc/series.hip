@@ -277,7 +277,7 @@ ctStatus_t dev_ema(const float* x, float* y, int len, int rows, int p, hipStream
     ctxPackOutputs po{};
     po.ema = y;
     bool served = false;
-    CHECK_RET(pipe_family(pp, po, x, x, x, len, rows, s, kPipeFewRowsSegsEma, &served));
+    CHECK_RET(pipe_family(pp, po, x, x, x, len, rows, s, &served));
     if (served) return CT_STATUS_SUCCESS;
   }
   return run_Ema(len, rows, s, {p}, io1(x, y));
@@ -334,7 +334,7 @@ ctStatus_t dev_rsi(const float* x, float* y, int len, int rows, int p, hipStream
     ctxPackOutputs po{};
     po.rsi = y;
     bool served = false;
-    CHECK_RET(pipe_family(pp, po, x, x, x, len, rows, s, kPipeFewRowsSegsRsi, &served));
+    CHECK_RET(pipe_family(pp, po, x, x, x, len, rows, s, &served));
     if (served) return CT_STATUS_SUCCESS;
   }
   return run_Rsi(len, rows, s, {p}, io1(x, y));
@@ -359,7 +359,7 @@ ctStatus_t dev_macd(const float* x, float* m, float* sg, float* h, int len, int 
     po.macd_signal = sg;
     po.macd_hist = h;
     bool served = false;
-    CHECK_RET(pipe_family(pp, po, x, x, x, len, rows, s, kPipeFewRowsSegsMacd, &served));
+    CHECK_RET(pipe_family(pp, po, x, x, x, len, rows, s, &served));
     if (served) return CT_STATUS_SUCCESS;
   }
   return run_Macd(len, rows, s, {fast, slow, sig}, io1(x, m, sg, h));
@@ -419,7 +419,7 @@ ctStatus_t dev_atr(const float* h, const float* l, const float* c, float* y, int
     ctxPackOutputs po{};
     po.atr = y;
     bool served = false;
-    CHECK_RET(pipe_family(pp, po, h, l, c, len, rows, s, kPipeFewRowsSegs, &served));
+    CHECK_RET(pipe_family(pp, po, h, l, c, len, rows, s, &served));
     if (served) return CT_STATUS_SUCCESS;
   }
   return run_Atr(len, rows, s, {p}, io3(h, l, c, y));
