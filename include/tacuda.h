@@ -460,6 +460,18 @@ CTAPI_EXPORT ctStatus_t ct_midprice_batch_d(const float* d_highs, const float* d
 /* :3285 */ CTAPI_EXPORT ctStatus_t ct_asi_wilder_batch(const float* host_opens, const float* host_highs, const float* host_lows, const float* host_closes, float* host_outputs, int seriesLen, int numSeries, double limitMove, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
 /* :3293 */ CTAPI_EXPORT ctStatus_t ct_asi_wilder_batch_d(const float* d_opens, const float* d_highs, const float* d_lows, const float* d_closes, float* d_outputs, int seriesLen, int numSeries, double limitMove, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
 
+/* ---------------------------------------------------------------------------
+ * OBV and AD (csrc/families.hpp FCumT on the look-back scan engine): running
+ * sums in double of sign(close change) x volume (OBV; bar 0 the volume) and
+ * of the close-location value x volume (AD), rounded to float per bar.
+ * ------------------------------------------------------------------------- */
+/* :367 */ CTAPI_EXPORT ctStatus_t ct_obv(const float* host_price, const float* host_volume, float* host_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :400 */ CTAPI_EXPORT ctStatus_t ct_ad(const float* host_high, const float* host_low, const float* host_close, const float* host_volume, float* host_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1435 */ CTAPI_EXPORT ctStatus_t ct_obv_batch(const float* host_closes, const float* host_volumes, float* host_outputs, int seriesLen, int numSeries, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1440 */ CTAPI_EXPORT ctStatus_t ct_obv_batch_d(const float* d_closes, const float* d_volumes, float* d_outputs, int seriesLen, int numSeries, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1461 */ CTAPI_EXPORT ctStatus_t ct_ad_batch(const float* host_highs, const float* host_lows, const float* host_closes, const float* host_volumes, float* host_outputs, int seriesLen, int numSeries, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1468 */ CTAPI_EXPORT ctStatus_t ct_ad_batch_d(const float* d_highs, const float* d_lows, const float* d_closes, const float* d_volumes, float* d_outputs, int seriesLen, int numSeries, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+
 #ifdef __cplusplus
 }
 #endif

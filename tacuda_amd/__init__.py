@@ -323,6 +323,15 @@ for _n, (_i, _o, _p) in ROW_FUNCTIONS.items():
         _sig(_f, *([P] * (_i + _o)), c_int, *_PARAMS[_p])
     for _f in (f"ct_{_n}_batch_d", f"ct_{_n}_batch"):
         _sig(_f, *([P] * (_i + _o)), c_int, c_int, *_PARAMS[_p], c_int)
+# OBV and AD (running sums on the look-back scan engine)
+for _f in ("ct_obv_d", "ct_obv"):
+    _sig(_f, P, P, P, c_int)
+for _f in ("ct_obv_batch_d", "ct_obv_batch"):
+    _sig(_f, P, P, P, c_int, c_int, c_int)
+for _f in ("ct_ad_d", "ct_ad"):
+    _sig(_f, P, P, P, P, P, c_int)
+for _f in ("ct_ad_batch_d", "ct_ad_batch"):
+    _sig(_f, P, P, P, P, P, c_int, c_int, c_int)
 _sig("ctx_macd_bwd_d", P, P, P, P, c_int, c_int, c_int, c_int, c_int)
 _sig("ctx_bbands_bwd_d", P, P, P, P, P, c_int, c_int, c_int, c_float, c_float)
 

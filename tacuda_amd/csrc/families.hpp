@@ -1563,6 +1563,79 @@ struct FTrange {
   }
 };
 
+// OBV and AD: running sums in double of a per-bar term, rounded to float
+// per bar (reference OBV.cu:17-34 and AD.cu:22-71: the terms in double, a
+// double inclusive scan, a float cast).  OBV's term at bar 0 is the volume,
+// after that sign(close - previous close) x volume (0 when unchanged or
+// either close is NaN); AD's is ((close - low) - (high - close)) / (high -
+// low) x volume for finite bars with a positive range, 0 otherwise.  Bar 0
+// is the scan's seed; every later bar adds its term, so the segment
+// transforms are translations (A = 1) chained by the look-back like any
+// other recurrence.  For volumes and prices of ordinary range every partial
+// OBV sum is exact, so any association gives the sequential sums bit for
+// bit; AD's quotients are rounded, so its sums may differ from a sequential
+// loop in the last double bits (the reference's scan order differs too).
+template <bool AD>
+struct FCumT {
+  static constexpr bool kScan = true;
+  static constexpr int kLevels = 1;
+  static constexpr int kMaxR = 1;
+  struct P {
+    int unused;
+  };
+  template <class T>
+  __device__ __forceinline__ static double term(const float* li, const IO& io, const Seg& c,
+                                                int t) {
+    if constexpr (AD) {
+      const double h = at<T>(li, io.in[0], c, t), l = at<T>(li + T::PLEN, io.in[1], c, t);
+      const double cl = at<T>(li + 2 * T::PLEN, io.in[2], c, t);
+      const double v = at<T>(li + 3 * T::PLEN, io.in[3], c, t);
+      if (!(__builtin_isfinite(h) && __builtin_isfinite(l) && __builtin_isfinite(cl) &&
+            __builtin_isfinite(v)))
+        return 0.0;
+      const double range = h - l;
+      return range > 0.0 ? (((cl - l) - (h - cl)) / range) * v : 0.0;
+    } else {
+      const double v = at<T>(li + T::PLEN, io.in[1], c, t);
+      if (t == 0) return v;
+      const double p = at<T>(li, io.in[0], c, t), q = at<T>(li, io.in[0], c, t - 1);
+      const double sg = p > q ? 1.0 : (p < q ? -1.0 : 0.0);
+      return sg * v;
+    }
+  }
+  template <class T>
+  __device__ __forceinline__ static void run(const Seg& c, float* li, float* lo, const P&,
+                                             const IO& io) {
+    constexpr int NIN = AD ? 4 : 2;
+#pragma unroll
+    for (int j = 0; j < NIN; ++j) stage<T>(li + j * T::PLEN, io.in[j], c);
+    wave_sync();
+    const int a = c.s0 + lane_id() * T::K;
+    double tv[T::K];
+    Aff<1> loc;
+    aff_id(loc);
+#pragma unroll
+    for (int i = 0; i < T::K; ++i) {
+      const int t = a + i;
+      tv[i] = t < c.n ? term<T>(li, io, c, t) : 0.0;
+      if (t > 0 && t < c.n) loc.b[0] = loc.b[0] + tv[i];
+    }
+    double seed[1] = {c.s0 == 0 ? term<T>(li, io, c, 0) : 0.0}, yin[1];
+    level_carry<1>(c, 0, 0, seed, loc, yin);
+    double y = yin[0];
+    float ov[T::K];
+#pragma unroll
+    for (int i = 0; i < T::K; ++i) {
+      const int t = a + i;
+      y = t == 0 ? tv[i] : y + tv[i];
+      ov[i] = (float)y;
+    }
+    store_lane<T>(lo, ov, io.out[0], c);
+  }
+};
+using FObv = FCumT<false>;
+using FAd = FCumT<true>;
+
 // STOCH (ta_STOCH.c:36-322): fastK over kP, SMA(dP) twice; both outputs
 // start at kP-1 + 2(dP-1).  For dP <= kStochMaxD the fastK values of the
 // lane's bars and the 2(dP-1) before them are kept in registers; longer

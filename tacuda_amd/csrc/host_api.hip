@@ -391,6 +391,42 @@ CTAPI_EXPORT ctStatus_t ct_t3_batch(const float* hx, float* hy, int len, int row
     return r.finish();
   });
 }
+CTAPI_EXPORT ctStatus_t ct_obv(const float* hc, const float* hv, float* hy, int n, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    R_(check_shape("obv", n, 1));
+    HostRun r(s); const float *c, *v; float* y;
+    R_(r.in(hc, n, &c)); R_(r.in(hv, n, &v)); R_(r.out(hy, n, &y));
+    R_(dev_obv(c, v, y, n, 1, s));
+    return r.finish();
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_obv_batch(const float* hc, const float* hv, float* hy, int len, int rows, int, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    R_(check_shape("obv", len, rows)); const size_t n = elems(len, rows);
+    HostRun r(s); const float *c, *v; float* y;
+    R_(r.in(hc, n, &c)); R_(r.in(hv, n, &v)); R_(r.out(hy, n, &y));
+    R_(dev_obv(c, v, y, len, rows, s));
+    return r.finish();
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_ad(const float* hh, const float* hl, const float* hc, const float* hv, float* hy, int n, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    R_(check_shape("ad", n, 1));
+    HostRun r(s); const float *h, *l, *c, *v; float* y;
+    R_(r.in(hh, n, &h)); R_(r.in(hl, n, &l)); R_(r.in(hc, n, &c)); R_(r.in(hv, n, &v)); R_(r.out(hy, n, &y));
+    R_(dev_ad(h, l, c, v, y, n, 1, s));
+    return r.finish();
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_ad_batch(const float* hh, const float* hl, const float* hc, const float* hv, float* hy, int len, int rows, int, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    R_(check_shape("ad", len, rows)); const size_t n = elems(len, rows);
+    HostRun r(s); const float *h, *l, *c, *v; float* y;
+    R_(r.in(hh, n, &h)); R_(r.in(hl, n, &l)); R_(r.in(hc, n, &c)); R_(r.in(hv, n, &v)); R_(r.out(hy, n, &y));
+    R_(dev_ad(h, l, c, v, y, len, rows, s));
+    return r.finish();
+  });
+}
 CTAPI_EXPORT ctStatus_t ct_trange_batch(const float* hh, const float* hl, const float* hc, float* hy, int len, int rows, int, hipStream_t s) {
   return guarded([&]() -> ctStatus_t {
     R_(check_shape("trange", len, rows)); const size_t n = elems(len, rows);

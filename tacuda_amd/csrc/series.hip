@@ -115,6 +115,8 @@ TCA_POLICY(Aroon, TCA_KL, 64, 4, 64, 2, 2)
 TCA_POLICY(AroonOsc, TCA_KL, 64, 4, 64, 2, 2)
 TCA_POLICY(Trange, TCA_KL, 16, 4, 4, 3, 2)
 TCA_POLICY(Stoch, 8, 64, 4, 64, 3, 2)
+TCA_POLICY(Obv, TCA_KL, 16, 4, 4, 2, 4)
+TCA_POLICY(Ad, TCA_KL, 16, 4, 4, 4, 2)
 #undef TCA_POLICY
 
 #define TCA_DIR_POLICY(NAME, KIND)                                                          \
@@ -431,6 +433,27 @@ ctStatus_t dev_natr(const float* h, const float* l, const float* c, float* y, in
   if (p <= 0) return bad("natr", "invalid period");
   return run_Natr(len, rows, s, {p}, io3(h, l, c, y));
 }
+ctStatus_t dev_obv(const float* c, const float* v, float* y, int len, int rows, hipStream_t s) {
+  CHECK_PTR(c, "obv"); CHECK_PTR(v, "obv"); CHECK_PTR(y, "obv");
+  CHECK_RET(check_shape("obv", len, rows));
+  IO io{};
+  io.in[0] = c;
+  io.in[1] = v;
+  io.out[0] = y;
+  return run_Obv(len, rows, s, {0}, io);
+}
+ctStatus_t dev_ad(const float* h, const float* l, const float* c, const float* v, float* y,
+                  int len, int rows, hipStream_t s) {
+  CHECK_PTR(h, "ad"); CHECK_PTR(l, "ad"); CHECK_PTR(c, "ad"); CHECK_PTR(v, "ad"); CHECK_PTR(y, "ad");
+  CHECK_RET(check_shape("ad", len, rows));
+  IO io{};
+  io.in[0] = h;
+  io.in[1] = l;
+  io.in[2] = c;
+  io.in[3] = v;
+  io.out[0] = y;
+  return run_Ad(len, rows, s, {0}, io);
+}
 ctStatus_t dev_trange(const float* h, const float* l, const float* c, float* y, int len, int rows,
                       hipStream_t s) {
   CHECK_PTR(h, "trange"); CHECK_PTR(l, "trange"); CHECK_PTR(c, "trange"); CHECK_PTR(y, "trange");
@@ -597,6 +620,14 @@ CTAPI_EXPORT ctStatus_t ct_kama_batch_d(const float* x, float* y, int len, int r
 }
 CTAPI_EXPORT ctStatus_t ct_t3_batch_d(const float* x, float* y, int len, int rows, int p, float v, int, hipStream_t s) {
   return guarded([&]() -> ctStatus_t { R_(dev_t3(x, y, len, rows, p, v, s)); return finish_batch(s); });
+}
+CTAPI_EXPORT ctStatus_t ct_obv_d(const float* c, const float* v, float* y, int n, hipStream_t s) { return guarded([&] { return dev_obv(c, v, y, n, 1, s); }); }
+CTAPI_EXPORT ctStatus_t ct_obv_batch_d(const float* c, const float* v, float* y, int len, int rows, int, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t { R_(dev_obv(c, v, y, len, rows, s)); return finish_batch(s); });
+}
+CTAPI_EXPORT ctStatus_t ct_ad_d(const float* h, const float* l, const float* c, const float* v, float* y, int n, hipStream_t s) { return guarded([&] { return dev_ad(h, l, c, v, y, n, 1, s); }); }
+CTAPI_EXPORT ctStatus_t ct_ad_batch_d(const float* h, const float* l, const float* c, const float* v, float* y, int len, int rows, int, hipStream_t s) {
+  return guarded([&]() -> ctStatus_t { R_(dev_ad(h, l, c, v, y, len, rows, s)); return finish_batch(s); });
 }
 CTAPI_EXPORT ctStatus_t ct_trange_batch_d(const float* h, const float* l, const float* c, float* y, int len, int rows, int, hipStream_t s) {
   return guarded([&]() -> ctStatus_t { R_(dev_trange(h, l, c, y, len, rows, s)); return finish_batch(s); });

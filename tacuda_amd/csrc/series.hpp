@@ -28,6 +28,9 @@ ctStatus_t dev_atr(const float* h, const float* l, const float* c, float* y, int
                    int p, hipStream_t s);
 ctStatus_t dev_natr(const float* h, const float* l, const float* c, float* y, int len, int rows,
                     int p, hipStream_t s);
+ctStatus_t dev_obv(const float* c, const float* v, float* y, int len, int rows, hipStream_t s);
+ctStatus_t dev_ad(const float* h, const float* l, const float* c, const float* v, float* y,
+                  int len, int rows, hipStream_t s);
 ctStatus_t dev_trange(const float* h, const float* l, const float* c, float* y, int len, int rows,
                       hipStream_t s);
 ctStatus_t dev_stoch(const float* h, const float* l, const float* c, float* k, float* d, int len,
