@@ -472,6 +472,17 @@ CTAPI_EXPORT ctStatus_t ct_midprice_batch_d(const float* d_highs, const float* d
 /* :1461 */ CTAPI_EXPORT ctStatus_t ct_ad_batch(const float* host_highs, const float* host_lows, const float* host_closes, const float* host_volumes, float* host_outputs, int seriesLen, int numSeries, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
 /* :1468 */ CTAPI_EXPORT ctStatus_t ct_ad_batch_d(const float* d_highs, const float* d_lows, const float* d_closes, const float* d_volumes, float* d_outputs, int seriesLen, int numSeries, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
 
+/* ---------------------------------------------------------------------------
+ * ULTOSC (float windows in the reference's order) and MFI (double typical
+ * price flows summed per window) in csrc/transforms.hip.
+ * ------------------------------------------------------------------------- */
+/* :396 */ CTAPI_EXPORT ctStatus_t ct_ultosc(const float* host_high, const float* host_low, const float* host_close, float* host_output, int size, int shortPeriod, int mediumPeriod, int longPeriod, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :360 */ CTAPI_EXPORT ctStatus_t ct_mfi(const float* host_high, const float* host_low, const float* host_close, const float* host_volume, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1318 */ CTAPI_EXPORT ctStatus_t ct_ultosc_batch(const float* host_highs, const float* host_lows, const float* host_closes, float* host_outputs, int seriesLen, int numSeries, int period1, int period2, int period3, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1326 */ CTAPI_EXPORT ctStatus_t ct_ultosc_batch_d(const float* d_highs, const float* d_lows, const float* d_closes, float* d_outputs, int seriesLen, int numSeries, int period1, int period2, int period3, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1445 */ CTAPI_EXPORT ctStatus_t ct_mfi_batch(const float* host_highs, const float* host_lows, const float* host_closes, const float* host_volumes, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1453 */ CTAPI_EXPORT ctStatus_t ct_mfi_batch_d(const float* d_highs, const float* d_lows, const float* d_closes, const float* d_volumes, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+
 #ifdef __cplusplus
 }
 #endif

@@ -308,6 +308,12 @@ class _Family:
             raise ValueError(f"rows {name} returned status {rc}")
         return outs[0] if nout == 1 else tuple(outs)
 
+    def ultosc(self, h, l, c, p1, p2, p3):
+        return _call(self._fn("ultosc", 3, 1, [_I, _I, _I]), [h, l, c], 1, len(c), p1, p2, p3)
+
+    def mfi(self, h, l, c, v, p):
+        return _call(self._fn("mfi", 4, 1, [_I]), [h, l, c, v], 1, len(c), p)
+
     def candle(self, name, o, h, l, c):
         """int32 signal of one of the 61 TA-Lib patterns (see candles.c)."""
         L = self._get()

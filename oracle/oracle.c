@@ -2102,3 +2102,59 @@ API int orc_rows(const char* name, const float* const* in, float* const* out, in
   free(buf);
   return ORC_OK;
 }
+
+/* ------------------------------------------------------- ULTOSC / MFI ---
+ * ULTOSC: reference ULTOSC.cu:15-39 (not TA-Lib's running sums): float
+ * buying pressure and true range summed newest first over each of the three
+ * (sorted) windows, 100 (4 a1 + 2 a2 + a3) / 7 in float.  pinned: integer
+ * bars where the float sums are exact, so the value is the exact formula;
+ * its identity BP + (high' - close) = TR per bar; and numpy brute force.
+ * MFI: reference MFI.cu:15-95 with the window summed directly (the value its
+ * prefix differences approximate): tp = (h+l+c)/3 in double, flow tp x v
+ * positive / negative by the tp change, 0 below a total of 1, else
+ * 100 pos / total.  pinned: numpy brute force and monotone closed forms. */
+API int orc_ultosc(const float* h, const float* l, const float* c, float* out, int n, int p1,
+                   int p2, int p3) {
+  if (!h || !l || !c || !out || n <= 0 || p1 <= 0 || p2 <= 0 || p3 <= 0) return ORC_BAD;
+  int q;
+  if (p1 > p2) { q = p1; p1 = p2; p2 = q; }
+  if (p2 > p3) { q = p2; p2 = p3; p3 = q; }
+  if (p1 > p2) { q = p1; p1 = p2; p2 = q; }
+  fill_nan(out, n);
+  for (int t = p3; t < n; ++t) {
+    float b1 = 0, r1 = 0, b2 = 0, r2 = 0, b3 = 0, r3 = 0;
+    for (int j = 0; j < p3; ++j) {
+      const int i = t - j;
+      const float pc = i > 0 ? c[i - 1] : c[i];
+      const float bp = c[i] - fminf(l[i], pc);
+      const float tr = fmaxf(h[i], pc) - fminf(l[i], pc);
+      if (j < p1) { b1 += bp; r1 += tr; }
+      if (j < p2) { b2 += bp; r2 += tr; }
+      b3 += bp;
+      r3 += tr;
+    }
+    const float a1 = r1 == 0.0f ? 0.0f : b1 / r1, a2 = r2 == 0.0f ? 0.0f : b2 / r2;
+    const float a3 = r3 == 0.0f ? 0.0f : b3 / r3;
+    out[t] = 100.0f * (4.0f * a1 + 2.0f * a2 + a3) / 7.0f;
+  }
+  return ORC_OK;
+}
+API int orc_mfi(const float* h, const float* l, const float* c, const float* v, float* out, int n,
+                int p) {
+  if (!h || !l || !c || !v || !out || n <= 0 || p < 2) return ORC_BAD;
+  fill_nan(out, n);
+  for (int t = p; t < n; ++t) {
+    double pos = 0.0, neg = 0.0;
+    for (int k = t - p + 1; k <= t; ++k) {
+      const double cur = ((double)h[k] + (double)l[k] + (double)c[k]) / 3.0;
+      const double prv = ((double)h[k - 1] + (double)l[k - 1] + (double)c[k - 1]) / 3.0;
+      if (!isfinite(cur) || !isfinite(prv)) continue;
+      const double f = cur * (double)v[k];
+      if (cur > prv) pos += f;
+      else if (cur < prv) neg += f;
+    }
+    const double tot = pos + neg;
+    out[t] = tot < 1.0 ? 0.0f : (float)(100.0 * (pos / tot));
+  }
+  return ORC_OK;
+}

@@ -332,6 +332,15 @@ for _f in ("ct_ad_d", "ct_ad"):
     _sig(_f, P, P, P, P, P, c_int)
 for _f in ("ct_ad_batch_d", "ct_ad_batch"):
     _sig(_f, P, P, P, P, P, c_int, c_int, c_int)
+# ULTOSC and MFI
+for _f in ("ct_ultosc_d", "ct_ultosc"):
+    _sig(_f, P, P, P, P, c_int, c_int, c_int, c_int)
+for _f in ("ct_ultosc_batch_d", "ct_ultosc_batch"):
+    _sig(_f, P, P, P, P, c_int, c_int, c_int, c_int, c_int, c_int)
+for _f in ("ct_mfi_d", "ct_mfi"):
+    _sig(_f, P, P, P, P, P, c_int, c_int)
+for _f in ("ct_mfi_batch_d", "ct_mfi_batch"):
+    _sig(_f, P, P, P, P, P, c_int, c_int, c_int, c_int)
 _sig("ctx_macd_bwd_d", P, P, P, P, c_int, c_int, c_int, c_int, c_int)
 _sig("ctx_bbands_bwd_d", P, P, P, P, P, c_int, c_int, c_int, c_float, c_float)
 

@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <utility>
 
 #include "../../include/tacuda.h"
 #include "../../include/tacuda_device.h"
@@ -210,6 +211,87 @@ __global__ __launch_bounds__(kTB) void k_minmax(const float* __restrict__ x,
   }
 }
 
+// ------------------------------------------------------ ULTOSC and MFI --
+// ULTOSC (reference ULTOSC.cu:15-39): over the p3 bars ending at t, newest
+// first, float buying pressure close - min(low, prev close) and true range
+// max(high, prev close) - min(low, prev close) (the first bar of a row is
+// its own previous close), summed into the p1-, p2- and p3-bar windows;
+// 100 (4 a1 + 2 a2 + a3) / 7 with a = bp / tr (0 on a zero range).  The
+// periods arrive sorted.
+__global__ __launch_bounds__(kTB) void k_ultosc(const float* __restrict__ h,
+                                                const float* __restrict__ l,
+                                                const float* __restrict__ c,
+                                                float* __restrict__ y, int len, long long total,
+                                                int p1, int p2, int p3) {
+  for (long long i = (long long)blockIdx.x * kTB + threadIdx.x; i < total;
+       i += (long long)gridDim.x * kTB) {
+    const int t = (int)(i % len);
+    if (t < p3) {
+      y[i] = qnan_f();
+      continue;
+    }
+    float bp1 = 0.0f, tr1 = 0.0f, bp2 = 0.0f, tr2 = 0.0f, bp3 = 0.0f, tr3 = 0.0f;
+    for (int j = 0; j < p3; ++j) {
+      const long long k = i - j;
+      const float pc = (t - j > 0) ? c[k - 1] : c[k];
+      const float lo = fminf(l[k], pc);
+      const float bp = c[k] - lo;
+      const float tr = fmaxf(h[k], pc) - lo;
+      if (j < p1) {
+        bp1 += bp;
+        tr1 += tr;
+      }
+      if (j < p2) {
+        bp2 += bp;
+        tr2 += tr;
+      }
+      bp3 += bp;
+      tr3 += tr;
+    }
+    const float a1 = tr1 == 0.0f ? 0.0f : bp1 / tr1;
+    const float a2 = tr2 == 0.0f ? 0.0f : bp2 / tr2;
+    const float a3 = tr3 == 0.0f ? 0.0f : bp3 / tr3;
+    y[i] = 100.0f * (4.0f * a1 + 2.0f * a2 + a3) / 7.0f;
+  }
+}
+
+// MFI (reference MFI.cu:15-95): typical price tp = (h + l + c) / 3 in
+// double, flow tp x volume counted positive when tp rose from the bar
+// before and negative when it fell (non-finite prices count nothing), summed
+// over the p bars ending at t; 0 when the total flow is below 1, else
+// 100 pos / total.  The windows are summed directly per bar (the reference
+// differences two prefix scans, which rounds differently and re-sums a
+// window whose difference went negative).
+__global__ __launch_bounds__(kTB) void k_mfi(const float* __restrict__ h,
+                                             const float* __restrict__ l,
+                                             const float* __restrict__ c,
+                                             const float* __restrict__ v, float* __restrict__ y,
+                                             int len, long long total, int p) {
+  for (long long i = (long long)blockIdx.x * kTB + threadIdx.x; i < total;
+       i += (long long)gridDim.x * kTB) {
+    const int t = (int)(i % len);
+    if (t < p) {
+      y[i] = qnan_f();
+      continue;
+    }
+    auto tp = [&](long long k) {
+      return ((double)h[k] + (double)l[k] + (double)c[k]) / 3.0;
+    };
+    double pos = 0.0, neg = 0.0, prev = tp(i - p);
+    for (long long k = i - p + 1; k <= i; ++k) {
+      const double cur = tp(k);
+      if (__builtin_isfinite(cur) && __builtin_isfinite(prev)) {
+        const double f = cur * (double)v[k];
+        if (cur > prev) pos += f;
+        else if (cur < prev) neg += f;
+      }
+      prev = cur;
+    }
+    const double tot = pos + neg;
+    y[i] = tot < 1.0 ? 0.0f : (float)(100.0 * (pos / tot));
+  }
+}
+
 // ------------------------------------------------------- linear regression --
 enum RegOut { kLinReg, kSlope, kIntercept, kAngle, kTsf };
 
@@ -374,6 +456,37 @@ ctStatus_t run_minmax(const char* fn, const float* x, float* mn, float* mx, int 
     return r != CT_STATUS_SUCCESS ? r : all_nan(mx, total, s);
   }
   hipLaunchKernelGGL(k_minmax, grid_for(total), dim3(kTB), 0, s, x, mn, mx, len, total, p);
+  TCA_HIP(hipGetLastError());
+  return CT_STATUS_SUCCESS;
+}
+
+ctStatus_t run_ultosc(const char* fn, const float* h, const float* l, const float* c, float* y,
+                      int len, int rows, int p1, int p2, int p3, hipStream_t s) {
+  ctStatus_t r = shape_ok(fn, len, rows);
+  if (r != CT_STATUS_SUCCESS) return r;
+  if (!h || !l || !c || !y) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: null pointer", fn);
+  if (p1 <= 0 || p2 <= 0 || p3 <= 0) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: invalid period", fn);
+  // sorted as TA-Lib sorts them (reference ULTOSC.cu:50-54)
+  if (p1 > p2) std::swap(p1, p2);
+  if (p2 > p3) std::swap(p2, p3);
+  if (p1 > p2) std::swap(p1, p2);
+  const long long total = (long long)len * rows;
+  if (p3 >= len) return all_nan(y, total, s);
+  hipLaunchKernelGGL(k_ultosc, grid_for(total), dim3(kTB), 0, s, h, l, c, y, len, total, p1, p2,
+                     p3);
+  TCA_HIP(hipGetLastError());
+  return CT_STATUS_SUCCESS;
+}
+
+ctStatus_t run_mfi(const char* fn, const float* h, const float* l, const float* c, const float* v,
+                   float* y, int len, int rows, int p, hipStream_t s) {
+  ctStatus_t r = shape_ok(fn, len, rows);
+  if (r != CT_STATUS_SUCCESS) return r;
+  if (!h || !l || !c || !v || !y) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: null pointer", fn);
+  if (p < 2) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: invalid period", fn);
+  const long long total = (long long)len * rows;
+  if (p > len) return all_nan(y, total, s);
+  hipLaunchKernelGGL(k_mfi, grid_for(total), dim3(kTB), 0, s, h, l, c, v, y, len, total, p);
   TCA_HIP(hipGetLastError());
   return CT_STATUS_SUCCESS;
 }
@@ -772,5 +885,74 @@ CTAPI_EXPORT ctStatus_t ct_minmax(const float* hx, float* hmn, float* hmx, int n
 CTAPI_EXPORT ctStatus_t ct_minmax_batch(const float* hx, float* hmn, float* hmx, int len, int rows,
                                         int p, int, hipStream_t s) {
   return guarded([&] { return host_minmax("ct_minmax_batch", hx, hmn, hmx, len, rows, p, s); });
+}
+}  // extern "C"
+
+// ------------------------------------------------------ ULTOSC and MFI --
+extern "C" {
+CTAPI_EXPORT ctStatus_t ct_ultosc_d(const float* h, const float* l, const float* c, float* y,
+                                    int n, int p1, int p2, int p3, hipStream_t s) {
+  return guarded([&] { return run_ultosc("ct_ultosc_d", h, l, c, y, n, 1, p1, p2, p3, s); });
+}
+CTAPI_EXPORT ctStatus_t ct_ultosc_batch_d(const float* h, const float* l, const float* c, float* y,
+                                          int len, int rows, int p1, int p2, int p3, int,
+                                          hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    T_RET(run_ultosc("ct_ultosc_batch_d", h, l, c, y, len, rows, p1, p2, p3, s));
+    return finish_batch(s);
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_ultosc(const float* hh, const float* hl, const float* hc, float* hy,
+                                  int n, int p1, int p2, int p3, hipStream_t s) {
+  return guarded([&] {
+    const float* in[3] = {hh, hl, hc};
+    return host_multi("ct_ultosc", n, 1, 3, in, hy, s, [&](const float* const* d, float* y) {
+      return run_ultosc("ct_ultosc", d[0], d[1], d[2], y, n, 1, p1, p2, p3, s);
+    });
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_ultosc_batch(const float* hh, const float* hl, const float* hc,
+                                        float* hy, int len, int rows, int p1, int p2, int p3, int,
+                                        hipStream_t s) {
+  return guarded([&] {
+    const float* in[3] = {hh, hl, hc};
+    return host_multi("ct_ultosc_batch", len, rows, 3, in, hy, s,
+                      [&](const float* const* d, float* y) {
+                        return run_ultosc("ct_ultosc_batch", d[0], d[1], d[2], y, len, rows, p1,
+                                          p2, p3, s);
+                      });
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_mfi_d(const float* h, const float* l, const float* c, const float* v,
+                                 float* y, int n, int p, hipStream_t s) {
+  return guarded([&] { return run_mfi("ct_mfi_d", h, l, c, v, y, n, 1, p, s); });
+}
+CTAPI_EXPORT ctStatus_t ct_mfi_batch_d(const float* h, const float* l, const float* c,
+                                       const float* v, float* y, int len, int rows, int p, int,
+                                       hipStream_t s) {
+  return guarded([&]() -> ctStatus_t {
+    T_RET(run_mfi("ct_mfi_batch_d", h, l, c, v, y, len, rows, p, s));
+    return finish_batch(s);
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_mfi(const float* hh, const float* hl, const float* hc, const float* hv,
+                               float* hy, int n, int p, hipStream_t s) {
+  return guarded([&] {
+    const float* in[4] = {hh, hl, hc, hv};
+    return host_multi("ct_mfi", n, 1, 4, in, hy, s, [&](const float* const* d, float* y) {
+      return run_mfi("ct_mfi", d[0], d[1], d[2], d[3], y, n, 1, p, s);
+    });
+  });
+}
+CTAPI_EXPORT ctStatus_t ct_mfi_batch(const float* hh, const float* hl, const float* hc,
+                                     const float* hv, float* hy, int len, int rows, int p, int,
+                                     hipStream_t s) {
+  return guarded([&] {
+    const float* in[4] = {hh, hl, hc, hv};
+    return host_multi("ct_mfi_batch", len, rows, 4, in, hy, s,
+                      [&](const float* const* d, float* y) {
+                        return run_mfi("ct_mfi_batch", d[0], d[1], d[2], d[3], y, len, rows, p, s);
+                      });
+  });
 }
 }  // extern "C"
