@@ -379,7 +379,8 @@ CTAPI_EXPORT ctStatus_t ct_midprice_batch_d(const float* d_highs, const float* d
 /* ---------------------------------------------------------------------------
  * TA-Lib 0.8.3 row functions (csrc/rowfn.hip): CUMSUM, PVT, RMA, ZLEMA, EFI,
  * VWMA, ER, VHF, DPO, CHOP, ADR, AO, QSTICK, DONCHIAN, MARKETFI, WAD, MCGD,
- * RVOL, CMF, EMV, FOSC, ERI, PERCENTRANK, SI and ASI_WILDER (TA_ASI).  Per row: the leading bars where any input is NaN are skipped,
+ * RVOL, CMF, EMV, FOSC, ERI, PERCENTRANK, SI, ASI_WILDER (TA_ASI), VORTEX
+ * and CMOU.  Per row: the leading bars where any input is NaN are skipped,
  * TA-Lib's routine runs in double over the rest, outputs are float with NaN
  * before the first one.  Periods outside TA-Lib's ranges are
  * CT_STATUS_INVALID_ARGUMENT; device calls on stream 0 synchronize.
@@ -459,6 +460,12 @@ CTAPI_EXPORT ctStatus_t ct_midprice_batch_d(const float* d_highs, const float* d
 /* :4195 */ CTAPI_EXPORT ctStatus_t ct_si_batch_d(const float* d_opens, const float* d_highs, const float* d_lows, const float* d_closes, float* d_outputs, int seriesLen, int numSeries, double limitMove, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
 /* :3285 */ CTAPI_EXPORT ctStatus_t ct_asi_wilder_batch(const float* host_opens, const float* host_highs, const float* host_lows, const float* host_closes, float* host_outputs, int seriesLen, int numSeries, double limitMove, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
 /* :3293 */ CTAPI_EXPORT ctStatus_t ct_asi_wilder_batch_d(const float* d_opens, const float* d_highs, const float* d_lows, const float* d_closes, float* d_outputs, int seriesLen, int numSeries, double limitMove, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :4477 */ CTAPI_EXPORT ctStatus_t ct_vortex(const float* host_high, const float* host_low, const float* host_close, float* host_plusVI, float* host_minusVI, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3434 */ CTAPI_EXPORT ctStatus_t ct_cmou(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :4482 */ CTAPI_EXPORT ctStatus_t ct_vortex_batch(const float* host_highs, const float* host_lows, const float* host_closes, float* host_plusVI, float* host_minusVI, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :4490 */ CTAPI_EXPORT ctStatus_t ct_vortex_batch_d(const float* d_highs, const float* d_lows, const float* d_closes, float* d_plusVI, float* d_minusVI, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3437 */ CTAPI_EXPORT ctStatus_t ct_cmou_batch(const float* host_inputs, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3442 */ CTAPI_EXPORT ctStatus_t ct_cmou_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
 
 /* ---------------------------------------------------------------------------
  * OBV and AD (csrc/families.hpp FCumT on the look-back scan engine): running

@@ -263,6 +263,8 @@ CT_CDL_DECL(bearish_engulfing)
 /* :1191 */ CTAPI_EXPORT ctStatus_t ct_percentrank_d(const float* d_input, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
 /* :1232 */ CTAPI_EXPORT ctStatus_t ct_si_d(const float* d_open, const float* d_high, const float* d_low, const float* d_close, float* d_output, int size, double limitMove, hipStream_t stream CTAPI_STREAM_DEFAULT);
 /* :1013 */ CTAPI_EXPORT ctStatus_t ct_asi_wilder_d(const float* d_open, const float* d_high, const float* d_low, const float* d_close, float* d_output, int size, double limitMove, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1303 */ CTAPI_EXPORT ctStatus_t ct_vortex_d(const float* d_high, const float* d_low, const float* d_close, float* d_plusVI, float* d_minusVI, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :1050 */ CTAPI_EXPORT ctStatus_t ct_cmou_d(const float* d_input, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
 /* :307 */ CTAPI_EXPORT ctStatus_t ct_obv_d(const float* d_close, const float* d_volume, float* d_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
 /* :298 */ CTAPI_EXPORT ctStatus_t ct_ad_d(const float* d_high, const float* d_low, const float* d_close, const float* d_volume, float* d_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
 /* :270 */ CTAPI_EXPORT ctStatus_t ct_ultosc_d(const float* d_high, const float* d_low, const float* d_close, float* d_output, int size, int period1, int period2, int period3, hipStream_t stream CTAPI_STREAM_DEFAULT);

@@ -286,7 +286,7 @@ class _Family:
             "adr": (2, 1), "ao": (2, 1), "qstick": (2, 1), "donchian": (2, 3),
             "marketfi": (3, 1), "wad": (3, 1), "mcgd": (1, 1), "rvol": (1, 1), "cmf": (4, 1),
             "emv": (3, 1), "fosc": (1, 1), "eri": (3, 2), "percentrank": (1, 1), "si": (4, 1),
-            "asi_wilder": (4, 1)}
+            "asi_wilder": (4, 1), "vortex": (3, 2), "cmou": (1, 1)}
 
     def rows(self, name, ins, p0=0, p1=0, d0=0.0):
         """One row through the backend's leading-NaN rule; a tuple for

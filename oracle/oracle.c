@@ -2024,6 +2024,93 @@ static void r_asi_wilder(const RowsIO* io, const int* par) {
   }
 }
 
+/* ta_VORTEX: running sums of the true range (against the previous close),
+ * |high - previous low| and |low - previous high| over the p bars ending at
+ * t; VI+ and VI- their ratios to the true-range sum (0 when it is not
+ * positive); a full window of zero true ranges zeroes the true-range sum. */
+static double vx_tr(const RowsIO* io, int i) {
+  const double h = io->x[0][i], l = io->x[1][i], pc = io->x[2][i - 1];
+  double tr = h - l, d = fabs(pc - h);
+  if (d > tr) tr = d;
+  d = fabs(pc - l);
+  if (d > tr) tr = d;
+  return tr;
+}
+static void r_vortex(const RowsIO* io, const int* par) {
+  const int p = par[0], n = io->n;
+  const double *h = io->x[0], *l = io->x[1];
+  if (n <= p) return;
+  double st = 0, sp = 0, sm = 0;
+  int run = 0;
+  for (int i = 1; i < p; ++i) {
+    const double tr = vx_tr(io, i);
+    st += tr;
+    sp += fabs(h[i] - l[i - 1]);
+    sm += fabs(l[i] - h[i - 1]);
+    run = (run + 1) & ((tr <= 0.0 ? 0 : 1) - 1);
+  }
+  for (int t = p, tr0 = 1; t < n; ++t, ++tr0) {
+    const double tr = vx_tr(io, t);
+    st += tr;
+    sp += fabs(h[t] - l[t - 1]);
+    sm += fabs(l[t] - h[t - 1]);
+    run = (run + 1) & ((tr <= 0.0 ? 0 : 1) - 1);
+    if (run >= p) {
+      run = p;
+      st = 0.0;
+    }
+    const double ct = st, cp = sp, cm = sm;
+    st -= vx_tr(io, tr0);
+    sp -= fabs(h[tr0] - l[tr0 - 1]);
+    sm -= fabs(l[tr0] - h[tr0 - 1]);
+    io->y[0][t] = ct > 0.0 ? cp / ct : 0.0;
+    io->y[1][t] = ct > 0.0 ? cm / ct : 0.0;
+  }
+}
+/* ta_CMOU: Chande's momentum oscillator with unsmoothed running sums of the
+ * up and down moves over the p changes ending at t: 100 (up - down) /
+ * (up + down), 0 when the sum is not positive; a full window of unchanged
+ * bars restarts the sums and gives 0. */
+static void r_cmou(const RowsIO* io, const int* par) {
+  const int p = par[0], n = io->n;
+  const double* x = io->x[0];
+  if (n <= p) return;
+  double us = 0.0, ds = 0.0;
+  int run = 0;
+  for (int t = 1; t <= p; ++t) {
+    const double d = x[t] - x[t - 1], up = (d + fabs(d)) * 0.5;
+    us += up;
+    ds += up - d;
+    run = (run + 1) & ((fabs(d) <= 0.0 ? 0 : 1) - 1);
+  }
+  double sum = us + ds;
+  io->y[0][p] = sum > 0.0 ? 100.0 * (us - ds) / sum : 0.0;
+  {
+    const double d = x[1] - x[0], up = (d + fabs(d)) * 0.5;
+    us -= up;
+    ds -= up - d;
+  }
+  for (int t = p + 1; t < n; ++t) {
+    const double d = x[t] - x[t - 1], up = (d + fabs(d)) * 0.5;
+    us += up;
+    ds += up - d;
+    run = (run + 1) & ((fabs(d) <= 0.0 ? 0 : 1) - 1);
+    if (run >= p) {
+      run = p;
+      us = 0.0;
+      ds = 0.0;
+      io->y[0][t] = 0.0;
+    } else {
+      sum = us + ds;
+      io->y[0][t] = sum > 0.0 ? 100.0 * (us - ds) / sum : 0.0;
+    }
+    const int o = t - p + 1;
+    const double e = x[o] - x[o - 1], ue = (e + fabs(e)) * 0.5;
+    us -= ue;
+    ds -= ue - e;
+  }
+}
+
 typedef struct {
   const char* name;
   RowRoutine fn;
@@ -2058,6 +2145,8 @@ static const RowSpec kRows[] = {
     {"percentrank", r_percentrank, 1, 1, 1, {2, 0}, {10000, 0}, 1, 0},
     {"si", r_si, 4, 1, 0, {0, 0}, {0, 0}, 1e-8, 3e37},
     {"asi_wilder", r_asi_wilder, 4, 1, 0, {0, 0}, {0, 0}, 1e-8, 3e37},
+    {"vortex", r_vortex, 3, 2, 1, {1, 0}, {100000, 0}, 1, 0},
+    {"cmou", r_cmou, 1, 1, 1, {2, 0}, {100000, 0}, 1, 0},
 };
 
 /* orc_rows(name, inputs, outputs, n, p0, p1, d0): one row, the backend's

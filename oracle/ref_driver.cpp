@@ -249,7 +249,7 @@ API int ref_rows(const char* name, const float* const* in, float* const* out, in
                                {"wad", 3, 1},    {"mcgd", 1, 1},     {"rvol", 1, 1},
                                {"cmf", 4, 1},    {"emv", 3, 1},      {"fosc", 1, 1},
                                {"eri", 3, 2},    {"percentrank", 1, 1}, {"si", 4, 1},
-                               {"asi_wilder", 4, 1}};
+                               {"asi_wilder", 4, 1}, {"vortex", 3, 2}, {"cmou", 1, 1}};
   const Spec* s = nullptr;
   for (const Spec& q : specs)
     if (std::string(q.name) == name) s = &q;
@@ -308,6 +308,10 @@ API int ref_rows(const char* name, const float* const* in, float* const* out, in
   else if (nm == "si")
     rc = TA_SI(a, 0, e, x[0].data(), x[1].data(), x[2].data(), x[3].data(), d0, &beg, &nb,
                y[0].data());
+  else if (nm == "vortex")
+    rc = TA_VORTEX(a, 0, e, x[0].data(), x[1].data(), x[2].data(), p0, &beg, &nb, y[0].data(),
+                   y[1].data());
+  else if (nm == "cmou") rc = TA_CMOU(a, 0, e, x[0].data(), p0, &beg, &nb, y[0].data());
   else if (nm == "asi_wilder")
     rc = TA_ASI(a, 0, e, x[0].data(), x[1].data(), x[2].data(), x[3].data(), d0, &beg, &nb,
                 y[0].data());

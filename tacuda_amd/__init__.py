@@ -315,7 +315,8 @@ ROW_FUNCTIONS = {"cumsum": (1, 1, 0), "pvt": (2, 1, 0), "rma": (1, 1, 1), "zlema
                  "qstick": (2, 1, 1), "donchian": (2, 3, 1), "marketfi": (3, 1, 0),
                  "wad": (3, 1, 0), "mcgd": (1, 1, 1), "rvol": (1, 1, 1), "cmf": (4, 1, 1),
                  "emv": (3, 1, "1d"), "fosc": (1, 1, 1), "eri": (3, 2, 1),
-                 "percentrank": (1, 1, 1), "si": (4, 1, "d"), "asi_wilder": (4, 1, "d")}
+                 "percentrank": (1, 1, 1), "si": (4, 1, "d"), "asi_wilder": (4, 1, "d"),
+                 "vortex": (3, 2, 1), "cmou": (1, 1, 1)}
 _PARAMS = {0: [], 1: [c_int], 2: [c_int, c_int], "d": [ctypes.c_double],
            "1d": [c_int, ctypes.c_double]}
 for _n, (_i, _o, _p) in ROW_FUNCTIONS.items():

@@ -2,8 +2,8 @@
 // (src/compat/Talib08.cu:47-103, one work-item per row running the TA_*
 // routine in double): CUMSUM, PVT, RMA, ZLEMA, EFI, VWMA, ER, VHF, DPO,
 // CHOP, ADR, AO, QSTICK, DONCHIAN, MARKETFI, WAD, MCGD, RVOL, CMF, EMV,
-// FOSC, ERI, PERCENTRANK, SI and ASI_WILDER (TA_ASI), each in host, device,
-// batch and host-batch forms.  Reference entry points:
+// FOSC, ERI, PERCENTRANK, SI, ASI_WILDER (TA_ASI), VORTEX and CMOU, each in
+// host, device, batch and host-batch forms.  Reference entry points:
 // include/tacuda.h (ct_<name> and ct_<name>_batch[_d], cited per
 // declaration), include/tacuda_device.h (ct_<name>_d).
 //
@@ -37,7 +37,7 @@ namespace {
 enum RowFn {
   kCumsum, kPvt, kRma, kZlema, kEfi, kVwma, kEr, kVhf, kDpo, kChop, kAdr, kAo, kQstick,
   kDonchian, kMarketfi, kWad, kMcgd, kRvol, kCmf, kEmv, kFosc, kEri, kPercentrank, kSi,
-  kAsiWilder
+  kAsiWilder, kVortex, kCmou
 };
 
 struct RowSpec {
@@ -75,6 +75,8 @@ constexpr RowSpec kSpec[] = {
     {1, 1, 1, {2, 0}, {10000, 0}, 1, 0},        // PERCENTRANK
     {4, 1, 0, {0, 0}, {0, 0}, 1e-8, kRealMax},  // SI (open, high, low, close; limit move)
     {4, 1, 0, {0, 0}, {0, 0}, 1e-8, kRealMax},  // ASI_WILDER (the same)
+    {3, 2, 1, {1, 0}, {100000, 0}, 1, 0},       // VORTEX (high, low, close -> VI+, VI-)
+    {1, 1, 1, {2, 0}, {100000, 0}, 1, 0},       // CMOU
 };
 
 struct RowArgs {
@@ -621,6 +623,92 @@ __device__ void f_asi(const Row& r, double lim) {
   }
 }
 
+// ta_VORTEX: running sums over the p bars ending at t of the true range
+// against the previous close, |high - previous low| and |low - previous
+// high|; VI+ and VI- are the last two over the first (0 when it is not
+// positive).  A full window of zero true ranges zeroes the true-range sum
+// only.
+__device__ __forceinline__ double vortex_tr(const Row& r, int i) {
+  const double h = r.in(0, i), l = r.in(1, i), pc = r.in(2, i - 1);
+  double tr = h - l;
+  const double a = fabs(pc - h);
+  if (a > tr) tr = a;
+  const double b = fabs(pc - l);
+  if (b > tr) tr = b;
+  return tr;
+}
+__device__ void f_vortex(const Row& r, int p) {
+  if (r.n <= p) return;
+  double st = 0.0, sp = 0.0, sm = 0.0;
+  int run = 0;
+  for (int i = 1; i < p; ++i) {
+    const double tr = vortex_tr(r, i);
+    st += tr;
+    sp += fabs(r.in(0, i) - r.in(1, i - 1));
+    sm += fabs(r.in(1, i) - r.in(0, i - 1));
+    run = tr <= 0.0 ? run + 1 : 0;
+  }
+  for (int t = p; t < r.n; ++t) {
+    const double tr = vortex_tr(r, t);
+    st += tr;
+    sp += fabs(r.in(0, t) - r.in(1, t - 1));
+    sm += fabs(r.in(1, t) - r.in(0, t - 1));
+    run = tr <= 0.0 ? run + 1 : 0;
+    if (run >= p) {
+      run = p;
+      st = 0.0;
+    }
+    const double ct = st, cp = sp, cm = sm;
+    const int o = t - p + 1;
+    st -= vortex_tr(r, o);
+    sp -= fabs(r.in(0, o) - r.in(1, o - 1));
+    sm -= fabs(r.in(1, o) - r.in(0, o - 1));
+    r.put(0, t, ct > 0.0 ? cp / ct : 0.0);
+    r.put(1, t, ct > 0.0 ? cm / ct : 0.0);
+  }
+}
+
+// ta_CMOU: running sums of the up and down moves of the p changes ending at
+// t (up = (d + |d|) / 2, down = up - d); 100 (up - down) / (up + down), 0
+// when that sum is not positive; a full window of unchanged bars restarts
+// the sums and gives 0
+__device__ void f_cmou(const Row& r, int p) {
+  if (r.n <= p) return;
+  double us = 0.0, ds = 0.0;
+  int run = 0;
+  auto add = [&](int t) {
+    const double d = r.in(0, t) - r.in(0, t - 1), up = (d + fabs(d)) * 0.5;
+    us += up;
+    ds += up - d;
+    return d;
+  };
+  auto drop = [&](int t) {
+    const double d = r.in(0, t) - r.in(0, t - 1), up = (d + fabs(d)) * 0.5;
+    us -= up;
+    ds -= up - d;
+  };
+  for (int t = 1; t <= p; ++t) run = fabs(add(t)) <= 0.0 ? run + 1 : 0;
+  {
+    const double sum = us + ds;
+    r.put(0, p, sum > 0.0 ? 100.0 * (us - ds) / sum : 0.0);
+  }
+  drop(1);
+  for (int t = p + 1; t < r.n; ++t) {
+    run = fabs(add(t)) <= 0.0 ? run + 1 : 0;
+    double o = 0.0;
+    if (run >= p) {
+      run = p;
+      us = 0.0;
+      ds = 0.0;
+    } else {
+      const double sum = us + ds;
+      o = sum > 0.0 ? 100.0 * (us - ds) / sum : 0.0;
+    }
+    r.put(0, t, o);
+    drop(t - p + 1);
+  }
+}
+
 template <int F>
 __device__ void run_row(const Row& r, int p0, int p1, double d0) {
   if constexpr (F == kCumsum) f_cumsum(r);
@@ -647,7 +735,9 @@ __device__ void run_row(const Row& r, int p0, int p1, double d0) {
   else if constexpr (F == kEri) f_eri(r, p0);
   else if constexpr (F == kPercentrank) f_percentrank(r, p0);
   else if constexpr (F == kSi) f_si(r, d0);
-  else f_asi(r, d0);
+  else if constexpr (F == kAsiWilder) f_asi(r, d0);
+  else if constexpr (F == kVortex) f_vortex(r, p0);
+  else f_cmou(r, p0);
 }
 
 // SPREAD: one row per wave (lane 0 works, the block is one wave); else one
@@ -831,3 +921,5 @@ TCA_ROWFN(eri, kEri, 3, 2, 1)
 TCA_ROWFN(percentrank, kPercentrank, 1, 1, 1)
 TCA_ROWFN(si, kSi, 4, 1, D)
 TCA_ROWFN(asi_wilder, kAsiWilder, 4, 1, D)
+TCA_ROWFN(vortex, kVortex, 3, 2, 1)
+TCA_ROWFN(cmou, kCmou, 1, 1, 1)

@@ -1,6 +1,6 @@
 """TA-Lib 0.8.3 row functions (CUMSUM, PVT, RMA, ZLEMA, EFI, VWMA, ER, VHF,
 DPO, CHOP, ADR, AO, QSTICK, DONCHIAN, MARKETFI, WAD, MCGD, RVOL, CMF, EMV,
-FOSC, ERI, PERCENTRANK, SI, ASI_WILDER) on the GPU
+FOSC, ERI, PERCENTRANK, SI, ASI_WILDER, VORTEX, CMOU) on the GPU
 against the oracle, bit for bit: device single calls on long rows (one wave
 per row), dense batches of many short rows (one lane per row) with leading
 and interior NaNs, zero-volume and unchanged-price runs, host single and
@@ -20,7 +20,7 @@ pytestmark = pytest.mark.gpu
 
 torch = pytest.importorskip("torch")
 
-NOUT = {"donchian": 3, "eri": 2}
+NOUT = {"donchian": 3, "eri": 2, "vortex": 2}
 
 
 def _check(name, got, want, what):

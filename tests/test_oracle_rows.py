@@ -16,7 +16,7 @@ SPEC = {"cumsum": "x", "pvt": "cv", "rma": "x", "zlema": "x", "efi": "cv", "vwma
         "er": "x", "vhf": "x", "dpo": "x", "chop": "hlc", "adr": "hl", "ao": "hl",
         "qstick": "oc", "donchian": "hl", "marketfi": "hlv", "wad": "hlc", "mcgd": "x",
         "rvol": "v", "cmf": "hlcv", "emv": "hlv", "fosc": "x", "eri": "hlc", "percentrank": "x",
-        "si": "ohlc", "asi_wilder": "ohlc"}
+        "si": "ohlc", "asi_wilder": "ohlc", "vortex": "hlc", "cmou": "x"}
 # integer periods: 0, 1 or 2; "d": one real parameter; "1d": a period and a real
 NPAR = {"cumsum": 0, "pvt": 0, "marketfi": 0, "wad": 0, "ao": 2, "emv": "1d", "si": "d",
         "asi_wilder": "d"}
@@ -24,7 +24,7 @@ PERIODS = {1: [(1, 0), (2, 0), (14, 0), (57, 0)], 2: [(2, 3), (5, 34), (34, 5), 
            0: [(0, 0)], "d": [(0, 0, 3.0), (0, 0, 0.5), (0, 0, 1e4)],
            "1d": [(1, 0, 1.0), (2, 0, 1e4), (14, 0, 1e8), (57, 0, 10.0)]}
 MIN_P = {"er": 2, "vhf": 2, "dpo": 2, "chop": 2, "donchian": 2, "mcgd": 2, "cmf": 2, "fosc": 2,
-         "percentrank": 2}
+         "percentrank": 2, "cmou": 2}
 
 
 def bars(n, seed, holes=False):
