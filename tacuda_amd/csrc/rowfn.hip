@@ -86,14 +86,38 @@ struct RowArgs {
   double d0;
 };
 
+typedef __attribute__((address_space(3))) float lds_float;
+
 // One row's view: inputs and outputs offset to the first bar after the
-// leading-NaN prefix; n bars remain.
+// leading-NaN prefix; n bars remain.  Outputs collect in an LDS run of up to
+// cap values per output and go to memory when the run is full, breaks or
+// ends: on gfx950 a load waits for every earlier store of its wave (one
+// in-order counter), so storing each bar as it is made would put a store's
+// full round trip between consecutive loads of a sequential routine.  The
+// routines write each output's bars in increasing order.
 struct Row {
   const float* x[4];
   float* y[3];
   int n;
+  lds_float* buf;  // [3][cap]
+  int cap;
+  mutable int t0[3];
+  mutable int cnt[3];
   __device__ double in(int j, int t) const { return (double)x[j][t]; }
-  __device__ void put(int k, int t, double v) const { y[k][t] = (float)v; }
+  __device__ void flush(int k) const {
+    for (int i = 0; i < cnt[k]; ++i) y[k][t0[k] + i] = buf[k * cap + i];
+    cnt[k] = 0;
+  }
+  __device__ void put(int k, int t, double v) const {
+    if (cnt[k] > 0 && t != t0[k] + cnt[k]) flush(k);
+    if (cnt[k] == 0) t0[k] = t;
+    buf[k * cap + cnt[k]] = (float)v;
+    if (++cnt[k] == cap) flush(k);
+  }
+  __device__ void flush_all() const {
+    for (int k = 0; k < 3; ++k)
+      if (cnt[k] > 0) flush(k);
+  }
 };
 
 __device__ __forceinline__ bool finite_d(double v) {
@@ -269,18 +293,16 @@ __device__ void f_er(const Row& r, int p) {
 
 // ta_VHF: (max - min) / sum |changes| over the p + 1 bars ending at t, every
 // window evaluated afresh oldest first (the extremes seeded with x[t])
-__device__ void f_vhf(const Row& r, int p) {
-  for (int t = p; t < r.n; ++t) {
-    double hi = r.in(0, t), lo = hi, prev = r.in(0, t - p), chg = 0.0;
-    for (int s = t - p + 1; s <= t; ++s) {
-      const double v = r.in(0, s);
-      chg += fabs(v - prev);
-      if (v > hi) hi = v;
-      if (v < lo) lo = v;
-      prev = v;
-    }
-    r.put(0, t, chg > 0.0 ? (hi - lo) / chg : 0.0);
+__device__ __forceinline__ double pt_vhf(const Row& r, int t, int p) {
+  double hi = r.in(0, t), lo = hi, prev = r.in(0, t - p), chg = 0.0;
+  for (int s = t - p + 1; s <= t; ++s) {
+    const double v = r.in(0, s);
+    chg += fabs(v - prev);
+    if (v > hi) hi = v;
+    if (v < lo) lo = v;
+    prev = v;
   }
+  return chg > 0.0 ? (hi - lo) / chg : 0.0;
 }
 
 // ta_DPO: x[t - (p/2 + 1)] minus the running p-bar mean ending at t
@@ -300,21 +322,18 @@ __device__ void f_dpo(const Row& r, int p) {
 // ta_CHOP: 100 log10(sum TR / (max high - min low)) / log10(p) over the p
 // bars ending at t (true ranges against the close before each bar), 100
 // when the range or the sum is not positive
-__device__ void f_chop(const Row& r, int p) {
+__device__ __forceinline__ double pt_chop(const Row& r, int t, int p) {
   const double lp = log10((double)p);
-  for (int t = p; t < r.n; ++t) {
-    double hi = r.in(0, t), lo = r.in(1, t), str = 0.0, pc = r.in(2, t - p);
-    for (int s = t - p + 1; s <= t; ++s) {
-      const double h = r.in(0, s), l = r.in(1, s);
-      const double th = pc > h ? pc : h, tl = pc < l ? pc : l;
-      str += th - tl;
-      if (h > hi) hi = h;
-      if (l < lo) lo = l;
-      pc = r.in(2, s);
-    }
-    r.put(0, t,
-          (hi - lo <= 0.0 || str <= 0.0) ? 100.0 : 100.0 * (log10(str / (hi - lo)) / lp));
+  double hi = r.in(0, t), lo = r.in(1, t), str = 0.0, pc = r.in(2, t - p);
+  for (int s = t - p + 1; s <= t; ++s) {
+    const double h = r.in(0, s), l = r.in(1, s);
+    const double th = pc > h ? pc : h, tl = pc < l ? pc : l;
+    str += th - tl;
+    if (h > hi) hi = h;
+    if (l < lo) lo = l;
+    pc = r.in(2, s);
   }
+  return (hi - lo <= 0.0 || str <= 0.0) ? 100.0 : 100.0 * (log10(str / (hi - lo)) / lp);
 }
 
 // ta_ADR (high - low) and ta_QSTICK (close - open): the running p-bar mean
@@ -397,11 +416,9 @@ __device__ void f_donchian(const Row& r, int p) {
   }
 }
 
-__device__ void f_marketfi(const Row& r) {
-  for (int t = 0; t < r.n; ++t) {
-    const double v = r.in(2, t);
-    r.put(0, t, v != 0.0 ? (r.in(0, t) - r.in(1, t)) / v : 0.0);
-  }
+__device__ __forceinline__ double pt_marketfi(const Row& r, int t) {
+  const double v = r.in(2, t);
+  return v != 0.0 ? (r.in(0, t) - r.in(1, t)) / v : 0.0;
 }
 
 // ta_WAD: accumulation against the true low on up bars, the true high on
@@ -586,13 +603,11 @@ __device__ void f_eri(const Row& r, int p) {
 }
 
 // ta_PERCENTRANK: 100 x the share of the p bars before t below x[t]
-__device__ void f_percentrank(const Row& r, int p) {
-  for (int t = p; t < r.n; ++t) {
-    const double c = r.in(0, t);
-    int below = 0;
-    for (int s = t - p; s < t; ++s) below += r.in(0, s) < c;
-    r.put(0, t, (double)below / (double)p * 100.0);
-  }
+__device__ __forceinline__ double pt_percentrank(const Row& r, int t, int p) {
+  const double c = r.in(0, t);
+  int below = 0;
+  for (int s = t - p; s < t; ++s) below += r.in(0, s) < c;
+  return (double)below / (double)p * 100.0;
 }
 
 // Wilder's swing index of bar t (inputs open, high, low, close) against bar
@@ -610,9 +625,6 @@ __device__ __forceinline__ double swing_index(const Row& r, int t, double lim) {
   den = den > rg ? den : rg;
   den += 0.25 * fabs(pb);
   return den == 0.0 ? 0.0 : 50.0 * (num / den) * (kk / lim);
-}
-__device__ void f_si(const Row& r, double lim) {
-  for (int t = 1; t < r.n; ++t) r.put(0, t, swing_index(r, t, lim));
 }
 __device__ void f_asi(const Row& r, double lim) {
   double acc = 0.0;
@@ -718,14 +730,11 @@ __device__ void run_row(const Row& r, int p0, int p1, double d0) {
   else if constexpr (F == kEfi) f_efi(r, p0);
   else if constexpr (F == kVwma) f_vwma(r, p0);
   else if constexpr (F == kEr) f_er(r, p0);
-  else if constexpr (F == kVhf) f_vhf(r, p0);
   else if constexpr (F == kDpo) f_dpo(r, p0);
-  else if constexpr (F == kChop) f_chop(r, p0);
   else if constexpr (F == kAdr) f_mean_diff(r, 0, 1, p0);
   else if constexpr (F == kAo) f_ao(r, p0, p1);
   else if constexpr (F == kQstick) f_mean_diff(r, 1, 0, p0);
   else if constexpr (F == kDonchian) f_donchian(r, p0);
-  else if constexpr (F == kMarketfi) f_marketfi(r);
   else if constexpr (F == kWad) f_wad(r);
   else if constexpr (F == kMcgd) f_mcgd(r, p0);
   else if constexpr (F == kRvol) f_rvol(r, p0);
@@ -733,36 +742,175 @@ __device__ void run_row(const Row& r, int p0, int p1, double d0) {
   else if constexpr (F == kEmv) f_emv(r, p0, d0);
   else if constexpr (F == kFosc) f_fosc(r, p0);
   else if constexpr (F == kEri) f_eri(r, p0);
-  else if constexpr (F == kPercentrank) f_percentrank(r, p0);
-  else if constexpr (F == kSi) f_si(r, d0);
   else if constexpr (F == kAsiWilder) f_asi(r, d0);
   else if constexpr (F == kVortex) f_vortex(r, p0);
   else f_cmou(r, p0);
 }
 
-// SPREAD: one row per wave (lane 0 works, the block is one wave); else one
-// row per lane.  Each row: NaN fill, the leading-NaN prefix, the routine.
-template <int F, bool SPREAD>
-__global__ __launch_bounds__(64) void k_rows(RowArgs a) {
-  constexpr RowSpec S = kSpec[F];
-  const long long row = SPREAD ? (long long)blockIdx.x : (long long)blockIdx.x * 64 + threadIdx.x;
-  if (SPREAD && threadIdx.x != 0) return;
-  if (row >= a.rows) return;
-  const size_t base = (size_t)row * (size_t)a.len;
-  const float nanf_ = __int_as_float(0x7fc00000);
-  for (int k = 0; k < S.nout; ++k)
-    for (int t = 0; t < a.len; ++t) a.out[k][base + t] = nanf_;
-  int f = 0;
-  for (; f < a.len - 1; ++f) {
-    bool any = false;
-    for (int j = 0; j < S.nin; ++j) any |= __builtin_isnan(a.in[j][base + f]);
-    if (!any) break;
+// The functions whose every output is a function of a window of inputs
+// alone (no state carried between bars): one work-item per output.
+// DONCHIAN joins them for NaN-free rows: its kept extremes are then the
+// window's maximum and minimum, which the first strict improvement from the
+// oldest bar finds as TA-Lib's re-search does; a row with a NaN after its
+// prefix runs the sequential routine afterwards (k_rows_fix), whose
+// NaN-dependent re-search points the direct form cannot know.
+constexpr bool pointwise(int F) {
+  return F == kVhf || F == kChop || F == kMarketfi || F == kPercentrank || F == kSi ||
+         F == kDonchian;
+}
+
+__device__ __forceinline__ void pt_donchian(const Row& r, int t, int p, double& hi, double& lo) {
+  hi = r.in(0, t - p + 1);
+  lo = r.in(1, t - p + 1);
+  for (int s = t - p + 2; s <= t; ++s) {
+    const double h = r.in(0, s), l = r.in(1, s);
+    if (h > hi) hi = h;
+    if (l < lo) lo = l;
   }
-  Row r;
+}
+
+// Output at bar t of the row view r (NaN before the first output).
+template <int F>
+__device__ __forceinline__ double pt_eval(const Row& r, int t, int p0, double d0) {
+  const double nan_ = __longlong_as_double(0x7ff8000000000000ll);
+  if constexpr (F == kVhf) return t >= p0 ? pt_vhf(r, t, p0) : nan_;
+  else if constexpr (F == kChop) return t >= p0 ? pt_chop(r, t, p0) : nan_;
+  else if constexpr (F == kMarketfi) return pt_marketfi(r, t);
+  else if constexpr (F == kPercentrank) return t >= p0 ? pt_percentrank(r, t, p0) : nan_;
+  else return t >= 1 ? swing_index(r, t, d0) : nan_;
+}
+
+// Leading bars of a row where any input is NaN (at most len - 1), found by
+// the whole block a chunk at a time.
+template <int NIN>
+__device__ int block_prefix(const float* const* in, size_t base, int len, int* s_f) {
+  if (threadIdx.x == 0) *s_f = len - 1;
+  __syncthreads();
+  for (int c = 0; c < len - 1; c += (int)blockDim.x) {
+    const int i = c + (int)threadIdx.x;
+    if (i < len - 1) {
+      bool any = false;
+      for (int j = 0; j < NIN; ++j) any |= __builtin_isnan(in[j][base + i]);
+      if (!any) atomicMin(s_f, i);
+    }
+    __syncthreads();
+    if (*s_f < c + (int)blockDim.x) break;
+  }
+  const int f = *s_f;
+  __syncthreads();
+  return f;
+}
+
+// Pointwise functions: a block of 256 outputs of one row.
+template <int F>
+__global__ __launch_bounds__(256) void k_rows_pt(RowArgs a, int tilesPerRow) {
+  constexpr RowSpec S = kSpec[F];
+  __shared__ int s_f;
+  const long long row = blockIdx.x / tilesPerRow;
+  const int t = (int)(blockIdx.x % tilesPerRow) * 256 + (int)threadIdx.x;
+  const size_t base = (size_t)row * (size_t)a.len;
+  const int f = block_prefix<S.nin>(a.in, base, a.len, &s_f);
+  if (t >= a.len) return;
+  Row r{};
+  for (int j = 0; j < 4; ++j) r.x[j] = j < S.nin ? a.in[j] + base + f : nullptr;
+  r.n = a.len - f;
+  const float nanf_ = __int_as_float(0x7fc00000);
+  if constexpr (F == kDonchian) {
+    float u = nanf_, m = nanf_, l = nanf_;
+    if (t - f >= a.p0 - 1) {
+      double hi, lo;
+      pt_donchian(r, t - f, a.p0, hi, lo);
+      u = (float)hi;
+      m = (float)((hi + lo) / 2.0);
+      l = (float)lo;
+    }
+    a.out[0][base + t] = u;
+    a.out[1][base + t] = m;
+    a.out[2][base + t] = l;
+  } else {
+    a.out[0][base + t] = t < f ? nanf_ : (float)pt_eval<F>(r, t - f, a.p0, a.d0);
+  }
+}
+
+// Rows with a NaN after the prefix: the sequential routine over the row, one
+// block of four waves per row (the block finds the prefix and looks for a
+// NaN, four loads in flight per lane; lane 0 runs the routine, overwriting
+// every output from the first one on).
+template <int F>
+__global__ __launch_bounds__(256) void k_rows_fix(RowArgs a) {
+  constexpr RowSpec S = kSpec[F];
+  constexpr int CAP = 256;
+  __shared__ float s_buf[3 * CAP];
+  __shared__ int s_f;
+  __shared__ int s_nan;
+  const long long row = blockIdx.x;
+  const size_t base = (size_t)row * (size_t)a.len;
+  const int f = block_prefix<S.nin>(a.in, base, a.len, &s_f);
+  if (threadIdx.x == 0) s_nan = 0;
+  __syncthreads();
+  bool any = false;
+  for (int t = f + (int)threadIdx.x; t < a.len; t += 4 * 256) {
+    float v[S.nin][4];
+#pragma unroll
+    for (int j = 0; j < S.nin; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int u = t + q * 256;
+        v[j][q] = u < a.len ? a.in[j][base + u] : 0.0f;
+      }
+#pragma unroll
+    for (int j = 0; j < S.nin; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) any |= __builtin_isnan(v[j][q]);
+  }
+  if (any) s_nan = 1;
+  __syncthreads();
+  if (!s_nan || threadIdx.x != 0) return;
+  Row r{};
   for (int j = 0; j < 4; ++j) r.x[j] = j < S.nin ? a.in[j] + base + f : nullptr;
   for (int k = 0; k < 3; ++k) r.y[k] = k < S.nout ? a.out[k] + base + f : nullptr;
   r.n = a.len - f;
+  r.buf = (lds_float*)s_buf;
+  r.cap = CAP;
   run_row<F>(r, a.p0, a.p1, a.d0);
+  r.flush_all();
+}
+
+// SPREAD: one row per wave (the wave fills the row with NaN and finds its
+// prefix, then lane 0 runs the routine); else one row per lane.
+template <int F, bool SPREAD>
+__global__ __launch_bounds__(64) void k_rows(RowArgs a) {
+  constexpr RowSpec S = kSpec[F];
+  constexpr int CAP = SPREAD ? 256 : 16;
+  __shared__ float s_buf[(SPREAD ? 1 : 64) * 3 * CAP];
+  __shared__ int s_f;
+  const float nanf_ = __int_as_float(0x7fc00000);
+  const long long row = SPREAD ? (long long)blockIdx.x : (long long)blockIdx.x * 64 + threadIdx.x;
+  const size_t base = (size_t)row * (size_t)a.len;
+  int f = 0;
+  if constexpr (SPREAD) {
+    for (int k = 0; k < S.nout; ++k)
+      for (int t = (int)threadIdx.x; t < a.len; t += 64) a.out[k][base + t] = nanf_;
+    f = block_prefix<S.nin>(a.in, base, a.len, &s_f);  // also orders the fill
+    if (threadIdx.x != 0) return;
+  } else {
+    if (row >= a.rows) return;
+    for (int k = 0; k < S.nout; ++k)
+      for (int t = 0; t < a.len; ++t) a.out[k][base + t] = nanf_;
+    for (; f < a.len - 1; ++f) {
+      bool any = false;
+      for (int j = 0; j < S.nin; ++j) any |= __builtin_isnan(a.in[j][base + f]);
+      if (!any) break;
+    }
+  }
+  Row r{};
+  for (int j = 0; j < 4; ++j) r.x[j] = j < S.nin ? a.in[j] + base + f : nullptr;
+  for (int k = 0; k < 3; ++k) r.y[k] = k < S.nout ? a.out[k] + base + f : nullptr;
+  r.n = a.len - f;
+  r.buf = (lds_float*)(s_buf + (SPREAD ? 0 : threadIdx.x * 3 * CAP));
+  r.cap = CAP;
+  run_row<F>(r, a.p0, a.p1, a.d0);
+  r.flush_all();
 }
 
 template <int F>
@@ -792,6 +940,18 @@ ctStatus_t run_rows(const char* fn, const float* const* in, float* const* out, i
   a.rows = rows;
   a.p0 = p0;
   a.p1 = p1;
+  if constexpr (pointwise(F)) {
+    const int tiles = (len + 255) / 256;
+    if ((long long)tiles * rows >= (1ll << 31))
+      return fail(CT_STATUS_UNSUPPORTED, "%s: batch too large", fn);
+    hipLaunchKernelGGL(k_rows_pt<F>, dim3((unsigned)(tiles * rows)), dim3(256), 0, s, a, tiles);
+    TCA_HIP(hipGetLastError());
+    if constexpr (F == kDonchian) {
+      hipLaunchKernelGGL(k_rows_fix<F>, dim3((unsigned)rows), dim3(256), 0, s, a);
+      TCA_HIP(hipGetLastError());
+    }
+    return CT_STATUS_SUCCESS;
+  }
   // few long rows: a wave each, so the rows' loads do not share one CU
   if (rows <= 2048 && len >= 4096)
     hipLaunchKernelGGL((k_rows<F, true>), dim3(rows), dim3(64), 0, s, a);
