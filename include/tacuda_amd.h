@@ -223,7 +223,7 @@ CTAPI_EXPORT ctStatus_t ctx_plan_destroy(ctxPlan plan);
 /* ---------------------------------------------------------------------------
  * Differentiable indicators: forward and adjoint kernels used by the torch
  * autograd binding.  Dense series-major rows (numSeries x seriesLen, at most
- * 16384 bars per row), device pointers, enqueued on `stream`.
+ * 32768 bars per row), device pointers, enqueued on `stream`.
  *
  * EMA_alpha / RSI_alpha take the smoothing weight alpha as a continuous
  * parameter and keep the TA-Lib seed of `period` samples (EMA: mean of bars

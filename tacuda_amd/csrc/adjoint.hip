@@ -38,7 +38,7 @@ namespace {
 
 constexpr int kAW = 8;            // waves per row
 constexpr int kAT = kAW * kWave;  // threads per row (one workgroup)
-constexpr int kMaxChunk = 32;     // C <= 32: rows up to 16384 bars
+constexpr int kMaxChunk = 64;     // C <= 64: rows up to 32768 bars (LDS columns <= 130 KB)
 
 struct Chunk {
   int tid, lane, wave, C, a, e;  // chunk [a, e) of a row of n bars

@@ -38,11 +38,12 @@ def test_gradients_against_fp64_reference(dev, name):
 
 
 # Rows of 40 and 500 bars leave most of a row's 512 threads without bars; 777
-# and 4099 bars end inside a thread's chunk and inside a wave; 16384 is the
-# longest row (32 bars a thread).
+# and 4099 bars end inside a thread's chunk and inside a wave; 32768 is the
+# longest row (64 bars a thread, the largest LDS columns).
 @pytest.mark.parametrize("name,n", [(name, n) for name in sorted(CASES)
                                     for n in (40, 500, 777, 4099)] +
-                         [("ema", 16384), ("rsi", 16384)])
+                         [("ema", 16384), ("rsi", 16384), ("ema", 32768), ("rsi", 32768),
+                          ("macd", 32768), ("bbands", 32768)])
 def test_gradients_row_lengths(dev, name, n):
     _check_against_reference(dev, name, 3, n, seed=n)
 
