@@ -490,6 +490,24 @@ CTAPI_EXPORT ctStatus_t ct_midprice_batch_d(const float* d_highs, const float* d
 /* :1445 */ CTAPI_EXPORT ctStatus_t ct_mfi_batch(const float* host_highs, const float* host_lows, const float* host_closes, const float* host_volumes, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
 /* :1453 */ CTAPI_EXPORT ctStatus_t ct_mfi_batch_d(const float* d_highs, const float* d_lows, const float* d_closes, const float* d_volumes, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
 
+/* ---------------------------------------------------------------------------
+ * Strict rolling statistics (csrc/transforms.hip): ROLLING_MEAN, _MAX, _MIN
+ * and _STD over the period bars ending at each bar, NaN when any of them is
+ * not finite; a period beyond the series is CT_STATUS_INVALID_ARGUMENT.
+ * ------------------------------------------------------------------------- */
+/* :3148 */ CTAPI_EXPORT ctStatus_t ct_rolling_mean(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3162 */ CTAPI_EXPORT ctStatus_t ct_rolling_max(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3176 */ CTAPI_EXPORT ctStatus_t ct_rolling_min(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3190 */ CTAPI_EXPORT ctStatus_t ct_rolling_std(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3152 */ CTAPI_EXPORT ctStatus_t ct_rolling_mean_batch(const float* host_inputs, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3157 */ CTAPI_EXPORT ctStatus_t ct_rolling_mean_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3166 */ CTAPI_EXPORT ctStatus_t ct_rolling_max_batch(const float* host_inputs, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3171 */ CTAPI_EXPORT ctStatus_t ct_rolling_max_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3180 */ CTAPI_EXPORT ctStatus_t ct_rolling_min_batch(const float* host_inputs, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3185 */ CTAPI_EXPORT ctStatus_t ct_rolling_min_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3194 */ CTAPI_EXPORT ctStatus_t ct_rolling_std_batch(const float* host_inputs, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3199 */ CTAPI_EXPORT ctStatus_t ct_rolling_std_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+
 #ifdef __cplusplus
 }
 #endif

@@ -269,6 +269,10 @@ CT_CDL_DECL(bearish_engulfing)
 /* :298 */ CTAPI_EXPORT ctStatus_t ct_ad_d(const float* d_high, const float* d_low, const float* d_close, const float* d_volume, float* d_output, int size, hipStream_t stream CTAPI_STREAM_DEFAULT);
 /* :270 */ CTAPI_EXPORT ctStatus_t ct_ultosc_d(const float* d_high, const float* d_low, const float* d_close, float* d_output, int size, int period1, int period2, int period3, hipStream_t stream CTAPI_STREAM_DEFAULT);
 /* :310 */ CTAPI_EXPORT ctStatus_t ct_mfi_d(const float* d_high, const float* d_low, const float* d_close, const float* d_volume, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :981 */ CTAPI_EXPORT ctStatus_t ct_rolling_mean_d(const float* d_input, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :985 */ CTAPI_EXPORT ctStatus_t ct_rolling_max_d(const float* d_input, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :988 */ CTAPI_EXPORT ctStatus_t ct_rolling_min_d(const float* d_input, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :991 */ CTAPI_EXPORT ctStatus_t ct_rolling_std_d(const float* d_input, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
 
 #ifdef __cplusplus
 }

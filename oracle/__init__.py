@@ -314,6 +314,19 @@ class _Family:
     def mfi(self, h, l, c, v, p):
         return _call(self._fn("mfi", 4, 1, [_I]), [h, l, c, v], 1, len(c), p)
 
+    ROLLING = ("mean", "max", "min", "std")
+
+    def rolling(self, kind, x, p):
+        fn = getattr(self._get(), self._prefix + "rolling")
+        fn.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        fn.restype = ctypes.c_int
+        x = _f32(x)
+        out = np.empty(len(x), np.float32)
+        rc = fn(self.ROLLING.index(kind), _fp(x), _fp(out), len(x), int(p))
+        if rc != 0:
+            raise ValueError(f"rolling {kind} returned status {rc}")
+        return out
+
     def candle(self, name, o, h, l, c):
         """int32 signal of one of the 61 TA-Lib patterns (see candles.c)."""
         L = self._get()

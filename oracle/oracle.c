@@ -2247,3 +2247,40 @@ API int orc_mfi(const float* h, const float* l, const float* c, const float* v, 
   }
   return ORC_OK;
 }
+
+/* ------------------------------------------------------ strict rolling ---
+ * ROLLING_MEAN / _MAX / _MIN / _STD: reference ClubPack.cu:377-413.  The p
+ * bars ending at t, newest first; a non-finite bar anywhere in the window
+ * leaves NaN; double sum and sum of squares, mean = sum / p, std =
+ * sqrt(max(0, sq / p - mean^2)); float fmaxf / fminf from x[t]; period <= 0
+ * or > size is invalid.  pinned: a float64 brute force in the same order,
+ * windows of integers (exact), constant windows (std 0). */
+API int orc_rolling(int mode, const float* x, float* out, int n, int p) {
+  if (!x || !out || n <= 0 || p <= 0 || p > n || mode < 0 || mode > 3) return ORC_BAD;
+  fill_nan(out, n);
+  for (int t = p - 1; t < n; ++t) {
+    int ok = 1;
+    double sum = 0.0, sq = 0.0;
+    float mx = x[t], mn = x[t];
+    for (int k = 0; k < p; ++k) {
+      const float v = x[t - k];
+      if (!isfinite(v)) {
+        ok = 0;
+        break;
+      }
+      sum += v;
+      sq += (double)v * v;
+      mx = fmaxf(mx, v);
+      mn = fminf(mn, v);
+    }
+    if (!ok) continue;
+    if (mode == 0) out[t] = (float)(sum / p);
+    else if (mode == 1) out[t] = mx;
+    else if (mode == 2) out[t] = mn;
+    else {
+      const double mean = sum / p;
+      out[t] = (float)sqrt(fmax(0.0, sq / p - mean * mean));
+    }
+  }
+  return ORC_OK;
+}

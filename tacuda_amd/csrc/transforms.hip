@@ -211,6 +211,53 @@ __global__ __launch_bounds__(kTB) void k_minmax(const float* __restrict__ x,
   }
 }
 
+// ------------------------------------------------------ strict rolling --
+// ROLLING_MEAN / _MAX / _MIN / _STD (reference ClubPack.cu:377-407): the p
+// bars ending at t, newest first; any non-finite bar leaves the output NaN;
+// sums and squares in double, mean = sum / p, std = sqrt(max(0, sq / p -
+// mean^2)); extremes folded with fmaxf / fminf from x[t].
+enum RollOp { kRollMean, kRollMax, kRollMin, kRollStd };
+
+template <int OP>
+__global__ __launch_bounds__(kTB) void k_rolling(const float* __restrict__ x,
+                                                 float* __restrict__ y, int len, long long total,
+                                                 int p) {
+  for (long long i = (long long)blockIdx.x * kTB + threadIdx.x; i < total;
+       i += (long long)gridDim.x * kTB) {
+    const int t = (int)(i % len);
+    float o = qnan_f();
+    if (t >= p - 1) {
+      bool ok = true;
+      double sum = 0.0, sq = 0.0;
+      float mx = x[i], mn = x[i];
+      for (int k = 0; k < p; ++k) {
+        const float v = x[i - k];
+        if (!__builtin_isfinite(v)) {
+          ok = false;
+          break;
+        }
+        sum += v;
+        sq += (double)v * v;
+        mx = fmaxf(mx, v);
+        mn = fminf(mn, v);
+      }
+      if (ok) {
+        if constexpr (OP == kRollMean) {
+          o = (float)(sum / p);
+        } else if constexpr (OP == kRollMax) {
+          o = mx;
+        } else if constexpr (OP == kRollMin) {
+          o = mn;
+        } else {
+          const double mean = sum / p;
+          o = (float)sqrt(fmax(0.0, sq / p - mean * mean));
+        }
+      }
+    }
+    y[i] = o;
+  }
+}
+
 // ------------------------------------------------------ ULTOSC and MFI --
 // ULTOSC (reference ULTOSC.cu:15-39): over the p3 bars ending at t, newest
 // first, float buying pressure close - min(low, prev close) and true range
@@ -460,6 +507,20 @@ ctStatus_t run_minmax(const char* fn, const float* x, float* mn, float* mx, int 
   return CT_STATUS_SUCCESS;
 }
 
+template <int OP>
+ctStatus_t run_rolling(const char* fn, const float* x, float* y, int len, int rows, int p,
+                       hipStream_t s) {
+  ctStatus_t r = shape_ok(fn, len, rows);
+  if (r != CT_STATUS_SUCCESS) return r;
+  if (!x || !y) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: null pointer", fn);
+  // period beyond the series is an invalid argument here (ClubPack.cu:409-413)
+  if (p <= 0 || p > len) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: invalid period", fn);
+  const long long total = (long long)len * rows;
+  hipLaunchKernelGGL(k_rolling<OP>, grid_for(total), dim3(kTB), 0, s, x, y, len, total, p);
+  TCA_HIP(hipGetLastError());
+  return CT_STATUS_SUCCESS;
+}
+
 ctStatus_t run_ultosc(const char* fn, const float* h, const float* l, const float* c, float* y,
                       int len, int rows, int p1, int p2, int p3, hipStream_t s) {
   ctStatus_t r = shape_ok(fn, len, rows);
@@ -568,8 +629,16 @@ ctStatus_t midpoint(const char* fn, const float* x, float* y, int len, int rows,
                     hipStream_t s) {
   return run_mid<true>(fn, x, x, y, len, rows, p, s);
 }
+template <int OP>
+ctStatus_t roll(const char* fn, const float* x, float* y, int len, int rows, int p, hipStream_t s) {
+  return run_rolling<OP>(fn, x, y, len, rows, p, s);
+}
 }  // namespace
 
+TCA_X1P(rolling_mean, roll<kRollMean>)
+TCA_X1P(rolling_max, roll<kRollMax>)
+TCA_X1P(rolling_min, roll<kRollMin>)
+TCA_X1P(rolling_std, roll<kRollStd>)
 TCA_X1P(change, mom<kChange>)
 TCA_DEVICE_ALIAS(change)
 TCA_X1P(avgdev, run_avgdev)

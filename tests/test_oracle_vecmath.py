@@ -182,3 +182,54 @@ def test_minmax_against_numpy():
         assert np.isnan(mn[:p - 1]).all() and np.isnan(mx[:p - 1]).all()
         _bits(mn[p - 1:], np.nanmin(w, axis=1), f"min {p}")
         _bits(mx[p - 1:], np.nanmax(w, axis=1), f"max {p}")
+
+
+def _rolling_brute(kind, x, p):
+    """float64 restatement, newest first (independent loop)."""
+    out = np.full(len(x), np.nan, np.float32)
+    for t in range(p - 1, len(x)):
+        w = x[t - p + 1:t + 1][::-1]
+        if not np.all(np.isfinite(w)):
+            continue
+        s = 0.0
+        q = 0.0
+        for v in w.astype(np.float64):
+            s += v
+            q += v * v
+        if kind == "mean":
+            out[t] = np.float32(s / p)
+        elif kind == "max":
+            out[t] = w.max()
+        elif kind == "min":
+            out[t] = w.min()
+        else:
+            m = s / p
+            out[t] = np.float32(np.sqrt(max(0.0, q / p - m * m)))
+    return out
+
+
+@pytest.mark.parametrize("kind", ["mean", "max", "min", "std"])
+def test_rolling_against_brute_force(kind):
+    import tacuda_amd as ta
+
+    x = ta.synth_ohlcv(600, seed=12)["close"].copy()
+    x[100] = np.nan
+    x[300] = np.inf
+    for p in (1, 5, 33):
+        _bits(ORC.rolling(kind, x, p), _rolling_brute(kind, x, p), f"rolling {kind} {p}")
+    with pytest.raises(ValueError):
+        ORC.rolling(kind, x, 601)  # period beyond the series
+    with pytest.raises(ValueError):
+        ORC.rolling(kind, x, 0)
+
+
+def test_rolling_closed_forms():
+    c = np.full(50, 7.25, np.float32)
+    assert np.all(ORC.rolling("std", c, 8)[7:] == 0.0)
+    assert np.all(ORC.rolling("mean", c, 8)[7:] == 7.25)
+    i = np.arange(1, 51, dtype=np.float32)  # integers: exact sums
+    assert np.array_equal(ORC.rolling("mean", i, 4)[3:], (i[3:] - 1.5).astype(np.float32))
+    assert np.array_equal(ORC.rolling("max", i, 4)[3:], i[3:])
+    assert np.array_equal(ORC.rolling("min", i, 4)[3:], i[:-3])
+    # population std of 4 consecutive integers: sqrt(1.25)
+    assert np.allclose(ORC.rolling("std", i, 4)[3:], np.sqrt(1.25), rtol=1e-7)
