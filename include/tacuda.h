@@ -494,6 +494,7 @@ CTAPI_EXPORT ctStatus_t ct_midprice_batch_d(const float* d_highs, const float* d
  * Strict rolling statistics (csrc/transforms.hip): ROLLING_MEAN, _MAX, _MIN
  * and _STD over the period bars ending at each bar, NaN when any of them is
  * not finite; a period beyond the series is CT_STATUS_INVALID_ARGUMENT.
+ * PIVOT_HIGH / PIVOT_LOW: the pivot's value at the bar that confirms it.
  * ------------------------------------------------------------------------- */
 /* :3148 */ CTAPI_EXPORT ctStatus_t ct_rolling_mean(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
 /* :3162 */ CTAPI_EXPORT ctStatus_t ct_rolling_max(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
@@ -507,6 +508,12 @@ CTAPI_EXPORT ctStatus_t ct_midprice_batch_d(const float* d_highs, const float* d
 /* :3185 */ CTAPI_EXPORT ctStatus_t ct_rolling_min_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
 /* :3194 */ CTAPI_EXPORT ctStatus_t ct_rolling_std_batch(const float* host_inputs, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
 /* :3199 */ CTAPI_EXPORT ctStatus_t ct_rolling_std_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3107 */ CTAPI_EXPORT ctStatus_t ct_pivot_high(const float* host_input, float* host_output, int size, int strength, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3121 */ CTAPI_EXPORT ctStatus_t ct_pivot_low(const float* host_input, float* host_output, int size, int strength, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3111 */ CTAPI_EXPORT ctStatus_t ct_pivot_high_batch(const float* host_inputs, float* host_outputs, int seriesLen, int numSeries, int strength, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3116 */ CTAPI_EXPORT ctStatus_t ct_pivot_high_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int strength, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3125 */ CTAPI_EXPORT ctStatus_t ct_pivot_low_batch(const float* host_inputs, float* host_outputs, int seriesLen, int numSeries, int strength, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3130 */ CTAPI_EXPORT ctStatus_t ct_pivot_low_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int strength, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
 
 #ifdef __cplusplus
 }

@@ -314,6 +314,17 @@ class _Family:
     def mfi(self, h, l, c, v, p):
         return _call(self._fn("mfi", 4, 1, [_I]), [h, l, c, v], 1, len(c), p)
 
+    def pivot(self, high, x, s):
+        fn = getattr(self._get(), self._prefix + "pivot")
+        fn.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        fn.restype = ctypes.c_int
+        x = _f32(x)
+        out = np.empty(len(x), np.float32)
+        rc = fn(1 if high else 0, _fp(x), _fp(out), len(x), int(s))
+        if rc != 0:
+            raise ValueError(f"pivot returned status {rc}")
+        return out
+
     ROLLING = ("mean", "max", "min", "std")
 
     def rolling(self, kind, x, p):

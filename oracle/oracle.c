@@ -2284,3 +2284,22 @@ API int orc_rolling(int mode, const float* x, float* out, int n, int p) {
   }
   return ORC_OK;
 }
+
+/* ------------------------------------------------------------- pivots ---
+ * PIVOT_HIGH / PIVOT_LOW: reference ClubPack.cu:357-374 and :635-657.  Bar
+ * b = t - s is a pivot high when x[b] > x[b-k] and x[b] >= x[b+k] for
+ * k = 1..s (low: < and <=); its value is reported at t, NaN elsewhere;
+ * strength <= 0 or 2 s >= size is invalid.  pinned: numpy brute force and
+ * hand-built peaks, plateaus and troughs. */
+API int orc_pivot(int high, const float* x, float* out, int n, int s) {
+  if (!x || !out || n <= 0 || s <= 0 || 2 * s >= n) return ORC_BAD;
+  fill_nan(out, n);
+  for (int t = 2 * s; t < n; ++t) {
+    const int b = t - s;
+    int piv = 1;
+    for (int k = 1; k <= s && piv; ++k)
+      piv = high ? (x[b] > x[b - k] && x[b] >= x[b + k]) : (x[b] < x[b - k] && x[b] <= x[b + k]);
+    if (piv) out[t] = x[b];
+  }
+  return ORC_OK;
+}

@@ -211,6 +211,31 @@ __global__ __launch_bounds__(kTB) void k_minmax(const float* __restrict__ x,
   }
 }
 
+// ------------------------------------------------------------ pivots --
+// PIVOT_HIGH / PIVOT_LOW (reference ClubPack.cu:357-374): bar b = t - s is
+// a pivot when it is strictly above (below) each of the s bars before it
+// and at or above (at or below) each of the s bars after it; the pivot's
+// value is reported at t, when the s bars after it are known; NaN otherwise.
+template <bool HIGH>
+__global__ __launch_bounds__(kTB) void k_pivot(const float* __restrict__ x,
+                                               float* __restrict__ y, int len, long long total,
+                                               int st) {
+  for (long long i = (long long)blockIdx.x * kTB + threadIdx.x; i < total;
+       i += (long long)gridDim.x * kTB) {
+    const int t = (int)(i % len);
+    float o = qnan_f();
+    if (t - st >= st) {
+      const long long b = i - st;
+      const float v = x[b];
+      bool piv = true;
+      for (int k = 1; k <= st && piv; ++k)
+        piv = HIGH ? (v > x[b - k] && v >= x[b + k]) : (v < x[b - k] && v <= x[b + k]);
+      if (piv) o = v;
+    }
+    y[i] = o;
+  }
+}
+
 // ------------------------------------------------------ strict rolling --
 // ROLLING_MEAN / _MAX / _MIN / _STD (reference ClubPack.cu:377-407): the p
 // bars ending at t, newest first; any non-finite bar leaves the output NaN;
@@ -507,6 +532,20 @@ ctStatus_t run_minmax(const char* fn, const float* x, float* mn, float* mx, int 
   return CT_STATUS_SUCCESS;
 }
 
+template <bool HIGH>
+ctStatus_t run_pivot(const char* fn, const float* x, float* y, int len, int rows, int st,
+                     hipStream_t s) {
+  ctStatus_t r = shape_ok(fn, len, rows);
+  if (r != CT_STATUS_SUCCESS) return r;
+  if (!x || !y) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: null pointer", fn);
+  // ClubPack.cu:637-639: strength > 0 and both sides inside the series
+  if (st <= 0 || 2ll * st >= len) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: invalid strength", fn);
+  const long long total = (long long)len * rows;
+  hipLaunchKernelGGL(k_pivot<HIGH>, grid_for(total), dim3(kTB), 0, s, x, y, len, total, st);
+  TCA_HIP(hipGetLastError());
+  return CT_STATUS_SUCCESS;
+}
+
 template <int OP>
 ctStatus_t run_rolling(const char* fn, const float* x, float* y, int len, int rows, int p,
                        hipStream_t s) {
@@ -633,8 +672,15 @@ template <int OP>
 ctStatus_t roll(const char* fn, const float* x, float* y, int len, int rows, int p, hipStream_t s) {
   return run_rolling<OP>(fn, x, y, len, rows, p, s);
 }
+template <bool HIGH>
+ctStatus_t pivot(const char* fn, const float* x, float* y, int len, int rows, int st,
+                 hipStream_t s) {
+  return run_pivot<HIGH>(fn, x, y, len, rows, st, s);
+}
 }  // namespace
 
+TCA_X1P(pivot_high, pivot<true>)
+TCA_X1P(pivot_low, pivot<false>)
 TCA_X1P(rolling_mean, roll<kRollMean>)
 TCA_X1P(rolling_max, roll<kRollMax>)
 TCA_X1P(rolling_min, roll<kRollMin>)

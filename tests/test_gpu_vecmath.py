@@ -150,8 +150,8 @@ def test_bar_families(dev, name, rows, n):
 
 
 @pytest.mark.parametrize("name", ["change", "avgdev", "minmax", "rolling_mean", "rolling_max",
-                                  "rolling_min", "rolling_std"])
-@pytest.mark.parametrize("rows,n,p", [(1, 20011, 14), (1, 300, 1), (12, 640, 30), (5, 40, 40)])
+                                  "rolling_min", "rolling_std", "pivot_high", "pivot_low"])
+@pytest.mark.parametrize("rows,n,p", [(1, 20011, 14), (1, 300, 1), (12, 640, 30), (5, 40, 19)])
 def test_window_families(dev, name, rows, n, p):
     import tacuda_amd as ta
     from oracle import ORC
@@ -174,8 +174,12 @@ def test_window_families(dev, name, rows, n, p):
     torch.cuda.synchronize()
     got = [y.cpu().numpy() for y in ys]
     for r in range(rows):
-        want = (ORC.rolling(name[8:], x[r], p) if name.startswith("rolling")
-                else getattr(ORC, name)(x[r], p))
+        if name.startswith("rolling"):
+            want = ORC.rolling(name[8:], x[r], p)
+        elif name.startswith("pivot"):
+            want = ORC.pivot(name == "pivot_high", x[r], p)
+        else:
+            want = getattr(ORC, name)(x[r], p)
         want = want if nout == 2 else (want,)
         for k in range(nout):
             compare(got[k][r], want[k], tol=0.0, min_exact=1.0, name=f"{name} row {r} out {k}")
@@ -213,6 +217,7 @@ def test_window_period_edges(dev):
     assert torch.isnan(y).all() and torch.isnan(z).all()
     assert L.ct_avgdev_d(x.data_ptr(), y.data_ptr(), 50, 0, None) == 4
     assert L.ct_rolling_mean_d(x.data_ptr(), y.data_ptr(), 50, 51, None) == 4
+    assert L.ct_pivot_high_d(x.data_ptr(), y.data_ptr(), 50, 25, None) == 4
     assert L.ct_rolling_std_batch_d(x.data_ptr(), y.data_ptr(), 50, 1, 0, 1, None) == 4
     assert L.ct_minmax_d(x.data_ptr(), y.data_ptr(), None, 50, 5, None) == 4
     assert L.ct_bop_d(x.data_ptr(), x.data_ptr(), x.data_ptr(), None, y.data_ptr(), 50, None) == 4

@@ -233,3 +233,29 @@ def test_rolling_closed_forms():
     assert np.array_equal(ORC.rolling("min", i, 4)[3:], i[:-3])
     # population std of 4 consecutive integers: sqrt(1.25)
     assert np.allclose(ORC.rolling("std", i, 4)[3:], np.sqrt(1.25), rtol=1e-7)
+
+
+def test_pivots_brute_force_and_shapes():
+    import tacuda_amd as ta
+
+    x = ta.synth_ohlcv(700, seed=14)["close"]
+    for s in (1, 3, 8):
+        for high in (True, False):
+            want = np.full(len(x), np.nan, np.float32)
+            for t in range(2 * s, len(x)):
+                b = t - s
+                left, right = x[b - s:b], x[b + 1:b + s + 1]
+                if high and np.all(x[b] > left) and np.all(x[b] >= right):
+                    want[t] = x[b]
+                if not high and np.all(x[b] < left) and np.all(x[b] <= right):
+                    want[t] = x[b]
+            _bits(ORC.pivot(high, x, s), want, f"pivot high={high} s={s}")
+    # a peak, a plateau (equal right neighbour counts, equal left one does not)
+    y = np.array([1, 2, 5, 3, 1, 4, 4, 2, 0, 0], np.float32)
+    hi = ORC.pivot(True, y, 1)
+    assert hi[3] == 5.0 and hi[6] == 4.0 and np.isnan(hi[7])
+    lo = ORC.pivot(False, y, 1)
+    # a trough at bar 4; bar 8 (0 before an equal 0) counts: the right side may tie
+    assert lo[5] == 1.0 and lo[9] == 0.0 and np.isnan(lo[8])
+    with pytest.raises(ValueError):
+        ORC.pivot(True, y, 5)  # 2 s >= size
