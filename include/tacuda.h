@@ -495,6 +495,7 @@ CTAPI_EXPORT ctStatus_t ct_midprice_batch_d(const float* d_highs, const float* d
  * and _STD over the period bars ending at each bar, NaN when any of them is
  * not finite; a period beyond the series is CT_STATUS_INVALID_ARGUMENT.
  * PIVOT_HIGH / PIVOT_LOW: the pivot's value at the bar that confirms it.
+ * SKEW: the population skewness of the window.
  * ------------------------------------------------------------------------- */
 /* :3148 */ CTAPI_EXPORT ctStatus_t ct_rolling_mean(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
 /* :3162 */ CTAPI_EXPORT ctStatus_t ct_rolling_max(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
@@ -514,6 +515,9 @@ CTAPI_EXPORT ctStatus_t ct_midprice_batch_d(const float* d_highs, const float* d
 /* :3116 */ CTAPI_EXPORT ctStatus_t ct_pivot_high_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int strength, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
 /* :3125 */ CTAPI_EXPORT ctStatus_t ct_pivot_low_batch(const float* host_inputs, float* host_outputs, int seriesLen, int numSeries, int strength, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
 /* :3130 */ CTAPI_EXPORT ctStatus_t ct_pivot_low_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int strength, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3081 */ CTAPI_EXPORT ctStatus_t ct_skew(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3084 */ CTAPI_EXPORT ctStatus_t ct_skew_batch(const float* host_inputs, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3089 */ CTAPI_EXPORT ctStatus_t ct_skew_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
 
 #ifdef __cplusplus
 }

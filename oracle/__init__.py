@@ -314,6 +314,9 @@ class _Family:
     def mfi(self, h, l, c, v, p):
         return _call(self._fn("mfi", 4, 1, [_I]), [h, l, c, v], 1, len(c), p)
 
+    def skew(self, x, p):
+        return _call(self._fn("skew", 1, 1, [_I]), [x], 1, len(x), p)
+
     def pivot(self, high, x, s):
         fn = getattr(self._get(), self._prefix + "pivot")
         fn.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int]

@@ -2303,3 +2303,28 @@ API int orc_pivot(int high, const float* x, float* out, int n, int s) {
   }
   return ORC_OK;
 }
+
+/* --------------------------------------------------------------- SKEW ---
+ * Reference ClubPack.cu:327-355 (RollingSkew): over the p bars ending at t,
+ * newest first, the double mean, central moments m2, m3 (/ p); 0 when
+ * m2 <= 0, else m3 / m2^1.5; period <= 0 or > size invalid.  pinned: the
+ * numpy population skewness, symmetric windows (0), constant windows (0). */
+API int orc_skew(const float* x, float* out, int n, int p) {
+  if (!x || !out || n <= 0 || p <= 0 || p > n) return ORC_BAD;
+  fill_nan(out, n);
+  for (int t = p - 1; t < n; ++t) {
+    double mean = 0.0;
+    for (int k = 0; k < p; ++k) mean += x[t - k];
+    mean /= p;
+    double m2 = 0.0, m3 = 0.0;
+    for (int k = 0; k < p; ++k) {
+      const double d = (double)x[t - k] - mean, d2 = d * d;
+      m2 += d2;
+      m3 += d2 * d;
+    }
+    m2 /= p;
+    m3 /= p;
+    out[t] = m2 <= 0.0 ? 0.0f : (float)(m3 / pow(m2, 1.5));
+  }
+  return ORC_OK;
+}

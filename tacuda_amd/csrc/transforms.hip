@@ -236,6 +236,34 @@ __global__ __launch_bounds__(kTB) void k_pivot(const float* __restrict__ x,
   }
 }
 
+// ------------------------------------------------------------- SKEW --
+// SKEW (reference ClubPack.cu:327-355, RollingSkew): over the p bars ending
+// at t, newest first, the double mean, then the central moments m2 and m3
+// (/ p); 0 when m2 <= 0, else m3 / m2^1.5.
+__global__ __launch_bounds__(kTB) void k_skew(const float* __restrict__ x, float* __restrict__ y,
+                                              int len, long long total, int p) {
+  for (long long i = (long long)blockIdx.x * kTB + threadIdx.x; i < total;
+       i += (long long)gridDim.x * kTB) {
+    const int t = (int)(i % len);
+    float o = qnan_f();
+    if (t >= p - 1) {
+      double mean = 0.0;
+      for (int k = 0; k < p; ++k) mean += x[i - k];
+      mean /= p;
+      double m2 = 0.0, m3 = 0.0;
+      for (int k = 0; k < p; ++k) {
+        const double d = (double)x[i - k] - mean, d2 = d * d;
+        m2 += d2;
+        m3 += d2 * d;
+      }
+      m2 /= p;
+      m3 /= p;
+      o = m2 <= 0.0 ? 0.0f : (float)(m3 / pow(m2, 1.5));
+    }
+    y[i] = o;
+  }
+}
+
 // ------------------------------------------------------ strict rolling --
 // ROLLING_MEAN / _MAX / _MIN / _STD (reference ClubPack.cu:377-407): the p
 // bars ending at t, newest first; any non-finite bar leaves the output NaN;
@@ -546,6 +574,18 @@ ctStatus_t run_pivot(const char* fn, const float* x, float* y, int len, int rows
   return CT_STATUS_SUCCESS;
 }
 
+ctStatus_t run_skew(const char* fn, const float* x, float* y, int len, int rows, int p,
+                    hipStream_t s) {
+  ctStatus_t r = shape_ok(fn, len, rows);
+  if (r != CT_STATUS_SUCCESS) return r;
+  if (!x || !y) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: null pointer", fn);
+  if (p <= 0 || p > len) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: invalid period", fn);
+  const long long total = (long long)len * rows;
+  hipLaunchKernelGGL(k_skew, grid_for(total), dim3(kTB), 0, s, x, y, len, total, p);
+  TCA_HIP(hipGetLastError());
+  return CT_STATUS_SUCCESS;
+}
+
 template <int OP>
 ctStatus_t run_rolling(const char* fn, const float* x, float* y, int len, int rows, int p,
                        hipStream_t s) {
@@ -679,6 +719,7 @@ ctStatus_t pivot(const char* fn, const float* x, float* y, int len, int rows, in
 }
 }  // namespace
 
+TCA_X1P(skew, run_skew)
 TCA_X1P(pivot_high, pivot<true>)
 TCA_X1P(pivot_low, pivot<false>)
 TCA_X1P(rolling_mean, roll<kRollMean>)

@@ -259,3 +259,22 @@ def test_pivots_brute_force_and_shapes():
     assert lo[5] == 1.0 and lo[9] == 0.0 and np.isnan(lo[8])
     with pytest.raises(ValueError):
         ORC.pivot(True, y, 5)  # 2 s >= size
+
+
+def test_skew_against_numpy_and_closed_forms():
+    import tacuda_amd as ta
+
+    x = ta.synth_ohlcv(500, seed=15)["close"]
+    for p in (3, 20):
+        w = np.lib.stride_tricks.sliding_window_view(x.astype(np.float64), p)
+        d = w - w.mean(1, keepdims=True)
+        m2, m3 = (d ** 2).mean(1), (d ** 3).mean(1)
+        want = np.where(m2 <= 0, 0.0, m3 / m2 ** 1.5)
+        got = ORC.skew(x, p)
+        assert np.isnan(got[:p - 1]).all()
+        np.testing.assert_allclose(got[p - 1:], want, rtol=1e-4, atol=1e-6)
+    sym = np.tile(np.array([1, 2, 3], np.float32), 20)  # symmetric windows
+    assert np.allclose(ORC.skew(sym, 3)[2:], 0.0, atol=1e-7)
+    assert np.all(ORC.skew(np.full(30, 4.0, np.float32), 5)[4:] == 0.0)
+    with pytest.raises(ValueError):
+        ORC.skew(sym, 61)
