@@ -495,7 +495,8 @@ CTAPI_EXPORT ctStatus_t ct_midprice_batch_d(const float* d_highs, const float* d
  * and _STD over the period bars ending at each bar, NaN when any of them is
  * not finite; a period beyond the series is CT_STATUS_INVALID_ARGUMENT.
  * PIVOT_HIGH / PIVOT_LOW: the pivot's value at the bar that confirms it.
- * SKEW: the population skewness of the window.
+ * SKEW: the population skewness of the window.  TRENDSCORE and
+ * MARKET_STRENGTH: the reference's own counting and change-ratio scores.
  * ------------------------------------------------------------------------- */
 /* :3148 */ CTAPI_EXPORT ctStatus_t ct_rolling_mean(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
 /* :3162 */ CTAPI_EXPORT ctStatus_t ct_rolling_max(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
@@ -518,6 +519,12 @@ CTAPI_EXPORT ctStatus_t ct_midprice_batch_d(const float* d_highs, const float* d
 /* :3081 */ CTAPI_EXPORT ctStatus_t ct_skew(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
 /* :3084 */ CTAPI_EXPORT ctStatus_t ct_skew_batch(const float* host_inputs, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
 /* :3089 */ CTAPI_EXPORT ctStatus_t ct_skew_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3054 */ CTAPI_EXPORT ctStatus_t ct_trendscore(const float* host_input, float* host_output, int size, int lookback, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3038 */ CTAPI_EXPORT ctStatus_t ct_market_strength(const float* host_input, float* host_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3058 */ CTAPI_EXPORT ctStatus_t ct_trendscore_batch(const float* host_inputs, float* host_outputs, int seriesLen, int numSeries, int lookback, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3063 */ CTAPI_EXPORT ctStatus_t ct_trendscore_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int lookback, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3042 */ CTAPI_EXPORT ctStatus_t ct_market_strength_batch(const float* host_inputs, float* host_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :3048 */ CTAPI_EXPORT ctStatus_t ct_market_strength_batch_d(const float* d_inputs, float* d_outputs, int seriesLen, int numSeries, int period, int maxConcurrency, hipStream_t stream CTAPI_STREAM_DEFAULT);
 
 #ifdef __cplusplus
 }

@@ -276,6 +276,8 @@ CT_CDL_DECL(bearish_engulfing)
 /* :972 */ CTAPI_EXPORT ctStatus_t ct_pivot_high_d(const float* d_input, float* d_output, int size, int strength, hipStream_t stream CTAPI_STREAM_DEFAULT);
 /* :975 */ CTAPI_EXPORT ctStatus_t ct_pivot_low_d(const float* d_input, float* d_output, int size, int strength, hipStream_t stream CTAPI_STREAM_DEFAULT);
 /* :966 */ CTAPI_EXPORT ctStatus_t ct_skew_d(const float* d_input, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :960 */ CTAPI_EXPORT ctStatus_t ct_trendscore_d(const float* d_input, float* d_output, int size, int lookback, hipStream_t stream CTAPI_STREAM_DEFAULT);
+/* :956 */ CTAPI_EXPORT ctStatus_t ct_market_strength_d(const float* d_input, float* d_output, int size, int period, hipStream_t stream CTAPI_STREAM_DEFAULT);
 
 #ifdef __cplusplus
 }

@@ -317,6 +317,12 @@ class _Family:
     def skew(self, x, p):
         return _call(self._fn("skew", 1, 1, [_I]), [x], 1, len(x), p)
 
+    def trendscore(self, x, lb):
+        return _call(self._fn("trendscore", 1, 1, [_I]), [x], 1, len(x), lb)
+
+    def market_strength(self, x, p):
+        return _call(self._fn("market_strength", 1, 1, [_I]), [x], 1, len(x), p)
+
     def pivot(self, high, x, s):
         fn = getattr(self._get(), self._prefix + "pivot")
         fn.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int]

@@ -2328,3 +2328,37 @@ API int orc_skew(const float* x, float* out, int n, int p) {
   }
   return ORC_OK;
 }
+
+/* --------------------------------------- TRENDSCORE / MARKET_STRENGTH ---
+ * Reference ClubPack.cu:274-306 and :550-590.  TRENDSCORE: sum over k =
+ * 1..10 of +1 if x[t] >= x[t-lb-k] else -1, from bar lb + 10 (lb < 0 or
+ * size <= lb + 10 invalid).  MARKET_STRENGTH: the p one-bar changes ending
+ * at t in double: 100 total / ups when total >= 0, 100 total / downs
+ * otherwise, 0 on a zero divisor; p >= size all NaN.  pinned: numpy brute
+ * force and monotone closed forms (+10 / -10, +100 / -100). */
+API int orc_trendscore(const float* x, float* out, int n, int lb) {
+  if (!x || !out || n <= 0 || lb < 0 || n <= lb + 10) return ORC_BAD;
+  fill_nan(out, n);
+  for (int t = lb + 10; t < n; ++t) {
+    int sc = 0;
+    for (int k = 1; k <= 10; ++k) sc += x[t] >= x[t - lb - k] ? 1 : -1;
+    out[t] = (float)sc;
+  }
+  return ORC_OK;
+}
+API int orc_market_strength(const float* x, float* out, int n, int p) {
+  if (!x || !out || n <= 0 || p <= 0) return ORC_BAD;
+  fill_nan(out, n);
+  for (int t = p; t < n; ++t) {
+    double tot = 0.0, ups = 0.0, dns = 0.0;
+    for (int k = 0; k < p; ++k) {
+      const double d = (double)x[t - k] - x[t - k - 1];
+      tot += d;
+      if (d > 0.0) ups += d;
+      else dns += -d;
+    }
+    if (tot >= 0.0) out[t] = ups > 0.0 ? (float)(100.0 * tot / ups) : 0.0f;
+    else out[t] = dns > 0.0 ? (float)(100.0 * tot / dns) : 0.0f;
+  }
+  return ORC_OK;
+}

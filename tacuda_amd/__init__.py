@@ -294,7 +294,7 @@ for _n in MATH_BINARY:
     for _f in (f"ct_{_n}_d", f"ct_{_n}"):
         _sig(_f, P, P, P, c_int)
 for _n in ("change", "avgdev", "rolling_mean", "rolling_max", "rolling_min", "rolling_std",
-           "pivot_high", "pivot_low", "skew"):
+           "pivot_high", "pivot_low", "skew", "trendscore", "market_strength"):
     for _f in (f"ct_{_n}_d", f"ct_{_n}"):
         _sig(_f, P, P, c_int, c_int)
     for _f in (f"ct_{_n}_batch_d", f"ct_{_n}_batch"):

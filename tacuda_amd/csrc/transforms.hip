@@ -236,6 +236,40 @@ __global__ __launch_bounds__(kTB) void k_pivot(const float* __restrict__ x,
   }
 }
 
+// ------------------------------------------- TRENDSCORE, MARKET_STRENGTH --
+// TRENDSCORE (reference ClubPack.cu:294-306): the sum over k = 1..10 of +1
+// when x[t] >= x[t - lookback - k], else -1, from bar lookback + 10.
+// MARKET_STRENGTH (ClubPack.cu:274-292): over the p one-bar changes ending
+// at t (double), total / ups (total >= 0) or total / downs, x100, 0 when the
+// divisor is 0; from bar p.
+template <bool TREND>
+__global__ __launch_bounds__(kTB) void k_club(const float* __restrict__ x, float* __restrict__ y,
+                                              int len, long long total, int p) {
+  for (long long i = (long long)blockIdx.x * kTB + threadIdx.x; i < total;
+       i += (long long)gridDim.x * kTB) {
+    const int t = (int)(i % len);
+    float o = qnan_f();
+    if constexpr (TREND) {
+      if (t >= p + 10) {
+        int sc = 0;
+        for (int k = 1; k <= 10; ++k) sc += x[i] >= x[i - p - k] ? 1 : -1;
+        o = (float)sc;
+      }
+    } else if (t >= p) {
+      double tot = 0.0, ups = 0.0, dns = 0.0;
+      for (int k = 0; k < p; ++k) {
+        const double d = (double)x[i - k] - x[i - k - 1];
+        tot += d;
+        if (d > 0.0) ups += d;
+        else dns += -d;
+      }
+      if (tot >= 0.0) o = ups > 0.0 ? (float)(100.0 * tot / ups) : 0.0f;
+      else o = dns > 0.0 ? (float)(100.0 * tot / dns) : 0.0f;
+    }
+    y[i] = o;
+  }
+}
+
 // ------------------------------------------------------------- SKEW --
 // SKEW (reference ClubPack.cu:327-355, RollingSkew): over the p bars ending
 // at t, newest first, the double mean, then the central moments m2 and m3
@@ -574,6 +608,25 @@ ctStatus_t run_pivot(const char* fn, const float* x, float* y, int len, int rows
   return CT_STATUS_SUCCESS;
 }
 
+template <bool TREND>
+ctStatus_t run_club(const char* fn, const float* x, float* y, int len, int rows, int p,
+                    hipStream_t s) {
+  ctStatus_t r = shape_ok(fn, len, rows);
+  if (r != CT_STATUS_SUCCESS) return r;
+  if (!x || !y) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: null pointer", fn);
+  const long long total = (long long)len * rows;
+  if (TREND) {
+    // ClubPack.cu:574-582: lookback >= 0 and the series longer than lookback + 10
+    if (p < 0 || len <= p + 10) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: invalid lookback", fn);
+  } else {
+    if (p <= 0) return fail(CT_STATUS_INVALID_ARGUMENT, "%s: invalid period", fn);
+    if (p >= len) return all_nan(y, total, s);
+  }
+  hipLaunchKernelGGL(k_club<TREND>, grid_for(total), dim3(kTB), 0, s, x, y, len, total, p);
+  TCA_HIP(hipGetLastError());
+  return CT_STATUS_SUCCESS;
+}
+
 ctStatus_t run_skew(const char* fn, const float* x, float* y, int len, int rows, int p,
                     hipStream_t s) {
   ctStatus_t r = shape_ok(fn, len, rows);
@@ -720,6 +773,8 @@ ctStatus_t pivot(const char* fn, const float* x, float* y, int len, int rows, in
 }  // namespace
 
 TCA_X1P(skew, run_skew)
+TCA_X1P(trendscore, run_club<true>)
+TCA_X1P(market_strength, run_club<false>)
 TCA_X1P(pivot_high, pivot<true>)
 TCA_X1P(pivot_low, pivot<false>)
 TCA_X1P(rolling_mean, roll<kRollMean>)

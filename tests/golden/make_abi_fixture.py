@@ -31,7 +31,7 @@ FAMILIES = ["sma", "ema", "wma", "dema", "tema", "t3", "kama", "trima", "rsi", "
             "ao", "qstick", "donchian", "marketfi", "wad", "mcgd", "rvol", "cmf", "emv", "fosc",
             "eri", "percentrank", "si", "asi_wilder", "obv", "ad", "ultosc", "mfi",
             "vortex", "cmou", "rolling_mean", "rolling_max", "rolling_min", "rolling_std",
-            "pivot_high", "pivot_low", "skew"]
+            "pivot_high", "pivot_low", "skew", "trendscore", "market_strength"]
 SUFFIXES = ["", "_d", "_device", "_batch", "_batch_d", "_ex", "_ex_d"]
 EXTRA = ["ct_status_string", "ct_get_last_error", "ct_version", "ct_device_alloc",
          "ct_device_alloc_ex", "ct_device_free", "ct_device_free_ex", "ct_device_upload",

@@ -151,12 +151,14 @@ def test_bar_families(dev, name, rows, n):
 
 @pytest.mark.parametrize("name", ["change", "avgdev", "minmax", "rolling_mean", "rolling_max",
                                   "rolling_min", "rolling_std", "pivot_high", "pivot_low",
-                                  "skew"])
+                                  "skew", "trendscore", "market_strength"])
 @pytest.mark.parametrize("rows,n,p", [(1, 20011, 14), (1, 300, 1), (12, 640, 30), (5, 40, 19)])
 def test_window_families(dev, name, rows, n, p):
     import tacuda_amd as ta
     from oracle import ORC
 
+    if name == "trendscore" and n <= p + 10:
+        p = 0  # TRENDSCORE needs the series longer than lookback + 10: lookback 0 here
     print(f"window {name} {rows}x{n} p{p}", flush=True)
     L = ta.lib()
     x = np.asarray(ta.synth_ohlcv(n, rows=rows, seed=p)["close"], np.float32).reshape(rows, n)

@@ -278,3 +278,30 @@ def test_skew_against_numpy_and_closed_forms():
     assert np.all(ORC.skew(np.full(30, 4.0, np.float32), 5)[4:] == 0.0)
     with pytest.raises(ValueError):
         ORC.skew(sym, 61)
+
+
+def test_trendscore_and_market_strength():
+    import tacuda_amd as ta
+
+    x = ta.synth_ohlcv(400, seed=16)["close"]
+    for lb in (0, 5):
+        want = np.full(400, np.nan, np.float32)
+        for t in range(lb + 10, 400):
+            want[t] = sum(1 if x[t] >= x[t - lb - k] else -1 for k in range(1, 11))
+        _bits(ORC.trendscore(x, lb), want, f"trendscore {lb}")
+    for p in (1, 9):
+        want = np.full(400, np.nan, np.float32)
+        d = np.diff(x.astype(np.float64))
+        for t in range(p, 400):
+            w = d[t - p:t]
+            tot, ups, dns = w.sum(), w[w > 0].sum(), -w[w < 0].sum()
+            want[t] = (100 * tot / ups if ups > 0 else 0.0) if tot >= 0 else \
+                (100 * tot / dns if dns > 0 else 0.0)
+        np.testing.assert_allclose(ORC.market_strength(x, p)[p:], want[p:], rtol=1e-6, atol=1e-5)
+    up = np.arange(1, 60, dtype=np.float32)
+    assert np.all(ORC.trendscore(up, 3)[13:] == 10.0)
+    assert np.all(ORC.trendscore(up[::-1].copy(), 3)[13:] == -10.0)
+    assert np.all(ORC.market_strength(up, 7)[7:] == 100.0)
+    assert np.all(ORC.market_strength(up[::-1].copy(), 7)[7:] == -100.0)
+    with pytest.raises(ValueError):
+        ORC.trendscore(up, 49)  # size <= lookback + 10
